@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-updates/s per RK stage of the fused FP32 MUSCL-HLLC-RK3 stage.
+
+Workload (one step = one SSP-RK3 step = 3 fused stage launches): 2-D Euler,
+MUSCL with the monotonized-central limiter + HLLC (Thornber) + SSP-RK3 on a
+periodic 4096^2 grid, gamma = 1.4, CFL 0.4 with the dt reduction fused into
+stage 3; initial state = isentropic vortex (beta = 5) plus 8 seeded Fourier
+modes (amplitude 0.02).  Synthetic data, generated on the host in FP64.
+
+Prints ONE JSON line (rank 0):
+  value     = N_cells * 3 * steps / wall time of the timed steps (state resident in HBM)
+  roofline  = algorithmic HBM bytes per stage launch / mean stage-kernel time (HIP events
+              on the solver's stream) vs the MI355X HBM peak (8 TB/s)
+  cpu_baseline = the CPU FP64 oracle (a single-threaded port of the reference path) on a
+              bounded sample of the same workload, run on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "cell-updates/s per RK stage, % HBM peak; time-to-L1-error learned vs MUSCL"
+# algorithmic bytes per cell per RK3 step (4 stored components):
+#   stage 1 reads U, writes U1; stages 2 and 3 read U^k and U^0, write U^{k+1}
+BYTES_PER_CELL_STEP = {"fp32": 4 * 4 * (2 + 3 + 3), "fp64": 8 * 4 * (2 + 3 + 3)}
+
+
+def cpu_baseline(n_cpu: int = 1024, steps: int = 2):
+    """Time the oracle (single thread, -O3, FP64) on an n_cpu^2 sample of the same workload."""
+    from oracle import orc
+    from dev_field_efvm_amd import cases
+    st = cases.vortex_modes_state(n_cpu, n_cpu, seed=1)
+    sim = orc.Sim(n_cpu, n_cpu, 1, hi=(10.0, 10.0, 10.0 / n_cpu), flux="hllc", muscl=True, limiter="mc",
+                  cfl=0.4, bcs=["periodic"] * 4 + ["slip", "slip"])
+    sim.set_interior(st.reshape(1, n_cpu, n_cpu, 5))
+    sim.step(1)                      # warm-up (page-in)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    dt = time.perf_counter() - t0
+    return {"value": n_cpu * n_cpu * 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/efvm_oracle.c FP64 MUSCL-MC+HLLC+RK3, periodic {n_cpu}^2 vortex+modes, {steps} steps "
+                      f"({dt:.1f} s, 1 thread, {os.cpu_count()} host cores visible)"}
+
+
+def load_traffic(n: int, prec: str):
+    """HBM bytes per stage launch from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", f"stage_{prec}_{n}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--strip-rows", type=int, default=0)
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=1024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (multi-GPU sharding is a design "
+                                   "note, not a runnable path)"}))
+        sys.exit(2)
+
+    import torch
+    import dev_field_efvm_amd as efvm
+    from dev_field_efvm_amd import cases
+
+    n = args.n
+    t_ic = time.perf_counter()
+    st = cases.vortex_modes_state(n, n, seed=1)
+    t_ic = time.perf_counter() - t_ic
+    ctx = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, flux="hllc", muscl=True, limiter="mc",
+                       integrator="rk3", precision=args.precision, cfl=0.4)
+    ctx.set_boundaries(["periodic"] * 4)
+    if args.strip_rows or args.waves:
+        ctx.set_launch(args.strip_rows, args.waves)
+    ctx.upload(st)
+    del st
+
+    # warm-up, then the timed region: K whole steps, bracketed by device synchronisation
+    ctx.step(args.warmup)
+    ctx.synchronize()
+    use_graph = args.graph and args.steps % 2 == 0 and args.steps > 0
+    if use_graph:
+        ctx.graph_capture(2)
+        ctx.graph_launch()
+        ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev_ms = 0.0
+    if use_graph:
+        for _ in range(args.steps // 2):
+            ctx.graph_launch()
+            ev_ms += 0.0
+        ctx.synchronize()
+    else:
+        ctx.step(args.steps)
+        ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # kernel time: one eager timed pass of the same K steps with HIP events on the solver stream
+    ctx.step(args.steps)
+    ev_ms = ctx.last_elapsed_ms()
+    clk = ctx.clock()
+
+    cells = n * n
+    value = cells * 3 * args.steps / wall
+    launches = 3 * args.steps
+    mean_launch_s = ev_ms * 1e-3 / launches
+    bytes_per_launch = BYTES_PER_CELL_STEP[args.precision] / 3.0 * cells
+    achieved = bytes_per_launch / mean_launch_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": load_traffic(n, args.precision),
+            "kernel": f"stage_{args.precision}_kernel", "mean_launch_us": round(mean_launch_s * 1e6, 2),
+            "algorithmic_bytes_per_launch": bytes_per_launch}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic (isentropic vortex + 8 seeded Fourier modes, host FP64)",
+        "config": {"workload": f"2-D Euler MUSCL(MC)+HLLC+SSP-RK3, periodic {n}x{n}, CFL 0.4, fused FP64 CFL",
+                   "grid": [n, n], "graph": bool(use_graph), "sim_time": clk["time"], "parallelism": "single GPU"},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

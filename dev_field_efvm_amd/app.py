@@ -1,0 +1,445 @@
+"""Host-side mirror of the reference's plugin API for the FV update path.
+
+A reference app is assembled from GRASS plugins (crates/cfd_solver/src/lib.rs:691-984,
+crates/cfd_boundary/src/lib.rs:457-533, crates/cfd_ibm/src/lib.rs:621-680):
+
+    app = App()
+    app.add_plugins(FieldDefaultPlugins(mesh=UniformMeshConfig(nx=64, ny=64, nz=1, ng=2,
+                                                               bounds_lo=(0, 0, 0), bounds_hi=(10, 10, 10 / 64))))
+    app.add_plugins(CfdStatePlugin(init))
+    app.add_plugins(IdealGasPlugin())
+    app.add_plugins(StructuredBoundaryPlugin(BoundaryRegistry().with_axis(0, Periodic()).with_axis(2, SlipWall())))
+    app.add_plugins(MusclPlugin.hllc().with_limiter(SlopeLimiter.MonotonizedCentral))
+    app.add_plugins(IntegratorPlugin.rk3())
+    app.add_plugins(SolverPlugin(SolverConfig(cfl=0.4)))
+    app.prepare()
+    app.run()          # one app.run() = one RK substage through every phase
+
+Here the plugins only collect configuration; `prepare()` creates one GPU
+context through the C ABI (include/efvm.h) and every `run()` enqueues the
+fused stage kernels.  `GpuPlugin` selects precision (FP64 parity / FP32
+production), the periodic-seam mode and the launch shape.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from enum import Enum
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+from .efvm import Context, EfvmError
+
+R_AIR = 287.058
+
+
+# ---------------------------------------------------------------- mesh / state
+@dataclass
+class UniformMeshConfig:
+    nx: int
+    ny: int
+    nz: int = 1
+    ng: int = 2
+    bounds_lo: Sequence[float] = (0.0, 0.0, 0.0)
+    bounds_hi: Sequence[float] = (1.0, 1.0, 1.0)
+
+
+@dataclass
+class FieldDefaultPlugins:
+    mesh: UniformMeshConfig
+
+
+class CfdStatePlugin:
+    """Initial condition: a callable init(x, y) -> (rho, rho_u, rho_v, rho_w, rho_E) over
+    cell-centroid arrays, or an array of shape (ny, nx, 5) (crates/cfd_solver/src/lib.rs:691-719)."""
+
+    def __init__(self, init):
+        self.init = init
+
+
+# ---------------------------------------------------------------- gas
+@dataclass
+class IdealGas:
+    gamma: float = 1.4
+    r_gas: float = R_AIR
+
+    @staticmethod
+    def air() -> "IdealGas":
+        return IdealGas(1.4, R_AIR)
+
+    def prim_to_cons(self, rho, u, v, w, p):
+        """IdealGas::prim_to_cons (crates/cfd_eos/src/lib.rs:55-59), vectorised."""
+        ke = 0.5 * rho * (u * u + v * v + w * w)
+        return np.stack(np.broadcast_arrays(rho, rho * u, rho * v, rho * w, p / (self.gamma - 1.0) + ke), axis=-1)
+
+
+class Viscosity:
+    """Viscosity::{Sutherland, Constant(mu)} (crates/cfd_eos/src/lib.rs:100-117)."""
+
+    def __init__(self, mu: Optional[float] = None):
+        self.mu = mu
+
+    @staticmethod
+    def sutherland() -> "Viscosity":
+        return Viscosity(None)
+
+    @staticmethod
+    def constant(mu: float) -> "Viscosity":
+        return Viscosity(mu)
+
+
+class GasPlugin:
+    def __init__(self, gas: IdealGas):
+        self.gas = gas
+
+
+class IdealGasPlugin(GasPlugin):
+    def __init__(self):
+        super().__init__(IdealGas.air())
+
+
+# ---------------------------------------------------------------- flux / limiter / integrator
+class SlopeLimiter(Enum):
+    Minmod = "minmod"
+    MonotonizedCentral = "mc"
+
+
+class FluxPlugin:
+    """First-order flux sweep (crates/cfd_solver/src/lib.rs:764-797)."""
+    muscl = False
+
+    def __init__(self, flux: str = "rusanov"):
+        self.flux = flux
+        self.limiter = SlopeLimiter.Minmod
+
+    @classmethod
+    def rusanov(cls):
+        return cls("rusanov")
+
+    @classmethod
+    def hllc(cls):
+        return cls("hllc")
+
+    @classmethod
+    def hllc_textbook(cls):
+        return cls("hllc_textbook")
+
+
+class MusclPlugin(FluxPlugin):
+    """MUSCL reconstruction + flux (crates/cfd_solver/src/lib.rs:806-842); default limiter Minmod."""
+    muscl = True
+
+    def with_limiter(self, limiter: SlopeLimiter) -> "MusclPlugin":
+        self.limiter = limiter
+        return self
+
+
+class IntegratorPlugin:
+    def __init__(self, kind: str):
+        self.kind = kind
+
+    @classmethod
+    def euler(cls):
+        return cls("euler")
+
+    @classmethod
+    def rk3(cls):
+        return cls("rk3")
+
+
+@dataclass
+class SolverConfig:
+    """SolverConfig (crates/cfd_solver/src/lib.rs:39-58)."""
+    cfl: float = 0.4
+    fixed_dt: Optional[float] = None
+    viscous: bool = False
+    muscl: bool = False
+
+
+class SolverPlugin:
+    def __init__(self, config: SolverConfig = None):
+        self.config = config or SolverConfig()
+
+
+# ---------------------------------------------------------------- boundaries
+class BoundarySide(Enum):
+    XLo = 0
+    XHi = 1
+    YLo = 2
+    YHi = 3
+    ZLo = 4
+    ZHi = 5
+
+    def axis(self) -> int:
+        return self.value // 2
+
+
+class BoundaryCondition:
+    kind = "outflow"
+    params: tuple = ()
+
+    def is_periodic(self) -> bool:
+        return False
+
+
+class SupersonicOutflow(BoundaryCondition):
+    kind = "outflow"
+
+
+class Periodic(BoundaryCondition):
+    kind = "periodic"
+
+    def is_periodic(self) -> bool:
+        return True
+
+
+class SlipWall(BoundaryCondition):
+    kind = "slip"
+
+
+class Symmetry(BoundaryCondition):
+    kind = "symmetry"
+
+
+class NoSlipWall(BoundaryCondition):
+    kind = "noslip"
+
+
+class SupersonicInflow(BoundaryCondition):
+    kind = "supersonic_inflow"
+
+    def __init__(self, rho, u, v, w, p):
+        self.params = (rho, u, v, w, p)
+
+
+class SubsonicInflow(BoundaryCondition):
+    kind = "subsonic_inflow"
+
+    def __init__(self, rho, u, v, w, t):
+        self.params = (rho, u, v, w, t)
+
+
+class PressureOutlet(BoundaryCondition):
+    kind = "pressure_outlet"
+
+    def __init__(self, p):
+        self.params = (p,)
+
+
+class BoundaryRegistry:
+    """Per-side boundary conditions; every side defaults to SupersonicOutflow
+    (crates/cfd_boundary/src/lib.rs:369-449)."""
+
+    def __init__(self):
+        self.patches = [SupersonicOutflow() for _ in range(6)]
+
+    def set(self, side: BoundarySide, bc: BoundaryCondition) -> "BoundaryRegistry":
+        self.patches[side.value] = bc
+        return self
+
+    def with_(self, side: BoundarySide, bc: BoundaryCondition) -> "BoundaryRegistry":
+        return self.set(side, bc)
+
+    def with_axis(self, axis: int, bc: BoundaryCondition) -> "BoundaryRegistry":
+        self.patches[2 * axis] = bc
+        self.patches[2 * axis + 1] = bc
+        return self
+
+    def get(self, side: BoundarySide) -> BoundaryCondition:
+        return self.patches[side.value]
+
+    def validate_periodic_pairs(self):
+        for axis in range(3):
+            lo, hi = self.patches[2 * axis], self.patches[2 * axis + 1]
+            if lo.is_periodic() != hi.is_periodic():
+                raise ValueError(f"periodic boundary on axis {axis} requires both sides; use with_axis(axis, Periodic())")
+
+
+class BoundaryPlugin:
+    def __init__(self, registry: BoundaryRegistry = None):
+        self.registry = registry or BoundaryRegistry()
+
+
+class StructuredBoundaryPlugin(BoundaryPlugin):
+    def __init__(self, registry: BoundaryRegistry = None):
+        super().__init__(registry)
+        self.registry.validate_periodic_pairs()   # panics in the reference (cfd_boundary:516-518)
+
+
+# ---------------------------------------------------------------- IBM / case systems
+@dataclass
+class Cylinder:
+    center_xy: Sequence[float]
+    radius: float
+
+
+class GhostCellIbmPlugin:
+    pass
+
+
+class CylinderIbmPlugin:
+    def __init__(self, cylinder: Cylinder, freestream):
+        self.cylinders = [cylinder]
+        self.freestream = np.asarray(freestream, dtype=np.float64)
+
+    @classmethod
+    def with_cylinders(cls, cylinders, freestream):
+        p = cls(cylinders[0], freestream)
+        p.cylinders = list(cylinders)
+        return p
+
+
+@dataclass
+class ParabolicInflow:
+    """examples/cylinder_drag_validation/main.rs:146-177."""
+    u_max: float
+    height: float
+    rho: float
+
+
+@dataclass
+class OutletSponge:
+    """examples/cylinder_drag_validation/main.rs:179-229."""
+    x_start: float
+    x_end: float
+    sigma: float
+    u_max: float
+    height: float
+    rho: float
+    p: float
+
+
+@dataclass
+class BodyForceX:
+    """crates/cfd_solver/tests/poiseuille_channel_validation.rs:102-120."""
+    gx: float
+
+
+@dataclass
+class GpuPlugin:
+    precision: str = "fp64"        # "fp64" (parity) or "fp32" (production)
+    seam_mode: int = 0             # 0 reference, 1 conservative periodic seams
+    strip_rows: int = 0            # 0: default / autotuned
+    waves_per_block: int = 0
+
+
+@dataclass
+class SolverState:
+    dt: float
+    time: float
+    step: int
+    rk_stage: int
+
+
+# ---------------------------------------------------------------- the App
+class App:
+    def __init__(self):
+        self.plugins = []
+        self.resources = {}
+        self.ctx: Optional[Context] = None
+
+    def add_plugins(self, p) -> "App":
+        self.plugins.append(p)
+        return self
+
+    def add_resource(self, r) -> "App":
+        self.resources[type(r)] = r
+        return self
+
+    def _find(self, cls):
+        for p in self.plugins:
+            if isinstance(p, cls):
+                return p
+        return None
+
+    def prepare(self):
+        mesh = self._find(FieldDefaultPlugins)
+        if mesh is None:
+            raise ValueError("mesh plugin missing (FieldDefaultPlugins)")
+        m = mesh.mesh
+        if m.nz != 1 or m.ng != 2:
+            raise EfvmError(4, "the GPU path is the 2-D nz = 1, ng = 2 UniformMesh")
+        gasp = self._find(GasPlugin) or IdealGasPlugin()
+        flux = self._find(FluxPlugin)
+        if flux is None:
+            raise ValueError("riemann_flux plugin missing (FluxPlugin / MusclPlugin)")
+        integ = self._find(IntegratorPlugin) or IntegratorPlugin.euler()
+        solver = self._find(SolverPlugin)
+        if solver is None:
+            raise ValueError("solver plugin missing (SolverPlugin)")
+        bcp = self._find(BoundaryPlugin) or BoundaryPlugin()
+        gpu = self._find(GpuPlugin) or GpuPlugin()
+        visc = self.resources.get(Viscosity, Viscosity.sutherland())
+        reg = bcp.registry
+        reg.validate_periodic_pairs()
+        for s in (BoundarySide.ZLo, BoundarySide.ZHi):
+            if reg.get(s).kind not in ("slip", "symmetry"):
+                raise EfvmError(4, "z sides must be slip walls on the 2-D GPU path")
+        lo, hi = m.bounds_lo, m.bounds_hi
+        cfg = solver.config
+        self.ctx = Context(m.nx, m.ny, lo=(lo[0], lo[1]), hi=(hi[0], hi[1]), dz=hi[2] - lo[2],
+                           gamma=gasp.gas.gamma, r_gas=gasp.gas.r_gas, mu=visc.mu, flux=flux.flux,
+                           muscl=flux.muscl, limiter=flux.limiter.value, integrator=integ.kind,
+                           precision=gpu.precision, viscous=cfg.viscous, seam_mode=gpu.seam_mode,
+                           cfl=cfg.cfl, fixed_dt=cfg.fixed_dt)
+        if gpu.strip_rows or gpu.waves_per_block:
+            self.ctx.set_launch(gpu.strip_rows, gpu.waves_per_block)
+        self.ctx.set_boundaries([(reg.get(BoundarySide(s)).kind, reg.get(BoundarySide(s)).params) for s in range(4)])
+        inflow = self.resources.get(ParabolicInflow)
+        if inflow:
+            self.ctx.set_parabolic_inflow(inflow.u_max, inflow.height, inflow.rho)
+        sp = self.resources.get(OutletSponge)
+        if sp:
+            self.ctx.set_sponge(sp.x_start, sp.x_end, sp.sigma, sp.u_max, sp.height, sp.rho, sp.p)
+        bf = self.resources.get(BodyForceX)
+        if bf:
+            self.ctx.set_body_force(bf.gx)
+        cyl = self._find(CylinderIbmPlugin)
+        if cyl:
+            for cy in cyl.cylinders:
+                self.ctx.add_cylinder(cy.center_xy[0], cy.center_xy[1], cy.radius, cyl.freestream)
+        st = self._find(CfdStatePlugin)
+        if st is None:
+            raise ValueError("CfdStatePlugin missing")
+        init = st.init
+        if callable(init):
+            x, y = self.cell_centroids()
+            u = np.asarray(init(x, y), dtype=np.float64)
+        else:
+            u = np.asarray(init, dtype=np.float64)
+        self.ctx.upload(u.reshape(m.ny, m.nx, 5))
+        self.mesh = m
+        return self
+
+    def cell_centroids(self):
+        m = self._find(FieldDefaultPlugins).mesh
+        dx = (m.bounds_hi[0] - m.bounds_lo[0]) / m.nx
+        dy = (m.bounds_hi[1] - m.bounds_lo[1]) / m.ny
+        x = m.bounds_lo[0] + (np.arange(m.nx) + 0.5) * dx
+        y = m.bounds_lo[1] + (np.arange(m.ny) + 0.5) * dy
+        return np.meshgrid(x, y)
+
+    # one app.run() = one RK substage
+    def run(self):
+        self.ctx.run()
+
+    def step(self, n: int = 1):
+        self.ctx.step(n)
+
+    def advance_to(self, t_final: float, max_steps: int = 10 ** 9) -> int:
+        return self.ctx.advance_to(t_final, max_steps)
+
+    def solver_state(self) -> SolverState:
+        c = self.ctx.clock()
+        return SolverState(c["dt"], c["time"], c["step"], c["stage"])
+
+    def state(self) -> np.ndarray:
+        """CfdState.u of the interior, shape (ny, nx, 5), Component wire order."""
+        return self.ctx.download()
+
+    def marker_force(self) -> np.ndarray:
+        return self.ctx.marker_force()
+
+    def conservation_sums(self) -> np.ndarray:
+        return self.ctx.sums()

@@ -1,0 +1,664 @@
+// efvm_api.cpp -- implementation of the C ABI declared in include/efvm.h.
+//
+// A context owns three plane sets A, B, C (4 SoA planes each) in HBM:
+//   RK3 step:  stage 1  A -> B ;  stage 2  (B, U0=A) -> C ;  stage 3  (C, U0=A) -> A
+// Stage 3 writes U^{n+1} over U^n in place (pointwise read of U0, no stencil on
+// it), so U0 never needs a copy and the whole step is 128 B/cell of HBM traffic
+// in FP32 (256 B in FP64).  Euler ping-pongs A <-> B.  dt lives in device
+// memory (DevClock): the stage kernels read lambda_max and form dt = cfl/lambda
+// themselves, so a step needs no host round trip and can be graph-captured.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/efvm.h"
+#include "efvm_stage.h"
+
+namespace efvm {
+hipError_t launch_stage_f64(const StageArgs& a, int block, hipStream_t st);
+hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st);
+template <class T> hipError_t launch_cfl(const StageArgs& a, const void* const* planes, double* target, hipStream_t st);
+hipError_t launch_clock_set(DevClock* clk, double dt, double time, long long step, hipStream_t st);
+struct IbmArgs {
+    int nx, ny;
+    double lo_x, lo_y, dx, dy, dz;
+    double gamma, r_gas;
+    int per_x, per_y;
+    const uint8_t* solid;
+};
+template <class T> hipError_t launch_ibm_mirror(const StageArgs& sa, const IbmArgs& a, int n, const int* idx, const double* img, hipStream_t st);
+template <class T> hipError_t launch_marker_force(const StageArgs& sa, const IbmArgs& a, int n, const double* mk, double off,
+                                                  int visc_const, double mu, double* out, hipStream_t st);
+template <class T> hipError_t launch_sponge(const StageArgs& a, double x0, double x1, double sig, double um, double h,
+                                            double rho, double p, hipStream_t st);
+template <class T> hipError_t launch_sums(const StageArgs& a, double* part, int nb, double vol, double* out, hipStream_t st);
+hipError_t launch_ibm_flip(void* const* planes, int esz, const uint8_t* solid, long n, const double* stagnant, hipStream_t st);
+}  // namespace efvm
+
+using namespace efvm;
+
+struct efvm_ctx {
+    efvm_grid grid{};
+    efvm_gas gas{};
+    efvm_scheme sch{};
+    efvm_bc bc[4]{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t esz = 8;
+    long ncell = 0;
+    void* P[3][4] = {};            // plane sets A, B, C
+    int cur = 0;                   // index of the set holding U^n (A)
+    DevClock* clock = nullptr;
+    double* yw = nullptr;
+    uint8_t* solid = nullptr;
+    int* ghost_idx = nullptr;
+    double* ghost_img = nullptr;
+    int n_ghost = 0;
+    double* markers = nullptr;
+    int n_mark = 0;
+    double* scratch = nullptr;     // sums partials (1024*4) + 6 marker + 5 sums
+    double* stagnant = nullptr;
+    int ibm_flipped = 0;
+    int rk_stage = 0;
+    int parity = 0;
+    int lam_valid = 0;
+    int strip_h = 32;
+    int waves_per_block = 4;
+    int xcd_remap = 1;
+    int parabolic = 0;
+    double inflow[3] = {0, 0, 0};
+    int sponge = 0;
+    double sp[7] = {0, 0, 0, 0, 0, 0, 0};
+    int body_force = 0;
+    double gx = 0.0;
+    int n_cyl = 0;
+    double cyl[8][3] = {};
+    double freestream[5] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_steps = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int timed = 0;
+    std::string err;
+};
+
+namespace {
+
+efvm_status fail(efvm_ctx* c, efvm_status s, const std::string& msg) {
+    if (c) c->err = msg;
+    return s;
+}
+#define HIPCHK(ctx, expr)                                                                              \
+    do {                                                                                               \
+        hipError_t _e = (expr);                                                                        \
+        if (_e != hipSuccess) return fail(ctx, EFVM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+bool periodic_axis(const efvm_ctx* c, int axis) { return c->bc[2 * axis].kind == EFVM_BC_PERIODIC; }
+
+StageArgs base_args(const efvm_ctx* c) {
+    StageArgs a;
+    std::memset(&a, 0, sizeof(a));
+    const efvm_grid& g = c->grid;
+    a.nx = g.nx; a.ny = g.ny;
+    a.per_x = periodic_axis(c, 0); a.per_y = periodic_axis(c, 1);
+    a.seam_mode = c->sch.seam_mode;
+    for (int s = 0; s < 4; s++) {
+        a.bc_kind[s] = c->bc[s].kind;
+        for (int q = 0; q < 5; q++) a.bc_par[s][q] = c->bc[s].par[q];
+        if (c->bc[s].kind == EFVM_BC_SUPERSONIC_INFLOW) {
+            const double* p = c->bc[s].par;
+            P6<double> pv = {p[0], p[1], p[2], p[3], p[4], p[4] / (p[0] * 287.058)};
+            C5<double> o = ref::prim_to_cons(pv, c->gas.gamma);
+            a.bc_cons[s][0] = o.r; a.bc_cons[s][1] = o.ru; a.bc_cons[s][2] = o.rv; a.bc_cons[s][3] = o.rw; a.bc_cons[s][4] = o.re;
+        }
+    }
+    a.parabolic = c->parabolic;
+    a.inflow_umax = c->inflow[0]; a.inflow_h = c->inflow[1]; a.inflow_rho = c->inflow[2];
+    a.lo_x = g.lo_x; a.lo_y = g.lo_y;
+    a.dx = (g.hi_x - g.lo_x) / (double)g.nx;
+    a.dy = (g.hi_y - g.lo_y) / (double)g.ny;
+    a.dz = g.dz;
+    double vol = a.dx * a.dy * a.dz;
+    a.inv_x = (a.dy * a.dz) / vol;
+    a.inv_y = (a.dx * a.dz) / vol;
+    a.inv_z = (a.dx * a.dy) / vol;
+    a.gamma = c->gas.gamma; a.r_gas = c->gas.r_gas;
+    a.flux = c->sch.flux; a.muscl = c->sch.recon == EFVM_RECON_MUSCL; a.limiter = c->sch.limiter;
+    a.viscous = c->sch.viscous; a.visc_const = c->gas.viscosity == EFVM_VISC_CONSTANT; a.mu = c->gas.mu;
+    a.dxi2 = 1.0 / (a.dx * a.dx);
+    a.yw = c->yw;
+    a.body_force = c->body_force; a.gx = c->gx;
+    a.solid = c->n_cyl > 0 ? c->solid : nullptr;
+    a.euler = c->sch.integrator == EFVM_INTEGRATOR_EULER;
+    a.cfl = c->sch.cfl; a.fixed_dt = c->sch.fixed_dt;
+    a.clock = c->clock;
+    a.strip_h = c->strip_h;
+    a.n_strips_x = (g.nx + STRIP_OUT - 1) / STRIP_OUT;
+    a.n_strips_y = (g.ny + c->strip_h - 1) / c->strip_h;
+    a.xcd_remap = c->xcd_remap;
+    return a;
+}
+
+IbmArgs ibm_args(const efvm_ctx* c, const StageArgs& a) {
+    IbmArgs b;
+    b.nx = a.nx; b.ny = a.ny; b.lo_x = a.lo_x; b.lo_y = a.lo_y; b.dx = a.dx; b.dy = a.dy; b.dz = a.dz;
+    b.gamma = a.gamma; b.r_gas = a.r_gas; b.per_x = a.per_x; b.per_y = a.per_y; b.solid = c->solid;
+    return b;
+}
+
+void set_planes(StageArgs& a, void* const* in, void* const* u0, void* const* out) {
+    for (int m = 0; m < 4; m++) { a.uin[m] = in[m]; a.u0[m] = u0[m]; a.uout[m] = out[m]; }
+}
+
+efvm_status launch_stage(efvm_ctx* c, const StageArgs& a) {
+    int block = WAVE * c->waves_per_block;
+    hipError_t e = c->esz == 8 ? launch_stage_f64(a, block, c->stream) : launch_stage_f32(a, block, c->stream);
+    if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("stage kernel launch: ") + hipGetErrorString(e));
+    return EFVM_OK;
+}
+
+efvm_status launch_cfl_into(efvm_ctx* c, void* const* planes, double* target) {
+    StageArgs a = base_args(c);
+    hipError_t e = c->esz == 8 ? launch_cfl<double>(a, planes, target, c->stream) : launch_cfl<float>(a, planes, target, c->stream);
+    if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("cfl kernel: ") + hipGetErrorString(e));
+    return EFVM_OK;
+}
+
+// One substage (one app.run()).  Enqueue only; no host synchronisation.
+efvm_status run_substage(efvm_ctx* c) {
+    const bool rk3 = c->sch.integrator == EFVM_INTEGRATOR_RK3;
+    void* const* A = c->P[c->cur];
+    void* const* B = c->P[(c->cur + 1) % 3];
+    void* const* Cc = c->P[(c->cur + 2) % 3];
+    efvm_status st;
+    // Setup: CFL of U^n (first substage only) ...
+    if (c->rk_stage == 0 && !c->lam_valid && c->sch.fixed_dt <= 0.0) {
+        if ((st = launch_cfl_into(c, A, &c->clock->lam[c->parity])) != EFVM_OK) return st;
+        c->lam_valid = 1;
+    }
+    // ... then the cylinder mask refresh: cells inside a body become the stagnant state (once, static body).
+    if (c->n_cyl > 0 && !c->ibm_flipped) {
+        hipError_t e = launch_ibm_flip(A, (int)c->esz, c->solid, c->ncell, c->stagnant, c->stream);
+        if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "ibm flip");
+        c->ibm_flipped = 1;
+    }
+    StageArgs a = base_args(c);
+    a.parity = c->parity;
+    a.stage = rk3 ? c->rk_stage : 0;
+    void* const* in = A;
+    void* const* out = B;
+    if (rk3 && c->rk_stage == 1) { in = B; out = Cc; }
+    if (rk3 && c->rk_stage == 2) { in = Cc; out = A; }
+    set_planes(a, in, A, out);
+    const bool last = !rk3 || c->rk_stage == 2;
+    a.fuse_cfl = last && rk3 && !c->sponge && c->sch.fixed_dt <= 0.0;
+    // PreFlux: IBM mirror into the stage input
+    if (c->n_cyl > 0 && c->n_ghost > 0) {
+        IbmArgs ib = ibm_args(c, a);
+        hipError_t e = c->esz == 8 ? launch_ibm_mirror<double>(a, ib, c->n_ghost, c->ghost_idx, c->ghost_img, c->stream)
+                                   : launch_ibm_mirror<float>(a, ib, c->n_ghost, c->ghost_idx, c->ghost_img, c->stream);
+        if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "ibm mirror");
+    }
+    if ((st = launch_stage(c, a)) != EFVM_OK) return st;
+    if (last) {
+        if (!rk3) c->cur = (c->cur + 1) % 3;   // Euler: U^{n+1} is in B
+        StageArgs b = base_args(c);
+        set_planes(b, c->P[c->cur], c->P[c->cur], c->P[c->cur]);
+        if (c->sponge) {
+            hipError_t e = c->esz == 8 ? launch_sponge<double>(b, c->sp[0], c->sp[1], c->sp[2], c->sp[3], c->sp[4], c->sp[5], c->sp[6], c->stream)
+                                       : launch_sponge<float>(b, c->sp[0], c->sp[1], c->sp[2], c->sp[3], c->sp[4], c->sp[5], c->sp[6], c->stream);
+            if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "sponge");
+        }
+        if (c->sch.fixed_dt <= 0.0 && !a.fuse_cfl) {
+            if ((st = launch_cfl_into(c, c->P[c->cur], &c->clock->lam[c->parity ^ 1])) != EFVM_OK) return st;
+        }
+        c->parity ^= 1;
+        c->rk_stage = 0;
+    } else {
+        c->rk_stage++;
+    }
+    return EFVM_OK;
+}
+
+efvm_status device_alloc(efvm_ctx* c, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return EFVM_OK;
+}
+
+void free_ibm(efvm_ctx* c) {
+    if (c->solid) hipFree(c->solid);
+    if (c->ghost_idx) hipFree(c->ghost_idx);
+    if (c->ghost_img) hipFree(c->ghost_img);
+    if (c->markers) hipFree(c->markers);
+    if (c->stagnant) hipFree(c->stagnant);
+    c->solid = nullptr; c->ghost_idx = nullptr; c->ghost_img = nullptr; c->markers = nullptr; c->stagnant = nullptr;
+}
+
+void drop_graph(efvm_ctx* c) {
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr; c->graph = nullptr; c->graph_steps = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+efvm_status efvm_ctx_create(const efvm_grid* grid, const efvm_gas* gas, const efvm_scheme* scheme, efvm_ctx** out) {
+    if (!grid || !gas || !scheme || !out) return EFVM_ERR_ARG;
+    *out = nullptr;
+    if (grid->nx < 1 || grid->ny < 1 || !(grid->hi_x > grid->lo_x) || !(grid->hi_y > grid->lo_y) || !(grid->dz > 0.0))
+        return EFVM_ERR_CONFIG;
+    if (scheme->flux < 0 || scheme->flux > 2 || scheme->recon < 0 || scheme->recon > 1 || scheme->limiter < 0 ||
+        scheme->limiter > 1 || scheme->integrator < 0 || scheme->integrator > 1 || scheme->precision < 0 || scheme->precision > 1)
+        return EFVM_ERR_CONFIG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return EFVM_ERR_NO_DEVICE;
+    efvm_ctx* c = new (std::nothrow) efvm_ctx();
+    if (!c) return EFVM_ERR_ARG;
+    c->grid = *grid; c->gas = *gas; c->sch = *scheme;
+    for (int s = 0; s < 4; s++) { c->bc[s].kind = EFVM_BC_OUTFLOW; for (int q = 0; q < 5; q++) c->bc[s].par[q] = 0.0; }
+    c->esz = scheme->precision == EFVM_FP64 ? 8 : 4;
+    c->ncell = (long)grid->nx * grid->ny;
+    if (hipGetDevice(&c->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return EFVM_ERR_HIP;
+    }
+    efvm_status st = EFVM_OK;
+    for (int s = 0; s < 3 && st == EFVM_OK; s++)
+        for (int m = 0; m < 4 && st == EFVM_OK; m++) st = device_alloc(c, &c->P[s][m], c->esz * c->ncell);
+    if (st == EFVM_OK) st = device_alloc(c, (void**)&c->clock, sizeof(DevClock));
+    if (st == EFVM_OK) st = device_alloc(c, (void**)&c->yw, sizeof(double) * 3 * grid->ny);
+    if (st == EFVM_OK) st = device_alloc(c, (void**)&c->scratch, sizeof(double) * (1024 * 4 + 16));
+    if (st != EFVM_OK) { efvm_ctx_destroy(c); return st; }
+    for (int s = 0; s < 3; s++)
+        for (int m = 0; m < 4; m++) hipMemsetAsync(c->P[s][m], 0, c->esz * c->ncell, c->stream);
+    // viscous y weights from cell-centre distances (crates/cfd_solver/src/lib.rs:483-493)
+    std::vector<double> yw(3 * grid->ny);
+    double dy = (grid->hi_y - grid->lo_y) / (double)grid->ny;
+    auto yc = [&](int j) { return grid->lo_y + ((double)j + 0.5) * dy; };
+    for (int j = 0; j < grid->ny; j++) {
+        double hm = yc(j) - yc(j - 1), hp = yc(j + 1) - yc(j);
+        yw[j] = 1.0 / hm; yw[grid->ny + j] = 1.0 / hp; yw[2 * grid->ny + j] = 2.0 / (hm + hp);
+    }
+    hipMemcpyAsync(c->yw, yw.data(), sizeof(double) * yw.size(), hipMemcpyHostToDevice, c->stream);
+    launch_clock_set(c->clock, 0.0, 0.0, 0, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { efvm_ctx_destroy(c); return EFVM_ERR_HIP; }
+    *out = c;
+    return EFVM_OK;
+}
+
+void efvm_ctx_destroy(efvm_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (int s = 0; s < 3; s++) for (int m = 0; m < 4; m++) if (c->P[s][m]) hipFree(c->P[s][m]);
+    if (c->clock) hipFree(c->clock);
+    if (c->yw) hipFree(c->yw);
+    if (c->scratch) hipFree(c->scratch);
+    free_ibm(c);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* efvm_last_error(const efvm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+efvm_status efvm_set_launch(efvm_ctx* c, int strip_rows, int waves_per_block) {
+    if (!c) return EFVM_ERR_ARG;
+    if (strip_rows < 0 || strip_rows > 4096 || waves_per_block < 0 || waves_per_block > 16)
+        return fail(c, EFVM_ERR_ARG, "launch shape out of range");
+    if (strip_rows) c->strip_h = strip_rows;
+    if (waves_per_block) c->waves_per_block = waves_per_block;
+    drop_graph(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm_set_boundaries(efvm_ctx* c, const efvm_bc sides[4]) {
+    if (!c || !sides) return EFVM_ERR_ARG;
+    for (int s = 0; s < 4; s++)
+        if (sides[s].kind < 0 || sides[s].kind > EFVM_BC_PRESSURE_OUTLET) return fail(c, EFVM_ERR_CONFIG, "unknown boundary kind");
+    for (int ax = 0; ax < 2; ax++)
+        if ((sides[2 * ax].kind == EFVM_BC_PERIODIC) != (sides[2 * ax + 1].kind == EFVM_BC_PERIODIC))
+            return fail(c, EFVM_ERR_CONFIG, std::string("periodic boundary on axis ") + std::to_string(ax) +
+                                                " requires both sides periodic (crates/cfd_boundary/src/lib.rs:437-448)");
+    for (int s = 0; s < 4; s++) c->bc[s] = sides[s];
+    drop_graph(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm_set_parabolic_inflow(efvm_ctx* c, double u_max, double height, double rho) {
+    if (!c) return EFVM_ERR_ARG;
+    c->parabolic = 1; c->inflow[0] = u_max; c->inflow[1] = height; c->inflow[2] = rho;
+    drop_graph(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm_set_sponge(efvm_ctx* c, double x0, double x1, double sigma, double u_max, double height, double rho, double p) {
+    if (!c) return EFVM_ERR_ARG;
+    if (c->sch.integrator != EFVM_INTEGRATOR_RK3) return fail(c, EFVM_ERR_UNSUPPORTED, "sponge needs RK3");
+    c->sponge = 1;
+    double v[7] = {x0, x1, sigma, u_max, height, rho, p};
+    std::memcpy(c->sp, v, sizeof(v));
+    drop_graph(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm_set_body_force(efvm_ctx* c, double gx) {
+    if (!c) return EFVM_ERR_ARG;
+    c->body_force = 1; c->gx = gx;
+    drop_graph(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm_upload_state(efvm_ctx* c, const double* cons5, size_t n) {
+    if (!c || !cons5) return EFVM_ERR_ARG;
+    if ((long)n != c->ncell) return fail(c, EFVM_ERR_ARG, "upload: n_cells != nx*ny");
+    std::vector<char> h[4];
+    for (int m = 0; m < 4; m++) h[m].resize(c->esz * n);
+    const int comp[4] = {0, 1, 2, 4};
+    for (size_t q = 0; q < n; q++) {
+        if (cons5[5 * q + 3] != 0.0) return fail(c, EFVM_ERR_UNSUPPORTED, "rho_w must be 0 on the 2-D GPU path");
+        for (int m = 0; m < 4; m++) {
+            double v = cons5[5 * q + comp[m]];
+            if (c->esz == 8) reinterpret_cast<double*>(h[m].data())[q] = v;
+            else reinterpret_cast<float*>(h[m].data())[q] = (float)v;
+        }
+    }
+    c->rk_stage = 0;
+    for (int m = 0; m < 4; m++)
+        HIPCHK(c, hipMemcpyAsync(c->P[c->cur][m], h[m].data(), c->esz * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->lam_valid = 0;
+    return EFVM_OK;
+}
+
+efvm_status efvm_download_state(efvm_ctx* c, double* cons5, size_t n) {
+    if (!c || !cons5) return EFVM_ERR_ARG;
+    if ((long)n != c->ncell) return fail(c, EFVM_ERR_ARG, "download: n_cells != nx*ny");
+    // the state of the current substage: A at a step boundary, else B / C
+    int set = c->cur;
+    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 1) set = (c->cur + 1) % 3;
+    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 2) set = (c->cur + 2) % 3;
+    std::vector<char> h[4];
+    for (int m = 0; m < 4; m++) {
+        h[m].resize(c->esz * n);
+        HIPCHK(c, hipMemcpyAsync(h[m].data(), c->P[set][m], c->esz * n, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int comp[4] = {0, 1, 2, 4};
+    for (size_t q = 0; q < n; q++) {
+        cons5[5 * q + 3] = 0.0;
+        for (int m = 0; m < 4; m++)
+            cons5[5 * q + comp[m]] = c->esz == 8 ? reinterpret_cast<double*>(h[m].data())[q]
+                                                 : (double)reinterpret_cast<float*>(h[m].data())[q];
+    }
+    return EFVM_OK;
+}
+
+efvm_status efvm_get_clock(efvm_ctx* c, double* dt, double* time, long* step, int* stage) {
+    if (!c) return EFVM_ERR_ARG;
+    DevClock h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->clock, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (dt) *dt = h.dt;
+    if (time) *time = h.time;
+    if (step) *step = (long)h.step;
+    if (stage) *stage = c->rk_stage;
+    if (h.bad) return fail(c, EFVM_ERR_NONPHYSICAL, "CFL scan saw " + std::to_string(h.bad) + " non-finite or non-positive-density cells");
+    return EFVM_OK;
+}
+
+efvm_status efvm_set_clock(efvm_ctx* c, double time, long step) {
+    if (!c) return EFVM_ERR_ARG;
+    HIPCHK(c, launch_clock_set(c->clock, 0.0, time, step, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->lam_valid = 0; c->rk_stage = 0; c->parity = 0;
+    return EFVM_OK;
+}
+
+efvm_status efvm_cfl(efvm_ctx* c, double* dt_out) {
+    if (!c || !dt_out) return EFVM_ERR_ARG;
+    if (c->sch.fixed_dt > 0.0) { *dt_out = c->sch.fixed_dt; return EFVM_OK; }
+    efvm_status st = launch_cfl_into(c, c->P[c->cur], &c->clock->lam_tmp);
+    if (st != EFVM_OK) return st;
+    double lam = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&lam, &c->clock->lam_tmp, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *dt_out = lam > 0.0 ? c->sch.cfl / lam : INFINITY;
+    return EFVM_OK;
+}
+
+efvm_status efvm_run(efvm_ctx* c) {
+    if (!c) return EFVM_ERR_ARG;
+    return run_substage(c);
+}
+
+efvm_status efvm_step(efvm_ctx* c, int n_steps) {
+    if (!c || n_steps < 0) return EFVM_ERR_ARG;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    const int nsub = c->sch.integrator == EFVM_INTEGRATOR_RK3 ? 3 : 1;
+    for (int k = 0; k < n_steps; k++)
+        for (int q = 0; q < nsub; q++) {
+            efvm_status st = run_substage(c);
+            if (st != EFVM_OK) return st;
+        }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = 1;
+    return EFVM_OK;
+}
+
+efvm_status efvm_advance_to(efvm_ctx* c, double t_final, long max_steps, long* done) {
+    if (!c) return EFVM_ERR_ARG;
+    if (c->rk_stage != 0) return fail(c, EFVM_ERR_ARG, "advance_to must start at a step boundary");
+    long k = 0;
+    const int nsub = c->sch.integrator == EFVM_INTEGRATOR_RK3 ? 3 : 1;
+    double t = 0.0;
+    efvm_status st = efvm_get_clock(c, nullptr, &t, nullptr, nullptr);
+    if (st != EFVM_OK && st != EFVM_ERR_NONPHYSICAL) return st;
+    // the loop condition needs the clock every step (t >= t_final before each whole step)
+    while (t < t_final && k < max_steps) {
+        for (int q = 0; q < nsub; q++) if ((st = run_substage(c)) != EFVM_OK) return st;
+        k++;
+        DevClock h;
+        HIPCHK(c, hipMemcpyAsync(&h, c->clock, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        t = h.time;
+        if (h.bad) { if (done) *done = k; return fail(c, EFVM_ERR_NONPHYSICAL, "non-physical state during advance"); }
+    }
+    if (done) *done = k;
+    return EFVM_OK;
+}
+
+efvm_status efvm_synchronize(efvm_ctx* c) {
+    if (!c) return EFVM_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm_graph_capture(efvm_ctx* c, int n_steps) {
+    if (!c || n_steps <= 0 || (n_steps & 1)) return fail(c, EFVM_ERR_ARG, "graph capture needs an even positive step count");
+    if (c->rk_stage != 0) return fail(c, EFVM_ERR_ARG, "graph capture must start at a step boundary");
+    drop_graph(c);
+    // dt of the first captured step must already be on the device
+    if (!c->lam_valid && c->sch.fixed_dt <= 0.0) {
+        efvm_status st = launch_cfl_into(c, c->P[c->cur], &c->clock->lam[c->parity]);
+        if (st != EFVM_OK) return st;
+        c->lam_valid = 1;
+    }
+    if (c->n_cyl > 0 && !c->ibm_flipped) {
+        HIPCHK(c, launch_ibm_flip(c->P[c->cur], (int)c->esz, c->solid, c->ncell, c->stagnant, c->stream));
+        c->ibm_flipped = 1;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int cur0 = c->cur, par0 = c->parity;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    efvm_status st = EFVM_OK;
+    const int nsub = c->sch.integrator == EFVM_INTEGRATOR_RK3 ? 3 : 1;
+    for (int k = 0; k < n_steps && st == EFVM_OK; k++)
+        for (int q = 0; q < nsub && st == EFVM_OK; q++) st = run_substage(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (st != EFVM_OK) { if (g) hipGraphDestroy(g); return st; }
+    if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("end capture: ") + hipGetErrorString(e));
+    c->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+    c->graph_steps = n_steps;
+    // capture only recorded work: restore the host-side step machinery
+    c->cur = cur0; c->parity = par0; c->rk_stage = 0;
+    // Euler with an even step count returns the state to set cur0; RK3 keeps it in place.
+    return EFVM_OK;
+}
+
+efvm_status efvm_graph_launch(efvm_ctx* c) {
+    if (!c || !c->graph_exec) return fail(c, EFVM_ERR_ARG, "no captured graph");
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = 1;
+    return EFVM_OK;
+}
+
+efvm_status efvm_last_elapsed_ms(efvm_ctx* c, double* ms) {
+    if (!c || !ms || !c->timed) return EFVM_ERR_ARG;
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f;
+    return EFVM_OK;
+}
+
+efvm_status efvm_ibm_add_cylinder(efvm_ctx* c, double cx, double cy, double r, const double fs[5]) {
+    if (!c || !fs || !(r > 0.0)) return EFVM_ERR_ARG;
+    if (c->n_cyl >= 8) return fail(c, EFVM_ERR_UNSUPPORTED, "at most 8 cylinders");
+    c->cyl[c->n_cyl][0] = cx; c->cyl[c->n_cyl][1] = cy; c->cyl[c->n_cyl][2] = r;
+    c->n_cyl++;
+    std::memcpy(c->freestream, fs, sizeof(double) * 5);
+    drop_graph(c);
+    free_ibm(c);
+    // mask, ghost list and markers on the host in FP64, exactly as the reference
+    // builds them (cfd_ibm:155-254, 566-598); a static body needs them once.
+    const efvm_grid& g = c->grid;
+    const long n = c->ncell;
+    double dx = (g.hi_x - g.lo_x) / g.nx, dy = (g.hi_y - g.lo_y) / g.ny, dz = g.dz;
+    std::vector<uint8_t> solid(n, 0);
+    auto sd = [&](int b, double x, double y) {
+        double ddx = x - c->cyl[b][0], ddy = y - c->cyl[b][1];
+        return std::sqrt(ddx * ddx + ddy * ddy) - c->cyl[b][2];
+    };
+    for (int j = 0; j < g.ny; j++)
+        for (int i = 0; i < g.nx; i++) {
+            double x = g.lo_x + ((double)i + 0.5) * dx, y = g.lo_y + ((double)j + 0.5) * dy;
+            for (int b = 0; b < c->n_cyl; b++) if (sd(b, x, y) < 0.0) solid[(long)j * g.nx + i] = 1;
+        }
+    std::vector<int> gidx;
+    std::vector<double> gimg;
+    for (int j = 0; j < g.ny; j++)
+        for (int i = 0; i < g.nx; i++) {
+            long q = (long)j * g.nx + i;
+            if (!solid[q]) continue;
+            // z neighbours are slip-wall ghosts (never solid) on the one-cell-thick mesh,
+            // so every solid cell has a fluid neighbour and is an IBM ghost.
+            double x = g.lo_x + ((double)i + 0.5) * dx, y = g.lo_y + ((double)j + 0.5) * dy, z = 0.0 + 0.5 * dz;
+            double best_abs = 1.79769313486231570815e308, best_signed = 0.0, bn[3] = {1.0, 0.0, 0.0};
+            for (int b = 0; b < c->n_cyl; b++) {
+                double s = sd(b, x, y);
+                if (std::fabs(s) < best_abs) {
+                    best_abs = std::fabs(s); best_signed = s;
+                    double ddx = x - c->cyl[b][0], ddy = y - c->cyl[b][1];
+                    double rr = std::sqrt(ddx * ddx + ddy * ddy);
+                    if (rr < 1e-12) { bn[0] = 1.0; bn[1] = 0.0; } else { bn[0] = ddx / rr; bn[1] = ddy / rr; }
+                    bn[2] = 0.0;
+                }
+            }
+            double push = 2.0 * (-best_signed);
+            gidx.push_back((int)q);
+            gimg.push_back(x + push * bn[0]); gimg.push_back(y + push * bn[1]); gimg.push_back(z + push * bn[2]);
+        }
+    std::vector<double> mk;
+    const double PI = 3.14159265358979323846;
+    double z_lo = (0.0 + 0.5 * dz) - 0.5 * dz, z_hi = (0.0 + 0.5 * dz) + 0.5 * dz, lz = z_hi - z_lo;
+    for (int b = 0; b < c->n_cyl; b++) {
+        double area = 2.0 * PI * c->cyl[b][2] * lz / (double)(64 * 1);
+        for (int it = 0; it < 64; it++) {
+            double th = 2.0 * PI * ((double)it + 0.5) / 64.0;
+            double nxv = std::cos(th), nyv = std::sin(th);
+            double z = z_lo + (0.0 + 0.5) * lz / 1.0;
+            double m[7] = {c->cyl[b][0] + c->cyl[b][2] * nxv, c->cyl[b][1] + c->cyl[b][2] * nyv, z, nxv, nyv, 0.0, area};
+            mk.insert(mk.end(), m, m + 7);
+        }
+    }
+    C5<double> f = {fs[0], fs[1], fs[2], fs[3], fs[4]};
+    double stag[5] = {f.r, 0.0, 0.0, 0.0, f.re - ref::kinetic(f)};
+    efvm_status st = EFVM_OK;
+    if ((st = device_alloc(c, (void**)&c->solid, n)) != EFVM_OK) return st;
+    if ((st = device_alloc(c, (void**)&c->ghost_idx, sizeof(int) * std::max<size_t>(1, gidx.size()))) != EFVM_OK) return st;
+    if ((st = device_alloc(c, (void**)&c->ghost_img, sizeof(double) * std::max<size_t>(3, gimg.size()))) != EFVM_OK) return st;
+    if ((st = device_alloc(c, (void**)&c->markers, sizeof(double) * mk.size())) != EFVM_OK) return st;
+    if ((st = device_alloc(c, (void**)&c->stagnant, sizeof(double) * 5)) != EFVM_OK) return st;
+    HIPCHK(c, hipMemcpy(c->solid, solid.data(), n, hipMemcpyHostToDevice));
+    if (!gidx.empty()) {
+        HIPCHK(c, hipMemcpy(c->ghost_idx, gidx.data(), sizeof(int) * gidx.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->ghost_img, gimg.data(), sizeof(double) * gimg.size(), hipMemcpyHostToDevice));
+    }
+    HIPCHK(c, hipMemcpy(c->markers, mk.data(), sizeof(double) * mk.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->stagnant, stag, sizeof(stag), hipMemcpyHostToDevice));
+    c->n_ghost = (int)gidx.size();
+    c->n_mark = (int)(mk.size() / 7);
+    c->ibm_flipped = 0;
+    return EFVM_OK;
+}
+
+efvm_status efvm_marker_force(efvm_ctx* c, double out[6]) {
+    if (!c || !out) return EFVM_ERR_ARG;
+    if (c->n_cyl == 0) { for (int d = 0; d < 6; d++) out[d] = 0.0; return EFVM_OK; }
+    StageArgs a = base_args(c);
+    int set = c->cur;
+    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 1) set = (c->cur + 1) % 3;
+    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 2) set = (c->cur + 2) % 3;
+    set_planes(a, c->P[set], c->P[set], c->P[set]);
+    IbmArgs ib = ibm_args(c, a);
+    double* dout = c->scratch + 4096;
+    double dx = a.dx;
+    hipError_t e = c->esz == 8 ? launch_marker_force<double>(a, ib, c->n_mark, c->markers, dx, a.visc_const, a.mu, dout, c->stream)
+                               : launch_marker_force<float>(a, ib, c->n_mark, c->markers, dx, a.visc_const, a.mu, dout, c->stream);
+    if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "marker force");
+    HIPCHK(c, hipMemcpyAsync(out, dout, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm_conservation_sums(efvm_ctx* c, double out[5]) {
+    if (!c || !out) return EFVM_ERR_ARG;
+    StageArgs a = base_args(c);
+    set_planes(a, c->P[c->cur], c->P[c->cur], c->P[c->cur]);
+    double vol = a.dx * a.dy * a.dz;
+    double* dout = c->scratch + 4096 + 8;
+    int nb = (int)std::min<long>(1024, (c->ncell + 255) / 256);
+    hipError_t e = c->esz == 8 ? launch_sums<double>(a, c->scratch, nb, vol, dout, c->stream)
+                               : launch_sums<float>(a, c->scratch, nb, vol, dout, c->stream);
+    if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "sums");
+    double h[4];
+    HIPCHK(c, hipMemcpyAsync(h, dout, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out[0] = h[0]; out[1] = h[1]; out[2] = h[2]; out[3] = 0.0; out[4] = h[3];
+    return EFVM_OK;
+}
+
+efvm_status efvm_device_state(efvm_ctx* c, void* planes[4]) {
+    if (!c || !planes) return EFVM_ERR_ARG;
+    for (int m = 0; m < 4; m++) planes[m] = c->P[c->cur][m];
+    return EFVM_OK;
+}
+
+}  // extern "C"

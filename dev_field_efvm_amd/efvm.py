@@ -1,0 +1,241 @@
+"""ctypes binding of the C ABI in ``include/efvm.h`` (``libefvm.so``).
+
+The library is loaded from this package directory.  There is no fallback: if
+the HIP library is missing or no GPU is visible, constructing a context raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libefvm.so")
+
+OK, ERR_ARG, ERR_CONFIG, ERR_HIP, ERR_UNSUPPORTED, ERR_NONPHYSICAL, ERR_NO_DEVICE = range(7)
+STATUS_NAMES = {0: "EFVM_OK", 1: "EFVM_ERR_ARG", 2: "EFVM_ERR_CONFIG", 3: "EFVM_ERR_HIP",
+                4: "EFVM_ERR_UNSUPPORTED", 5: "EFVM_ERR_NONPHYSICAL", 6: "EFVM_ERR_NO_DEVICE"}
+
+FLUX = {"rusanov": 0, "hllc": 1, "hllc_textbook": 2}
+LIMITER = {"minmod": 0, "mc": 1}
+BC = {"outflow": 0, "periodic": 1, "slip": 2, "noslip": 3, "symmetry": 4,
+      "supersonic_inflow": 5, "subsonic_inflow": 6, "pressure_outlet": 7}
+
+# Every entry point include/efvm.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "efvm_ctx_create", "efvm_ctx_destroy", "efvm_last_error", "efvm_set_launch", "efvm_set_boundaries",
+    "efvm_set_parabolic_inflow", "efvm_set_sponge", "efvm_set_body_force", "efvm_upload_state",
+    "efvm_download_state", "efvm_get_clock", "efvm_set_clock", "efvm_cfl", "efvm_run", "efvm_step",
+    "efvm_advance_to", "efvm_synchronize", "efvm_graph_capture", "efvm_graph_launch",
+    "efvm_ibm_add_cylinder", "efvm_marker_force", "efvm_conservation_sums", "efvm_device_state",
+    "efvm_last_elapsed_ms",
+]
+
+
+class EfvmError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("lo_x", C.c_double), ("lo_y", C.c_double),
+                ("hi_x", C.c_double), ("hi_y", C.c_double), ("dz", C.c_double)]
+
+
+class Gas(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("r_gas", C.c_double), ("viscosity", C.c_int), ("mu", C.c_double)]
+
+
+class Scheme(C.Structure):
+    _fields_ = [("flux", C.c_int), ("recon", C.c_int), ("limiter", C.c_int), ("integrator", C.c_int),
+                ("precision", C.c_int), ("viscous", C.c_int), ("seam_mode", C.c_int),
+                ("cfl", C.c_double), ("fixed_dt", C.c_double)]
+
+
+class Bc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("par", C.c_double * 5)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libefvm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library {path} is missing: build it with dev_field_efvm_amd/build.py")
+    L = C.CDLL(path)
+    vp, d, pd = C.c_void_p, C.c_double, C.POINTER(C.c_double)
+    sig = {
+        "efvm_ctx_create": [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Scheme), C.POINTER(vp)],
+        "efvm_set_launch": [vp, C.c_int, C.c_int],
+        "efvm_set_boundaries": [vp, C.POINTER(Bc)],
+        "efvm_set_parabolic_inflow": [vp, d, d, d],
+        "efvm_set_sponge": [vp, d, d, d, d, d, d, d],
+        "efvm_set_body_force": [vp, d],
+        "efvm_upload_state": [vp, pd, C.c_size_t],
+        "efvm_download_state": [vp, pd, C.c_size_t],
+        "efvm_get_clock": [vp, pd, pd, C.POINTER(C.c_long), C.POINTER(C.c_int)],
+        "efvm_set_clock": [vp, d, C.c_long],
+        "efvm_cfl": [vp, pd],
+        "efvm_run": [vp],
+        "efvm_step": [vp, C.c_int],
+        "efvm_advance_to": [vp, d, C.c_long, C.POINTER(C.c_long)],
+        "efvm_synchronize": [vp],
+        "efvm_graph_capture": [vp, C.c_int],
+        "efvm_graph_launch": [vp],
+        "efvm_ibm_add_cylinder": [vp, d, d, d, pd],
+        "efvm_marker_force": [vp, pd],
+        "efvm_conservation_sums": [vp, pd],
+        "efvm_device_state": [vp, C.POINTER(vp)],
+        "efvm_last_elapsed_ms": [vp, pd],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.efvm_ctx_destroy.argtypes = [vp]
+    L.efvm_ctx_destroy.restype = None
+    L.efvm_last_error.argtypes = [vp]
+    L.efvm_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _pd(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One solver context on the current HIP device (thin, 1:1 over the C ABI)."""
+
+    def __init__(self, nx, ny, lo=(0.0, 0.0), hi=(1.0, 1.0), dz=None, gamma=1.4, r_gas=287.058, mu=None,
+                 flux="hllc", muscl=False, limiter="minmod", integrator="rk3", precision="fp64",
+                 viscous=False, seam_mode=0, cfl=0.4, fixed_dt=None):
+        L = load_library()
+        self._L = L
+        self.nx, self.ny = int(nx), int(ny)
+        dz = (hi[0] - lo[0]) / nx if dz is None else dz
+        g = Grid(self.nx, self.ny, lo[0], lo[1], hi[0], hi[1], dz)
+        gs = Gas(gamma, r_gas, 0 if mu is None else 1, 0.0 if mu is None else float(mu))
+        sc = Scheme(FLUX[flux], 1 if muscl else 0, LIMITER[limiter], 1 if integrator == "rk3" else 0,
+                    0 if precision == "fp64" else 1, int(viscous), int(seam_mode), cfl,
+                    -1.0 if fixed_dt is None else float(fixed_dt))
+        self.precision = precision
+        self.grid, self.gas, self.scheme = g, gs, sc
+        p = C.c_void_p()
+        st = L.efvm_ctx_create(C.byref(g), C.byref(gs), C.byref(sc), C.byref(p))
+        if st != OK:
+            raise EfvmError(st, "efvm_ctx_create failed")
+        self._p = p
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._L.efvm_ctx_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st: int):
+        if st != OK:
+            raise EfvmError(st, self._L.efvm_last_error(self._p).decode())
+
+    @property
+    def ncell(self) -> int:
+        return self.nx * self.ny
+
+    def set_launch(self, strip_rows=0, waves_per_block=0):
+        self._chk(self._L.efvm_set_launch(self._p, strip_rows, waves_per_block))
+
+    def set_boundaries(self, sides):
+        """sides: 4 x (kind, params) for XLo, XHi, YLo, YHi."""
+        arr = (Bc * 4)()
+        for s, spec in enumerate(sides):
+            kind, par = (spec, ()) if isinstance(spec, str) else spec
+            arr[s].kind = BC[kind]
+            for q, v in enumerate(par):
+                arr[s].par[q] = v
+        self._chk(self._L.efvm_set_boundaries(self._p, arr))
+
+    def set_parabolic_inflow(self, u_max, height, rho):
+        self._chk(self._L.efvm_set_parabolic_inflow(self._p, u_max, height, rho))
+
+    def set_sponge(self, x0, x1, sigma, u_max, height, rho, p):
+        self._chk(self._L.efvm_set_sponge(self._p, x0, x1, sigma, u_max, height, rho, p))
+
+    def set_body_force(self, gx):
+        self._chk(self._L.efvm_set_body_force(self._p, gx))
+
+    def upload(self, cons: np.ndarray):
+        a = np.ascontiguousarray(cons, dtype=np.float64).reshape(-1, 5)
+        if a.shape[0] != self.ncell:
+            raise ValueError("state has the wrong number of cells")
+        self._chk(self._L.efvm_upload_state(self._p, _pd(a), a.shape[0]))
+
+    def download(self) -> np.ndarray:
+        out = np.empty((self.ny, self.nx, 5))
+        self._chk(self._L.efvm_download_state(self._p, _pd(out), self.ncell))
+        return out
+
+    def clock(self):
+        dt, t = C.c_double(), C.c_double()
+        st, sg = C.c_long(), C.c_int()
+        self._chk(self._L.efvm_get_clock(self._p, C.byref(dt), C.byref(t), C.byref(st), C.byref(sg)))
+        return dict(dt=dt.value, time=t.value, step=st.value, stage=sg.value)
+
+    def set_clock(self, time=0.0, step=0):
+        self._chk(self._L.efvm_set_clock(self._p, time, step))
+
+    def cfl_dt(self) -> float:
+        d = C.c_double()
+        self._chk(self._L.efvm_cfl(self._p, C.byref(d)))
+        return d.value
+
+    def run(self):
+        self._chk(self._L.efvm_run(self._p))
+
+    def step(self, n=1):
+        self._chk(self._L.efvm_step(self._p, n))
+
+    def advance_to(self, t_final, max_steps=10 ** 9) -> int:
+        k = C.c_long()
+        self._chk(self._L.efvm_advance_to(self._p, t_final, max_steps, C.byref(k)))
+        return k.value
+
+    def synchronize(self):
+        self._chk(self._L.efvm_synchronize(self._p))
+
+    def graph_capture(self, n_steps):
+        self._chk(self._L.efvm_graph_capture(self._p, n_steps))
+
+    def graph_launch(self):
+        self._chk(self._L.efvm_graph_launch(self._p))
+
+    def last_elapsed_ms(self) -> float:
+        d = C.c_double()
+        self._chk(self._L.efvm_last_elapsed_ms(self._p, C.byref(d)))
+        return d.value
+
+    def add_cylinder(self, cx, cy, r, freestream):
+        fs = np.ascontiguousarray(freestream, dtype=np.float64)
+        self._chk(self._L.efvm_ibm_add_cylinder(self._p, cx, cy, r, _pd(fs)))
+
+    def marker_force(self) -> np.ndarray:
+        out = np.zeros(6)
+        self._chk(self._L.efvm_marker_force(self._p, _pd(out)))
+        return out
+
+    def sums(self) -> np.ndarray:
+        out = np.zeros(5)
+        self._chk(self._L.efvm_conservation_sums(self._p, _pd(out)))
+        return out
+
+    def device_planes(self):
+        arr = (C.c_void_p * 4)()
+        self._chk(self._L.efvm_device_state(self._p, arr))
+        return [arr[m] for m in range(4)]

@@ -1,0 +1,142 @@
+/*
+ * efvm.h -- C ABI of the MI355X explicit compressible finite-volume update.
+ *
+ * This is the drop-in boundary for the reference's FIELD/GRASS plugin path of
+ * SueHeir/dev_field_efvm: a host-side plugin (Rust `GpuSolverPlugin`, or the
+ * C++/Python mirrors shipped here) registers the same schedule phases as
+ * SolverPlugin + FluxPlugin/MusclPlugin + IntegratorPlugin + BoundaryPlugin +
+ * GhostCellIbmPlugin/CylinderIbmPlugin and calls these entry points instead of
+ * running the CPU systems.  Plain C types only: pointers, sizes, doubles, ints.
+ *
+ * State wire order is the reference's `Component` serialisation of `ConsVar`:
+ * 5 doubles per cell (rho, rho_u, rho_v, rho_w, rho_E)
+ * (crates/cfd_state/src/lib.rs:54-76), interior cells only, x fastest.  The GPU
+ * path is 2-D (the reference's nz = 1 UniformMesh with z slip walls): rho_w must
+ * be 0 on upload and is 0 on download.
+ *
+ * Errors: every call returns an efvm_status; no C++ exception or abort crosses
+ * the ABI (the reference panics, e.g. crates/cfd_solver/src/lib.rs:99,
+ * crates/cfd_boundary/src/lib.rs:516-518).  efvm_last_error() gives the text.
+ * One context is not thread-safe; separate contexts may be used concurrently.
+ */
+#ifndef EFVM_H
+#define EFVM_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    EFVM_OK = 0,
+    EFVM_ERR_ARG = 1,          /* bad argument (null pointer, size mismatch)          */
+    EFVM_ERR_CONFIG = 2,       /* invalid configuration (half-periodic axis, ...)     */
+    EFVM_ERR_HIP = 3,          /* HIP runtime error                                   */
+    EFVM_ERR_UNSUPPORTED = 4,  /* feature outside the 2-D GPU path (e.g. rho_w != 0)  */
+    EFVM_ERR_NONPHYSICAL = 5,  /* CFL scan saw non-finite or non-positive density     */
+    EFVM_ERR_NO_DEVICE = 6     /* no HIP device visible                               */
+} efvm_status;
+
+/* Flux schemes: cfd_riemann Rusanov / Hllc (with Thornber) / HllcTextbook
+ * (crates/cfd_riemann/src/lib.rs:213-441). */
+enum { EFVM_FLUX_RUSANOV = 0, EFVM_FLUX_HLLC = 1, EFVM_FLUX_HLLC_TEXTBOOK = 2 };
+/* Reconstruction: FluxPlugin (first order) or MusclPlugin with a SlopeLimiter
+ * (crates/cfd_solver/src/lib.rs:159-163,799-842). */
+enum { EFVM_RECON_FIRST_ORDER = 0, EFVM_RECON_MUSCL = 1 };
+enum { EFVM_LIMITER_MINMOD = 0, EFVM_LIMITER_MC = 1 };
+enum { EFVM_INTEGRATOR_EULER = 0, EFVM_INTEGRATOR_RK3 = 1 };   /* integrate.rs:46-105 */
+enum { EFVM_FP64 = 0, EFVM_FP32 = 1 };
+/* Viscosity model, crates/cfd_eos/src/lib.rs:100-117. */
+enum { EFVM_VISC_SUTHERLAND = 0, EFVM_VISC_CONSTANT = 1 };
+/* Periodic-seam treatment: 0 = the reference's (ghost side of a seam face is not
+ * reconstructed, crates/cfd_solver/src/lib.rs:327-331); 1 = conservative. */
+enum { EFVM_SEAM_REFERENCE = 0, EFVM_SEAM_CONSERVATIVE = 1 };
+/* Boundary conditions, crates/cfd_boundary/src/lib.rs:173-362.  Sides are
+ * XLo=0, XHi=1, YLo=2, YHi=3 (BoundarySide order; z sides are slip walls). */
+enum {
+    EFVM_BC_OUTFLOW = 0,            /* SupersonicOutflow (registry default)     */
+    EFVM_BC_PERIODIC = 1,           /* Periodic (both sides of an axis)         */
+    EFVM_BC_SLIP = 2,               /* SlipWall                                 */
+    EFVM_BC_NOSLIP = 3,             /* NoSlipWall                               */
+    EFVM_BC_SYMMETRY = 4,           /* Symmetry                                 */
+    EFVM_BC_SUPERSONIC_INFLOW = 5,  /* par = rho, u, v, w, p                    */
+    EFVM_BC_SUBSONIC_INFLOW = 6,    /* par = rho, u, v, w, T                    */
+    EFVM_BC_PRESSURE_OUTLET = 7     /* par[0] = p                               */
+};
+
+/* UniformMesh {nx, ny, nz=1, ng=2, bounds_lo, bounds_hi}: x in [lo_x, hi_x],
+ * y in [lo_y, hi_y], z in [0, dz]. */
+typedef struct { int nx, ny; double lo_x, lo_y, hi_x, hi_y, dz; } efvm_grid;
+/* IdealGas {gamma, r_gas} + Viscosity. */
+typedef struct { double gamma, r_gas; int viscosity; double mu; } efvm_gas;
+/* SolverConfig {cfl, fixed_dt, viscous} + flux/reconstruction/integrator choice. */
+typedef struct {
+    int flux, recon, limiter, integrator, precision, viscous, seam_mode;
+    double cfl, fixed_dt;   /* fixed_dt <= 0: CFL-limited */
+} efvm_scheme;
+typedef struct { int kind; double par[5]; } efvm_bc;
+
+typedef struct efvm_ctx efvm_ctx;
+
+/* Context = one App with its mesh, state columns and plugins
+ * (replaces FieldDefaultPlugins + CfdStatePlugin + SolverPlugin + flux plugin). */
+efvm_status efvm_ctx_create(const efvm_grid* grid, const efvm_gas* gas, const efvm_scheme* scheme, efvm_ctx** out);
+void efvm_ctx_destroy(efvm_ctx* ctx);
+const char* efvm_last_error(const efvm_ctx* ctx);
+/* Tile shape of the stage kernel (rows per wave strip, waves per workgroup);
+ * 0 keeps the current value.  Used by the autotuner. */
+efvm_status efvm_set_launch(efvm_ctx* ctx, int strip_rows, int waves_per_block);
+
+/* BoundaryRegistry + StructuredBoundaryPlugin (crates/cfd_boundary/src/lib.rs:369-533):
+ * sides[4] = XLo, XHi, YLo, YHi.  A half-periodic axis is EFVM_ERR_CONFIG. */
+efvm_status efvm_set_boundaries(efvm_ctx* ctx, const efvm_bc sides[4]);
+/* Case systems of examples/cylinder_drag_validation/main.rs:151-229. */
+efvm_status efvm_set_parabolic_inflow(efvm_ctx* ctx, double u_max, double height, double rho);
+efvm_status efvm_set_sponge(efvm_ctx* ctx, double x_start, double x_end, double sigma, double u_max, double height,
+                            double rho, double p);
+/* Uniform x body force (crates/cfd_solver/tests/poiseuille_channel_validation.rs:105-120). */
+efvm_status efvm_set_body_force(efvm_ctx* ctx, double gx);
+
+/* State transfer, CfdState.u in Component wire order (interior cells, x fastest).
+ * Synchronous.  Upload also resets the CFL so the next step recomputes dt. */
+efvm_status efvm_upload_state(efvm_ctx* ctx, const double* cons5, size_t n_cells);
+efvm_status efvm_download_state(efvm_ctx* ctx, double* cons5, size_t n_cells);
+
+/* SolverState {dt, time, step, rk_stage} (crates/cfd_solver/src/lib.rs:61-73). */
+efvm_status efvm_get_clock(efvm_ctx* ctx, double* dt, double* time, long* step, int* rk_stage);
+efvm_status efvm_set_clock(efvm_ctx* ctx, double time, long step);
+/* cfl_system (crates/cfd_solver/src/lib.rs:626-677): dt of the current state, FP64. */
+efvm_status efvm_cfl(efvm_ctx* ctx, double* dt_out);
+
+/* One app.run() = one RK substage through every phase (Setup .. Output). */
+efvm_status efvm_run(efvm_ctx* ctx);
+/* n full time steps, enqueued without host synchronisation (dt stays on device). */
+efvm_status efvm_step(efvm_ctx* ctx, int n_steps);
+/* The harness loop `loop { if time >= t_final {break}; step }`. */
+efvm_status efvm_advance_to(efvm_ctx* ctx, double t_final, long max_steps, long* steps_done);
+efvm_status efvm_synchronize(efvm_ctx* ctx);
+
+/* hipGraph of `n_steps` whole steps (n_steps even: the CFL slots alternate by
+ * step parity); launch replays it. */
+efvm_status efvm_graph_capture(efvm_ctx* ctx, int n_steps);
+efvm_status efvm_graph_launch(efvm_ctx* ctx);
+
+/* GhostCellIbmPlugin + CylinderIbmPlugin (crates/cfd_ibm/src/lib.rs:621-680):
+ * a static z-aligned cylinder; `freestream` seeds cells that flip solid/fluid. */
+efvm_status efvm_ibm_add_cylinder(efvm_ctx* ctx, double cx, double cy, double r, const double freestream[5]);
+/* MarkerForce (pressure xyz, viscous xyz), FP64 (crates/cfd_ibm/src/lib.rs:442-480,601-615). */
+efvm_status efvm_marker_force(efvm_ctx* ctx, double out[6]);
+/* FP64 volume-weighted interior sums of (rho, rho_u, rho_v, rho_w, rho_E). */
+efvm_status efvm_conservation_sums(efvm_ctx* ctx, double out[5]);
+
+/* Raw device planes (rho, rho_u, rho_v, rho_E), nx*ny elements each of the
+ * context precision (double or float), for zero-copy consumers. */
+efvm_status efvm_device_state(efvm_ctx* ctx, void* planes[4]);
+/* Busy time of the last efvm_step/efvm_graph_launch as measured by HIP events
+ * around the stage kernels on the context stream, in milliseconds. */
+efvm_status efvm_last_elapsed_ms(efvm_ctx* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EFVM_H */

@@ -1,0 +1,414 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU FP64 oracle on the
+same inputs.  FP64 mode must match to 1e-12 relative L-infinity (per component,
+normalised by the oracle's max |component|); dt and the clock must agree
+bit for bit.  FP32 mode is compared against FP64 with its tolerance stated.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import dev_field_efvm_amd as efvm
+from dev_field_efvm_amd import cases
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+R = orc.R_AIR
+
+
+def rel_linf(gpu, ref):
+    """max over components of max|gpu - ref| / max|ref| (rho_w skipped: identically 0)."""
+    worst = 0.0
+    for m in (0, 1, 2, 4):
+        scale = max(np.max(np.abs(ref[..., m])), 1e-300)
+        worst = max(worst, float(np.max(np.abs(gpu[..., m] - ref[..., m])) / scale))
+    return worst
+
+
+def make_pair(nx, ny, hi, bcs, flux="hllc", muscl=False, limiter="minmod", cfl=0.4, integrator="rk3",
+              viscous=False, mu=None, seam_mode=0, precision="fp64", gamma=1.4, lo=(0.0, 0.0), body_gx=None,
+              fixed_dt=None):
+    """An oracle Sim and a GPU Context with identical configuration (z = one slip cell)."""
+    dz = hi[2]
+    sim = orc.Sim(nx, ny, 1, lo=(lo[0], lo[1], 0.0), hi=hi, gamma=gamma, flux=flux, muscl=muscl, limiter=limiter,
+                  integrator=integrator, viscous=viscous, mu=mu, cfl=cfl, bcs=list(bcs) + ["slip", "slip"],
+                  seam_mode=seam_mode, body_gx=body_gx, fixed_dt=fixed_dt)
+    ctx = efvm.Context(nx, ny, lo=lo, hi=(hi[0], hi[1]), dz=dz, gamma=gamma, flux=flux, muscl=muscl,
+                       limiter=limiter, integrator=integrator, precision=precision, viscous=viscous, mu=mu,
+                       seam_mode=seam_mode, cfl=cfl, fixed_dt=fixed_dt)
+    ctx.set_boundaries(list(bcs))
+    if body_gx is not None:
+        ctx.set_body_force(body_gx)
+    return sim, ctx
+
+
+def load_both(sim, ctx, state):
+    sim.set_interior(state.reshape(1, ctx.ny, ctx.nx, 5))
+    ctx.upload(state)
+
+
+def gpu_state(ctx):
+    return ctx.download()
+
+
+def ora_state(sim):
+    return sim.interior()[0]
+
+
+# ------------------------------------------------------------------ config 1
+def test_config1_vortex_first_order_periodic_fp64_matches_oracle_to_t2():
+    """64x64 periodic vortex, 1st-order HLLC + SSP-RK3, CFL 0.4, FP64, to t = 2."""
+    n = 64
+    sim, ctx = make_pair(n, n, (10.0, 10.0, 10.0 / n), ["periodic"] * 4)
+    load_both(sim, ctx, cases.vortex_state(n, n, mean_u=1.0, mean_v=1.0, periodic=True))
+    steps = 0
+    while sim.info()["time"] < 2.0:
+        sim.step(1)
+        ctx.step(1)
+        steps += 1
+        if steps % 20 == 0:
+            co, cg = sim.info(), ctx.clock()
+            assert cg["dt"] == co["dt"] and cg["time"] == co["time"], (steps, co, cg)
+    co, cg = sim.info(), ctx.clock()
+    assert cg["step"] == co["step"] and cg["time"] == co["time"] and cg["dt"] == co["dt"]
+    err = rel_linf(gpu_state(ctx), ora_state(sim))
+    print(f"config1 steps={steps} t={co['time']:.6f} rel_linf={err:.3e}")
+    assert err <= TOL64
+
+
+# ------------------------------------------------------------------ MUSCL variants
+@pytest.mark.parametrize("seam_mode", [0, 1])
+@pytest.mark.parametrize("limiter", ["mc", "minmod"])
+def test_muscl_periodic_vortex_modes_fp64(seam_mode, limiter):
+    """The config-2 physics (periodic vortex + 8 seeded modes, MUSCL + HLLC + RK3) at 128^2, 30 steps."""
+    n = 128
+    sim, ctx = make_pair(n, n, (10.0, 10.0, 10.0 / n), ["periodic"] * 4, muscl=True, limiter=limiter, seam_mode=seam_mode)
+    load_both(sim, ctx, cases.vortex_modes_state(n, n, seed=1))
+    sim.step(30)
+    ctx.step(30)
+    co, cg = sim.info(), ctx.clock()
+    assert cg["time"] == co["time"]
+    err = rel_linf(gpu_state(ctx), ora_state(sim))
+    print(f"muscl {limiter} seam={seam_mode} rel_linf={err:.3e}")
+    assert err <= TOL64
+
+
+def test_muscl_outflow_vortex_reference_gate_setup_fp64():
+    """The reference's MUSCL order-gate case (outflow x/y, CFL 0.3, mean (0.5,0.5)) at 40^2 to t = 1."""
+    n = 40
+    sim, ctx = make_pair(n, n, (10.0, 10.0, 10.0 / n), ["outflow"] * 4, muscl=True, limiter="mc", cfl=0.3)
+    st = cases.vortex_state(n, n, mean_u=0.5, mean_v=0.5, periodic=False)
+    load_both(sim, ctx, st)
+    sim.advance_to(1.0)
+    ctx.advance_to(1.0)
+    assert ctx.clock()["time"] == sim.info()["time"]
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
+
+
+def test_muscl_order_gate_on_gpu():
+    """crates/cfd_solver/tests/isentropic_vortex_muscl_convergence.rs:146-192 run on the GPU (FP64)."""
+    ns, errs = [40, 80, 160], []
+    for n in ns:
+        ctx = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc", cfl=0.3)
+        ctx.set_boundaries(["outflow"] * 4)
+        ctx.upload(cases.vortex_state(n, n, mean_u=0.5, mean_v=0.5, periodic=False))
+        ctx.advance_to(1.0)
+        t = ctx.clock()["time"]
+        rho = ctx.download()[..., 0]
+        x, y = cases.cell_centroids(n, n, (0, 0), (10, 10))
+        ex = cases.vortex_prim(x, y, t, mean_u=0.5, mean_v=0.5, periodic=False)[0]
+        errs.append(float(np.mean(np.abs(rho - ex))))
+    xs, ys = np.log(ns), np.log(errs)
+    order = -np.polyfit(xs, ys, 1)[0]
+    print("gpu muscl errs", errs, "order", order)
+    assert order >= 1.8 and errs[-1] < 3e-4
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "hllc_textbook"])
+def test_other_flux_schemes_fp64(flux):
+    n = 48
+    sim, ctx = make_pair(n, n, (10.0, 10.0, 10.0 / n), ["periodic"] * 4, flux=flux, muscl=True, limiter="mc")
+    load_both(sim, ctx, cases.vortex_modes_state(n, n, seed=3))
+    sim.step(20)
+    ctx.step(20)
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
+
+
+def test_euler_integrator_and_ragged_sizes_fp64():
+    """Euler substeps, a grid that is not a multiple of the strip shape, mixed BCs."""
+    nx, ny = 61, 37
+    sim, ctx = make_pair(nx, ny, (6.1, 3.7, 0.1), ["outflow", "pressure_outlet", "slip", "noslip"], muscl=True,
+                         limiter="minmod", integrator="euler", cfl=0.3)
+    ctx.set_boundaries(["outflow", ("pressure_outlet", (0.9,)), "slip", "noslip"])
+    sim = orc.Sim(nx, ny, 1, hi=(6.1, 3.7, 0.1), flux="hllc", muscl=True, limiter="minmod", integrator="euler",
+                  cfl=0.3, bcs=["outflow", "pressure_outlet", "slip", "noslip", "slip", "slip"],
+                  bc_params=[None, (0.9,), None, None, None, None])
+    st = cases.vortex_modes_state(nx, ny, seed=5, L=10.0)
+    load_both(sim, ctx, st)
+    sim.step(25)
+    ctx.step(25)
+    assert ctx.clock()["time"] == sim.info()["time"]
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
+
+
+def test_inflow_boundaries_fp64():
+    nx, ny = 70, 20
+    sim = orc.Sim(nx, ny, 1, hi=(1.0, 0.3, 0.01), flux="hllc", muscl=True, limiter="mc", cfl=0.4,
+                  bcs=["supersonic_inflow", "outflow", "subsonic_inflow", "symmetry", "slip", "slip"],
+                  bc_params=[(1.2, 500.0, 10.0, 0.0, 1.1e5), None, (1.0, 20.0, 0.0, 0.0, 300.0), None, None, None])
+    ctx = efvm.Context(nx, ny, hi=(1.0, 0.3), dz=0.01, muscl=True, limiter="mc", cfl=0.4)
+    ctx.set_boundaries([("supersonic_inflow", (1.2, 500.0, 10.0, 0.0, 1.1e5)), "outflow",
+                        ("subsonic_inflow", (1.0, 20.0, 0.0, 0.0, 300.0)), "symmetry"])
+    x, y = cases.cell_centroids(nx, ny, (0, 0), (1.0, 0.3))
+    rho = 1.0 + 0.2 * np.sin(6 * x) * np.cos(9 * y)
+    st = cases.prim_to_cons(1.4, rho, 100.0 + 0 * x, 5.0 * np.sin(3 * x), 1e5 + 1e3 * np.cos(4 * y))
+    load_both(sim, ctx, st)
+    sim.step(15)
+    ctx.step(15)
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
+
+
+def test_sod_tube_ny1_fp64():
+    """crates/cfd_solver/tests/sod_exact_convergence.rs setup (ny = 1) on the GPU."""
+    n = 200
+    sim, ctx = make_pair(n, 1, (1.0, 0.01, 0.01), ["outflow", "outflow", "slip", "slip"])
+    left = orc.prim_to_cons([1.0, 0, 0, 0, 1e5, 1e5 / R], 1.4)
+    right = orc.prim_to_cons([0.125, 0, 0, 0, 1e4, 1e4 / (0.125 * R)], 1.4)
+    x = (np.arange(n) + 0.5) / n
+    st = np.where((x < 0.5)[:, None], left[None, :], right[None, :]).reshape(1, n, 5)
+    load_both(sim, ctx, st)
+    sim.advance_to(6e-4)
+    ctx.advance_to(6e-4)
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
+
+
+def test_poiseuille_viscous_noslip_body_force_fp64():
+    """crates/cfd_solver/tests/poiseuille_channel_validation.rs setup, 300 RK3 steps."""
+    p0 = R * 300.0
+    sim, ctx = make_pair(3, 32, (1.0, 1.0, 1.0), ["outflow", "outflow", "noslip", "noslip"], viscous=True, mu=10.0,
+                         cfl=0.3, body_gx=160.0)
+    st = np.tile(orc.prim_to_cons([1.0, 0, 0, 0, p0, 300.0], 1.4), (32, 3, 1))
+    load_both(sim, ctx, st)
+    sim.step(300)
+    ctx.step(300)
+    assert ctx.clock()["time"] == sim.info()["time"]
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
+
+
+def test_viscous_sutherland_muscl_fp64():
+    """Sutherland mu(T) uses pow(): device and host pow may differ in the last bit, so this
+    one case is held to 1e-10 rather than 1e-12 (documented tolerance)."""
+    n = 40
+    sim, ctx = make_pair(n, n, (1.0, 1.0, 0.025), ["noslip", "noslip", "noslip", "noslip"], viscous=True,
+                         muscl=True, limiter="mc", cfl=0.3)
+    x, y = cases.cell_centroids(n, n, (0, 0), (1, 1))
+    st = cases.prim_to_cons(1.4, 1.0 + 0 * x, 30.0 * np.sin(np.pi * y), 10.0 * np.sin(np.pi * x), 1e5 + 0 * x)
+    load_both(sim, ctx, st)
+    sim.step(20)
+    ctx.step(20)
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= 1e-10
+
+
+# ------------------------------------------------------------------ cylinder (ghost-cell IBM)
+def cylinder_setup(nx, ny, precision="fp64", muscl=False):
+    """examples/cylinder_drag_validation (config.toml at D/dx = 20: 440 x 82), case systems included."""
+    D, H, L, lz = 0.1, 0.41, 2.2, 0.041
+    rho0, p0, u_mean = 1.0, 1.0e5, 50.0
+    mu = rho0 * u_mean * D / 20.0
+    u_max = 1.5 * u_mean
+    fs = orc.prim_to_cons([rho0, u_mean, 0, 0, p0, p0 / (rho0 * R)], 1.4)
+    common = dict(flux="hllc", muscl=muscl, limiter="mc", cfl=0.3, viscous=True, mu=mu)
+    sim = orc.Sim(nx, ny, 1, hi=(L, H, lz), bcs=["outflow", "pressure_outlet", "noslip", "noslip", "slip", "slip"],
+                  bc_params=[None, (p0,), None, None, None, None], parabolic_inflow=(u_max, H, rho0),
+                  sponge=(1.4, L, 1.0e4, u_max, H, rho0, p0), cylinders=[(0.2, 0.2, 0.5 * D)], freestream=fs, **common)
+    sim.ic_channel(u_max, H, rho0, p0)
+    ctx = efvm.Context(nx, ny, hi=(L, H), dz=lz, precision=precision, **common)
+    ctx.set_boundaries(["outflow", ("pressure_outlet", (p0,)), "noslip", "noslip"])
+    ctx.set_parabolic_inflow(u_max, H, rho0)
+    ctx.set_sponge(1.4, L, 1.0e4, u_max, H, rho0, p0)
+    ctx.add_cylinder(0.2, 0.2, 0.5 * D, fs)
+    ctx.upload(sim.interior()[0])
+    q = 0.5 * rho0 * u_mean * u_mean * D * lz
+    return sim, ctx, q
+
+
+def test_cylinder_ibm_first_order_fp64_matches_oracle():
+    """Reference's own 440 x 82 first-order setup: 20 steps, state and marker forces."""
+    sim, ctx, q = cylinder_setup(440, 82)
+    for _ in range(20):
+        sim.step(1)
+        ctx.step(1)
+    assert ctx.clock()["time"] == sim.info()["time"]
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
+    fo, fg = sim.marker_force(), ctx.marker_force()
+    print("marker force oracle", fo, "gpu", fg, "Cd", (fo[0] + fo[3]) / q)
+    assert np.allclose(fg, fo, rtol=1e-11, atol=1e-12 * np.max(np.abs(fo)))
+
+
+def test_cylinder_ibm_muscl_fp64_matches_oracle():
+    sim, ctx, q = cylinder_setup(220, 41, muscl=True)
+    for _ in range(15):
+        sim.step(1)
+        ctx.step(1)
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
+
+
+# ------------------------------------------------------------------ FP32 production path
+def test_fp32_error_against_fp64_config2_physics():
+    """FP32 (flux/update arithmetic) vs the FP64 path on the config-2 physics, 256^2, 50 steps.
+    Stated tolerance: relative L-inf <= 1e-4 of max|component|; density L1 <= 1e-5."""
+    n = 256
+    st = cases.vortex_modes_state(n, n, seed=1)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        ctx = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc", precision=prec)
+        ctx.set_boundaries(["periodic"] * 4)
+        ctx.upload(st)
+        ctx.step(50)
+        out[prec] = (ctx.download(), ctx.clock())
+    e = rel_linf(out["fp32"][0], out["fp64"][0])
+    l1 = float(np.mean(np.abs(out["fp32"][0][..., 0] - out["fp64"][0][..., 0])))
+    print(f"fp32 vs fp64: rel_linf={e:.3e} rho_L1={l1:.3e} t32={out['fp32'][1]['time']} t64={out['fp64'][1]['time']}")
+    assert e <= 1e-4 and l1 <= 1e-5
+    assert abs(out["fp32"][1]["time"] / out["fp64"][1]["time"] - 1.0) < 1e-5
+
+
+def test_fp32_first_order_and_rusanov_close_to_fp64():
+    n = 96
+    st = cases.vortex_state(n, n, mean_u=1.0, mean_v=1.0)
+    for flux in ("hllc", "rusanov"):
+        res = {}
+        for prec in ("fp64", "fp32"):
+            ctx = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, flux=flux, precision=prec)
+            ctx.set_boundaries(["periodic"] * 4)
+            ctx.upload(st)
+            ctx.step(40)
+            res[prec] = ctx.download()
+        assert rel_linf(res["fp32"], res["fp64"]) <= 1e-4
+
+
+def test_fp32_cylinder_runs_and_tracks_fp64():
+    out = {}
+    for prec in ("fp64", "fp32"):
+        sim, ctx, q = cylinder_setup(220, 41, precision=prec)
+        ctx.step(30)
+        out[prec] = (ctx.download(), ctx.marker_force())
+    assert rel_linf(out["fp32"][0], out["fp64"][0]) <= 1e-4
+    assert abs(out["fp32"][1][0] - out["fp64"][1][0]) <= 1e-4 * abs(out["fp64"][1][0])
+
+
+# ------------------------------------------------------------------ invariants and API behaviour
+def test_periodic_mass_conservation_fp64():
+    n = 128
+    ctx = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc", seam_mode=1)
+    ctx.set_boundaries(["periodic"] * 4)
+    ctx.upload(cases.vortex_modes_state(n, n, seed=2))
+    m0 = ctx.sums()[0]
+    ctx.step(50)
+    assert abs(ctx.sums()[0] - m0) / m0 < 1e-12
+
+
+def test_graph_replay_equals_eager():
+    n = 200
+    st = cases.vortex_modes_state(n, n, seed=4)
+    for prec in ("fp64", "fp32"):
+        eager = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc", precision=prec)
+        eager.set_boundaries(["periodic"] * 4)
+        eager.upload(st)
+        graph = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc", precision=prec)
+        graph.set_boundaries(["periodic"] * 4)
+        graph.upload(st)
+        graph.graph_capture(4)
+        for rep in range(3):
+            eager.step(4)
+            graph.graph_launch()
+            graph.synchronize()
+            a, b = eager.download(), graph.download()
+            assert np.array_equal(a, b), (prec, rep)
+            assert eager.clock()["time"] == graph.clock()["time"]
+
+
+def test_run_substages_equal_step():
+    n = 64
+    st = cases.vortex_modes_state(n, n, seed=6)
+    a = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True)
+    b = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True)
+    for c in (a, b):
+        c.set_boundaries(["periodic"] * 4)
+        c.upload(st)
+    a.step(3)
+    for _ in range(9):
+        b.run()
+    assert np.array_equal(a.download(), b.download())
+    assert b.clock()["step"] == 3
+
+
+def test_launch_shapes_give_identical_results():
+    n = 150
+    st = cases.vortex_modes_state(n, n, seed=7)
+    ref = None
+    for rows, waves in [(32, 4), (8, 1), (64, 2), (200, 8)]:
+        c = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc")
+        c.set_boundaries(["periodic"] * 4)
+        c.set_launch(rows, waves)
+        c.upload(st)
+        c.step(5)
+        s = c.download()
+        if ref is None:
+            ref = s
+        assert np.array_equal(s, ref), (rows, waves)
+
+
+def test_error_paths():
+    c = efvm.Context(16, 16)
+    with pytest.raises(efvm.EfvmError) as e:
+        c.set_boundaries(["periodic", "outflow", "outflow", "outflow"])
+    assert e.value.status == 2
+    st = np.zeros((16, 16, 5))
+    st[..., 0] = 1.0
+    st[..., 4] = 2.5
+    st[3, 3, 3] = 1.0
+    with pytest.raises(efvm.EfvmError) as e:
+        c.upload(st)
+    assert e.value.status == 4
+    with pytest.raises(ValueError):
+        c.upload(np.zeros((10, 5)))
+
+
+def test_nonphysical_state_is_flagged():
+    """Failure detection: a negative density planted in the uploaded state trips the CFL scan."""
+    n = 32
+    c = efvm.Context(n, n, hi=(1.0, 1.0), dz=1.0 / n)
+    st = cases.vortex_state(n, n, L=1.0, beta=0.0, mean_u=0.1, mean_v=0.0)
+    st[5, 7, 0] = -1.0
+    c.upload(st)
+    c.step(1)
+    with pytest.raises(efvm.EfvmError) as e:
+        c.clock()
+    assert e.value.status == 5
+
+
+def test_app_plugin_api_matches_context():
+    """The plugin-style mirror builds the same solver as the raw context."""
+    n = 64
+    st = cases.vortex_state(n, n, mean_u=1.0, mean_v=1.0)
+    app = efvm.App()
+    app.add_plugins(efvm.FieldDefaultPlugins(efvm.UniformMeshConfig(nx=n, ny=n, nz=1, ng=2, bounds_lo=(0, 0, 0),
+                                                                    bounds_hi=(10.0, 10.0, 10.0 / n))))
+    app.add_plugins(efvm.CfdStatePlugin(st))
+    app.add_plugins(efvm.IdealGasPlugin())
+    app.add_plugins(efvm.StructuredBoundaryPlugin(efvm.BoundaryRegistry().with_axis(0, efvm.Periodic())
+                                                  .with_axis(1, efvm.Periodic()).with_axis(2, efvm.SlipWall())))
+    app.add_plugins(efvm.MusclPlugin.hllc().with_limiter(efvm.SlopeLimiter.MonotonizedCentral))
+    app.add_plugins(efvm.IntegratorPlugin.rk3())
+    app.add_plugins(efvm.SolverPlugin(efvm.SolverConfig(cfl=0.4)))
+    app.prepare()
+    for _ in range(6):
+        app.run()
+    c = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc")
+    c.set_boundaries(["periodic"] * 4)
+    c.upload(st)
+    c.step(2)
+    assert np.array_equal(app.state(), c.download())
+    assert app.solver_state().step == 2

@@ -1,0 +1,309 @@
+"""Pins the CPU FP64 oracle (oracle/efvm_oracle.c) before anything is checked
+against it: the reference's own published accuracy figures (7 significant
+digits), its unit known-answer tests, and closed forms / invariants.
+
+Published figures: examples/make_validation_plots.py:86-97,140-142 (see
+tests/golden/published.json); gates: crates/cfd_solver/tests/*.rs.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import orc
+import exact_riemann as er
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PUB = json.load(open(os.path.join(HERE, "golden", "published.json")))
+R = orc.R_AIR
+
+
+def fitted_order(ns, errs):
+    xs = [math.log(n) for n in ns]
+    ys = [math.log(e) for e in errs]
+    m = len(xs)
+    sx, sy = sum(xs), sum(ys)
+    sxx = sum(x * x for x in xs)
+    sxy = sum(x * y for x, y in zip(xs, ys))
+    return -(m * sxy - sx * sy) / (m * sxx - sx * sx)
+
+
+def vortex_sim(n, muscl=False, limiter="minmod", flux="hllc"):
+    s = orc.Sim(n, n, 1, hi=(10.0, 10.0, 10.0 / n), flux=flux, muscl=muscl, limiter=limiter, cfl=0.3,
+                bcs=["outflow"] * 4 + ["slip"] * 2)
+    s.ic_vortex(10.0, 5.0, 1.4, 0.5, 0.5)
+    return s
+
+
+# ---------------------------------------------------------------- published figures
+def test_vortex_first_order_matches_published_l1():
+    """examples/make_validation_plots.py:86-89 + examples/isentropic_vortex/main.rs:128-212."""
+    pub = PUB["vortex_first_order"]
+    errs = []
+    for n, ref in zip(pub["N"], pub["L1"]):
+        s = vortex_sim(n)
+        s.advance_to(1.0)
+        e = s.vortex_l1()
+        errs.append(e)
+        assert abs(e / ref - 1.0) < 1e-6, (n, e, ref)
+    assert round(fitted_order(pub["N"], errs), 3) == pub["fitted_order"]
+
+
+def test_periodic_pulse_matches_published_l1_and_mass_drift():
+    """examples/make_validation_plots.py:94-97; examples/periodic_advected_pulse/main.rs:75-153."""
+    pub = PUB["periodic_pulse"]
+    drifts = []
+    for n, ref in zip(pub["N"], pub["L1"]):
+        s = orc.Sim(n, 1, 1, hi=(1.0, 1.0 / n, 1.0 / n), flux="hllc", cfl=0.45,
+                    bcs=["periodic", "periodic", "slip", "slip", "slip", "slip"])
+        s.ic_pulse()
+        m0 = s.mass()
+        s.advance_to(0.3)
+        e = s.pulse_l1()
+        assert abs(e / ref - 1.0) < 1e-6, (n, e, ref)
+        drifts.append(abs((s.mass() - m0) / m0))
+    assert max(drifts) < pub["mass_tol"]
+    # the published maximum drift is a rounding-level figure: matching it to 4 digits
+    # pins the periodic halo copy and the scatter order, not just the scheme
+    assert abs(max(drifts) / pub["max_mass_drift"] - 1.0) < 1e-3
+
+
+def sod_error(n, gamma):
+    g = gamma
+    left = orc.prim_to_cons([1.0, 0, 0, 0, 1e5, 1e5 / R], g)
+    right = orc.prim_to_cons([0.125, 0, 0, 0, 1e4, 1e4 / (0.125 * R)], g)
+    s = orc.Sim(n, 1, 1, hi=(1.0, 0.01, 0.01), gamma=g, flux="hllc", cfl=0.4,
+                bcs=["outflow", "outflow", "slip", "slip", "slip", "slip"])
+    s.ic_split_x(0.5, left, right)
+    s.advance_to(6e-4)
+    t = s.info()["time"]
+    rho = s.interior()[0, 0, :, 0]
+    ps, us = er.star_state(1.4, 1.0, 0.0, 1e5, 0.125, 0.0, 1e4)
+    x = (np.arange(n) + 0.5) / n
+    ex = np.array([er.sample_density(1.4, 1.0, 0.0, 1e5, 0.125, 0.0, 1e4, ps, us, (xi - 0.5) / t) for xi in x])
+    return float(np.mean(np.abs(rho - ex)))
+
+
+def test_sod_matches_published_l1_and_wrong_gamma_fails():
+    """examples/make_validation_plots.py:140-142; crates/cfd_solver/tests/sod_exact_convergence.rs:292-390."""
+    pub = PUB["sod"]
+    for n, ref in zip(pub["N"], pub["L1"]):
+        e = sod_error(n, 1.4)
+        assert abs(e / ref - 1.0) < 1e-6, (n, e, ref)
+        assert e < 0.25 / math.sqrt(n)
+    # negative control: gamma = 1.67 must blow the same gate at every resolution
+    for n in pub["N"]:
+        assert sod_error(n, 1.67) > 0.25 / math.sqrt(n)
+
+
+def test_muscl_mc_vortex_is_second_order():
+    """crates/cfd_solver/tests/isentropic_vortex_muscl_convergence.rs:146-192."""
+    ns = [40, 80, 160]
+    errs = []
+    for n in ns:
+        s = vortex_sim(n, muscl=True, limiter="mc")
+        s.advance_to(1.0)
+        errs.append(s.vortex_l1())
+    assert all(b < a for a, b in zip(errs, errs[1:]))
+    assert fitted_order(ns, errs) >= 1.8
+    assert math.log2(errs[1] / errs[2]) >= 1.8
+    assert errs[-1] < 3.0e-4
+
+
+def test_muscl_minmod_stalls_below_mc():
+    """The minmod limiter clips the smooth extremum (crates/cfd_solver/src/lib.rs:144-148):
+    it must be less accurate than MC on the same grid."""
+    e = {}
+    for lim in ("minmod", "mc"):
+        s = vortex_sim(80, muscl=True, limiter=lim)
+        s.advance_to(1.0)
+        e[lim] = s.vortex_l1()
+    assert e["mc"] < e["minmod"]
+
+
+def test_poiseuille_profile_matches_parabola():
+    """crates/cfd_solver/tests/poiseuille_channel_validation.rs:122-245 (measured 7.6e-4)."""
+    p0 = 1.0 * R * 300.0
+    s = orc.Sim(3, 32, 1, hi=(1.0, 1.0, 1.0), flux="hllc", cfl=0.3, viscous=True, mu=10.0, body_gx=160.0,
+                bcs=["outflow", "outflow", "noslip", "noslip", "slip", "slip"])
+    s.ic_uniform(orc.prim_to_cons([1.0, 0, 0, 0, p0, 300.0], 1.4))
+    while s.info()["time"] < 0.081:
+        s.run()
+    u = s.interior()[0, :, 1, :]
+    ys = (np.arange(32) + 0.5) / 32
+
+    def rel(mu):
+        pu = 160.0 / (2 * mu) * ys * (1 - ys)
+        return math.sqrt(np.mean((u[:, 1] / u[:, 0] - pu) ** 2)) / (160.0 / (2 * mu) * 0.25)
+
+    assert rel(10.0) < 5e-3
+    assert abs(rel(10.0) / 7.6e-4 - 1.0) < 0.02
+    assert rel(15.0) > 10 * 5e-3
+
+
+# ---------------------------------------------------------------- unit known-answer tests
+AIR = 1.4
+
+
+def test_eos_roundtrip_and_air_sound_speed():
+    """crates/cfd_eos/src/lib.rs:123-150."""
+    t = 101325.0 / (1.225 * R)
+    c = orc.prim_to_cons([1.225, 100.0, 0, 0, 101325.0, t], AIR)
+    p = orc.cons_to_prim(c, AIR)
+    assert abs(p[0] - 1.225) < 1e-10 and abs(p[1] - 100.0) < 1e-10
+    assert abs(p[4] - 101325.0) < 1e-6 and abs(p[5] - t) < 1e-6
+    c2 = orc.prim_to_cons([1.225, 0, 0, 0, 101325.0, 288.15], AIR)
+    assert abs(orc.sound_speed(c2, AIR) - 340.3) < 1.0
+    # closed form: a = sqrt(gamma p / rho)
+    assert orc.sound_speed(c2, AIR) == pytest.approx(math.sqrt(1.4 * 101325.0 / 1.225), rel=1e-14)
+    assert abs(orc.sutherland(288.15) - 1.8e-5) < 1e-6
+
+
+def test_flux_normal_axis_identities():
+    """crates/cfd_riemann/src/lib.rs:685-719: +x face == flux(a,b); -x face == -flux(b,a);
+    two-sided conservation at 1e-12."""
+    a = orc.prim_to_cons([1.0, 30.0, 0, 0, 1e5, 348.4], AIR)
+    b = orc.prim_to_cons([0.5, 10.0, 0, 0, 5e4, 348.4], AIR)
+    for scheme in ("rusanov", "hllc", "hllc_textbook"):
+        assert np.allclose(orc.flux_normal_axis(scheme, a, b, AIR, 0, 1), orc.flux(scheme, a, b, AIR, 0), atol=1e-12, rtol=0)
+        assert np.allclose(orc.flux_normal_axis(scheme, a, b, AIR, 0, -1), -orc.flux(scheme, b, a, AIR, 0), atol=1e-12, rtol=0)
+        c = orc.prim_to_cons([1.0, 50.0, 0, 0, 1e5, 348.4], AIR)
+        d = orc.prim_to_cons([0.4, -20.0, 0, 0, 4e4, 348.4], AIR)
+        f1 = orc.flux_normal_axis(scheme, c, d, AIR, 0, 1)
+        f2 = orc.flux_normal_axis(scheme, d, c, AIR, 0, -1)
+        assert abs(f1[0] + f2[0]) < 1e-12 and abs(f1[4] + f2[4]) < 1e-12
+
+
+def test_uniform_state_flux_is_physical_flux():
+    """crates/cfd_riemann/src/lib.rs:721-764: no jump, no dissipation."""
+    c = orc.prim_to_cons([1.225, 100.0, 0, 0, 101325.0, 288.15], AIR)
+    fp = orc.physical_flux(c, AIR, 0)
+    for scheme in ("rusanov", "hllc", "hllc_textbook"):
+        f = orc.flux(scheme, c, c, AIR, 0)
+        assert abs(f[0] - fp[0]) < 1e-10 and abs(f[1] - fp[1]) < 1e-6 and abs(f[4] - fp[4]) < 1e-4
+    # physical flux closed form: (rho u, rho u^2 + p, 0, 0, (E + p) u)
+    assert fp[0] == pytest.approx(1.225 * 100.0, rel=1e-15)
+    assert fp[1] == pytest.approx(1.225 * 100.0 ** 2 + 101325.0, rel=1e-13)
+
+
+def test_hllc_sod_flux_and_direction_consistency():
+    """crates/cfd_riemann/src/lib.rs:766-803."""
+    l = orc.prim_to_cons([1.0, 0, 0, 0, 1e5, 348.4], AIR)
+    r = orc.prim_to_cons([0.125, 0, 0, 0, 1e4, 278.7], AIR)
+    f = orc.flux("hllc", l, r, AIR, 0)
+    assert np.all(np.isfinite(f)) and f[0] > 0.0
+    px = orc.prim_to_cons([1.0, 100.0, 50.0, 25.0, 101325.0, 288.15], AIR)
+    py = orc.prim_to_cons([1.0, 25.0, 100.0, 50.0, 101325.0, 288.15], AIR)
+    fx, fy = orc.flux("hllc", px, px, AIR, 0), orc.flux("hllc", py, py, AIR, 1)
+    assert abs(fx[0] - fy[0]) < 1e-10 and abs(fx[1] - fy[2]) < 1e-6
+
+
+def test_textbook_hllc_equals_exact_riemann_for_a_contact():
+    """Special case with a textbook answer: a stationary contact (u = 0, equal p) is
+    resolved exactly by HLLC (Toro sec. 10.4): the flux is (0, p, 0, 0, 0)."""
+    l = orc.prim_to_cons([1.0, 0, 0, 0, 1e5, 348.4], AIR)
+    r = orc.prim_to_cons([0.3, 0, 0, 0, 1e5, 348.4], AIR)
+    for scheme in ("hllc", "hllc_textbook"):
+        f = orc.flux(scheme, l, r, AIR, 0)
+        assert abs(f[0]) < 1e-9 and abs(f[4]) < 1e-4 and f[1] == pytest.approx(1e5, rel=1e-12)
+    # Rusanov smears it: nonzero mass flux proportional to the density jump
+    f = orc.flux("rusanov", l, r, AIR, 0)
+    a = max(orc.sound_speed(l, AIR), orc.sound_speed(r, AIR))
+    assert f[0] == pytest.approx(-0.5 * a * (0.3 - 1.0), rel=1e-12)
+
+
+def test_limiters_closed_forms():
+    """crates/cfd_solver/src/lib.rs:170-192."""
+    assert orc.limiter("minmod", 1.0, 2.0) == 1.0
+    assert orc.limiter("minmod", -3.0, -2.0) == -2.0
+    assert orc.limiter("minmod", 1.0, -2.0) == 0.0
+    assert orc.limiter("mc", 1.0, 3.0) == 2.0          # min(2, 6, 2) = 2
+    assert orc.limiter("mc", 1.0, 1.2) == 1.1          # central slope
+    assert orc.limiter("mc", -1.0, -10.0) == -2.0
+    assert orc.limiter("mc", 0.0, 1.0) == 0.0
+
+
+# ---------------------------------------------------------------- invariants (crates/cfd_solver/src/tests.rs)
+MESH3 = dict(nx=16, ny=4, nz=4, hi=(1.0, 0.25, 0.25))
+
+
+def test_uniform_flow_is_preserved():
+    """crates/cfd_solver/src/tests.rs:30-62."""
+    u = orc.prim_to_cons([1.0, 30.0, 0, 0, 1e5, 348.4], AIR)
+    s = orc.Sim(16, 4, 4, hi=(1.0, 0.25, 0.25), flux="rusanov", integrator="euler", cfl=0.4)
+    s.ic_uniform(u)
+    s.step(25)
+    st = s.interior()
+    assert np.all(np.abs(st[..., 0] - u[0]) < 1e-9) and np.all(np.abs(st[..., 1] - u[1]) < 1e-6)
+    assert s.info()["step"] == 25
+
+
+def test_closed_slip_box_conserves_mass():
+    """crates/cfd_solver/src/tests.rs:141-196."""
+    s = orc.Sim(16, 4, 4, hi=(1.0, 0.25, 0.25), flux="rusanov", integrator="euler", cfl=0.3, bcs=["slip"] * 6)
+    raw = np.zeros((8, 8, 20, 5))
+    h = (1.0 / 16, 0.25 / 4, 0.25 / 4)
+    for k in range(8):
+        for j in range(8):
+            for i in range(20):
+                x = ((i - 2 + 0.5) * h[0], (j - 2 + 0.5) * h[1], (k - 2 + 0.5) * h[2])
+                r2 = (x[0] - 0.5) ** 2 + (x[1] - 0.125) ** 2 + (x[2] - 0.125) ** 2
+                p = 3e5 if r2 < 0.15 ** 2 else 1e5
+                raw[k, j, i] = orc.prim_to_cons([1.0, 0, 0, 0, p, p / R], AIR)
+    s.set_raw(raw)
+    m0 = s.mass()
+    s.step(30)
+    assert abs(s.mass() - m0) / m0 < 1e-9
+
+
+def test_viscous_operator_matches_laplacian():
+    """crates/cfd_solver/src/tests.rs:250-322: d(rho u)/dt gains mu * 2a exactly."""
+    a = 10.0
+    t0 = 1e5 / R
+
+    def rate(viscous):
+        s = orc.Sim(6, 8, 4, hi=(1.0, 1.0, 0.5), flux="rusanov", integrator="euler", cfl=0.3, viscous=viscous)
+        raw = np.zeros((8, 12, 10, 5))
+        for j in range(12):
+            y = (j - 2 + 0.5) / 8
+            raw[:, j, :] = orc.prim_to_cons([1.0, a * y * y, 0, 0, 1e5, t0], AIR)
+        s.set_raw(raw)
+        before = s.interior()[2, 4, 3, 1]
+        s.run()
+        return (s.interior()[2, 4, 3, 1] - before) / s.info()["dt"]
+
+    got = rate(True) - rate(False)
+    expected = orc.sutherland(t0) * 2.0 * a
+    assert abs(got - expected) / abs(expected) < 1e-4
+
+
+def test_periodic_halo_fills_from_opposite_slab():
+    """crates/cfd_boundary/src/lib.rs:804-828 (ghost raw x = 0,1,6,7 <- interior 12,13,10,11)."""
+    s = orc.Sim(4, 1, 1, hi=(1.0, 0.25, 0.25), flux="rusanov", integrator="euler", fixed_dt=1e-12,
+                bcs=["periodic", "periodic", "slip", "slip", "slip", "slip"])
+    a = np.zeros((1, 1, 4, 5))
+    for i in range(4):
+        a[0, 0, i] = [10 + i, 0, 0, 0, 1e5]
+    s.set_interior(a)
+    s.run()
+    raw = s.raw()
+    # after one step of 1e-12 the values moved by far less than 1e-3
+    assert np.allclose(raw[2, 2, [0, 1, 6, 7], 0], [12.0, 13.0, 10.0, 11.0], atol=1e-3)
+
+
+def test_half_periodic_axis_is_rejected():
+    """crates/cfd_boundary/src/lib.rs:437-448 (a panic there; an error here)."""
+    with pytest.raises(ValueError):
+        orc.Sim(8, 1, 1, bcs=["periodic", "outflow", "slip", "slip", "slip", "slip"])
+
+
+def test_planted_mistake_is_caught_by_published_figure():
+    """The published 7-digit L1 figure discriminates: dropping the z term of the CFL
+    (a plausible restatement slip, cfd_solver/src/lib.rs:659-661) moves N=40 far
+    outside 1e-6.  Planted in the oracle's input (dz -> huge), not in its code."""
+    s = orc.Sim(40, 40, 1, hi=(10.0, 10.0, 1e9), flux="hllc", cfl=0.3, bcs=["outflow"] * 4 + ["slip"] * 2)
+    s.ic_vortex(10.0, 5.0, 1.4, 0.5, 0.5)
+    s.advance_to(1.0)
+    assert abs(s.vortex_l1() / PUB["vortex_first_order"]["L1"][0] - 1.0) > 1e-3
