@@ -1,0 +1,143 @@
+"""Launch-shape autotuner for the stage kernels.
+
+The stage kernels are wave strips (64 columns x `strip_rows` rows per wave);
+the shape trades halo re-reads (4 extra rows per strip) against parallelism
+(strips per CU) and the tail of the last wave round.  The tuner times real
+steps of the caller's own context or batch with HIP events on its stream
+(library-side `efvm_last_elapsed_ms`), restores the state from a device
+snapshot afterwards, and caches the winner as JSON keyed by
+architecture / precision / workload size.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE_PATH = os.environ.get("EFVM_TUNE_CACHE", os.path.join(HERE, "tuned.json"))
+
+CTX_ROWS = (8, 16, 32, 64, 128, 256)
+CTX_WAVES = (1, 2, 4)
+BATCH_ROWS = (8, 16, 32, 64, 128)
+BATCH_BLOCKS = (0, 512, 1024, 2048, 4096)
+
+
+def arch_name() -> str:
+    try:
+        import torch
+        return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
+    except Exception:
+        return "unknown"
+
+
+def _load():
+    try:
+        return json.load(open(CACHE_PATH))
+    except Exception:
+        return {}
+
+
+def _store(db):
+    tmp = CACHE_PATH + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(db, f, indent=1, sort_keys=True)
+    os.replace(tmp, CACHE_PATH)
+
+
+def size_class(n: int) -> int:
+    """Power-of-two bucket of a grid side."""
+    return 1 << max(0, int(math.floor(math.log2(max(1, n)))))
+
+
+def _time_ctx(ctx, steps):
+    ctx.step(1)                     # compile/warm
+    ctx.step(steps)
+    return ctx.last_elapsed_ms() / steps
+
+
+def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=True, verbose=False):
+    """Pick (strip_rows, waves_per_block) for a context; returns (rows, waves, ms_per_step)."""
+    key = f"{arch_name()}:{ctx.precision}:ctx:{size_class(ctx.nx)}x{size_class(ctx.ny)}"
+    db = _load()
+    if use_cache and key in db:
+        r, w = db[key]["rows"], db[key]["waves"]
+        ctx.set_launch(r, w)
+        return r, w, db[key]["ms"]
+    ctx.snapshot()
+    best = None
+    for r in rows:
+        for w in waves:
+            ctx.set_launch(r, w)
+            ms = _time_ctx(ctx, steps)
+            ctx.snapshot(restore=True)
+            if verbose:
+                print(f"  rows={r:4d} waves={w}  {ms:.4f} ms/step", flush=True)
+            if best is None or ms < best[2]:
+                best = (r, w, ms)
+    ctx.set_launch(best[0], best[1])
+    db[key] = {"rows": best[0], "waves": best[1], "ms": best[2], "when": time.time()}
+    _store(db)
+    return best
+
+
+def _time_batch(b, steps):
+    b.step(1)
+    b.step(steps)
+    return b.last_elapsed_ms() / steps
+
+
+def tune_batch(batch, sides, steps: int = 2, rows=BATCH_ROWS, blocks=BATCH_BLOCKS, use_cache=True, verbose=False,
+               key_extra=""):
+    """Per size class strip rows, then the persistent grid size (coordinate descent).
+
+    sides: per-case grid side (nx) used to form the size classes.
+    Returns ({class: rows}, blocks, ms_per_step)."""
+    classes = sorted(set(size_class(s) for s in sides))
+    key = f"{arch_name()}:fp32:batch:{len(sides)}:{min(sides)}-{max(sides)}{key_extra}"
+    db = _load()
+    if use_cache and key in db:
+        choice = {int(k): v for k, v in db[key]["rows"].items()}
+        _apply_batch(batch, sides, choice)
+        batch.set_grid(db[key]["blocks"], 4)
+        return choice, db[key]["blocks"], db[key]["ms"]
+    batch.snapshot()
+    choice = {c: 32 for c in classes}
+    _apply_batch(batch, sides, choice)
+    nb = 0
+    batch.set_grid(nb, 4)
+    best_ms = _time_batch(batch, steps)
+    batch.snapshot(restore=True)
+    for c in sorted(classes, reverse=True):          # largest classes dominate the time
+        for r in rows:
+            if r == choice[c]:
+                continue
+            trial = dict(choice)
+            trial[c] = r
+            _apply_batch(batch, sides, trial)
+            ms = _time_batch(batch, steps)
+            batch.snapshot(restore=True)
+            if verbose:
+                print(f"  class {c:5d} rows={r:4d}  {ms:.3f} ms/step", flush=True)
+            if ms < best_ms:
+                best_ms, choice = ms, trial
+        _apply_batch(batch, sides, choice)
+    for g in blocks:
+        if g == nb:
+            continue
+        batch.set_grid(g, 4)
+        ms = _time_batch(batch, steps)
+        batch.snapshot(restore=True)
+        if verbose:
+            print(f"  blocks={g:5d}  {ms:.3f} ms/step", flush=True)
+        if ms < best_ms:
+            best_ms, nb = ms, g
+    batch.set_grid(nb, 4)
+    db[key] = {"rows": {str(k): v for k, v in choice.items()}, "blocks": nb, "ms": best_ms, "when": time.time()}
+    _store(db)
+    return choice, nb, best_ms
+
+
+def _apply_batch(batch, sides, choice):
+    batch.set_rows([choice[size_class(s)] for s in sides])

@@ -115,3 +115,39 @@ def vortex_modes_state(nx, ny, seed=1, amplitude=0.02, L=10.0, beta=5.0, gamma=1
 def density_l1(rho_num, rho_exact, cell_volume=1.0):
     """Volume-weighted L1 density error (examples/isentropic_vortex/main.rs:189-211)."""
     return float(np.sum(np.abs(rho_num - rho_exact) * cell_volume) / (rho_num.size * cell_volume))
+
+
+def riemann4_state(nx, ny, quads, gamma, L=1.0):
+    """Four-quadrant Riemann problem on [0,L]^2 split at (L/2, L/2); quads = 4 x (rho, u, v, p)
+    for the quadrants (x>, y>), (x<, y>), (x<, y<), (x>, y<)."""
+    x, y = cell_centroids(nx, ny, (0.0, 0.0), (L, L))
+    q = np.where(x >= 0.5 * L, np.where(y >= 0.5 * L, 0, 3), np.where(y >= 0.5 * L, 1, 2))
+    Q = np.asarray(quads, dtype=np.float64)
+    rho, u, v, p = (Q[q, k] for k in range(4))
+    return prim_to_cons(gamma, rho, u, v, p)
+
+
+def ensemble(n_cases=512, seed=2024, nmin=64, nmax=1024):
+    """The batched-ensemble workload: square grids with N = round(nmin * (nmax/nmin)^U) per side,
+    gamma ~ U[1.3, 1.67]; even cases are periodic isentropic vortices (beta ~ U[2, 6], mean flow
+    (1, 1), [0,10]^2), odd cases are four-quadrant Riemann problems on [0,1]^2 with
+    rho ~ U[0.1, 1.5], p ~ U[0.03, 1.5], |u|, |v| <= 1 and zero-gradient outflow sides.
+    Returns (case specs for Batch, initial-state builders)."""
+    r = Rng(seed)
+    specs, builders = [], []
+    ratio = nmax / nmin
+    for i in range(n_cases):
+        n = int(round(nmin * ratio ** r.uniform()))
+        gamma = r.uniform(1.3, 1.67)
+        if i % 2 == 0:
+            beta = r.uniform(2.0, 6.0)
+            specs.append(dict(nx=n, ny=n, hi=(10.0, 10.0), gamma=gamma, bcs=["periodic"] * 4,
+                              kind="vortex", beta=beta))
+            builders.append(lambda n=n, beta=beta, gamma=gamma: vortex_state(n, n, beta=beta, gamma=gamma))
+        else:
+            quads = [(r.uniform(0.1, 1.5), r.uniform(-1.0, 1.0), r.uniform(-1.0, 1.0), r.uniform(0.03, 1.5))
+                     for _ in range(4)]
+            specs.append(dict(nx=n, ny=n, hi=(1.0, 1.0), gamma=gamma, bcs=["outflow"] * 4, kind="riemann4",
+                              quads=quads))
+            builders.append(lambda n=n, quads=quads, gamma=gamma: riemann4_state(n, n, quads, gamma))
+    return specs, builders

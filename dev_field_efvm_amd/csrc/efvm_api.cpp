@@ -84,6 +84,8 @@ struct efvm_ctx {
     int graph_steps = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int timed = 0;
+    void* snap = nullptr;          // autotuner snapshot (state + clock)
+    int snap_parity = 0, snap_lam = 0;
     std::string err;
 };
 
@@ -173,8 +175,10 @@ efvm_status launch_cfl_into(efvm_ctx* c, void* const* planes, double* target) {
 // One substage (one app.run()).  Enqueue only; no host synchronisation.
 efvm_status run_substage(efvm_ctx* c) {
     const bool rk3 = c->sch.integrator == EFVM_INTEGRATOR_RK3;
+    // RK3: A = U^n (set cur, never rotates), B = U1, C = U2.  Euler ping-pongs sets 0 <-> 1,
+    // so an even number of steps returns the state to its starting set (graph replay).
     void* const* A = c->P[c->cur];
-    void* const* B = c->P[(c->cur + 1) % 3];
+    void* const* B = rk3 ? c->P[(c->cur + 1) % 3] : c->P[c->cur ^ 1];
     void* const* Cc = c->P[(c->cur + 2) % 3];
     efvm_status st;
     // Setup: CFL of U^n (first substage only) ...
@@ -207,7 +211,7 @@ efvm_status run_substage(efvm_ctx* c) {
     }
     if ((st = launch_stage(c, a)) != EFVM_OK) return st;
     if (last) {
-        if (!rk3) c->cur = (c->cur + 1) % 3;   // Euler: U^{n+1} is in B
+        if (!rk3) c->cur ^= 1;   // Euler: U^{n+1} is in B
         StageArgs b = base_args(c);
         set_planes(b, c->P[c->cur], c->P[c->cur], c->P[c->cur]);
         if (c->sponge) {
@@ -304,7 +308,8 @@ void efvm_ctx_destroy(efvm_ctx* c) {
     for (int s = 0; s < 3; s++) for (int m = 0; m < 4; m++) if (c->P[s][m]) hipFree(c->P[s][m]);
     if (c->clock) hipFree(c->clock);
     if (c->yw) hipFree(c->yw);
-    if (c->scratch) hipFree(c->scratch);
+    if (c->scratch) (void)hipFree(c->scratch);
+    if (c->snap) (void)hipFree(c->snap);
     free_ibm(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -316,7 +321,7 @@ const char* efvm_last_error(const efvm_ctx* c) { return c ? c->err.c_str() : "nu
 
 efvm_status efvm_set_launch(efvm_ctx* c, int strip_rows, int waves_per_block) {
     if (!c) return EFVM_ERR_ARG;
-    if (strip_rows < 0 || strip_rows > 4096 || waves_per_block < 0 || waves_per_block > 16)
+    if (strip_rows < 0 || strip_rows > 4096 || waves_per_block < 0 || waves_per_block > 4)
         return fail(c, EFVM_ERR_ARG, "launch shape out of range");
     if (strip_rows) c->strip_h = strip_rows;
     if (waves_per_block) c->waves_per_block = waves_per_block;
@@ -514,9 +519,9 @@ efvm_status efvm_graph_capture(efvm_ctx* c, int n_steps) {
     c->graph = g;
     HIPCHK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
     c->graph_steps = n_steps;
-    // capture only recorded work: restore the host-side step machinery
+    // capture only recorded work: restore the host-side step machinery (an even step count
+    // returns both the CFL slot parity and the Euler ping-pong to their starting values)
     c->cur = cur0; c->parity = par0; c->rk_stage = 0;
-    // Euler with an even step count returns the state to set cur0; RK3 keeps it in place.
     return EFVM_OK;
 }
 
@@ -658,6 +663,365 @@ efvm_status efvm_conservation_sums(efvm_ctx* c, double out[5]) {
 efvm_status efvm_device_state(efvm_ctx* c, void* planes[4]) {
     if (!c || !planes) return EFVM_ERR_ARG;
     for (int m = 0; m < 4; m++) planes[m] = c->P[c->cur][m];
+    return EFVM_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched ensembles: hundreds of independent 2-D cases in one persistent launch
+// per RK stage (FP32, MUSCL/first-order, any flux, RK3), per-case dt.
+// ===========================================================================
+namespace efvm {
+hipError_t launch_stage_f32_batch(const BatchCase* cases, const int2* work, int n_work, int flux, int lim, int stage,
+                                  int parity, int blocks, int waves, hipStream_t st);
+}
+
+struct efvm_batch {
+    int n = 0;
+    efvm_scheme sch{};
+    std::vector<efvm_case> cases;
+    std::vector<BatchCase> hc;
+    std::vector<int> strip_rows;
+    BatchCase* dc = nullptr;
+    int2* dwork = nullptr;
+    int n_work = 0;
+    DevClock* clocks = nullptr;
+    float* pool = nullptr;
+    std::vector<long> offset;      // per-case start (cells) in the pool
+    long total_cells = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int timed = 0;
+    int parity = 0;
+    int lam_valid = 0;
+    int blocks = 0, waves = 4;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    void* snap = nullptr;
+    int snap_parity = 0, snap_lam = 0;
+    std::string err;
+};
+
+namespace {
+efvm_status bfail(efvm_batch* b, efvm_status s, const std::string& m) {
+    if (b) b->err = m;
+    return s;
+}
+#define BHIPCHK(b, expr)                                                                                 \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+void batch_drop_graph(efvm_batch* b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr; b->graph = nullptr;
+}
+
+// Stage arguments of case i (the same builder as a single context, no viscous/IBM).
+StageArgs batch_case_args(const efvm_batch* b, int i) {
+    efvm_ctx tmp;
+    const efvm_case& c = b->cases[i];
+    tmp.grid = {c.nx, c.ny, c.lo_x, c.lo_y, c.hi_x, c.hi_y, c.dz};
+    tmp.gas = {c.gamma, 287.058, EFVM_VISC_SUTHERLAND, 0.0};
+    tmp.sch = b->sch;
+    for (int s = 0; s < 4; s++) tmp.bc[s] = c.sides[s];
+    tmp.clock = b->clocks + i;
+    tmp.strip_h = b->strip_rows[i];
+    tmp.xcd_remap = 0;
+    StageArgs a = base_args(&tmp);
+    a.fuse_cfl = b->sch.fixed_dt <= 0.0 ? 1 : 0;
+    return a;
+}
+
+efvm_status batch_rebuild(efvm_batch* b) {
+    batch_drop_graph(b);
+    std::vector<std::pair<long, int2>> items;
+    for (int i = 0; i < b->n; i++) {
+        BatchCase& bc = b->hc[i];
+        bc.a = batch_case_args(b, i);
+        long nc = (long)b->cases[i].nx * b->cases[i].ny;
+        for (int s = 0; s < 3; s++)
+            for (int m = 0; m < 4; m++) bc.P[s][m] = b->pool + (long)(4 * s + m) * b->total_cells + b->offset[i];
+        int ns = bc.a.n_strips_x * bc.a.n_strips_y;
+        long strip_cells = (long)std::min(b->cases[i].ny, b->strip_rows[i]) * std::min(b->cases[i].nx, (int)STRIP_OUT);
+        for (int s = 0; s < ns; s++) items.push_back({strip_cells * 1000000L + nc, make_int2(i, s)});
+    }
+    // largest strips of the largest cases first (LPT order over the persistent grid)
+    std::stable_sort(items.begin(), items.end(), [](const std::pair<long, int2>& x, const std::pair<long, int2>& y) { return x.first > y.first; });
+    std::vector<int2> work(items.size());
+    for (size_t q = 0; q < items.size(); q++) work[q] = items[q].second;
+    if (b->dwork) (void)hipFree(b->dwork);
+    b->dwork = nullptr;
+    BHIPCHK(b, hipMalloc((void**)&b->dwork, sizeof(int2) * std::max<size_t>(1, work.size())));
+    BHIPCHK(b, hipMemcpyAsync(b->dwork, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, hipMemcpyAsync(b->dc, b->hc.data(), sizeof(BatchCase) * b->n, hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    b->n_work = (int)work.size();
+    return EFVM_OK;
+}
+
+efvm_status batch_cfl_all(efvm_batch* b, int slot) {
+    for (int i = 0; i < b->n; i++) {
+        StageArgs a = b->hc[i].a;
+        hipError_t e = launch_cfl<float>(a, b->hc[i].P[0], &b->clocks[i].lam[slot], b->stream);
+        if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, "batch cfl");
+    }
+    return EFVM_OK;
+}
+
+efvm_status batch_enqueue_steps(efvm_batch* b, int n_steps) {
+    const int lim = b->sch.recon == EFVM_RECON_MUSCL ? b->sch.limiter : LIM_NONE;
+    int blocks = b->blocks > 0 ? b->blocks : std::max(1, std::min((b->n_work + b->waves - 1) / b->waves, 256 * 8));
+    for (int k = 0; k < n_steps; k++) {
+        for (int st = 0; st < 3; st++) {
+            hipError_t e = launch_stage_f32_batch(b->dc, b->dwork, b->n_work, b->sch.flux, lim, st, b->parity, blocks,
+                                                  b->waves, b->stream);
+            if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("batch stage: ") + hipGetErrorString(e));
+        }
+        b->parity ^= 1;
+    }
+    return EFVM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+efvm_status efvm_batch_create(int n_cases, const efvm_case* cases, const efvm_scheme* scheme, efvm_batch** out) {
+    if (!cases || !scheme || !out || n_cases <= 0) return EFVM_ERR_ARG;
+    *out = nullptr;
+    if (scheme->precision != EFVM_FP32 || scheme->integrator != EFVM_INTEGRATOR_RK3 || scheme->viscous)
+        return EFVM_ERR_UNSUPPORTED;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return EFVM_ERR_NO_DEVICE;
+    for (int i = 0; i < n_cases; i++) {
+        const efvm_case& c = cases[i];
+        if (c.nx < 1 || c.ny < 1 || !(c.hi_x > c.lo_x) || !(c.hi_y > c.lo_y) || !(c.dz > 0.0)) return EFVM_ERR_CONFIG;
+        for (int ax = 0; ax < 2; ax++)
+            if ((c.sides[2 * ax].kind == EFVM_BC_PERIODIC) != (c.sides[2 * ax + 1].kind == EFVM_BC_PERIODIC)) return EFVM_ERR_CONFIG;
+    }
+    efvm_batch* b = new (std::nothrow) efvm_batch();
+    if (!b) return EFVM_ERR_ARG;
+    b->n = n_cases;
+    b->sch = *scheme;
+    b->cases.assign(cases, cases + n_cases);
+    b->hc.resize(n_cases);
+    b->strip_rows.assign(n_cases, 32);
+    b->offset.resize(n_cases);
+    long tot = 0;
+    for (int i = 0; i < n_cases; i++) {
+        b->offset[i] = tot;
+        tot += (long)cases[i].nx * cases[i].ny;
+        tot = (tot + 63) & ~63L;   // 256-byte aligned case planes
+    }
+    b->total_cells = tot;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+        hipEventCreate(&b->ev1) != hipSuccess || hipMalloc((void**)&b->pool, sizeof(float) * 12 * tot) != hipSuccess ||
+        hipMalloc((void**)&b->clocks, sizeof(DevClock) * n_cases) != hipSuccess ||
+        hipMalloc((void**)&b->dc, sizeof(BatchCase) * n_cases) != hipSuccess) {
+        efvm_batch_destroy(b);
+        return EFVM_ERR_HIP;
+    }
+    (void)hipMemsetAsync(b->pool, 0, sizeof(float) * 12 * tot, b->stream);
+    for (int i = 0; i < n_cases; i++) (void)launch_clock_set(b->clocks + i, 0.0, 0.0, 0, b->stream);
+    efvm_status st = batch_rebuild(b);
+    if (st != EFVM_OK) { efvm_batch_destroy(b); return st; }
+    *out = b;
+    return EFVM_OK;
+}
+
+void efvm_batch_destroy(efvm_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    if (b->pool) (void)hipFree(b->pool);
+    if (b->clocks) (void)hipFree(b->clocks);
+    if (b->dc) (void)hipFree(b->dc);
+    if (b->dwork) (void)hipFree(b->dwork);
+    if (b->snap) (void)hipFree(b->snap);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* efvm_batch_last_error(const efvm_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+efvm_status efvm_batch_set_launch(efvm_batch* b, int case_id, int strip_rows) {
+    if (!b || case_id >= b->n || strip_rows < 1 || strip_rows > 4096) return bfail(b, EFVM_ERR_ARG, "bad launch shape");
+    if (case_id < 0) std::fill(b->strip_rows.begin(), b->strip_rows.end(), strip_rows);
+    else b->strip_rows[case_id] = strip_rows;
+    return batch_rebuild(b);
+}
+
+efvm_status efvm_batch_set_rows(efvm_batch* b, const int* rows) {
+    if (!b || !rows) return EFVM_ERR_ARG;
+    for (int i = 0; i < b->n; i++)
+        if (rows[i] < 1 || rows[i] > 4096) return bfail(b, EFVM_ERR_ARG, "bad launch shape");
+    b->strip_rows.assign(rows, rows + b->n);
+    return batch_rebuild(b);
+}
+
+efvm_status efvm_batch_set_grid(efvm_batch* b, int blocks, int waves_per_block) {
+    if (!b || blocks < 0 || waves_per_block < 1 || waves_per_block > 4) return bfail(b, EFVM_ERR_ARG, "bad grid");
+    b->blocks = blocks; b->waves = waves_per_block;
+    batch_drop_graph(b);
+    return EFVM_OK;
+}
+
+efvm_status efvm_batch_upload(efvm_batch* b, int i, const double* cons5, size_t n) {
+    if (!b || !cons5 || i < 0 || i >= b->n) return EFVM_ERR_ARG;
+    long nc = (long)b->cases[i].nx * b->cases[i].ny;
+    if ((long)n != nc) return bfail(b, EFVM_ERR_ARG, "batch upload: n_cells mismatch");
+    std::vector<float> h(4 * nc);
+    const int comp[4] = {0, 1, 2, 4};
+    for (long q = 0; q < nc; q++) {
+        if (cons5[5 * q + 3] != 0.0) return bfail(b, EFVM_ERR_UNSUPPORTED, "rho_w must be 0");
+        for (int m = 0; m < 4; m++) h[m * nc + q] = (float)cons5[5 * q + comp[m]];
+    }
+    for (int m = 0; m < 4; m++)
+        BHIPCHK(b, hipMemcpyAsync(b->hc[i].P[0][m], h.data() + m * nc, sizeof(float) * nc, hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    b->lam_valid = 0;
+    return EFVM_OK;
+}
+
+efvm_status efvm_batch_download(efvm_batch* b, int i, double* cons5, size_t n) {
+    if (!b || !cons5 || i < 0 || i >= b->n) return EFVM_ERR_ARG;
+    long nc = (long)b->cases[i].nx * b->cases[i].ny;
+    if ((long)n != nc) return bfail(b, EFVM_ERR_ARG, "batch download: n_cells mismatch");
+    std::vector<float> h(4 * nc);
+    for (int m = 0; m < 4; m++)
+        BHIPCHK(b, hipMemcpyAsync(h.data() + m * nc, b->hc[i].P[0][m], sizeof(float) * nc, hipMemcpyDeviceToHost, b->stream));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    const int comp[4] = {0, 1, 2, 4};
+    for (long q = 0; q < nc; q++) {
+        cons5[5 * q + 3] = 0.0;
+        for (int m = 0; m < 4; m++) cons5[5 * q + comp[m]] = h[m * nc + q];
+    }
+    return EFVM_OK;
+}
+
+efvm_status efvm_batch_step(efvm_batch* b, int n_steps) {
+    if (!b || n_steps < 0) return EFVM_ERR_ARG;
+    if (!b->lam_valid && b->sch.fixed_dt <= 0.0) {
+        efvm_status st = batch_cfl_all(b, b->parity);
+        if (st != EFVM_OK) return st;
+        b->lam_valid = 1;
+    }
+    BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
+    efvm_status st = batch_enqueue_steps(b, n_steps);
+    if (st != EFVM_OK) return st;
+    BHIPCHK(b, hipEventRecord(b->ev1, b->stream));
+    b->timed = 1;
+    return EFVM_OK;
+}
+
+efvm_status efvm_batch_graph_capture(efvm_batch* b, int n_steps) {
+    if (!b || n_steps <= 0 || (n_steps & 1)) return bfail(b, EFVM_ERR_ARG, "graph capture needs an even positive step count");
+    batch_drop_graph(b);
+    if (!b->lam_valid && b->sch.fixed_dt <= 0.0) {
+        efvm_status st = batch_cfl_all(b, b->parity);
+        if (st != EFVM_OK) return st;
+        b->lam_valid = 1;
+    }
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    const int par0 = b->parity;
+    BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    efvm_status st = batch_enqueue_steps(b, n_steps);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    b->parity = par0;
+    if (st != EFVM_OK) { if (g) (void)hipGraphDestroy(g); return st; }
+    if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("end capture: ") + hipGetErrorString(e));
+    b->graph = g;
+    BHIPCHK(b, hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+    return EFVM_OK;
+}
+
+efvm_status efvm_batch_graph_launch(efvm_batch* b) {
+    if (!b || !b->graph_exec) return bfail(b, EFVM_ERR_ARG, "no captured graph");
+    BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
+    BHIPCHK(b, hipGraphLaunch(b->graph_exec, b->stream));
+    BHIPCHK(b, hipEventRecord(b->ev1, b->stream));
+    b->timed = 1;
+    return EFVM_OK;
+}
+
+efvm_status efvm_batch_clock(efvm_batch* b, int i, double* dt, double* time, long* step) {
+    if (!b || i < 0 || i >= b->n) return EFVM_ERR_ARG;
+    DevClock h;
+    BHIPCHK(b, hipMemcpyAsync(&h, b->clocks + i, sizeof(h), hipMemcpyDeviceToHost, b->stream));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    if (dt) *dt = h.dt;
+    if (time) *time = h.time;
+    if (step) *step = (long)h.step;
+    if (h.bad) return bfail(b, EFVM_ERR_NONPHYSICAL, "case " + std::to_string(i) + ": non-physical cells");
+    return EFVM_OK;
+}
+
+efvm_status efvm_batch_synchronize(efvm_batch* b) {
+    if (!b) return EFVM_ERR_ARG;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm_batch_last_elapsed_ms(efvm_batch* b, double* ms) {
+    if (!b || !ms || !b->timed) return EFVM_ERR_ARG;
+    BHIPCHK(b, hipEventSynchronize(b->ev1));
+    float f = 0.f;
+    BHIPCHK(b, hipEventElapsedTime(&f, b->ev0, b->ev1));
+    *ms = f;
+    return EFVM_OK;
+}
+
+// Device-side snapshot of the state and clocks (op 0 = save, 1 = restore), so the
+// autotuner can time real steps without disturbing a running simulation.
+efvm_status efvm_batch_snapshot(efvm_batch* b, int op) {
+    if (!b || op < 0 || op > 1) return EFVM_ERR_ARG;
+    static_assert(sizeof(DevClock) % 8 == 0, "clock layout");
+    const size_t sbytes = sizeof(float) * 4 * b->total_cells, cbytes = sizeof(DevClock) * b->n;
+    if (!b->snap) {
+        if (op == 1) return bfail(b, EFVM_ERR_ARG, "no snapshot saved");
+        BHIPCHK(b, hipMalloc(&b->snap, sbytes + cbytes));
+    }
+    char* s = (char*)b->snap;
+    if (op == 0) {
+        BHIPCHK(b, hipMemcpyAsync(s, b->pool, sbytes, hipMemcpyDeviceToDevice, b->stream));
+        BHIPCHK(b, hipMemcpyAsync(s + sbytes, b->clocks, cbytes, hipMemcpyDeviceToDevice, b->stream));
+        b->snap_parity = b->parity; b->snap_lam = b->lam_valid;
+    } else {
+        BHIPCHK(b, hipMemcpyAsync(b->pool, s, sbytes, hipMemcpyDeviceToDevice, b->stream));
+        BHIPCHK(b, hipMemcpyAsync(b->clocks, s + sbytes, cbytes, hipMemcpyDeviceToDevice, b->stream));
+        b->parity = b->snap_parity; b->lam_valid = b->snap_lam;
+    }
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm_snapshot(efvm_ctx* c, int op) {
+    if (!c || op < 0 || op > 1) return EFVM_ERR_ARG;
+    if (c->rk_stage != 0) return fail(c, EFVM_ERR_ARG, "snapshot at a step boundary only");
+    const size_t pbytes = c->esz * c->ncell;
+    if (!c->snap) {
+        if (op == 1) return fail(c, EFVM_ERR_ARG, "no snapshot saved");
+        HIPCHK(c, hipMalloc(&c->snap, 4 * pbytes + sizeof(DevClock)));
+    }
+    char* s = (char*)c->snap;
+    for (int m = 0; m < 4; m++) {
+        if (op == 0) HIPCHK(c, hipMemcpyAsync(s + m * pbytes, c->P[c->cur][m], pbytes, hipMemcpyDeviceToDevice, c->stream));
+        else HIPCHK(c, hipMemcpyAsync(c->P[c->cur][m], s + m * pbytes, pbytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (op == 0) {
+        HIPCHK(c, hipMemcpyAsync(s + 4 * pbytes, c->clock, sizeof(DevClock), hipMemcpyDeviceToDevice, c->stream));
+        c->snap_parity = c->parity; c->snap_lam = c->lam_valid;
+    } else {
+        HIPCHK(c, hipMemcpyAsync(c->clock, s + 4 * pbytes, sizeof(DevClock), hipMemcpyDeviceToDevice, c->stream));
+        c->parity = c->snap_parity; c->lam_valid = c->snap_lam;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;
 }
 

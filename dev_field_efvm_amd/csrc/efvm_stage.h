@@ -70,6 +70,12 @@ struct StageArgs {
     int xcd_remap;
 };
 
+// One case of a batched ensemble: its stage arguments and plane sets A, B, C.
+struct BatchCase {
+    StageArgs a;
+    void* P[3][4];
+};
+
 // ---------------------------------------------------------------------------
 // wave helpers
 // ---------------------------------------------------------------------------

@@ -33,27 +33,28 @@ __device__ __forceinline__ void recon_pair(const F4& um, const F4& uc, const F4&
     if (rec_hi) { hi.r += 0.5f * s.r; hi.ru += 0.5f * s.ru; hi.rv += 0.5f * s.rv; hi.re += 0.5f * s.re; }
 }
 
+// One wave processes strip `s` of the case described by `a`; the planes and the
+// stage/parity are passed separately so the persistent batched kernel can reuse it.
+// The wave owning strip 0 does the per-case clock bookkeeping.
 template <int FLUX, int LIM>   // LIM = LIM_NONE: first order
-__global__ __launch_bounds__(256) void stage_f32_kernel(StageArgs a) {
+__device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s, const float* const* U,
+                                                const float* const* U0, float* const* O, const int stage,
+                                                const int parity) {
     constexpr bool MUSCL = LIM != LIM_NONE;
     const int lane = lane_id();
-    const int s = strip_index(a, blockDim.x / WAVE);
-    if (s >= a.n_strips_x * a.n_strips_y) return;
     const int nx = a.nx, ny = a.ny;
     const int sxi = s % a.n_strips_x, syi = s / a.n_strips_x;
     const int c = sxi * STRIP_OUT - 2 + lane;
     const int j0 = syi * a.strip_h;
     const int j1 = min(j0 + a.strip_h, ny);
-    const float* U[4] = {(const float*)a.uin[0], (const float*)a.uin[1], (const float*)a.uin[2], (const float*)a.uin[3]};
-    const float* U0[4] = {(const float*)a.u0[0], (const float*)a.u0[1], (const float*)a.u0[2], (const float*)a.u0[3]};
-    float* O[4] = {(float*)a.uout[0], (float*)a.uout[1], (float*)a.uout[2], (float*)a.uout[3]};
     const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0);
-    const double dtd = stage_dt(a);
+    const double dtd = a.fixed_dt > 0.0 ? a.fixed_dt : a.cfl / a.clock->lam[parity];
     const float dt = (float)dtd;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && a.stage == 0) { a.clock->dt = dtd; a.clock->lam[a.parity ^ 1] = 0.0; }
+    const bool fuse = a.fuse_cfl && (a.euler || stage == 2);
+    if (s == 0 && lane == 0 && stage == 0) { a.clock->dt = dtd; a.clock->lam[parity ^ 1] = 0.0; }
     float ca, cb;   // out = ca*u0 + cb*(u + dt*rhs)
-    if (a.euler || a.stage == 0) { ca = 0.f; cb = 1.f; }
-    else if (a.stage == 1) { ca = 0.75f; cb = 0.25f; }
+    if (a.euler || stage == 0) { ca = 0.f; cb = 1.f; }
+    else if (stage == 1) { ca = 0.75f; cb = 0.25f; }
     else { ca = 1.f / 3.f; cb = 2.f / 3.f; }
     const float ix = (float)a.inv_x, iy = (float)a.inv_y;
 
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(256) void stage_f32_kernel(StageArgs a) {
                 o2 = ca * U0[2][q] + cb * o2; o3 = ca * U0[3][q] + cb * o3;
             }
             O[0][q] = o0; O[1][q] = o1; O[2][q] = o2; O[3][q] = o3;
-            if (a.fuse_cfl) {
+            if (fuse) {
                 float ir = fast::rcp(o0);
                 float vu = fabsf(o1 * ir), vv = fabsf(o2 * ir);
                 float p = gm1 * (o3 - 0.5f * (o1 * o1 + o2 * o2) * ir);
@@ -190,18 +191,70 @@ __global__ __launch_bounds__(256) void stage_f32_kernel(StageArgs a) {
             if (ibm) spl = solid_at(c, jc + 2);
         }
     }
-    if (a.fuse_cfl) {
+    if (fuse) {
         double w = wave_max((double)lam_max);
         long long b = wave_sum_ll((long long)bad);
         if (lane == 0) {
-            if (w > 0.0) atomic_max_pos_double(&a.clock->lam[a.parity ^ 1], w);
+            if (w > 0.0) atomic_max_pos_double(&a.clock->lam[parity ^ 1], w);
             if (b) atomicAdd((unsigned long long*)&a.clock->bad, (unsigned long long)b);
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0 && (a.euler || a.stage == 2)) {
+    if (s == 0 && lane == 0 && (a.euler || stage == 2)) {
         a.clock->time += dtd;
         a.clock->step += 1;
     }
+}
+
+template <int FLUX, int LIM>
+__global__ __launch_bounds__(256) void stage_f32_kernel(StageArgs a) {
+    const int s = strip_index(a, blockDim.x / WAVE);
+    if (s >= a.n_strips_x * a.n_strips_y) return;
+    const float* U[4] = {(const float*)a.uin[0], (const float*)a.uin[1], (const float*)a.uin[2], (const float*)a.uin[3]};
+    const float* U0[4] = {(const float*)a.u0[0], (const float*)a.u0[1], (const float*)a.u0[2], (const float*)a.u0[3]};
+    float* O[4] = {(float*)a.uout[0], (float*)a.uout[1], (float*)a.uout[2], (float*)a.uout[3]};
+    stage_f32_strip<FLUX, LIM>(a, s, U, U0, O, a.stage, a.parity);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent batched stage: hundreds of independent cases of uneven size in one
+// launch.  A host-built work table lists (case, strip) pairs, largest strips
+// first; a fixed grid of waves strides through it.  Every case keeps its own
+// DevClock (per-case dt from its own CFL maximum) and its own plane sets
+// A, B, C; all cases advance in lock-step stage by stage.
+// ---------------------------------------------------------------------------
+template <int FLUX, int LIM>
+__global__ __launch_bounds__(256) void stage_f32_batch_kernel(const BatchCase* __restrict__ cases,
+                                                              const int2* __restrict__ work, int n_work,
+                                                              int stage, int parity) {
+    const int wpb = blockDim.x / WAVE;
+    const int nw = gridDim.x * wpb;
+    for (int w = blockIdx.x * wpb + (int)(threadIdx.x / WAVE); w < n_work; w += nw) {
+        const int wu = __builtin_amdgcn_readfirstlane(w);
+        const int2 wk = work[wu];
+        const BatchCase& bc = cases[wk.x];
+        const int in = stage == 0 ? 0 : (stage == 1 ? 1 : 2);
+        const int out = stage == 0 ? 1 : (stage == 1 ? 2 : 0);
+        const float* U[4] = {(const float*)bc.P[in][0], (const float*)bc.P[in][1], (const float*)bc.P[in][2], (const float*)bc.P[in][3]};
+        const float* U0[4] = {(const float*)bc.P[0][0], (const float*)bc.P[0][1], (const float*)bc.P[0][2], (const float*)bc.P[0][3]};
+        float* O[4] = {(float*)bc.P[out][0], (float*)bc.P[out][1], (float*)bc.P[out][2], (float*)bc.P[out][3]};
+        stage_f32_strip<FLUX, LIM>(bc.a, wk.y, U, U0, O, stage, parity);
+    }
+}
+
+hipError_t launch_stage_f32_batch(const BatchCase* cases, const int2* work, int n_work, int flux, int lim, int stage,
+                                  int parity, int blocks, int waves, hipStream_t st) {
+    dim3 grid(blocks), block(WAVE * waves);
+#define EFVM_F32B_CASE(F)                                                                                        \
+    if (lim == LIM_NONE) stage_f32_batch_kernel<F, LIM_NONE><<<grid, block, 0, st>>>(cases, work, n_work, stage, parity);      \
+    else if (lim == LIM_MINMOD) stage_f32_batch_kernel<F, LIM_MINMOD><<<grid, block, 0, st>>>(cases, work, n_work, stage, parity); \
+    else stage_f32_batch_kernel<F, LIM_MC><<<grid, block, 0, st>>>(cases, work, n_work, stage, parity);
+    switch (flux) {
+    case FLUX_RUSANOV: EFVM_F32B_CASE(FLUX_RUSANOV); break;
+    case FLUX_HLLC: EFVM_F32B_CASE(FLUX_HLLC); break;
+    default: EFVM_F32B_CASE(FLUX_HLLC_TEXTBOOK); break;
+    }
+#undef EFVM_F32B_CASE
+    return hipGetLastError();
 }
 
 hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st) {

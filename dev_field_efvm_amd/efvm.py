@@ -29,7 +29,7 @@ EXPORTS = [
     "efvm_download_state", "efvm_get_clock", "efvm_set_clock", "efvm_cfl", "efvm_run", "efvm_step",
     "efvm_advance_to", "efvm_synchronize", "efvm_graph_capture", "efvm_graph_launch",
     "efvm_ibm_add_cylinder", "efvm_marker_force", "efvm_conservation_sums", "efvm_device_state",
-    "efvm_last_elapsed_ms",
+    "efvm_last_elapsed_ms", "efvm_snapshot",
 ]
 
 
@@ -93,6 +93,7 @@ def load_library(path: str = LIB_PATH):
         "efvm_conservation_sums": [vp, pd],
         "efvm_device_state": [vp, C.POINTER(vp)],
         "efvm_last_elapsed_ms": [vp, pd],
+        "efvm_snapshot": [vp, C.c_int],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -239,3 +240,141 @@ class Context:
         arr = (C.c_void_p * 4)()
         self._chk(self._L.efvm_device_state(self._p, arr))
         return [arr[m] for m in range(4)]
+
+    def snapshot(self, restore=False):
+        self._chk(self._L.efvm_snapshot(self._p, 1 if restore else 0))
+
+
+class Case(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("lo_x", C.c_double), ("lo_y", C.c_double),
+                ("hi_x", C.c_double), ("hi_y", C.c_double), ("dz", C.c_double), ("gamma", C.c_double),
+                ("sides", Bc * 4)]
+
+
+def _bind_batch(L):
+    if getattr(L, "_batch_bound", False):
+        return
+    vp, d, pd = C.c_void_p, C.c_double, C.POINTER(C.c_double)
+    sig = {
+        "efvm_batch_create": [C.c_int, C.POINTER(Case), C.POINTER(Scheme), C.POINTER(vp)],
+        "efvm_batch_set_launch": [vp, C.c_int, C.c_int],
+        "efvm_batch_set_grid": [vp, C.c_int, C.c_int],
+        "efvm_batch_upload": [vp, C.c_int, pd, C.c_size_t],
+        "efvm_batch_download": [vp, C.c_int, pd, C.c_size_t],
+        "efvm_batch_step": [vp, C.c_int],
+        "efvm_batch_graph_capture": [vp, C.c_int],
+        "efvm_batch_graph_launch": [vp],
+        "efvm_batch_clock": [vp, C.c_int, pd, pd, C.POINTER(C.c_long)],
+        "efvm_batch_synchronize": [vp],
+        "efvm_batch_last_elapsed_ms": [vp, pd],
+        "efvm_batch_snapshot": [vp, C.c_int],
+        "efvm_batch_set_rows": [vp, C.POINTER(C.c_int)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.efvm_batch_destroy.argtypes = [vp]
+    L.efvm_batch_destroy.restype = None
+    L.efvm_batch_last_error.argtypes = [vp]
+    L.efvm_batch_last_error.restype = C.c_char_p
+    L._batch_bound = True
+
+
+EXPORTS += [
+    "efvm_batch_create", "efvm_batch_destroy", "efvm_batch_last_error", "efvm_batch_set_launch",
+    "efvm_batch_set_grid", "efvm_batch_upload", "efvm_batch_download", "efvm_batch_step",
+    "efvm_batch_graph_capture", "efvm_batch_graph_launch", "efvm_batch_clock", "efvm_batch_synchronize",
+    "efvm_batch_last_elapsed_ms", "efvm_batch_snapshot", "efvm_batch_set_rows",
+]
+
+
+class Batch:
+    """A batched ensemble of independent 2-D cases (FP32, RK3) in one persistent launch per stage.
+
+    cases: list of dicts {nx, ny, lo, hi, dz, gamma, bcs} with bcs as in Context.set_boundaries.
+    """
+
+    def __init__(self, cases, flux="hllc", muscl=True, limiter="mc", cfl=0.4, fixed_dt=None):
+        L = load_library()
+        _bind_batch(L)
+        self._L = L
+        arr = (Case * len(cases))()
+        self.shapes = []
+        for i, c in enumerate(cases):
+            lo, hi = c.get("lo", (0.0, 0.0)), c["hi"]
+            arr[i].nx, arr[i].ny = c["nx"], c["ny"]
+            arr[i].lo_x, arr[i].lo_y, arr[i].hi_x, arr[i].hi_y = lo[0], lo[1], hi[0], hi[1]
+            arr[i].dz = c.get("dz", (hi[0] - lo[0]) / c["nx"])
+            arr[i].gamma = c.get("gamma", 1.4)
+            for s, spec in enumerate(c["bcs"]):
+                kind, par = (spec, ()) if isinstance(spec, str) else spec
+                arr[i].sides[s].kind = BC[kind]
+                for q, v in enumerate(par):
+                    arr[i].sides[s].par[q] = v
+            self.shapes.append((c["ny"], c["nx"]))
+        sc = Scheme(FLUX[flux], 1 if muscl else 0, LIMITER[limiter], 1, 1, 0, 0, cfl,
+                    -1.0 if fixed_dt is None else float(fixed_dt))
+        p = C.c_void_p()
+        st = L.efvm_batch_create(len(cases), arr, C.byref(sc), C.byref(p))
+        if st != OK:
+            raise EfvmError(st, "efvm_batch_create failed")
+        self._p = p
+        self.n = len(cases)
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._L.efvm_batch_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st):
+        if st != OK:
+            raise EfvmError(st, self._L.efvm_batch_last_error(self._p).decode())
+
+    def set_launch(self, strip_rows, case_id=-1):
+        self._chk(self._L.efvm_batch_set_launch(self._p, case_id, strip_rows))
+
+    def set_rows(self, rows):
+        arr = (C.c_int * self.n)(*[int(r) for r in rows])
+        self._chk(self._L.efvm_batch_set_rows(self._p, arr))
+
+    def set_grid(self, blocks, waves_per_block=4):
+        self._chk(self._L.efvm_batch_set_grid(self._p, blocks, waves_per_block))
+
+    def upload(self, i, cons):
+        a = np.ascontiguousarray(cons, dtype=np.float64).reshape(-1, 5)
+        self._chk(self._L.efvm_batch_upload(self._p, i, _pd(a), a.shape[0]))
+
+    def download(self, i):
+        ny, nx = self.shapes[i]
+        out = np.empty((ny, nx, 5))
+        self._chk(self._L.efvm_batch_download(self._p, i, _pd(out), nx * ny))
+        return out
+
+    def step(self, n=1):
+        self._chk(self._L.efvm_batch_step(self._p, n))
+
+    def graph_capture(self, n_steps):
+        self._chk(self._L.efvm_batch_graph_capture(self._p, n_steps))
+
+    def graph_launch(self):
+        self._chk(self._L.efvm_batch_graph_launch(self._p))
+
+    def synchronize(self):
+        self._chk(self._L.efvm_batch_synchronize(self._p))
+
+    def clock(self, i):
+        dt, t, s = C.c_double(), C.c_double(), C.c_long()
+        self._chk(self._L.efvm_batch_clock(self._p, i, C.byref(dt), C.byref(t), C.byref(s)))
+        return dict(dt=dt.value, time=t.value, step=s.value)
+
+    def last_elapsed_ms(self):
+        d = C.c_double()
+        self._chk(self._L.efvm_batch_last_elapsed_ms(self._p, C.byref(d)))
+        return d.value
+
+    def snapshot(self, restore=False):
+        self._chk(self._L.efvm_batch_snapshot(self._p, 1 if restore else 0))

@@ -136,6 +136,43 @@ efvm_status efvm_device_state(efvm_ctx* ctx, void* planes[4]);
  * around the stage kernels on the context stream, in milliseconds. */
 efvm_status efvm_last_elapsed_ms(efvm_ctx* ctx, double* ms);
 
+/* ---------------------------------------------------------------------------
+ * Batched ensembles: many independent 2-D cases of uneven size advanced by one
+ * persistent stage launch per RK stage, each with its own gamma, boundaries and
+ * CFL-limited dt.  The reference runs one App per case; this has no single
+ * counterpart and replaces N copies of the context path above.  FP32, RK3.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int nx, ny;
+    double lo_x, lo_y, hi_x, hi_y, dz;
+    double gamma;
+    efvm_bc sides[4];
+} efvm_case;
+typedef struct efvm_batch efvm_batch;
+
+efvm_status efvm_batch_create(int n_cases, const efvm_case* cases, const efvm_scheme* scheme, efvm_batch** out);
+void efvm_batch_destroy(efvm_batch* b);
+const char* efvm_batch_last_error(const efvm_batch* b);
+/* rows per wave strip for one case (case_id < 0: all cases); used by the autotuner */
+efvm_status efvm_batch_set_launch(efvm_batch* b, int case_id, int strip_rows);
+/* rows per wave strip for every case at once (n_cases ints) */
+efvm_status efvm_batch_set_rows(efvm_batch* b, const int* rows);
+/* persistent grid: workgroups (0 = one per 4 strips, capped) and waves per workgroup */
+efvm_status efvm_batch_set_grid(efvm_batch* b, int blocks, int waves_per_block);
+efvm_status efvm_batch_upload(efvm_batch* b, int case_id, const double* cons5, size_t n_cells);
+efvm_status efvm_batch_download(efvm_batch* b, int case_id, double* cons5, size_t n_cells);
+efvm_status efvm_batch_step(efvm_batch* b, int n_steps);
+efvm_status efvm_batch_graph_capture(efvm_batch* b, int n_steps);
+efvm_status efvm_batch_graph_launch(efvm_batch* b);
+efvm_status efvm_batch_clock(efvm_batch* b, int case_id, double* dt, double* time, long* step);
+efvm_status efvm_batch_synchronize(efvm_batch* b);
+efvm_status efvm_batch_last_elapsed_ms(efvm_batch* b, double* ms);
+
+/* Device-side snapshot of the state and clock(s): op 0 saves, op 1 restores.
+ * Lets the autotuner time real steps and then put the simulation back. */
+efvm_status efvm_snapshot(efvm_ctx* ctx, int op);
+efvm_status efvm_batch_snapshot(efvm_batch* b, int op);
+
 #ifdef __cplusplus
 }
 #endif
