@@ -22,7 +22,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-x", "hip", f"--offload-arch={ARCH}",
           "-Wno-unused-result", "-Wno-unused-command-line-argument"]
 UNITS = [
     ("efvm_stage_f64.hip", ["-ffp-contract=off"]),
-    ("efvm_stage_f32.hip", ["-ffp-contract=fast"]),
+    ("efvm_stage_f32.hip", ["-ffp-contract=fast", "-fgpu-flush-denormals-to-zero"]),
     ("efvm_mlp.hip", ["-ffp-contract=fast"]),
     ("efvm_batch.hip", ["-ffp-contract=fast"]),
     ("efvm_api.cpp", ["-ffp-contract=off"]),

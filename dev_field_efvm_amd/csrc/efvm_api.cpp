@@ -673,7 +673,7 @@ efvm_status efvm_device_state(efvm_ctx* c, void* planes[4]) {
 // per RK stage (FP32, MUSCL/first-order, any flux, RK3), per-case dt.
 // ===========================================================================
 namespace efvm {
-hipError_t launch_stage_f32_batch(const BatchCase* cases, const int2* work, int n_work, int flux, int lim, int stage,
+hipError_t launch_stage_f32_batch(const BatchCase* cases, const StageArgs& rep, const int2* work, int n_work, int stage,
                                   int parity, int blocks, int waves, hipStream_t st);
 }
 
@@ -773,11 +773,17 @@ efvm_status batch_cfl_all(efvm_batch* b, int slot) {
 }
 
 efvm_status batch_enqueue_steps(efvm_batch* b, int n_steps) {
-    const int lim = b->sch.recon == EFVM_RECON_MUSCL ? b->sch.limiter : LIM_NONE;
+    // kernel features = union over the cases (an EOS-based boundary anywhere selects the general kernel)
+    StageArgs rep = b->hc[0].a;
+    for (int i = 1; i < b->n; i++)
+        for (int s = 0; s < 4; s++) {
+            int k = b->hc[i].a.bc_kind[s];
+            if (k == BC_SUP_INFLOW || k == BC_SUB_INFLOW || k == BC_PRESSURE_OUTLET) rep.bc_kind[s] = k;
+        }
     int blocks = b->blocks > 0 ? b->blocks : std::max(1, std::min((b->n_work + b->waves - 1) / b->waves, 256 * 8));
     for (int k = 0; k < n_steps; k++) {
         for (int st = 0; st < 3; st++) {
-            hipError_t e = launch_stage_f32_batch(b->dc, b->dwork, b->n_work, b->sch.flux, lim, st, b->parity, blocks,
+            hipError_t e = launch_stage_f32_batch(b->dc, rep, b->dwork, b->n_work, st, b->parity, blocks,
                                                   b->waves, b->stream);
             if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("batch stage: ") + hipGetErrorString(e));
         }

@@ -1,0 +1,141 @@
+// efvm_fast.h -- FP32 production numerics for the 2-D path (rho_w == 0 dropped).
+//
+// Same schemes as the reference (HLLC with Thornber's low-Mach correction,
+// crates/cfd_riemann/src/lib.rs:395-506; Rusanov :222-240; minmod / MC limiters,
+// crates/cfd_solver/src/lib.rs:170-192), rewritten for the VALU:
+//  * per face state one reciprocal (1/rho) and one reciprocal square root
+//    (1/a, with a = (gamma p/rho) * rsq), shared by the flux that uses it;
+//  * Thornber's z = min(|v|_max / a_max, 1) uses the stored 1/a;
+//  * HLLC star state without divisions except 1/(S_K - S*):
+//      rho* = rho_K (S_K - u_K) D,  (rho E)* = D [(S_K - u_K) E_K + (S* - u_K)((S_K - u_K) rho_K S* + p_K)],
+//    D = 1/(S_K - S*)  (Toro sec. 10.4, multiplied through by rho*);
+//  * upwind side chosen by selects; the star correction added by select so an
+//    out-of-fan inf/NaN never leaks into the flux.
+// Mathematically identical to the reference; not bitwise.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+namespace efvm {
+namespace fast {
+
+template <class T> struct C4 { T r, ru, rv, re; };
+
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+#else
+inline float rcp(float x) { return 1.0f / x; }
+inline float rsq(float x) { return 1.0f / std::sqrt(x); }
+inline float fsqrt(float x) { return std::sqrt(x); }
+#endif
+
+__host__ __device__ __forceinline__ float minmod(float a, float b) {
+    float m = fabsf(a) < fabsf(b) ? a : b;
+    return a * b > 0.f ? m : 0.f;
+}
+// MC: sign(a) min(2|a|, 2|b|, |a+b|/2) when a b > 0, else 0
+__host__ __device__ __forceinline__ float mc(float a, float b) {
+    float s = a + b;
+    float m = fminf(2.f * fminf(fabsf(a), fabsf(b)), 0.5f * fabsf(s));
+    return a * b > 0.f ? copysignf(m, s) : 0.f;
+}
+template <int LIM> __host__ __device__ __forceinline__ float limit(float a, float b) {
+    if (LIM == 1) return mc(a, b);
+    return minmod(a, b);
+}
+
+// Face state with the quantities every flux needs.
+struct Prep {
+    float r, ru, rv, re;   // conserved
+    float u, v, p;         // velocity, pressure
+    float a, ia, ir;       // sound speed, 1/a, 1/rho
+};
+__host__ __device__ __forceinline__ Prep prep(const C4<float>& q, float g, float gm1) {
+    Prep o;
+    o.r = q.r; o.ru = q.ru; o.rv = q.rv; o.re = q.re;
+    o.ir = rcp(q.r);
+    o.u = q.ru * o.ir;
+    o.v = q.rv * o.ir;
+    o.p = gm1 * (q.re - 0.5f * (q.ru * o.u + q.rv * o.v));
+    float x = g * o.p * o.ir;          // a^2
+    o.ia = rsq(x);
+    o.a = x * o.ia;
+    return o;
+}
+
+template <int DIR, bool THORNBER>
+__host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R) {
+    float ul = L.u, vl = L.v, ur = R.u, vr = R.v;
+    float el = L.re, er = R.re;
+    if (THORNBER) {
+        float ql = ul * ul + vl * vl, qr = ur * ur + vr * vr;
+        float z = fminf(fsqrt(fmaxf(ql, qr)) * fminf(L.ia, R.ia), 1.f);
+        float hz = 0.5f * z;
+        float au = 0.5f * (ul + ur), av = 0.5f * (vl + vr);
+        float du = hz * (ul - ur), dv = hz * (vl - vr);
+        ul = au + du; ur = au - du; vl = av + dv; vr = av - dv;
+        // rebuilt energies keep the internal energy (and p, a) unchanged
+        el = L.re + 0.5f * L.r * ((ul * ul + vl * vl) - ql);
+        er = R.re + 0.5f * R.r * ((ur * ur + vr * vr) - qr);
+    }
+    const float unl = DIR == 0 ? ul : vl, unr = DIR == 0 ? ur : vr;
+    const float sl = fminf(unl - L.a, unr - R.a);
+    const float sr = fmaxf(unl + L.a, unr + R.a);
+    const float fl = L.r * (sl - unl), fr = R.r * (sr - unr);
+    // S* = N / Dn with Dn = fl - fr < 0 (fl < 0 < fr for positive rho, a), so the
+    // contact side is known from the sign of N alone: S* >= 0  <=>  N <= 0.
+    // One reciprocal then gives both S* and D = 1/(S_K - S*):
+    //   X = S_K Dn - N = Dn (S_K - S*),  r = 1/(Dn X),  S* = N X r,  D = Dn^2 r.
+    const float N = (R.p - L.p) + fl * unl - fr * unr;
+    const float Dn = fl - fr;
+    const bool left = N <= 0.f;
+    const float rK = left ? L.r : R.r, uK = left ? ul : ur, vK = left ? vl : vr;
+    const float pK = left ? L.p : R.p, eK = left ? el : er, unK = left ? unl : unr, sK = left ? sl : sr;
+    const float X = sK * Dn - N;
+    const float rr = rcp(Dn * X);
+    const float ss = N * X * rr;
+    const float D = Dn * Dn * rr;
+    const float ruK = rK * uK, rvK = rK * vK;
+    C4<float> f;
+    f.r = rK * unK;
+    f.ru = ruK * unK + (DIR == 0 ? pK : 0.f);
+    f.rv = rvK * unK + (DIR == 1 ? pK : 0.f);
+    f.re = (eK + pK) * unK;
+    const bool fan = left ? (sl < 0.f) : (sr >= 0.f);
+    const float sm = sK - unK;
+    const float rs = rK * sm * D;
+    const float es = D * (sm * eK + (ss - unK) * (sm * rK * ss + pK));
+    const float mun = rs * ss;                 // normal momentum of the star state
+    const float dr = sK * (rs - rK);
+    const float dru = sK * ((DIR == 0 ? mun : rs * uK) - ruK);
+    const float drv = sK * ((DIR == 1 ? mun : rs * vK) - rvK);
+    const float dre = sK * (es - eK);
+    f.r += fan ? dr : 0.f;
+    f.ru += fan ? dru : 0.f;
+    f.rv += fan ? drv : 0.f;
+    f.re += fan ? dre : 0.f;
+    return f;
+}
+
+template <int DIR>
+__host__ __device__ __forceinline__ C4<float> rusanov(const Prep& L, const Prep& R) {
+    const float unl = DIR == 0 ? L.u : L.v, unr = DIR == 0 ? R.u : R.v;
+    const float lam = fmaxf(fabsf(unl) + L.a, fabsf(unr) + R.a);
+    C4<float> f;
+    f.r = 0.5f * (L.r * unl + R.r * unr) - 0.5f * lam * (R.r - L.r);
+    f.ru = 0.5f * (L.ru * unl + R.ru * unr + (DIR == 0 ? L.p + R.p : 0.f)) - 0.5f * lam * (R.ru - L.ru);
+    f.rv = 0.5f * (L.rv * unl + R.rv * unr + (DIR == 1 ? L.p + R.p : 0.f)) - 0.5f * lam * (R.rv - L.rv);
+    f.re = 0.5f * ((L.re + L.p) * unl + (R.re + R.p) * unr) - 0.5f * lam * (R.re - L.re);
+    return f;
+}
+
+template <int FLUX, int DIR>
+__host__ __device__ __forceinline__ C4<float> flux(const Prep& L, const Prep& R) {
+    if (FLUX == 0) return rusanov<DIR>(L, R);
+    return hllc<DIR, FLUX == 1>(L, R);
+}
+
+}  // namespace fast
+}  // namespace efvm

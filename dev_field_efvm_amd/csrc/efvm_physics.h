@@ -1,16 +1,11 @@
 // efvm_physics.h -- equation of state, numerical fluxes and limiters as inline
 // host/device functions, templated on the arithmetic type.
 //
-// Two families live here:
-//  * ref::*   -- the reference's exact operation order, five conserved
-//               components (rho, rho_u, rho_v, rho_w, rho_E), for the FP64
-//               parity path.  Every function names the reference lines it
-//               restates; compiled with -ffp-contract=off it reproduces the
-//               reference's rounding op for op.
-//  * fast::*  -- the same schemes specialised to 2-D (rho_w == 0 dropped),
-//               with shared reciprocals and branch-free region selection,
-//               for the FP32 production path.  Mathematically identical,
-//               not bitwise.
+// ref::* is the reference's exact operation order over five conserved
+// components (rho, rho_u, rho_v, rho_w, rho_E), for the FP64 parity path.
+// Every function names the reference lines it restates; compiled with
+// -ffp-contract=off it reproduces the reference's rounding op for op.  The
+// FP32 production schemes live in efvm_fast.h.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cmath>
@@ -148,121 +143,4 @@ template <class T> EFVM_HD T sutherland(T t) {
 }
 template <class T> EFVM_HD T conductivity(T mu, T g, T R) { T cp = g * R / (g - T(1)); return mu * cp / T(0.72); }
 }  // namespace ref
-
-// ---------------------------------------------------------------------------
-// fast:: 2-D production numerics (rho_w == 0), FP32.  Same schemes, fewer ops.
-// ---------------------------------------------------------------------------
-namespace fast {
-
-template <class T> struct C4 { T r, ru, rv, re; };
-
-#if defined(__HIP_DEVICE_COMPILE__)
-__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-#else
-inline float rcp(float x) { return 1.0f / x; }
-inline float fsqrt(float x) { return std::sqrt(x); }
-#endif
-__host__ __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-__host__ __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
-
-template <class T> EFVM_HD T minmod(T a, T b) {
-    T m = fabs(a) < fabs(b) ? a : b;
-    return a * b > T(0) ? m : T(0);
-}
-template <class T> EFVM_HD T mc(T a, T b) {
-    T m = fmin(fmin(T(2) * fabs(a), T(2) * fabs(b)), T(0.5) * fabs(a + b));
-    return a * b > T(0) ? copysign(m, a) : T(0);
-}
-template <int LIM, class T> EFVM_HD T limit(T a, T b) {
-    if (LIM == LIM_MC) return mc(a, b);
-    return minmod(a, b);
-}
-
-// Per-state quantities shared by every use of one face state.
-template <class T> struct Prep {
-    T r, ru, rv, re;   // conserved
-    T u, v, p, a;      // velocity, pressure, sound speed
-};
-template <class T> EFVM_HD Prep<T> prep(const C4<T>& q, T g, T gm1) {
-    Prep<T> o;
-    o.r = q.r; o.ru = q.ru; o.rv = q.rv; o.re = q.re;
-    T ir = rcp(q.r);
-    o.u = q.ru * ir; o.v = q.rv * ir;
-    T ke = T(0.5) * (q.ru * o.u + q.rv * o.v);
-    o.p = gm1 * (q.re - ke);
-    o.a = fsqrt(g * o.p * ir);
-    return o;
-}
-
-// HLLC with Thornber's low-Mach reconstruction (crates/cfd_riemann/src/lib.rs:395-506),
-// 2-D, DIR = 0 (x) or 1 (y).  Thornber keeps rho and p, so a_L, a_R and p_L, p_R are
-// reused from the unmodified states (crates/cfd_riemann/src/lib.rs:586-588).
-template <int DIR, bool THORNBER, class T>
-EFVM_HD C4<T> hllc(const Prep<T>& L, const Prep<T>& R, T g, T gm1) {
-    T ul = L.u, vl = L.v, ur = R.u, vr = R.v;
-    if (THORNBER) {
-        T ql = ul * ul + vl * vl, qr = ur * ur + vr * vr;
-        T vmax = fmax(ql, qr), amax = fmax(L.a, R.a);
-        T z = fmin(fsqrt(vmax) * rcp(amax), T(1));
-        T hz = T(0.5) * z;
-        T au = T(0.5) * (ul + ur), av = T(0.5) * (vl + vr);
-        T du = hz * (ul - ur), dv = hz * (vl - vr);
-        ul = au + du; ur = au - du; vl = av + dv; vr = av - dv;
-    }
-    // rebuilt conserved momenta / energies (p preserved)
-    T rul = L.r * ul, rvl = L.r * vl, rur = R.r * ur, rvr = R.r * vr;
-    T el = L.p * (T(1) / gm1) + T(0.5) * (rul * ul + rvl * vl);
-    T er = R.p * (T(1) / gm1) + T(0.5) * (rur * ur + rvr * vr);
-    if (!THORNBER) { el = L.re; er = R.re; }
-    T unl = DIR == 0 ? ul : vl, unr = DIR == 0 ? ur : vr;
-    T sl = fmin(unl - L.a, unr - R.a);
-    T sr = fmax(unl + L.a, unr + R.a);
-    T fl = L.r * (sl - unl), fr = R.r * (sr - unr);
-    T ss = ((R.p - L.p) + fl * unl - fr * unr) * rcp(fl - fr);
-    // upwind side K of the contact, then F = F_K + S_K (U*_K - U_K) inside the fan
-    bool left = ss >= T(0);
-    T rK = left ? L.r : R.r, uK = left ? ul : ur, vK = left ? vl : vr, pK = left ? L.p : R.p;
-    T eK = left ? el : er, unK = left ? unl : unr, sK = left ? sl : sr;
-    T ruK = rK * uK, rvK = rK * vK;
-    C4<T> f;
-    f.r = rK * unK;
-    f.ru = ruK * unK + (DIR == 0 ? pK : T(0));
-    f.rv = rvK * unK + (DIR == 1 ? pK : T(0));
-    f.re = (eK + pK) * unK;
-    bool fan = left ? (sl < T(0)) : (sr >= T(0));
-    T den = rcp(sK - ss);
-    T rs = rK * (sK - unK) * den;
-    T es = eK * rcp(rK) + (ss - unK) * (ss + pK * rcp(rK * (sK - unK)));
-    // Outside the fan the star terms may be inf/NaN (S_K == S*), so select, not scale.
-    T dr = sK * (rs - rK);
-    T dru = sK * ((DIR == 0 ? rs * ss : rs * uK) - ruK);
-    T drv = sK * ((DIR == 1 ? rs * ss : rs * vK) - rvK);
-    T dre = sK * (rs * es - eK);
-    f.r += fan ? dr : T(0);
-    f.ru += fan ? dru : T(0);
-    f.rv += fan ? drv : T(0);
-    f.re += fan ? dre : T(0);
-    return f;
-}
-
-// Rusanov (crates/cfd_riemann/src/lib.rs:222-240), 2-D.
-template <int DIR, class T>
-EFVM_HD C4<T> rusanov(const Prep<T>& L, const Prep<T>& R) {
-    T unl = DIR == 0 ? L.u : L.v, unr = DIR == 0 ? R.u : R.v;
-    T lam = fmax(fabs(unl) + L.a, fabs(unr) + R.a);
-    C4<T> f;
-    f.r = T(0.5) * (L.r * unl + R.r * unr) - T(0.5) * lam * (R.r - L.r);
-    f.ru = T(0.5) * (L.ru * unl + R.ru * unr + (DIR == 0 ? L.p + R.p : T(0))) - T(0.5) * lam * (R.ru - L.ru);
-    f.rv = T(0.5) * (L.rv * unl + R.rv * unr + (DIR == 1 ? L.p + R.p : T(0))) - T(0.5) * lam * (R.rv - L.rv);
-    f.re = T(0.5) * ((L.re + L.p) * unl + (R.re + R.p) * unr) - T(0.5) * lam * (R.re - L.re);
-    return f;
-}
-
-template <int FLUX, int DIR, class T>
-EFVM_HD C4<T> flux(const Prep<T>& L, const Prep<T>& R, T g, T gm1) {
-    if (FLUX == FLUX_RUSANOV) return rusanov<DIR>(L, R);
-    return hllc<DIR, FLUX == FLUX_HLLC>(L, R, g, gm1);
-}
-}  // namespace fast
 }  // namespace efvm

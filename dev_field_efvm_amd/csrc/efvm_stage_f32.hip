@@ -1,46 +1,117 @@
 // efvm_stage_f32.hip -- FP32 production stage kernel (contraction allowed).
 //
 // Same wave-strip decomposition and boundary semantics as the FP64 parity
-// kernel (efvm_stage_f64.hip); the numerics are the 2-D specialised schemes of
-// efvm_physics.h (fast::).  The z faces of the one-cell-thick mesh are dropped:
-// their two contributions cancel exactly in exact arithmetic.  The divergence
-// is (F_w - F_e)*A_x/V + (G_s - G_n)*A_y/V, then viscous, body force, solid
-// zero and the RK combine as FMAs.  Stage 3 fuses the CFL scan: lambda of the
-// new state per cell (FP32), max-reduced in FP64 across the wave and the grid.
+// kernel (efvm_stage_f64.hip); the numerics are the 2-D schemes of efvm_fast.h.
+// The z faces of the one-cell-thick mesh are dropped: their two contributions
+// cancel exactly in exact arithmetic.  Per cell: rhs = (F_w - F_e) A_x/V +
+// (G_s - G_n) A_y/V (+ viscous, body force, solid zero), then the RK combine.
+// Stage 3 fuses the CFL scan: lambda of the new state per cell, max-reduced in
+// FP64 across the wave and the grid.
+//
+// gfx950 specifics:
+//  * lane shifts use DPP wave_shr:1 / wave_shl:1 (one VALU move, no LDS);
+//  * the column of each lane (and its ghost/periodic mapping) is fixed for
+//    the whole strip; the row index is wave-uniform, so the per-row boundary
+//    logic is a scalar branch;
+//  * rows are prefetched two iterations ahead and U0 at the top of the
+//    iteration that consumes it, so HBM latency overlaps the flux arithmetic;
+//  * rarely used features (viscous terms, immersed body, inflow/outlet
+//    boundaries needing the EOS) are template features, so the common
+//    periodic/reflecting case carries none of their registers.
 #include "efvm_stage.h"
+#include "efvm_fast.h"
 
 namespace efvm {
 
 using F4 = fast::C4<float>;
-using FP = fast::Prep<float>;
+using fast::Prep;
 
-__device__ __forceinline__ F4 shfl_up4(const F4& v) { return {shfl_up1(v.r), shfl_up1(v.ru), shfl_up1(v.rv), shfl_up1(v.re)}; }
-__device__ __forceinline__ F4 shfl_dn4(const F4& v) { return {shfl_dn1(v.r), shfl_dn1(v.ru), shfl_dn1(v.rv), shfl_dn1(v.re)}; }
-__device__ __forceinline__ FP shfl_upP(const FP& v) {
-    return {shfl_up1(v.r), shfl_up1(v.ru), shfl_up1(v.rv), shfl_up1(v.re), shfl_up1(v.u), shfl_up1(v.v), shfl_up1(v.p), shfl_up1(v.a)};
+enum { FEAT_VISC = 1, FEAT_IBM = 2, FEAT_CBC = 4 };
+
+// lane i receives the value of lane i-1 (from_left) / lane i+1 (from_right)
+__device__ __forceinline__ float from_left(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, false));
 }
-__device__ __forceinline__ F4 as4(const Cell<float>& c) { return {c.r, c.ru, c.rv, c.re}; }
+__device__ __forceinline__ float from_right(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ int from_left_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int from_right_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ F4 from_left4(const F4& v) { return {from_left(v.r), from_left(v.ru), from_left(v.rv), from_left(v.re)}; }
+__device__ __forceinline__ F4 from_right4(const F4& v) { return {from_right(v.r), from_right(v.ru), from_right(v.rv), from_right(v.re)}; }
+__device__ __forceinline__ Prep from_leftP(const Prep& v) {
+    Prep o;
+    o.r = from_left(v.r); o.ru = from_left(v.ru); o.rv = from_left(v.rv); o.re = from_left(v.re);
+    o.u = from_left(v.u); o.v = from_left(v.v); o.p = from_left(v.p);
+    o.a = from_left(v.a); o.ia = from_left(v.ia); o.ir = from_left(v.ir);
+    return o;
+}
+__device__ __forceinline__ F4 sub4(const F4& a, const F4& b) { return {a.r - b.r, a.ru - b.ru, a.rv - b.rv, a.re - b.re}; }
+
+// reflecting / copying ghost of an owner cell (SupersonicOutflow, SlipWall, Symmetry, NoSlipWall)
+__device__ __forceinline__ F4 reflect(int kind, int axis, F4 v) {
+    if (kind == BC_SLIP || kind == BC_SYMMETRY) {
+        if (axis == 0) v.ru = -v.ru; else v.rv = -v.rv;
+    } else if (kind == BC_NOSLIP) {
+        v.ru = -v.ru; v.rv = -v.rv;
+    }
+    return v;
+}
+
+// Row r (wave-uniform) of the lane's column.  `cq` is the column to read (the
+// periodic image or, for a ghost lane, the owner column); `xkind` is -1 for a
+// real column, else the boundary kind of the x side the lane is a ghost of
+// (selected once per lane, so no per-lane indexing of the argument block).
+template <int FEAT>
+__device__ __forceinline__ F4 load_row(const StageArgs& a, const float* const* U, int c, int cq, int xkind, int r) {
+    int rr = r, ykind = -1;
+    if (r < 0 || r >= a.ny) {
+        if (a.per_y) { rr = r % a.ny; if (rr < 0) rr += a.ny; }
+        else { ykind = r < 0 ? a.bc_kind[2] : a.bc_kind[3]; rr = r < 0 ? 0 : a.ny - 1; }
+    }
+    if ((FEAT & FEAT_CBC) && (ykind >= 0 || xkind >= 0)) {
+        Cell<float> cl = load_cell(a, U, c, r);   // EOS-based ghosts, evaluated in FP64
+        return {cl.r, cl.ru, cl.rv, cl.re};
+    }
+    const long q = (long)rr * a.nx + cq;
+    F4 v = {U[0][q], U[1][q], U[2][q], U[3][q]};
+    if (ykind >= 0) v = reflect(ykind, 1, v);
+    if (xkind >= 0) v = reflect(xkind, 0, v);
+    return v;
+}
 
 template <int LIM>
-__device__ __forceinline__ void recon_pair(const F4& um, const F4& uc, const F4& up, bool rec_lo, bool rec_hi, F4& lo, F4& hi) {
-    F4 s;
-    s.r = fast::limit<LIM>(up.r - uc.r, uc.r - um.r);
-    s.ru = fast::limit<LIM>(up.ru - uc.ru, uc.ru - um.ru);
-    s.rv = fast::limit<LIM>(up.rv - uc.rv, uc.rv - um.rv);
-    s.re = fast::limit<LIM>(up.re - uc.re, uc.re - um.re);
-    lo = uc; hi = uc;
-    if (rec_lo) { lo.r -= 0.5f * s.r; lo.ru -= 0.5f * s.ru; lo.rv -= 0.5f * s.rv; lo.re -= 0.5f * s.re; }
-    if (rec_hi) { hi.r += 0.5f * s.r; hi.ru += 0.5f * s.ru; hi.rv += 0.5f * s.rv; hi.re += 0.5f * s.re; }
+__device__ __forceinline__ F4 slope4(const F4& dr, const F4& dl) {
+    return {fast::limit<LIM>(dr.r, dl.r), fast::limit<LIM>(dr.ru, dl.ru), fast::limit<LIM>(dr.rv, dl.rv),
+            fast::limit<LIM>(dr.re, dl.re)};
+}
+__device__ __forceinline__ F4 axpy4(const F4& u, float h, const F4& s) {
+    return {fmaf(h, s.r, u.r), fmaf(h, s.ru, u.ru), fmaf(h, s.rv, u.rv), fmaf(h, s.re, u.re)};
 }
 
-// One wave processes strip `s` of the case described by `a`; the planes and the
-// stage/parity are passed separately so the persistent batched kernel can reuse it.
-// The wave owning strip 0 does the per-case clock bookkeeping.
-template <int FLUX, int LIM>   // LIM = LIM_NONE: first order
+struct Prim3 { float u, v, t; };
+__device__ __forceinline__ Prim3 prim3(const F4& q, float gm1, float ir_gas) {
+    float iq = fast::rcp(q.r);
+    float u = q.ru * iq, v = q.rv * iq;
+    return {u, v, (gm1 * (q.re - 0.5f * (q.ru * u + q.rv * v))) * iq * ir_gas};
+}
+
+// One wave processes strip `s` of the case `a`; planes and stage/parity are passed
+// separately so the persistent batched kernel can reuse it.  The wave owning
+// strip 0 does the per-case clock bookkeeping.
+//
+// Row loop: a prologue builds the y "north" state of row j0-1, then one step per
+// row jc = j0 .. j1 computes the y face jc-1/2, the x faces of row jc (jc < j1)
+// and writes row jc-1 (jc > j0).  The steady state is unrolled four times with
+// the register roles rotated (row window of 4, ping-pong for the carried face
+// data), so advancing a row costs no register moves.
+template <int FLUX, int LIM, int FEAT>   // LIM = LIM_NONE: first order
 __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s, const float* const* U,
                                                 const float* const* U0, float* const* O, const int stage,
                                                 const int parity) {
     constexpr bool MUSCL = LIM != LIM_NONE;
+    constexpr bool VISC = (FEAT & FEAT_VISC) != 0;
+    constexpr bool IBM = (FEAT & FEAT_IBM) != 0;
     const int lane = lane_id();
     const int nx = a.nx, ny = a.ny;
     const int sxi = s % a.n_strips_x, syi = s / a.n_strips_x;
@@ -56,144 +127,171 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
     if (a.euler || stage == 0) { ca = 0.f; cb = 1.f; }
     else if (stage == 1) { ca = 0.75f; cb = 0.25f; }
     else { ca = 1.f / 3.f; cb = 2.f / 3.f; }
-    const float ix = (float)a.inv_x, iy = (float)a.inv_y;
+    const float dtix = dt * (float)a.inv_x, dtiy = dt * (float)a.inv_y;
+    const float idx = (float)(1.0 / a.dx), idy = (float)(1.0 / a.dy), idz = (float)(1.0 / a.dz);
+    const float vsum = (float)(1.0 / (a.dx * a.dx) + 1.0 / (a.dy * a.dy) + 1.0 / (a.dz * a.dz));
+    const float ir_gas = (float)(1.0 / a.r_gas);
+    const float kap_fac = (float)(a.gamma * a.r_gas / (a.gamma - 1.0) / 0.72);
+    const float gxdt = (float)a.gx * dt;
 
+    // ---- per-lane column mapping (fixed for the strip) ----
     const bool col_ghost = (c < 0 || c >= nx);
+    int cq = c, xkind = -1;
+    if (col_ghost) {
+        if (a.per_x) { cq = c % nx; if (cq < 0) cq += nx; }
+        else { xkind = c < 0 ? a.bc_kind[0] : a.bc_kind[1]; cq = c < 0 ? 0 : nx - 1; }
+    }
     const bool seam_rec = a.seam_mode == 1;
     const bool xW_rec = MUSCL && (col_ghost ? (a.per_x && seam_rec) : !(c == 0 && !a.per_x));
     const bool xE_rec = MUSCL && (col_ghost ? (a.per_x && seam_rec) : !(c == nx - 1 && !a.per_x));
     const bool out_lane = lane >= 2 && lane < WAVE - 2 && c < nx;
-    const bool visc = a.viscous != 0;
-    const bool ibm = a.solid != nullptr;
-
-    F4 um = as4(load_cell(a, U, c, j0 - 2));
-    F4 uc = as4(load_cell(a, U, c, j0 - 1));
-    F4 up = as4(load_cell(a, U, c, j0));
-    int sm = 0, scl = 0, spl = 0;
-    auto solid_at = [&](int cc, int rr) -> int {
-        if (!ibm || cc < 0 || cc >= nx || rr < 0 || rr >= ny) return 0;
-        return a.solid[(long)rr * nx + cc];
+    auto solid_at = [&](int rr) -> int {
+        if (!IBM || col_ghost || rr < 0 || rr >= ny) return 0;
+        return a.solid[(long)rr * nx + c];
     };
-    if (ibm) { sm = solid_at(c, j0 - 2); scl = solid_at(c, j0 - 1); spl = solid_at(c, j0); }
 
-    FP hi_prev{};
-    F4 G_prev{0.f, 0.f, 0.f, 0.f}, dX_prev{0.f, 0.f, 0.f, 0.f};
-    float vis_prev[3] = {0.f, 0.f, 0.f};
     float lam_max = 0.f;
     int bad = 0;
 
-    for (int jc = j0 - 1; jc <= j1; ++jc) {
-        const bool row_ghost = (jc < 0 || jc >= ny);
-        const bool yS_rec = MUSCL && (row_ghost ? (a.per_y && seam_rec) : !(jc == 0 && !a.per_y));
-        const bool yN_rec = MUSCL && (row_ghost ? (a.per_y && seam_rec) : !(jc == ny - 1 && !a.per_y));
-        FP lo_p, hi_p;
+    // y reconstruction of row jc: south (lo) and north (hi) face states
+    auto y_states = [&](int jc, const F4& uc, const F4& fwd, const F4& bwd, Prep& lo_p, Prep& hi_p) {
         if (MUSCL) {
-            F4 lo, hi;
-            recon_pair<LIM>(um, uc, up, yS_rec, yN_rec, lo, hi);
-            lo_p = fast::prep(lo, g, gm1);
-            hi_p = fast::prep(hi, g, gm1);
+            const bool row_ghost = (jc < 0 || jc >= ny);
+            const bool yS_rec = row_ghost ? (a.per_y && seam_rec) : !(jc == 0 && !a.per_y);
+            const bool yN_rec = row_ghost ? (a.per_y && seam_rec) : !(jc == ny - 1 && !a.per_y);
+            const F4 sy = slope4<LIM>(fwd, bwd);
+            lo_p = fast::prep(yS_rec ? axpy4(uc, -0.5f, sy) : uc, g, gm1);
+            hi_p = fast::prep(yN_rec ? axpy4(uc, 0.5f, sy) : uc, g, gm1);
         } else {
             lo_p = fast::prep(uc, g, gm1);
             hi_p = lo_p;
         }
-        F4 G{0.f, 0.f, 0.f, 0.f};
-        if (jc >= j0) G = fast::flux<FLUX, 1>(hi_prev, lo_p, g, gm1);
+    };
 
-        F4 dX{0.f, 0.f, 0.f, 0.f};
-        float vis[3] = {0.f, 0.f, 0.f};
-        if (jc >= j0 && jc < j1) {
-            FP W_p, E_p;
+    // One row step.  In: um,uc,up = U(jc-1..jc+1), un2 = U(jc+2) (in flight), din = U(jc)-U(jc-1),
+    // hin = north state of row jc-1, gin = G(jc-3/2), xin = x divergence of row jc-1,
+    // vin = viscous terms of row jc-1.  Out: dout, hout, gout = G(jc-1/2), xout, vout.
+    // Solid flags: s_m, s_c, s_p of rows jc-1, jc, jc+1, s_n of jc+2.  Reloads um with U(jc+3).
+    auto step = [&](int jc, F4& um, const F4& uc, const F4& up, F4& din, F4& dout, Prep& hin, Prep& hout,
+                    F4& gin, F4& gout, F4& xin, F4& xout, float* vin, float* vout, int& s_m, int s_c, int s_p) {
+        const bool full = jc < j1;
+        const bool out = jc > j0;
+        F4 u0{0.f, 0.f, 0.f, 0.f};
+        if (out && ca != 0.f && out_lane) {
+            const long q = (long)(jc - 1) * nx + c;
+            u0 = {U0[0][q], U0[1][q], U0[2][q], U0[3][q]};
+        }
+        dout = sub4(up, uc);
+        Prep lo_p;
+        y_states(jc, uc, dout, din, lo_p, hout);
+        gout = fast::flux<FLUX, 1>(hin, lo_p);
+        if (full) {
+            Prep W_p, E_p;
             if (MUSCL) {
-                F4 ul = shfl_up4(uc), ur = shfl_dn4(uc);
-                F4 W, E;
-                recon_pair<LIM>(ul, uc, ur, xW_rec, xE_rec, W, E);
-                W_p = fast::prep(W, g, gm1);
-                E_p = fast::prep(E, g, gm1);
+                const F4 ul = from_left4(uc), ur = from_right4(uc);
+                const F4 sx = slope4<LIM>(sub4(ur, uc), sub4(uc, ul));
+                W_p = fast::prep(xW_rec ? axpy4(uc, -0.5f, sx) : uc, g, gm1);
+                E_p = fast::prep(xE_rec ? axpy4(uc, 0.5f, sx) : uc, g, gm1);
             } else {
-                W_p = (jc == j0 - 1) ? lo_p : lo_p;   // first order: both x states are the cell value
-                E_p = W_p;
+                W_p = lo_p;
+                E_p = lo_p;
             }
-            FP El = shfl_upP(E_p);
-            F4 Fw = fast::flux<FLUX, 0>(El, W_p, g, gm1);
-            F4 Fe = shfl_dn4(Fw);
-            dX = {Fw.r - Fe.r, Fw.ru - Fe.ru, Fw.rv - Fe.rv, Fw.re - Fe.re};
-            if (visc) {
-                // primitives of the centre row and its x-neighbours
-                float ir = fast::rcp(uc.r);
-                float pu = uc.ru * ir, pv = uc.rv * ir;
-                float pt = (gm1 * (uc.re - 0.5f * (uc.ru * pu + uc.rv * pv))) * ir / (float)a.r_gas;
-                float lu = shfl_up1(pu), lv = shfl_up1(pv), lt = shfl_up1(pt);
-                float ru_ = shfl_dn1(pu), rv_ = shfl_dn1(pv), rt = shfl_dn1(pt);
-                int sl_ = shfl_up1(scl), sr_ = shfl_dn1(scl);
-                auto prim_of = [&](const F4& q, float& u, float& v, float& t) {
-                    float iq = fast::rcp(q.r);
-                    u = q.ru * iq; v = q.rv * iq;
-                    t = (gm1 * (q.re - 0.5f * (q.ru * u + q.rv * v))) * iq / (float)a.r_gas;
-                };
-                float du, dv, dtt, uu, vu, tu;
-                prim_of(um, du, dv, dtt);
-                prim_of(up, uu, vu, tu);
-                if (!scl && !col_ghost) {
-                    float mu = a.visc_const ? (float)a.mu : (float)ref::sutherland((double)pt);
-                    float kap = mu * (float)(a.gamma * a.r_gas / (a.gamma - 1.0) / 0.72);
-                    if (sr_) { ru_ = 0.f; rv_ = 0.f; rt = pt; }
-                    if (sl_) { lu = 0.f; lv = 0.f; lt = pt; }
-                    if (spl) { uu = 0.f; vu = 0.f; tu = pt; }
-                    if (sm) { du = 0.f; dv = 0.f; dtt = pt; }
+            const Prep El = from_leftP(E_p);
+            const F4 Fw = fast::flux<FLUX, 0>(El, W_p);
+            const F4 Fe = from_right4(Fw);
+            xout = sub4(Fw, Fe);
+            if (VISC) {
+                const Prim3 pc = prim3(uc, gm1, ir_gas);
+                Prim3 pl{from_left(pc.u), from_left(pc.v), from_left(pc.t)};
+                Prim3 pr{from_right(pc.u), from_right(pc.v), from_right(pc.t)};
+                const int sl_ = from_left_i(s_c), sr_ = from_right_i(s_c);
+                Prim3 pd = prim3(um, gm1, ir_gas), pu = prim3(up, gm1, ir_gas);
+                vout[0] = 0.f; vout[1] = 0.f; vout[2] = 0.f;
+                if (!s_c && !col_ghost) {
+                    const float mu = a.visc_const ? (float)a.mu : (float)ref::sutherland((double)pc.t);
+                    const float kap = mu * kap_fac;
+                    if (sr_) pr = {0.f, 0.f, pc.t};
+                    if (sl_) pl = {0.f, 0.f, pc.t};
+                    if (IBM && s_p) pu = {0.f, 0.f, pc.t};
+                    if (IBM && s_m) pd = {0.f, 0.f, pc.t};
                     const float yim = (float)a.yw[jc], yip = (float)a.yw[ny + jc], yis = (float)a.yw[2 * ny + jc];
                     const float dxi2 = (float)a.dxi2;
-                    float lapu = (ru_ - 2.f * pu + lu) * dxi2 + ((uu - pu) * yip - (pu - du) * yim) * yis;
-                    float lapv = (rv_ - 2.f * pv + lv) * dxi2 + ((vu - pv) * yip - (pv - dv) * yim) * yis;
-                    float lapt = (rt - 2.f * pt + lt) * dxi2 + ((tu - pt) * yip - (pt - dtt) * yim) * yis;
-                    vis[0] = mu * lapu; vis[1] = mu * lapv;
-                    vis[2] = pu * vis[0] + pv * vis[1] + kap * lapt;
+                    const float lapu = (pr.u - 2.f * pc.u + pl.u) * dxi2 + ((pu.u - pc.u) * yip - (pc.u - pd.u) * yim) * yis;
+                    const float lapv = (pr.v - 2.f * pc.v + pl.v) * dxi2 + ((pu.v - pc.v) * yip - (pc.v - pd.v) * yim) * yis;
+                    const float lapt = (pr.t - 2.f * pc.t + pl.t) * dxi2 + ((pu.t - pc.t) * yip - (pc.t - pd.t) * yim) * yis;
+                    vout[0] = mu * lapu;
+                    vout[1] = mu * lapv;
+                    vout[2] = pc.u * vout[0] + pc.v * vout[1] + kap * lapt;
                 }
             }
         }
-
-        if (jc - 1 >= j0 && out_lane) {
-            const int j = jc - 1;
-            const long q = (long)j * nx + c;
-            float rr = dX_prev.r * ix + (G_prev.r - G.r) * iy;
-            float rru = dX_prev.ru * ix + (G_prev.ru - G.ru) * iy + vis_prev[0];
-            float rrv = dX_prev.rv * ix + (G_prev.rv - G.rv) * iy + vis_prev[1];
-            float rre = dX_prev.re * ix + (G_prev.re - G.re) * iy + vis_prev[2];
-            if (a.body_force) { float gx = (float)a.gx; rru += um.r * gx; rre += gx * um.ru; }
-            if (ibm && a.solid[q]) { rr = 0.f; rru = 0.f; rrv = 0.f; rre = 0.f; }
-            float o0 = um.r + dt * rr, o1 = um.ru + dt * rru, o2 = um.rv + dt * rrv, o3 = um.re + dt * rre;
+        if (out && out_lane) {
+            const long q = (long)(jc - 1) * nx + c;
+            float rr = xin.r * dtix + (gin.r - gout.r) * dtiy;
+            float rru = xin.ru * dtix + (gin.ru - gout.ru) * dtiy;
+            float rrv = xin.rv * dtix + (gin.rv - gout.rv) * dtiy;
+            float rre = xin.re * dtix + (gin.re - gout.re) * dtiy;
+            if (VISC) { rru += dt * vin[0]; rrv += dt * vin[1]; rre += dt * vin[2]; }
+            if (a.body_force) { rru += um.r * gxdt; rre += gxdt * um.ru; }
+            if (IBM && a.solid[q]) { rr = 0.f; rru = 0.f; rrv = 0.f; rre = 0.f; }
+            float o0 = um.r + rr, o1 = um.ru + rru, o2 = um.rv + rrv, o3 = um.re + rre;
             if (ca != 0.f) {
-                o0 = ca * U0[0][q] + cb * o0; o1 = ca * U0[1][q] + cb * o1;
-                o2 = ca * U0[2][q] + cb * o2; o3 = ca * U0[3][q] + cb * o3;
+                o0 = ca * u0.r + cb * o0; o1 = ca * u0.ru + cb * o1;
+                o2 = ca * u0.rv + cb * o2; o3 = ca * u0.re + cb * o3;
             }
             O[0][q] = o0; O[1][q] = o1; O[2][q] = o2; O[3][q] = o3;
             if (fuse) {
-                float ir = fast::rcp(o0);
-                float vu = fabsf(o1 * ir), vv = fabsf(o2 * ir);
-                float p = gm1 * (o3 - 0.5f * (o1 * o1 + o2 * o2) * ir);
-                float snd = fast::fsqrt(g * p * ir);
-                float l = (vu + snd) * (float)(1.0 / a.dx) + (vv + snd) * (float)(1.0 / a.dy) + snd * (float)(1.0 / a.dz);
-                if (a.viscous) {
-                    float t = p * ir / (float)a.r_gas;
-                    float mu = a.visc_const ? (float)a.mu : (float)ref::sutherland((double)t);
-                    l += 2.f * mu * ir * (float)(1.0 / (a.dx * a.dx) + 1.0 / (a.dy * a.dy) + 1.0 / (a.dz * a.dz));
+                const float ir = fast::rcp(o0);
+                const float vu = fabsf(o1 * ir), vv = fabsf(o2 * ir);
+                const float p = gm1 * (o3 - 0.5f * (o1 * o1 + o2 * o2) * ir);
+                const float snd = fast::fsqrt(g * p * ir);
+                float l = (vu + snd) * idx + (vv + snd) * idy + snd * idz;
+                if (VISC) {
+                    const float t = p * ir * ir_gas;
+                    const float mu = a.visc_const ? (float)a.mu : (float)ref::sutherland((double)t);
+                    l += 2.f * mu * ir * vsum;
                 }
                 if (o0 > 0.f && l == l && p > 0.f) lam_max = fmaxf(lam_max, l);
                 else bad++;
             }
         }
-
-        hi_prev = hi_p; G_prev = G; dX_prev = dX;
-        vis_prev[0] = vis[0]; vis_prev[1] = vis[1]; vis_prev[2] = vis[2];
-        um = uc; uc = up;
-        if (ibm) { sm = scl; scl = spl; }
-        if (jc + 2 <= j1 + 1) {
-            up = as4(load_cell(a, U, c, jc + 2));
-            if (ibm) spl = solid_at(c, jc + 2);
+        if (full && jc + 3 <= j1 + 1) {
+            um = load_row<FEAT>(a, U, c, cq, xkind, jc + 3);
+            if (IBM) s_m = solid_at(jc + 3);
         }
+    };
+
+    // ---- prologue: rows j0-2 .. j0+2, north state of row j0-1 ----
+    F4 R0 = load_row<FEAT>(a, U, c, cq, xkind, j0 - 1);
+    F4 R1 = load_row<FEAT>(a, U, c, cq, xkind, j0);
+    F4 R2 = load_row<FEAT>(a, U, c, cq, xkind, j0 + 1);
+    F4 R3 = load_row<FEAT>(a, U, c, cq, xkind, j0 + 2);
+    int S0 = 0, S1 = 0, S2 = 0, S3 = 0;
+    if (IBM) { S0 = solid_at(j0 - 1); S1 = solid_at(j0); S2 = solid_at(j0 + 1); S3 = solid_at(j0 + 2); }
+    F4 D0, D1;
+    Prep H0, H1;
+    F4 G0{0.f, 0.f, 0.f, 0.f}, G1{0.f, 0.f, 0.f, 0.f}, X0{0.f, 0.f, 0.f, 0.f}, X1{0.f, 0.f, 0.f, 0.f};
+    float V0[3] = {0.f, 0.f, 0.f}, V1[3] = {0.f, 0.f, 0.f};
+    {
+        const F4 rm2 = load_row<FEAT>(a, U, c, cq, xkind, j0 - 2);
+        D0 = sub4(R1, R0);                 // U(j0) - U(j0-1): forward difference of row j0-1
+        Prep lo_unused;
+        y_states(j0 - 1, R0, D0, sub4(R0, rm2), lo_unused, H0);
     }
+    // ---- steady state: rows j0 .. j1, unrolled x4 with rotated roles ----
+    for (int jc = j0; jc <= j1; jc += 4) {
+        step(jc, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, V0, V1, S0, S1, S2);
+        if (jc + 1 > j1) break;
+        step(jc + 1, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, V1, V0, S1, S2, S3);
+        if (jc + 2 > j1) break;
+        step(jc + 2, R2, R3, R0, D0, D1, H0, H1, G0, G1, X0, X1, V0, V1, S2, S3, S0);
+        if (jc + 3 > j1) break;
+        step(jc + 3, R3, R0, R1, D1, D0, H1, H0, G1, G0, X1, X0, V1, V0, S3, S0, S1);
+    }
+
     if (fuse) {
-        double w = wave_max((double)lam_max);
-        long long b = wave_sum_ll((long long)bad);
+        const double w = wave_max((double)lam_max);
+        const long long b = wave_sum_ll((long long)bad);
         if (lane == 0) {
             if (w > 0.0) atomic_max_pos_double(&a.clock->lam[parity ^ 1], w);
             if (b) atomicAdd((unsigned long long*)&a.clock->bad, (unsigned long long)b);
@@ -205,14 +303,14 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
     }
 }
 
-template <int FLUX, int LIM>
+template <int FLUX, int LIM, int FEAT>
 __global__ __launch_bounds__(256) void stage_f32_kernel(StageArgs a) {
     const int s = strip_index(a, blockDim.x / WAVE);
     if (s >= a.n_strips_x * a.n_strips_y) return;
     const float* U[4] = {(const float*)a.uin[0], (const float*)a.uin[1], (const float*)a.uin[2], (const float*)a.uin[3]};
     const float* U0[4] = {(const float*)a.u0[0], (const float*)a.u0[1], (const float*)a.u0[2], (const float*)a.u0[3]};
     float* O[4] = {(float*)a.uout[0], (float*)a.uout[1], (float*)a.uout[2], (float*)a.uout[3]};
-    stage_f32_strip<FLUX, LIM>(a, s, U, U0, O, a.stage, a.parity);
+    stage_f32_strip<FLUX, LIM, FEAT>(a, s, U, U0, O, a.stage, a.parity);
 }
 
 // ---------------------------------------------------------------------------
@@ -222,7 +320,7 @@ __global__ __launch_bounds__(256) void stage_f32_kernel(StageArgs a) {
 // DevClock (per-case dt from its own CFL maximum) and its own plane sets
 // A, B, C; all cases advance in lock-step stage by stage.
 // ---------------------------------------------------------------------------
-template <int FLUX, int LIM>
+template <int FLUX, int LIM, int FEAT>
 __global__ __launch_bounds__(256) void stage_f32_batch_kernel(const BatchCase* __restrict__ cases,
                                                               const int2* __restrict__ work, int n_work,
                                                               int stage, int parity) {
@@ -237,41 +335,65 @@ __global__ __launch_bounds__(256) void stage_f32_batch_kernel(const BatchCase* _
         const float* U[4] = {(const float*)bc.P[in][0], (const float*)bc.P[in][1], (const float*)bc.P[in][2], (const float*)bc.P[in][3]};
         const float* U0[4] = {(const float*)bc.P[0][0], (const float*)bc.P[0][1], (const float*)bc.P[0][2], (const float*)bc.P[0][3]};
         float* O[4] = {(float*)bc.P[out][0], (float*)bc.P[out][1], (float*)bc.P[out][2], (float*)bc.P[out][3]};
-        stage_f32_strip<FLUX, LIM>(bc.a, wk.y, U, U0, O, stage, parity);
+        stage_f32_strip<FLUX, LIM, FEAT>(bc.a, wk.y, U, U0, O, stage, parity);
     }
 }
 
-hipError_t launch_stage_f32_batch(const BatchCase* cases, const int2* work, int n_work, int flux, int lim, int stage,
-                                  int parity, int blocks, int waves, hipStream_t st) {
-    dim3 grid(blocks), block(WAVE * waves);
-#define EFVM_F32B_CASE(F)                                                                                        \
-    if (lim == LIM_NONE) stage_f32_batch_kernel<F, LIM_NONE><<<grid, block, 0, st>>>(cases, work, n_work, stage, parity);      \
-    else if (lim == LIM_MINMOD) stage_f32_batch_kernel<F, LIM_MINMOD><<<grid, block, 0, st>>>(cases, work, n_work, stage, parity); \
-    else stage_f32_batch_kernel<F, LIM_MC><<<grid, block, 0, st>>>(cases, work, n_work, stage, parity);
-    switch (flux) {
-    case FLUX_RUSANOV: EFVM_F32B_CASE(FLUX_RUSANOV); break;
-    case FLUX_HLLC: EFVM_F32B_CASE(FLUX_HLLC); break;
-    default: EFVM_F32B_CASE(FLUX_HLLC_TEXTBOOK); break;
-    }
-#undef EFVM_F32B_CASE
+namespace {
+int features_of(const StageArgs& a) {
+    int f = 0;
+    if (a.viscous) f |= FEAT_VISC;
+    if (a.solid) f |= FEAT_IBM;
+    for (int s = 0; s < 4; s++)
+        if (a.bc_kind[s] == BC_SUP_INFLOW || a.bc_kind[s] == BC_SUB_INFLOW || a.bc_kind[s] == BC_PRESSURE_OUTLET) f |= FEAT_CBC;
+    if (a.parabolic) f |= FEAT_CBC;
+    return f;
+}
+}  // namespace
+
+// Host dispatch over (flux, limiter, features).  The common production shapes are
+// specialised; any other feature mix uses the all-features kernel.
+#define EFVM_F32_DISPATCH(LAUNCH)                                                           \
+    do {                                                                                    \
+        const int lim = a_.muscl ? a_.limiter : LIM_NONE;                                   \
+        const int ft = features_of(a_);                                                     \
+        const int all = FEAT_VISC | FEAT_IBM | FEAT_CBC;                                    \
+        switch (a_.flux) {                                                                  \
+        case FLUX_RUSANOV:                                                                  \
+            if (lim == LIM_NONE) { if (ft == 0) LAUNCH(FLUX_RUSANOV, LIM_NONE, 0); else LAUNCH(FLUX_RUSANOV, LIM_NONE, all); } \
+            else if (lim == LIM_MINMOD) { if (ft == 0) LAUNCH(FLUX_RUSANOV, LIM_MINMOD, 0); else LAUNCH(FLUX_RUSANOV, LIM_MINMOD, all); } \
+            else { if (ft == 0) LAUNCH(FLUX_RUSANOV, LIM_MC, 0); else LAUNCH(FLUX_RUSANOV, LIM_MC, all); } \
+            break;                                                                          \
+        case FLUX_HLLC:                                                                     \
+            if (lim == LIM_NONE) { if (ft == 0) LAUNCH(FLUX_HLLC, LIM_NONE, 0); else LAUNCH(FLUX_HLLC, LIM_NONE, all); } \
+            else if (lim == LIM_MINMOD) { if (ft == 0) LAUNCH(FLUX_HLLC, LIM_MINMOD, 0); else LAUNCH(FLUX_HLLC, LIM_MINMOD, all); } \
+            else { if (ft == 0) LAUNCH(FLUX_HLLC, LIM_MC, 0); else if (ft == FEAT_CBC) LAUNCH(FLUX_HLLC, LIM_MC, FEAT_CBC); else LAUNCH(FLUX_HLLC, LIM_MC, all); } \
+            break;                                                                          \
+        default:                                                                            \
+            if (lim == LIM_NONE) { if (ft == 0) LAUNCH(FLUX_HLLC_TEXTBOOK, LIM_NONE, 0); else LAUNCH(FLUX_HLLC_TEXTBOOK, LIM_NONE, all); } \
+            else if (lim == LIM_MINMOD) { if (ft == 0) LAUNCH(FLUX_HLLC_TEXTBOOK, LIM_MINMOD, 0); else LAUNCH(FLUX_HLLC_TEXTBOOK, LIM_MINMOD, all); } \
+            else { if (ft == 0) LAUNCH(FLUX_HLLC_TEXTBOOK, LIM_MC, 0); else LAUNCH(FLUX_HLLC_TEXTBOOK, LIM_MC, all); } \
+            break;                                                                          \
+        }                                                                                   \
+    } while (0)
+
+hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st) {
+    const int waves = block / WAVE;
+    const int nstrips = a.n_strips_x * a.n_strips_y;
+    const dim3 grid((nstrips + waves - 1) / waves);
+    const StageArgs& a_ = a;
+#define EFVM_LAUNCH_ONE(F, L, X) stage_f32_kernel<F, L, X><<<grid, block, 0, st>>>(a)
+    EFVM_F32_DISPATCH(EFVM_LAUNCH_ONE);
+#undef EFVM_LAUNCH_ONE
     return hipGetLastError();
 }
 
-hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st) {
-    int waves = block / WAVE;
-    int nstrips = a.n_strips_x * a.n_strips_y;
-    dim3 grid((nstrips + waves - 1) / waves);
-    const int lim = a.muscl ? a.limiter : LIM_NONE;
-#define EFVM_F32_CASE(F)                                                                   \
-    if (lim == LIM_NONE) stage_f32_kernel<F, LIM_NONE><<<grid, block, 0, st>>>(a);          \
-    else if (lim == LIM_MINMOD) stage_f32_kernel<F, LIM_MINMOD><<<grid, block, 0, st>>>(a); \
-    else stage_f32_kernel<F, LIM_MC><<<grid, block, 0, st>>>(a);
-    switch (a.flux) {
-    case FLUX_RUSANOV: EFVM_F32_CASE(FLUX_RUSANOV); break;
-    case FLUX_HLLC: EFVM_F32_CASE(FLUX_HLLC); break;
-    default: EFVM_F32_CASE(FLUX_HLLC_TEXTBOOK); break;
-    }
-#undef EFVM_F32_CASE
+hipError_t launch_stage_f32_batch(const BatchCase* cases, const StageArgs& a_, const int2* work, int n_work, int stage,
+                                  int parity, int blocks, int waves, hipStream_t st) {
+    const dim3 grid(blocks), block(WAVE * waves);
+#define EFVM_LAUNCH_B(F, L, X) stage_f32_batch_kernel<F, L, X><<<grid, block, 0, st>>>(cases, work, n_work, stage, parity)
+    EFVM_F32_DISPATCH(EFVM_LAUNCH_B);
+#undef EFVM_LAUNCH_B
     return hipGetLastError();
 }
 
