@@ -57,7 +57,7 @@ def _time_ctx(ctx, steps):
     return ctx.last_elapsed_ms() / steps
 
 
-def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=True, verbose=False):
+def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=True, verbose=False, store=True):
     """Pick (strip_rows, waves_per_block) for a context; returns (rows, waves, ms_per_step)."""
     key = f"{arch_name()}:{ctx.precision}:ctx:{size_class(ctx.nx)}x{size_class(ctx.ny)}"
     db = _load()
@@ -77,8 +77,9 @@ def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=
             if best is None or ms < best[2]:
                 best = (r, w, ms)
     ctx.set_launch(best[0], best[1])
-    db[key] = {"rows": best[0], "waves": best[1], "ms": best[2], "when": time.time()}
-    _store(db)
+    if store:
+        db[key] = {"rows": best[0], "waves": best[1], "ms": best[2], "when": time.time()}
+        _store(db)
     return best
 
 
@@ -88,7 +89,7 @@ def _time_batch(b, steps):
     return b.last_elapsed_ms() / steps
 
 
-def tune_batch(batch, sides, steps: int = 2, rows=BATCH_ROWS, blocks=BATCH_BLOCKS, use_cache=True, verbose=False,
+def tune_batch(batch, sides, steps: int = 2, rows=BATCH_ROWS, blocks=BATCH_BLOCKS, use_cache=True, verbose=False, store=True,
                key_extra=""):
     """Per size class strip rows, then the persistent grid size (coordinate descent).
 
@@ -134,8 +135,9 @@ def tune_batch(batch, sides, steps: int = 2, rows=BATCH_ROWS, blocks=BATCH_BLOCK
         if ms < best_ms:
             best_ms, nb = ms, g
     batch.set_grid(nb, 4)
-    db[key] = {"rows": {str(k): v for k, v in choice.items()}, "blocks": nb, "ms": best_ms, "when": time.time()}
-    _store(db)
+    if store:
+        db[key] = {"rows": {str(k): v for k, v in choice.items()}, "blocks": nb, "ms": best_ms, "when": time.time()}
+        _store(db)
     return choice, nb, best_ms
 
 

@@ -83,7 +83,7 @@ def test_batch_graph_replay_equals_eager_and_survives_retuning():
         eager.upload(i, st)
         graph.upload(i, st)
     sides = [s["nx"] for s in specs]
-    tune_batch(graph, sides, steps=1, rows=(8, 32), blocks=(0, 64), use_cache=False)
+    tune_batch(graph, sides, steps=1, rows=(8, 32), blocks=(0, 64), use_cache=False, store=False)
     graph.graph_capture(2)
     for rep in range(3):
         eager.step(2)
