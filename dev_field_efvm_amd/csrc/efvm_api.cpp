@@ -85,7 +85,7 @@ struct efvm_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int timed = 0;
     void* snap = nullptr;          // autotuner snapshot (state + clock)
-    int snap_parity = 0, snap_lam = 0;
+    int snap_parity = 0, snap_lam = 0, snap_flip = 0;
     std::string err;
 };
 
@@ -1022,10 +1022,10 @@ efvm_status efvm_snapshot(efvm_ctx* c, int op) {
     }
     if (op == 0) {
         HIPCHK(c, hipMemcpyAsync(s + 4 * pbytes, c->clock, sizeof(DevClock), hipMemcpyDeviceToDevice, c->stream));
-        c->snap_parity = c->parity; c->snap_lam = c->lam_valid;
+        c->snap_parity = c->parity; c->snap_lam = c->lam_valid; c->snap_flip = c->ibm_flipped;
     } else {
         HIPCHK(c, hipMemcpyAsync(c->clock, s + 4 * pbytes, sizeof(DevClock), hipMemcpyDeviceToDevice, c->stream));
-        c->parity = c->snap_parity; c->lam_valid = c->snap_lam;
+        c->parity = c->snap_parity; c->lam_valid = c->snap_lam; c->ibm_flipped = c->snap_flip;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;
