@@ -112,6 +112,67 @@ def vortex_modes_state(nx, ny, seed=1, amplitude=0.02, L=10.0, beta=5.0, gamma=1
     return prim_to_cons(gamma, rho, u, v, p)
 
 
+def learned_vortices(seed: int, L: float = 10.0, min_sep: float = 3.0):
+    """1..4 vortices (cx, cy, beta), beta ~ U[2, 6], centres uniform on the periodic box and at
+    least `min_sep` apart (minimum-image distance), so superposed cores keep a positive temperature."""
+    r = Rng(splitmix64(seed) ^ 0x6A09E667F3BCC909)
+    nv = r.randint(1, 4)
+    out = []
+    while len(out) < nv:
+        cx, cy, beta = r.uniform(0.0, L), r.uniform(0.0, L), r.uniform(2.0, 6.0)
+        ok = True
+        for ox, oy, _ in out:
+            ddx = cx - ox - L * round((cx - ox) / L)
+            ddy = cy - oy - L * round((cy - oy) / L)
+            ok &= ddx * ddx + ddy * ddy >= min_sep * min_sep
+        if ok:
+            out.append((cx, cy, beta))
+    return out
+
+
+def learned_state(seed: int, n: int, L: float = 10.0, sub: int = 4, amplitude: float = 0.05, gamma: float = 1.4,
+                  mean_u: float = 1.0, mean_v: float = 1.0):
+    """Learned-reconstruction workload on the periodic [0,L]^2 box: mean flow (1, 1), rho = p = 1,
+    1..4 superposed isentropic vortices (``learned_vortices``) and 8 seeded Fourier modes of
+    amplitude `amplitude` on rho, u, v (pressure from the vortices).  Returned as cell averages
+    (sub x sub midpoint samples per cell), so grids of different resolution discretise the same
+    continuous field.  (n, n, 5) in the wire order."""
+    vort = learned_vortices(seed, L)
+    modes = fourier_modes(seed, 8)
+    h = L / n
+    acc = np.zeros((n, n, 5))
+    g = gamma
+    for a in range(sub):
+        x1 = (np.arange(n) + (a + 0.5) / sub) * h
+        for b in range(sub):
+            y1 = (np.arange(n) + (b + 0.5) / sub) * h
+            x, y = np.meshgrid(x1, y1)
+            du = np.zeros_like(x)
+            dv = np.zeros_like(x)
+            dtemp = np.zeros_like(x)
+            for cx, cy, beta in vort:
+                ddx = x - cx
+                ddx -= L * np.round(ddx / L)
+                ddy = y - cy
+                ddy -= L * np.round(ddy / L)
+                r2 = ddx * ddx + ddy * ddy
+                f = np.exp(0.5 * (1.0 - r2))
+                du += -beta / (2.0 * math.pi) * ddy * f
+                dv += beta / (2.0 * math.pi) * ddx * f
+                dtemp += -(g - 1.0) * beta * beta / (8.0 * g * math.pi * math.pi) * np.exp(1.0 - r2)
+            rho = (1.0 + dtemp) ** (1.0 / (g - 1.0))
+            p = rho ** g
+            u = mean_u + du
+            v = mean_v + dv
+            for kx, ky, pr, pu, pv in modes:
+                arg = 2.0 * math.pi * (kx * x + ky * y) / L
+                rho = rho + amplitude * np.sin(arg + pr)
+                u = u + amplitude * np.sin(arg + pu)
+                v = v + amplitude * np.sin(arg + pv)
+            acc += prim_to_cons(g, rho, u, v, p)
+    return acc / (sub * sub)
+
+
 def density_l1(rho_num, rho_exact, cell_volume=1.0):
     """Volume-weighted L1 density error (examples/isentropic_vortex/main.rs:189-211)."""
     return float(np.sum(np.abs(rho_num - rho_exact) * cell_volume) / (rho_num.size * cell_volume))

@@ -18,6 +18,7 @@
 
 #include "../../include/efvm.h"
 #include "efvm_stage.h"
+#include "efvm_mlp.h"
 
 namespace efvm {
 hipError_t launch_stage_f64(const StageArgs& a, int block, hipStream_t st);
@@ -38,6 +39,13 @@ template <class T> hipError_t launch_sponge(const StageArgs& a, double x0, doubl
                                             double rho, double p, hipStream_t st);
 template <class T> hipError_t launch_sums(const StageArgs& a, double* part, int nb, double vol, double* out, hipStream_t st);
 hipError_t launch_ibm_flip(void* const* planes, int esz, const uint8_t* solid, long n, const double* stagnant, hipStream_t st);
+hipError_t launch_stage_f32_faces(const StageArgs& a, const float* F, int block, hipStream_t st);
+hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st);
+hipError_t launch_mlp_infer(const mlp::Packed* P, const float* X, long n, float* Y, hipStream_t st);
+namespace mlp { struct TrainArgs; struct SampleArgs; }
+hipError_t launch_mlp_train(const mlp::TrainArgs& a, int nblk, float* grad, double* loss_out, hipStream_t st);
+hipError_t launch_mlp_adam(float* p, const float* g, float* m, float* v, float lr, int t, hipStream_t st);
+hipError_t launch_mlp_samples(const mlp::SampleArgs& a, long n_out, hipStream_t st);
 }  // namespace efvm
 
 using namespace efvm;
@@ -86,6 +94,10 @@ struct efvm_ctx {
     int timed = 0;
     void* snap = nullptr;          // autotuner snapshot (state + clock)
     int snap_parity = 0, snap_lam = 0, snap_flip = 0;
+    int learned = 0;               // learned face reconstruction attached
+    float* faces = nullptr;        // 16 planes of face primitives
+    mlp::Packed* wpack = nullptr;  // the attached network's inference weights
+    unsigned long long* lstats = nullptr;   // [fallback faces, faces evaluated]
     std::string err;
 };
 
@@ -209,7 +221,29 @@ efvm_status run_substage(efvm_ctx* c) {
                                    : launch_ibm_mirror<float>(a, ib, c->n_ghost, c->ghost_idx, c->ghost_img, c->stream);
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "ibm mirror");
     }
-    if ((st = launch_stage(c, a)) != EFVM_OK) return st;
+    if (c->learned) {
+        // face states of the stage input from the network (with the MUSCL fallback), then the stage
+        mlp::FaceArgs fa;
+        fa.nx = c->grid.nx; fa.ny = c->grid.ny;
+        fa.n_chunks = (c->grid.nx + 63) / 64;
+        fa.n_tasks = 2 * c->grid.ny * fa.n_chunks;
+        fa.gamma = (float)c->gas.gamma;
+        for (int m = 0; m < 4; m++) fa.U[m] = (const float*)in[m];
+        fa.F = c->faces;
+        fa.w = c->wpack;
+        fa.stats = c->lstats;
+        hipError_t e = launch_mlp_faces(fa, c->stream);
+        if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("face-state kernel: ") + hipGetErrorString(e));
+        // strips of 63 columns; rows per strip so that about 4096 strips fill the 256 CUs
+        // (each strip row is only two fluxes, there is no stencil to amortise)
+        a.n_strips_x = (c->grid.nx + WAVE - 2) / (WAVE - 1);
+        a.strip_h = std::max(1, std::min(c->strip_h, (int)((long)a.n_strips_x * c->grid.ny / 4096)));
+        a.n_strips_y = (c->grid.ny + a.strip_h - 1) / a.strip_h;
+        e = launch_stage_f32_faces(a, c->faces, WAVE * c->waves_per_block, c->stream);
+        if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("learned stage kernel: ") + hipGetErrorString(e));
+    } else if ((st = launch_stage(c, a)) != EFVM_OK) {
+        return st;
+    }
     if (last) {
         if (!rk3) c->cur ^= 1;   // Euler: U^{n+1} is in B
         StageArgs b = base_args(c);
@@ -310,6 +344,9 @@ void efvm_ctx_destroy(efvm_ctx* c) {
     if (c->yw) hipFree(c->yw);
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->snap) (void)hipFree(c->snap);
+    if (c->faces) (void)hipFree(c->faces);
+    if (c->wpack) (void)hipFree(c->wpack);
+    if (c->lstats) (void)hipFree(c->lstats);
     free_ibm(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -355,6 +392,16 @@ efvm_status efvm_set_sponge(efvm_ctx* c, double x0, double x1, double sigma, dou
     c->sponge = 1;
     double v[7] = {x0, x1, sigma, u_max, height, rho, p};
     std::memcpy(c->sp, v, sizeof(v));
+    drop_graph(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm_set_time_step(efvm_ctx* c, double cfl, double fixed_dt) {
+    if (!c || !(cfl > 0.0)) return EFVM_ERR_ARG;
+    if (c->rk_stage != 0) return fail(c, EFVM_ERR_ARG, "time-step control changes at a step boundary only");
+    c->sch.cfl = cfl;
+    c->sch.fixed_dt = fixed_dt;
+    c->lam_valid = 0;              // back in CFL mode the next step rescans the state
     drop_graph(c);
     return EFVM_OK;
 }
@@ -1027,6 +1074,392 @@ efvm_status efvm_snapshot(efvm_ctx* c, int op) {
         HIPCHK(c, hipMemcpyAsync(c->clock, s + 4 * pbytes, sizeof(DevClock), hipMemcpyDeviceToDevice, c->stream));
         c->parity = c->snap_parity; c->lam_valid = c->snap_lam; c->ibm_flipped = c->snap_flip;
     }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned face reconstruction: network object, training, attachment to a context
+// ---------------------------------------------------------------------------
+struct efvm_mlp {
+    hipStream_t stream = nullptr;
+    float* p = nullptr;            // FP32 master parameters
+    float* g = nullptr;            // gradient
+    float* m = nullptr;            // Adam moments
+    float* v = nullptr;
+    long adam_t = 0;
+    float* X = nullptr;            // samples x 20
+    float* Y = nullptr;            // samples x 4
+    long n_data = 0, cap = 0;
+    float* partial = nullptr;      // per-block gradient partials
+    double* loss_partial = nullptr;
+    long part_blocks = 0;
+    double* losses = nullptr;      // per-step minibatch losses
+    long loss_cap = 0;
+    mlp::Packed* packed = nullptr;
+    int packed_valid = 0;
+    std::string err;
+};
+
+namespace {
+
+efvm_status mfail(efvm_mlp* m, efvm_status s, const std::string& msg) {
+    if (m) m->err = msg;
+    return s;
+}
+#define MHIPCHK(mm, expr)                                                                              \
+    do {                                                                                               \
+        hipError_t _e = (expr);                                                                        \
+        if (_e != hipSuccess) return mfail(mm, EFVM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// round to nearest bfloat16, ties to even
+uint16_t to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+void put8(uint32_t dst[4], const float vals[8]) {
+    for (int d = 0; d < 4; d++) dst[d] = (uint32_t)to_bf16(vals[2 * d]) | ((uint32_t)to_bf16(vals[2 * d + 1]) << 16);
+}
+
+// hidden unit that K index (step s, lane group g, element e) stands for between layers
+int perm_h(int s, int g, int e) { return 16 * (2 * s + e / 4) + 4 * g + e % 4; }
+
+void pack_weights(const float* p, mlp::Packed& P) {
+    using namespace mlp;
+    const float* W1 = p + OFF_W1;
+    const float* W2 = p + OFF_W2;
+    const float* W3 = p + OFF_W3;
+    float v8[8];
+    for (int t = 0; t < 4; t++)
+        for (int ln = 0; ln < 64; ln++) {
+            const int o = 16 * t + ln % 16, g = ln / 16;
+            for (int e = 0; e < 8; e++) {
+                const int k = 8 * g + e;
+                v8[e] = k < NIN ? W1[o * NIN + k] : (k == NIN ? p[OFF_B1 + o] : 0.f);   // feature 20 == 1
+            }
+            put8(P.w1[t][ln], v8);
+            for (int s = 0; s < 2; s++) {
+                for (int e = 0; e < 8; e++) v8[e] = W2[o * NH + perm_h(s, g, e)];
+                put8(P.w2[t][s][ln], v8);
+            }
+        }
+    for (int s = 0; s < 2; s++)
+        for (int ln = 0; ln < 64; ln++) {
+            const int r = (ln % 16) % 4, g = ln / 16;
+            for (int e = 0; e < 8; e++) v8[e] = W3[r * NH + perm_h(s, g, e)];
+            put8(P.w3[s][ln], v8);
+        }
+    std::memcpy(P.b1, p + OFF_B1, sizeof(float) * NH);
+    std::memcpy(P.b2, p + OFF_B2, sizeof(float) * NH);
+    std::memcpy(P.b3, p + OFF_B3, sizeof(float) * NOUT);
+    std::memcpy(P.L, p + OFF_L, sizeof(float) * NOUT * NIN);
+    std::memcpy(P.bL, p + OFF_BL, sizeof(float) * NOUT);
+}
+
+efvm_status mlp_repack(efvm_mlp* m) {
+    if (m->packed_valid) return EFVM_OK;
+    std::vector<float> h(mlp::NPARAM);
+    MHIPCHK(m, hipMemcpyAsync(h.data(), m->p, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    mlp::Packed P;
+    pack_weights(h.data(), P);
+    MHIPCHK(m, hipMemcpyAsync(m->packed, &P, sizeof(P), hipMemcpyHostToDevice, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    m->packed_valid = 1;
+    return EFVM_OK;
+}
+
+efvm_status mlp_reserve_partials(efvm_mlp* m, long blocks) {
+    if (blocks <= m->part_blocks) return EFVM_OK;
+    if (m->partial) (void)hipFree(m->partial);
+    if (m->loss_partial) (void)hipFree(m->loss_partial);
+    m->partial = nullptr; m->loss_partial = nullptr; m->part_blocks = 0;
+    MHIPCHK(m, hipMalloc((void**)&m->partial, sizeof(float) * mlp::NPARAM * blocks));
+    MHIPCHK(m, hipMalloc((void**)&m->loss_partial, sizeof(double) * blocks));
+    m->part_blocks = blocks;
+    return EFVM_OK;
+}
+
+efvm_status mlp_grow(efvm_mlp* m, long need) {
+    if (need <= m->cap) return EFVM_OK;
+    long cap = std::max(need, 2 * m->cap);
+    float *X = nullptr, *Y = nullptr;
+    MHIPCHK(m, hipMalloc((void**)&X, sizeof(float) * mlp::NIN * cap));
+    MHIPCHK(m, hipMalloc((void**)&Y, sizeof(float) * mlp::NOUT * cap));
+    if (m->n_data) {
+        MHIPCHK(m, hipMemcpyAsync(X, m->X, sizeof(float) * mlp::NIN * m->n_data, hipMemcpyDeviceToDevice, m->stream));
+        MHIPCHK(m, hipMemcpyAsync(Y, m->Y, sizeof(float) * mlp::NOUT * m->n_data, hipMemcpyDeviceToDevice, m->stream));
+        MHIPCHK(m, hipStreamSynchronize(m->stream));
+    }
+    if (m->X) (void)hipFree(m->X);
+    if (m->Y) (void)hipFree(m->Y);
+    m->X = X; m->Y = Y; m->cap = cap;
+    return EFVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+efvm_status efvm_mlp_create(efvm_mlp** out) {
+    if (!out) return EFVM_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return EFVM_ERR_NO_DEVICE;
+    efvm_mlp* m = new (std::nothrow) efvm_mlp();
+    if (!m) return EFVM_ERR_ARG;
+    const size_t pb = sizeof(float) * mlp::NPARAM;
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess || hipMalloc((void**)&m->p, pb) != hipSuccess ||
+        hipMalloc((void**)&m->g, pb) != hipSuccess || hipMalloc((void**)&m->m, pb) != hipSuccess ||
+        hipMalloc((void**)&m->v, pb) != hipSuccess || hipMalloc((void**)&m->packed, sizeof(mlp::Packed)) != hipSuccess) {
+        efvm_mlp_destroy(m);
+        return EFVM_ERR_HIP;
+    }
+    if (hipMemsetAsync(m->p, 0, pb, m->stream) != hipSuccess || hipMemsetAsync(m->m, 0, pb, m->stream) != hipSuccess ||
+        hipMemsetAsync(m->v, 0, pb, m->stream) != hipSuccess || hipStreamSynchronize(m->stream) != hipSuccess) {
+        efvm_mlp_destroy(m);
+        return EFVM_ERR_HIP;
+    }
+    *out = m;
+    return EFVM_OK;
+}
+
+void efvm_mlp_destroy(efvm_mlp* m) {
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (void* q : {(void*)m->p, (void*)m->g, (void*)m->m, (void*)m->v, (void*)m->X, (void*)m->Y, (void*)m->partial,
+                    (void*)m->loss_partial, (void*)m->losses, (void*)m->packed})
+        if (q) (void)hipFree(q);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+const char* efvm_mlp_last_error(const efvm_mlp* m) { return m ? m->err.c_str() : "null network"; }
+
+efvm_status efvm_mlp_load(efvm_mlp* m, const float* params, size_t n) {
+    if (!m || !params) return EFVM_ERR_ARG;
+    if (n != (size_t)mlp::NPARAM) return mfail(m, EFVM_ERR_ARG, "parameter count must be " + std::to_string(mlp::NPARAM));
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(params[i])) return mfail(m, EFVM_ERR_ARG, "non-finite parameter");
+    MHIPCHK(m, hipMemcpyAsync(m->p, params, sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+    MHIPCHK(m, hipMemsetAsync(m->m, 0, sizeof(float) * n, m->stream));
+    MHIPCHK(m, hipMemsetAsync(m->v, 0, sizeof(float) * n, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    m->adam_t = 0;
+    m->packed_valid = 0;
+    return mlp_repack(m);
+}
+
+efvm_status efvm_mlp_save(efvm_mlp* m, float* params, size_t n) {
+    if (!m || !params) return EFVM_ERR_ARG;
+    if (n != (size_t)mlp::NPARAM) return mfail(m, EFVM_ERR_ARG, "parameter count must be " + std::to_string(mlp::NPARAM));
+    MHIPCHK(m, hipMemcpyAsync(params, m->p, sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm_mlp_clear_samples(efvm_mlp* m) {
+    if (!m) return EFVM_ERR_ARG;
+    m->n_data = 0;
+    return EFVM_OK;
+}
+
+efvm_status efvm_mlp_num_samples(efvm_mlp* m, size_t* n) {
+    if (!m || !n) return EFVM_ERR_ARG;
+    *n = (size_t)m->n_data;
+    return EFVM_OK;
+}
+
+efvm_status efvm_mlp_add_samples(efvm_mlp* m, const float* X, const float* Y, size_t n) {
+    if (!m || (n && (!X || !Y))) return EFVM_ERR_ARG;
+    efvm_status st = mlp_grow(m, m->n_data + (long)n);
+    if (st != EFVM_OK) return st;
+    MHIPCHK(m, hipMemcpyAsync(m->X + mlp::NIN * m->n_data, X, sizeof(float) * mlp::NIN * n, hipMemcpyHostToDevice, m->stream));
+    MHIPCHK(m, hipMemcpyAsync(m->Y + mlp::NOUT * m->n_data, Y, sizeof(float) * mlp::NOUT * n, hipMemcpyHostToDevice, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    m->n_data += (long)n;
+    return EFVM_OK;
+}
+
+efvm_status efvm_mlp_get_samples(efvm_mlp* m, size_t first, size_t n, float* X, float* Y) {
+    if (!m || !X || !Y) return EFVM_ERR_ARG;
+    if ((long)(first + n) > m->n_data) return mfail(m, EFVM_ERR_ARG, "sample range out of bounds");
+    MHIPCHK(m, hipMemcpyAsync(X, m->X + mlp::NIN * first, sizeof(float) * mlp::NIN * n, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipMemcpyAsync(Y, m->Y + mlp::NOUT * first, sizeof(float) * mlp::NOUT * n, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm_mlp_add_snapshot(efvm_mlp* m, efvm_ctx* fine, int factor) {
+    if (!m || !fine || factor < 1) return EFVM_ERR_ARG;
+    const int nx = fine->grid.nx, ny = fine->grid.ny;
+    if (!periodic_axis(fine, 0) || !periodic_axis(fine, 1))
+        return mfail(m, EFVM_ERR_UNSUPPORTED, "training snapshots need a periodic grid");
+    if (nx % factor || ny % factor || nx / factor < 5 || ny / factor < 5)
+        return mfail(m, EFVM_ERR_ARG, "grid must divide by the factor into at least 5x5 coarse cells");
+    if (fine->rk_stage != 0) return mfail(m, EFVM_ERR_ARG, "snapshot at a step boundary only");
+    const long ncx = nx / factor, ncy = ny / factor, nc = ncx * ncy;
+    const long n_out = 4 * nc;
+    efvm_status st = mlp_grow(m, m->n_data + n_out);
+    if (st != EFVM_OK) return st;
+    MHIPCHK(m, hipStreamSynchronize(fine->stream));
+    double* tmp = nullptr;
+    MHIPCHK(m, hipMalloc((void**)&tmp, sizeof(double) * 12 * nc));
+    mlp::SampleArgs a;
+    a.nxf = nx; a.nyf = ny; a.f = factor; a.nc_x = (int)ncx; a.nc_y = (int)ncy;
+    a.gamma = fine->gas.gamma;
+    for (int q = 0; q < 4; q++) a.U[q] = fine->P[fine->cur][q];
+    a.esz = (int)fine->esz;
+    a.cells = tmp;
+    a.faces = tmp + 4 * nc;
+    a.X = m->X + mlp::NIN * m->n_data;
+    a.Y = m->Y + mlp::NOUT * m->n_data;
+    hipError_t e = launch_mlp_samples(a, n_out, m->stream);
+    hipError_t e2 = hipStreamSynchronize(m->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return mfail(m, EFVM_ERR_HIP, std::string("sample kernels: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    m->n_data += n_out;
+    return EFVM_OK;
+}
+
+efvm_status efvm_mlp_gradient(efvm_mlp* m, const float* X, const float* Y, size_t n, float* grad, double* loss) {
+    if (!m || !X || !Y || !grad || n == 0) return EFVM_ERR_ARG;
+    float *dX = nullptr, *dY = nullptr;
+    MHIPCHK(m, hipMalloc((void**)&dX, sizeof(float) * mlp::NIN * n));
+    MHIPCHK(m, hipMalloc((void**)&dY, sizeof(float) * mlp::NOUT * n));
+    const long nblk = ((long)n + mlp::TB - 1) / mlp::TB;
+    efvm_status st = mlp_reserve_partials(m, nblk);
+    if (st == EFVM_OK && m->loss_cap < 1) {
+        if (hipMalloc((void**)&m->losses, sizeof(double) * 1024) == hipSuccess) m->loss_cap = 1024;
+        else st = mfail(m, EFVM_ERR_HIP, "hipMalloc losses");
+    }
+    if (st == EFVM_OK) {
+        hipError_t e = hipMemcpyAsync(dX, X, sizeof(float) * mlp::NIN * n, hipMemcpyHostToDevice, m->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dY, Y, sizeof(float) * mlp::NOUT * n, hipMemcpyHostToDevice, m->stream);
+        mlp::TrainArgs a;
+        a.params = m->p; a.X = dX; a.Y = dY; a.n_data = (long)n; a.n = (long)n; a.random = 0; a.seed = 0; a.step = 0;
+        a.scale = (float)(2.0 / ((double)n * mlp::NOUT));
+        a.partial = m->partial; a.loss_partial = m->loss_partial;
+        if (e == hipSuccess) e = launch_mlp_train(a, (int)nblk, m->g, m->losses, m->stream);
+        double l = 0.0;
+        if (e == hipSuccess) e = hipMemcpyAsync(grad, m->g, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost, m->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&l, m->losses, sizeof(double), hipMemcpyDeviceToHost, m->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+        if (e != hipSuccess) st = mfail(m, EFVM_ERR_HIP, std::string("training kernels: ") + hipGetErrorString(e));
+        if (loss) *loss = l;
+    }
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    return st;
+}
+
+efvm_status efvm_mlp_train_step(efvm_mlp* m, int n_steps, int batch, float lr0, float lr1, unsigned long long seed,
+                                double* losses) {
+    if (!m || n_steps < 0 || batch < 1) return EFVM_ERR_ARG;
+    if (m->n_data < 1) return mfail(m, EFVM_ERR_ARG, "no training samples");
+    const long nblk = ((long)batch + mlp::TB - 1) / mlp::TB;
+    efvm_status st = mlp_reserve_partials(m, nblk);
+    if (st != EFVM_OK) return st;
+    if (m->loss_cap < n_steps) {
+        if (m->losses) (void)hipFree(m->losses);
+        m->losses = nullptr; m->loss_cap = 0;
+        MHIPCHK(m, hipMalloc((void**)&m->losses, sizeof(double) * std::max(n_steps, 1024)));
+        m->loss_cap = std::max(n_steps, 1024);
+    }
+    for (int k = 0; k < n_steps; k++) {
+        // cosine schedule from lr0 to lr1 over this call's steps
+        const double f = n_steps > 1 ? (double)k / (double)(n_steps - 1) : 0.0;
+        const float lr = (float)(lr1 + 0.5 * (lr0 - lr1) * (1.0 + std::cos(M_PI * f)));
+        const long t = ++m->adam_t;
+        mlp::TrainArgs a;
+        a.params = m->p; a.X = m->X; a.Y = m->Y; a.n_data = m->n_data; a.n = batch; a.random = 1;
+        a.seed = seed; a.step = (unsigned long long)t;
+        a.scale = (float)(2.0 / ((double)batch * mlp::NOUT));
+        a.partial = m->partial; a.loss_partial = m->loss_partial;
+        MHIPCHK(m, launch_mlp_train(a, (int)nblk, m->g, m->losses + k, m->stream));
+        MHIPCHK(m, launch_mlp_adam(m->p, m->g, m->m, m->v, lr, (int)std::min<long>(t, 1 << 30), m->stream));
+    }
+    if (losses && n_steps)
+        MHIPCHK(m, hipMemcpyAsync(losses, m->losses, sizeof(double) * n_steps, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    m->packed_valid = 0;
+    return mlp_repack(m);
+}
+
+efvm_status efvm_mlp_infer(efvm_mlp* m, const float* X, size_t n, float* Y) {
+    if (!m || !X || !Y || n == 0) return EFVM_ERR_ARG;
+    efvm_status st = mlp_repack(m);
+    if (st != EFVM_OK) return st;
+    float *dX = nullptr, *dY = nullptr;
+    MHIPCHK(m, hipMalloc((void**)&dX, sizeof(float) * mlp::NIN * n));
+    MHIPCHK(m, hipMalloc((void**)&dY, sizeof(float) * mlp::NOUT * n));
+    hipError_t e = hipMemcpyAsync(dX, X, sizeof(float) * mlp::NIN * n, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = launch_mlp_infer(m->packed, dX, (long)n, dY, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(Y, dY, sizeof(float) * mlp::NOUT * n, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    if (e != hipSuccess) return mfail(m, EFVM_ERR_HIP, std::string("inference kernel: ") + hipGetErrorString(e));
+    return EFVM_OK;
+}
+
+efvm_status efvm_set_learned(efvm_ctx* c, efvm_mlp* m) {
+    if (!c) return EFVM_ERR_ARG;
+    drop_graph(c);
+    if (!m) { c->learned = 0; return EFVM_OK; }
+    if (c->esz != 4) return fail(c, EFVM_ERR_UNSUPPORTED, "learned reconstruction runs in FP32");
+    if (!periodic_axis(c, 0) || !periodic_axis(c, 1)) return fail(c, EFVM_ERR_UNSUPPORTED, "learned reconstruction needs a periodic grid");
+    if (c->sch.viscous || c->n_cyl > 0 || c->body_force || c->sponge || c->parabolic)
+        return fail(c, EFVM_ERR_UNSUPPORTED, "learned reconstruction covers the inviscid periodic path only");
+    if (c->grid.nx < 5 || c->grid.ny < 5) return fail(c, EFVM_ERR_CONFIG, "learned reconstruction needs at least 5x5 cells");
+    efvm_status st = mlp_repack(m);
+    if (st != EFVM_OK) return fail(c, st, m->err);
+    if (!c->faces) HIPCHK(c, hipMalloc((void**)&c->faces, sizeof(float) * 16 * c->ncell));
+    if (!c->wpack) HIPCHK(c, hipMalloc((void**)&c->wpack, sizeof(mlp::Packed)));
+    if (!c->lstats) {
+        HIPCHK(c, hipMalloc((void**)&c->lstats, 2 * sizeof(unsigned long long)));
+        HIPCHK(c, hipMemsetAsync(c->lstats, 0, 2 * sizeof(unsigned long long), c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->wpack, m->packed, sizeof(mlp::Packed), hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(c, hipStreamSynchronize(m->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->learned = 1;
+    return EFVM_OK;
+}
+
+efvm_status efvm_mlp_infer_stats(efvm_ctx* c, int reset, unsigned long long out[2]) {
+    if (!c || !out) return EFVM_ERR_ARG;
+    out[0] = out[1] = 0;
+    if (!c->lstats) return EFVM_OK;
+    HIPCHK(c, hipMemcpyAsync(out, c->lstats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    if (reset) HIPCHK(c, hipMemsetAsync(c->lstats, 0, 2 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm_learned_faces(efvm_ctx* c, float* out, size_t n) {
+    if (!c || !out) return EFVM_ERR_ARG;
+    if (!c->learned) return fail(c, EFVM_ERR_ARG, "no learned reconstruction attached");
+    if ((long)n != 16 * c->ncell) return fail(c, EFVM_ERR_ARG, "face buffer must hold 16 * nx * ny floats");
+    if (c->rk_stage != 0) return fail(c, EFVM_ERR_ARG, "face states at a step boundary only");
+    mlp::FaceArgs fa;
+    fa.nx = c->grid.nx; fa.ny = c->grid.ny;
+    fa.n_chunks = (c->grid.nx + 63) / 64;
+    fa.n_tasks = 2 * c->grid.ny * fa.n_chunks;
+    fa.gamma = (float)c->gas.gamma;
+    for (int q = 0; q < 4; q++) fa.U[q] = (const float*)c->P[c->cur][q];
+    fa.F = c->faces;
+    fa.w = c->wpack;
+    fa.stats = nullptr;
+    HIPCHK(c, launch_mlp_faces(fa, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, c->faces, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;
 }

@@ -65,6 +65,20 @@ __host__ __device__ __forceinline__ Prep prep(const C4<float>& q, float g, float
     return o;
 }
 
+// Face state from primitives (rho, u, v, p), as stored by the learned reconstruction.
+__host__ __device__ __forceinline__ Prep prep_prim(float r, float u, float v, float p, float g, float igm1) {
+    Prep o;
+    o.r = r; o.u = u; o.v = v; o.p = p;
+    o.ru = r * u;
+    o.rv = r * v;
+    o.re = p * igm1 + 0.5f * r * (u * u + v * v);
+    o.ir = rcp(r);
+    float x = g * p * o.ir;
+    o.ia = rsq(x);
+    o.a = x * o.ia;
+    return o;
+}
+
 template <int DIR, bool THORNBER>
 __host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R) {
     float ul = L.u, vl = L.v, ur = R.u, vr = R.v;

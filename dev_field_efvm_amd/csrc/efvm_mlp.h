@@ -1,0 +1,100 @@
+// efvm_mlp.h -- the learned face reconstruction: network layout, feature
+// definition and the device argument blocks shared by the inference, training
+// and sample-generation kernels (efvm_mlp.hip), the learned stage kernel
+// (efvm_stage_f32.hip) and the host (efvm_api.cpp).
+//
+// The network maps the 20 features of one face side (a 5-cell stencil of the
+// primitives rho, u_n, u_t, p, oriented toward the face) to the normalised face
+// offsets y (4 values):
+//     y = (x[+1] - x[-1]) / 4 + L x + bL + W3 gelu(W2 gelu(W1 x + b1) + b2) + b3,
+// face value (oriented) = w_centre + sigma * y, per variable.  The first term is
+// the unlimited central slope; the learned linear skip L and the MLP (20-64-64-4,
+// tanh-form GELU) supply everything else, limiting included.  Inference runs the
+// MLP on bf16 matrix cores with FP32 accumulation; the skip and the offsets stay
+// FP32, so the bf16 rounding only touches the nonlinear correction.
+#pragma once
+#include <stdint.h>
+
+namespace efvm {
+namespace mlp {
+
+constexpr int NIN = 20, NH = 64, NOUT = 4;
+// flat FP32 parameter vector (row-major matrices, [out][in])
+constexpr int OFF_W1 = 0;
+constexpr int OFF_B1 = OFF_W1 + NH * NIN;
+constexpr int OFF_W2 = OFF_B1 + NH;
+constexpr int OFF_B2 = OFF_W2 + NH * NH;
+constexpr int OFF_W3 = OFF_B2 + NH;
+constexpr int OFF_B3 = OFF_W3 + NOUT * NH;
+constexpr int OFF_L = OFF_B3 + NOUT;
+constexpr int OFF_BL = OFF_L + NOUT * NIN;
+constexpr int NPARAM = OFF_BL + NOUT;   // 5848
+constexpr int TB = 64;                  // samples per training block (one wave)
+
+// features: x[4k+v], k = 0..4 <-> stencil cells -2..2 (k = 3 is the neighbour across
+// the face), v = rho, u_n, u_t, p.  x = (w_k - w_0) / sigma_v for k != 2 and
+// x[8+v] = sigma_v / (sigma_v + X0C scale_v); sigma_v = max(max_k |w_k - w_0|, FLOOR scale_v),
+// scale = (rho_0, a_0, a_0, p_0).
+constexpr float FLOOR = 1.0e-4f;
+constexpr float X0C = 0.1f;
+constexpr float GELU_C = 0.7978845608028654f;   // sqrt(2/pi)
+constexpr float GELU_A = 0.044715f;
+
+// Inference weights in the lane order of the bf16 16x16x32 matrix-core operands.
+// Layer 1 takes K = 32 inputs: the 20 features, a constant 1 (feature 20, whose
+// weight column is b1) and zeros; b3 is added with the FP32 offsets.
+// Layer l computes Z^T (hidden x 16 face sides) = W (hidden x K) . X^T (K x 16):
+// the A operand of lane `ln`, element e is W[row 16t + ln%16][k = 8(ln/16) + e].
+// Between layers the K index is permuted so that the accumulator a lane holds is
+// exactly the B operand element it needs: hidden unit
+//     h(s, g, e) = 16 (2 s + e / 4) + 4 g + e % 4      (K step s, lane group g)
+// Layer 3 replicates the 4 output rows over the 4 row groups, so every lane
+// group receives the outputs of its own 16 face sides.
+struct Packed {
+    uint32_t w1[4][64][4];       // [M tile][lane][8 bf16]
+    uint32_t w2[4][2][64][4];    // [M tile][K step][lane][8 bf16]
+    uint32_t w3[2][64][4];       // [K step][lane][8 bf16]
+    float b1[NH], b2[NH], b3[NOUT];
+    float L[NOUT][NIN], bL[NOUT];
+};
+
+// one launch of the face-state kernel
+struct FaceArgs {
+    int nx, ny;
+    int n_chunks;                // ceil(nx / 64)
+    int n_tasks;                 // 2 * ny * n_chunks (x faces, then y faces)
+    float gamma;
+    const float* U[4];           // conserved planes of the stage input
+    float* F;                    // 16 planes: ((axis * 2 + side) * 4 + var) * nx*ny, primitives
+    const Packed* w;
+    unsigned long long* stats;   // [faces that fell back to MUSCL, faces evaluated], accumulated
+};
+
+// one training minibatch (loss and gradient)
+struct TrainArgs {
+    const float* params;          // NPARAM
+    const float* X;               // samples x 20
+    const float* Y;               // samples x 4
+    long n_data;                  // samples available
+    long n;                       // samples in this batch
+    int random;                   // 1: batch drawn uniformly from the data (counter hash of seed, step, slot)
+    unsigned long long seed, step;
+    float scale;                  // 2 / (n * NOUT): d loss / d y
+    float* partial;               // blocks x NPARAM
+    double* loss_partial;         // blocks
+};
+
+// training samples from a fine periodic state box-filtered by f
+struct SampleArgs {
+    int nxf, nyf, f, nc_x, nc_y;  // fine size, factor, coarse size
+    double gamma;
+    const void* U[4];             // fine conserved planes
+    int esz;                      // 4 or 8
+    double* cells;                // 4 planes of coarse cells
+    double* faces;                // 2 axes x 4 planes: east / north face line averages
+    float* X;                     // output rows (samples x 20)
+    float* Y;                     // output rows (samples x 4)
+};
+
+}  // namespace mlp
+}  // namespace efvm

@@ -1,0 +1,688 @@
+// efvm_mlp.hip -- learned face reconstruction on gfx950.
+//
+//  * faces_kernel: for every face of a periodic grid, the two face sides' stencil
+//    features, the MLP on bf16 16x16x32 matrix cores (FP32 accumulation), the
+//    FP32 skip and offsets, the per-face positivity / TVD check with fallback to
+//    the solver's MUSCL (conserved variables, MC limiter), and the final face
+//    primitives for the learned stage kernel.
+//  * train_kernel / reduce_kernel / adam_kernel: FP32 minibatch loss, gradient
+//    (per-block partials reduced in a fixed order, so gradients are
+//    deterministic) and the Adam update.
+//  * coarse / face-target / sample kernels: training samples from a fine solver
+//    state, box-filtered on the device (the data never leaves HBM).
+//  * infer_kernel: the matrix-core MLP on given features (for checking).
+//
+// The reference has no learned part; the layout and formulas are in efvm_mlp.h.
+#include "efvm_stage.h"
+#include "efvm_fast.h"
+#include "efvm_mlp.h"
+
+namespace efvm {
+namespace mlp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
+// tanh-form GELU: x * sigmoid(2 c (x + a x^3)) = x / (1 + 2^(x (K0 + K1 x^2)))
+__device__ __forceinline__ float gelu_f(float x) {
+    constexpr float K0 = -2.f * GELU_C * 1.4426950408889634f;
+    constexpr float K1 = K0 * GELU_A;
+    const float e = __builtin_amdgcn_exp2f(x * fmaf(K1, x * x, K0));
+    return x * fast::rcp(1.f + e);
+}
+
+// Per-lane matrix-core operands of the three layers (see Packed).
+struct Regs {
+    bf16x8 a1[4], a2[4][2], a3[2];
+    f32x4 c2[4];                 // layer-2 bias (layer 1's rides on the constant feature 20)
+};
+
+__device__ __forceinline__ void load_regs(const Packed* __restrict__ P, int lane, Regs& R) {
+    const int g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        R.a1[t] = __builtin_bit_cast(bf16x8, *(const u32x4*)P->w1[t][lane]);
+#pragma unroll
+        for (int s = 0; s < 2; s++) R.a2[t][s] = __builtin_bit_cast(bf16x8, *(const u32x4*)P->w2[t][s][lane]);
+        R.c2[t] = f32x4{P->b2[16 * t + 4 * g], P->b2[16 * t + 4 * g + 1], P->b2[16 * t + 4 * g + 2], P->b2[16 * t + 4 * g + 3]};
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++) R.a3[s] = __builtin_bit_cast(bf16x8, *(const u32x4*)P->w3[s][lane]);
+}
+
+// gelu of two accumulator tiles -> the bf16 B operand of the next layer
+__device__ __forceinline__ bf16x8 act_pack(const f32x4& lo, const f32x4& hi) {
+    const u32x4 v = {pack2(gelu_f(lo[0]), gelu_f(lo[1])), pack2(gelu_f(lo[2]), gelu_f(lo[3])),
+                     pack2(gelu_f(hi[0]), gelu_f(hi[1])), pack2(gelu_f(hi[2]), gelu_f(hi[3]))};
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+// MLP of 16 face sides: bx = the lane's 8 input features (layer-1 B operand);
+// returns the layer-3 accumulator (without b3): element r of lane (g, j) = output r of side j.
+__device__ __forceinline__ f32x4 mlp16(const Regs& R, bf16x8 bx) {
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    f32x4 d1[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) d1[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a1[t], bx, zero, 0, 0, 0);
+    const bf16x8 h0 = act_pack(d1[0], d1[1]), h1 = act_pack(d1[2], d1[3]);
+    f32x4 d2[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        d2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a2[t][0], h0, R.c2[t], 0, 0, 0);
+        d2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a2[t][1], h1, d2[t], 0, 0, 0);
+    }
+    const bf16x8 k0 = act_pack(d2[0], d2[1]), k1 = act_pack(d2[2], d2[3]);
+    f32x4 d3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a3[0], k0, zero, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a3[1], k1, d3, 0, 0, 0);
+}
+
+constexpr int SLOT = 20;
+// FP32 skip weights staged in LDS (read as broadcasts, so they occupy no registers)
+constexpr int SK_L = 0, SK_BL = NOUT * NIN, SK_B3 = SK_BL + NOUT, SK_N = SK_B3 + NOUT;
+
+__device__ __forceinline__ void stage_skip(const Packed* __restrict__ P, float* skip) {
+    for (int i = threadIdx.x; i < SK_N; i += blockDim.x)
+        skip[i] = i < SK_BL ? P->L[i / NIN][i % NIN] : (i < SK_B3 ? P->bL[i - SK_BL] : P->b3[i - SK_B3]);
+    __syncthreads();
+}   // dwords per face-side slot in LDS (16 used; 20 keeps b128 reads conflict-free)
+
+// Features of one oriented stencil ws[k][v] (k = 0..4 <-> cells -2..2).  Writes the 20
+// features as bf16 into the slot (dwords 0..9, plus the constant feature 20 = 1 in dword
+// 10), returns sigma, the centre, the bound tolerance and the FP32 offset base + skip.
+__device__ __forceinline__ void featurize(const float ws[5][4], float gamma, const float* skip, uint32_t* slot,
+                                          float sig[4], float cen[4], float yb[4], float tol[4]) {
+    float x[20];
+    const float a0 = fast::fsqrt(gamma * ws[2][3] * fast::rcp(ws[2][0]));
+    const float scale[4] = {ws[2][0], a0, a0, ws[2][3]};
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const float c = ws[2][v];
+        float m = FLOOR * scale[v];
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            if (k != 2) m = fmaxf(m, fabsf(ws[k][v] - c));
+        const float is = fast::rcp(m);
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            if (k != 2) x[4 * k + v] = (ws[k][v] - c) * is;
+        x[8 + v] = m * fast::rcp(m + X0C * scale[v]);
+        sig[v] = m;
+        cen[v] = c;
+        tol[v] = FLOOR * scale[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        float y = 0.25f * (x[12 + v] - x[4 + v]) + skip[SK_BL + v] + skip[SK_B3 + v];
+#pragma unroll
+        for (int f = 0; f < 20; f++) y = fmaf(skip[SK_L + v * NIN + f], x[f], y);
+        yb[v] = y;
+    }
+    const u32x4 q0 = {pack2(x[0], x[1]), pack2(x[2], x[3]), pack2(x[4], x[5]), pack2(x[6], x[7])};
+    const u32x4 q1 = {pack2(x[8], x[9]), pack2(x[10], x[11]), pack2(x[12], x[13]), pack2(x[14], x[15])};
+    *(u32x4*)(slot) = q0;
+    *(u32x4*)(slot + 4) = q1;
+    slot[8] = pack2(x[16], x[17]);
+    slot[9] = pack2(x[18], x[19]);
+    slot[10] = pack2(1.f, 0.f);      // constant feature 20 carries the layer-1 bias
+}
+
+// Positivity and the TVD bound relaxed at smooth extrema: every variable of the oriented
+// face value within [min(w0, w1) - d, max(w0, w1) + d], d = |minmod(curvatures)| / 2 + tol
+// (tol = the feature floor, below which offsets are not resolved).
+__device__ __forceinline__ bool side_ok(const float ws[5][4], const float f[4], const float tol[4]) {
+    bool ok = f[0] > 0.f && f[3] > 0.f;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const float w0 = ws[2][v], w1 = ws[3][v];
+        const float c0 = w1 - 2.f * w0 + ws[1][v];
+        const float c1 = ws[4][v] - 2.f * w1 + w0;
+        const float d = (c0 * c1 > 0.f ? 0.5f * fminf(fabsf(c0), fabsf(c1)) : 0.f) + tol[v];
+        ok = ok && f[v] >= fminf(w0, w1) - d && f[v] <= fmaxf(w0, w1) + d;
+    }
+    return ok;
+}
+
+__device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Intra-wave LDS hand-off: the slots a wave writes are read only by its own lanes, and a
+// wave's LDS operations execute in order, so only compiler reordering must be fenced.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The 6 cells m = -2..3 straddling the lane's face along `axis`: conserved q, primitives w.
+__device__ __forceinline__ void load_cells(const FaceArgs& a, int axis, int row, int cc, float q[6][4], float w[6][4]) {
+    const float gm1 = a.gamma - 1.f;
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const long idx = axis == 0 ? (long)row * a.nx + wrap(cc + m - 2, a.nx) : (long)wrap(row + m - 2, a.ny) * a.nx + cc;
+#pragma unroll
+        for (int v = 0; v < 4; v++) q[m][v] = a.U[v][idx];
+        const float ir = fast::rcp(q[m][0]);
+        w[m][0] = q[m][0];
+        w[m][1] = q[m][1] * ir;
+        w[m][2] = q[m][2] * ir;
+        w[m][3] = gm1 * (q[m][3] - 0.5f * (q[m][1] * w[m][1] + q[m][2] * w[m][2]));
+    }
+}
+
+// oriented stencil of side sd: 0 = the cell m=0 toward +, 1 = the cell m=1 toward -
+__device__ __forceinline__ void orient(const float w[6][4], int axis, int sd, float ws[5][4]) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const int m = sd == 0 ? k : 5 - k;   // k = 0..4 <-> cells -2..2 of the side's owner
+        const float un = axis == 0 ? w[m][1] : w[m][2];
+        const float ut = axis == 0 ? w[m][2] : w[m][1];
+        ws[k][0] = w[m][0];
+        ws[k][1] = sd == 0 ? un : -un;
+        ws[k][2] = ut;
+        ws[k][3] = w[m][3];
+    }
+}
+
+// One wave = one task = 64 consecutive faces of one row (x faces, east of cells
+// chunk*64 + lane) or of one face row (y faces, north of those cells).  Each lane
+// loads the 6 cells straddling its face, builds both sides' features (left side =
+// its cell oriented +, right side = the neighbour oriented -), and the wave runs the
+// MLP over its 128 face sides in 8 matrix-core blocks of 16.  The stencil is read
+// again for the checks after the matrix-core phase, which keeps it out of the
+// registers the network needs.
+__global__ __launch_bounds__(256) void faces_kernel(FaceArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t lds[4][128][SLOT];
+    __shared__ float skip[SK_N];
+    stage_skip(a.w, skip);
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int g = lane >> 4, jl = lane & 15;
+    uint32_t(*feat)[SLOT] = lds[wv];
+    for (int e = lane; e < 128; e += 64)
+        for (int d = 10; d < SLOT; d++) feat[e][d] = 0u;
+    Regs R;
+    load_regs(a.w, lane, R);
+    const float gm = a.gamma, gm1 = a.gamma - 1.f;
+    const long np = (long)a.nx * a.ny;
+    unsigned long long nfb = 0, ntot = 0;
+    const int half = a.ny * a.n_chunks;
+    for (int task = blockIdx.x * 4 + wv; task < a.n_tasks; task += gridDim.x * 4) {
+        const int axis = task >= half ? 1 : 0;
+        const int rem = task - axis * half;
+        const int row = rem / a.n_chunks, chunk = rem - row * a.n_chunks;
+        const int ci = chunk * 64 + lane;
+        const bool valid = ci < a.nx;
+        const int cc = valid ? ci : a.nx - 1;
+        float sig[2][4], cen[2][4], y[2][4], tol[2][4];
+        {
+            float q[6][4], w[6][4], ws[5][4];
+            load_cells(a, axis, row, cc, q, w);
+            orient(w, axis, 0, ws);
+            featurize(ws, gm, skip, feat[lane], sig[0], cen[0], y[0], tol[0]);
+            orient(w, axis, 1, ws);
+            featurize(ws, gm, skip, feat[64 + lane], sig[1], cen[1], y[1], tol[1]);
+        }
+        wave_sync();
+        f32x4 o0 = {0.f, 0.f, 0.f, 0.f}, o1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int b = 0; b < 8; b++) {
+            const int sd = b >> 2, qd = b & 3;
+            const bf16x8 bx = __builtin_bit_cast(bf16x8, *(const u32x4*)&feat[sd * 64 + qd * 16 + jl][4 * g]);
+            const f32x4 d3 = mlp16(R, bx);
+            if (g == qd) {
+                if (sd == 0) o0 = d3; else o1 = d3;
+            }
+        }
+        wave_sync();
+        // face values (oriented), checks, fallback
+        float fo[2][4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            fo[0][v] = fmaf(sig[0][v], y[0][v] + o0[v], cen[0][v]);
+            fo[1][v] = fmaf(sig[1][v], y[1][v] + o1[v], cen[1][v]);
+        }
+        float q[6][4], w[6][4];
+        load_cells(a, axis, row, cc, q, w);
+        bool ok;
+        {
+            float ws[5][4];
+            orient(w, axis, 0, ws);
+            ok = side_ok(ws, fo[0], tol[0]);
+            orient(w, axis, 1, ws);
+            ok = side_ok(ws, fo[1], tol[1]) && ok;
+        }
+        float L[4], Rr[4];   // lab-frame primitives of the left / right face states
+        {
+            const float unl = fo[0][1], unr = -fo[1][1];
+            L[0] = fo[0][0]; L[3] = fo[0][3];
+            Rr[0] = fo[1][0]; Rr[3] = fo[1][3];
+            L[1] = axis == 0 ? unl : fo[0][2];
+            L[2] = axis == 0 ? fo[0][2] : unl;
+            Rr[1] = axis == 0 ? unr : fo[1][2];
+            Rr[2] = axis == 0 ? fo[1][2] : unr;
+        }
+        if (!ok) {
+            // the solver's MUSCL: MC-limited conserved slopes, cells m = -1..2 (indices 1..4)
+            float ql[4], qr[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const float sl = fast::mc(q[3][v] - q[2][v], q[2][v] - q[1][v]);
+                const float sr = fast::mc(q[4][v] - q[3][v], q[3][v] - q[2][v]);
+                ql[v] = fmaf(0.5f, sl, q[2][v]);
+                qr[v] = fmaf(-0.5f, sr, q[3][v]);
+            }
+            const float il = fast::rcp(ql[0]), irr = fast::rcp(qr[0]);
+            L[0] = ql[0]; L[1] = ql[1] * il; L[2] = ql[2] * il;
+            L[3] = gm1 * (ql[3] - 0.5f * (ql[1] * L[1] + ql[2] * L[2]));
+            Rr[0] = qr[0]; Rr[1] = qr[1] * irr; Rr[2] = qr[2] * irr;
+            Rr[3] = gm1 * (qr[3] - 0.5f * (qr[1] * Rr[1] + qr[2] * Rr[2]));
+        }
+        nfb += __popcll(__ballot(valid && !ok));
+        ntot += __popcll(__ballot(valid));
+        if (valid) {
+            const long cell = (long)row * a.nx + cc;
+            float* Fl = a.F + (long)(axis * 2) * 4 * np + cell;
+            float* Fr = Fl + 4 * np;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                Fl[v * np] = L[v];
+                Fr[v * np] = Rr[v];
+            }
+        }
+    }
+    if (a.stats && lane == 0 && ntot) {
+        if (nfb) atomicAdd(a.stats, nfb);
+        atomicAdd(a.stats + 1, ntot);
+    }
+}
+
+// The matrix-core MLP alone on n given feature rows X (n x 20): Y = base + skip + MLP.
+__global__ __launch_bounds__(256) void infer_kernel(const Packed* __restrict__ P, const float* __restrict__ X, long n,
+                                                    float* __restrict__ Y) {
+    __shared__ __attribute__((aligned(16))) uint32_t lds[4][64][SLOT];
+    __shared__ float skip[SK_N];
+    stage_skip(P, skip);
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int g = lane >> 4, jl = lane & 15;
+    uint32_t(*feat)[SLOT] = lds[wv];
+    for (int d = 10; d < SLOT; d++) feat[lane][d] = 0u;
+    Regs R;
+    load_regs(P, lane, R);
+    for (long base = (long)blockIdx.x * 256; base < n; base += (long)gridDim.x * 256) {
+        const long i = base + wv * 64 + lane;
+        const long ii = i < n ? i : n - 1;
+        float x[20];
+#pragma unroll
+        for (int f = 0; f < 20; f++) x[f] = X[ii * 20 + f];
+        float yb[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            float y = 0.25f * (x[12 + v] - x[4 + v]) + skip[SK_BL + v] + skip[SK_B3 + v];
+#pragma unroll
+            for (int f = 0; f < 20; f++) y = fmaf(skip[SK_L + v * NIN + f], x[f], y);
+            yb[v] = y;
+        }
+        uint32_t* slot = feat[lane];
+        *(u32x4*)(slot) = u32x4{pack2(x[0], x[1]), pack2(x[2], x[3]), pack2(x[4], x[5]), pack2(x[6], x[7])};
+        *(u32x4*)(slot + 4) = u32x4{pack2(x[8], x[9]), pack2(x[10], x[11]), pack2(x[12], x[13]), pack2(x[14], x[15])};
+        slot[8] = pack2(x[16], x[17]);
+        slot[9] = pack2(x[18], x[19]);
+        slot[10] = pack2(1.f, 0.f);
+        __syncthreads();
+        f32x4 o = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int qd = 0; qd < 4; qd++) {
+            const bf16x8 bx = __builtin_bit_cast(bf16x8, *(const u32x4*)&feat[qd * 16 + jl][4 * g]);
+            const f32x4 d3 = mlp16(R, bx);
+            if (g == qd) o = d3;
+        }
+        __syncthreads();
+        if (i < n) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) Y[i * 4 + v] = yb[v] + o[v];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// training (FP32).  One block = one wave = 64 samples, one thread per sample.
+// The weights and every per-sample vector live in LDS (rows of stride 65, so a
+// thread's own row hits a different bank from its neighbours'), the loops stay
+// rolled and the registers few.  Weight gradients are sums over the block's
+// samples, written as per-block partials and reduced in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int RS = NH + 1;
+
+__device__ __forceinline__ float gelu_d(float x) {
+    const float t = tanhf(GELU_C * (x + GELU_A * x * x * x));
+    return 0.5f * x * (1.f + t);
+}
+__device__ __forceinline__ float gelu_grad_d(float x) {
+    const float t = tanhf(GELU_C * (x + GELU_A * x * x * x));
+    return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * GELU_C * (1.f + 3.f * GELU_A * x * x);
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(TB) void train_kernel(TrainArgs a) {
+    __shared__ float sW1[NH][NIN];
+    __shared__ float sW2[NH][NH];
+    __shared__ float sW3[NOUT][NH];
+    __shared__ float sL[NOUT][NIN];
+    __shared__ float sB1[NH], sB2[NH], sB3[NOUT], sBL[NOUT];
+    __shared__ float sX[TB][NIN + 1];
+    __shared__ float sZ1[TB][RS];   // z1, then dz1
+    __shared__ float sH1[TB][RS];
+    __shared__ float sZ2[TB][RS];   // z2, then dz2
+    __shared__ float sH2[TB][RS];
+    __shared__ float sDy[TB][NOUT];
+    __shared__ double sLoss[TB];
+    const int t = threadIdx.x;
+    const float* P = a.params;
+    for (int i = t; i < NH * NIN; i += TB) sW1[i / NIN][i % NIN] = P[OFF_W1 + i];
+    for (int i = t; i < NH * NH; i += TB) sW2[i / NH][i % NH] = P[OFF_W2 + i];
+    for (int i = t; i < NOUT * NH; i += TB) sW3[i / NH][i % NH] = P[OFF_W3 + i];
+    for (int i = t; i < NOUT * NIN; i += TB) sL[i / NIN][i % NIN] = P[OFF_L + i];
+    for (int i = t; i < NH; i += TB) { sB1[i] = P[OFF_B1 + i]; sB2[i] = P[OFF_B2 + i]; }
+    if (t < NOUT) { sB3[t] = P[OFF_B3 + t]; sBL[t] = P[OFF_BL + t]; }
+    const long bi = (long)blockIdx.x * TB + t;
+    const bool live = bi < a.n;
+    long si = 0;
+    if (live)
+        si = a.random ? (long)(mix64(a.seed ^ mix64(a.step * 0x100000001B3ull + (unsigned long long)bi)) % (unsigned long long)a.n_data)
+                      : bi;
+    float x[NIN];
+#pragma unroll
+    for (int f = 0; f < NIN; f++) { x[f] = live ? a.X[si * NIN + f] : 0.f; sX[t][f] = x[f]; }
+    __syncthreads();
+    // forward
+    for (int o = 0; o < NH; o++) {
+        float z = sB1[o];
+#pragma unroll
+        for (int f = 0; f < NIN; f++) z = fmaf(sW1[o][f], x[f], z);
+        sZ1[t][o] = z;
+        sH1[t][o] = gelu_d(z);
+    }
+    float yo[NOUT];
+#pragma unroll
+    for (int v = 0; v < NOUT; v++) {
+        float y = 0.25f * (x[12 + v] - x[4 + v]) + sBL[v] + sB3[v];
+#pragma unroll
+        for (int f = 0; f < NIN; f++) y = fmaf(sL[v][f], x[f], y);
+        yo[v] = y;
+    }
+    for (int o = 0; o < NH; o++) {
+        float z = sB2[o];
+#pragma unroll 8
+        for (int j = 0; j < NH; j++) z = fmaf(sW2[o][j], sH1[t][j], z);
+        sZ2[t][o] = z;
+        const float h = gelu_d(z);
+        sH2[t][o] = h;
+#pragma unroll
+        for (int v = 0; v < NOUT; v++) yo[v] = fmaf(sW3[v][o], h, yo[v]);
+    }
+    float dy[NOUT];
+    double l2 = 0.0;
+#pragma unroll
+    for (int v = 0; v < NOUT; v++) {
+        const float r = live ? yo[v] - a.Y[si * NOUT + v] : 0.f;
+        l2 += (double)r * (double)r;
+        dy[v] = a.scale * r;
+        sDy[t][v] = dy[v];
+    }
+    sLoss[t] = l2;
+    // backward through layer 3 and 2 (per sample)
+    for (int j = 0; j < NH; j++) {
+        float d = 0.f;
+#pragma unroll
+        for (int v = 0; v < NOUT; v++) d = fmaf(sW3[v][j], dy[v], d);
+        sZ2[t][j] = d * gelu_grad_d(sZ2[t][j]);
+    }
+    for (int j = 0; j < NH; j++) {
+        float d = 0.f;
+#pragma unroll 8
+        for (int o = 0; o < NH; o++) d = fmaf(sW2[o][j], sZ2[t][o], d);
+        sZ1[t][j] = d * gelu_grad_d(sZ1[t][j]);
+    }
+    __syncthreads();
+    float* part = a.partial + (long)blockIdx.x * NPARAM;
+    // dW2 row t and db2[t]: sum over samples of dz2[s][t] * h1[s][:]
+    {
+        float acc[NH];
+#pragma unroll
+        for (int j = 0; j < NH; j++) acc[j] = 0.f;
+        float bsum = 0.f;
+        for (int s = 0; s < TB; s++) {
+            const float d = sZ2[s][t];
+            bsum += d;
+#pragma unroll
+            for (int j = 0; j < NH; j++) acc[j] = fmaf(d, sH1[s][j], acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < NH; j++) part[OFF_W2 + t * NH + j] = acc[j];
+        part[OFF_B2 + t] = bsum;
+    }
+    // dW1 row t and db1[t]
+    {
+        float acc[NIN];
+#pragma unroll
+        for (int f = 0; f < NIN; f++) acc[f] = 0.f;
+        float bsum = 0.f;
+        for (int s = 0; s < TB; s++) {
+            const float d = sZ1[s][t];
+            bsum += d;
+#pragma unroll
+            for (int f = 0; f < NIN; f++) acc[f] = fmaf(d, sX[s][f], acc[f]);
+        }
+#pragma unroll
+        for (int f = 0; f < NIN; f++) part[OFF_W1 + t * NIN + f] = acc[f];
+        part[OFF_B1 + t] = bsum;
+    }
+    // dW3[v][t], dL, db3 = dbL
+#pragma unroll
+    for (int v = 0; v < NOUT; v++) {
+        float acc = 0.f;
+        for (int s = 0; s < TB; s++) acc = fmaf(sDy[s][v], sH2[s][t], acc);
+        part[OFF_W3 + v * NH + t] = acc;
+    }
+    for (int i = t; i < NOUT * NIN + NOUT; i += TB) {
+        float acc = 0.f;
+        if (i < NOUT * NIN) {
+            const int v = i / NIN, f = i % NIN;
+            for (int s = 0; s < TB; s++) acc = fmaf(sDy[s][v], sX[s][f], acc);
+            part[OFF_L + i] = acc;
+        } else {
+            const int v = i - NOUT * NIN;
+            for (int s = 0; s < TB; s++) acc += sDy[s][v];
+            part[OFF_B3 + v] = acc;
+            part[OFF_BL + v] = acc;
+        }
+    }
+    if (t == 0) {
+        double l = 0.0;
+        for (int s = 0; s < TB; s++) l += sLoss[s];
+        a.loss_partial[blockIdx.x] = l;
+    }
+}
+
+// grad[p] = sum over blocks of partial[b][p] (fixed order); loss = sum of block losses / (n NOUT)
+__global__ void reduce_kernel(const float* __restrict__ partial, int nblk, float* __restrict__ grad,
+                              const double* __restrict__ loss_partial, double inv, double* loss_out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < NPARAM) {
+        float acc = 0.f;
+        for (int b = 0; b < nblk; b++) acc += partial[(long)b * NPARAM + p];
+        grad[p] = acc;
+    }
+    if (p == 0 && loss_out) {
+        double l = 0.0;
+        for (int b = 0; b < nblk; b++) l += loss_partial[b];
+        *loss_out = l * inv;
+    }
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1), step t >= 1
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                            float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPARAM) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// training samples from a fine state (FP64 arithmetic; fine planes FP32 or FP64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double ldf(const SampleArgs& a, int v, long i) {
+    return a.esz == 8 ? ((const double*)a.U[v])[i] : (double)((const float*)a.U[v])[i];
+}
+
+// box filter f x f -> coarse cell averages
+__global__ void coarse_kernel(SampleArgs a) {
+    const long nc = (long)a.nc_x * a.nc_y;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int I = (int)(c % a.nc_x), J = (int)(c / a.nc_x);
+    for (int v = 0; v < 4; v++) {
+        double s = 0.0;
+        for (int b = 0; b < a.f; b++)
+            for (int e = 0; e < a.f; e++) s += ldf(a, v, (long)(J * a.f + b) * a.nxf + (I * a.f + e));
+        a.cells[v * nc + c] = s / (double)(a.f * a.f);
+    }
+}
+
+// Line averages of the fine solution over the coarse east / north faces: per fine row
+// (column) the fourth-order face value (-u[m-1] + 7u[m] + 7u[m+1] - u[m+2]) / 12.
+__global__ void face_target_kernel(SampleArgs a) {
+    const long nc = (long)a.nc_x * a.nc_y;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= 2 * nc) return;
+    const int axis = c >= nc ? 1 : 0;
+    const long cc = c - axis * nc;
+    const int I = (int)(cc % a.nc_x), J = (int)(cc / a.nc_x);
+    for (int v = 0; v < 4; v++) {
+        double s = 0.0;
+        for (int b = 0; b < a.f; b++) {
+            long i[4];
+            for (int k = 0; k < 4; k++) {
+                if (axis == 0) {
+                    const int m = wrap(I * a.f + a.f - 2 + k, a.nxf);
+                    i[k] = (long)(J * a.f + b) * a.nxf + m;
+                } else {
+                    const int m = wrap(J * a.f + a.f - 2 + k, a.nyf);
+                    i[k] = (long)m * a.nxf + (I * a.f + b);
+                }
+            }
+            s += (-ldf(a, v, i[0]) + 7.0 * ldf(a, v, i[1]) + 7.0 * ldf(a, v, i[2]) - ldf(a, v, i[3])) / 12.0;
+        }
+        a.faces[(axis * 4 + v) * nc + cc] = s / (double)a.f;
+    }
+}
+
+__device__ __forceinline__ void prim_d(const double q[4], double gm1, double w[4]) {
+    w[0] = q[0];
+    w[1] = q[1] / q[0];
+    w[2] = q[2] / q[0];
+    w[3] = gm1 * (q[3] - 0.5 * (q[1] * w[1] + q[2] * w[2]));
+}
+
+// one sample per (axis, side, coarse cell), in that order
+__global__ void sample_kernel(SampleArgs a, long n_out) {
+    const long nc = (long)a.nc_x * a.nc_y;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n_out) return;
+    const int axis = (int)(id / (2 * nc));
+    const int sd = (int)((id / nc) & 1);
+    const long cell = id % nc;
+    const int I = (int)(cell % a.nc_x), J = (int)(cell / a.nc_x);
+    const int sgn = sd == 0 ? 1 : -1;
+    const double gm1 = a.gamma - 1.0;
+    double ws[5][4];
+    for (int k = 0; k < 5; k++) {
+        const int off = sgn * (k - 2);
+        const long ci = axis == 0 ? (long)J * a.nc_x + wrap(I + off, a.nc_x) : (long)wrap(J + off, a.nc_y) * a.nc_x + I;
+        double q[4], w[4];
+        for (int v = 0; v < 4; v++) q[v] = a.cells[v * nc + ci];
+        prim_d(q, gm1, w);
+        const double un = axis == 0 ? w[1] : w[2], ut = axis == 0 ? w[2] : w[1];
+        ws[k][0] = w[0]; ws[k][1] = sgn * un; ws[k][2] = ut; ws[k][3] = w[3];
+    }
+    // target: the face between the cell and its +sgn neighbour
+    long fi = cell;
+    if (sgn < 0) fi = axis == 0 ? (long)J * a.nc_x + wrap(I - 1, a.nc_x) : (long)wrap(J - 1, a.nc_y) * a.nc_x + I;
+    double qf[4], wf[4];
+    for (int v = 0; v < 4; v++) qf[v] = a.faces[(axis * 4 + v) * nc + fi];
+    prim_d(qf, gm1, wf);
+    const double tun = axis == 0 ? wf[1] : wf[2], tut = axis == 0 ? wf[2] : wf[1];
+    const double to[4] = {wf[0], sgn * tun, tut, wf[3]};
+    const double a0 = sqrt(a.gamma * ws[2][3] / ws[2][0]);
+    const double scale[4] = {ws[2][0], a0, a0, ws[2][3]};
+    float* X = a.X + id * NIN;
+    float* Y = a.Y + id * NOUT;
+    for (int v = 0; v < 4; v++) {
+        const double c = ws[2][v];
+        double m = (double)FLOOR * scale[v];
+        for (int k = 0; k < 5; k++)
+            if (k != 2) m = fmax(m, fabs(ws[k][v] - c));
+        for (int k = 0; k < 5; k++)
+            if (k != 2) X[4 * k + v] = (float)((ws[k][v] - c) / m);
+        X[8 + v] = (float)(m / (m + (double)X0C * scale[v]));
+        Y[v] = (float)fmin(4.0, fmax(-4.0, (to[v] - c) / m));
+    }
+}
+
+}  // namespace mlp
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st) {
+    const int blocks = std::min((a.n_tasks + 3) / 4, 2048);
+    mlp::faces_kernel<<<blocks, 256, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_infer(const mlp::Packed* P, const float* X, long n, float* Y, hipStream_t st) {
+    const long blocks = std::min<long>((n + 255) / 256, 4096);
+    mlp::infer_kernel<<<(int)blocks, 256, 0, st>>>(P, X, n, Y);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_train(const mlp::TrainArgs& a, int nblk, float* grad, double* loss_out, hipStream_t st) {
+    mlp::train_kernel<<<nblk, mlp::TB, 0, st>>>(a);
+    mlp::reduce_kernel<<<(mlp::NPARAM + 255) / 256, 256, 0, st>>>(a.partial, nblk, grad, a.loss_partial,
+                                                                  1.0 / ((double)a.n * mlp::NOUT), loss_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(float* p, const float* g, float* m, float* v, float lr, int t, hipStream_t st) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, t))), c2 = (float)(1.0 / (1.0 - std::pow(0.999, t)));
+    mlp::adam_kernel<<<(mlp::NPARAM + 255) / 256, 256, 0, st>>>(p, g, m, v, lr, b1, b2, 1e-8f, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_samples(const mlp::SampleArgs& a, long n_out, hipStream_t st) {
+    const long nc = (long)a.nc_x * a.nc_y;
+    mlp::coarse_kernel<<<(int)((nc + 255) / 256), 256, 0, st>>>(a);
+    mlp::face_target_kernel<<<(int)((2 * nc + 255) / 256), 256, 0, st>>>(a);
+    mlp::sample_kernel<<<(int)((n_out + 255) / 256), 256, 0, st>>>(a, n_out);
+    return hipGetLastError();
+}
+
+}  // namespace efvm

@@ -314,6 +314,90 @@ __global__ __launch_bounds__(256) void stage_f32_kernel(StageArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Stage from precomputed face states (learned reconstruction, periodic grids).
+// F holds 16 planes of primitives: x-face left/right, y-face left/right states of
+// the east/north face of every cell.  A wave owns 63 output columns (lane 0 is the
+// column left of the strip, whose east flux is the strip's first west flux) and
+// marches its rows carrying the north flux of the previous row.
+// ---------------------------------------------------------------------------
+template <int FLUX>
+__global__ __launch_bounds__(256) void stage_f32_faces_kernel(StageArgs a, const float* __restrict__ F) {
+    const int s = strip_index(a, blockDim.x / WAVE);
+    if (s >= a.n_strips_x * a.n_strips_y) return;
+    const float* U[4] = {(const float*)a.uin[0], (const float*)a.uin[1], (const float*)a.uin[2], (const float*)a.uin[3]};
+    const float* U0[4] = {(const float*)a.u0[0], (const float*)a.u0[1], (const float*)a.u0[2], (const float*)a.u0[3]};
+    float* O[4] = {(float*)a.uout[0], (float*)a.uout[1], (float*)a.uout[2], (float*)a.uout[3]};
+    const int lane = lane_id();
+    const int nx = a.nx, ny = a.ny;
+    const int stage = a.stage, parity = a.parity;
+    const int sxi = s % a.n_strips_x, syi = s / a.n_strips_x;
+    const int c = sxi * (WAVE - 1) - 1 + lane;
+    const int cq = c < 0 ? c + nx : (c < nx ? c : nx - 1);
+    const bool out_lane = lane >= 1 && c < nx;
+    const int j0 = syi * a.strip_h;
+    const int j1 = min(j0 + a.strip_h, ny);
+    const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0), igm1 = (float)(1.0 / (a.gamma - 1.0));
+    const double dtd = a.fixed_dt > 0.0 ? a.fixed_dt : a.cfl / a.clock->lam[parity];
+    const float dt = (float)dtd;
+    const bool fuse = a.fuse_cfl && (a.euler || stage == 2);
+    if (s == 0 && lane == 0 && stage == 0) { a.clock->dt = dtd; a.clock->lam[parity ^ 1] = 0.0; }
+    float ca, cb;
+    if (a.euler || stage == 0) { ca = 0.f; cb = 1.f; }
+    else if (stage == 1) { ca = 0.75f; cb = 0.25f; }
+    else { ca = 1.f / 3.f; cb = 2.f / 3.f; }
+    const float dtix = dt * (float)a.inv_x, dtiy = dt * (float)a.inv_y;
+    const float idx = (float)(1.0 / a.dx), idy = (float)(1.0 / a.dy), idz = (float)(1.0 / a.dz);
+    const long np = (long)nx * ny;
+    auto face = [&](int p0, long q) -> Prep {
+        const float* f = F + p0 * np + q;
+        return fast::prep_prim(f[0], f[np], f[2 * np], f[3 * np], g, igm1);
+    };
+    float lam_max = 0.f;
+    int bad = 0;
+    const int jm = j0 > 0 ? j0 - 1 : ny - 1;
+    F4 Gs = fast::flux<FLUX, 1>(face(8, (long)jm * nx + cq), face(12, (long)jm * nx + cq));
+    for (int j = j0; j < j1; j++) {
+        const long q = (long)j * nx + cq;
+        const F4 Fe = fast::flux<FLUX, 0>(face(0, q), face(4, q));
+        const F4 Fw = from_left4(Fe);
+        const F4 Gn = fast::flux<FLUX, 1>(face(8, q), face(12, q));
+        if (out_lane) {
+            float o0 = U[0][q] + (Fw.r - Fe.r) * dtix + (Gs.r - Gn.r) * dtiy;
+            float o1 = U[1][q] + (Fw.ru - Fe.ru) * dtix + (Gs.ru - Gn.ru) * dtiy;
+            float o2 = U[2][q] + (Fw.rv - Fe.rv) * dtix + (Gs.rv - Gn.rv) * dtiy;
+            float o3 = U[3][q] + (Fw.re - Fe.re) * dtix + (Gs.re - Gn.re) * dtiy;
+            if (ca != 0.f) {
+                o0 = ca * U0[0][q] + cb * o0; o1 = ca * U0[1][q] + cb * o1;
+                o2 = ca * U0[2][q] + cb * o2; o3 = ca * U0[3][q] + cb * o3;
+            }
+            O[0][q] = o0; O[1][q] = o1; O[2][q] = o2; O[3][q] = o3;
+            if (fuse) {
+                const float ir = fast::rcp(o0);
+                const float vu = fabsf(o1 * ir), vv = fabsf(o2 * ir);
+                const float p = gm1 * (o3 - 0.5f * (o1 * o1 + o2 * o2) * ir);
+                const float snd = fast::fsqrt(g * p * ir);
+                const float l = (vu + snd) * idx + (vv + snd) * idy + snd * idz;
+                if (o0 > 0.f && l == l && p > 0.f) lam_max = fmaxf(lam_max, l);
+                else bad++;
+            }
+        }
+        Gs = Gn;
+    }
+    if (fuse) {
+        const double w = wave_max((double)lam_max);
+        const long long b = wave_sum_ll((long long)bad);
+        if (lane == 0) {
+            if (w > 0.0) atomic_max_pos_double(&a.clock->lam[parity ^ 1], w);
+            if (b) atomicAdd((unsigned long long*)&a.clock->bad, (unsigned long long)b);
+        }
+    }
+    if (s == 0 && lane == 0 && (a.euler || stage == 2)) {
+        a.clock->time += dtd;
+        a.clock->step += 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Persistent batched stage: hundreds of independent cases of uneven size in one
 // launch.  A host-built work table lists (case, strip) pairs, largest strips
 // first; a fixed grid of waves strides through it.  Every case keeps its own
@@ -385,6 +469,17 @@ hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st) {
 #define EFVM_LAUNCH_ONE(F, L, X) stage_f32_kernel<F, L, X><<<grid, block, 0, st>>>(a)
     EFVM_F32_DISPATCH(EFVM_LAUNCH_ONE);
 #undef EFVM_LAUNCH_ONE
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_f32_faces(const StageArgs& a, const float* F, int block, hipStream_t st) {
+    const int waves = block / WAVE;
+    const dim3 grid((a.n_strips_x * a.n_strips_y + waves - 1) / waves);
+    switch (a.flux) {
+    case FLUX_RUSANOV: stage_f32_faces_kernel<FLUX_RUSANOV><<<grid, block, 0, st>>>(a, F); break;
+    case FLUX_HLLC: stage_f32_faces_kernel<FLUX_HLLC><<<grid, block, 0, st>>>(a, F); break;
+    default: stage_f32_faces_kernel<FLUX_HLLC_TEXTBOOK><<<grid, block, 0, st>>>(a, F); break;
+    }
     return hipGetLastError();
 }
 

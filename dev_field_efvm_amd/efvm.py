@@ -25,7 +25,7 @@ BC = {"outflow": 0, "periodic": 1, "slip": 2, "noslip": 3, "symmetry": 4,
 # Every entry point include/efvm.h declares (checked by the CPU test suite).
 EXPORTS = [
     "efvm_ctx_create", "efvm_ctx_destroy", "efvm_last_error", "efvm_set_launch", "efvm_set_boundaries",
-    "efvm_set_parabolic_inflow", "efvm_set_sponge", "efvm_set_body_force", "efvm_upload_state",
+    "efvm_set_parabolic_inflow", "efvm_set_sponge", "efvm_set_body_force", "efvm_set_time_step", "efvm_upload_state",
     "efvm_download_state", "efvm_get_clock", "efvm_set_clock", "efvm_cfl", "efvm_run", "efvm_step",
     "efvm_advance_to", "efvm_synchronize", "efvm_graph_capture", "efvm_graph_launch",
     "efvm_ibm_add_cylinder", "efvm_marker_force", "efvm_conservation_sums", "efvm_device_state",
@@ -77,6 +77,7 @@ def load_library(path: str = LIB_PATH):
         "efvm_set_parabolic_inflow": [vp, d, d, d],
         "efvm_set_sponge": [vp, d, d, d, d, d, d, d],
         "efvm_set_body_force": [vp, d],
+        "efvm_set_time_step": [vp, d, d],
         "efvm_upload_state": [vp, pd, C.c_size_t],
         "efvm_download_state": [vp, pd, C.c_size_t],
         "efvm_get_clock": [vp, pd, pd, C.POINTER(C.c_long), C.POINTER(C.c_int)],
@@ -172,6 +173,9 @@ class Context:
     def set_body_force(self, gx):
         self._chk(self._L.efvm_set_body_force(self._p, gx))
 
+    def set_time_step(self, cfl=0.4, fixed_dt=None):
+        self._chk(self._L.efvm_set_time_step(self._p, cfl, -1.0 if fixed_dt is None else float(fixed_dt)))
+
     def upload(self, cons: np.ndarray):
         a = np.ascontiguousarray(cons, dtype=np.float64).reshape(-1, 5)
         if a.shape[0] != self.ncell:
@@ -243,6 +247,25 @@ class Context:
 
     def snapshot(self, restore=False):
         self._chk(self._L.efvm_snapshot(self._p, 1 if restore else 0))
+
+    def set_learned(self, mlp):
+        """Use the network's current weights for every face state (None detaches)."""
+        _bind_mlp(self._L)
+        self._chk(self._L.efvm_set_learned(self._p, None if mlp is None else mlp._p))
+
+    def learned_stats(self, reset=False):
+        """(faces that fell back to MUSCL, faces evaluated)."""
+        _bind_mlp(self._L)
+        out = (C.c_ulonglong * 2)()
+        self._chk(self._L.efvm_mlp_infer_stats(self._p, 1 if reset else 0, out))
+        return int(out[0]), int(out[1])
+
+    def learned_faces(self):
+        """Face primitives of the current state: (2 axes, 2 sides, 4 vars, ny, nx)."""
+        _bind_mlp(self._L)
+        out = np.empty((2, 2, 4, self.ny, self.nx), dtype=np.float32)
+        self._chk(self._L.efvm_learned_faces(self._p, _pf(out), out.size))
+        return out
 
 
 class Case(C.Structure):
@@ -378,3 +401,132 @@ class Batch:
 
     def snapshot(self, restore=False):
         self._chk(self._L.efvm_batch_snapshot(self._p, 1 if restore else 0))
+
+
+# ---------------------------------------------------------------------------
+# learned face reconstruction
+# ---------------------------------------------------------------------------
+NPARAM = 5848
+
+
+def _bind_mlp(L):
+    if getattr(L, "_mlp_bound", False):
+        return
+    vp, pf, pd, sz = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_size_t
+    sig = {
+        "efvm_mlp_create": [C.POINTER(vp)],
+        "efvm_mlp_load": [vp, pf, sz],
+        "efvm_mlp_save": [vp, pf, sz],
+        "efvm_mlp_add_snapshot": [vp, vp, C.c_int],
+        "efvm_mlp_add_samples": [vp, pf, pf, sz],
+        "efvm_mlp_get_samples": [vp, sz, sz, pf, pf],
+        "efvm_mlp_num_samples": [vp, C.POINTER(sz)],
+        "efvm_mlp_clear_samples": [vp],
+        "efvm_mlp_train_step": [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_ulonglong, pd],
+        "efvm_mlp_gradient": [vp, pf, pf, sz, pf, pd],
+        "efvm_mlp_infer": [vp, pf, sz, pf],
+        "efvm_set_learned": [vp, vp],
+        "efvm_mlp_infer_stats": [vp, C.c_int, C.POINTER(C.c_ulonglong)],
+        "efvm_learned_faces": [vp, pf, sz],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.efvm_mlp_destroy.argtypes = [vp]
+    L.efvm_mlp_destroy.restype = None
+    L.efvm_mlp_last_error.argtypes = [vp]
+    L.efvm_mlp_last_error.restype = C.c_char_p
+    L._mlp_bound = True
+
+
+EXPORTS += [
+    "efvm_mlp_create", "efvm_mlp_destroy", "efvm_mlp_last_error", "efvm_mlp_load", "efvm_mlp_save",
+    "efvm_mlp_add_snapshot", "efvm_mlp_add_samples", "efvm_mlp_get_samples", "efvm_mlp_num_samples",
+    "efvm_mlp_clear_samples", "efvm_mlp_train_step", "efvm_mlp_gradient", "efvm_mlp_infer", "efvm_set_learned",
+    "efvm_mlp_infer_stats", "efvm_learned_faces",
+]
+
+
+def _pf(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The face-reconstruction network (device parameters, Adam state and training samples)."""
+
+    def __init__(self, params=None):
+        L = load_library()
+        _bind_mlp(L)
+        self._L = L
+        p = C.c_void_p()
+        st = L.efvm_mlp_create(C.byref(p))
+        if st != OK:
+            raise EfvmError(st, "efvm_mlp_create failed")
+        self._p = p
+        if params is not None:
+            self.load(params)
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._L.efvm_mlp_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st):
+        if st != OK:
+            raise EfvmError(st, self._L.efvm_mlp_last_error(self._p).decode())
+
+    def load(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        self._chk(self._L.efvm_mlp_load(self._p, _pf(a), a.size))
+
+    def save(self) -> np.ndarray:
+        a = np.empty(NPARAM, dtype=np.float32)
+        self._chk(self._L.efvm_mlp_save(self._p, _pf(a), a.size))
+        return a
+
+    def add_snapshot(self, fine: "Context", factor: int):
+        self._chk(self._L.efvm_mlp_add_snapshot(self._p, fine._p, int(factor)))
+
+    def add_samples(self, X, Y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        assert X.shape[1] == 20 and Y.shape[1] == 4 and len(X) == len(Y)
+        self._chk(self._L.efvm_mlp_add_samples(self._p, _pf(X), _pf(Y), len(X)))
+
+    def get_samples(self, first, n):
+        X = np.empty((n, 20), dtype=np.float32)
+        Y = np.empty((n, 4), dtype=np.float32)
+        self._chk(self._L.efvm_mlp_get_samples(self._p, first, n, _pf(X), _pf(Y)))
+        return X, Y
+
+    def num_samples(self) -> int:
+        n = C.c_size_t()
+        self._chk(self._L.efvm_mlp_num_samples(self._p, C.byref(n)))
+        return n.value
+
+    def clear_samples(self):
+        self._chk(self._L.efvm_mlp_clear_samples(self._p))
+
+    def train(self, steps, batch=8192, lr0=2e-3, lr1=2e-5, seed=1):
+        losses = np.zeros(max(steps, 1))
+        self._chk(self._L.efvm_mlp_train_step(self._p, int(steps), int(batch), lr0, lr1, int(seed), _pd(losses)))
+        return losses[:steps]
+
+    def gradient(self, X, Y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        g = np.empty(NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        self._chk(self._L.efvm_mlp_gradient(self._p, _pf(X), _pf(Y), len(X), _pf(g), C.byref(loss)))
+        return loss.value, g
+
+    def infer(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty((len(X), 4), dtype=np.float32)
+        self._chk(self._L.efvm_mlp_infer(self._p, _pf(X), len(X), _pf(Y)))
+        return Y
+

@@ -94,6 +94,9 @@ efvm_status efvm_set_boundaries(efvm_ctx* ctx, const efvm_bc sides[4]);
 efvm_status efvm_set_parabolic_inflow(efvm_ctx* ctx, double u_max, double height, double rho);
 efvm_status efvm_set_sponge(efvm_ctx* ctx, double x_start, double x_end, double sigma, double u_max, double height,
                             double rho, double p);
+/* SolverConfig {cfl, fixed_dt} after creation (crates/cfd_solver/src/lib.rs:39-58):
+ * fixed_dt > 0 steps with that dt, else dt = cfl / lambda_max.  At a step boundary. */
+efvm_status efvm_set_time_step(efvm_ctx* ctx, double cfl, double fixed_dt);
 /* Uniform x body force (crates/cfd_solver/tests/poiseuille_channel_validation.rs:105-120). */
 efvm_status efvm_set_body_force(efvm_ctx* ctx, double gx);
 
@@ -172,6 +175,60 @@ efvm_status efvm_batch_last_elapsed_ms(efvm_batch* b, double* ms);
  * Lets the autotuner time real steps and then put the simulation back. */
 efvm_status efvm_snapshot(efvm_ctx* ctx, int op);
 efvm_status efvm_batch_snapshot(efvm_batch* b, int op);
+
+/* ---------------------------------------------------------------------------
+ * Learned face reconstruction.  The reference has no learned part: these entry
+ * points extend its reconstruction choice (MusclPlugin::with_limiter,
+ * crates/cfd_solver/src/lib.rs:806-842) with a network that stands in for the
+ * limited MUSCL slope on coarse periodic grids.
+ *
+ * Per face side the network reads the 5-cell stencil of primitives (rho, u_n,
+ * u_t, p) oriented toward the face (20 features, normalised per variable) and
+ * returns 4 normalised face offsets:
+ *   y = (x[+1] - x[-1]) / 4 + L x + bL + W3 gelu(W2 gelu(W1 x + b1) + b2) + b3,
+ * 20 -> 64 -> 64 -> 4 with tanh-form GELU; inference runs the hidden layers on
+ * bf16 matrix cores with FP32 accumulation.  A face whose either side fails
+ * positivity or the curvature-relaxed TVD bound uses the solver's MUSCL (MC
+ * limiter) on both sides.  Parameters are one FP32 vector of
+ * EFVM_MLP_NPARAM values: W1[64][20], b1[64], W2[64][64], b2[64], W3[4][64],
+ * b3[4], L[4][20], bL[4] (row-major).
+ * ------------------------------------------------------------------------- */
+#define EFVM_MLP_NPARAM 5848
+typedef struct efvm_mlp efvm_mlp;
+
+efvm_status efvm_mlp_create(efvm_mlp** out);
+void efvm_mlp_destroy(efvm_mlp* m);
+const char* efvm_mlp_last_error(const efvm_mlp* m);
+/* parameters in / out (n = EFVM_MLP_NPARAM); load also resets the Adam state */
+efvm_status efvm_mlp_load(efvm_mlp* m, const float* params, size_t n);
+efvm_status efvm_mlp_save(efvm_mlp* m, float* params, size_t n);
+/* Training samples (rows of 20 features, 4 target offsets) held in device memory.
+ * add_snapshot box-filters a fine periodic context's current state by `factor`
+ * and appends 4 samples per coarse cell (x/y faces, both sides); the targets are
+ * the fine solution's line averages over the coarse faces. */
+efvm_status efvm_mlp_add_snapshot(efvm_mlp* m, efvm_ctx* fine, int factor);
+efvm_status efvm_mlp_add_samples(efvm_mlp* m, const float* X, const float* Y, size_t n);
+efvm_status efvm_mlp_get_samples(efvm_mlp* m, size_t first, size_t n, float* X, float* Y);
+efvm_status efvm_mlp_num_samples(efvm_mlp* m, size_t* n);
+efvm_status efvm_mlp_clear_samples(efvm_mlp* m);
+/* n_steps Adam steps on minibatches of `batch` samples drawn from the held
+ * samples (counter hash of seed and step), learning rate cosine from lr0 to lr1;
+ * losses (optional, n_steps doubles) receive each minibatch's mean squared error. */
+efvm_status efvm_mlp_train_step(efvm_mlp* m, int n_steps, int batch, float lr0, float lr1, unsigned long long seed,
+                                double* losses);
+/* Loss and gradient over n given samples with the training kernels (FP32). */
+efvm_status efvm_mlp_gradient(efvm_mlp* m, const float* X, const float* Y, size_t n, float* grad, double* loss);
+/* Matrix-core inference of y on n given feature rows. */
+efvm_status efvm_mlp_infer(efvm_mlp* m, const float* X, size_t n, float* Y);
+/* Attach a copy of the network's current weights to an FP32 periodic inviscid
+ * context (m = NULL detaches): every stage then builds its face states with it. */
+efvm_status efvm_set_learned(efvm_ctx* ctx, efvm_mlp* m);
+/* out[0] = faces that fell back to MUSCL, out[1] = faces evaluated (since reset) */
+efvm_status efvm_mlp_infer_stats(efvm_ctx* ctx, int reset, unsigned long long out[2]);
+/* Face primitives of the current state: 16 planes of nx*ny floats, plane
+ * (axis*2 + side)*4 + var, side 0 = left / lower, var = rho, u, v, p, for the
+ * east (axis 0) and north (axis 1) face of each cell. */
+efvm_status efvm_learned_faces(efvm_ctx* ctx, float* out, size_t n);
 
 #ifdef __cplusplus
 }
