@@ -44,8 +44,10 @@ hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st);
 hipError_t launch_mlp_infer(const mlp::Packed* P, const float* X, long n, float* Y, hipStream_t st);
 namespace mlp { struct TrainArgs; struct SampleArgs; }
 hipError_t launch_mlp_train(const mlp::TrainArgs& a, int nblk, float* grad, double* loss_out, hipStream_t st);
-hipError_t launch_mlp_adam(float* p, const float* g, float* m, float* v, float lr, int t, hipStream_t st);
+hipError_t launch_mlp_adam(float* p, const float* g, float* m, float* v, float lr, int t, int n_update, hipStream_t st);
 hipError_t launch_mlp_samples(const mlp::SampleArgs& a, long n_out, hipStream_t st);
+hipError_t launch_mlp_gram(const float* X, const float* Y, long n, long stride, double* partial, int blocks,
+                           hipStream_t st);
 }  // namespace efvm
 
 using namespace efvm;
@@ -1297,6 +1299,10 @@ efvm_status efvm_mlp_get_samples(efvm_mlp* m, size_t first, size_t n, float* X, 
 }
 
 efvm_status efvm_mlp_add_snapshot(efvm_mlp* m, efvm_ctx* fine, int factor) {
+    return efvm_mlp_add_snapshot_pair(m, fine, nullptr, factor);
+}
+
+efvm_status efvm_mlp_add_snapshot_pair(efvm_mlp* m, efvm_ctx* fine, efvm_ctx* coarse, int factor) {
     if (!m || !fine || factor < 1) return EFVM_ERR_ARG;
     const int nx = fine->grid.nx, ny = fine->grid.ny;
     if (!periodic_axis(fine, 0) || !periodic_axis(fine, 1))
@@ -1305,6 +1311,11 @@ efvm_status efvm_mlp_add_snapshot(efvm_mlp* m, efvm_ctx* fine, int factor) {
         return mfail(m, EFVM_ERR_ARG, "grid must divide by the factor into at least 5x5 coarse cells");
     if (fine->rk_stage != 0) return mfail(m, EFVM_ERR_ARG, "snapshot at a step boundary only");
     const long ncx = nx / factor, ncy = ny / factor, nc = ncx * ncy;
+    if (coarse) {
+        if (coarse->grid.nx != ncx || coarse->grid.ny != ncy || coarse->rk_stage != 0)
+            return mfail(m, EFVM_ERR_ARG, "coarse context must be the fine grid divided by the factor, at a step boundary");
+        MHIPCHK(m, hipStreamSynchronize(coarse->stream));
+    }
     const long n_out = 4 * nc;
     efvm_status st = mlp_grow(m, m->n_data + n_out);
     if (st != EFVM_OK) return st;
@@ -1316,6 +1327,8 @@ efvm_status efvm_mlp_add_snapshot(efvm_mlp* m, efvm_ctx* fine, int factor) {
     a.gamma = fine->gas.gamma;
     for (int q = 0; q < 4; q++) a.U[q] = fine->P[fine->cur][q];
     a.esz = (int)fine->esz;
+    for (int q = 0; q < 4; q++) a.Uc[q] = coarse ? coarse->P[coarse->cur][q] : nullptr;
+    a.esz_c = coarse ? (int)coarse->esz : 0;
     a.cells = tmp;
     a.faces = tmp + 4 * nc;
     a.X = m->X + mlp::NIN * m->n_data;
@@ -1361,7 +1374,7 @@ efvm_status efvm_mlp_gradient(efvm_mlp* m, const float* X, const float* Y, size_
 }
 
 efvm_status efvm_mlp_train_step(efvm_mlp* m, int n_steps, int batch, float lr0, float lr1, unsigned long long seed,
-                                double* losses) {
+                                int train_skip, double* losses) {
     if (!m || n_steps < 0 || batch < 1) return EFVM_ERR_ARG;
     if (m->n_data < 1) return mfail(m, EFVM_ERR_ARG, "no training samples");
     const long nblk = ((long)batch + mlp::TB - 1) / mlp::TB;
@@ -1384,10 +1397,83 @@ efvm_status efvm_mlp_train_step(efvm_mlp* m, int n_steps, int batch, float lr0, 
         a.scale = (float)(2.0 / ((double)batch * mlp::NOUT));
         a.partial = m->partial; a.loss_partial = m->loss_partial;
         MHIPCHK(m, launch_mlp_train(a, (int)nblk, m->g, m->losses + k, m->stream));
-        MHIPCHK(m, launch_mlp_adam(m->p, m->g, m->m, m->v, lr, (int)std::min<long>(t, 1 << 30), m->stream));
+        MHIPCHK(m, launch_mlp_adam(m->p, m->g, m->m, m->v, lr, (int)std::min<long>(t, 1 << 30),
+                                   train_skip ? mlp::NPARAM : mlp::OFF_L, m->stream));
     }
     if (losses && n_steps)
         MHIPCHK(m, hipMemcpyAsync(losses, m->losses, sizeof(double) * n_steps, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    m->packed_valid = 0;
+    return mlp_repack(m);
+}
+
+efvm_status efvm_mlp_fit_linear(efvm_mlp* m, size_t stride, double ridge, int constrained) {
+    if (!m || stride < 1 || ridge < 0.0 || constrained < 0 || constrained > 1) return EFVM_ERR_ARG;
+    if (m->n_data < 1) return mfail(m, EFVM_ERR_ARG, "no training samples");
+    constexpr int NA = mlp::NIN + 1, NG = mlp::NOUT * NA * (NA + 1);
+    const int blocks = 512;
+    double* part = nullptr;
+    MHIPCHK(m, hipMalloc((void**)&part, sizeof(double) * NG * blocks));
+    std::vector<double> h((size_t)NG * blocks);
+    hipError_t e = launch_mlp_gram(m->X, m->Y, m->n_data, (long)stride, part, blocks, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) return mfail(m, EFVM_ERR_HIP, std::string("normal-equation kernel: ") + hipGetErrorString(e));
+    std::vector<float> p(mlp::NPARAM);
+    MHIPCHK(m, hipMemcpyAsync(p.data(), m->p, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    for (int v = 0; v < mlp::NOUT; v++) {
+        // sum the block partials, then solve (G + ridge tr(G)/NA I) c = b by Gaussian elimination with pivoting
+        double A[NA][NA + 1];
+        for (int i = 0; i < NA; i++)
+            for (int j = 0; j <= NA; j++) {
+                double s = 0.0;
+                for (int b = 0; b < blocks; b++) s += h[(size_t)b * NG + (size_t)v * NA * (NA + 1) + i * (NA + 1) + j];
+                A[i][j] = s;
+            }
+        double tr = 0.0;
+        for (int i = 0; i < NA; i++) tr += A[i][i];
+        if (!(tr > 0.0)) return mfail(m, EFVM_ERR_NONPHYSICAL, "degenerate normal equations (no weighted samples)");
+        if (constrained) {
+            // Offsets exact for polynomial profiles up to degree 3 on the own variable: the
+            // coefficients of x[-2], x[-1], x[+1], x[+2] are the fifth-order face value's
+            // (2, -13, 27, -3) / 60, less the central base (0, -1/4, +1/4, 0), plus t (1, -4, -4, 1),
+            // the one direction that keeps exactness; every other coefficient is 0.  t is the
+            // weighted least-squares optimum along that line: t = d.(b - G c0) / (d.G d).
+            double c0[NA] = {}, d[NA] = {};
+            const int kk[4] = {0, 1, 3, 4};
+            const double lin5[4] = {2.0 / 60.0, -13.0 / 60.0 + 0.25, 27.0 / 60.0 - 0.25, -3.0 / 60.0};
+            const double dir[4] = {1.0, -4.0, -4.0, 1.0};
+            for (int q = 0; q < 4; q++) { c0[4 * kk[q] + v] = lin5[q]; d[4 * kk[q] + v] = dir[q]; }
+            double num = 0.0, den = 0.0;
+            for (int i = 0; i < NA; i++) {
+                double Gc = 0.0, Gd = 0.0;
+                for (int j = 0; j < NA; j++) { Gc += A[i][j] * c0[j]; Gd += A[i][j] * d[j]; }
+                num += d[i] * (A[i][NA] - Gc);
+                den += d[i] * Gd;
+            }
+            const double t = den > 0.0 ? num / (den + ridge * tr) : 0.0;
+            for (int i = 0; i < mlp::NIN; i++) p[mlp::OFF_L + v * mlp::NIN + i] = (float)(c0[i] + t * d[i]);
+            p[mlp::OFF_BL + v] = 0.f;
+            continue;
+        }
+        for (int i = 0; i < NA; i++) A[i][i] += ridge * tr / NA;
+        for (int c = 0; c < NA; c++) {
+            int piv = c;
+            for (int r = c + 1; r < NA; r++) if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+            if (std::fabs(A[piv][c]) < 1e-300) return mfail(m, EFVM_ERR_NONPHYSICAL, "singular normal equations");
+            if (piv != c) for (int j = 0; j <= NA; j++) std::swap(A[c][j], A[piv][j]);
+            for (int r = 0; r < NA; r++) {
+                if (r == c) continue;
+                const double f = A[r][c] / A[c][c];
+                for (int j = c; j <= NA; j++) A[r][j] -= f * A[c][j];
+            }
+        }
+        for (int i = 0; i < mlp::NIN; i++) p[mlp::OFF_L + v * mlp::NIN + i] = (float)(A[i][NA] / A[i][i]);
+        p[mlp::OFF_BL + v] = (float)(A[mlp::NIN][NA] / A[mlp::NIN][mlp::NIN]);
+    }
+    MHIPCHK(m, hipMemcpyAsync(m->p, p.data(), sizeof(float) * mlp::NPARAM, hipMemcpyHostToDevice, m->stream));
     MHIPCHK(m, hipStreamSynchronize(m->stream));
     m->packed_valid = 0;
     return mlp_repack(m);

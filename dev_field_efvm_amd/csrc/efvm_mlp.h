@@ -6,13 +6,16 @@
 // The network maps the 20 features of one face side (a 5-cell stencil of the
 // primitives rho, u_n, u_t, p, oriented toward the face) to the normalised face
 // offsets y (4 values):
-//     y = (x[+1] - x[-1]) / 4 + L x + bL + W3 gelu(W2 gelu(W1 x + b1) + b2) + b3,
-// face value (oriented) = w_centre + sigma * y, per variable.  The first term is
-// the unlimited central slope; the learned linear skip L and the MLP (20-64-64-4,
-// tanh-form GELU) supply everything else, limiting included.  Inference runs the
-// MLP on bf16 matrix cores with FP32 accumulation; the skip and the offsets stay
-// FP32, so the bf16 rounding only touches the nonlinear correction.
+//     y = (x[+1] - x[-1]) / 4 + phi * (L x + bL),
+//     phi = sigmoid(W3 gelu(W2 gelu(W1 x + b1) + b2) + b3)      (per variable),
+// face value (oriented) = w_centre + sigma * y.  The first term is the unlimited
+// central slope, L x + bL a learned high-order correction, and the MLP (20-64-64-4,
+// tanh-form GELU) a learned limiter that gates the correction between fully applied
+// (phi = 1) and off (phi = 0): it can limit, never amplify.  Inference runs the MLP
+// on bf16 matrix cores with FP32 accumulation; the correction, the gate and the
+// offsets stay FP32.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace efvm {
@@ -37,6 +40,19 @@ constexpr int TB = 64;                  // samples per training block (one wave)
 // scale = (rho_0, a_0, a_0, p_0).
 constexpr float FLOOR = 1.0e-4f;
 constexpr float X0C = 0.1f;
+
+// Training loss: mean over samples and variables of w (y - y*)^2 with the weight
+//     w_v = min((r_v / WREF)^2, WMAX),   r_v = sigma_v / scale_v = X0C x0 / (1 - x0)
+// recovered from the centre feature x0 = x[8+v].  Normalised offsets make every face
+// side count alike; weighting by sigma^2 turns the loss into the squared error of the
+// face values themselves (what the solution sees), capped so shocks do not dominate.
+constexpr float WREF = 0.01f;
+constexpr float WMAX = 100.f;
+__host__ __device__ inline float sample_weight(float x0) {
+    const float r = X0C * x0 / (1.f - x0);
+    const float w = (r / WREF) * (r / WREF);
+    return w < WMAX ? w : WMAX;
+}
 constexpr float GELU_C = 0.7978845608028654f;   // sqrt(2/pi)
 constexpr float GELU_A = 0.044715f;
 
@@ -90,6 +106,8 @@ struct SampleArgs {
     double gamma;
     const void* U[4];             // fine conserved planes
     int esz;                      // 4 or 8
+    const void* Uc[4];            // coarse solver's planes (the features' cells), or null: box filter
+    int esz_c;
     double* cells;                // 4 planes of coarse cells
     double* faces;                // 2 axes x 4 planes: east / north face line averages
     float* X;                     // output rows (samples x 20)

@@ -84,7 +84,7 @@ __device__ __forceinline__ f32x4 mlp16(const Regs& R, bf16x8 bx) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a3[1], k1, d3, 0, 0, 0);
 }
 
-constexpr int SLOT = 20;
+constexpr int SLOT = 20;   // dwords per face-side slot in LDS (16 used; 20 keeps b128 reads conflict-free)
 // FP32 skip weights staged in LDS (read as broadcasts, so they occupy no registers)
 constexpr int SK_L = 0, SK_BL = NOUT * NIN, SK_B3 = SK_BL + NOUT, SK_N = SK_B3 + NOUT;
 
@@ -92,13 +92,19 @@ __device__ __forceinline__ void stage_skip(const Packed* __restrict__ P, float* 
     for (int i = threadIdx.x; i < SK_N; i += blockDim.x)
         skip[i] = i < SK_BL ? P->L[i / NIN][i % NIN] : (i < SK_B3 ? P->bL[i - SK_BL] : P->b3[i - SK_B3]);
     __syncthreads();
-}   // dwords per face-side slot in LDS (16 used; 20 keeps b128 reads conflict-free)
+}
+
+// the gate of one output: sigmoid(z) = 1 / (1 + 2^(-z log2 e))
+__device__ __forceinline__ float gate(float z) {
+    return fast::rcp(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
 
 // Features of one oriented stencil ws[k][v] (k = 0..4 <-> cells -2..2).  Writes the 20
 // features as bf16 into the slot (dwords 0..9, plus the constant feature 20 = 1 in dword
-// 10), returns sigma, the centre, the bound tolerance and the FP32 offset base + skip.
+// 10), returns sigma, the centre, the bound tolerance, the FP32 central-slope base yb and
+// the FP32 learned correction sk = L x + bL (which the MLP's gate scales).
 __device__ __forceinline__ void featurize(const float ws[5][4], float gamma, const float* skip, uint32_t* slot,
-                                          float sig[4], float cen[4], float yb[4], float tol[4]) {
+                                          float sig[4], float cen[4], float yb[4], float sk[4], float tol[4]) {
     float x[20];
     const float a0 = fast::fsqrt(gamma * ws[2][3] * fast::rcp(ws[2][0]));
     const float scale[4] = {ws[2][0], a0, a0, ws[2][3]};
@@ -120,10 +126,11 @@ __device__ __forceinline__ void featurize(const float ws[5][4], float gamma, con
     }
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        float y = 0.25f * (x[12 + v] - x[4 + v]) + skip[SK_BL + v] + skip[SK_B3 + v];
+        float s = skip[SK_BL + v];
 #pragma unroll
-        for (int f = 0; f < 20; f++) y = fmaf(skip[SK_L + v * NIN + f], x[f], y);
-        yb[v] = y;
+        for (int f = 0; f < 20; f++) s = fmaf(skip[SK_L + v * NIN + f], x[f], s);
+        yb[v] = 0.25f * (x[12 + v] - x[4 + v]);
+        sk[v] = s;
     }
     const u32x4 q0 = {pack2(x[0], x[1]), pack2(x[2], x[3]), pack2(x[4], x[5]), pack2(x[6], x[7])};
     const u32x4 q1 = {pack2(x[8], x[9]), pack2(x[10], x[11]), pack2(x[12], x[13]), pack2(x[14], x[15])};
@@ -219,14 +226,14 @@ __global__ __launch_bounds__(256) void faces_kernel(FaceArgs a) {
         const int ci = chunk * 64 + lane;
         const bool valid = ci < a.nx;
         const int cc = valid ? ci : a.nx - 1;
-        float sig[2][4], cen[2][4], y[2][4], tol[2][4];
+        float sig[2][4], cen[2][4], y[2][4], sk[2][4], tol[2][4];
         {
             float q[6][4], w[6][4], ws[5][4];
             load_cells(a, axis, row, cc, q, w);
             orient(w, axis, 0, ws);
-            featurize(ws, gm, skip, feat[lane], sig[0], cen[0], y[0], tol[0]);
+            featurize(ws, gm, skip, feat[lane], sig[0], cen[0], y[0], sk[0], tol[0]);
             orient(w, axis, 1, ws);
-            featurize(ws, gm, skip, feat[64 + lane], sig[1], cen[1], y[1], tol[1]);
+            featurize(ws, gm, skip, feat[64 + lane], sig[1], cen[1], y[1], sk[1], tol[1]);
         }
         wave_sync();
         f32x4 o0 = {0.f, 0.f, 0.f, 0.f}, o1 = {0.f, 0.f, 0.f, 0.f};
@@ -244,8 +251,9 @@ __global__ __launch_bounds__(256) void faces_kernel(FaceArgs a) {
         float fo[2][4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            fo[0][v] = fmaf(sig[0][v], y[0][v] + o0[v], cen[0][v]);
-            fo[1][v] = fmaf(sig[1][v], y[1][v] + o1[v], cen[1][v]);
+            const float b3 = skip[SK_B3 + v];
+            fo[0][v] = fmaf(sig[0][v], fmaf(gate(o0[v] + b3), sk[0][v], y[0][v]), cen[0][v]);
+            fo[1][v] = fmaf(sig[1][v], fmaf(gate(o1[v] + b3), sk[1][v], y[1][v]), cen[1][v]);
         }
         float q[6][4], w[6][4];
         load_cells(a, axis, row, cc, q, w);
@@ -320,13 +328,14 @@ __global__ __launch_bounds__(256) void infer_kernel(const Packed* __restrict__ P
         float x[20];
 #pragma unroll
         for (int f = 0; f < 20; f++) x[f] = X[ii * 20 + f];
-        float yb[4];
+        float yb[4], sk[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            float y = 0.25f * (x[12 + v] - x[4 + v]) + skip[SK_BL + v] + skip[SK_B3 + v];
+            float s = skip[SK_BL + v];
 #pragma unroll
-            for (int f = 0; f < 20; f++) y = fmaf(skip[SK_L + v * NIN + f], x[f], y);
-            yb[v] = y;
+            for (int f = 0; f < 20; f++) s = fmaf(skip[SK_L + v * NIN + f], x[f], s);
+            yb[v] = 0.25f * (x[12 + v] - x[4 + v]);
+            sk[v] = s;
         }
         uint32_t* slot = feat[lane];
         *(u32x4*)(slot) = u32x4{pack2(x[0], x[1]), pack2(x[2], x[3]), pack2(x[4], x[5]), pack2(x[6], x[7])};
@@ -345,7 +354,7 @@ __global__ __launch_bounds__(256) void infer_kernel(const Packed* __restrict__ P
         __syncthreads();
         if (i < n) {
 #pragma unroll
-            for (int v = 0; v < 4; v++) Y[i * 4 + v] = yb[v] + o[v];
+            for (int v = 0; v < 4; v++) Y[i * 4 + v] = fmaf(gate(o[v] + skip[SK_B3 + v]), sk[v], yb[v]);
         }
     }
 }
@@ -386,7 +395,8 @@ __global__ __launch_bounds__(TB) void train_kernel(TrainArgs a) {
     __shared__ float sH1[TB][RS];
     __shared__ float sZ2[TB][RS];   // z2, then dz2
     __shared__ float sH2[TB][RS];
-    __shared__ float sDy[TB][NOUT];
+    __shared__ float sDz3[TB][NOUT];     // d loss / d (gate pre-activation)
+    __shared__ float sDs[TB][NOUT];      // d loss / d (learned correction) = dy * phi
     __shared__ double sLoss[TB];
     const int t = threadIdx.x;
     const float* P = a.params;
@@ -414,13 +424,14 @@ __global__ __launch_bounds__(TB) void train_kernel(TrainArgs a) {
         sZ1[t][o] = z;
         sH1[t][o] = gelu_d(z);
     }
-    float yo[NOUT];
+    float z3[NOUT], sk[NOUT];        // gate pre-activations, learned correction L x + bL
 #pragma unroll
     for (int v = 0; v < NOUT; v++) {
-        float y = 0.25f * (x[12 + v] - x[4 + v]) + sBL[v] + sB3[v];
+        float s = sBL[v];
 #pragma unroll
-        for (int f = 0; f < NIN; f++) y = fmaf(sL[v][f], x[f], y);
-        yo[v] = y;
+        for (int f = 0; f < NIN; f++) s = fmaf(sL[v][f], x[f], s);
+        sk[v] = s;
+        z3[v] = sB3[v];
     }
     for (int o = 0; o < NH; o++) {
         float z = sB2[o];
@@ -430,23 +441,28 @@ __global__ __launch_bounds__(TB) void train_kernel(TrainArgs a) {
         const float h = gelu_d(z);
         sH2[t][o] = h;
 #pragma unroll
-        for (int v = 0; v < NOUT; v++) yo[v] = fmaf(sW3[v][o], h, yo[v]);
+        for (int v = 0; v < NOUT; v++) z3[v] = fmaf(sW3[v][o], h, z3[v]);
     }
-    float dy[NOUT];
+    float dz3[NOUT];
     double l2 = 0.0;
 #pragma unroll
     for (int v = 0; v < NOUT; v++) {
-        const float r = live ? yo[v] - a.Y[si * NOUT + v] : 0.f;
-        l2 += (double)r * (double)r;
-        dy[v] = a.scale * r;
-        sDy[t][v] = dy[v];
+        const float phi = 1.f / (1.f + expf(-z3[v]));
+        const float y = 0.25f * (x[12 + v] - x[4 + v]) + phi * sk[v];
+        const float r = live ? y - a.Y[si * NOUT + v] : 0.f;
+        const float wv = sample_weight(x[8 + v]);
+        l2 += (double)wv * (double)r * (double)r;
+        const float dy = a.scale * wv * r;
+        dz3[v] = dy * sk[v] * phi * (1.f - phi);
+        sDz3[t][v] = dz3[v];
+        sDs[t][v] = dy * phi;
     }
     sLoss[t] = l2;
     // backward through layer 3 and 2 (per sample)
     for (int j = 0; j < NH; j++) {
         float d = 0.f;
 #pragma unroll
-        for (int v = 0; v < NOUT; v++) d = fmaf(sW3[v][j], dy[v], d);
+        for (int v = 0; v < NOUT; v++) d = fmaf(sW3[v][j], dz3[v], d);
         sZ2[t][j] = d * gelu_grad_d(sZ2[t][j]);
     }
     for (int j = 0; j < NH; j++) {
@@ -489,30 +505,90 @@ __global__ __launch_bounds__(TB) void train_kernel(TrainArgs a) {
         for (int f = 0; f < NIN; f++) part[OFF_W1 + t * NIN + f] = acc[f];
         part[OFF_B1 + t] = bsum;
     }
-    // dW3[v][t], dL, db3 = dbL
+    // dW3[v][t], db3 (through the gate); dL, dbL (through the gated correction)
 #pragma unroll
     for (int v = 0; v < NOUT; v++) {
         float acc = 0.f;
-        for (int s = 0; s < TB; s++) acc = fmaf(sDy[s][v], sH2[s][t], acc);
+        for (int s = 0; s < TB; s++) acc = fmaf(sDz3[s][v], sH2[s][t], acc);
         part[OFF_W3 + v * NH + t] = acc;
     }
-    for (int i = t; i < NOUT * NIN + NOUT; i += TB) {
+    for (int i = t; i < NOUT * NIN + 2 * NOUT; i += TB) {
         float acc = 0.f;
         if (i < NOUT * NIN) {
             const int v = i / NIN, f = i % NIN;
-            for (int s = 0; s < TB; s++) acc = fmaf(sDy[s][v], sX[s][f], acc);
+            for (int s = 0; s < TB; s++) acc = fmaf(sDs[s][v], sX[s][f], acc);
             part[OFF_L + i] = acc;
-        } else {
+        } else if (i < NOUT * NIN + NOUT) {
             const int v = i - NOUT * NIN;
-            for (int s = 0; s < TB; s++) acc += sDy[s][v];
-            part[OFF_B3 + v] = acc;
+            for (int s = 0; s < TB; s++) acc += sDs[s][v];
             part[OFF_BL + v] = acc;
+        } else {
+            const int v = i - NOUT * NIN - NOUT;
+            for (int s = 0; s < TB; s++) acc += sDz3[s][v];
+            part[OFF_B3 + v] = acc;
         }
     }
     if (t == 0) {
         double l = 0.0;
         for (int s = 0; s < TB; s++) l += sLoss[s];
         a.loss_partial[blockIdx.x] = l;
+    }
+}
+
+// Weighted normal equations of the linear skip, per variable v:
+//     G_v = sum_s w_sv a_s a_s^T,   b_v = sum_s w_sv a_s (y*_sv - base_sv),   a_s = (x_s, 1),
+// in FP64 over every `stride`-th sample.  Entry e of the packed [v][NA][NA + 1] array
+// (Gram columns, then the right-hand side) is owned by thread e % 256 of each block; a
+// block sums its tiles of 64 samples and writes one partial row.
+constexpr int NA = NIN + 1;
+constexpr int NGRAM = NOUT * NA * (NA + 1);
+constexpr int GRAM_PER_THREAD = (NGRAM + 255) / 256;
+
+__global__ __launch_bounds__(256) void gram_kernel(const float* __restrict__ X, const float* __restrict__ Y, long n,
+                                                   long stride, double* __restrict__ partial) {
+    __shared__ float sa[64][NA + 1];
+    __shared__ float sr[64][NOUT];
+    __shared__ float sw[64][NOUT];
+    double acc[GRAM_PER_THREAD];
+#pragma unroll
+    for (int k = 0; k < GRAM_PER_THREAD; k++) acc[k] = 0.0;
+    const long n_tiles = (n + 64 * stride - 1) / (64 * stride);
+    for (long tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        if (threadIdx.x < 64) {
+            const int i = threadIdx.x;
+            const long s = (tile * 64 + i) * stride;
+            const bool ok = s < n;
+            float x[NIN];
+#pragma unroll
+            for (int f = 0; f < NIN; f++) { x[f] = ok ? X[s * NIN + f] : 0.f; sa[i][f] = x[f]; }
+            sa[i][NIN] = 1.f;
+#pragma unroll
+            for (int v = 0; v < NOUT; v++) {
+                sr[i][v] = ok ? Y[s * NOUT + v] - 0.25f * (x[12 + v] - x[4 + v]) : 0.f;
+                sw[i][v] = ok ? sample_weight(x[8 + v]) : 0.f;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < GRAM_PER_THREAD; k++) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < NGRAM) {
+                const int v = e / (NA * (NA + 1)), rem = e % (NA * (NA + 1));
+                const int i = rem / (NA + 1), j = rem % (NA + 1);
+                double s = 0.0;
+                for (int q = 0; q < 64; q++) {
+                    const float other = j < NA ? sa[q][j] : sr[q][v];
+                    s += (double)(sw[q][v] * sa[q][i]) * (double)other;
+                }
+                acc[k] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < GRAM_PER_THREAD; k++) {
+        const int e = threadIdx.x + 256 * k;
+        if (e < NGRAM) partial[(long)blockIdx.x * NGRAM + e] = acc[k];
     }
 }
 
@@ -533,10 +609,11 @@ __global__ void reduce_kernel(const float* __restrict__ partial, int nblk, float
 }
 
 // Adam (Kingma & Ba 2015, Algorithm 1), step t >= 1
+// over the first n_update parameters (NPARAM, or OFF_L to keep the linear skip fixed)
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
-                            float lr, float b1, float b2, float eps, float c1, float c2) {
+                            float lr, float b1, float b2, float eps, float c1, float c2, int n_update) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= NPARAM) return;
+    if (i >= n_update) return;
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
@@ -552,12 +629,18 @@ __device__ __forceinline__ double ldf(const SampleArgs& a, int v, long i) {
     return a.esz == 8 ? ((const double*)a.U[v])[i] : (double)((const float*)a.U[v])[i];
 }
 
-// box filter f x f -> coarse cell averages
+// box filter f x f -> coarse cell averages, or the coarse solver's own cells when given
+// (features of the states the network meets inside the solver, targets from the fine run)
 __global__ void coarse_kernel(SampleArgs a) {
     const long nc = (long)a.nc_x * a.nc_y;
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const int I = (int)(c % a.nc_x), J = (int)(c / a.nc_x);
+    if (a.Uc[0]) {
+        for (int v = 0; v < 4; v++)
+            a.cells[v * nc + c] = a.esz_c == 8 ? ((const double*)a.Uc[v])[c] : (double)((const float*)a.Uc[v])[c];
+        return;
+    }
     for (int v = 0; v < 4; v++) {
         double s = 0.0;
         for (int b = 0; b < a.f; b++)
@@ -670,10 +753,16 @@ hipError_t launch_mlp_train(const mlp::TrainArgs& a, int nblk, float* grad, doub
     return hipGetLastError();
 }
 
-hipError_t launch_mlp_adam(float* p, const float* g, float* m, float* v, float lr, int t, hipStream_t st) {
+hipError_t launch_mlp_adam(float* p, const float* g, float* m, float* v, float lr, int t, int n_update, hipStream_t st) {
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, t))), c2 = (float)(1.0 / (1.0 - std::pow(0.999, t)));
-    mlp::adam_kernel<<<(mlp::NPARAM + 255) / 256, 256, 0, st>>>(p, g, m, v, lr, b1, b2, 1e-8f, c1, c2);
+    mlp::adam_kernel<<<(mlp::NPARAM + 255) / 256, 256, 0, st>>>(p, g, m, v, lr, b1, b2, 1e-8f, c1, c2, n_update);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_gram(const float* X, const float* Y, long n, long stride, double* partial, int blocks,
+                           hipStream_t st) {
+    mlp::gram_kernel<<<blocks, 256, 0, st>>>(X, Y, n, stride, partial);
     return hipGetLastError();
 }
 

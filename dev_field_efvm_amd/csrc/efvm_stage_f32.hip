@@ -234,10 +234,15 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
             if (VISC) { rru += dt * vin[0]; rrv += dt * vin[1]; rre += dt * vin[2]; }
             if (a.body_force) { rru += um.r * gxdt; rre += gxdt * um.ru; }
             if (IBM && a.solid[q]) { rr = 0.f; rru = 0.f; rrv = 0.f; rre = 0.f; }
-            float o0 = um.r + rr, o1 = um.ru + rru, o2 = um.rv + rrv, o3 = um.re + rre;
+            float o0, o1, o2, o3;
             if (ca != 0.f) {
-                o0 = ca * u0.r + cb * o0; o1 = ca * u0.ru + cb * o1;
-                o2 = ca * u0.rv + cb * o2; o3 = ca * u0.re + cb * o3;
+                // ca u0 + cb (u + dt L) written as u0 + cb ((u - u0) + dt L): the state keeps the
+                // coefficient 1 exactly (FP32 1/3 + 2/3 != 1 would scale it every step) and the
+                // rounding of cb touches only the increment
+                o0 = u0.r + cb * ((um.r - u0.r) + rr); o1 = u0.ru + cb * ((um.ru - u0.ru) + rru);
+                o2 = u0.rv + cb * ((um.rv - u0.rv) + rrv); o3 = u0.re + cb * ((um.re - u0.re) + rre);
+            } else {
+                o0 = um.r + rr; o1 = um.ru + rru; o2 = um.rv + rrv; o3 = um.re + rre;
             }
             O[0][q] = o0; O[1][q] = o1; O[2][q] = o2; O[3][q] = o3;
             if (fuse) {
@@ -362,13 +367,17 @@ __global__ __launch_bounds__(256) void stage_f32_faces_kernel(StageArgs a, const
         const F4 Fw = from_left4(Fe);
         const F4 Gn = fast::flux<FLUX, 1>(face(8, q), face(12, q));
         if (out_lane) {
-            float o0 = U[0][q] + (Fw.r - Fe.r) * dtix + (Gs.r - Gn.r) * dtiy;
-            float o1 = U[1][q] + (Fw.ru - Fe.ru) * dtix + (Gs.ru - Gn.ru) * dtiy;
-            float o2 = U[2][q] + (Fw.rv - Fe.rv) * dtix + (Gs.rv - Gn.rv) * dtiy;
-            float o3 = U[3][q] + (Fw.re - Fe.re) * dtix + (Gs.re - Gn.re) * dtiy;
-            if (ca != 0.f) {
-                o0 = ca * U0[0][q] + cb * o0; o1 = ca * U0[1][q] + cb * o1;
-                o2 = ca * U0[2][q] + cb * o2; o3 = ca * U0[3][q] + cb * o3;
+            const float d0 = (Fw.r - Fe.r) * dtix + (Gs.r - Gn.r) * dtiy;
+            const float d1 = (Fw.ru - Fe.ru) * dtix + (Gs.ru - Gn.ru) * dtiy;
+            const float d2 = (Fw.rv - Fe.rv) * dtix + (Gs.rv - Gn.rv) * dtiy;
+            const float d3 = (Fw.re - Fe.re) * dtix + (Gs.re - Gn.re) * dtiy;
+            float o0, o1, o2, o3;
+            if (ca != 0.f) {   // increment form of the RK combine, as in stage_f32_strip
+                const float a0 = U0[0][q], a1 = U0[1][q], a2 = U0[2][q], a3 = U0[3][q];
+                o0 = a0 + cb * ((U[0][q] - a0) + d0); o1 = a1 + cb * ((U[1][q] - a1) + d1);
+                o2 = a2 + cb * ((U[2][q] - a2) + d2); o3 = a3 + cb * ((U[3][q] - a3) + d3);
+            } else {
+                o0 = U[0][q] + d0; o1 = U[1][q] + d1; o2 = U[2][q] + d2; o3 = U[3][q] + d3;
             }
             O[0][q] = o0; O[1][q] = o1; O[2][q] = o2; O[3][q] = o3;
             if (fuse) {

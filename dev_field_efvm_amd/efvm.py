@@ -418,12 +418,14 @@ def _bind_mlp(L):
         "efvm_mlp_load": [vp, pf, sz],
         "efvm_mlp_save": [vp, pf, sz],
         "efvm_mlp_add_snapshot": [vp, vp, C.c_int],
+        "efvm_mlp_add_snapshot_pair": [vp, vp, vp, C.c_int],
         "efvm_mlp_add_samples": [vp, pf, pf, sz],
         "efvm_mlp_get_samples": [vp, sz, sz, pf, pf],
         "efvm_mlp_num_samples": [vp, C.POINTER(sz)],
         "efvm_mlp_clear_samples": [vp],
-        "efvm_mlp_train_step": [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_ulonglong, pd],
+        "efvm_mlp_train_step": [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_ulonglong, C.c_int, pd],
         "efvm_mlp_gradient": [vp, pf, pf, sz, pf, pd],
+        "efvm_mlp_fit_linear": [vp, sz, C.c_double, C.c_int],
         "efvm_mlp_infer": [vp, pf, sz, pf],
         "efvm_set_learned": [vp, vp],
         "efvm_mlp_infer_stats": [vp, C.c_int, C.POINTER(C.c_ulonglong)],
@@ -442,8 +444,8 @@ def _bind_mlp(L):
 
 EXPORTS += [
     "efvm_mlp_create", "efvm_mlp_destroy", "efvm_mlp_last_error", "efvm_mlp_load", "efvm_mlp_save",
-    "efvm_mlp_add_snapshot", "efvm_mlp_add_samples", "efvm_mlp_get_samples", "efvm_mlp_num_samples",
-    "efvm_mlp_clear_samples", "efvm_mlp_train_step", "efvm_mlp_gradient", "efvm_mlp_infer", "efvm_set_learned",
+    "efvm_mlp_add_snapshot", "efvm_mlp_add_snapshot_pair", "efvm_mlp_add_samples", "efvm_mlp_get_samples", "efvm_mlp_num_samples",
+    "efvm_mlp_clear_samples", "efvm_mlp_train_step", "efvm_mlp_gradient", "efvm_mlp_fit_linear", "efvm_mlp_infer", "efvm_set_learned",
     "efvm_mlp_infer_stats", "efvm_learned_faces",
 ]
 
@@ -488,8 +490,11 @@ class Mlp:
         self._chk(self._L.efvm_mlp_save(self._p, _pf(a), a.size))
         return a
 
-    def add_snapshot(self, fine: "Context", factor: int):
-        self._chk(self._L.efvm_mlp_add_snapshot(self._p, fine._p, int(factor)))
+    def add_snapshot(self, fine: "Context", factor: int, coarse: "Context" = None):
+        if coarse is None:
+            self._chk(self._L.efvm_mlp_add_snapshot(self._p, fine._p, int(factor)))
+        else:
+            self._chk(self._L.efvm_mlp_add_snapshot_pair(self._p, fine._p, coarse._p, int(factor)))
 
     def add_samples(self, X, Y):
         X = np.ascontiguousarray(X, dtype=np.float32)
@@ -511,10 +516,16 @@ class Mlp:
     def clear_samples(self):
         self._chk(self._L.efvm_mlp_clear_samples(self._p))
 
-    def train(self, steps, batch=8192, lr0=2e-3, lr1=2e-5, seed=1):
+    def train(self, steps, batch=8192, lr0=2e-3, lr1=2e-5, seed=1, train_skip=True):
         losses = np.zeros(max(steps, 1))
-        self._chk(self._L.efvm_mlp_train_step(self._p, int(steps), int(batch), lr0, lr1, int(seed), _pd(losses)))
+        self._chk(self._L.efvm_mlp_train_step(self._p, int(steps), int(batch), lr0, lr1, int(seed),
+                                              1 if train_skip else 0, _pd(losses)))
         return losses[:steps]
+
+    def fit_linear(self, stride=1, ridge=1e-9, constrained=True):
+        """Closed-form weighted least-squares fit of the linear skip to the held samples
+        (constrained: exact for cubic profiles, one fitted coefficient per variable)."""
+        self._chk(self._L.efvm_mlp_fit_linear(self._p, int(stride), float(ridge), 1 if constrained else 0))
 
     def gradient(self, X, Y):
         X = np.ascontiguousarray(X, dtype=np.float32)

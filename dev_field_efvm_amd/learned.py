@@ -38,13 +38,16 @@ L_BOX = 10.0
 NIN, NH, NOUT = 20, 64, 4
 
 
+B3_OPEN = 8.0     # initial gate bias: sigmoid(8) = 0.99966, the limiter starts open
+
+
 def init_params(seed: int = 0) -> np.ndarray:
-    """He-normal hidden layers; output layer and linear skip start at zero (the
-    untrained network is the unlimited central slope)."""
+    """He-normal hidden layers; output weights zero with the gates open (bias B3_OPEN); the
+    learned correction zero (the untrained network is the unlimited central slope)."""
     rng = np.random.default_rng(seed)
     parts = [rng.normal(0.0, math.sqrt(2.0 / NIN), (NH, NIN)).ravel(), np.zeros(NH),
              rng.normal(0.0, math.sqrt(2.0 / NH), (NH, NH)).ravel(), np.zeros(NH),
-             np.zeros(NOUT * NH), np.zeros(NOUT), np.zeros(NOUT * NIN), np.zeros(NOUT)]
+             np.zeros(NOUT * NH), np.full(NOUT, B3_OPEN), np.zeros(NOUT * NIN), np.zeros(NOUT)]
     p = np.concatenate(parts).astype(np.float32)
     assert p.size == NPARAM
     return p
@@ -87,8 +90,45 @@ def make_training_set(mlp: Mlp, seeds, n_fine: int = 2048, factor: int = 4, time
         c.close()
 
 
+def advance_exact(c: Context, T: float, cfl: float = 0.4, safety: float = 0.9) -> int:
+    """Advance from the context's current time to exactly T in equal CFL-limited steps."""
+    c.set_time_step(cfl)
+    t = c.clock()["time"]
+    if T <= t:
+        return 0
+    n = int(math.ceil((T - t) / (safety * c.cfl_dt())))
+    c.set_time_step(cfl, (T - t) / n)
+    c.step(n)
+    c.set_time_step(cfl)
+    return n
+
+
+def make_loop_set(mlp: Mlp, net: Mlp, seeds, n_fine: int = 2048, factor: int = 4, times=None, verbose: bool = False):
+    """Samples of the states the network meets inside the solver: a coarse run with `net`
+    attached and the fine MUSCL run advance side by side to the same exact times; the features
+    come from the coarse run's own cells, the targets from the fine run's face line averages."""
+    times = np.round(np.arange(0.1, 1.01, 0.1), 10) if times is None else times
+    n_c = n_fine // factor
+    for s in seeds:
+        fine = periodic_context(n_fine, "fp64")
+        fine.upload(initial_state(s, n_fine))
+        coarse = periodic_context(n_c, "fp32")
+        coarse.upload(initial_state(s, n_c))
+        coarse.set_learned(net)
+        t0 = time.perf_counter()
+        for T in times:
+            advance_exact(fine, float(T))
+            advance_exact(coarse, float(T))
+            mlp.add_snapshot(fine, factor, coarse=coarse)
+        if verbose:
+            print(f"  seed {s}: {len(times)} paired snapshots in {time.perf_counter() - t0:.1f} s "
+                  f"({mlp.num_samples()} samples)", flush=True)
+        fine.close()
+        coarse.close()
+
+
 def train(mlp: Mlp, steps: int = 20000, batch: int = 16384, lr0: float = 2e-3, lr1: float = 2e-5, seed: int = 1,
-          chunk: int = 2000, verbose: bool = False):
+          chunk: int = 2000, train_skip: bool = False, verbose: bool = False):
     """Adam with a cosine schedule over `steps`, run in chunks (one schedule per chunk would
     restart the cosine, so the chunk boundaries interpolate the global schedule)."""
     losses = []
@@ -98,7 +138,7 @@ def train(mlp: Mlp, steps: int = 20000, batch: int = 16384, lr0: float = 2e-3, l
         f0, f1 = done / max(steps - 1, 1), (done + k - 1) / max(steps - 1, 1)
         lr_a = lr1 + 0.5 * (lr0 - lr1) * (1 + math.cos(math.pi * f0))
         lr_b = lr1 + 0.5 * (lr0 - lr1) * (1 + math.cos(math.pi * f1))
-        losses.append(mlp.train(k, batch, lr_a, lr_b, seed))
+        losses.append(mlp.train(k, batch, lr_a, lr_b, seed, train_skip))
         done += k
         if verbose:
             print(f"  step {done}: minibatch loss {losses[-1][-50:].mean():.3e}", flush=True)
@@ -182,24 +222,50 @@ def evaluate(mlp: Mlp, seed: int, n: int = 512, fine: int = 2048, truth_n: int =
     return out
 
 
+def validation_error(p, seed: int, n: int, truth_rho: np.ndarray, T: float = 1.0) -> float:
+    """Density L1 of a learned n^2 run (network p) against a finer run box-filtered to n."""
+    r = run_case(seed, n, T, Mlp(p), timed=False)
+    return float(np.mean(np.abs(r["rho"] - truth_rho)))
+
+
 def train_default(train_seeds=range(1, 9), n_fine: int = 2048, factor: int = 4, steps: int = 20000, batch: int = 16384,
-                  out: str = WEIGHTS, verbose: bool = True):
-    """The committed network: 8 seeds of 2048^2 FP64 MUSCL, 10 snapshots each (t = 0.1 .. 1.0)."""
+                  out: str = WEIGHTS, val_seed: int = 9, verbose: bool = True):
+    """The committed network: 8 seeds of 2048^2 FP64 MUSCL, 10 snapshots each (t = 0.1 .. 1.0).
+
+    1. the learned correction L x + bL in closed form (weighted normal equations on the
+       device), kept exact for cubic profiles: the unconstrained optimum of the smooth
+       training data amplifies grid-scale noise once it runs inside the solver;
+    2. Adam on the limiter MLP (its output layer starts at zero with the gates open), the
+       correction held fixed;
+    3. model selection: the stage-1 and stage-2 networks each run the held-out seed
+       `val_seed` at the coarse resolution, and the one with the smaller density L1 against
+       an FP64 run 8x finer is kept (offline training loss and the error inside the solver
+       need not agree)."""
+    n_c = n_fine // factor
     mlp = Mlp(init_params(0))
     t0 = time.perf_counter()
     make_training_set(mlp, train_seeds, n_fine, factor, verbose=verbose)
     t1 = time.perf_counter()
-    losses = train(mlp, steps, batch, verbose=verbose)
+    mlp.fit_linear(stride=4, constrained=True)
+    linear_only = mlp.save()
+    losses = train(mlp, steps, batch, train_skip=False, verbose=verbose)
+    full = mlp.save()
     t2 = time.perf_counter()
-    p = mlp.save()
+    truth = run_case(val_seed, 8 * n_c, 1.0, precision="fp64", timed=False)
+    truth_rho = box(truth["rho"], 8)
+    e_lin = validation_error(linear_only, val_seed, n_c, truth_rho)
+    e_full = validation_error(full, val_seed, n_c, truth_rho)
+    p = full if e_full <= e_lin else linear_only
     if out:
         os.makedirs(os.path.dirname(out), exist_ok=True)
         np.save(out, p)
     info = dict(samples=mlp.num_samples(), data_s=t1 - t0, train_s=t2 - t1, final_loss=float(losses[-200:].mean()),
                 first_loss=float(losses[:50].mean()), steps=steps, batch=batch, seeds=list(train_seeds), n_fine=n_fine,
-                factor=factor)
+                factor=factor, val_seed=val_seed, val_l1_correction_only=e_lin, val_l1_with_limiter=e_full,
+                kept="with_limiter" if p is full else "correction_only")
     if verbose:
         print(json.dumps(info), flush=True)
+    info["linear_params"] = linear_only
     return mlp, info
 
 

@@ -185,9 +185,11 @@ efvm_status efvm_batch_snapshot(efvm_batch* b, int op);
  * Per face side the network reads the 5-cell stencil of primitives (rho, u_n,
  * u_t, p) oriented toward the face (20 features, normalised per variable) and
  * returns 4 normalised face offsets:
- *   y = (x[+1] - x[-1]) / 4 + L x + bL + W3 gelu(W2 gelu(W1 x + b1) + b2) + b3,
- * 20 -> 64 -> 64 -> 4 with tanh-form GELU; inference runs the hidden layers on
- * bf16 matrix cores with FP32 accumulation.  A face whose either side fails
+ *   y = (x[+1] - x[-1]) / 4 + phi * (L x + bL),
+ *   phi = sigmoid(W3 gelu(W2 gelu(W1 x + b1) + b2) + b3)      (per variable):
+ * a learned high-order correction gated by an MLP limiter, 20 -> 64 -> 64 -> 4
+ * with tanh-form GELU; inference runs the MLP on bf16 matrix cores with FP32
+ * accumulation.  A face whose either side fails
  * positivity or the curvature-relaxed TVD bound uses the solver's MUSCL (MC
  * limiter) on both sides.  Parameters are one FP32 vector of
  * EFVM_MLP_NPARAM values: W1[64][20], b1[64], W2[64][64], b2[64], W3[4][64],
@@ -207,16 +209,31 @@ efvm_status efvm_mlp_save(efvm_mlp* m, float* params, size_t n);
  * and appends 4 samples per coarse cell (x/y faces, both sides); the targets are
  * the fine solution's line averages over the coarse faces. */
 efvm_status efvm_mlp_add_snapshot(efvm_mlp* m, efvm_ctx* fine, int factor);
+/* As add_snapshot, but the features come from a coarse context's own state (nx/factor
+ * by ny/factor, advanced to the same time) and only the targets from the fine one:
+ * samples of the states the network meets inside the solver. */
+efvm_status efvm_mlp_add_snapshot_pair(efvm_mlp* m, efvm_ctx* fine, efvm_ctx* coarse, int factor);
 efvm_status efvm_mlp_add_samples(efvm_mlp* m, const float* X, const float* Y, size_t n);
 efvm_status efvm_mlp_get_samples(efvm_mlp* m, size_t first, size_t n, float* X, float* Y);
 efvm_status efvm_mlp_num_samples(efvm_mlp* m, size_t* n);
 efvm_status efvm_mlp_clear_samples(efvm_mlp* m);
 /* n_steps Adam steps on minibatches of `batch` samples drawn from the held
  * samples (counter hash of seed and step), learning rate cosine from lr0 to lr1;
- * losses (optional, n_steps doubles) receive each minibatch's mean squared error. */
+ * train_skip = 0 keeps the linear skip (L, bL) fixed; losses (optional, n_steps
+ * doubles) receive each minibatch's loss. */
 efvm_status efvm_mlp_train_step(efvm_mlp* m, int n_steps, int batch, float lr0, float lr1, unsigned long long seed,
-                                double* losses);
-/* Loss and gradient over n given samples with the training kernels (FP32). */
+                                int train_skip, double* losses);
+/* Closed-form fit of the linear skip (L, bL) to the held samples: the weighted
+ * normal equations of y* - base over every `stride`-th sample, accumulated in FP64
+ * on the device and solved on the host with a ridge of ridge * trace / 21.
+ * constrained = 1 keeps the offsets exact for polynomial profiles up to degree 3
+ * (per variable, only the coefficient along (1, -4, -4, 1) on cells -2, -1, +1, +2
+ * is fitted; the unconstrained optimum of smooth training data amplifies grid-scale
+ * noise in the solver). */
+efvm_status efvm_mlp_fit_linear(efvm_mlp* m, size_t stride, double ridge, int constrained);
+/* Loss and gradient over n given samples with the training kernels (FP32).  The loss
+ * is the mean of w (y - y*)^2 with w = min((sigma / (0.01 scale))^2, 100) per variable
+ * (the squared face-value error), sigma / scale recovered from the centre feature. */
 efvm_status efvm_mlp_gradient(efvm_mlp* m, const float* X, const float* Y, size_t n, float* grad, double* loss);
 /* Matrix-core inference of y on n given feature rows. */
 efvm_status efvm_mlp_infer(efvm_mlp* m, const float* X, size_t n, float* Y);

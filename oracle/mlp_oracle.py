@@ -7,8 +7,10 @@ checker.  The product (``dev_field_efvm_amd``) never imports it.
 The reference has no learned part; this module is the specification the HIP
 kernels are checked against:
 
-* the network: 20 inputs -> 64 -> 64 -> 4 with tanh-form GELU, plus a learned
-  linear skip (20 -> 4) and the fixed central-slope offset;
+* the network: the fixed central-slope offset plus a learned linear high-order
+  correction (20 -> 4) gated per variable by sigmoid(MLP), the MLP 20 -> 64 -> 64 -> 4
+  with tanh-form GELU: a learned limiter between the high-order map (gate open)
+  and the unlimited central slope (gate closed);
 * the stencil features: the 5-cell stencil of primitives (rho, u_n, u_t, p)
   oriented toward the face, normalised per variable by the stencil's largest
   deviation from the centre cell;
@@ -16,8 +18,9 @@ kernels are checked against:
   faces, from fourth-order face interpolation of the fine cell averages;
 * the per-face fallback to the solver's plain MUSCL (conserved variables, MC
   limiter) when a side fails positivity or the curvature-relaxed TVD bound;
-* the loss (mean squared error of the normalised face offsets), its gradient
-  and the Adam update.
+* the loss (sigma^2-weighted mean squared error of the normalised face offsets,
+  i.e. the squared face-value error), its gradient, the closed-form weighted
+  least-squares fit of the linear skip, and the Adam update.
 
 How each part is pinned without a reference binary: the backward pass by
 central finite differences of the forward pass; the features by their
@@ -62,13 +65,20 @@ def pack(d):
                            d["L"].ravel(), d["bL"]])
 
 
+B3_OPEN = 8.0        # initial gate bias: sigmoid(8) = 0.99966, the limiter starts open
+
+
 def init_params(seed=0):
-    """He-style init of the hidden layers; output layer and linear skip start at zero."""
+    """He-style init of the hidden layers; output weights zero with the gates open; skip zero."""
     rng = np.random.default_rng(seed)
     d = dict(W1=rng.normal(0, math.sqrt(2.0 / NIN), (NH, NIN)), b1=np.zeros(NH),
              W2=rng.normal(0, math.sqrt(2.0 / NH), (NH, NH)), b2=np.zeros(NH),
-             W3=np.zeros((NOUT, NH)), b3=np.zeros(NOUT), L=np.zeros((NOUT, NIN)), bL=np.zeros(NOUT))
+             W3=np.zeros((NOUT, NH)), b3=np.full(NOUT, B3_OPEN), L=np.zeros((NOUT, NIN)), bL=np.zeros(NOUT))
     return pack(d)
+
+
+def sigmoid(z):
+    return 1.0 / (1.0 + np.exp(-z))
 
 
 def gelu(x):
@@ -94,7 +104,8 @@ def base(X):
 
 
 def forward(p, X, quantized=False):
-    """y = base(X) + L X + bL + W3 gelu(W2 gelu(W1 X + b1) + b2) + b3.
+    """y = base(X) + phi * (L X + bL),  phi = sigmoid(W3 gelu(W2 gelu(W1 X + b1) + b2) + b3):
+    the MLP gates (limits) the learned high-order correction per variable.
 
     quantized=True models the inference kernel: the MLP's weights (b1 included: it rides on
     a constant input of the bf16 layer), inputs and hidden activations rounded to bfloat16,
@@ -106,12 +117,25 @@ def forward(p, X, quantized=False):
     h1 = q(gelu(z1))
     z2 = h1 @ q(d["W2"]).T + d["b2"]
     h2 = q(gelu(z2))
-    mlp = h2 @ q(d["W3"]).T + d["b3"]
-    return base(X) + X @ d["L"].T + d["bL"] + mlp
+    phi = sigmoid(h2 @ q(d["W3"]).T + d["b3"])
+    return base(X) + phi * (X @ d["L"].T + d["bL"])
+
+
+WREF = 0.01
+WMAX = 100.0
+
+
+def sample_weight(X):
+    """Per-variable loss weight min((sigma / (WREF scale))^2, WMAX), sigma / scale = X0C x0 / (1 - x0)
+    from the centre features x0 = X[..., 8:12]: the squared face-value error in units of the
+    centre cell's scale, capped so discontinuities do not dominate."""
+    x0 = np.asarray(X, dtype=np.float64)[..., 8:12]
+    r = X0C * x0 / (1.0 - x0)
+    return np.minimum((r / WREF) ** 2, WMAX)
 
 
 def loss_and_grad(p, X, Y):
-    """Mean over samples and outputs of (y - y*)^2 and its gradient w.r.t. the flat parameters."""
+    """Mean over samples and outputs of w (y - y*)^2 and its gradient w.r.t. the flat parameters."""
     d = unpack(p)
     X = np.asarray(X, dtype=np.float64)
     Y = np.asarray(Y, dtype=np.float64)
@@ -120,18 +144,58 @@ def loss_and_grad(p, X, Y):
     h1 = gelu(z1)
     z2 = h1 @ d["W2"].T + d["b2"]
     h2 = gelu(z2)
-    y = base(X) + X @ d["L"].T + d["bL"] + h2 @ d["W3"].T + d["b3"]
+    phi = sigmoid(h2 @ d["W3"].T + d["b3"])
+    s = X @ d["L"].T + d["bL"]
+    y = base(X) + phi * s
     r = y - Y
-    loss = float(np.mean(r * r))
-    dy = 2.0 * r / (n * NOUT)
-    g = dict(W3=dy.T @ h2, b3=dy.sum(0), L=dy.T @ X, bL=dy.sum(0))
-    dz2 = (dy @ d["W3"]) * gelu_grad(z2)
+    w = sample_weight(X)
+    loss = float(np.mean(w * r * r))
+    dy = 2.0 * w * r / (n * NOUT)
+    dz3 = dy * s * phi * (1.0 - phi)
+    g = dict(W3=dz3.T @ h2, b3=dz3.sum(0), L=(dy * phi).T @ X, bL=(dy * phi).sum(0))
+    dz2 = (dz3 @ d["W3"]) * gelu_grad(z2)
     g["W2"] = dz2.T @ h1
     g["b2"] = dz2.sum(0)
     dz1 = (dz2 @ d["W2"]) * gelu_grad(z1)
     g["W1"] = dz1.T @ X
     g["b1"] = dz1.sum(0)
     return loss, pack(g)
+
+
+LIN5 = np.array([2.0, -13.0, 27.0, -3.0]) / 60.0      # fifth-order face value from cells -2, -1, +1, +2
+NULL3 = np.array([1.0, -4.0, -4.0, 1.0])               # keeps exactness up to degree 3
+
+
+def fit_linear(p, X, Y, ridge=1e-9, constrained=False):
+    """Weighted least squares of the linear skip with the gates open (phi = 1): per variable, (L_v, bL_v) minimising
+    sum_s w_sv ((L_v x_s + bL_v) - (y*_sv - base_sv))^2 with a ridge of ridge * trace / 21.
+
+    constrained: the offsets stay exact for polynomial profiles up to degree 3 -- own-variable
+    coefficients (LIN5 - base) + t NULL3 on cells -2, -1, +1, +2, all others 0 -- and only t is
+    fitted, t = d.(b - G c0) / (d.G d + ridge trace)."""
+    d = unpack(p)
+    X = np.asarray(X, dtype=np.float64)
+    A = np.c_[X, np.ones(len(X))]
+    R = np.asarray(Y, dtype=np.float64) - base(X)
+    W = sample_weight(X)
+    for v in range(NOUT):
+        G = A.T @ (A * W[:, v:v + 1])
+        b = A.T @ (W[:, v] * R[:, v])
+        tr = np.trace(G)
+        if constrained:
+            idx = [v, 4 + v, 12 + v, 16 + v]
+            c0 = np.zeros(NIN + 1)
+            dv = np.zeros(NIN + 1)
+            c0[idx] = LIN5 - np.array([0.0, -0.25, 0.25, 0.0])
+            dv[idx] = NULL3
+            t = dv @ (b - G @ c0) / (dv @ G @ dv + ridge * tr)
+            c = c0 + t * dv
+        else:
+            G = G + ridge * tr / G.shape[0] * np.eye(G.shape[0])
+            c = np.linalg.solve(G, b)
+        d["L"][v] = c[:NIN]
+        d["bL"][v] = c[NIN]
+    return pack(d)
 
 
 def adam(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):

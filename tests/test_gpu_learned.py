@@ -71,6 +71,27 @@ def test_gradient_matches_oracle():
     assert rel.max() < 1e-3
 
 
+def test_linear_fit_matches_oracle():
+    p = random_params(8)
+    net = efvm.Mlp(p)
+    rng = np.random.default_rng(9)
+    X = random_features(3000, 9)
+    X[:, 8:12] = rng.uniform(0.0, 0.2, (3000, 4)).astype(np.float32)
+    Y = rng.normal(0, 0.3, (3000, 4)).astype(np.float32)
+    net.add_samples(X, Y)
+    sl = slice(M.OFF_L, M.NPARAM)
+    for constrained in (False, True):
+        net.load(p)
+        net.fit_linear(stride=1, ridge=1e-9, constrained=constrained)
+        got = net.save()
+        ref = M.fit_linear(p.astype(np.float64), X.astype(np.float64), Y.astype(np.float64), ridge=1e-9,
+                           constrained=constrained)
+        err = np.max(np.abs(got[sl] - ref[sl]))
+        print(f"linear fit (constrained={constrained}) vs oracle: max abs {err:.2e}")
+        assert err < 1e-4 * max(1.0, np.abs(ref[sl]).max())
+        assert np.array_equal(got[:M.OFF_L], p[:M.OFF_L])      # only the skip is fitted
+
+
 def _mix64(z):
     m = (1 << 64) - 1
     z = (z + 0x9E3779B97F4A7C15) & m
@@ -127,7 +148,7 @@ def _oracle_faces(p, U):
 def test_learned_face_states_match_oracle():
     n = 96
     st = cases.learned_state(12, n)
-    p = random_params(6, scale=0.03)              # rough network: a mix of accepted and fallen-back faces
+    p = random_params(6, scale=0.1)               # rough network: a mix of accepted and fallen-back faces
     net = efvm.Mlp(p)
     c = learned.periodic_context(n, "fp32")
     c.upload(st)

@@ -45,7 +45,7 @@ def test_backward_matches_finite_differences():
     X = rng.uniform(-1, 1, (37, 20))
     Y = rng.normal(0, 0.5, (37, 4))
     loss, g = M.loss_and_grad(p, X, Y)
-    assert np.isclose(loss, np.mean((M.forward(p, X) - Y) ** 2))
+    assert np.isclose(loss, np.mean(M.sample_weight(X) * (M.forward(p, X) - Y) ** 2))
     idx = rng.choice(M.NPARAM, 300, replace=False)
     idx = np.concatenate([idx, [M.OFF_B1, M.OFF_B2 + 5, M.OFF_B3 + 2, M.OFF_BL + 1, M.OFF_L + 7, M.OFF_W3 + 11]])
     h = 1e-6
@@ -54,6 +54,54 @@ def test_backward_matches_finite_differences():
         e[i] = h
         fd = (M.loss_and_grad(p + e, X, Y)[0] - M.loss_and_grad(p - e, X, Y)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(g[i]), (i, fd, g[i])
+
+
+def test_sample_weight_is_the_squared_face_value_scale():
+    # x0 = sigma / (sigma + X0C scale): sigma / scale = 0.002 -> x0 = 0.002 / 0.102, weight (0.002 / 0.01)^2
+    x0 = 0.002 / (0.002 + M.X0C)
+    X = np.zeros((1, 20))
+    X[0, 8:12] = [x0, 0.0, 0.5, 0.999]
+    w = M.sample_weight(X)[0]
+    assert np.allclose(w, [0.04, 0.0, 100.0, 100.0])
+
+
+def test_linear_fit_solves_the_weighted_normal_equations():
+    rng = np.random.default_rng(7)
+    X = rng.uniform(-1, 1, (400, 20))
+    X[:, 8:12] = rng.uniform(0.0, 0.2, (400, 4))
+    Y = rng.normal(0, 0.3, (400, 4))
+    d = M.unpack(random_params(3))
+    d["W3"][:] = 0.0
+    d["b3"][:] = 40.0                                    # gates fully open: phi = 1
+    p = M.fit_linear(M.pack(d), X, Y, ridge=0.0)
+    loss, g = M.loss_and_grad(p, X, Y)
+    gl = np.concatenate([g[M.OFF_L:M.OFF_BL], g[M.OFF_BL:]])
+    assert np.max(np.abs(gl)) < 1e-12 * max(1.0, loss) + 1e-14
+    # and it beats the linear map it started from
+    assert loss < M.loss_and_grad(M.pack(d), X, Y)[0]
+
+
+def test_constrained_linear_fit_is_exact_for_cubic_profiles():
+    rng = np.random.default_rng(8)
+    X = rng.uniform(-1, 1, (300, 20))
+    X[:, 8:12] = rng.uniform(0.0, 0.2, (300, 4))
+    Y = rng.normal(0, 0.3, (300, 4))
+    d = M.unpack(M.init_params(0))
+    d["b3"][:] = 40.0                                    # gates fully open
+    p = M.fit_linear(M.pack(d), X, Y, constrained=True)
+    # cell averages of a cubic on unit cells centred at -2..2: the face value at +1/2 is exact
+    coef = np.array([0.3, -1.1, 0.7, 0.9])               # c0 + c1 x + c2 x^2 + c3 x^3
+    def cavg(k):
+        P = np.polynomial.Polynomial(coef).integ()
+        return P(k + 0.5) - P(k - 0.5)
+    St = np.zeros((5, 4, 1))
+    for k in range(5):
+        St[k, :, 0] = 2.0 + 0.1 * cavg(k - 2)             # same shape on every variable, rho, p > 0
+    Xs, sig, c = M.features(St)
+    y = M.forward(p, Xs.reshape(1, 20))[0]
+    face = c[0] + sig[0] * y
+    exact = 2.0 + 0.1 * np.polynomial.Polynomial(coef)(0.5)
+    assert np.allclose(face, exact, atol=1e-12)
 
 
 def test_bf16_rounding_is_nearest_even():
@@ -187,7 +235,7 @@ def test_side_ok_bounds():
 def test_face_states_fall_back_to_muscl_where_a_side_fails():
     rng = np.random.default_rng(5)
     U = M.cons(smooth_field(24))
-    p = random_params(2, scale=0.03)                              # a rough network: many faces fail
+    p = random_params(2, scale=0.1)                               # a rough network: many faces fail
     out, fb = M.face_states(p, U, quantized=False)
     assert fb[0].any() and (~fb[0]).any()
     up = np.roll(U, -1, 2)
