@@ -134,9 +134,53 @@ def bench_cylinder(args):
     }
 
 
+def bench_learned(args):
+    """Learned reconstruction on the periodic [0,10]^2 box (8 Fourier modes of amplitude 0.05
+    plus 1-4 isentropic vortices, held-out seed): learned 512^2 against plain MUSCL at 512^2
+    and 2048^2, density L1 at t = 1 against an FP32 8192^2 MUSCL run, all box-filtered to
+    512^2.  value = GPU time of MUSCL 2048^2 / GPU time of learned 512^2 (HIP events around
+    hipGraph replays), reported with whether the learned run reaches the 2048^2 error."""
+    import numpy as np
+    import torch
+    from dev_field_efvm_amd import learned
+    from dev_field_efvm_amd.efvm import Mlp
+    torch.cuda.synchronize()
+    net = Mlp(learned.load_weights())
+    n, fine, truth_n, T, seed = args.learned_n, 4 * args.learned_n, 16 * args.learned_n, 1.0, args.learned_seed
+    truth = learned.run_case(seed, truth_n, T, precision="fp32", timed=False)
+    ref = learned.box(truth["rho"], truth_n // n)
+    runs = {}
+    for name, nn, m in (("learned", n, net), ("muscl", n, None), ("muscl_fine", fine, None)):
+        r = learned.run_case(seed, nn, T, m)
+        runs[name] = dict(l1=float(np.mean(np.abs(learned.box(r["rho"], nn // n) - ref))), steps=r["steps"],
+                          gpu_ms=round(r["gpu_ms"], 3), fallback_frac=(r["fallback"] / r["faces"]) if r["faces"] else 0.0)
+    L, F = runs["learned"], runs["muscl_fine"]
+    speedup = F["gpu_ms"] / L["gpu_ms"]
+    # matrix-core work of the network per learned stage: 4 face sides per cell, 20-64-64-4
+    flops_stage = 4 * n * n * 2 * (20 * 64 + 64 * 64 + 64 * 4)
+    stage_s = L["gpu_ms"] * 1e-3 / (3 * L["steps"])
+    achieved = flops_stage / stage_s / 1e12
+    return {
+        "metric": METRIC, "value": speedup, "unit": f"x (GPU time MUSCL {fine}^2 / learned {n}^2 to equal t)",
+        "n_gpus": 1, "steps": L["steps"], "warmup": 0, "ms_per_step": L["gpu_ms"] / L["steps"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (network) / f32 (flux, update)",
+        "data": f"synthetic (seed {seed}: 8 Fourier modes amp 0.05 + isentropic vortices, held out of training)",
+        "config": {"workload": f"learned reconstruction {n}^2 vs MUSCL {n}^2 / {fine}^2, t = {T}, truth MUSCL {truth_n}^2",
+                   "l1_learned": L["l1"], "l1_muscl": runs["muscl"]["l1"], "l1_muscl_fine": F["l1"],
+                   "reaches_fine_l1": L["l1"] <= F["l1"], "l1_gain_vs_muscl": runs["muscl"]["l1"] / L["l1"],
+                   "gpu_time_ratio_learned_over_fine": L["gpu_ms"] / F["gpu_ms"], "runs": runs},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(achieved / 2500.0, 5), "traffic": None,
+                     "kernel": "learned stage (face-state MLP kernel + face stage kernel), bf16 dense peak",
+                     "mean_stage_us": round(stage_s * 1e6, 2), "network_flops_per_stage": flops_stage},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="single", choices=["single", "batch", "cylinder"])
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "cylinder", "learned"])
+    ap.add_argument("--learned-n", type=int, default=512)
+    ap.add_argument("--learned-seed", type=int, default=101)
     ap.add_argument("--cyl-nx", type=int, default=1760)
     ap.add_argument("--cyl-ny", type=int, default=328)
     ap.add_argument("--cyl-max-steps", type=int, default=200000)
@@ -163,6 +207,9 @@ def main():
         return
     if args.workload == "cylinder":
         print(json.dumps(bench_cylinder(args)), flush=True)
+        return
+    if args.workload == "learned":
+        print(json.dumps(bench_learned(args)), flush=True)
         return
 
     import torch

@@ -735,7 +735,9 @@ __global__ void sample_kernel(SampleArgs a, long n_out) {
 // host launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st) {
-    const int blocks = std::min((a.n_tasks + 3) / 4, 2048);
+    // persistent: two 4-wave blocks per CU (the kernel's register budget allows 2 waves per
+    // SIMD); each wave loads the network into registers once and loops over tasks
+    const int blocks = std::min((a.n_tasks + 3) / 4, 512);
     mlp::faces_kernel<<<blocks, 256, 0, st>>>(a);
     return hipGetLastError();
 }
