@@ -137,8 +137,8 @@ def bench_cylinder(args):
 def bench_learned(args):
     """Learned reconstruction on the periodic [0,10]^2 box (8 Fourier modes of amplitude 0.05
     plus 1-4 isentropic vortices, held-out seed): learned 512^2 against plain MUSCL at 512^2
-    and 2048^2, density L1 at t = 1 against an FP32 8192^2 MUSCL run, all box-filtered to
-    512^2.  value = GPU time of MUSCL 2048^2 / GPU time of learned 512^2 (HIP events around
+    and 2048^2, density L1 at t = 1 against an FP64 MUSCL run 16x finer (8192^2, about three
+    minutes), all box-filtered to 512^2.  value = GPU time of MUSCL 2048^2 / GPU time of learned 512^2 (HIP events around
     hipGraph replays), reported with whether the learned run reaches the 2048^2 error."""
     import numpy as np
     import torch
@@ -147,7 +147,8 @@ def bench_learned(args):
     torch.cuda.synchronize()
     net = Mlp(learned.load_weights())
     n, fine, truth_n, T, seed = args.learned_n, 4 * args.learned_n, 16 * args.learned_n, 1.0, args.learned_seed
-    truth = learned.run_case(seed, truth_n, T, precision="fp32", timed=False)
+    truth = learned.run_case(seed, truth_n, T, precision="fp64", timed=False,
+                             progress=lambda d, s: print(f"truth {truth_n}^2: {d}/{s} steps", file=sys.stderr, flush=True))
     ref = learned.box(truth["rho"], truth_n // n)
     runs = {}
     for name, nn, m in (("learned", n, net), ("muscl", n, None), ("muscl_fine", fine, None)):
@@ -165,7 +166,7 @@ def bench_learned(args):
         "n_gpus": 1, "steps": L["steps"], "warmup": 0, "ms_per_step": L["gpu_ms"] / L["steps"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (network) / f32 (flux, update)",
         "data": f"synthetic (seed {seed}: 8 Fourier modes amp 0.05 + isentropic vortices, held out of training)",
-        "config": {"workload": f"learned reconstruction {n}^2 vs MUSCL {n}^2 / {fine}^2, t = {T}, truth MUSCL {truth_n}^2",
+        "config": {"workload": f"learned reconstruction {n}^2 vs MUSCL {n}^2 / {fine}^2, t = {T}, truth FP64 MUSCL {truth_n}^2",
                    "l1_learned": L["l1"], "l1_muscl": runs["muscl"]["l1"], "l1_muscl_fine": F["l1"],
                    "reaches_fine_l1": L["l1"] <= F["l1"], "l1_gain_vs_muscl": runs["muscl"]["l1"] / L["l1"],
                    "gpu_time_ratio_learned_over_fine": L["gpu_ms"] / F["gpu_ms"], "runs": runs},
@@ -176,9 +177,86 @@ def bench_learned(args):
     }
 
 
+def bench_learned_stage(args):
+    """Per-stage GPU time of the learned reconstruction at n^2 (face-state MLP kernel + face
+    stage kernel) next to plain MUSCL at n^2 and 4n^2, hipGraph replays of 10 steps (the
+    short run to put under a kernel-trace profiler)."""
+    import torch
+    from dev_field_efvm_amd import learned
+    from dev_field_efvm_amd.autotune import tune_context
+    from dev_field_efvm_amd.efvm import Mlp
+    torch.cuda.synchronize()
+    net = Mlp(learned.load_weights())
+    out = {}
+    for name, n, use in (("learned", args.learned_n, True), ("muscl", args.learned_n, False),
+                         ("muscl_fine", 4 * args.learned_n, False)):
+        c = learned.periodic_context(n)
+        c.upload(learned.initial_state(args.learned_seed, n))
+        if use:
+            c.set_learned(net)
+        else:
+            tune_context(c, steps=2, use_cache=False, store=False)
+        c.step(2)
+        c.graph_capture(10)
+        ms = 0.0
+        for _ in range(4):
+            c.graph_launch()
+            ms += c.last_elapsed_ms()
+        out[name] = {"n": n, "us_per_stage": round(ms / 40 / 3 * 1e3, 2)}
+        c.close()
+    return {"metric": METRIC, "value": out["learned"]["us_per_stage"], "unit": "us per RK stage (learned, graph)",
+            "n_gpus": 1, "steps": 40, "warmup": 2, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16 (network) / f32", "data": "synthetic", "config": {"workload": "learned stage timing", **out}}
+
+
+def bench_fp32_error(args):
+    """The FP32 production path against the FP64 path on the same workload (periodic n^2
+    vortex + 8 modes, MUSCL(MC)+HLLC+RK3, CFL 0.4, `steps` steps): relative L-inf per stored
+    component (normalised by the FP64 max |component|), density L1, final times."""
+    import numpy as np
+    import dev_field_efvm_amd as efvm
+    from dev_field_efvm_amd import cases
+    n = args.n
+    st = cases.vortex_modes_state(n, n, seed=1)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        ctx = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, flux="hllc", muscl=True, limiter="mc",
+                           precision=prec, cfl=0.4)
+        ctx.set_boundaries(["periodic"] * 4)
+        ctx.upload(st)
+        ctx.step(args.steps)
+        res[prec] = (ctx.download(), ctx.clock())
+        ctx.close()
+    a, b = res["fp32"][0], res["fp64"][0]
+    rel = {name: float(np.max(np.abs(a[..., m] - b[..., m])) / np.max(np.abs(b[..., m])))
+           for name, m in (("rho", 0), ("rho_u", 1), ("rho_v", 2), ("rho_E", 4))}
+    l1 = float(np.mean(np.abs(a[..., 0] - b[..., 0])))
+    return {"metric": "FP32 error against the FP64 path", "value": max(rel.values()), "unit": "relative L-inf",
+            "n_gpus": 1, "steps": args.steps, "warmup": 0, "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 vs f64", "data": "synthetic (isentropic vortex + 8 seeded Fourier modes)",
+            "config": {"workload": f"periodic {n}x{n} MUSCL(MC)+HLLC+SSP-RK3, CFL 0.4", "rel_linf": rel, "rho_l1": l1,
+                       "time_fp32": res["fp32"][1]["time"], "time_fp64": res["fp64"][1]["time"]}}
+
+
+def bench_copy(args):
+    """A 256 MiB device-to-device copy, three times: the known-size reference that scales the
+    FETCH_SIZE / WRITE_SIZE counters of a PMC pass to bytes."""
+    import torch
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"metric": "copy calibration", "value": 4 * n, "unit": "bytes per copy", "n_gpus": 1, "steps": 3,
+            "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": "256 MiB hipMemcpy device-to-device x3"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="single", choices=["single", "batch", "cylinder", "learned"])
+    ap.add_argument("--workload", default="single",
+                    choices=["single", "batch", "cylinder", "learned", "learned-stage", "fp32-error", "copy"])
     ap.add_argument("--learned-n", type=int, default=512)
     ap.add_argument("--learned-seed", type=int, default=101)
     ap.add_argument("--cyl-nx", type=int, default=1760)
@@ -210,6 +288,15 @@ def main():
         return
     if args.workload == "learned":
         print(json.dumps(bench_learned(args)), flush=True)
+        return
+    if args.workload == "learned-stage":
+        print(json.dumps(bench_learned_stage(args)), flush=True)
+        return
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)), flush=True)
+        return
+    if args.workload == "fp32-error":
+        print(json.dumps(bench_fp32_error(args)), flush=True)
         return
 
     import torch

@@ -178,7 +178,10 @@ def timed_steps(c: Context, n: int, graph_steps: int = 20) -> float:
     return ms
 
 
-def run_case(seed: int, n: int, T: float, mlp: Mlp | None = None, precision: str = "fp32", timed: bool = True):
+def run_case(seed: int, n: int, T: float, mlp: Mlp | None = None, precision: str = "fp32", timed: bool = True,
+             progress=None, chunk: int = 1000):
+    """One run to exactly T.  Untimed runs advance in chunks of `chunk` steps and report
+    (steps done, steps) to `progress` after each, if given."""
     c = periodic_context(n, precision)
     c.upload(initial_state(seed, n))
     if mlp is not None:
@@ -187,7 +190,14 @@ def run_case(seed: int, n: int, T: float, mlp: Mlp | None = None, precision: str
     if timed:
         ms = timed_steps(c, steps)
     else:
-        c.step(steps)
+        done = 0
+        while done < steps:
+            k = min(chunk, steps - done)
+            c.step(k)
+            c.synchronize()
+            done += k
+            if progress is not None:
+                progress(done, steps)
         ms = float("nan")
     state = c.download()
     # the fixed dt must have stayed within the CFL limit of the final state too
