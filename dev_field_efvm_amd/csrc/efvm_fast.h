@@ -79,6 +79,14 @@ __host__ __device__ __forceinline__ Prep prep_prim(float r, float u, float v, fl
     return o;
 }
 
+// c ? a : b as a bit select.  Written with masks so that the seven selections on
+// one condition stay lane selects: as conditional expressions they are merged
+// into one divergent branch that copies every value on both paths.
+__host__ __device__ __forceinline__ float pick(bool c, float a, float b) {
+    const unsigned m = 0u - (unsigned)c;
+    return __builtin_bit_cast(float, (__builtin_bit_cast(unsigned, a) & m) | (__builtin_bit_cast(unsigned, b) & ~m));
+}
+
 template <int DIR, bool THORNBER>
 __host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R) {
     float ul = L.u, vl = L.v, ur = R.u, vr = R.v;
@@ -105,8 +113,8 @@ __host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R)
     const float N = (R.p - L.p) + fl * unl - fr * unr;
     const float Dn = fl - fr;
     const bool left = N <= 0.f;
-    const float rK = left ? L.r : R.r, uK = left ? ul : ur, vK = left ? vl : vr;
-    const float pK = left ? L.p : R.p, eK = left ? el : er, unK = left ? unl : unr, sK = left ? sl : sr;
+    const float rK = pick(left, L.r, R.r), uK = pick(left, ul, ur), vK = pick(left, vl, vr);
+    const float pK = pick(left, L.p, R.p), eK = pick(left, el, er), unK = pick(left, unl, unr), sK = pick(left, sl, sr);
     const float X = sK * Dn - N;
     const float rr = rcp(Dn * X);
     const float ss = N * X * rr;
@@ -117,7 +125,7 @@ __host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R)
     f.ru = ruK * unK + (DIR == 0 ? pK : 0.f);
     f.rv = rvK * unK + (DIR == 1 ? pK : 0.f);
     f.re = (eK + pK) * unK;
-    const bool fan = left ? (sl < 0.f) : (sr >= 0.f);
+    const bool fan = (left & (sl < 0.f)) | (!left & (sr >= 0.f));   // lane-mask logic, no select
     const float sm = sK - unK;
     const float rs = rK * sm * D;
     const float es = D * (sm * eK + (ss - unK) * (sm * rK * ss + pK));

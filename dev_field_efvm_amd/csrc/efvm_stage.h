@@ -95,7 +95,32 @@ __device__ __forceinline__ int strip_index(const StageArgs& a, int waves_per_blo
         int x = b % 8, k = b / 8;                 // k-th workgroup dealt to XCD x
         b = x * per + (x < rem ? x : rem) + k;    // contiguous range per XCD (bijective)
     }
-    return b * waves_per_block + (int)(threadIdx.x / WAVE);
+    // wave-uniform: keeps the strip's rows, and every per-row decision, in scalar registers
+    return __builtin_amdgcn_readfirstlane(b * waves_per_block + (int)(threadIdx.x / WAVE));
+}
+
+// Plane element at a wave-uniform base plus a per-lane byte offset (32-bit): the
+// load/store takes the base in scalar registers and the offset in one VGPR, so
+// the four planes of a row cost no per-lane 64-bit address arithmetic.  The
+// row_ld / row_st forms go through a buffer descriptor of one row (`nbytes`),
+// built in scalar registers: no vector address arithmetic at all, and an offset
+// past the row reads 0 instead of touching memory.
+constexpr int BUF_DWORD3 = 0x00020000;   // gfx9 raw buffer: 32-bit data format, no swizzle
+__device__ __forceinline__ float row_ld(const float* row, unsigned nbytes, unsigned byte_off) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)row, 0, (int)nbytes, BUF_DWORD3);
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+}
+__device__ __forceinline__ void row_st(float* row, unsigned nbytes, unsigned byte_off, float v) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)row, 0, (int)nbytes, BUF_DWORD3);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, byte_off, 0, 0);
+}
+template <class T>
+__device__ __forceinline__ T ld_at(const T* base, unsigned byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <class T>
+__device__ __forceinline__ void st_at(T* base, unsigned byte_off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
 
 // ---------------------------------------------------------------------------

@@ -13,8 +13,8 @@
 //  * the column of each lane (and its ghost/periodic mapping) is fixed for
 //    the whole strip; the row index is wave-uniform, so the per-row boundary
 //    logic is a scalar branch;
-//  * rows are prefetched two iterations ahead and U0 at the top of the
-//    iteration that consumes it, so HBM latency overlaps the flux arithmetic;
+//  * rows are prefetched two iterations ahead and U0 one iteration ahead,
+//    so HBM latency overlaps the flux arithmetic;
 //  * rarely used features (viscous terms, immersed body, inflow/outlet
 //    boundaries needing the EOS) are template features, so the common
 //    periodic/reflecting case carries none of their registers.
@@ -28,15 +28,17 @@ using fast::Prep;
 
 enum { FEAT_VISC = 1, FEAT_IBM = 2, FEAT_CBC = 4 };
 
-// lane i receives the value of lane i-1 (from_left) / lane i+1 (from_right)
+// lane i receives the value of lane i-1 (from_left) / lane i+1 (from_right); the
+// lane with no source (0, resp. 63) receives 0 (bound_ctrl), so the move needs no
+// initialised destination
 __device__ __forceinline__ float from_left(float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_right(float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x130, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x130, 0xf, 0xf, true));
 }
-__device__ __forceinline__ int from_left_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int from_right_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int from_left_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int from_right_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, true); }
 __device__ __forceinline__ F4 from_left4(const F4& v) { return {from_left(v.r), from_left(v.ru), from_left(v.rv), from_left(v.re)}; }
 __device__ __forceinline__ F4 from_right4(const F4& v) { return {from_right(v.r), from_right(v.ru), from_right(v.rv), from_right(v.re)}; }
 __device__ __forceinline__ Prep from_leftP(const Prep& v) {
@@ -73,17 +75,30 @@ __device__ __forceinline__ F4 load_row(const StageArgs& a, const float* const* U
         Cell<float> cl = load_cell(a, U, c, r);   // EOS-based ghosts, evaluated in FP64
         return {cl.r, cl.ru, cl.rv, cl.re};
     }
-    const long q = (long)rr * a.nx + cq;
-    F4 v = {U[0][q], U[1][q], U[2][q], U[3][q]};
+    const long rb = (long)rr * a.nx;
+    const unsigned off = (unsigned)cq * 4u, nb = (unsigned)a.nx * 4u;
+    F4 v = {row_ld(U[0] + rb, nb, off), row_ld(U[1] + rb, nb, off), row_ld(U[2] + rb, nb, off), row_ld(U[3] + rb, nb, off)};
     if (ykind >= 0) v = reflect(ykind, 1, v);
     if (xkind >= 0) v = reflect(xkind, 0, v);
     return v;
 }
 
+// Half the limited slope: the face states are u -/+ half_slope.  For MC the
+// factor 1/2 is folded into the limiter's own power-of-two scalings (bitwise equal
+// to 0.5 * mc(a, b)).
 template <int LIM>
-__device__ __forceinline__ F4 slope4(const F4& dr, const F4& dl) {
-    return {fast::limit<LIM>(dr.r, dl.r), fast::limit<LIM>(dr.ru, dl.ru), fast::limit<LIM>(dr.rv, dl.rv),
-            fast::limit<LIM>(dr.re, dl.re)};
+__device__ __forceinline__ float half_slope(float a, float b) {
+    if (LIM == 1) {
+        const float s = a + b;
+        const float m = fminf(fminf(fabsf(a), fabsf(b)), 0.25f * fabsf(s));
+        return a * b > 0.f ? copysignf(m, s) : 0.f;
+    }
+    return 0.5f * fast::minmod(a, b);
+}
+template <int LIM>
+__device__ __forceinline__ F4 half_slope4(const F4& dr, const F4& dl) {
+    return {half_slope<LIM>(dr.r, dl.r), half_slope<LIM>(dr.ru, dl.ru), half_slope<LIM>(dr.rv, dl.rv),
+            half_slope<LIM>(dr.re, dl.re)};
 }
 __device__ __forceinline__ F4 axpy4(const F4& u, float h, const F4& s) {
     return {fmaf(h, s.r, u.r), fmaf(h, s.ru, u.ru), fmaf(h, s.rv, u.rv), fmaf(h, s.re, u.re)};
@@ -144,6 +159,9 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
     const bool seam_rec = a.seam_mode == 1;
     const bool xW_rec = MUSCL && (col_ghost ? (a.per_x && seam_rec) : !(c == 0 && !a.per_x));
     const bool xE_rec = MUSCL && (col_ghost ? (a.per_x && seam_rec) : !(c == nx - 1 && !a.per_x));
+    // face state = u + h * half_slope with h = -1 (west/south), +1 (east/north) or 0
+    // where the reference keeps that side first order; fma(0, s, u) == u for finite s
+    const float hW = xW_rec ? -1.f : 0.f, hE = xE_rec ? 1.f : 0.f;
     const bool out_lane = lane >= 2 && lane < WAVE - 2 && c < nx;
     auto solid_at = [&](int rr) -> int {
         if (!IBM || col_ghost || rr < 0 || rr >= ny) return 0;
@@ -159,9 +177,12 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
             const bool row_ghost = (jc < 0 || jc >= ny);
             const bool yS_rec = row_ghost ? (a.per_y && seam_rec) : !(jc == 0 && !a.per_y);
             const bool yN_rec = row_ghost ? (a.per_y && seam_rec) : !(jc == ny - 1 && !a.per_y);
-            const F4 sy = slope4<LIM>(fwd, bwd);
-            lo_p = fast::prep(yS_rec ? axpy4(uc, -0.5f, sy) : uc, g, gm1);
-            hi_p = fast::prep(yN_rec ? axpy4(uc, 0.5f, sy) : uc, g, gm1);
+            // row flags are wave-uniform: keep them (and the factors) in scalar registers
+            const float hS = __builtin_amdgcn_readfirstlane((int)yS_rec) ? -1.f : 0.f;
+            const float hN = __builtin_amdgcn_readfirstlane((int)yN_rec) ? 1.f : 0.f;
+            const F4 sy = half_slope4<LIM>(fwd, bwd);
+            lo_p = fast::prep(axpy4(uc, hS, sy), g, gm1);
+            hi_p = fast::prep(axpy4(uc, hN, sy), g, gm1);
         } else {
             lo_p = fast::prep(uc, g, gm1);
             hi_p = lo_p;
@@ -173,13 +194,19 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
     // vin = viscous terms of row jc-1.  Out: dout, hout, gout = G(jc-1/2), xout, vout.
     // Solid flags: s_m, s_c, s_p of rows jc-1, jc, jc+1, s_n of jc+2.  Reloads um with U(jc+3).
     auto step = [&](int jc, F4& um, const F4& uc, const F4& up, F4& din, F4& dout, Prep& hin, Prep& hout,
-                    F4& gin, F4& gout, F4& xin, F4& xout, float* vin, float* vout, int& s_m, int s_c, int s_p) {
+                    F4& gin, F4& gout, F4& xin, F4& xout, float* vin, float* vout, int& s_m, int s_c, int s_p,
+                    const F4& u0in, F4& u0out) {
         const bool full = jc < j1;
         const bool out = jc > j0;
-        F4 u0{0.f, 0.f, 0.f, 0.f};
-        if (out && ca != 0.f && out_lane) {
-            const long q = (long)(jc - 1) * nx + c;
-            u0 = {U0[0][q], U0[1][q], U0[2][q], U0[3][q]};
+        const long rb = (long)(jc - 1) * nx;    // wave-uniform row base
+        const unsigned coff = (unsigned)c * 4u;  // meaningful on output lanes only
+        const unsigned rowb = (unsigned)nx * 4u;
+        const F4 u0 = u0in;
+        // U0 of row jc (the next step's output row): loaded a whole row step ahead of its use
+        if (full && ca != 0.f && out_lane) {
+            const long rn = (long)jc * nx;
+            u0out = {row_ld(U0[0] + rn, rowb, coff), row_ld(U0[1] + rn, rowb, coff), row_ld(U0[2] + rn, rowb, coff),
+                     row_ld(U0[3] + rn, rowb, coff)};
         }
         dout = sub4(up, uc);
         Prep lo_p;
@@ -189,9 +216,9 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
             Prep W_p, E_p;
             if (MUSCL) {
                 const F4 ul = from_left4(uc), ur = from_right4(uc);
-                const F4 sx = slope4<LIM>(sub4(ur, uc), sub4(uc, ul));
-                W_p = fast::prep(xW_rec ? axpy4(uc, -0.5f, sx) : uc, g, gm1);
-                E_p = fast::prep(xE_rec ? axpy4(uc, 0.5f, sx) : uc, g, gm1);
+                const F4 sx = half_slope4<LIM>(sub4(ur, uc), sub4(uc, ul));
+                W_p = fast::prep(axpy4(uc, hW, sx), g, gm1);
+                E_p = fast::prep(axpy4(uc, hE, sx), g, gm1);
             } else {
                 W_p = lo_p;
                 E_p = lo_p;
@@ -226,7 +253,7 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
             }
         }
         if (out && out_lane) {
-            const long q = (long)(jc - 1) * nx + c;
+            const long q = rb + c;
             float rr = xin.r * dtix + (gin.r - gout.r) * dtiy;
             float rru = xin.ru * dtix + (gin.ru - gout.ru) * dtiy;
             float rrv = xin.rv * dtix + (gin.rv - gout.rv) * dtiy;
@@ -244,7 +271,8 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
             } else {
                 o0 = um.r + rr; o1 = um.ru + rru; o2 = um.rv + rrv; o3 = um.re + rre;
             }
-            O[0][q] = o0; O[1][q] = o1; O[2][q] = o2; O[3][q] = o3;
+            row_st(O[0] + rb, rowb, coff, o0); row_st(O[1] + rb, rowb, coff, o1); row_st(O[2] + rb, rowb, coff, o2);
+            row_st(O[3] + rb, rowb, coff, o3);
             if (fuse) {
                 const float ir = fast::rcp(o0);
                 const float vu = fabsf(o1 * ir), vv = fabsf(o2 * ir);
@@ -277,6 +305,7 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
     Prep H0, H1;
     F4 G0{0.f, 0.f, 0.f, 0.f}, G1{0.f, 0.f, 0.f, 0.f}, X0{0.f, 0.f, 0.f, 0.f}, X1{0.f, 0.f, 0.f, 0.f};
     float V0[3] = {0.f, 0.f, 0.f}, V1[3] = {0.f, 0.f, 0.f};
+    F4 Q0{0.f, 0.f, 0.f, 0.f}, Q1{0.f, 0.f, 0.f, 0.f};   // U0 rows in flight (ping-pong)
     {
         const F4 rm2 = load_row<FEAT>(a, U, c, cq, xkind, j0 - 2);
         D0 = sub4(R1, R0);                 // U(j0) - U(j0-1): forward difference of row j0-1
@@ -285,13 +314,13 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
     }
     // ---- steady state: rows j0 .. j1, unrolled x4 with rotated roles ----
     for (int jc = j0; jc <= j1; jc += 4) {
-        step(jc, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, V0, V1, S0, S1, S2);
+        step(jc, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, V0, V1, S0, S1, S2, Q0, Q1);
         if (jc + 1 > j1) break;
-        step(jc + 1, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, V1, V0, S1, S2, S3);
+        step(jc + 1, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, V1, V0, S1, S2, S3, Q1, Q0);
         if (jc + 2 > j1) break;
-        step(jc + 2, R2, R3, R0, D0, D1, H0, H1, G0, G1, X0, X1, V0, V1, S2, S3, S0);
+        step(jc + 2, R2, R3, R0, D0, D1, H0, H1, G0, G1, X0, X1, V0, V1, S2, S3, S0, Q0, Q1);
         if (jc + 3 > j1) break;
-        step(jc + 3, R3, R0, R1, D1, D0, H1, H0, G1, G0, X1, X0, V1, V0, S3, S0, S1);
+        step(jc + 3, R3, R0, R1, D1, D0, H1, H0, G1, G0, X1, X0, V1, V0, S3, S0, S1, Q1, Q0);
     }
 
     if (fuse) {
