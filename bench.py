@@ -32,7 +32,7 @@ METRIC = "cell-updates/s per RK stage, % HBM peak; time-to-L1-error learned vs M
 BYTES_PER_CELL_STEP = {"fp32": 4 * 4 * (2 + 3 + 3), "fp64": 8 * 4 * (2 + 3 + 3)}
 
 
-def cpu_baseline(n_cpu: int = 1024, steps: int = 2):
+def cpu_baseline(n_cpu: int = 1024, steps: int = 14):
     """Time the oracle (single thread, -O3, FP64) on an n_cpu^2 sample of the same workload."""
     from oracle import orc
     from dev_field_efvm_amd import cases
@@ -47,6 +47,27 @@ def cpu_baseline(n_cpu: int = 1024, steps: int = 2):
     return {"value": n_cpu * n_cpu * 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": f"oracle/efvm_oracle.c FP64 MUSCL-MC+HLLC+RK3, periodic {n_cpu}^2 vortex+modes, {steps} steps "
                       f"({dt:.1f} s, 1 thread, {os.cpu_count()} host cores visible)"}
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """Achievable HBM bandwidth on this device: a 1 GiB device-to-device copy (read + write
+    bytes / time, HIP events on torch's stream), best of `reps`."""
+    import torch
+    a = torch.ones(nbytes // 4, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del a, b
+    torch.cuda.empty_cache()
+    return round(2 * nbytes / (best * 1e-3) / 1e9, 1)
 
 
 def load_traffic(n: int, prec: str):
@@ -151,11 +172,13 @@ def bench_learned(args):
                              progress=lambda d, s: print(f"truth {truth_n}^2: {d}/{s} steps", file=sys.stderr, flush=True))
     ref = learned.box(truth["rho"], truth_n // n)
     runs = {}
-    for name, nn, m in (("learned", n, net), ("muscl", n, None), ("muscl_fine", fine, None)):
-        r = learned.run_case(seed, nn, T, m)
+    for name, nn, m, byp in (("learned", n, net, False), ("learned_bypass", n, net, True), ("muscl", n, None, False),
+                             ("muscl_fine", fine, None, False)):
+        r = learned.run_case(seed, nn, T, m, bypass=byp)
         runs[name] = dict(l1=float(np.mean(np.abs(learned.box(r["rho"], nn // n) - ref))), steps=r["steps"],
-                          gpu_ms=round(r["gpu_ms"], 3), fallback_frac=(r["fallback"] / r["faces"]) if r["faces"] else 0.0)
-    L, F = runs["learned"], runs["muscl_fine"]
+                          gpu_ms=round(r["gpu_ms"], 3), fallback_frac=(r["fallback"] / r["faces"]) if r["faces"] else 0.0,
+                          gated_frac=(r["gated_sides"] / (2 * r["faces"])) if r["faces"] else 0.0)
+    L, F, B = runs["learned"], runs["muscl_fine"], runs["learned_bypass"]
     speedup = F["gpu_ms"] / L["gpu_ms"]
     # matrix-core work of the network per learned stage: 4 face sides per cell, 20-64-64-4
     flops_stage = 4 * n * n * 2 * (20 * 64 + 64 * 64 + 64 * 4)
@@ -169,10 +192,13 @@ def bench_learned(args):
         "config": {"workload": f"learned reconstruction {n}^2 vs MUSCL {n}^2 / {fine}^2, t = {T}, truth FP64 MUSCL {truth_n}^2",
                    "l1_learned": L["l1"], "l1_muscl": runs["muscl"]["l1"], "l1_muscl_fine": F["l1"],
                    "reaches_fine_l1": L["l1"] <= F["l1"], "l1_gain_vs_muscl": runs["muscl"]["l1"] / L["l1"],
-                   "gpu_time_ratio_learned_over_fine": L["gpu_ms"] / F["gpu_ms"], "runs": runs},
+                   "gpu_time_ratio_learned_over_fine": L["gpu_ms"] / F["gpu_ms"],
+                   "bypass": {"reaches_fine_l1": B["l1"] <= F["l1"], "gpu_time_ratio_over_fine": B["gpu_ms"] / F["gpu_ms"],
+                              "note": "network gate evaluated only where the ungated face value is near the TVD bound"},
+                   "runs": runs},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(achieved / 2500.0, 5), "traffic": None,
-                     "kernel": "learned stage (face-state MLP kernel + face stage kernel), bf16 dense peak",
+                     "kernel": "learned stage (face kernel: MLP + fluxes; update kernel), bf16 dense peak",
                      "mean_stage_us": round(stage_s * 1e6, 2), "network_flops_per_stage": flops_stage},
     }
 
@@ -188,12 +214,12 @@ def bench_learned_stage(args):
     torch.cuda.synchronize()
     net = Mlp(learned.load_weights())
     out = {}
-    for name, n, use in (("learned", args.learned_n, True), ("muscl", args.learned_n, False),
-                         ("muscl_fine", 4 * args.learned_n, False)):
+    for name, n, use in (("learned", args.learned_n, 1), ("learned_bypass", args.learned_n, 2),
+                         ("muscl", args.learned_n, 0), ("muscl_fine", 4 * args.learned_n, 0)):
         c = learned.periodic_context(n)
         c.upload(learned.initial_state(args.learned_seed, n))
         if use:
-            c.set_learned(net)
+            c.set_learned(net, bypass=use == 2)
         else:
             tune_context(c, steps=2, use_cache=False, store=False)
         c.step(2)
@@ -310,10 +336,17 @@ def main():
     ctx = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, flux="hllc", muscl=True, limiter="mc",
                        integrator="rk3", precision=args.precision, cfl=0.4)
     ctx.set_boundaries(["periodic"] * 4)
-    if args.strip_rows or args.waves:
-        ctx.set_launch(args.strip_rows, args.waves)
     ctx.upload(st)
     del st
+    if args.strip_rows or args.waves:
+        ctx.set_launch(args.strip_rows, args.waves)
+        tuned = {"rows": args.strip_rows, "waves": args.waves, "source": "command line"}
+    else:
+        # launch shape from the autotuner (cached per architecture / precision / size class; it
+        # times real steps and restores the state from a device snapshot afterwards)
+        from dev_field_efvm_amd.autotune import tune_context
+        r, w, _ = tune_context(ctx, steps=2, use_cache=not args.retune)
+        tuned = {"rows": r, "waves": w, "source": "autotuner"}
 
     # warm-up, then the timed region: K whole steps, bracketed by device synchronisation
     ctx.step(args.warmup)
@@ -339,6 +372,7 @@ def main():
     ev_ms = ctx.last_elapsed_ms()
     clk = ctx.clock()
 
+    copy_gbs = copy_bandwidth()
     cells = n * n
     value = cells * 3 * args.steps / wall
     launches = 3 * args.steps
@@ -348,7 +382,7 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": load_traffic(n, args.precision),
             "kernel": f"stage_{args.precision}_kernel", "mean_launch_us": round(mean_launch_s * 1e6, 2),
-            "algorithmic_bytes_per_launch": bytes_per_launch}
+            "algorithmic_bytes_per_launch": bytes_per_launch, "measured_copy_GBps": copy_gbs}
     out = {
         "metric": METRIC,
         "value": value,
@@ -363,7 +397,8 @@ def main():
         "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic (isentropic vortex + 8 seeded Fourier modes, host FP64)",
         "config": {"workload": f"2-D Euler MUSCL(MC)+HLLC+SSP-RK3, periodic {n}x{n}, CFL 0.4, fused FP64 CFL",
-                   "grid": [n, n], "graph": bool(use_graph), "sim_time": clk["time"], "parallelism": "single GPU"},
+                   "grid": [n, n], "graph": bool(use_graph), "sim_time": clk["time"], "launch": tuned,
+                   "parallelism": "single GPU"},
         "roofline": roof,
     }
     if not args.no_cpu:

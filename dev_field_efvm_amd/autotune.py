@@ -18,7 +18,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE_PATH = os.environ.get("EFVM_TUNE_CACHE", os.path.join(HERE, "tuned.json"))
 
-CTX_ROWS = (8, 16, 32, 64, 128, 256)
+CTX_ROWS = (8, 16, 24, 28, 32, 40, 48, 64, 128, 256)
 CTX_WAVES = (1, 2, 4)
 BATCH_ROWS = (8, 16, 32, 64, 128)
 BATCH_BLOCKS = (0, 512, 1024, 2048, 4096)

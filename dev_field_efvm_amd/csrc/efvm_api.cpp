@@ -39,8 +39,10 @@ template <class T> hipError_t launch_sponge(const StageArgs& a, double x0, doubl
                                             double rho, double p, hipStream_t st);
 template <class T> hipError_t launch_sums(const StageArgs& a, double* part, int nb, double vol, double* out, hipStream_t st);
 hipError_t launch_ibm_flip(void* const* planes, int esz, const uint8_t* solid, long n, const double* stagnant, hipStream_t st);
-hipError_t launch_stage_f32_faces(const StageArgs& a, const float* F, int block, hipStream_t st);
+hipError_t launch_stage_f32_faces(const StageArgs& a, const float* F, hipStream_t st);
+hipError_t launch_lam_reduce(const double* part, int n, double* target, hipStream_t st);
 hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st);
+hipError_t launch_mlp_faces_bypass(const mlp::FaceArgs& a, hipStream_t st);
 hipError_t launch_mlp_infer(const mlp::Packed* P, const float* X, long n, float* Y, hipStream_t st);
 namespace mlp { struct TrainArgs; struct SampleArgs; }
 hipError_t launch_mlp_train(const mlp::TrainArgs& a, int nblk, float* grad, double* loss_out, hipStream_t st);
@@ -64,6 +66,7 @@ struct efvm_ctx {
     void* P[3][4] = {};            // plane sets A, B, C
     int cur = 0;                   // index of the set holding U^n (A)
     DevClock* clock = nullptr;
+    double* lam_part = nullptr;    // FP32: per-wave maxima of the fused CFL
     double* yw = nullptr;
     uint8_t* solid = nullptr;
     int* ghost_idx = nullptr;
@@ -97,9 +100,11 @@ struct efvm_ctx {
     void* snap = nullptr;          // autotuner snapshot (state + clock)
     int snap_parity = 0, snap_lam = 0, snap_flip = 0;
     int learned = 0;               // learned face reconstruction attached
+    int learned_bypass = 0;        // gate evaluated only where the ungated face value is near the TVD bound
     float* faces = nullptr;        // 16 planes of face primitives
+    unsigned* dlist = nullptr;     // bypass: deferred faces (2 nx ny entries) and their count
     mlp::Packed* wpack = nullptr;  // the attached network's inference weights
-    unsigned long long* lstats = nullptr;   // [fallback faces, faces evaluated]
+    unsigned long long* lstats = nullptr;   // [fallback faces, faces evaluated, sides the network gated]
     std::string err;
 };
 
@@ -158,6 +163,7 @@ StageArgs base_args(const efvm_ctx* c) {
     a.n_strips_x = (g.nx + STRIP_OUT - 1) / STRIP_OUT;
     a.n_strips_y = (g.ny + c->strip_h - 1) / c->strip_h;
     a.xcd_remap = c->xcd_remap;
+    a.lam_part = c->lam_part;
     return a;
 }
 
@@ -234,17 +240,21 @@ efvm_status run_substage(efvm_ctx* c) {
         fa.F = c->faces;
         fa.w = c->wpack;
         fa.stats = c->lstats;
-        hipError_t e = launch_mlp_faces(fa, c->stream);
-        if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("face-state kernel: ") + hipGetErrorString(e));
-        // strips of 63 columns; rows per strip so that about 4096 strips fill the 256 CUs
-        // (each strip row is only two fluxes, there is no stencil to amortise)
-        a.n_strips_x = (c->grid.nx + WAVE - 2) / (WAVE - 1);
-        a.strip_h = std::max(1, std::min(c->strip_h, (int)((long)a.n_strips_x * c->grid.ny / 4096)));
-        a.n_strips_y = (c->grid.ny + a.strip_h - 1) / a.strip_h;
-        e = launch_stage_f32_faces(a, c->faces, WAVE * c->waves_per_block, c->stream);
+        fa.bypass = c->learned_bypass;
+        fa.list = c->dlist;
+        fa.n_list = c->dlist ? c->dlist + 2 * c->ncell : nullptr;
+        hipError_t e = c->learned_bypass ? launch_mlp_faces_bypass(fa, c->stream) : launch_mlp_faces(fa, c->stream);
+        if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("face kernels: ") + hipGetErrorString(e));
+        e = launch_stage_f32_faces(a, c->faces, c->stream);
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("learned stage kernel: ") + hipGetErrorString(e));
     } else if ((st = launch_stage(c, a)) != EFVM_OK) {
         return st;
+    }
+    if (a.fuse_cfl && a.lam_part) {
+        // the stage left one CFL maximum per wave: strips, or 64 cells of the learned stage
+        const int nw = c->learned ? (int)((c->ncell + 63) / 64) : a.n_strips_x * a.n_strips_y;
+        hipError_t e = launch_lam_reduce(a.lam_part, nw, &c->clock->lam[a.parity ^ 1], c->stream);
+        if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("cfl reduction: ") + hipGetErrorString(e));
     }
     if (last) {
         if (!rk3) c->cur ^= 1;   // Euler: U^{n+1} is in B
@@ -317,6 +327,12 @@ efvm_status efvm_ctx_create(const efvm_grid* grid, const efvm_gas* gas, const ef
     for (int s = 0; s < 3 && st == EFVM_OK; s++)
         for (int m = 0; m < 4 && st == EFVM_OK; m++) st = device_alloc(c, &c->P[s][m], c->esz * c->ncell);
     if (st == EFVM_OK) st = device_alloc(c, (void**)&c->clock, sizeof(DevClock));
+    if (st == EFVM_OK && c->esz == 4) {
+        // slots for the most waves a fused-CFL stage can launch: strips of one row, or 64 cells
+        const long strips = (long)((grid->nx + STRIP_OUT - 1) / STRIP_OUT) * grid->ny;
+        const long n = std::max(strips, (c->ncell + 63) / 64) + 64;
+        st = device_alloc(c, (void**)&c->lam_part, sizeof(double) * n);
+    }
     if (st == EFVM_OK) st = device_alloc(c, (void**)&c->yw, sizeof(double) * 3 * grid->ny);
     if (st == EFVM_OK) st = device_alloc(c, (void**)&c->scratch, sizeof(double) * (1024 * 4 + 16));
     if (st != EFVM_OK) { efvm_ctx_destroy(c); return st; }
@@ -343,12 +359,14 @@ void efvm_ctx_destroy(efvm_ctx* c) {
     drop_graph(c);
     for (int s = 0; s < 3; s++) for (int m = 0; m < 4; m++) if (c->P[s][m]) hipFree(c->P[s][m]);
     if (c->clock) hipFree(c->clock);
+    if (c->lam_part) (void)hipFree(c->lam_part);
     if (c->yw) hipFree(c->yw);
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->snap) (void)hipFree(c->snap);
     if (c->faces) (void)hipFree(c->faces);
     if (c->wpack) (void)hipFree(c->wpack);
     if (c->lstats) (void)hipFree(c->lstats);
+    if (c->dlist) (void)hipFree(c->dlist);
     free_ibm(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -1510,8 +1528,8 @@ efvm_status efvm_set_learned(efvm_ctx* c, efvm_mlp* m) {
     if (!c->faces) HIPCHK(c, hipMalloc((void**)&c->faces, sizeof(float) * 16 * c->ncell));
     if (!c->wpack) HIPCHK(c, hipMalloc((void**)&c->wpack, sizeof(mlp::Packed)));
     if (!c->lstats) {
-        HIPCHK(c, hipMalloc((void**)&c->lstats, 2 * sizeof(unsigned long long)));
-        HIPCHK(c, hipMemsetAsync(c->lstats, 0, 2 * sizeof(unsigned long long), c->stream));
+        HIPCHK(c, hipMalloc((void**)&c->lstats, 3 * sizeof(unsigned long long)));
+        HIPCHK(c, hipMemsetAsync(c->lstats, 0, 3 * sizeof(unsigned long long), c->stream));
     }
     HIPCHK(c, hipMemcpyAsync(c->wpack, m->packed, sizeof(mlp::Packed), hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(c, hipStreamSynchronize(m->stream));
@@ -1520,12 +1538,22 @@ efvm_status efvm_set_learned(efvm_ctx* c, efvm_mlp* m) {
     return EFVM_OK;
 }
 
-efvm_status efvm_mlp_infer_stats(efvm_ctx* c, int reset, unsigned long long out[2]) {
+efvm_status efvm_set_learned_bypass(efvm_ctx* c, int bypass) {
+    if (!c || (bypass != 0 && bypass != 1)) return EFVM_ERR_ARG;
+    if (bypass && 2.0 * (double)c->ncell >= 4294967295.0)
+        return fail(c, EFVM_ERR_UNSUPPORTED, "bypass face list indexes faces in 32 bits: grid too large");
+    drop_graph(c);
+    if (bypass && !c->dlist) HIPCHK(c, hipMalloc((void**)&c->dlist, sizeof(unsigned) * (2 * c->ncell + 1)));
+    c->learned_bypass = bypass;
+    return EFVM_OK;
+}
+
+efvm_status efvm_mlp_infer_stats(efvm_ctx* c, int reset, unsigned long long out[3]) {
     if (!c || !out) return EFVM_ERR_ARG;
-    out[0] = out[1] = 0;
+    out[0] = out[1] = out[2] = 0;
     if (!c->lstats) return EFVM_OK;
-    HIPCHK(c, hipMemcpyAsync(out, c->lstats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
-    if (reset) HIPCHK(c, hipMemsetAsync(c->lstats, 0, 2 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, c->lstats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    if (reset) HIPCHK(c, hipMemsetAsync(c->lstats, 0, 3 * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;
 }
@@ -1544,6 +1572,9 @@ efvm_status efvm_learned_faces(efvm_ctx* c, float* out, size_t n) {
     fa.F = c->faces;
     fa.w = c->wpack;
     fa.stats = nullptr;
+    fa.bypass = c->learned_bypass;
+    fa.list = nullptr;
+    fa.n_list = nullptr;
     HIPCHK(c, launch_mlp_faces(fa, c->stream));
     HIPCHK(c, hipMemcpyAsync(out, c->faces, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

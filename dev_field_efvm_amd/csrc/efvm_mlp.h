@@ -81,9 +81,15 @@ struct FaceArgs {
     int n_tasks;                 // 2 * ny * n_chunks (x faces, then y faces)
     float gamma;
     const float* U[4];           // conserved planes of the stage input
-    float* F;                    // 16 planes: ((axis * 2 + side) * 4 + var) * nx*ny, primitives
+    float* F;                    // 16 planes: ((axis * 2 + side) * 4 + var) * nx*ny, face primitives
     const Packed* w;
-    unsigned long long* stats;   // [faces that fell back to MUSCL, faces evaluated], accumulated
+    unsigned long long* stats;   // [faces that fell back to MUSCL, faces evaluated, face sides
+                                 // whose gate the network evaluated], accumulated
+    int bypass;                  // 1: the network's gate is evaluated only for face sides whose
+                                 // ungated value fails the TVD bound with half the relaxation;
+                                 // elsewhere the gate is fully open (phi = 1)
+    unsigned* list;              // deferred faces of the bypass passes: axis * nx*ny + row * nx + column
+    unsigned* n_list;            // their count (device)
 };
 
 // one training minibatch (loss and gradient)

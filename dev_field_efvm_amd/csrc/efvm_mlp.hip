@@ -103,6 +103,7 @@ __device__ __forceinline__ float gate(float z) {
 // features as bf16 into the slot (dwords 0..9, plus the constant feature 20 = 1 in dword
 // 10), returns sigma, the centre, the bound tolerance, the FP32 central-slope base yb and
 // the FP32 learned correction sk = L x + bL (which the MLP's gate scales).
+template <bool SLOT_OUT = true>
 __device__ __forceinline__ void featurize(const float ws[5][4], float gamma, const float* skip, uint32_t* slot,
                                           float sig[4], float cen[4], float yb[4], float sk[4], float tol[4]) {
     float x[20];
@@ -132,6 +133,7 @@ __device__ __forceinline__ void featurize(const float ws[5][4], float gamma, con
         yb[v] = 0.25f * (x[12 + v] - x[4 + v]);
         sk[v] = s;
     }
+    if (!SLOT_OUT) return;
     const u32x4 q0 = {pack2(x[0], x[1]), pack2(x[2], x[3]), pack2(x[4], x[5]), pack2(x[6], x[7])};
     const u32x4 q1 = {pack2(x[8], x[9]), pack2(x[10], x[11]), pack2(x[12], x[13]), pack2(x[14], x[15])};
     *(u32x4*)(slot) = q0;
@@ -142,16 +144,17 @@ __device__ __forceinline__ void featurize(const float ws[5][4], float gamma, con
 }
 
 // Positivity and the TVD bound relaxed at smooth extrema: every variable of the oriented
-// face value within [min(w0, w1) - d, max(w0, w1) + d], d = |minmod(curvatures)| / 2 + tol
-// (tol = the feature floor, below which offsets are not resolved).
-__device__ __forceinline__ bool side_ok(const float ws[5][4], const float f[4], const float tol[4]) {
+// face value within [min(w0, w1) - d, max(w0, w1) + d], d = relax |minmod(curvatures)| + tol
+// (relax = 1/2; tol = the feature floor, below which offsets are not resolved).
+__device__ __forceinline__ bool side_ok(const float ws[5][4], const float f[4], const float tol[4],
+                                        float relax = 0.5f) {
     bool ok = f[0] > 0.f && f[3] > 0.f;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         const float w0 = ws[2][v], w1 = ws[3][v];
         const float c0 = w1 - 2.f * w0 + ws[1][v];
         const float c1 = ws[4][v] - 2.f * w1 + w0;
-        const float d = (c0 * c1 > 0.f ? 0.5f * fminf(fabsf(c0), fabsf(c1)) : 0.f) + tol[v];
+        const float d = (c0 * c1 > 0.f ? relax * fminf(fabsf(c0), fabsf(c1)) : 0.f) + tol[v];
         ok = ok && f[v] >= fminf(w0, w1) - d && f[v] <= fmaxf(w0, w1) + d;
     }
     return ok;
@@ -168,19 +171,38 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // The 6 cells m = -2..3 straddling the lane's face along `axis`: conserved q, primitives w.
-__device__ __forceinline__ void load_cells(const FaceArgs& a, int axis, int row, int cc, float q[6][4], float w[6][4]) {
-    const float gm1 = a.gamma - 1.f;
+__device__ __forceinline__ void load_raw(const FaceArgs& a, int axis, int row, int cc, float q[6][4]) {
 #pragma unroll
     for (int m = 0; m < 6; m++) {
         const long idx = axis == 0 ? (long)row * a.nx + wrap(cc + m - 2, a.nx) : (long)wrap(row + m - 2, a.ny) * a.nx + cc;
 #pragma unroll
         for (int v = 0; v < 4; v++) q[m][v] = a.U[v][idx];
+    }
+}
+__device__ __forceinline__ void to_prim(const float q[6][4], float gm1, float w[6][4]) {
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
         const float ir = fast::rcp(q[m][0]);
         w[m][0] = q[m][0];
         w[m][1] = q[m][1] * ir;
         w[m][2] = q[m][2] * ir;
         w[m][3] = gm1 * (q[m][3] - 0.5f * (q[m][1] * w[m][1] + q[m][2] * w[m][2]));
     }
+}
+__device__ __forceinline__ void load_cells(const FaceArgs& a, int axis, int row, int cc, float q[6][4], float w[6][4]) {
+    load_raw(a, axis, row, cc, q);
+    to_prim(q, a.gamma - 1.f, w);
+}
+
+// task -> (axis, row, column of the lane's face); valid = the column exists
+__device__ __forceinline__ void task_face(const FaceArgs& a, int task, int lane, int& axis, int& row, int& cc, bool& valid) {
+    const int half = a.ny * a.n_chunks;
+    axis = task >= half ? 1 : 0;
+    const int rem = task - axis * half;
+    row = rem / a.n_chunks;
+    const int ci = (rem - row * a.n_chunks) * 64 + lane;
+    valid = ci < a.nx;
+    cc = valid ? ci : a.nx - 1;
 }
 
 // oriented stencil of side sd: 0 = the cell m=0 toward +, 1 = the cell m=1 toward -
@@ -197,68 +219,102 @@ __device__ __forceinline__ void orient(const float w[6][4], int axis, int sd, fl
     }
 }
 
-// One wave = one task = 64 consecutive faces of one row (x faces, east of cells
-// chunk*64 + lane) or of one face row (y faces, north of those cells).  Each lane
-// loads the 6 cells straddling its face, builds both sides' features (left side =
-// its cell oriented +, right side = the neighbour oriented -), and the wave runs the
-// MLP over its 128 face sides in 8 matrix-core blocks of 16.  The stencil is read
-// again for the checks after the matrix-core phase, which keeps it out of the
-// registers the network needs.
-__global__ __launch_bounds__(256) void faces_kernel(FaceArgs a) {
-    __shared__ __attribute__((aligned(16))) uint32_t lds[4][128][SLOT];
-    __shared__ float skip[SK_N];
-    stage_skip(a.w, skip);
+// Face kernels.  A task is 64 faces: in PASS_ALL / PASS_LIGHT 64 consecutive faces of
+// one row (x faces, east of cells chunk*64 + lane) or of one face row (y faces, north
+// of those cells); in PASS_LIST 64 entries of the deferred-face list.  Each lane loads
+// the 6 cells straddling its face and builds both sides' features (left side = its
+// cell oriented +, right side = the neighbour oriented -).
+//  * PASS_ALL: the network gates every face side (with a.bypass, only the sides whose
+//    ungated value fails the TVD bound with half its relaxation), in 8 matrix-core
+//    blocks of 16 sides; then the checks and the MUSCL fallback.  Register-bound by the
+//    network's operands: 2 waves per SIMD.
+//  * PASS_LIGHT (bypass stage, no network): faces whose two sides pass the strict bound
+//    are final with the gate fully open (the strict bound implies the check); faces
+//    with a side that fails it are appended to the deferred list.  No network
+//    registers, so it runs at full occupancy.
+//  * PASS_LIST: the deferred faces, as PASS_ALL with bypass; a persistent grid walks the
+//    list up to the count PASS_LIGHT left in device memory.
+// All passes write the face primitives (16 planes) the stage kernel reads.
+enum { PASS_ALL = 0, PASS_LIST = 2 };
+
+template <int PASS>   // PASS_ALL or PASS_LIST (PASS_LIGHT is faces_light_kernel)
+__device__ __forceinline__ void faces_body(const FaceArgs& a, const float* skip, uint32_t (*feat)[SLOT], const Regs& R) {
     const int lane = lane_id(), wv = threadIdx.x >> 6;
     const int g = lane >> 4, jl = lane & 15;
-    uint32_t(*feat)[SLOT] = lds[wv];
-    for (int e = lane; e < 128; e += 64)
-        for (int d = 10; d < SLOT; d++) feat[e][d] = 0u;
-    Regs R;
-    load_regs(a.w, lane, R);
     const float gm = a.gamma, gm1 = a.gamma - 1.f;
     const long np = (long)a.nx * a.ny;
-    unsigned long long nfb = 0, ntot = 0;
+    const bool bypass = PASS != PASS_ALL || a.bypass;
+    unsigned long long nfb = 0, ntot = 0, nnet = 0;
     const int half = a.ny * a.n_chunks;
-    for (int task = blockIdx.x * 4 + wv; task < a.n_tasks; task += gridDim.x * 4) {
-        const int axis = task >= half ? 1 : 0;
-        const int rem = task - axis * half;
-        const int row = rem / a.n_chunks, chunk = rem - row * a.n_chunks;
-        const int ci = chunk * 64 + lane;
-        const bool valid = ci < a.nx;
-        const int cc = valid ? ci : a.nx - 1;
+    const int n_tasks = PASS == PASS_LIST ? (int)((*a.n_list + 63u) / 64u) : a.n_tasks;
+    for (int task = blockIdx.x * 4 + wv; task < n_tasks; task += gridDim.x * 4) {
+        int axis, row, cc;
+        bool valid;
+        if (PASS == PASS_LIST) {
+            const unsigned e = (unsigned)task * 64u + (unsigned)lane;
+            valid = e < *a.n_list;
+            const unsigned f = valid ? a.list[e] : 0u;          // axis * nx*ny + row * nx + column
+            axis = f >= (unsigned)np ? 1 : 0;
+            const unsigned r = f - (unsigned)(axis * np);
+            row = (int)(r / (unsigned)a.nx);
+            cc = (int)(r - (unsigned)row * (unsigned)a.nx);
+        } else {
+            axis = task >= half ? 1 : 0;
+            const int rem = task - axis * half;
+            row = rem / a.n_chunks;
+            const int ci = (rem - row * a.n_chunks) * 64 + lane;
+            valid = ci < a.nx;
+            cc = valid ? ci : a.nx - 1;
+        }
         float sig[2][4], cen[2][4], y[2][4], sk[2][4], tol[2][4];
+        bool need0 = true, need1 = true;   // the side's gate is evaluated by the network
         {
-            float q[6][4], w[6][4], ws[5][4];
+            float q[6][4], w[6][4], ws[5][4], f1[4];
             load_cells(a, axis, row, cc, q, w);
             orient(w, axis, 0, ws);
             featurize(ws, gm, skip, feat[lane], sig[0], cen[0], y[0], sk[0], tol[0]);
+            if (bypass) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) f1[v] = fmaf(sig[0][v], y[0][v] + sk[0][v], cen[0][v]);
+                need0 = !side_ok(ws, f1, tol[0], 0.25f);
+            }
             orient(w, axis, 1, ws);
             featurize(ws, gm, skip, feat[64 + lane], sig[1], cen[1], y[1], sk[1], tol[1]);
-        }
-        wave_sync();
-        f32x4 o0 = {0.f, 0.f, 0.f, 0.f}, o1 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
-        for (int b = 0; b < 8; b++) {
-            const int sd = b >> 2, qd = b & 3;
-            const bf16x8 bx = __builtin_bit_cast(bf16x8, *(const u32x4*)&feat[sd * 64 + qd * 16 + jl][4 * g]);
-            const f32x4 d3 = mlp16(R, bx);
-            if (g == qd) {
-                if (sd == 0) o0 = d3; else o1 = d3;
+            if (bypass) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) f1[v] = fmaf(sig[1][v], y[1][v] + sk[1][v], cen[1][v]);
+                need1 = !side_ok(ws, f1, tol[1], 0.25f);
             }
         }
-        wave_sync();
-        // face values (oriented), checks, fallback
+        const long cell = (long)row * a.nx + cc;
         float fo[2][4];
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const float b3 = skip[SK_B3 + v];
-            fo[0][v] = fmaf(sig[0][v], fmaf(gate(o0[v] + b3), sk[0][v], y[0][v]), cen[0][v]);
-            fo[1][v] = fmaf(sig[1][v], fmaf(gate(o1[v] + b3), sk[1][v], y[1][v]), cen[1][v]);
-        }
+        bool ok = true;
         float q[6][4], w[6][4];
-        load_cells(a, axis, row, cc, q, w);
-        bool ok;
         {
+            wave_sync();
+            const unsigned long long nm0 = __ballot(valid && need0), nm1 = __ballot(valid && need1);
+            nnet += __popcll(nm0) + __popcll(nm1);
+            f32x4 o0 = {0.f, 0.f, 0.f, 0.f}, o1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+            for (int b = 0; b < 8; b++) {
+                const int sd = b >> 2, qd = b & 3;
+                if ((((sd == 0 ? nm0 : nm1) >> (16 * qd)) & 0xffffull) == 0ull) continue;   // no side needs the gate
+                const bf16x8 bx = __builtin_bit_cast(bf16x8, *(const u32x4*)&feat[sd * 64 + qd * 16 + jl][4 * g]);
+                const f32x4 d3 = mlp16(R, bx);
+                if (g == qd) {
+                    if (sd == 0) o0 = d3; else o1 = d3;
+                }
+            }
+            wave_sync();
+            // face values (oriented), checks
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const float b3 = skip[SK_B3 + v];
+                const float p0 = need0 ? gate(o0[v] + b3) : 1.f, p1 = need1 ? gate(o1[v] + b3) : 1.f;
+                fo[0][v] = fmaf(sig[0][v], fmaf(p0, sk[0][v], y[0][v]), cen[0][v]);
+                fo[1][v] = fmaf(sig[1][v], fmaf(p1, sk[1][v], y[1][v]), cen[1][v]);
+            }
+            load_cells(a, axis, row, cc, q, w);
             float ws[5][4];
             orient(w, axis, 0, ws);
             ok = side_ok(ws, fo[0], tol[0]);
@@ -294,7 +350,6 @@ __global__ __launch_bounds__(256) void faces_kernel(FaceArgs a) {
         nfb += __popcll(__ballot(valid && !ok));
         ntot += __popcll(__ballot(valid));
         if (valid) {
-            const long cell = (long)row * a.nx + cc;
             float* Fl = a.F + (long)(axis * 2) * 4 * np + cell;
             float* Fr = Fl + 4 * np;
 #pragma unroll
@@ -304,11 +359,110 @@ __global__ __launch_bounds__(256) void faces_kernel(FaceArgs a) {
             }
         }
     }
-    if (a.stats && lane == 0 && ntot) {
+    if (a.stats && lane == 0 && (ntot || nnet)) {
         if (nfb) atomicAdd(a.stats, nfb);
-        atomicAdd(a.stats + 1, ntot);
+        if (ntot) atomicAdd(a.stats + 1, ntot);
+        if (nnet) atomicAdd(a.stats + 2, nnet);
     }
 }
+
+__device__ __forceinline__ void init_feat_slots(uint32_t (*feat)[SLOT]) {
+    for (int e = lane_id(); e < 128; e += 64)
+        for (int d = 10; d < SLOT; d++) feat[e][d] = 0u;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void faces_kernel(FaceArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t lds[4][128][SLOT];
+    __shared__ float skip[SK_N];
+    stage_skip(a.w, skip);
+    init_feat_slots(lds[threadIdx.x >> 6]);
+    Regs R;
+    load_regs(a.w, lane_id(), R);
+    faces_body<PASS_ALL>(a, skip, lds[threadIdx.x >> 6], R);
+}
+
+// PASS_LIGHT on its own: one side at a time, so only the stencil and the two face
+// values stay live (no network operands, no per-side feature state).
+// PASS_LIGHT on its own: one side at a time, so only the stencil and the two face
+// values stay live (no network operands, no per-side feature state).  Each wave walks
+// several tasks and issues the next task's loads before it works on the current one.
+__global__ __launch_bounds__(256) void faces_light_kernel(FaceArgs a) {
+    __shared__ float skip[SK_N];
+    stage_skip(a.w, skip);
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const float gm = a.gamma, gm1 = a.gamma - 1.f;
+    const long np = (long)a.nx * a.ny;
+    unsigned long long ntot = 0;
+    const int stride = gridDim.x * 4;
+    int task = blockIdx.x * 4 + wv;
+    float qn[6][4];
+    if (task < a.n_tasks) {
+        int ax, rw, c;
+        bool vl;
+        task_face(a, task, lane, ax, rw, c, vl);
+        load_raw(a, ax, rw, c, qn);
+    }
+    for (; task < a.n_tasks; task += stride) {
+        int axis, row, cc;
+        bool valid;
+        task_face(a, task, lane, axis, row, cc, valid);
+        float w[6][4], fo[2][4];
+        to_prim(qn, gm1, w);
+        if (task + stride < a.n_tasks) {          // prefetch the next task's cells
+            int ax, rw, c;
+            bool vl;
+            task_face(a, task + stride, lane, ax, rw, c, vl);
+            load_raw(a, ax, rw, c, qn);
+        }
+        bool defer = false;
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++) {
+            float ws[5][4], sig[4], cen[4], y[4], sk[4], tol[4];
+            orient(w, axis, sd, ws);
+            featurize<false>(ws, gm, skip, nullptr, sig, cen, y, sk, tol);
+#pragma unroll
+            for (int v = 0; v < 4; v++) fo[sd][v] = fmaf(sig[v], y[v] + sk[v], cen[v]);   // gate fully open
+            defer = defer || !side_ok(ws, fo[sd], tol, 0.25f);
+        }
+        // defer the faces with a side near the bound (wave-aggregated append)
+        defer = defer && valid;
+        const unsigned long long dm = __ballot(defer);
+        const long cell = (long)row * a.nx + cc;
+        if (dm) {
+            unsigned base = 0u;
+            if (lane == 0) base = atomicAdd(a.n_list, (unsigned)__popcll(dm));
+            base = __shfl(base, 0, WAVE);
+            if (defer) a.list[base + (unsigned)__popcll(dm & ((1ull << lane) - 1ull))] = (unsigned)(axis * np + cell);
+        }
+        valid = valid && !defer;
+        ntot += __popcll(__ballot(valid));
+        if (valid) {
+            // both sides pass the strict bound, hence the check: the face is final
+            const float unl = fo[0][1], unr = -fo[1][1];
+            float* Fl = a.F + (long)(axis * 2) * 4 * np + cell;
+            float* Fr = Fl + 4 * np;
+            Fl[0] = fo[0][0]; Fl[3 * np] = fo[0][3];
+            Fr[0] = fo[1][0]; Fr[3 * np] = fo[1][3];
+            Fl[np] = axis == 0 ? unl : fo[0][2];
+            Fl[2 * np] = axis == 0 ? fo[0][2] : unl;
+            Fr[np] = axis == 0 ? unr : fo[1][2];
+            Fr[2 * np] = axis == 0 ? fo[1][2] : unr;
+        }
+    }
+    if (a.stats && lane == 0 && ntot) atomicAdd(a.stats + 1, ntot);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void faces_list_kernel(FaceArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t lds[4][128][SLOT];
+    __shared__ float skip[SK_N];
+    stage_skip(a.w, skip);
+    init_feat_slots(lds[threadIdx.x >> 6]);
+    Regs R;
+    load_regs(a.w, lane_id(), R);
+    faces_body<PASS_LIST>(a, skip, lds[threadIdx.x >> 6], R);
+}
+
+__global__ void zero_count_kernel(unsigned* n) { *n = 0u; }
 
 // The matrix-core MLP alone on n given feature rows X (n x 20): Y = base + skip + MLP.
 __global__ __launch_bounds__(256) void infer_kernel(const Packed* __restrict__ P, const float* __restrict__ X, long n,
@@ -739,6 +893,19 @@ hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st) {
     // SIMD); each wave loads the network into registers once and loops over tasks
     const int blocks = std::min((a.n_tasks + 3) / 4, 512);
     mlp::faces_kernel<<<blocks, 256, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+// Bypass stage: light pass over every face, then the network over the deferred ones.
+// The list length stays on the device (the list pass's persistent grid reads it), so
+// the sequence can be captured in a graph; the counter is cleared by a kernel.
+hipError_t launch_mlp_faces_bypass(const mlp::FaceArgs& a, hipStream_t st) {
+    mlp::zero_count_kernel<<<1, 1, 0, st>>>(a.n_list);
+#ifndef EFVM_LIGHT_TPW
+#define EFVM_LIGHT_TPW 8
+#endif
+    mlp::faces_light_kernel<<<(a.n_tasks + 4 * EFVM_LIGHT_TPW - 1) / (4 * EFVM_LIGHT_TPW), 256, 0, st>>>(a);
+    mlp::faces_list_kernel<<<512, 256, 0, st>>>(a);
     return hipGetLastError();
 }
 

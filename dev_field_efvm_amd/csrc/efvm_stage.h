@@ -68,6 +68,7 @@ struct StageArgs {
     int fuse_cfl;
     int strip_h, n_strips_x, n_strips_y;
     int xcd_remap;
+    double* lam_part;            // FP32 fused CFL: per-wave maxima (null: atomics on the clock)
 };
 
 // One case of a batched ensemble: its stage arguments and plane sets A, B, C.
@@ -202,8 +203,13 @@ __device__ __forceinline__ Cell<T> load_cell(const StageArgs& a, const T* const*
 }
 
 __device__ __forceinline__ void atomic_max_pos_double(double* addr, double v) {
-    // non-negative doubles order like their bit patterns
-    atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+    // non-negative doubles order like their bit patterns.  The maximum only grows during a
+    // launch, so a value not above the current one cannot change it: skipping those keeps
+    // thousands of waves that finish together from serialising on one L2 atomic.
+    unsigned long long* p = reinterpret_cast<unsigned long long*>(addr);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    if (b <= __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    atomicMax(p, b);
 }
 
 // Wave-wide max of a double via shuffles (xor butterfly).

@@ -327,7 +327,8 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
         const double w = wave_max((double)lam_max);
         const long long b = wave_sum_ll((long long)bad);
         if (lane == 0) {
-            if (w > 0.0) atomic_max_pos_double(&a.clock->lam[parity ^ 1], w);
+            if (a.lam_part) a.lam_part[s] = w;   // one slot per strip, reduced by lam_reduce_kernel
+            else if (w > 0.0) atomic_max_pos_double(&a.clock->lam[parity ^ 1], w);
             if (b) atomicAdd((unsigned long long*)&a.clock->bad, (unsigned long long)b);
         }
     }
@@ -349,87 +350,77 @@ __global__ __launch_bounds__(256) void stage_f32_kernel(StageArgs a) {
 
 // ---------------------------------------------------------------------------
 // Stage from precomputed face states (learned reconstruction, periodic grids).
-// F holds 16 planes of primitives: x-face left/right, y-face left/right states of
-// the east/north face of every cell.  A wave owns 63 output columns (lane 0 is the
-// column left of the strip, whose east flux is the strip's first west flux) and
-// marches its rows carrying the north flux of the previous row.
+// F holds 16 planes of primitives, plane (axis * 2 + side) * 4 + var: the left /
+// right states of the east (axis 0) and north (axis 1) face of every cell, as the
+// face kernels leave them.  One thread per cell: the Riemann fluxes of its four
+// faces (each face is evaluated by both cells it separates, which keeps the kernel a
+// single pass), divergence, increment-form RK combine, fused CFL, clock bookkeeping.
 // ---------------------------------------------------------------------------
 template <int FLUX>
 __global__ __launch_bounds__(256) void stage_f32_faces_kernel(StageArgs a, const float* __restrict__ F) {
-    const int s = strip_index(a, blockDim.x / WAVE);
-    if (s >= a.n_strips_x * a.n_strips_y) return;
-    const float* U[4] = {(const float*)a.uin[0], (const float*)a.uin[1], (const float*)a.uin[2], (const float*)a.uin[3]};
-    const float* U0[4] = {(const float*)a.u0[0], (const float*)a.u0[1], (const float*)a.u0[2], (const float*)a.u0[3]};
-    float* O[4] = {(float*)a.uout[0], (float*)a.uout[1], (float*)a.uout[2], (float*)a.uout[3]};
-    const int lane = lane_id();
-    const int nx = a.nx, ny = a.ny;
+    const int nx = a.nx;
+    const long np = (long)nx * a.ny;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int stage = a.stage, parity = a.parity;
-    const int sxi = s % a.n_strips_x, syi = s / a.n_strips_x;
-    const int c = sxi * (WAVE - 1) - 1 + lane;
-    const int cq = c < 0 ? c + nx : (c < nx ? c : nx - 1);
-    const bool out_lane = lane >= 1 && c < nx;
-    const int j0 = syi * a.strip_h;
-    const int j1 = min(j0 + a.strip_h, ny);
     const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0), igm1 = (float)(1.0 / (a.gamma - 1.0));
     const double dtd = a.fixed_dt > 0.0 ? a.fixed_dt : a.cfl / a.clock->lam[parity];
     const float dt = (float)dtd;
     const bool fuse = a.fuse_cfl && (a.euler || stage == 2);
-    if (s == 0 && lane == 0 && stage == 0) { a.clock->dt = dtd; a.clock->lam[parity ^ 1] = 0.0; }
+    if (q == 0 && stage == 0) { a.clock->dt = dtd; a.clock->lam[parity ^ 1] = 0.0; }
     float ca, cb;
     if (a.euler || stage == 0) { ca = 0.f; cb = 1.f; }
     else if (stage == 1) { ca = 0.75f; cb = 0.25f; }
     else { ca = 1.f / 3.f; cb = 2.f / 3.f; }
-    const float dtix = dt * (float)a.inv_x, dtiy = dt * (float)a.inv_y;
-    const float idx = (float)(1.0 / a.dx), idy = (float)(1.0 / a.dy), idz = (float)(1.0 / a.dz);
-    const long np = (long)nx * ny;
-    auto face = [&](int p0, long q) -> Prep {
-        const float* f = F + p0 * np + q;
-        return fast::prep_prim(f[0], f[np], f[2 * np], f[3 * np], g, igm1);
-    };
     float lam_max = 0.f;
     int bad = 0;
-    const int jm = j0 > 0 ? j0 - 1 : ny - 1;
-    F4 Gs = fast::flux<FLUX, 1>(face(8, (long)jm * nx + cq), face(12, (long)jm * nx + cq));
-    for (int j = j0; j < j1; j++) {
-        const long q = (long)j * nx + cq;
-        const F4 Fe = fast::flux<FLUX, 0>(face(0, q), face(4, q));
-        const F4 Fw = from_left4(Fe);
-        const F4 Gn = fast::flux<FLUX, 1>(face(8, q), face(12, q));
-        if (out_lane) {
-            const float d0 = (Fw.r - Fe.r) * dtix + (Gs.r - Gn.r) * dtiy;
-            const float d1 = (Fw.ru - Fe.ru) * dtix + (Gs.ru - Gn.ru) * dtiy;
-            const float d2 = (Fw.rv - Fe.rv) * dtix + (Gs.rv - Gn.rv) * dtiy;
-            const float d3 = (Fw.re - Fe.re) * dtix + (Gs.re - Gn.re) * dtiy;
-            float o0, o1, o2, o3;
+    if (q < np) {
+        const int j = (int)(q / nx), i = (int)(q - (long)j * nx);
+        const long qw = i > 0 ? q - 1 : q + nx - 1;          // periodic west / south neighbours
+        const long qs = j > 0 ? q - nx : q + np - nx;
+        auto state = [&](int plane, long c) -> Prep {
+            const float* f = F + plane * np + c;
+            return fast::prep_prim(f[0], f[np], f[2 * np], f[3 * np], g, igm1);
+        };
+        const F4 Fe = fast::flux<FLUX, 0>(state(0, q), state(4, q));
+        const F4 Fw = fast::flux<FLUX, 0>(state(0, qw), state(4, qw));
+        const F4 Gn = fast::flux<FLUX, 1>(state(8, q), state(12, q));
+        const F4 Gs = fast::flux<FLUX, 1>(state(8, qs), state(12, qs));
+        const float dtix = dt * (float)a.inv_x, dtiy = dt * (float)a.inv_y;
+        const float d[4] = {(Fw.r - Fe.r) * dtix + (Gs.r - Gn.r) * dtiy, (Fw.ru - Fe.ru) * dtix + (Gs.ru - Gn.ru) * dtiy,
+                            (Fw.rv - Fe.rv) * dtix + (Gs.rv - Gn.rv) * dtiy, (Fw.re - Fe.re) * dtix + (Gs.re - Gn.re) * dtiy};
+        float o[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const float u = ((const float*)a.uin[v])[q];
             if (ca != 0.f) {   // increment form of the RK combine, as in stage_f32_strip
-                const float a0 = U0[0][q], a1 = U0[1][q], a2 = U0[2][q], a3 = U0[3][q];
-                o0 = a0 + cb * ((U[0][q] - a0) + d0); o1 = a1 + cb * ((U[1][q] - a1) + d1);
-                o2 = a2 + cb * ((U[2][q] - a2) + d2); o3 = a3 + cb * ((U[3][q] - a3) + d3);
+                const float a0 = ((const float*)a.u0[v])[q];
+                o[v] = a0 + cb * ((u - a0) + d[v]);
             } else {
-                o0 = U[0][q] + d0; o1 = U[1][q] + d1; o2 = U[2][q] + d2; o3 = U[3][q] + d3;
+                o[v] = u + d[v];
             }
-            O[0][q] = o0; O[1][q] = o1; O[2][q] = o2; O[3][q] = o3;
-            if (fuse) {
-                const float ir = fast::rcp(o0);
-                const float vu = fabsf(o1 * ir), vv = fabsf(o2 * ir);
-                const float p = gm1 * (o3 - 0.5f * (o1 * o1 + o2 * o2) * ir);
-                const float snd = fast::fsqrt(g * p * ir);
-                const float l = (vu + snd) * idx + (vv + snd) * idy + snd * idz;
-                if (o0 > 0.f && l == l && p > 0.f) lam_max = fmaxf(lam_max, l);
-                else bad++;
-            }
+            ((float*)a.uout[v])[q] = o[v];
         }
-        Gs = Gn;
+        if (fuse) {
+            const float idx = (float)(1.0 / a.dx), idy = (float)(1.0 / a.dy), idz = (float)(1.0 / a.dz);
+            const float ir = fast::rcp(o[0]);
+            const float vu = fabsf(o[1] * ir), vv = fabsf(o[2] * ir);
+            const float p = gm1 * (o[3] - 0.5f * (o[1] * o[1] + o[2] * o[2]) * ir);
+            const float snd = fast::fsqrt(g * p * ir);
+            const float l = (vu + snd) * idx + (vv + snd) * idy + snd * idz;
+            if (o[0] > 0.f && l == l && p > 0.f) lam_max = l;
+            else bad = 1;
+        }
     }
     if (fuse) {
         const double w = wave_max((double)lam_max);
         const long long b = wave_sum_ll((long long)bad);
-        if (lane == 0) {
-            if (w > 0.0) atomic_max_pos_double(&a.clock->lam[parity ^ 1], w);
+        if (lane_id() == 0) {
+            if (a.lam_part) a.lam_part[q >> 6] = w;   // one slot per wave, reduced by lam_reduce_kernel
+            else if (w > 0.0) atomic_max_pos_double(&a.clock->lam[parity ^ 1], w);
             if (b) atomicAdd((unsigned long long*)&a.clock->bad, (unsigned long long)b);
         }
     }
-    if (s == 0 && lane == 0 && (a.euler || stage == 2)) {
+    if (q == 0 && (a.euler || stage == 2)) {
         a.clock->time += dtd;
         a.clock->step += 1;
     }
@@ -510,13 +501,34 @@ hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_stage_f32_faces(const StageArgs& a, const float* F, int block, hipStream_t st) {
-    const int waves = block / WAVE;
-    const dim3 grid((a.n_strips_x * a.n_strips_y + waves - 1) / waves);
+// The fused CFL's per-wave maxima -> lam[parity ^ 1] (one block).  Waves of a
+// one-row-per-thread launch finish together; a max over a few thousand slots here is
+// cheaper than that many atomics on one address.
+__global__ __launch_bounds__(1024) void lam_reduce_kernel(const double* __restrict__ part, int n, double* target) {
+    __shared__ double red[16];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, part[i]);
+    m = wave_max(m);
+    if (lane_id() == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
+        *target = fmax(*target, m);
+    }
+}
+
+hipError_t launch_lam_reduce(const double* part, int n, double* target, hipStream_t st) {
+    lam_reduce_kernel<<<1, 1024, 0, st>>>(part, n, target);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_f32_faces(const StageArgs& a, const float* F, hipStream_t st) {
+    const long n = (long)a.nx * a.ny;
+    const unsigned grid = (unsigned)((n + 255) / 256);
     switch (a.flux) {
-    case FLUX_RUSANOV: stage_f32_faces_kernel<FLUX_RUSANOV><<<grid, block, 0, st>>>(a, F); break;
-    case FLUX_HLLC: stage_f32_faces_kernel<FLUX_HLLC><<<grid, block, 0, st>>>(a, F); break;
-    default: stage_f32_faces_kernel<FLUX_HLLC_TEXTBOOK><<<grid, block, 0, st>>>(a, F); break;
+    case FLUX_RUSANOV: stage_f32_faces_kernel<FLUX_RUSANOV><<<grid, 256, 0, st>>>(a, F); break;
+    case FLUX_HLLC: stage_f32_faces_kernel<FLUX_HLLC><<<grid, 256, 0, st>>>(a, F); break;
+    default: stage_f32_faces_kernel<FLUX_HLLC_TEXTBOOK><<<grid, 256, 0, st>>>(a, F); break;
     }
     return hipGetLastError();
 }

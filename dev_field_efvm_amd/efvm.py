@@ -248,17 +248,19 @@ class Context:
     def snapshot(self, restore=False):
         self._chk(self._L.efvm_snapshot(self._p, 1 if restore else 0))
 
-    def set_learned(self, mlp):
-        """Use the network's current weights for every face state (None detaches)."""
+    def set_learned(self, mlp, bypass=False):
+        """Use the network's current weights for every face state (None detaches).  bypass:
+        evaluate the gate only where the ungated face value is near the TVD bound."""
         _bind_mlp(self._L)
         self._chk(self._L.efvm_set_learned(self._p, None if mlp is None else mlp._p))
+        self._chk(self._L.efvm_set_learned_bypass(self._p, 1 if bypass else 0))
 
     def learned_stats(self, reset=False):
-        """(faces that fell back to MUSCL, faces evaluated)."""
+        """(faces that fell back to MUSCL, faces evaluated, face sides the network gated)."""
         _bind_mlp(self._L)
-        out = (C.c_ulonglong * 2)()
+        out = (C.c_ulonglong * 3)()
         self._chk(self._L.efvm_mlp_infer_stats(self._p, 1 if reset else 0, out))
-        return int(out[0]), int(out[1])
+        return int(out[0]), int(out[1]), int(out[2])
 
     def learned_faces(self):
         """Face primitives of the current state: (2 axes, 2 sides, 4 vars, ny, nx)."""
@@ -429,6 +431,7 @@ def _bind_mlp(L):
         "efvm_mlp_infer": [vp, pf, sz, pf],
         "efvm_set_learned": [vp, vp],
         "efvm_mlp_infer_stats": [vp, C.c_int, C.POINTER(C.c_ulonglong)],
+        "efvm_set_learned_bypass": [vp, C.c_int],
         "efvm_learned_faces": [vp, pf, sz],
     }
     for name, args in sig.items():
@@ -446,7 +449,7 @@ EXPORTS += [
     "efvm_mlp_create", "efvm_mlp_destroy", "efvm_mlp_last_error", "efvm_mlp_load", "efvm_mlp_save",
     "efvm_mlp_add_snapshot", "efvm_mlp_add_snapshot_pair", "efvm_mlp_add_samples", "efvm_mlp_get_samples", "efvm_mlp_num_samples",
     "efvm_mlp_clear_samples", "efvm_mlp_train_step", "efvm_mlp_gradient", "efvm_mlp_fit_linear", "efvm_mlp_infer", "efvm_set_learned",
-    "efvm_mlp_infer_stats", "efvm_learned_faces",
+    "efvm_mlp_infer_stats", "efvm_learned_faces", "efvm_set_learned_bypass",
 ]
 
 

@@ -179,13 +179,14 @@ def timed_steps(c: Context, n: int, graph_steps: int = 20) -> float:
 
 
 def run_case(seed: int, n: int, T: float, mlp: Mlp | None = None, precision: str = "fp32", timed: bool = True,
-             progress=None, chunk: int = 1000):
+             progress=None, chunk: int = 1000, bypass: bool = False):
     """One run to exactly T.  Untimed runs advance in chunks of `chunk` steps and report
-    (steps done, steps) to `progress` after each, if given."""
+    (steps done, steps) to `progress` after each, if given.  bypass: the network's gate only
+    where the ungated face value is near the TVD bound."""
     c = periodic_context(n, precision)
     c.upload(initial_state(seed, n))
     if mlp is not None:
-        c.set_learned(mlp)
+        c.set_learned(mlp, bypass=bypass)
     steps = exact_steps(c, T)
     if timed:
         ms = timed_steps(c, steps)
@@ -205,9 +206,9 @@ def run_case(seed: int, n: int, T: float, mlp: Mlp | None = None, precision: str
     lam_dt = c.cfl_dt()
     if T / steps > 1.1 * lam_dt:
         raise RuntimeError(f"fixed dt {T / steps:.3e} exceeds the final CFL dt {lam_dt:.3e} by more than 10%")
-    stats = c.learned_stats() if mlp is not None else (0, 0)
+    stats = c.learned_stats() if mlp is not None else (0, 0, 0)
     c.close()
-    return dict(rho=state[..., 0], steps=steps, gpu_ms=ms, fallback=stats[0], faces=stats[1])
+    return dict(rho=state[..., 0], steps=steps, gpu_ms=ms, fallback=stats[0], faces=stats[1], gated_sides=stats[2])
 
 
 def evaluate(mlp: Mlp, seed: int, n: int = 512, fine: int = 2048, truth_n: int = 8192, T: float = 1.0,

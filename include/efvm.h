@@ -240,8 +240,14 @@ efvm_status efvm_mlp_infer(efvm_mlp* m, const float* X, size_t n, float* Y);
 /* Attach a copy of the network's current weights to an FP32 periodic inviscid
  * context (m = NULL detaches): every stage then builds its face states with it. */
 efvm_status efvm_set_learned(efvm_ctx* ctx, efvm_mlp* m);
-/* out[0] = faces that fell back to MUSCL, out[1] = faces evaluated (since reset) */
-efvm_status efvm_mlp_infer_stats(efvm_ctx* ctx, int reset, unsigned long long out[2]);
+/* bypass = 1: the network's gate is evaluated only for face sides whose ungated
+ * face value (phi = 1) fails the TVD bound with half its relaxation, i.e. where a
+ * limiter can act; every other side keeps the correction fully applied.  0 (the
+ * default): the network runs on every face side. */
+efvm_status efvm_set_learned_bypass(efvm_ctx* ctx, int bypass);
+/* out[0] = faces that fell back to MUSCL, out[1] = faces evaluated, out[2] = face
+ * sides whose gate the network evaluated (since reset) */
+efvm_status efvm_mlp_infer_stats(efvm_ctx* ctx, int reset, unsigned long long out[3]);
 /* Face primitives of the current state: 16 planes of nx*ny floats, plane
  * (axis*2 + side)*4 + var, side 0 = left / lower, var = rho, u, v, p, for the
  * east (axis 0) and north (axis 1) face of each cell. */

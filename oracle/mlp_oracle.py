@@ -286,7 +286,7 @@ def samples(cells, faces, gamma=1.4, clip=4.0):
     return np.concatenate(Xs), np.concatenate(Ys)
 
 
-def side_ok(St, face, gamma=1.4):
+def side_ok(St, face, gamma=1.4, relax=0.5):
     """Positivity (rho, p > 0) and the TVD bound relaxed at smooth extrema: every variable of the
     oriented face value lies in [min(w0, w1) - d, max(w0, w1) + d], d = |minmod(c0, c1)| / 2 +
     FLOOR * scale with the curvatures c0 = w1 - 2 w0 + w-1 and c1 = w2 - 2 w1 + w0 (the floor term
@@ -298,7 +298,7 @@ def side_ok(St, face, gamma=1.4):
     scale = np.stack([c[0], a0, a0, c[3]])
     c0 = St[3] - 2.0 * St[2] + St[1]
     c1 = St[4] - 2.0 * St[3] + St[2]
-    d = 0.5 * np.where(c0 * c1 <= 0.0, 0.0, np.minimum(np.abs(c0), np.abs(c1))) + FLOOR * scale
+    d = relax * np.where(c0 * c1 <= 0.0, 0.0, np.minimum(np.abs(c0), np.abs(c1))) + FLOOR * scale
     ok = (face >= np.minimum(w0, w1) - d) & (face <= np.maximum(w0, w1) + d)
     return np.all(ok, axis=0) & (face[0] > 0.0) & (face[3] > 0.0)
 
@@ -309,8 +309,11 @@ def mc(a, b):
     return np.where(a * b > 0.0, np.copysign(m, s), 0.0)
 
 
-def face_states(p, U, gamma=1.4, quantized=True):
+def face_states(p, U, gamma=1.4, quantized=True, bypass=False):
     """Final face primitive states of the learned reconstruction on a periodic grid.
+
+    bypass=True: the gate is evaluated only for face sides whose ungated value (phi = 1)
+    fails the TVD bound with half its relaxation; the other sides keep phi = 1.
 
     Returns ({(axis, 'L'|'R'): (4, ny, nx) primitives of the east/north face of each cell},
     fallback mask per axis (ny, nx) = True where the face fell back to plain MUSCL)."""
@@ -323,6 +326,12 @@ def face_states(p, U, gamma=1.4, quantized=True):
             St = stencil(W, axis, sgn)
             X, sig, c = features(St, gamma)
             y = forward(p, X.reshape(-1, 20), quantized).reshape(c.shape)
+            if bypass:
+                Xf = X.reshape(-1, 20)
+                d = unpack(p)
+                y1 = (base(Xf) + Xf @ d["L"].T + d["bL"]).reshape(c.shape)     # phi = 1
+                open_ = side_ok(St, np.moveaxis(c + sig * y1, -1, 0), gamma, relax=0.25)
+                y = np.where(open_[..., None], y1, y)
             fo = np.moveaxis(c + sig * y, -1, 0)
             ok = side_ok(St, fo, gamma)
             un, ut = sgn * fo[1], fo[2]

@@ -17,6 +17,7 @@ import pytest
 import dev_field_efvm_amd as efvm
 from dev_field_efvm_amd import cases, learned
 from oracle import mlp_oracle as M
+from oracle import orc
 
 pytestmark = pytest.mark.gpu
 
@@ -140,23 +141,24 @@ def test_device_samples_match_oracle():
     assert np.max(np.abs(Y - Yr)) < 1e-4
 
 
-def _oracle_faces(p, U):
-    out, fb = M.face_states(p.astype(np.float64), U, quantized=True)
+def _oracle_faces(p, U, bypass=False):
+    out, fb = M.face_states(p.astype(np.float64), U, quantized=True, bypass=bypass)
     return out, fb
 
 
-def test_learned_face_states_match_oracle():
+@pytest.mark.parametrize("bypass", [False, True])
+def test_learned_face_states_match_oracle(bypass):
     n = 96
     st = cases.learned_state(12, n)
     p = random_params(6, scale=0.1)               # rough network: a mix of accepted and fallen-back faces
     net = efvm.Mlp(p)
     c = learned.periodic_context(n, "fp32")
     c.upload(st)
-    c.set_learned(net)
+    c.set_learned(net, bypass=bypass)
     F = c.learned_faces()                         # (axis, side, var, ny, nx)
     U = c.download()
     Uo = np.stack([U[..., 0], U[..., 1], U[..., 2], U[..., 4]]).astype(np.float64)
-    out, fb = _oracle_faces(p, Uo)
+    out, fb = _oracle_faces(p, Uo, bypass)
     for axis in (0, 1):
         agree = ~fb[axis]
         for side, key in ((0, "L"), (1, "R")):
@@ -171,8 +173,47 @@ def test_learned_face_states_match_oracle():
             assert np.median(err) < 1e-5
         assert 0.02 < fb[axis].mean() < 0.98
     # fallback faces are the solver's MUSCL exactly as in the oracle
-    fbs, tot = c.learned_stats()
+    fbs, tot, nnet = c.learned_stats()
     assert tot == 0                               # learned_faces() does not count
+
+
+@pytest.mark.parametrize("bypass", [False, True])
+def test_learned_stage_fluxes_match_oracle(bypass):
+    """One learned substage (forward-Euler stage of RK3, fixed dt) against the oracle's FP64
+    Riemann fluxes of the face states the diagnostic face kernel returns: pins the stage
+    kernel (fluxes, divergence, update) and, with bypass, that the light pass plus the
+    deferred-face pass build the same face states as the single-pass kernel."""
+    n = 48
+    st = cases.learned_state(5, n)
+    net = efvm.Mlp(random_params(6, scale=0.1))
+    c = learned.periodic_context(n, "fp32")
+    c.upload(st)
+    c.set_learned(net, bypass=bypass)
+    F = c.learned_faces().astype(np.float64)      # (axis, side, var, ny, nx) primitives
+    U = c.download().astype(np.float64)           # (ny, nx, 5)
+    dt = 0.5 * c.cfl_dt()
+    c.set_time_step(0.4, dt)
+    c.run()                                       # stage 1 of RK3: U1 = U + dt L(U)
+    U1 = c.download().astype(np.float64)
+    h = 10.0 / n
+    flux = np.zeros((2, 5, n, n))
+    for axis in (0, 1):
+        for j in range(n):
+            for i in range(n):
+                L = orc.prim_to_cons(np.array([F[axis, 0, 0, j, i], F[axis, 0, 1, j, i], F[axis, 0, 2, j, i], 0.0,
+                                               F[axis, 0, 3, j, i], 1.0]), 1.4)
+                R = orc.prim_to_cons(np.array([F[axis, 1, 0, j, i], F[axis, 1, 1, j, i], F[axis, 1, 2, j, i], 0.0,
+                                               F[axis, 1, 3, j, i], 1.0]), 1.4)
+                flux[axis, :, j, i] = orc.flux("hllc", L, R, 1.4, axis)
+    fx, fy = np.moveaxis(flux[0], 0, -1), np.moveaxis(flux[1], 0, -1)
+    div = (np.roll(fx, 1, axis=1) - fx) / h + (np.roll(fy, 1, axis=0) - fy) / h
+    ref = U + dt * div
+    err = np.abs(U1 - ref)[..., [0, 1, 2, 4]].max() / np.abs(U).max()
+    fb, tot, nnet = c.learned_stats()
+    print("learned stage vs oracle fluxes: max rel err", err, "faces", tot, "gated sides", nnet)
+    assert tot == 2 * n * n
+    assert 0 < nnet < 2 * tot if bypass else nnet == 2 * tot
+    assert err < 1e-5
 
 
 def trained_net():
@@ -199,8 +240,9 @@ def test_learned_run_conserves_and_replays_from_graph():
         assert np.array_equal(eager.download(), graph.download()), rep
     m1 = eager.download()[..., 0].sum(dtype=np.float64)
     assert abs(m1 - m0) / m0 < 1e-6
-    fb, tot = eager.learned_stats()
+    fb, tot, nnet = eager.learned_stats()
     assert tot == 12 * 3 * 2 * n * n               # 12 steps x 3 stages x 2 faces per cell
+    assert nnet == 2 * tot                         # without bypass the network gates every face side
     assert fb < 0.05 * tot
 
 
