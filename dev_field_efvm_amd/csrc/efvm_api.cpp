@@ -40,6 +40,7 @@ template <class T> hipError_t launch_sponge(const StageArgs& a, double x0, doubl
 template <class T> hipError_t launch_sums(const StageArgs& a, double* part, int nb, double vol, double* out, hipStream_t st);
 hipError_t launch_ibm_flip(void* const* planes, int esz, const uint8_t* solid, long n, const double* stagnant, hipStream_t st);
 hipError_t launch_stage_f32_faces(const StageArgs& a, const float* F, hipStream_t st);
+hipError_t launch_stage_f32_flux(const StageArgs& a, const float* F, hipStream_t st);
 hipError_t launch_lam_reduce(const double* part, int n, double* target, hipStream_t st);
 hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st);
 hipError_t launch_mlp_faces_bypass(const mlp::FaceArgs& a, hipStream_t st);
@@ -164,6 +165,7 @@ StageArgs base_args(const efvm_ctx* c) {
     a.n_strips_y = (g.ny + c->strip_h - 1) / c->strip_h;
     a.xcd_remap = c->xcd_remap;
     a.lam_part = c->lam_part;
+    a.face_count = nullptr;
     return a;
 }
 
@@ -241,11 +243,16 @@ efvm_status run_substage(efvm_ctx* c) {
         fa.w = c->wpack;
         fa.stats = c->lstats;
         fa.bypass = c->learned_bypass;
+        // the bypass passes finish each face with its flux (they run at full occupancy); the
+        // network pass writes face states and the stage kernel evaluates the fluxes
+        fa.out_mode = c->learned_bypass ? 1 : 0;
+        fa.flux = c->sch.flux;
         fa.list = c->dlist;
         fa.n_list = c->dlist ? c->dlist + 2 * c->ncell : nullptr;
         hipError_t e = c->learned_bypass ? launch_mlp_faces_bypass(fa, c->stream) : launch_mlp_faces(fa, c->stream);
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("face kernels: ") + hipGetErrorString(e));
-        e = launch_stage_f32_faces(a, c->faces, c->stream);
+        a.face_count = c->learned_bypass ? fa.n_list : nullptr;
+        e = fa.out_mode == 1 ? launch_stage_f32_flux(a, c->faces, c->stream) : launch_stage_f32_faces(a, c->faces, c->stream);
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("learned stage kernel: ") + hipGetErrorString(e));
     } else if ((st = launch_stage(c, a)) != EFVM_OK) {
         return st;
@@ -1180,6 +1187,15 @@ void pack_weights(const float* p, mlp::Packed& P) {
     std::memcpy(P.b3, p + OFF_B3, sizeof(float) * NOUT);
     std::memcpy(P.L, p + OFF_L, sizeof(float) * NOUT * NIN);
     std::memcpy(P.bL, p + OFF_BL, sizeof(float) * NOUT);
+    // own-variable 4-tap correction without bias (what the cubic-exact fit produces)?
+    P.own4 = 1;
+    for (int v = 0; v < NOUT; v++) {
+        if (P.bL[v] != 0.f) P.own4 = 0;
+        for (int f = 0; f < NIN; f++) {
+            const bool tap = f % 4 == v && f / 4 != 2;
+            if (!tap && P.L[v][f] != 0.f) P.own4 = 0;
+        }
+    }
 }
 
 efvm_status mlp_repack(efvm_mlp* m) {
@@ -1528,8 +1544,8 @@ efvm_status efvm_set_learned(efvm_ctx* c, efvm_mlp* m) {
     if (!c->faces) HIPCHK(c, hipMalloc((void**)&c->faces, sizeof(float) * 16 * c->ncell));
     if (!c->wpack) HIPCHK(c, hipMalloc((void**)&c->wpack, sizeof(mlp::Packed)));
     if (!c->lstats) {
-        HIPCHK(c, hipMalloc((void**)&c->lstats, 3 * sizeof(unsigned long long)));
-        HIPCHK(c, hipMemsetAsync(c->lstats, 0, 3 * sizeof(unsigned long long), c->stream));
+        HIPCHK(c, hipMalloc((void**)&c->lstats, 3 * mlp::STAT_SLOTS * sizeof(unsigned long long)));
+        HIPCHK(c, hipMemsetAsync(c->lstats, 0, 3 * mlp::STAT_SLOTS * sizeof(unsigned long long), c->stream));
     }
     HIPCHK(c, hipMemcpyAsync(c->wpack, m->packed, sizeof(mlp::Packed), hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(c, hipStreamSynchronize(m->stream));
@@ -1543,7 +1559,11 @@ efvm_status efvm_set_learned_bypass(efvm_ctx* c, int bypass) {
     if (bypass && 2.0 * (double)c->ncell >= 4294967295.0)
         return fail(c, EFVM_ERR_UNSUPPORTED, "bypass face list indexes faces in 32 bits: grid too large");
     drop_graph(c);
-    if (bypass && !c->dlist) HIPCHK(c, hipMalloc((void**)&c->dlist, sizeof(unsigned) * (2 * c->ncell + 1)));
+    if (bypass && !c->dlist) {
+        HIPCHK(c, hipMalloc((void**)&c->dlist, sizeof(unsigned) * (2 * c->ncell + 1)));
+        HIPCHK(c, hipMemsetAsync(c->dlist + 2 * c->ncell, 0, sizeof(unsigned), c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     c->learned_bypass = bypass;
     return EFVM_OK;
 }
@@ -1552,9 +1572,12 @@ efvm_status efvm_mlp_infer_stats(efvm_ctx* c, int reset, unsigned long long out[
     if (!c || !out) return EFVM_ERR_ARG;
     out[0] = out[1] = out[2] = 0;
     if (!c->lstats) return EFVM_OK;
-    HIPCHK(c, hipMemcpyAsync(out, c->lstats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
-    if (reset) HIPCHK(c, hipMemsetAsync(c->lstats, 0, 3 * sizeof(unsigned long long), c->stream));
+    std::vector<unsigned long long> h(3 * mlp::STAT_SLOTS);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->lstats, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    if (reset) HIPCHK(c, hipMemsetAsync(c->lstats, 0, h.size() * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int w = 0; w < mlp::STAT_SLOTS; w++)
+        for (int k = 0; k < 3; k++) out[k] += h[3 * w + k];
     return EFVM_OK;
 }
 
@@ -1573,6 +1596,8 @@ efvm_status efvm_learned_faces(efvm_ctx* c, float* out, size_t n) {
     fa.w = c->wpack;
     fa.stats = nullptr;
     fa.bypass = c->learned_bypass;
+    fa.out_mode = 0;
+    fa.flux = c->sch.flux;
     fa.list = nullptr;
     fa.n_list = nullptr;
     HIPCHK(c, launch_mlp_faces(fa, c->stream));

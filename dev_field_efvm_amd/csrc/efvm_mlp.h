@@ -33,6 +33,7 @@ constexpr int OFF_L = OFF_B3 + NOUT;
 constexpr int OFF_BL = OFF_L + NOUT * NIN;
 constexpr int NPARAM = OFF_BL + NOUT;   // 5848
 constexpr int TB = 64;                  // samples per training block (one wave)
+constexpr int STAT_SLOTS = 4096;        // waves of the largest face-kernel launch
 
 // features: x[4k+v], k = 0..4 <-> stencil cells -2..2 (k = 3 is the neighbour across
 // the face), v = rho, u_n, u_t, p.  x = (w_k - w_0) / sigma_v for k != 2 and
@@ -72,6 +73,9 @@ struct Packed {
     uint32_t w3[2][64][4];       // [K step][lane][8 bf16]
     float b1[NH], b2[NH], b3[NOUT];
     float L[NOUT][NIN], bL[NOUT];
+    int own4;                    // L only couples each variable to its own 4 neighbour cells and
+                                 // bL = 0 (the cubic-exact fit): the correction is 4 taps, not 20,
+                                 // and with the gate open the face value is a fixed linear stencil
 };
 
 // one launch of the face-state kernel
@@ -81,10 +85,16 @@ struct FaceArgs {
     int n_tasks;                 // 2 * ny * n_chunks (x faces, then y faces)
     float gamma;
     const float* U[4];           // conserved planes of the stage input
-    float* F;                    // 16 planes: ((axis * 2 + side) * 4 + var) * nx*ny, face primitives
+    float* F;                    // out_mode 0: 16 planes ((axis * 2 + side) * 4 + var) * nx*ny of face
+                                 // primitives; out_mode 1: 8 planes (axis * 4 + var) * nx*ny of the
+                                 // Riemann flux through the east / north face of each cell
+    int out_mode;
+    int flux;                    // out_mode 1: 0 Rusanov, 1 HLLC (Thornber), 2 HLLC
     const Packed* w;
-    unsigned long long* stats;   // [faces that fell back to MUSCL, faces evaluated, face sides
-                                 // whose gate the network evaluated], accumulated
+    unsigned long long* stats;   // per-wave counters, 3 per wave slot (STAT_SLOTS slots), accumulated:
+                                 // [faces that fell back to MUSCL, faces evaluated, face sides
+                                 // whose gate the network evaluated]; each wave of a launch owns
+                                 // its slot, so no two waves touch one counter
     int bypass;                  // 1: the network's gate is evaluated only for face sides whose
                                  // ungated value fails the TVD bound with half the relaxation;
                                  // elsewhere the gate is fully open (phi = 1)

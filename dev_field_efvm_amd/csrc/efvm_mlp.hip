@@ -104,7 +104,7 @@ __device__ __forceinline__ float gate(float z) {
 // 10), returns sigma, the centre, the bound tolerance, the FP32 central-slope base yb and
 // the FP32 learned correction sk = L x + bL (which the MLP's gate scales).
 template <bool SLOT_OUT = true>
-__device__ __forceinline__ void featurize(const float ws[5][4], float gamma, const float* skip, uint32_t* slot,
+__device__ __forceinline__ void featurize(const float ws[5][4], float gamma, const float* skip, bool own4, uint32_t* slot,
                                           float sig[4], float cen[4], float yb[4], float sk[4], float tol[4]) {
     float x[20];
     const float a0 = fast::fsqrt(gamma * ws[2][3] * fast::rcp(ws[2][0]));
@@ -128,8 +128,15 @@ __device__ __forceinline__ void featurize(const float ws[5][4], float gamma, con
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         float s = skip[SK_BL + v];
+        if (own4) {
+            // the nonzero terms of the dense sum below, in its order: the same result bit for bit
 #pragma unroll
-        for (int f = 0; f < 20; f++) s = fmaf(skip[SK_L + v * NIN + f], x[f], s);
+            for (int k = 0; k < 5; k++)
+                if (k != 2) s = fmaf(skip[SK_L + v * NIN + 4 * k + v], x[4 * k + v], s);
+        } else {
+#pragma unroll
+            for (int f = 0; f < 20; f++) s = fmaf(skip[SK_L + v * NIN + f], x[f], s);
+        }
         yb[v] = 0.25f * (x[12 + v] - x[4 + v]);
         sk[v] = s;
     }
@@ -161,6 +168,29 @@ __device__ __forceinline__ bool side_ok(const float ws[5][4], const float f[4], 
 }
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Write one face: its primitives (out_mode 0) or its Riemann flux (out_mode 1).  L, R are
+// the lab-frame primitives (rho, u, v, p) of the left / lower and right / upper sides.
+__device__ __forceinline__ void put_face(const FaceArgs& a, int axis, long cell, long np, const float L[4], const float R[4]) {
+    if (a.out_mode == 1) {
+        const float gm = a.gamma, igm1 = 1.f / (a.gamma - 1.f);
+        const fast::Prep pl = fast::prep_prim(L[0], L[1], L[2], L[3], gm, igm1);
+        const fast::Prep pr = fast::prep_prim(R[0], R[1], R[2], R[3], gm, igm1);
+        fast::C4<float> f;
+        if (axis == 0) f = a.flux == 0 ? fast::flux<0, 0>(pl, pr) : (a.flux == 1 ? fast::flux<1, 0>(pl, pr) : fast::flux<2, 0>(pl, pr));
+        else f = a.flux == 0 ? fast::flux<0, 1>(pl, pr) : (a.flux == 1 ? fast::flux<1, 1>(pl, pr) : fast::flux<2, 1>(pl, pr));
+        float* Fo = a.F + (long)(axis * 4) * np + cell;
+        Fo[0] = f.r; Fo[np] = f.ru; Fo[2 * np] = f.rv; Fo[3 * np] = f.re;
+    } else {
+        float* Fl = a.F + (long)(axis * 2) * 4 * np + cell;
+        float* Fr = Fl + 4 * np;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            Fl[v * np] = L[v];
+            Fr[v * np] = R[v];
+        }
+    }
+}
 
 // Intra-wave LDS hand-off: the slots a wave writes are read only by its own lanes, and a
 // wave's LDS operations execute in order, so only compiler reordering must be fenced.
@@ -244,6 +274,7 @@ __device__ __forceinline__ void faces_body(const FaceArgs& a, const float* skip,
     const float gm = a.gamma, gm1 = a.gamma - 1.f;
     const long np = (long)a.nx * a.ny;
     const bool bypass = PASS != PASS_ALL || a.bypass;
+    const bool own4 = a.w->own4 != 0;
     unsigned long long nfb = 0, ntot = 0, nnet = 0;
     const int half = a.ny * a.n_chunks;
     const int n_tasks = PASS == PASS_LIST ? (int)((*a.n_list + 63u) / 64u) : a.n_tasks;
@@ -272,14 +303,14 @@ __device__ __forceinline__ void faces_body(const FaceArgs& a, const float* skip,
             float q[6][4], w[6][4], ws[5][4], f1[4];
             load_cells(a, axis, row, cc, q, w);
             orient(w, axis, 0, ws);
-            featurize(ws, gm, skip, feat[lane], sig[0], cen[0], y[0], sk[0], tol[0]);
+            featurize(ws, gm, skip, own4, feat[lane], sig[0], cen[0], y[0], sk[0], tol[0]);
             if (bypass) {
 #pragma unroll
                 for (int v = 0; v < 4; v++) f1[v] = fmaf(sig[0][v], y[0][v] + sk[0][v], cen[0][v]);
                 need0 = !side_ok(ws, f1, tol[0], 0.25f);
             }
             orient(w, axis, 1, ws);
-            featurize(ws, gm, skip, feat[64 + lane], sig[1], cen[1], y[1], sk[1], tol[1]);
+            featurize(ws, gm, skip, own4, feat[64 + lane], sig[1], cen[1], y[1], sk[1], tol[1]);
             if (bypass) {
 #pragma unroll
                 for (int v = 0; v < 4; v++) f1[v] = fmaf(sig[1][v], y[1][v] + sk[1][v], cen[1][v]);
@@ -349,20 +380,14 @@ __device__ __forceinline__ void faces_body(const FaceArgs& a, const float* skip,
         }
         nfb += __popcll(__ballot(valid && !ok));
         ntot += __popcll(__ballot(valid));
-        if (valid) {
-            float* Fl = a.F + (long)(axis * 2) * 4 * np + cell;
-            float* Fr = Fl + 4 * np;
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                Fl[v * np] = L[v];
-                Fr[v * np] = Rr[v];
-            }
-        }
+        if (valid) put_face(a, axis, cell, np, L, Rr);
     }
     if (a.stats && lane == 0 && (ntot || nnet)) {
-        if (nfb) atomicAdd(a.stats, nfb);
-        if (ntot) atomicAdd(a.stats + 1, ntot);
-        if (nnet) atomicAdd(a.stats + 2, nnet);
+        // the wave's own slot (kernels on one stream run one after another)
+        unsigned long long* st = a.stats + 3 * (blockIdx.x * 4 + wv);
+        st[0] += nfb;
+        st[1] += ntot;
+        st[2] += nnet;
     }
 }
 
@@ -384,44 +409,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // PASS_LIGHT on its own: one side at a time, so only the stencil and the two face
 // values stay live (no network operands, no per-side feature state).
 // PASS_LIGHT on its own: one side at a time, so only the stencil and the two face
-// values stay live (no network operands, no per-side feature state).  Each wave walks
-// several tasks and issues the next task's loads before it works on the current one.
+// values stay live (no network operands, no per-side feature state).
 __global__ __launch_bounds__(256) void faces_light_kernel(FaceArgs a) {
     __shared__ float skip[SK_N];
     stage_skip(a.w, skip);
     const int lane = lane_id(), wv = threadIdx.x >> 6;
     const float gm = a.gamma, gm1 = a.gamma - 1.f;
     const long np = (long)a.nx * a.ny;
+    const bool own4 = a.w->own4 != 0;
     unsigned long long ntot = 0;
-    const int stride = gridDim.x * 4;
-    int task = blockIdx.x * 4 + wv;
-    float qn[6][4];
-    if (task < a.n_tasks) {
-        int ax, rw, c;
-        bool vl;
-        task_face(a, task, lane, ax, rw, c, vl);
-        load_raw(a, ax, rw, c, qn);
-    }
-    for (; task < a.n_tasks; task += stride) {
+    for (int task = blockIdx.x * 4 + wv; task < a.n_tasks; task += gridDim.x * 4) {
         int axis, row, cc;
         bool valid;
         task_face(a, task, lane, axis, row, cc, valid);
-        float w[6][4], fo[2][4];
-        to_prim(qn, gm1, w);
-        if (task + stride < a.n_tasks) {          // prefetch the next task's cells
-            int ax, rw, c;
-            bool vl;
-            task_face(a, task + stride, lane, ax, rw, c, vl);
-            load_raw(a, ax, rw, c, qn);
-        }
+        float q[6][4], w[6][4], fo[2][4];
+        load_cells(a, axis, row, cc, q, w);
         bool defer = false;
 #pragma unroll
         for (int sd = 0; sd < 2; sd++) {
-            float ws[5][4], sig[4], cen[4], y[4], sk[4], tol[4];
+            float ws[5][4], tol[4];
             orient(w, axis, sd, ws);
-            featurize<false>(ws, gm, skip, nullptr, sig, cen, y, sk, tol);
+            if (own4) {
+                // gate open, 4-tap correction, no bias: the normalisation cancels and the face
+                // value is the linear stencil w0 + sum_k t_k (w_k - w0) with t = L + central slope
+                const float a0 = fast::fsqrt(gm * ws[2][3] * fast::rcp(ws[2][0]));
+                const float scale[4] = {ws[2][0], a0, a0, ws[2][3]};
 #pragma unroll
-            for (int v = 0; v < 4; v++) fo[sd][v] = fmaf(sig[v], y[v] + sk[v], cen[v]);   // gate fully open
+                for (int v = 0; v < 4; v++) {
+                    const float c = ws[2][v];
+                    const float* Lv = skip + SK_L + v * NIN;
+                    float d = Lv[v] * (ws[0][v] - c);
+                    d = fmaf(Lv[4 + v] - 0.25f, ws[1][v] - c, d);
+                    d = fmaf(Lv[12 + v] + 0.25f, ws[3][v] - c, d);
+                    d = fmaf(Lv[16 + v], ws[4][v] - c, d);
+                    fo[sd][v] = c + d;
+                    tol[v] = FLOOR * scale[v];
+                }
+            } else {
+                float sig[4], cen[4], y[4], sk[4];
+                featurize<false>(ws, gm, skip, own4, nullptr, sig, cen, y, sk, tol);
+#pragma unroll
+                for (int v = 0; v < 4; v++) fo[sd][v] = fmaf(sig[v], y[v] + sk[v], cen[v]);   // gate fully open
+            }
             defer = defer || !side_ok(ws, fo[sd], tol, 0.25f);
         }
         // defer the faces with a side near the bound (wave-aggregated append)
@@ -439,20 +468,16 @@ __global__ __launch_bounds__(256) void faces_light_kernel(FaceArgs a) {
         if (valid) {
             // both sides pass the strict bound, hence the check: the face is final
             const float unl = fo[0][1], unr = -fo[1][1];
-            float* Fl = a.F + (long)(axis * 2) * 4 * np + cell;
-            float* Fr = Fl + 4 * np;
-            Fl[0] = fo[0][0]; Fl[3 * np] = fo[0][3];
-            Fr[0] = fo[1][0]; Fr[3 * np] = fo[1][3];
-            Fl[np] = axis == 0 ? unl : fo[0][2];
-            Fl[2 * np] = axis == 0 ? fo[0][2] : unl;
-            Fr[np] = axis == 0 ? unr : fo[1][2];
-            Fr[2 * np] = axis == 0 ? fo[1][2] : unr;
+            const float L[4] = {fo[0][0], axis == 0 ? unl : fo[0][2], axis == 0 ? fo[0][2] : unl, fo[0][3]};
+            const float Rr[4] = {fo[1][0], axis == 0 ? unr : fo[1][2], axis == 0 ? fo[1][2] : unr, fo[1][3]};
+            put_face(a, axis, cell, np, L, Rr);
         }
     }
-    if (a.stats && lane == 0 && ntot) atomicAdd(a.stats + 1, ntot);
+    if (a.stats && lane == 0 && ntot) a.stats[3 * (blockIdx.x * 4 + wv) + 1] += ntot;   // the wave's own slot
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void faces_list_kernel(FaceArgs a) {
+    if (*a.n_list == 0u) return;    // nothing deferred (the usual case on smooth flow)
     __shared__ __attribute__((aligned(16))) uint32_t lds[4][128][SLOT];
     __shared__ float skip[SK_N];
     stage_skip(a.w, skip);
@@ -461,8 +486,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     load_regs(a.w, lane_id(), R);
     faces_body<PASS_LIST>(a, skip, lds[threadIdx.x >> 6], R);
 }
-
-__global__ void zero_count_kernel(unsigned* n) { *n = 0u; }
 
 // The matrix-core MLP alone on n given feature rows X (n x 20): Y = base + skip + MLP.
 __global__ __launch_bounds__(256) void infer_kernel(const Packed* __restrict__ P, const float* __restrict__ X, long n,
@@ -898,14 +921,14 @@ hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st) {
 
 // Bypass stage: light pass over every face, then the network over the deferred ones.
 // The list length stays on the device (the list pass's persistent grid reads it), so
-// the sequence can be captured in a graph; the counter is cleared by a kernel.
+// the sequence can be captured in a graph; the stage kernel that consumes the face
+// states clears the counter for the next stage (it starts at zero when allocated).
 hipError_t launch_mlp_faces_bypass(const mlp::FaceArgs& a, hipStream_t st) {
-    mlp::zero_count_kernel<<<1, 1, 0, st>>>(a.n_list);
-#ifndef EFVM_LIGHT_TPW
-#define EFVM_LIGHT_TPW 8
-#endif
-    mlp::faces_light_kernel<<<(a.n_tasks + 4 * EFVM_LIGHT_TPW - 1) / (4 * EFVM_LIGHT_TPW), 256, 0, st>>>(a);
-    mlp::faces_list_kernel<<<512, 256, 0, st>>>(a);
+    // two tasks per wave (fewer, longer waves measured faster than one task each); at most
+    // STAT_SLOTS waves.  The list pass is persistent and small: deferred faces are rare.
+    const int lb = std::min((a.n_tasks + 7) / 8, mlp::STAT_SLOTS / 4);
+    mlp::faces_light_kernel<<<lb, 256, 0, st>>>(a);
+    mlp::faces_list_kernel<<<128, 256, 0, st>>>(a);
     return hipGetLastError();
 }
 

@@ -69,6 +69,7 @@ struct StageArgs {
     int strip_h, n_strips_x, n_strips_y;
     int xcd_remap;
     double* lam_part;            // FP32 fused CFL: per-wave maxima (null: atomics on the clock)
+    unsigned* face_count;        // learned bypass: deferred-face count, cleared by the stage kernel
 };
 
 // One case of a batched ensemble: its stage arguments and plane sets A, B, C.

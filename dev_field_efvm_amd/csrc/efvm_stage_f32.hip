@@ -367,6 +367,7 @@ __global__ __launch_bounds__(256) void stage_f32_faces_kernel(StageArgs a, const
     const float dt = (float)dtd;
     const bool fuse = a.fuse_cfl && (a.euler || stage == 2);
     if (q == 0 && stage == 0) { a.clock->dt = dtd; a.clock->lam[parity ^ 1] = 0.0; }
+    if (q == 0 && a.face_count) *a.face_count = 0u;   // the face passes of this stage are done with it
     float ca, cb;
     if (a.euler || stage == 0) { ca = 0.f; cb = 1.f; }
     else if (stage == 1) { ca = 0.75f; cb = 0.25f; }
@@ -501,6 +502,74 @@ hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Stage from precomputed face fluxes (learned reconstruction, periodic grids): F holds
+// 8 planes, the Riemann flux through the east (planes 0..3) and north (4..7) face of
+// every cell, as the face kernels leave them in out_mode 1.  One thread per cell.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void stage_f32_flux_kernel(StageArgs a, const float* __restrict__ F) {
+    const int nx = a.nx;
+    const long np = (long)nx * a.ny;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int stage = a.stage, parity = a.parity;
+    const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0);
+    const double dtd = a.fixed_dt > 0.0 ? a.fixed_dt : a.cfl / a.clock->lam[parity];
+    const float dt = (float)dtd;
+    const bool fuse = a.fuse_cfl && (a.euler || stage == 2);
+    if (q == 0 && stage == 0) { a.clock->dt = dtd; a.clock->lam[parity ^ 1] = 0.0; }
+    if (q == 0 && a.face_count) *a.face_count = 0u;   // the face passes of this stage are done with it
+    float ca, cb;
+    if (a.euler || stage == 0) { ca = 0.f; cb = 1.f; }
+    else if (stage == 1) { ca = 0.75f; cb = 0.25f; }
+    else { ca = 1.f / 3.f; cb = 2.f / 3.f; }
+    float lam_max = 0.f;
+    int bad = 0;
+    if (q < np) {
+        const int j = (int)(q / nx), i = (int)(q - (long)j * nx);
+        const long qw = i > 0 ? q - 1 : q + nx - 1;          // periodic west / south neighbours
+        const long qs = j > 0 ? q - nx : q + np - nx;
+        const float dtix = dt * (float)a.inv_x, dtiy = dt * (float)a.inv_y;
+        float o[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const float* Fx = F + v * np;
+            const float* Fy = F + (4 + v) * np;
+            const float d = (Fx[qw] - Fx[q]) * dtix + (Fy[qs] - Fy[q]) * dtiy;
+            const float u = ((const float*)a.uin[v])[q];
+            if (ca != 0.f) {   // increment form of the RK combine, as in stage_f32_strip
+                const float a0 = ((const float*)a.u0[v])[q];
+                o[v] = a0 + cb * ((u - a0) + d);
+            } else {
+                o[v] = u + d;
+            }
+            ((float*)a.uout[v])[q] = o[v];
+        }
+        if (fuse) {
+            const float idx = (float)(1.0 / a.dx), idy = (float)(1.0 / a.dy), idz = (float)(1.0 / a.dz);
+            const float ir = fast::rcp(o[0]);
+            const float vu = fabsf(o[1] * ir), vv = fabsf(o[2] * ir);
+            const float p = gm1 * (o[3] - 0.5f * (o[1] * o[1] + o[2] * o[2]) * ir);
+            const float snd = fast::fsqrt(g * p * ir);
+            const float l = (vu + snd) * idx + (vv + snd) * idy + snd * idz;
+            if (o[0] > 0.f && l == l && p > 0.f) lam_max = l;
+            else bad = 1;
+        }
+    }
+    if (fuse) {
+        const double w = wave_max((double)lam_max);
+        const long long b = wave_sum_ll((long long)bad);
+        if (lane_id() == 0) {
+            if (a.lam_part) a.lam_part[q >> 6] = w;   // one slot per wave, reduced by lam_reduce_kernel
+            else if (w > 0.0) atomic_max_pos_double(&a.clock->lam[parity ^ 1], w);
+            if (b) atomicAdd((unsigned long long*)&a.clock->bad, (unsigned long long)b);
+        }
+    }
+    if (q == 0 && (a.euler || stage == 2)) {
+        a.clock->time += dtd;
+        a.clock->step += 1;
+    }
+}
+
 // The fused CFL's per-wave maxima -> lam[parity ^ 1] (one block).  Waves of a
 // one-row-per-thread launch finish together; a max over a few thousand slots here is
 // cheaper than that many atomics on one address.
@@ -519,6 +588,12 @@ __global__ __launch_bounds__(1024) void lam_reduce_kernel(const double* __restri
 
 hipError_t launch_lam_reduce(const double* part, int n, double* target, hipStream_t st) {
     lam_reduce_kernel<<<1, 1024, 0, st>>>(part, n, target);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_f32_flux(const StageArgs& a, const float* F, hipStream_t st) {
+    const long n = (long)a.nx * a.ny;
+    stage_f32_flux_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(a, F);
     return hipGetLastError();
 }
 

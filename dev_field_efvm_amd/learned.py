@@ -187,6 +187,10 @@ def run_case(seed: int, n: int, T: float, mlp: Mlp | None = None, precision: str
     c.upload(initial_state(seed, n))
     if mlp is not None:
         c.set_learned(mlp, bypass=bypass)
+    elif timed:
+        # a timed MUSCL run gets its autotuned launch shape: the baseline is not handicapped
+        from .autotune import tune_context
+        tune_context(c, steps=2)
     steps = exact_steps(c, T)
     if timed:
         ms = timed_steps(c, steps)

@@ -146,11 +146,25 @@ def _oracle_faces(p, U, bypass=False):
     return out, fb
 
 
-@pytest.mark.parametrize("bypass", [False, True])
-def test_learned_face_states_match_oracle(bypass):
+def own4(p):
+    """The correction restricted to each variable's own 4 neighbour cells (the cubic-exact
+    fit's structure, which the face kernels evaluate as 4 taps instead of 20)."""
+    p = p.copy()
+    L = p[M.OFF_L:M.OFF_L + 80].reshape(4, 20)
+    for v in range(4):
+        for f in range(20):
+            if not (f % 4 == v and f // 4 != 2):
+                L[v, f] = 0.0
+    return p
+
+
+@pytest.mark.parametrize("bypass,sparse", [(False, False), (True, False), (False, True), (True, True)])
+def test_learned_face_states_match_oracle(bypass, sparse):
     n = 96
     st = cases.learned_state(12, n)
     p = random_params(6, scale=0.1)               # rough network: a mix of accepted and fallen-back faces
+    if sparse:
+        p = own4(p)
     net = efvm.Mlp(p)
     c = learned.periodic_context(n, "fp32")
     c.upload(st)
@@ -177,15 +191,22 @@ def test_learned_face_states_match_oracle(bypass):
     assert tot == 0                               # learned_faces() does not count
 
 
-@pytest.mark.parametrize("bypass", [False, True])
-def test_learned_stage_fluxes_match_oracle(bypass):
+@pytest.mark.parametrize("bypass,sparse", [(False, False), (True, False), (True, True)])
+def test_learned_stage_fluxes_match_oracle(bypass, sparse):
     """One learned substage (forward-Euler stage of RK3, fixed dt) against the oracle's FP64
     Riemann fluxes of the face states the diagnostic face kernel returns: pins the stage
-    kernel (fluxes, divergence, update) and, with bypass, that the light pass plus the
-    deferred-face pass build the same face states as the single-pass kernel."""
+    kernels (fluxes, divergence, update) and, with bypass, that the light pass plus the
+    deferred-face pass build the same face states as the single-pass kernel.  With the
+    4-tap correction the light pass evaluates the open-gate value as a linear stencil (equal
+    up to rounding), so a face at the deferral bound may be decided differently: a few
+    cells are allowed to differ there."""
     n = 48
     st = cases.learned_state(5, n)
-    net = efvm.Mlp(random_params(6, scale=0.1))
+    p = random_params(6, scale=0.1)
+    if sparse:
+        p = own4(p)
+        p[M.OFF_BL:M.OFF_BL + 4] = 0.0
+    net = efvm.Mlp(p)
     c = learned.periodic_context(n, "fp32")
     c.upload(st)
     c.set_learned(net, bypass=bypass)
@@ -208,12 +229,17 @@ def test_learned_stage_fluxes_match_oracle(bypass):
     fx, fy = np.moveaxis(flux[0], 0, -1), np.moveaxis(flux[1], 0, -1)
     div = (np.roll(fx, 1, axis=1) - fx) / h + (np.roll(fy, 1, axis=0) - fy) / h
     ref = U + dt * div
-    err = np.abs(U1 - ref)[..., [0, 1, 2, 4]].max() / np.abs(U).max()
+    cell_err = np.abs(U1 - ref)[..., [0, 1, 2, 4]].max(axis=-1) / np.abs(U).max()
     fb, tot, nnet = c.learned_stats()
-    print("learned stage vs oracle fluxes: max rel err", err, "faces", tot, "gated sides", nnet)
+    print("learned stage vs oracle fluxes: max rel err", cell_err.max(), "cells above 1e-5",
+          int((cell_err >= 1e-5).sum()), "faces", tot, "gated sides", nnet)
     assert tot == 2 * n * n
     assert 0 < nnet < 2 * tot if bypass else nnet == 2 * tot
-    assert err < 1e-5
+    if sparse:
+        assert np.mean(cell_err < 1e-5) > 0.995
+        assert cell_err.max() < 1e-2
+    else:
+        assert cell_err.max() < 1e-5
 
 
 def trained_net():
