@@ -42,6 +42,7 @@ hipError_t launch_ibm_flip(void* const* planes, int esz, const uint8_t* solid, l
 hipError_t launch_stage_f32_faces(const StageArgs& a, const float* F, hipStream_t st);
 hipError_t launch_stage_f32_flux(const StageArgs& a, const float* F, hipStream_t st);
 hipError_t launch_lam_reduce(const double* part, int n, double* target, hipStream_t st);
+hipError_t launch_cfl_f32(const StageArgs& a, const void* const* planes, double* target, hipStream_t st);
 hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st);
 hipError_t launch_mlp_faces_bypass(const mlp::FaceArgs& a, hipStream_t st);
 hipError_t launch_mlp_infer(const mlp::Packed* P, const float* X, long n, float* Y, hipStream_t st);
@@ -189,7 +190,8 @@ efvm_status launch_stage(efvm_ctx* c, const StageArgs& a) {
 
 efvm_status launch_cfl_into(efvm_ctx* c, void* const* planes, double* target) {
     StageArgs a = base_args(c);
-    hipError_t e = c->esz == 8 ? launch_cfl<double>(a, planes, target, c->stream) : launch_cfl<float>(a, planes, target, c->stream);
+    // FP32: the fused epilogues' per-cell formula, so a restarted run takes the same dt
+    hipError_t e = c->esz == 8 ? launch_cfl<double>(a, planes, target, c->stream) : launch_cfl_f32(a, planes, target, c->stream);
     if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("cfl kernel: ") + hipGetErrorString(e));
     return EFVM_OK;
 }
@@ -840,7 +842,7 @@ efvm_status batch_rebuild(efvm_batch* b) {
 efvm_status batch_cfl_all(efvm_batch* b, int slot) {
     for (int i = 0; i < b->n; i++) {
         StageArgs a = b->hc[i].a;
-        hipError_t e = launch_cfl<float>(a, b->hc[i].P[0], &b->clocks[i].lam[slot], b->stream);
+        hipError_t e = launch_cfl_f32(a, b->hc[i].P[0], &b->clocks[i].lam[slot], b->stream);
         if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, "batch cfl");
     }
     return EFVM_OK;
@@ -1604,6 +1606,198 @@ efvm_status efvm_learned_faces(efvm_ctx* c, float* out, size_t n) {
     HIPCHK(c, hipMemcpyAsync(out, c->faces, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// State I/O: CFDRST01 checkpoints (examples/restart_determinism/main.rs:177-226)
+// and VTK legacy RECTILINEAR_GRID output (crates/cfd_output/src/lib.rs:36-113).
+// ---------------------------------------------------------------------------
+namespace {
+
+// the reference's raw index of interior cell (i, j): ng = 2 ghost layers on every axis of
+// the one-cell-thick mesh, x fastest (nx + 4) (ny + 4) (1 + 4) cells
+inline long raw_index(int nx, int ny, int i, int j) {
+    const long NX = nx + 4, NY = ny + 4;
+    return (long)(i + 2) + NX * ((long)(j + 2) + NY * 2);
+}
+
+efvm_status download_set(efvm_ctx* c, int set, double* cons5) {
+    const long n = c->ncell;
+    std::vector<char> h[4];
+    for (int m = 0; m < 4; m++) {
+        h[m].resize(c->esz * n);
+        HIPCHK(c, hipMemcpyAsync(h[m].data(), c->P[set][m], c->esz * n, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int comp[4] = {0, 1, 2, 4};
+    for (long q = 0; q < n; q++) {
+        cons5[5 * q + 3] = 0.0;
+        for (int m = 0; m < 4; m++)
+            cons5[5 * q + comp[m]] = c->esz == 8 ? reinterpret_cast<double*>(h[m].data())[q]
+                                                 : (double)reinterpret_cast<float*>(h[m].data())[q];
+    }
+    return EFVM_OK;
+}
+
+template <class T> void put(std::vector<unsigned char>& b, T v) {
+    unsigned char x[sizeof(T)];
+    std::memcpy(x, &v, sizeof(T));   // little-endian host (x86-64), as the format is
+    b.insert(b.end(), x, x + sizeof(T));
+}
+
+// VTK-safe scientific notation: 10 digits, signed exponent of at least two digits,
+// non-finite values as zero (crates/cfd_output/src/lib.rs:15-33)
+void fmt_e(FILE* f, double x) {
+    if (!std::isfinite(x)) x = 0.0;
+    std::fprintf(f, "%.10e", x);
+}
+
+}  // namespace
+
+extern "C" {
+
+efvm_status efvm_checkpoint_save(efvm_ctx* c, const char* path) {
+    if (!c || !path) return EFVM_ERR_ARG;
+    const int nx = c->grid.nx, ny = c->grid.ny;
+    const long n = c->ncell, nraw = (long)(nx + 4) * (ny + 4) * 5;
+    std::vector<double> u(5 * n), u0(5 * n);
+    efvm_status st = efvm_download_state(c, u.data(), n);          // the current substage's state
+    if (st != EFVM_OK) return st;
+    if ((st = download_set(c, c->cur, u0.data())) != EFVM_OK) return st;   // the RK backup U^n
+    double dt = 0.0, time = 0.0;
+    long step = 0;
+    int stage = 0;
+    if ((st = efvm_get_clock(c, &dt, &time, &step, &stage)) != EFVM_OK) return st;
+    std::vector<unsigned char> b;
+    b.reserve(8 + 25 + 3 * (8 + 40 * nraw));
+    const char magic[8] = {'C', 'F', 'D', 'R', 'S', 'T', '0', '1'};
+    b.insert(b.end(), magic, magic + 8);
+    put(b, dt);
+    put(b, time);
+    put(b, (unsigned long long)step);
+    put(b, (unsigned char)stage);
+    // u, rhs (zero between substages: the device path keeps no rhs column), u0; ghosts zero
+    const double* cols[3] = {u.data(), nullptr, u0.data()};
+    for (int k = 0; k < 3; k++) {
+        put(b, (unsigned long long)nraw);
+        std::vector<double> raw(5 * nraw, 0.0);
+        if (cols[k])
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++)
+                    std::memcpy(&raw[5 * raw_index(nx, ny, i, j)], cols[k] + 5 * ((long)j * nx + i), 5 * sizeof(double));
+        const unsigned char* p = reinterpret_cast<const unsigned char*>(raw.data());
+        b.insert(b.end(), p, p + raw.size() * sizeof(double));
+    }
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(c, EFVM_ERR_ARG, std::string("checkpoint: cannot open ") + path);
+    const size_t w = std::fwrite(b.data(), 1, b.size(), f);
+    const int cl = std::fclose(f);
+    if (w != b.size() || cl != 0) return fail(c, EFVM_ERR_ARG, std::string("checkpoint: short write to ") + path);
+    return EFVM_OK;
+}
+
+efvm_status efvm_checkpoint_load(efvm_ctx* c, const char* path) {
+    if (!c || !path) return EFVM_ERR_ARG;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(c, EFVM_ERR_ARG, std::string("checkpoint: cannot open ") + path);
+    std::vector<unsigned char> b;
+    unsigned char buf[1 << 16];
+    size_t r;
+    while ((r = std::fread(buf, 1, sizeof(buf), f)) > 0) b.insert(b.end(), buf, buf + r);
+    std::fclose(f);
+    size_t pos = 0;
+    auto take = [&](void* dst, size_t k) {
+        if (pos + k > b.size()) return false;
+        std::memcpy(dst, b.data() + pos, k);
+        pos += k;
+        return true;
+    };
+    char magic[8];
+    double dt, time;
+    unsigned long long step;
+    unsigned char stage;
+    if (!take(magic, 8) || std::memcmp(magic, "CFDRST01", 8) != 0)
+        return fail(c, EFVM_ERR_CONFIG, "checkpoint: not a CFDRST01 file");
+    if (!take(&dt, 8) || !take(&time, 8) || !take(&step, 8) || !take(&stage, 1))
+        return fail(c, EFVM_ERR_CONFIG, "checkpoint: truncated header");
+    if (stage > 2) return fail(c, EFVM_ERR_CONFIG, "checkpoint: bad rk stage");
+    if (stage != 0)
+        return fail(c, EFVM_ERR_UNSUPPORTED, "checkpoint: restarts are taken at a step boundary (rk stage 0)");
+    const int nx = c->grid.nx, ny = c->grid.ny;
+    const long n = c->ncell, nraw = (long)(nx + 4) * (ny + 4) * 5;
+    std::vector<double> u(5 * n);
+    for (int k = 0; k < 3; k++) {
+        unsigned long long len;
+        if (!take(&len, 8)) return fail(c, EFVM_ERR_CONFIG, "checkpoint: truncated column");
+        if ((long)len != nraw) return fail(c, EFVM_ERR_CONFIG, "checkpoint: cell count does not match this grid");
+        if (pos + 40 * (size_t)nraw > b.size()) return fail(c, EFVM_ERR_CONFIG, "checkpoint: truncated column");
+        if (k == 0)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++)
+                    std::memcpy(u.data() + 5 * ((long)j * nx + i), b.data() + pos + 40 * (size_t)raw_index(nx, ny, i, j), 40);
+        pos += 40 * (size_t)nraw;
+    }
+    if (pos != b.size()) return fail(c, EFVM_ERR_CONFIG, "checkpoint: trailing bytes");
+    efvm_status st = efvm_upload_state(c, u.data(), n);      // rk stage 0; the next step recomputes the CFL
+    if (st != EFVM_OK) return st;
+    drop_graph(c);
+    HIPCHK(c, launch_clock_set(c->clock, dt, time, (long long)step, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm_write_vtk(const efvm_grid* g, const efvm_gas* gas, const double* cons5, size_t n_cells, const char* path) {
+    if (!g || !gas || !cons5 || !path || g->nx < 1 || g->ny < 1) return EFVM_ERR_ARG;
+    const int ni = g->nx, nj = g->ny, nk = 1;
+    if ((long)n_cells != (long)ni * nj) return EFVM_ERR_ARG;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return EFVM_ERR_ARG;
+    std::fprintf(f, "# vtk DataFile Version 3.0\ndev_field_efvm output\nASCII\nDATASET RECTILINEAR_GRID\n");
+    std::fprintf(f, "DIMENSIONS %d %d %d\n", ni + 1, nj + 1, nk + 1);
+    // faces = centroid - half spacing, plus the last centroid + half spacing
+    const double h[3] = {(g->hi_x - g->lo_x) / (double)ni, (g->hi_y - g->lo_y) / (double)nj, g->dz};
+    const double lo[3] = {g->lo_x, g->lo_y, 0.0};
+    const int nn[3] = {ni, nj, nk};
+    const char* label[3] = {"X", "Y", "Z"};
+    for (int ax = 0; ax < 3; ax++) {
+        std::fprintf(f, "%s_COORDINATES %d double\n", label[ax], nn[ax] + 1);
+        for (int m = 0; m < nn[ax]; m++) {
+            const double cc = lo[ax] + ((double)m + 0.5) * h[ax];
+            fmt_e(f, cc - 0.5 * h[ax]);
+            std::fputc('\n', f);
+        }
+        const double cl = lo[ax] + ((double)(nn[ax] - 1) + 0.5) * h[ax];
+        fmt_e(f, cl + 0.5 * h[ax]);
+        std::fputc('\n', f);
+    }
+    const long nc = (long)ni * nj;
+    std::fprintf(f, "CELL_DATA %ld\n", nc);
+    auto cell = [&](long q) { return C5<double>{cons5[5 * q], cons5[5 * q + 1], cons5[5 * q + 2], cons5[5 * q + 3], cons5[5 * q + 4]}; };
+    auto scalar = [&](const char* name, auto fn) {
+        std::fprintf(f, "SCALARS %s double 1\nLOOKUP_TABLE default\n", name);
+        for (long q = 0; q < nc; q++) {
+            fmt_e(f, fn(cell(q)));
+            std::fputc('\n', f);
+        }
+    };
+    const double gm = gas->gamma, R = gas->r_gas;
+    scalar("density", [](const C5<double>& u) { return u.r; });
+    scalar("pressure", [&](const C5<double>& u) { return ref::pressure(u, gm); });
+    scalar("temperature", [&](const C5<double>& u) { return ref::cons_to_prim(u, gm, R).t; });
+    std::fprintf(f, "VECTORS velocity double\n");
+    for (long q = 0; q < nc; q++) {
+        double v[3];
+        ref::velocity(cell(q), v);
+        fmt_e(f, v[0]); std::fputc(' ', f); fmt_e(f, v[1]); std::fputc(' ', f); fmt_e(f, v[2]); std::fputc('\n', f);
+    }
+    scalar("mach", [&](const C5<double>& u) {
+        double v[3];
+        ref::velocity(u, v);
+        return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / ref::sound(u, gm);
+    });
+    return std::fclose(f) == 0 ? EFVM_OK : EFVM_ERR_ARG;
 }
 
 }  // extern "C"

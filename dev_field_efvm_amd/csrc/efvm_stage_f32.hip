@@ -104,6 +104,29 @@ __device__ __forceinline__ F4 axpy4(const F4& u, float h, const F4& s) {
     return {fmaf(h, s.r, u.r), fmaf(h, s.ru, u.ru), fmaf(h, s.rv, u.rv), fmaf(h, s.re, u.re)};
 }
 
+// CFL spectral radius of one FP32 cell (crates/cfd_solver/src/lib.rs:653-666, nz = 1), with
+// the operation order fixed by explicit fmaf so that the fused epilogues and the standalone
+// FP32 scan produce the same value: a run restarted from a checkpoint takes the same dt.
+// Returns false for a non-physical cell (rho <= 0, p <= 0 or a non-finite lambda).
+template <bool VISC>
+__device__ __forceinline__ bool lam_cell_f32(const StageArgs& a, float o0, float o1, float o2, float o3, float& l) {
+    const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0);
+    const float idx = (float)(1.0 / a.dx), idy = (float)(1.0 / a.dy), idz = (float)(1.0 / a.dz);
+    const float ir = fast::rcp(o0);
+    const float vu = fabsf(o1 * ir), vv = fabsf(o2 * ir);
+    const float q2 = fmaf(o1, o1, o2 * o2);
+    const float p = gm1 * fmaf(-0.5f * q2, ir, o3);
+    const float snd = fast::fsqrt(g * p * ir);
+    l = fmaf(vu + snd, idx, fmaf(vv + snd, idy, snd * idz));
+    if (VISC) {
+        const float vsum = (float)(1.0 / (a.dx * a.dx) + 1.0 / (a.dy * a.dy) + 1.0 / (a.dz * a.dz));
+        const float t = p * ir * (float)(1.0 / a.r_gas);
+        const float mu = a.visc_const ? (float)a.mu : (float)ref::sutherland((double)t);
+        l = fmaf(2.f * mu * ir, vsum, l);
+    }
+    return o0 > 0.f && l == l && p > 0.f;
+}
+
 struct Prim3 { float u, v, t; };
 __device__ __forceinline__ Prim3 prim3(const F4& q, float gm1, float ir_gas) {
     float iq = fast::rcp(q.r);
@@ -143,8 +166,6 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
     else if (stage == 1) { ca = 0.75f; cb = 0.25f; }
     else { ca = 1.f / 3.f; cb = 2.f / 3.f; }
     const float dtix = dt * (float)a.inv_x, dtiy = dt * (float)a.inv_y;
-    const float idx = (float)(1.0 / a.dx), idy = (float)(1.0 / a.dy), idz = (float)(1.0 / a.dz);
-    const float vsum = (float)(1.0 / (a.dx * a.dx) + 1.0 / (a.dy * a.dy) + 1.0 / (a.dz * a.dz));
     const float ir_gas = (float)(1.0 / a.r_gas);
     const float kap_fac = (float)(a.gamma * a.r_gas / (a.gamma - 1.0) / 0.72);
     const float gxdt = (float)a.gx * dt;
@@ -274,17 +295,8 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
             row_st(O[0] + rb, rowb, coff, o0); row_st(O[1] + rb, rowb, coff, o1); row_st(O[2] + rb, rowb, coff, o2);
             row_st(O[3] + rb, rowb, coff, o3);
             if (fuse) {
-                const float ir = fast::rcp(o0);
-                const float vu = fabsf(o1 * ir), vv = fabsf(o2 * ir);
-                const float p = gm1 * (o3 - 0.5f * (o1 * o1 + o2 * o2) * ir);
-                const float snd = fast::fsqrt(g * p * ir);
-                float l = (vu + snd) * idx + (vv + snd) * idy + snd * idz;
-                if (VISC) {
-                    const float t = p * ir * ir_gas;
-                    const float mu = a.visc_const ? (float)a.mu : (float)ref::sutherland((double)t);
-                    l += 2.f * mu * ir * vsum;
-                }
-                if (o0 > 0.f && l == l && p > 0.f) lam_max = fmaxf(lam_max, l);
+                float l;
+                if (lam_cell_f32<VISC>(a, o0, o1, o2, o3, l)) lam_max = fmaxf(lam_max, l);
                 else bad++;
             }
         }
@@ -402,13 +414,8 @@ __global__ __launch_bounds__(256) void stage_f32_faces_kernel(StageArgs a, const
             ((float*)a.uout[v])[q] = o[v];
         }
         if (fuse) {
-            const float idx = (float)(1.0 / a.dx), idy = (float)(1.0 / a.dy), idz = (float)(1.0 / a.dz);
-            const float ir = fast::rcp(o[0]);
-            const float vu = fabsf(o[1] * ir), vv = fabsf(o[2] * ir);
-            const float p = gm1 * (o[3] - 0.5f * (o[1] * o[1] + o[2] * o[2]) * ir);
-            const float snd = fast::fsqrt(g * p * ir);
-            const float l = (vu + snd) * idx + (vv + snd) * idy + snd * idz;
-            if (o[0] > 0.f && l == l && p > 0.f) lam_max = l;
+            float l;
+            if (lam_cell_f32<false>(a, o[0], o[1], o[2], o[3], l)) lam_max = l;
             else bad = 1;
         }
     }
@@ -545,13 +552,8 @@ __global__ __launch_bounds__(256) void stage_f32_flux_kernel(StageArgs a, const 
             ((float*)a.uout[v])[q] = o[v];
         }
         if (fuse) {
-            const float idx = (float)(1.0 / a.dx), idy = (float)(1.0 / a.dy), idz = (float)(1.0 / a.dz);
-            const float ir = fast::rcp(o[0]);
-            const float vu = fabsf(o[1] * ir), vv = fabsf(o[2] * ir);
-            const float p = gm1 * (o[3] - 0.5f * (o[1] * o[1] + o[2] * o[2]) * ir);
-            const float snd = fast::fsqrt(g * p * ir);
-            const float l = (vu + snd) * idx + (vv + snd) * idy + snd * idz;
-            if (o[0] > 0.f && l == l && p > 0.f) lam_max = l;
+            float l;
+            if (lam_cell_f32<false>(a, o[0], o[1], o[2], o[3], l)) lam_max = l;
             else bad = 1;
         }
     }
@@ -568,6 +570,40 @@ __global__ __launch_bounds__(256) void stage_f32_flux_kernel(StageArgs a, const 
         a.clock->time += dtd;
         a.clock->step += 1;
     }
+}
+
+// Standalone FP32 CFL scan (first step, after an upload or a checkpoint load) with the
+// same per-cell formula as the fused epilogues.
+template <bool VISC>
+__global__ __launch_bounds__(256) void cfl_f32_kernel(StageArgs a, const float* __restrict__ r, const float* __restrict__ ru,
+                                                     const float* __restrict__ rv, const float* __restrict__ re, double* target) {
+    const long n = (long)a.nx * a.ny;
+    float m = 0.f;
+    long long bad = 0;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        float l;
+        if (lam_cell_f32<VISC>(a, r[q], ru[q], rv[q], re[q], l)) m = fmaxf(m, l);
+        else bad++;
+    }
+    const double w = wave_max((double)m);
+    bad = wave_sum_ll(bad);
+    if (lane_id() == 0) {
+        if (w > 0.0) atomic_max_pos_double(target, w);
+        if (bad) atomicAdd((unsigned long long*)&a.clock->bad, (unsigned long long)bad);
+    }
+}
+
+__global__ void reset_target_kernel(double* t) { *t = 0.0; }
+
+hipError_t launch_cfl_f32(const StageArgs& a, const void* const* planes, double* target, hipStream_t st) {
+    const long n = (long)a.nx * a.ny;
+    const long nb = (n + 255) / 256;
+    const int blocks = (int)(nb < 2048 ? nb : 2048);
+    const float* P[4] = {(const float*)planes[0], (const float*)planes[1], (const float*)planes[2], (const float*)planes[3]};
+    reset_target_kernel<<<1, 1, 0, st>>>(target);
+    if (a.viscous) cfl_f32_kernel<true><<<blocks, 256, 0, st>>>(a, P[0], P[1], P[2], P[3], target);
+    else cfl_f32_kernel<false><<<blocks, 256, 0, st>>>(a, P[0], P[1], P[2], P[3], target);
+    return hipGetLastError();
 }
 
 // The fused CFL's per-wave maxima -> lam[parity ^ 1] (one block).  Waves of a

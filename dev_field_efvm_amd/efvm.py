@@ -29,8 +29,25 @@ EXPORTS = [
     "efvm_download_state", "efvm_get_clock", "efvm_set_clock", "efvm_cfl", "efvm_run", "efvm_step",
     "efvm_advance_to", "efvm_synchronize", "efvm_graph_capture", "efvm_graph_launch",
     "efvm_ibm_add_cylinder", "efvm_marker_force", "efvm_conservation_sums", "efvm_device_state",
-    "efvm_last_elapsed_ms", "efvm_snapshot",
+    "efvm_last_elapsed_ms", "efvm_snapshot", "efvm_checkpoint_save", "efvm_checkpoint_load", "efvm_write_vtk",
 ]
+
+
+def write_vtk(path: str, state, nx: int, ny: int, lo=(0.0, 0.0), hi=(1.0, 1.0), dz=None, gamma=1.4,
+              r_gas=287.058):
+    """VTK legacy RECTILINEAR_GRID file of a wire-order state (ny, nx, 5) on the host
+    (crates/cfd_output/src/lib.rs write_vtk_to); needs no device."""
+    L = load_library()
+    dz = (hi[0] - lo[0]) / nx if dz is None else dz
+    g = Grid(int(nx), int(ny), lo[0], lo[1], hi[0], hi[1], dz)
+    gs = Gas(gamma, r_gas, 0, 0.0)
+    u = np.ascontiguousarray(state, dtype=np.float64).reshape(-1)
+    if u.size != 5 * nx * ny:
+        raise ValueError("state must hold 5 values per cell")
+    st = L.efvm_write_vtk(C.byref(g), C.byref(gs), u.ctypes.data_as(C.POINTER(C.c_double)), nx * ny,
+                          os.fsencode(path))
+    if st != OK:
+        raise EfvmError(st, f"efvm_write_vtk({path}) failed")
 
 
 class EfvmError(RuntimeError):
@@ -95,6 +112,9 @@ def load_library(path: str = LIB_PATH):
         "efvm_device_state": [vp, C.POINTER(vp)],
         "efvm_last_elapsed_ms": [vp, pd],
         "efvm_snapshot": [vp, C.c_int],
+        "efvm_checkpoint_save": [vp, C.c_char_p],
+        "efvm_checkpoint_load": [vp, C.c_char_p],
+        "efvm_write_vtk": [C.POINTER(Grid), C.POINTER(Gas), pd, C.c_size_t, C.c_char_p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -247,6 +267,20 @@ class Context:
 
     def snapshot(self, restore=False):
         self._chk(self._L.efvm_snapshot(self._p, 1 if restore else 0))
+
+    def checkpoint_save(self, path: str):
+        """CFDRST01 restart file of the current state and clock (the reference's format)."""
+        self._chk(self._L.efvm_checkpoint_save(self._p, os.fsencode(path)))
+
+    def checkpoint_load(self, path: str):
+        """Resume from a step-boundary CFDRST01 file of the same grid."""
+        self._chk(self._L.efvm_checkpoint_load(self._p, os.fsencode(path)))
+
+    def write_vtk(self, path: str):
+        """VTK legacy output of the current state (OutputPlugin's file)."""
+        u = np.ascontiguousarray(self.download(), dtype=np.float64)
+        self._chk(self._L.efvm_write_vtk(C.byref(self.grid), C.byref(self.gas), _pd(u), self.nx * self.ny,
+                                         os.fsencode(path)))
 
     def set_learned(self, mlp, bypass=False):
         """Use the network's current weights for every face state (None detaches).  bypass:

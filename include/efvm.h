@@ -171,6 +171,26 @@ efvm_status efvm_batch_clock(efvm_batch* b, int case_id, double* dt, double* tim
 efvm_status efvm_batch_synchronize(efvm_batch* b);
 efvm_status efvm_batch_last_elapsed_ms(efvm_batch* b, double* ms);
 
+/* ---------------------------------------------------------------------------
+ * State I/O (the reference's restart files and VTK output).
+ * ------------------------------------------------------------------------- */
+/* CFDRST01 checkpoint, the format of examples/restart_determinism/main.rs:177-226
+ * (write_checkpoint / read_checkpoint): "CFDRST01", dt, time (f64), step (u64), rk stage
+ * (u8), then the columns u, rhs, u0, each a u64 cell count and 5 f64 per cell over the
+ * reference's raw padded array ((nx+4)(ny+4)(1+4) cells, x fastest), little-endian.
+ * Ghost cells are written as 0 (the Halo and Boundary phases refill them), rhs as 0.
+ * Save works at any substage.  Load takes a step-boundary checkpoint of the same grid
+ * and resumes it bit for bit: the next step's CFL of the loaded state gives the dt the
+ * uninterrupted run takes (crates/cfd_solver/tests/restart.rs:86-105). */
+efvm_status efvm_checkpoint_save(efvm_ctx* ctx, const char* path);
+efvm_status efvm_checkpoint_load(efvm_ctx* ctx, const char* path);
+/* VTK legacy RECTILINEAR_GRID output of the interior cells, write_vtk_to of
+ * crates/cfd_output/src/lib.rs:36-113: density, pressure, temperature, velocity and Mach
+ * in 10-digit scientific notation (non-finite values as 0).  Host only: it takes a state in
+ * the wire order (efvm_download_state), so it needs no device. */
+efvm_status efvm_write_vtk(const efvm_grid* grid, const efvm_gas* gas, const double* cons5, size_t n_cells,
+                           const char* path);
+
 /* Device-side snapshot of the state and clock(s): op 0 saves, op 1 restores.
  * Lets the autotuner time real steps and then put the simulation back. */
 efvm_status efvm_snapshot(efvm_ctx* ctx, int op);
