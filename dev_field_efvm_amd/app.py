@@ -317,6 +317,13 @@ class BodyForceX:
 
 
 @dataclass
+class OutputPlugin:
+    """crates/cfd_output/src/lib.rs:126-160: one VTK file of the state when the App is
+    finished (``App.finish()``, the reference's cleanup hook)."""
+    path: str = "output.vtk"
+
+
+@dataclass
 class GpuPlugin:
     precision: str = "fp64"        # "fp64" (parity) or "fp32" (production)
     seam_mode: int = 0             # 0 reference, 1 conservative periodic seams
@@ -443,3 +450,16 @@ class App:
 
     def conservation_sums(self) -> np.ndarray:
         return self.ctx.sums()
+
+    # restart files (examples/restart_determinism/main.rs:177-226 write_checkpoint / read_checkpoint)
+    def write_checkpoint(self, path: str):
+        self.ctx.checkpoint_save(path)
+
+    def read_checkpoint(self, path: str):
+        self.ctx.checkpoint_load(path)
+
+    def finish(self):
+        """The cleanup hooks: OutputPlugin writes its VTK file."""
+        out = self._find(OutputPlugin)
+        if out is not None:
+            self.ctx.write_vtk(out.path)

@@ -412,3 +412,37 @@ def test_app_plugin_api_matches_context():
     c.step(2)
     assert np.array_equal(app.state(), c.download())
     assert app.solver_state().step == 2
+
+
+def test_app_output_plugin_and_restart(tmp_path):
+    """OutputPlugin writes the VTK file at finish(); write_checkpoint / read_checkpoint resume
+    an App bit for bit (crates/cfd_solver/tests/restart.rs:86-105, with the plugin mirror)."""
+    n = 48
+    st = cases.vortex_state(n, n, mean_u=1.0, mean_v=1.0)
+
+    def build(out):
+        app = efvm.App()
+        app.add_plugins(efvm.FieldDefaultPlugins(efvm.UniformMeshConfig(nx=n, ny=n, nz=1, ng=2, bounds_lo=(0, 0, 0),
+                                                                        bounds_hi=(10.0, 10.0, 10.0 / n))))
+        app.add_plugins(efvm.CfdStatePlugin(st))
+        app.add_plugins(efvm.StructuredBoundaryPlugin(efvm.BoundaryRegistry().with_axis(0, efvm.Periodic())
+                                                      .with_axis(1, efvm.Periodic()).with_axis(2, efvm.SlipWall())))
+        app.add_plugins(efvm.MusclPlugin.hllc().with_limiter(efvm.SlopeLimiter.MonotonizedCentral))
+        app.add_plugins(efvm.IntegratorPlugin.rk3())
+        app.add_plugins(efvm.SolverPlugin(efvm.SolverConfig(cfl=0.4)))
+        app.add_plugins(efvm.OutputPlugin(str(out)))
+        return app.prepare()
+
+    a = build(tmp_path / "a.vtk")
+    a.step(5)
+    a.write_checkpoint(str(tmp_path / "a.bin"))
+    a.step(7)
+    a.finish()
+    b = build(tmp_path / "b.vtk")
+    b.read_checkpoint(str(tmp_path / "a.bin"))
+    b.step(7)
+    b.finish()
+    assert np.array_equal(a.state(), b.state())
+    assert a.solver_state() == b.solver_state()
+    text = open(tmp_path / "a.vtk").read()
+    assert f"CELL_DATA {n * n}" in text and text == open(tmp_path / "b.vtk").read()
