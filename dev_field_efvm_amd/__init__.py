@@ -11,6 +11,6 @@ from .app import (  # noqa: F401
     BoundarySide, SupersonicOutflow, Periodic, SlipWall, Symmetry, NoSlipWall, SupersonicInflow,
     SubsonicInflow, PressureOutlet, BoundaryRegistry, BoundaryPlugin, StructuredBoundaryPlugin,
     Cylinder, GhostCellIbmPlugin, CylinderIbmPlugin, ParabolicInflow, OutletSponge, BodyForceX,
-    GpuPlugin, SolverState, OutputPlugin,
+    GpuPlugin, SolverState, OutputPlugin, FromOutside, FnBoundary, BoundaryInputs,
 )
 from . import cases  # noqa: F401

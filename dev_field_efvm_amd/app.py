@@ -227,6 +227,35 @@ class PressureOutlet(BoundaryCondition):
         self.params = (p,)
 
 
+class FromOutside(BoundaryCondition):
+    """crates/cfd_boundary/src/lib.rs:108-147: the ghost state another code supplies per
+    boundary face through BoundaryInputs; faces without one copy the interior cell."""
+    kind = "table"
+
+
+class FnBoundary(BoundaryCondition):
+    """crates/cfd_boundary/src/lib.rs:152-170 for conditions that depend on where the face is:
+    fn(face_centroid) -> conserved state (wire order), evaluated once per face at prepare()
+    into a ghost table.  (A condition that reads the interior state each phase has no
+    device form here.)"""
+    kind = "table"
+
+    def __init__(self, fn):
+        self.fn = fn
+
+
+class BoundaryInputs:
+    """crates/cfd_boundary/src/lib.rs:108-111: supplied ghost states, here per side as
+    (states (n, 5), given (n,)), n = ny for x sides and nx for y sides."""
+
+    def __init__(self):
+        self.sides = {}
+
+    def set(self, side: "BoundarySide", states, given) -> "BoundaryInputs":
+        self.sides[side.value] = (np.asarray(states, dtype=np.float64), np.asarray(given, dtype=bool))
+        return self
+
+
 class BoundaryRegistry:
     """Per-side boundary conditions; every side defaults to SupersonicOutflow
     (crates/cfd_boundary/src/lib.rs:369-449)."""
@@ -393,6 +422,19 @@ class App:
         if gpu.strip_rows or gpu.waves_per_block:
             self.ctx.set_launch(gpu.strip_rows, gpu.waves_per_block)
         self.ctx.set_boundaries([(reg.get(BoundarySide(s)).kind, reg.get(BoundarySide(s)).params) for s in range(4)])
+        # ghost tables: FnBoundary from the face centroids, FromOutside from BoundaryInputs
+        dx = (hi[0] - lo[0]) / m.nx
+        dy = (hi[1] - lo[1]) / m.ny
+        for s in range(4):
+            bc = reg.get(BoundarySide(s))
+            if isinstance(bc, FnBoundary):
+                n = m.ny if s < 2 else m.nx
+                if s < 2:
+                    fc = [(lo[0] if s == 0 else hi[0], lo[1] + (k + 0.5) * dy, 0.5 * (lo[2] + hi[2])) for k in range(n)]
+                else:
+                    fc = [(lo[0] + (k + 0.5) * dx, lo[1] if s == 2 else hi[1], 0.5 * (lo[2] + hi[2])) for k in range(n)]
+                self.ctx.set_ghost_table(s, np.array([bc.fn(c) for c in fc]), np.ones(n, bool))
+        self.apply_boundary_inputs()
         inflow = self.resources.get(ParabolicInflow)
         if inflow:
             self.ctx.set_parabolic_inflow(inflow.u_max, inflow.height, inflow.rho)
@@ -418,6 +460,14 @@ class App:
         self.ctx.upload(u.reshape(m.ny, m.nx, 5))
         self.mesh = m
         return self
+
+    def apply_boundary_inputs(self):
+        """Push the BoundaryInputs resource to the FromOutside sides (call again after changing it)."""
+        inputs = self.resources.get(BoundaryInputs)
+        if inputs is None:
+            return
+        for s, (states, given) in inputs.sides.items():
+            self.ctx.set_ghost_table(s, states, given)
 
     def cell_centroids(self):
         m = self._find(FieldDefaultPlugins).mesh

@@ -69,6 +69,7 @@ struct efvm_ctx {
     int cur = 0;                   // index of the set holding U^n (A)
     DevClock* clock = nullptr;
     double* lam_part = nullptr;    // FP32: per-wave maxima of the fused CFL
+    double* bc_tab[4] = {nullptr, nullptr, nullptr, nullptr};   // EFVM_BC_TABLE ghost tables
     double* yw = nullptr;
     uint8_t* solid = nullptr;
     int* ghost_idx = nullptr;
@@ -133,6 +134,7 @@ StageArgs base_args(const efvm_ctx* c) {
     a.seam_mode = c->sch.seam_mode;
     for (int s = 0; s < 4; s++) {
         a.bc_kind[s] = c->bc[s].kind;
+        a.bc_tab[s] = c->bc[s].kind == EFVM_BC_TABLE ? c->bc_tab[s] : nullptr;
         for (int q = 0; q < 5; q++) a.bc_par[s][q] = c->bc[s].par[q];
         if (c->bc[s].kind == EFVM_BC_SUPERSONIC_INFLOW) {
             const double* p = c->bc[s].par;
@@ -369,6 +371,8 @@ void efvm_ctx_destroy(efvm_ctx* c) {
     for (int s = 0; s < 3; s++) for (int m = 0; m < 4; m++) if (c->P[s][m]) hipFree(c->P[s][m]);
     if (c->clock) hipFree(c->clock);
     if (c->lam_part) (void)hipFree(c->lam_part);
+    for (int s = 0; s < 4; s++)
+        if (c->bc_tab[s]) (void)hipFree(c->bc_tab[s]);
     if (c->yw) hipFree(c->yw);
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->snap) (void)hipFree(c->snap);
@@ -398,13 +402,33 @@ efvm_status efvm_set_launch(efvm_ctx* c, int strip_rows, int waves_per_block) {
 efvm_status efvm_set_boundaries(efvm_ctx* c, const efvm_bc sides[4]) {
     if (!c || !sides) return EFVM_ERR_ARG;
     for (int s = 0; s < 4; s++)
-        if (sides[s].kind < 0 || sides[s].kind > EFVM_BC_PRESSURE_OUTLET) return fail(c, EFVM_ERR_CONFIG, "unknown boundary kind");
+        if (sides[s].kind < 0 || sides[s].kind > EFVM_BC_TABLE) return fail(c, EFVM_ERR_CONFIG, "unknown boundary kind");
     for (int ax = 0; ax < 2; ax++)
         if ((sides[2 * ax].kind == EFVM_BC_PERIODIC) != (sides[2 * ax + 1].kind == EFVM_BC_PERIODIC))
             return fail(c, EFVM_ERR_CONFIG, std::string("periodic boundary on axis ") + std::to_string(ax) +
                                                 " requires both sides periodic (crates/cfd_boundary/src/lib.rs:437-448)");
     for (int s = 0; s < 4; s++) c->bc[s] = sides[s];
     drop_graph(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm_set_ghost_table(efvm_ctx* c, int side, const double* cons5, const unsigned char* given, size_t n) {
+    if (!c || !cons5 || !given || side < 0 || side > 3) return EFVM_ERR_ARG;
+    if (c->bc[side].kind != EFVM_BC_TABLE) return fail(c, EFVM_ERR_CONFIG, "ghost table on a side whose kind is not EFVM_BC_TABLE");
+    const size_t nf = side < 2 ? (size_t)c->grid.ny : (size_t)c->grid.nx;
+    if (n != nf) return fail(c, EFVM_ERR_ARG, "ghost table: one entry per boundary face (ny for x sides, nx for y sides)");
+    std::vector<double> h(6 * nf);
+    for (size_t k = 0; k < nf; k++) {
+        if (given[k] && cons5[5 * k + 3] != 0.0) return fail(c, EFVM_ERR_UNSUPPORTED, "rho_w must be 0 on the 2-D GPU path");
+        for (int m = 0; m < 5; m++) h[6 * k + m] = cons5[5 * k + m];
+        h[6 * k + 5] = given[k] ? 1.0 : 0.0;
+    }
+    if (!c->bc_tab[side]) {
+        HIPCHK(c, hipMalloc((void**)&c->bc_tab[side], sizeof(double) * 6 * nf));
+        drop_graph(c);   // a captured graph holds the stage arguments without this table
+    }
+    HIPCHK(c, hipMemcpyAsync(c->bc_tab[side], h.data(), sizeof(double) * 6 * nf, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;
 }
 
@@ -883,6 +907,8 @@ efvm_status efvm_batch_create(int n_cases, const efvm_case* cases, const efvm_sc
         if (c.nx < 1 || c.ny < 1 || !(c.hi_x > c.lo_x) || !(c.hi_y > c.lo_y) || !(c.dz > 0.0)) return EFVM_ERR_CONFIG;
         for (int ax = 0; ax < 2; ax++)
             if ((c.sides[2 * ax].kind == EFVM_BC_PERIODIC) != (c.sides[2 * ax + 1].kind == EFVM_BC_PERIODIC)) return EFVM_ERR_CONFIG;
+        for (int s = 0; s < 4; s++)
+            if (c.sides[s].kind < 0 || c.sides[s].kind > EFVM_BC_PRESSURE_OUTLET) return EFVM_ERR_UNSUPPORTED;   // no ghost tables
     }
     efvm_batch* b = new (std::nothrow) efvm_batch();
     if (!b) return EFVM_ERR_ARG;

@@ -17,7 +17,7 @@ namespace efvm {
 enum FluxKind { FLUX_RUSANOV = 0, FLUX_HLLC = 1, FLUX_HLLC_TEXTBOOK = 2 };
 enum LimiterKind { LIM_NONE = -1, LIM_MINMOD = 0, LIM_MC = 1 };
 enum BcKind { BC_OUTFLOW = 0, BC_PERIODIC = 1, BC_SLIP = 2, BC_NOSLIP = 3, BC_SYMMETRY = 4,
-              BC_SUP_INFLOW = 5, BC_SUB_INFLOW = 6, BC_PRESSURE_OUTLET = 7 };
+              BC_SUP_INFLOW = 5, BC_SUB_INFLOW = 6, BC_PRESSURE_OUTLET = 7, BC_TABLE = 8 };
 
 template <class T> struct C5 { T r, ru, rv, rw, re; };
 template <class T> struct P6 { T r, u, v, w, p, t; };

@@ -45,6 +45,8 @@ struct StageArgs {
     int bc_kind[4];              // XLo, XHi, YLo, YHi
     double bc_par[4][5];
     double bc_cons[4][5];        // precomputed supersonic-inflow states
+    const double* bc_tab[4];     // BC_TABLE: per boundary face (row of an x side, column of a y
+                                 // side) rho, rho_u, rho_v, rho_w, rho_E, given (0/1); null: none given
     int parabolic;               // x-lo parabolic velocity inlet (cylinder case)
     double inflow_umax, inflow_h, inflow_rho;
     double lo_x, lo_y, dx, dy, dz;
@@ -139,10 +141,20 @@ __device__ __forceinline__ Cell<T> load_plain(const StageArgs& a, const T* const
 // Ghost state for boundary side `side` from its owner cell (cfd_boundary:173-362),
 // evaluated in double with the reference's operation order, then rounded to T.
 template <class T>
-__device__ __forceinline__ Cell<T> bc_apply(const StageArgs& a, int side, Cell<T> in, int owner_row) {
+__device__ __forceinline__ Cell<T> bc_apply(const StageArgs& a, int side, Cell<T> in, int owner_row, int owner_col) {
     const int kind = a.bc_kind[side];
     const int axis = side >> 1;
     switch (kind) {
+    case BC_TABLE: {
+        // FromOutside (crates/cfd_boundary/src/lib.rs:108-147): the state another code supplied
+        // for this face, else its fallback SupersonicOutflow (a copy of the owner cell)
+        const double* t = a.bc_tab[side];
+        if (t) {
+            t += 6 * (axis == 0 ? owner_row : owner_col);
+            if (t[5] != 0.0) return {T(t[0]), T(t[1]), T(t[2]), T(t[4])};
+        }
+        return in;
+    }
     case BC_SLIP: case BC_SYMMETRY:
         if (axis == 0) in.ru = -in.ru; else in.rv = -in.rv;
         return in;
@@ -167,7 +179,6 @@ __device__ __forceinline__ Cell<T> bc_apply(const StageArgs& a, int side, Cell<T
         return {T(o.r), T(o.ru), T(o.rv), T(o.re)};
     }
     default:
-        (void)owner_row;
         return in;
     }
 }
@@ -195,12 +206,12 @@ __device__ __forceinline__ Cell<T> load_cell(const StageArgs& a, const T* const*
             C5<double> o = ref::prim_to_cons(p, a.gamma);
             return {T(o.r), T(o.ru), T(o.rv), T(o.re)};
         }
-        return bc_apply(a, side, own, orow);
+        return bc_apply(a, side, own, orow, oc);
     }
     int side = r < 0 ? 2 : 3;
     int orow = r < 0 ? 0 : ny - 1;
     Cell<T> own = load_plain(a, U, c, orow);
-    return bc_apply(a, side, own, orow);
+    return bc_apply(a, side, own, orow, c);
 }
 
 __device__ __forceinline__ void atomic_max_pos_double(double* addr, double v) {

@@ -466,7 +466,9 @@ int features_of(const StageArgs& a) {
     if (a.viscous) f |= FEAT_VISC;
     if (a.solid) f |= FEAT_IBM;
     for (int s = 0; s < 4; s++)
-        if (a.bc_kind[s] == BC_SUP_INFLOW || a.bc_kind[s] == BC_SUB_INFLOW || a.bc_kind[s] == BC_PRESSURE_OUTLET) f |= FEAT_CBC;
+        if (a.bc_kind[s] == BC_SUP_INFLOW || a.bc_kind[s] == BC_SUB_INFLOW || a.bc_kind[s] == BC_PRESSURE_OUTLET ||
+            a.bc_kind[s] == BC_TABLE)
+            f |= FEAT_CBC;
     if (a.parabolic) f |= FEAT_CBC;
     return f;
 }

@@ -20,7 +20,7 @@ STATUS_NAMES = {0: "EFVM_OK", 1: "EFVM_ERR_ARG", 2: "EFVM_ERR_CONFIG", 3: "EFVM_
 FLUX = {"rusanov": 0, "hllc": 1, "hllc_textbook": 2}
 LIMITER = {"minmod": 0, "mc": 1}
 BC = {"outflow": 0, "periodic": 1, "slip": 2, "noslip": 3, "symmetry": 4,
-      "supersonic_inflow": 5, "subsonic_inflow": 6, "pressure_outlet": 7}
+      "supersonic_inflow": 5, "subsonic_inflow": 6, "pressure_outlet": 7, "table": 8}
 
 # Every entry point include/efvm.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -30,6 +30,7 @@ EXPORTS = [
     "efvm_advance_to", "efvm_synchronize", "efvm_graph_capture", "efvm_graph_launch",
     "efvm_ibm_add_cylinder", "efvm_marker_force", "efvm_conservation_sums", "efvm_device_state",
     "efvm_last_elapsed_ms", "efvm_snapshot", "efvm_checkpoint_save", "efvm_checkpoint_load", "efvm_write_vtk",
+    "efvm_set_ghost_table",
 ]
 
 
@@ -112,6 +113,7 @@ def load_library(path: str = LIB_PATH):
         "efvm_device_state": [vp, C.POINTER(vp)],
         "efvm_last_elapsed_ms": [vp, pd],
         "efvm_snapshot": [vp, C.c_int],
+        "efvm_set_ghost_table": [vp, C.c_int, pd, C.POINTER(C.c_ubyte), C.c_size_t],
         "efvm_checkpoint_save": [vp, C.c_char_p],
         "efvm_checkpoint_load": [vp, C.c_char_p],
         "efvm_write_vtk": [C.POINTER(Grid), C.POINTER(Gas), pd, C.c_size_t, C.c_char_p],
@@ -267,6 +269,14 @@ class Context:
 
     def snapshot(self, restore=False):
         self._chk(self._L.efvm_snapshot(self._p, 1 if restore else 0))
+
+    def set_ghost_table(self, side: int, states, given):
+        """BoundaryInputs of a FromOutside side (kind "table"): states (n, 5) in the wire order,
+        given (n,) bools; faces not given copy the interior cell.  side 0..3 = XLo, XHi, YLo, YHi."""
+        st = np.ascontiguousarray(np.asarray(states, dtype=np.float64).reshape(-1, 5))
+        g = np.ascontiguousarray(np.asarray(given, dtype=np.uint8).reshape(-1))
+        self._chk(self._L.efvm_set_ghost_table(self._p, int(side), _pd(st), g.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                               st.shape[0]))
 
     def checkpoint_save(self, path: str):
         """CFDRST01 restart file of the current state and clock (the reference's format)."""

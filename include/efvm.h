@@ -61,7 +61,9 @@ enum {
     EFVM_BC_SYMMETRY = 4,           /* Symmetry                                 */
     EFVM_BC_SUPERSONIC_INFLOW = 5,  /* par = rho, u, v, w, p                    */
     EFVM_BC_SUBSONIC_INFLOW = 6,    /* par = rho, u, v, w, T                    */
-    EFVM_BC_PRESSURE_OUTLET = 7     /* par[0] = p                               */
+    EFVM_BC_PRESSURE_OUTLET = 7,    /* par[0] = p                               */
+    EFVM_BC_TABLE = 8               /* FromOutside: ghost states supplied per face
+                                       (efvm_set_ghost_table), else outflow copy */
 };
 
 /* UniformMesh {nx, ny, nz=1, ng=2, bounds_lo, bounds_hi}: x in [lo_x, hi_x],
@@ -90,6 +92,14 @@ efvm_status efvm_set_launch(efvm_ctx* ctx, int strip_rows, int waves_per_block);
 /* BoundaryRegistry + StructuredBoundaryPlugin (crates/cfd_boundary/src/lib.rs:369-533):
  * sides[4] = XLo, XHi, YLo, YHi.  A half-periodic axis is EFVM_ERR_CONFIG. */
 efvm_status efvm_set_boundaries(efvm_ctx* ctx, const efvm_bc sides[4]);
+/* FromOutside + BoundaryInputs (crates/cfd_boundary/src/lib.rs:108-147; the reference test
+ * crates/cfd_solver/tests/boundary_face.rs:96-118): ghost states another code supplies for
+ * the faces of a side whose kind is EFVM_BC_TABLE.  n = ny for an x side, nx for a y side;
+ * cons5 holds n states in the wire order (rho_w = 0); given[k] = 0 leaves face k to the
+ * fallback, a copy of the interior cell (SupersonicOutflow).  A position-dependent FnBoundary
+ * is the same table filled from the face centroids.  May be called again between steps
+ * (also with a captured graph: the table is read in place). */
+efvm_status efvm_set_ghost_table(efvm_ctx* ctx, int side, const double* cons5, const unsigned char* given, size_t n);
 /* Case systems of examples/cylinder_drag_validation/main.rs:151-229. */
 efvm_status efvm_set_parabolic_inflow(efvm_ctx* ctx, double u_max, double height, double rho);
 efvm_status efvm_set_sponge(efvm_ctx* ctx, double x_start, double x_end, double sigma, double u_max, double height,

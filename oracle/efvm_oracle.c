@@ -58,7 +58,7 @@ enum { LIM_MINMOD = 0, LIM_MC = 1 };
 enum { INTEG_EULER = 0, INTEG_RK3 = 1 };
 enum { VISC_SUTHERLAND = 0, VISC_CONSTANT = 1 };
 enum { BC_OUTFLOW = 0, BC_PERIODIC = 1, BC_SLIP = 2, BC_NOSLIP = 3, BC_SYMMETRY = 4,
-       BC_SUP_INFLOW = 5, BC_SUB_INFLOW = 6, BC_PRESSURE_OUTLET = 7 };
+       BC_SUP_INFLOW = 5, BC_SUB_INFLOW = 6, BC_PRESSURE_OUTLET = 7, BC_TABLE = 8 };
 enum { STAGE_FIRST = 0, STAGE_SECOND = 1, STAGE_THIRD = 2 };
 
 typedef struct {
@@ -97,6 +97,9 @@ typedef struct {
     long n_mark; double (*mark)[7];  /* pos[3], normal[3], area */
     double marker_force[6], wall_force[6];
     int ibm_ready;
+    /* FromOutside + BoundaryInputs (cfd_boundary:108-147): per boundary face of a side,
+       5 conserved values and a given flag; not given -> SupersonicOutflow */
+    double *tab[6]; long tab_n[6];
 } orc_sim;
 
 static inline long IDX(const orc_sim *s, int i, int j, int k) { return i + (long)s->NX * (j + (long)s->NY * k); }
@@ -303,6 +306,7 @@ orc_sim *orc_new(const orc_cfg *cfg) {
 void orc_free(orc_sim *s) {
     if (!s) return;
     free(s->u); free(s->rhs); free(s->u0); free(s->slope); free(s->prim); free(s->solid);
+    for (int q = 0; q < 6; q++) free(s->tab[q]);
     free(s->ghost_idx); free(s->ghost_img); free(s->mark);
     free(s);
 }
@@ -326,6 +330,17 @@ void orc_info(const orc_sim *s, double *dt, double *time, long *step, int *stage
     *dt = s->dt; *time = s->time; *step = s->step; *stage = s->stage;
 }
 void orc_set_clock(orc_sim *s, double dt, double time, long step, int stage) { s->dt = dt; s->time = time; s->step = step; s->stage = stage; }
+
+/* BoundaryInputs for a FromOutside side: n faces x (rho, rho_u, rho_v, rho_w, rho_E, given) */
+int orc_set_ghost_table(orc_sim *s, int side, const double *tab6, long n) {
+    if (side < 0 || side > 5 || s->c.bc[side] != BC_TABLE) return -1;
+    free(s->tab[side]);
+    s->tab[side] = (double *)malloc(sizeof(double) * 6 * (size_t)n);
+    if (!s->tab[side]) return -1;
+    memcpy(s->tab[side], tab6, sizeof(double) * 6 * (size_t)n);
+    s->tab_n[side] = n;
+    return 0;
+}
 
 /* ---------------- initial conditions (evaluated at every raw cell, cfd_solver:681-686) ---------------- */
 /* Shu isentropic vortex, examples/isentropic_vortex/main.rs:97-125; `periodic` selects the
@@ -499,6 +514,14 @@ static void boundary_system(orc_sim *s) {                           /* cfd_bound
             own[axis] = hi ? NG + n[axis] - 1 : NG;
             gh[axis] = hi ? NG + n[axis] : NG - 1;
             cv_t in = s->u[IDX(s, own[0], own[1], own[2])];
+            if (s->c.bc[side] == BC_TABLE) {
+                /* the face index along the side: the row of an x side, the column of a y side */
+                long f = axis == 0 ? q1 : (axis == 1 ? q2 : 0);
+                const double *t = (s->tab[side] && f < s->tab_n[side]) ? s->tab[side] + 6 * f : NULL;
+                if (t && t[5] != 0.0) { cv_t g = {t[0], t[1], t[2], t[3], t[4]}; s->u[IDX(s, gh[0], gh[1], gh[2])] = g; }
+                else s->u[IDX(s, gh[0], gh[1], gh[2])] = in;
+                continue;
+            }
             s->u[IDX(s, gh[0], gh[1], gh[2])] = bc_ghost(s, side, &in);
         }
     }

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "_build", "liborc.so")
 FLUX = {"rusanov": 0, "hllc": 1, "hllc_textbook": 2}
 LIMITER = {"minmod": 0, "mc": 1}
 BC = {"outflow": 0, "periodic": 1, "slip": 2, "noslip": 3, "symmetry": 4,
-      "supersonic_inflow": 5, "subsonic_inflow": 6, "pressure_outlet": 7}
+      "supersonic_inflow": 5, "subsonic_inflow": 6, "pressure_outlet": 7, "table": 8}
 R_AIR = 287.058
 
 
@@ -75,6 +75,8 @@ def lib():
             getattr(L, n).argtypes = [C.c_void_p, pd]
         L.orc_info.argtypes = [C.c_void_p, pd, pd, C.POINTER(C.c_long), C.POINTER(C.c_int)]
         L.orc_set_clock.argtypes = [C.c_void_p, d, d, C.c_long, C.c_int]
+        L.orc_set_ghost_table.restype = C.c_int
+        L.orc_set_ghost_table.argtypes = [C.c_void_p, C.c_int, pd, C.c_long]
         L.orc_run.restype = C.c_int
         L.orc_run.argtypes = [C.c_void_p]
         L.orc_step.argtypes = [C.c_void_p]
@@ -266,6 +268,16 @@ class Sim:
         st, sg = C.c_long(), C.c_int()
         lib().orc_info(self.ptr, C.byref(dt), C.byref(t), C.byref(st), C.byref(sg))
         return dict(dt=dt.value, time=t.value, step=st.value, stage=sg.value)
+
+    def set_ghost_table(self, side: int, states, given):
+        """BoundaryInputs of a FromOutside ("table") side: states (n, 5) wire order, given (n,) bools."""
+        st = np.asarray(states, dtype=np.float64).reshape(-1, 5)
+        t = np.zeros((st.shape[0], 6))
+        t[:, :5] = st
+        t[:, 5] = np.asarray(given, dtype=bool)
+        t = np.ascontiguousarray(t)
+        if lib().orc_set_ghost_table(self.ptr, side, _p(t), st.shape[0]) != 0:
+            raise ValueError("ghost table on a side whose kind is not 'table'")
 
     def set_clock(self, dt=0.0, time=0.0, step=0, stage=0):
         lib().orc_set_clock(self.ptr, dt, time, step, stage)

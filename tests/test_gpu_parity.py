@@ -169,6 +169,73 @@ def test_inflow_boundaries_fp64():
     assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
 
 
+@pytest.mark.parametrize("muscl", [False, True])
+def test_ghost_table_boundaries_fp64(muscl):
+    """FromOutside / BoundaryInputs (crates/cfd_boundary/src/lib.rs:108-147) on the device: an x-lo
+    inlet profile from the face centroids (boundary_face.rs:77-94, FnBoundary as a table) and an
+    x-hi side with every other row supplied (:96-118, the rest copying the interior), against
+    the oracle; then a new table between steps, replayed from a captured graph."""
+    nx, ny = 40, 16
+    sim, ctx = make_pair(nx, ny, (1.0, 1.0, 0.05), ["table", "table", "slip", "slip"], muscl=muscl, limiter="mc")
+    x, y = cases.cell_centroids(nx, ny, (0, 0), (1.0, 1.0))
+    p0 = 1e5
+    st = cases.prim_to_cons(1.4, 1.0 + 0.1 * np.sin(5 * x), 30.0 + 0 * x, 0 * x, p0 + 0 * x)
+    load_both(sim, ctx, st)
+    yc = (np.arange(ny) + 0.5) / ny
+    inlet = np.array([orc.prim_to_cons([1.1, 400.0 * yy * (1.0 - yy), 0.0, 0.0, p0, p0 / (1.1 * R)], 1.4) for yy in yc])
+    given = np.arange(ny) % 2 == 0
+    outside = np.array([orc.prim_to_cons([0.9 + 0.01 * j, -20.0, 3.0, 0.0, 0.95 * p0, 300.0], 1.4) for j in range(ny)])
+    for obj in (sim, ctx):
+        obj.set_ghost_table(0, inlet, np.ones(ny, bool))
+        obj.set_ghost_table(1, outside, given)
+    sim.step(20)
+    ctx.step(20)
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
+    # a new outside state, picked up by the captured graph in place
+    ctx.graph_capture(4)
+    outside2 = outside.copy()
+    outside2[:, 1] *= -1.0
+    for obj in (sim, ctx):
+        obj.set_ghost_table(1, outside2, ~given)
+    sim.step(4)
+    ctx.graph_launch()
+    ctx.synchronize()
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
+
+
+def test_app_from_outside_and_fn_boundary():
+    """The plugin mirror of boundary_face.rs: FnBoundary inlet at x-lo, FromOutside with
+    BoundaryInputs at x-hi (rows 0 and 2 given), HLLC first order, Euler, dt 1e-9."""
+    nx, ny = 6, 4
+    p0 = 1e5
+    air = lambda rho, u: orc.prim_to_cons([rho, u, 0.0, 0.0, p0, p0 / (rho * R)], 1.4)
+    app = efvm.App()
+    app.add_plugins(efvm.FieldDefaultPlugins(efvm.UniformMeshConfig(nx=nx, ny=ny, nz=1, ng=2, bounds_lo=(0, 0, 0),
+                                                                    bounds_hi=(1.0, 1.0, 0.1))))
+    app.add_plugins(efvm.CfdStatePlugin(np.tile(air(1.0, 0.0), (ny, nx, 1))))
+    reg = (efvm.BoundaryRegistry()
+           .with_(efvm.BoundarySide.XLo, efvm.FnBoundary(lambda c: air(1.0, 400.0 * c[1] * (1.0 - c[1]))))
+           .with_(efvm.BoundarySide.XHi, efvm.FromOutside()).with_axis(1, efvm.SlipWall()).with_axis(2, efvm.SlipWall()))
+    app.add_plugins(efvm.StructuredBoundaryPlugin(reg))
+    app.add_plugins(efvm.FluxPlugin.hllc())
+    app.add_plugins(efvm.IntegratorPlugin.euler())
+    app.add_plugins(efvm.SolverPlugin(efvm.SolverConfig(cfl=0.4, fixed_dt=1e-9)))
+    given = [j % 2 == 0 for j in range(ny)]
+    outside = [air(1.0 + 0.1 * j, -50.0) for j in range(ny)]
+    app.add_resource(efvm.BoundaryInputs().set(efvm.BoundarySide.XHi, outside, given))
+    app.prepare()
+    app.run()
+    sim = orc.Sim(nx, ny, 1, hi=(1.0, 1.0, 0.1), flux="hllc", integrator="euler", fixed_dt=1e-9,
+                  bcs=["table", "table", "slip", "slip", "slip", "slip"])
+    sim.set_interior(np.tile(air(1.0, 0.0), (1, ny, nx, 1)))
+    y = (np.arange(ny) + 0.5) / ny
+    sim.set_ghost_table(0, [air(1.0, 400.0 * yy * (1.0 - yy)) for yy in y], [True] * ny)
+    sim.set_ghost_table(1, outside, given)
+    sim.run()
+    assert rel_linf(app.state(), ora_state(sim)) <= TOL64
+    assert not np.array_equal(app.state()[:, 0], app.state()[:, 2])   # the inlet did act on the first column
+
+
 def test_sod_tube_ny1_fp64():
     """crates/cfd_solver/tests/sod_exact_convergence.rs setup (ny = 1) on the GPU."""
     n = 200

@@ -307,3 +307,49 @@ def test_planted_mistake_is_caught_by_published_figure():
     s.ic_vortex(10.0, 5.0, 1.4, 0.5, 0.5)
     s.advance_to(1.0)
     assert abs(s.vortex_l1() / PUB["vortex_first_order"]["L1"][0] - 1.0) > 1e-3
+
+
+def _table_case(x_lo="outflow", x_hi="outflow"):
+    """crates/cfd_solver/tests/boundary_face.rs:30-55: 6 x 4 cells, HLLC first order, Euler,
+    fixed dt 1e-9, uniform air at rest (rho 1, p 1e5); the x conditions under test."""
+    s = orc.Sim(6, 4, 1, hi=(1.0, 1.0, 0.1), flux="hllc", integrator="euler", fixed_dt=1e-9,
+                bcs=[x_lo, x_hi, "slip", "slip", "slip", "slip"])
+    rest = air_state(1.0, 0.0)
+    s.set_interior(np.tile(rest, (1, 4, 6, 1)))
+    return s
+
+
+def air_state(rho, u):
+    p = 1e5
+    return orc.prim_to_cons(np.array([rho, u, 0.0, 0.0, p, p / (rho * orc.R_AIR)]), 1.4)
+
+
+def test_from_outside_supplies_ghost_states():
+    """boundary_face.rs:96-118: rows 0 and 2 of the x-hi ghost layer take the supplied states,
+    the others fall back to a copy of the interior (SupersonicOutflow)."""
+    s = _table_case(x_hi="table")
+    given = [j % 2 == 0 for j in range(4)]
+    states = [air_state(1.0 + 0.1 * j, -50.0) if g else np.zeros(5) for j, g in enumerate(given)]
+    s.set_ghost_table(1, states, given)
+    s.run()
+    raw = s.raw()
+    for j in range(4):
+        ghost = raw[2, j + 2, 6 + 2]              # first ghost layer beyond x-hi
+        if given[j]:
+            assert np.array_equal(ghost, states[j]), j
+        else:
+            assert ghost[1] == 0.0, j             # copies the interior (at rest)
+
+
+def test_position_dependent_condition_as_a_table():
+    """boundary_face.rs:77-94 (FnBoundary): a parabolic inlet 400 y (1 - y) from the face
+    centroids, given for every x-lo face."""
+    s = _table_case(x_lo="table")
+    y = (np.arange(4) + 0.5) / 4.0
+    states = [air_state(1.0, 400.0 * yy * (1.0 - yy)) for yy in y]
+    s.set_ghost_table(0, states, [True] * 4)
+    s.run()
+    raw = s.raw()
+    speeds = raw[2, 2:6, 1, 1]                    # rho_u of the first ghost layer beyond x-lo
+    assert np.all(np.abs(speeds - 400.0 * y * (1.0 - y)) < 1e-12)
+    assert speeds[1] > speeds[0]
