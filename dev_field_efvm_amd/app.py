@@ -18,7 +18,9 @@ crates/cfd_boundary/src/lib.rs:457-533, crates/cfd_ibm/src/lib.rs:621-680):
 Here the plugins only collect configuration; `prepare()` creates one GPU
 context through the C ABI (include/efvm.h) and every `run()` enqueues the
 fused stage kernels.  `GpuPlugin` selects precision (FP64 parity / FP32
-production), the periodic-seam mode and the launch shape.
+production), the periodic-seam mode and the launch shape.  A one-cell-thick
+mesh with slip or symmetry z sides runs on the 2-D context (efvm_*); any other
+mesh (nz > 1, other z sides, a sphere) on the 3-D context (efvm3_*).
 """
 from __future__ import annotations
 
@@ -29,7 +31,7 @@ from typing import Callable, Optional, Sequence
 
 import numpy as np
 
-from .efvm import Context, EfvmError
+from .efvm import Context, Context3, EfvmError, ERR_UNSUPPORTED
 
 R_AIR = 287.058
 
@@ -52,7 +54,8 @@ class FieldDefaultPlugins:
 
 class CfdStatePlugin:
     """Initial condition: a callable init(x, y) -> (rho, rho_u, rho_v, rho_w, rho_E) over
-    cell-centroid arrays, or an array of shape (ny, nx, 5) (crates/cfd_solver/src/lib.rs:691-719)."""
+    cell-centroid arrays (init(x, y, z) on a 3-D mesh), or an array of shape (ny, nx, 5)
+    ((nz, ny, nx, 5)) (crates/cfd_solver/src/lib.rs:691-719)."""
 
     def __init__(self, init):
         self.init = init
@@ -320,6 +323,40 @@ class CylinderIbmPlugin:
 
 
 @dataclass
+class SphereBody:
+    """A fixed sphere (crates/cfd_ibm/src/sphere.rs:27-46, SphereBody::fixed)."""
+    center: Sequence[float]
+    radius: float
+
+    @classmethod
+    def fixed(cls, center, radius):
+        return cls(tuple(center), radius)
+
+
+class SphereIbmPlugin:
+    """SphereIbmPlugin::new(bodies, freestream).with_markers(n) (sphere.rs:239-278)."""
+
+    def __init__(self, bodies, freestream):
+        self.bodies = list(bodies)
+        self.freestream = np.asarray(freestream, dtype=np.float64)
+        self.markers = 2000
+
+    def with_markers(self, n_markers: int) -> "SphereIbmPlugin":
+        self.markers = int(n_markers)
+        return self
+
+
+@dataclass
+class Sponge:
+    """The uniform-freestream outlet sponge of examples/sphere_drag_validation/main.rs:110-160:
+    after each completed step u += min(sigma s^2 dt, 1) (target - u) for x >= x_start."""
+    x_start: float
+    x_end: float
+    sigma: float
+    target: Sequence[float]
+
+
+@dataclass
 class ParabolicInflow:
     """examples/cylinder_drag_validation/main.rs:146-177."""
     u_max: float
@@ -394,8 +431,8 @@ class App:
         if mesh is None:
             raise ValueError("mesh plugin missing (FieldDefaultPlugins)")
         m = mesh.mesh
-        if m.nz != 1 or m.ng != 2:
-            raise EfvmError(4, "the GPU path is the 2-D nz = 1, ng = 2 UniformMesh")
+        if m.ng != 2:
+            raise EfvmError(ERR_UNSUPPORTED, "the GPU path keeps ng = 2 ghost layers")
         gasp = self._find(GasPlugin) or IdealGasPlugin()
         flux = self._find(FluxPlugin)
         if flux is None:
@@ -409,9 +446,10 @@ class App:
         visc = self.resources.get(Viscosity, Viscosity.sutherland())
         reg = bcp.registry
         reg.validate_periodic_pairs()
-        for s in (BoundarySide.ZLo, BoundarySide.ZHi):
-            if reg.get(s).kind not in ("slip", "symmetry"):
-                raise EfvmError(4, "z sides must be slip walls on the 2-D GPU path")
+        self.mesh = m
+        thin = m.nz == 1 and all(reg.get(s).kind in ("slip", "symmetry") for s in (BoundarySide.ZLo, BoundarySide.ZHi))
+        if not thin or self._find(SphereIbmPlugin) is not None or Sponge in self.resources:
+            return self._prepare3(m, gasp, flux, integ, solver, reg, gpu, visc)
         lo, hi = m.bounds_lo, m.bounds_hi
         cfg = solver.config
         self.ctx = Context(m.nx, m.ny, lo=(lo[0], lo[1]), hi=(hi[0], hi[1]), dz=hi[2] - lo[2],
@@ -458,7 +496,46 @@ class App:
         else:
             u = np.asarray(init, dtype=np.float64)
         self.ctx.upload(u.reshape(m.ny, m.nx, 5))
-        self.mesh = m
+        return self
+
+    def _prepare3(self, m, gasp, flux, integ, solver, reg, gpu, visc):
+        """The 3-D context: six sides, bodies, the uniform sponge, body force."""
+        cfg = solver.config
+        for s in range(6):
+            if isinstance(reg.get(BoundarySide(s)), (FromOutside, FnBoundary)):
+                raise EfvmError(ERR_UNSUPPORTED, "FromOutside / FnBoundary tables are 2-D only")
+        if ParabolicInflow in self.resources or OutletSponge in self.resources:
+            raise EfvmError(ERR_UNSUPPORTED, "the parabolic channel inflow and sponge are 2-D cases")
+        self.ctx = Context3(m.nx, m.ny, m.nz, lo=tuple(m.bounds_lo), hi=tuple(m.bounds_hi), gamma=gasp.gas.gamma,
+                            r_gas=gasp.gas.r_gas, mu=visc.mu, flux=flux.flux, muscl=flux.muscl,
+                            limiter=flux.limiter.value, integrator=integ.kind, precision=gpu.precision,
+                            viscous=cfg.viscous, seam_mode=gpu.seam_mode, cfl=cfg.cfl, fixed_dt=cfg.fixed_dt)
+        self.ctx.set_boundaries([(reg.get(BoundarySide(s)).kind, reg.get(BoundarySide(s)).params) for s in range(6)])
+        sp = self.resources.get(Sponge)
+        if sp:
+            self.ctx.set_sponge(sp.x_start, sp.x_end, sp.sigma, sp.target)
+        bf = self.resources.get(BodyForceX)
+        if bf:
+            self.ctx.set_body_force(bf.gx)
+        sph = self._find(SphereIbmPlugin)
+        cyl = self._find(CylinderIbmPlugin)
+        if sph:
+            for b in sph.bodies:
+                self.ctx.add_sphere(b.center, b.radius, sph.freestream, markers=sph.markers)
+        if cyl:
+            for cy in cyl.cylinders:
+                self.ctx.add_cylinder(cy.center_xy[0], cy.center_xy[1], cy.radius, cyl.freestream)
+        st = self._find(CfdStatePlugin)
+        if st is None:
+            raise ValueError("CfdStatePlugin missing")
+        init = st.init
+        if callable(init):
+            x, y, z = self.cell_centroids()
+            u = np.asarray(init(x, y, z), dtype=np.float64)
+        else:
+            u = np.asarray(init, dtype=np.float64)
+        self.ctx.upload(np.broadcast_to(u, (m.nz, m.ny, m.nx, 5)).copy() if u.shape == (5,)
+                        else u.reshape(m.nz, m.ny, m.nx, 5))
         return self
 
     def apply_boundary_inputs(self):
@@ -470,12 +547,14 @@ class App:
             self.ctx.set_ghost_table(s, states, given)
 
     def cell_centroids(self):
+        """x, y (2-D context) or x, y, z (3-D context) centroid arrays, x fastest."""
         m = self._find(FieldDefaultPlugins).mesh
-        dx = (m.bounds_hi[0] - m.bounds_lo[0]) / m.nx
-        dy = (m.bounds_hi[1] - m.bounds_lo[1]) / m.ny
-        x = m.bounds_lo[0] + (np.arange(m.nx) + 0.5) * dx
-        y = m.bounds_lo[1] + (np.arange(m.ny) + 0.5) * dy
-        return np.meshgrid(x, y)
+        c = [m.bounds_lo[d] + (np.arange(n) + 0.5) * ((m.bounds_hi[d] - m.bounds_lo[d]) / n)
+             for d, n in enumerate((m.nx, m.ny, m.nz))]
+        if isinstance(self.ctx, Context3):
+            z, y, x = np.meshgrid(c[2], c[1], c[0], indexing="ij")
+            return x, y, z
+        return np.meshgrid(c[0], c[1])
 
     # one app.run() = one RK substage
     def run(self):
@@ -485,6 +564,12 @@ class App:
         self.ctx.step(n)
 
     def advance_to(self, t_final: float, max_steps: int = 10 ** 9) -> int:
+        if isinstance(self.ctx, Context3):
+            k = 0
+            while self.ctx.clock()["time"] < t_final and k < max_steps:
+                self.ctx.step(1)
+                k += 1
+            return k
         return self.ctx.advance_to(t_final, max_steps)
 
     def solver_state(self) -> SolverState:
@@ -492,7 +577,7 @@ class App:
         return SolverState(c["dt"], c["time"], c["step"], c["stage"])
 
     def state(self) -> np.ndarray:
-        """CfdState.u of the interior, shape (ny, nx, 5), Component wire order."""
+        """CfdState.u of the interior, shape (ny, nx, 5) ((nz, ny, nx, 5) in 3-D), Component wire order."""
         return self.ctx.download()
 
     def marker_force(self) -> np.ndarray:
@@ -503,13 +588,20 @@ class App:
 
     # restart files (examples/restart_determinism/main.rs:177-226 write_checkpoint / read_checkpoint)
     def write_checkpoint(self, path: str):
+        self._need_2d("restart files")
         self.ctx.checkpoint_save(path)
 
     def read_checkpoint(self, path: str):
+        self._need_2d("restart files")
         self.ctx.checkpoint_load(path)
+
+    def _need_2d(self, what):
+        if isinstance(self.ctx, Context3):
+            raise EfvmError(ERR_UNSUPPORTED, f"{what} are built for the 2-D context")
 
     def finish(self):
         """The cleanup hooks: OutputPlugin writes its VTK file."""
         out = self._find(OutputPlugin)
         if out is not None:
+            self._need_2d("VTK output files")
             self.ctx.write_vtk(out.path)

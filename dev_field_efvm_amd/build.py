@@ -25,7 +25,9 @@ UNITS = [
     ("efvm_stage_f32.hip", ["-ffp-contract=fast", "-fgpu-flush-denormals-to-zero"]),
     ("efvm_mlp.hip", ["-ffp-contract=fast"]),
     ("efvm_batch.hip", ["-ffp-contract=fast"]),
+    ("efvm_stage3d.hip", ["-ffp-contract=off"]),
     ("efvm_api.cpp", ["-ffp-contract=off"]),
+    ("efvm_api3d.cpp", ["-ffp-contract=off"]),
 ]
 
 

@@ -1,0 +1,655 @@
+// efvm_api3d.cpp -- C ABI of the 3-D structured-mesh context (efvm3_*).
+//
+// The host side of one App over a UniformMesh with nz cells along z: it owns the
+// device planes (three raw padded state sets, rotated through the RK substages),
+// the clock, the immersed-body tables, and enqueues the phase kernels of
+// efvm_stage3d.hip on its stream.  Compiled with -ffp-contract=off: the geometry
+// built here (viscous weights, image points, markers) is FP64 in the reference's
+// operation order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/efvm.h"
+#include "efvm_stage3d.h"
+
+using namespace efvm;
+
+static constexpr int NG3 = 2;
+
+struct efvm_ctx3 {
+    efvm_grid3 grid{};
+    efvm_gas gas{};
+    efvm_scheme sch{};
+    efvm_bc bc[6]{};
+    double h[3] = {0, 0, 0};
+    int NX = 0, NY = 0, NZ = 0;
+    long nraw = 0, ncell = 0;
+    hipStream_t stream = nullptr;
+    size_t esz = 8;
+    void* P[3][5] = {};      // raw padded state sets A, B, C
+    int cur = 0;             // set holding U^n
+    DevClock* clock = nullptr;
+    double* part = nullptr;  // CFL per-block maxima, then sums partials (5 per block)
+    double* out = nullptr;   // marker force (6) and sums (5)
+    double* yw = nullptr;
+    double* zw = nullptr;
+    uint8_t* solid = nullptr;     // raw-indexed
+    long* ghost_idx = nullptr;
+    double* ghost_img = nullptr;
+    int n_ghost = 0;
+    double* markers = nullptr;
+    int n_mark = 0;
+    double* stagnant = nullptr;
+    int ibm_flipped = 1;
+    int body_kind = 0;            // 0 none, 1 cylinders, 2 spheres
+    int n_body = 0;
+    double body[8][4] = {};       // cx, cy, cz, r (cylinders: cz unused)
+    int sphere_markers = 2000;
+    double freestream[5] = {};
+    int rk_stage = 0;
+    int parity = 0;
+    int sponge = 0;
+    Sponge3 sp{};
+    int body_force = 0;
+    double gx = 0.0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int timed = 0;
+    std::string err;
+};
+
+namespace {
+
+efvm_status fail3(efvm_ctx3* c, efvm_status s, const std::string& msg) {
+    if (c) c->err = msg;
+    return s;
+}
+#define HIPCHK3(ctx, expr)                                                                               \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) return fail3(ctx, EFVM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+bool periodic3(const efvm_ctx3* c, int axis) { return c->bc[2 * axis].kind == EFVM_BC_PERIODIC; }
+double centroid3(const efvm_ctx3* c, int d, int raw) { return c->grid.lo[d] + ((double)(raw - NG3) + 0.5) * c->h[d]; }
+long raw_index(const efvm_ctx3* c, int i, int j, int k) { return i + (long)c->NX * (j + (long)c->NY * k); }
+
+Args3 args3(const efvm_ctx3* c) {
+    Args3 a;
+    std::memset(&a, 0, sizeof(a));
+    a.nx = c->grid.nx; a.ny = c->grid.ny; a.nz = c->grid.nz;
+    a.NX = c->NX; a.NY = c->NY; a.NZ = c->NZ;
+    a.nraw = c->nraw; a.ncell = c->ncell;
+    for (int d = 0; d < 3; d++) { a.lo[d] = c->grid.lo[d]; a.h[d] = c->h[d]; a.per[d] = periodic3(c, d); }
+    const double vol = c->h[0] * c->h[1] * c->h[2];
+    a.aov[0] = (c->h[1] * c->h[2]) / vol;
+    a.aov[1] = (c->h[0] * c->h[2]) / vol;
+    a.aov[2] = (c->h[0] * c->h[1]) / vol;
+    a.seam_mode = c->sch.seam_mode;
+    for (int s = 0; s < 6; s++) {
+        a.bc_kind[s] = c->bc[s].kind;
+        for (int q = 0; q < 5; q++) a.bc_par[s][q] = c->bc[s].par[q];
+        if (c->bc[s].kind == EFVM_BC_SUPERSONIC_INFLOW) {
+            const double* p = c->bc[s].par;
+            P6<double> pv = {p[0], p[1], p[2], p[3], p[4], p[4] / (p[0] * 287.058)};
+            C5<double> o = ref::prim_to_cons(pv, c->gas.gamma);
+            a.bc_cons[s][0] = o.r; a.bc_cons[s][1] = o.ru; a.bc_cons[s][2] = o.rv; a.bc_cons[s][3] = o.rw; a.bc_cons[s][4] = o.re;
+        }
+    }
+    a.gamma = c->gas.gamma; a.r_gas = c->gas.r_gas;
+    a.flux = c->sch.flux; a.muscl = c->sch.recon == EFVM_RECON_MUSCL; a.limiter = c->sch.limiter;
+    a.viscous = c->sch.viscous; a.visc_const = c->gas.viscosity == EFVM_VISC_CONSTANT; a.mu = c->gas.mu;
+    a.dxi2 = 1.0 / (c->h[0] * c->h[0]);
+    a.yw = c->yw; a.zw = c->zw;
+    a.body_force = c->body_force; a.gx = c->gx;
+    a.solid = c->n_body > 0 ? c->solid : nullptr;
+    a.euler = c->sch.integrator == EFVM_INTEGRATOR_EULER;
+    a.cfl = c->sch.cfl; a.fixed_dt = c->sch.fixed_dt;
+    a.clock = c->clock;
+    return a;
+}
+
+void planes3(Args3& a, void* const* in, void* const* u0, void* const* out) {
+    for (int m = 0; m < 5; m++) { a.uin[m] = in[m]; a.u0[m] = u0[m]; a.uout[m] = out[m]; }
+}
+
+// the set holding the state of the current substage: A at a step boundary, else B / C
+int current_set(const efvm_ctx3* c) {
+    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 1) return (c->cur + 1) % 3;
+    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 2) return (c->cur + 2) % 3;
+    return c->cur;
+}
+
+void drop_graph3(efvm_ctx3* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr; c->graph = nullptr;
+}
+
+void free_ibm3(efvm_ctx3* c) {
+    if (c->solid) (void)hipFree(c->solid);
+    if (c->ghost_idx) (void)hipFree(c->ghost_idx);
+    if (c->ghost_img) (void)hipFree(c->ghost_img);
+    if (c->markers) (void)hipFree(c->markers);
+    if (c->stagnant) (void)hipFree(c->stagnant);
+    c->solid = nullptr; c->ghost_idx = nullptr; c->ghost_img = nullptr; c->markers = nullptr; c->stagnant = nullptr;
+    c->n_ghost = 0; c->n_mark = 0;
+}
+
+#define LAUNCH3(ctx, expr, what)                                                                         \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) return fail3(ctx, EFVM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// One app.run(): Setup (CFL on the first substage, mask refresh) -> Halo -> Boundary ->
+// PreFlux (mirror) -> Flux .. Update (fused) -> PostUpdate (sponge).  Enqueue only.
+efvm_status run3(efvm_ctx3* c) {
+    const bool rk3 = c->sch.integrator == EFVM_INTEGRATOR_RK3;
+    void* const* A = c->P[c->cur];
+    void* const* B = rk3 ? c->P[(c->cur + 1) % 3] : c->P[c->cur ^ 1];
+    void* const* Cc = c->P[(c->cur + 2) % 3];
+    const int esz = (int)c->esz;
+    Args3 a = args3(c);
+    a.parity = c->parity;
+    a.stage = rk3 ? c->rk_stage : 0;
+    void* const* in = A;
+    void* const* out = B;
+    if (rk3 && c->rk_stage == 1) { in = B; out = Cc; }
+    if (rk3 && c->rk_stage == 2) { in = Cc; out = A; }
+    planes3(a, in, A, out);
+    if (c->rk_stage == 0 && c->sch.fixed_dt <= 0.0)
+        LAUNCH3(c, launch3_cfl(a, esz, c->part, &c->clock->lam[c->parity], c->stream), "cfl");
+    if (c->n_body > 0 && !c->ibm_flipped) {
+        LAUNCH3(c, launch3_flip(a, esz, c->stagnant, c->stream), "mask refresh");
+        c->ibm_flipped = 1;
+    }
+    LAUNCH3(c, launch3_fill(a, esz, c->stream), "halo / boundary");
+    if (c->n_body > 0 && c->n_ghost > 0)
+        LAUNCH3(c, launch3_mirror(a, esz, c->n_ghost, c->ghost_idx, c->ghost_img, c->stream), "ibm mirror");
+    LAUNCH3(c, launch3_stage(a, esz, c->stream), "stage");
+    const bool last = !rk3 || c->rk_stage == 2;
+    if (last) {
+        if (!rk3) c->cur ^= 1;
+        if (c->sponge) {
+            Args3 b = args3(c);
+            planes3(b, c->P[c->cur], c->P[c->cur], c->P[c->cur]);
+            LAUNCH3(c, launch3_sponge(b, esz, c->sp, c->stream), "sponge");
+        }
+        c->parity ^= 1;
+        c->rk_stage = 0;
+    } else {
+        c->rk_stage++;
+    }
+    return EFVM_OK;
+}
+
+efvm_status alloc3(efvm_ctx3* c, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail3(c, EFVM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return EFVM_OK;
+}
+
+// viscous weights from cell-centre distances along one axis (crates/cfd_solver/src/lib.rs:483-501)
+std::vector<double> axis_weights(const efvm_ctx3* c, int d, int n) {
+    std::vector<double> w(3 * (size_t)n);
+    for (int q = 0; q < n; q++) {
+        const int r = q + NG3;
+        const double hm = centroid3(c, d, r) - centroid3(c, d, r - 1), hp = centroid3(c, d, r + 1) - centroid3(c, d, r);
+        w[q] = 1.0 / hm; w[n + q] = 1.0 / hp; w[2 * (size_t)n + q] = 2.0 / (hm + hp);
+    }
+    return w;
+}
+
+// Signed distance and outward normal of body b at point p (sphere.rs:43-59; lib.rs cylinder).
+double body_sd(const efvm_ctx3* c, int b, const double* p) {
+    const double* B = c->body[b];
+    const double dx = p[0] - B[0], dy = p[1] - B[1];
+    if (c->body_kind == 2) {
+        const double dz = p[2] - B[2];
+        return std::sqrt(dx * dx + dy * dy + dz * dz) - B[3];
+    }
+    return std::sqrt(dx * dx + dy * dy) - B[3];
+}
+void body_normal(const efvm_ctx3* c, int b, const double* p, double* n) {
+    const double* B = c->body[b];
+    const double dx = p[0] - B[0], dy = p[1] - B[1];
+    const double dz = c->body_kind == 2 ? p[2] - B[2] : 0.0;
+    const double r = c->body_kind == 2 ? std::sqrt(dx * dx + dy * dy + dz * dz) : std::sqrt(dx * dx + dy * dy);
+    if (r < 1e-12) { n[0] = 1.0; n[1] = 0.0; n[2] = 0.0; }
+    else { n[0] = dx / r; n[1] = dy / r; n[2] = c->body_kind == 2 ? dz / r : 0.0; }
+}
+
+// Mask, ghost list and markers on the host in FP64, as the reference's Setup systems
+// build them (refresh_*_mask, compute_*_ghosts, compute_*_markers); static bodies need
+// them once.  The cells inside flip to the stagnant state on the next substage.
+efvm_status build_ibm3(efvm_ctx3* c) {
+    drop_graph3(c);
+    free_ibm3(c);
+    const int nx = c->grid.nx, ny = c->grid.ny, nz = c->grid.nz;
+    std::vector<uint8_t> solid(c->nraw, 0);
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const double cc[3] = {centroid3(c, 0, i + NG3), centroid3(c, 1, j + NG3), centroid3(c, 2, k + NG3)};
+                for (int b = 0; b < c->n_body; b++)
+                    if (body_sd(c, b, cc) < 0.0) solid[raw_index(c, i + NG3, j + NG3, k + NG3)] = 1;
+            }
+    std::vector<long> gidx;
+    std::vector<double> gimg;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const int ir = i + NG3, jr = j + NG3, kr = k + NG3;
+                const long q = raw_index(c, ir, jr, kr);
+                if (!solid[q]) continue;
+                const long nb[6] = {raw_index(c, ir + 1, jr, kr), raw_index(c, ir - 1, jr, kr), raw_index(c, ir, jr + 1, kr),
+                                    raw_index(c, ir, jr - 1, kr), raw_index(c, ir, jr, kr + 1), raw_index(c, ir, jr, kr - 1)};
+                bool anyfluid = false;
+                for (int t = 0; t < 6; t++) anyfluid = anyfluid || !solid[nb[t]];
+                if (!anyfluid) continue;   // fully buried
+                const double cc[3] = {centroid3(c, 0, ir), centroid3(c, 1, jr), centroid3(c, 2, kr)};
+                double best_abs = 1.79769313486231570815e308, best_signed = 0.0, bn[3] = {1.0, 0.0, 0.0};
+                for (int b = 0; b < c->n_body; b++) {
+                    const double s = body_sd(c, b, cc);
+                    if (std::fabs(s) < best_abs) { best_abs = std::fabs(s); best_signed = s; body_normal(c, b, cc, bn); }
+                }
+                const double push = 2.0 * (-best_signed);
+                gidx.push_back(q);
+                for (int d = 0; d < 3; d++) gimg.push_back(cc[d] + push * bn[d]);
+            }
+    std::vector<double> mk;
+    const double PI = 3.14159265358979323846;
+    if (c->body_kind == 2) {
+        // spherical Fibonacci lattice, equal areas 4 pi R^2 / N, radial normals (sphere.rs:213-237)
+        const int n = std::max(c->sphere_markers, 12);
+        const double golden = PI * (3.0 - std::sqrt(5.0));
+        for (int b = 0; b < c->n_body; b++) {
+            const double* B = c->body[b];
+            const double area = 4.0 * PI * B[3] * B[3] / (double)n;
+            for (int m = 0; m < n; m++) {
+                const double zr = 1.0 - 2.0 * ((double)m + 0.5) / (double)n;
+                const double rr = std::sqrt(std::max(1.0 - zr * zr, 0.0));
+                const double phi = golden * (double)m;
+                const double dir[3] = {rr * std::cos(phi), rr * std::sin(phi), zr};
+                const double row[7] = {B[0] + B[3] * dir[0], B[1] + B[3] * dir[1], B[2] + B[3] * dir[2], dir[0], dir[1], dir[2], area};
+                mk.insert(mk.end(), row, row + 7);
+            }
+        }
+    } else {
+        // 64 x nz cylinder markers (cfd_ibm/src/lib.rs:566-598)
+        const double z_lo = centroid3(c, 2, NG3) - 0.5 * c->h[2];
+        const double z_hi = centroid3(c, 2, NG3 + nz - 1) + 0.5 * c->h[2];
+        const double lz = z_hi - z_lo;
+        const int n_theta = 64, n_z = nz > 1 ? nz : 1;
+        for (int b = 0; b < c->n_body; b++) {
+            const double* B = c->body[b];
+            const double area = 2.0 * PI * B[3] * lz / (double)(n_theta * n_z);
+            for (int it = 0; it < n_theta; it++) {
+                const double th = 2.0 * PI * ((double)it + 0.5) / (double)n_theta;
+                const double nxv = std::cos(th), nyv = std::sin(th);
+                for (int iz = 0; iz < n_z; iz++) {
+                    const double z = z_lo + ((double)iz + 0.5) * lz / (double)n_z;
+                    const double row[7] = {B[0] + B[3] * nxv, B[1] + B[3] * nyv, z, nxv, nyv, 0.0, area};
+                    mk.insert(mk.end(), row, row + 7);
+                }
+            }
+        }
+    }
+    const C5<double> f = {c->freestream[0], c->freestream[1], c->freestream[2], c->freestream[3], c->freestream[4]};
+    const double stag[5] = {f.r, 0.0, 0.0, 0.0, f.re - ref::kinetic(f)};
+    efvm_status st = EFVM_OK;
+    if ((st = alloc3(c, (void**)&c->solid, c->nraw)) != EFVM_OK) return st;
+    if ((st = alloc3(c, (void**)&c->ghost_idx, sizeof(long) * std::max<size_t>(1, gidx.size()))) != EFVM_OK) return st;
+    if ((st = alloc3(c, (void**)&c->ghost_img, sizeof(double) * std::max<size_t>(3, gimg.size()))) != EFVM_OK) return st;
+    if ((st = alloc3(c, (void**)&c->markers, sizeof(double) * std::max<size_t>(7, mk.size()))) != EFVM_OK) return st;
+    if ((st = alloc3(c, (void**)&c->stagnant, sizeof(double) * 5)) != EFVM_OK) return st;
+    HIPCHK3(c, hipMemcpy(c->solid, solid.data(), c->nraw, hipMemcpyHostToDevice));
+    if (!gidx.empty()) {
+        HIPCHK3(c, hipMemcpy(c->ghost_idx, gidx.data(), sizeof(long) * gidx.size(), hipMemcpyHostToDevice));
+        HIPCHK3(c, hipMemcpy(c->ghost_img, gimg.data(), sizeof(double) * gimg.size(), hipMemcpyHostToDevice));
+    }
+    if (!mk.empty()) HIPCHK3(c, hipMemcpy(c->markers, mk.data(), sizeof(double) * mk.size(), hipMemcpyHostToDevice));
+    HIPCHK3(c, hipMemcpy(c->stagnant, stag, sizeof(stag), hipMemcpyHostToDevice));
+    c->n_ghost = (int)gidx.size();
+    c->n_mark = (int)(mk.size() / 7);
+    c->ibm_flipped = 0;
+    return EFVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+efvm_status efvm3_create(const efvm_grid3* grid, const efvm_gas* gas, const efvm_scheme* scheme, efvm_ctx3** out) {
+    if (!grid || !gas || !scheme || !out) return EFVM_ERR_ARG;
+    *out = nullptr;
+    if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return EFVM_ERR_CONFIG;
+    for (int d = 0; d < 3; d++) if (!(grid->hi[d] > grid->lo[d])) return EFVM_ERR_CONFIG;
+    if ((long)(grid->nx + 4) * (grid->ny + 4) * (long)(grid->nz + 4) > (1L << 31)) return EFVM_ERR_UNSUPPORTED;
+    if (scheme->flux < 0 || scheme->flux > 2 || scheme->recon < 0 || scheme->recon > 1 || scheme->limiter < 0 ||
+        scheme->limiter > 1 || scheme->integrator < 0 || scheme->integrator > 1 || scheme->precision < 0 ||
+        scheme->precision > 1 || scheme->seam_mode < 0 || scheme->seam_mode > 1)
+        return EFVM_ERR_CONFIG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return EFVM_ERR_NO_DEVICE;
+    efvm_ctx3* c = new (std::nothrow) efvm_ctx3();
+    if (!c) return EFVM_ERR_ARG;
+    c->grid = *grid; c->gas = *gas; c->sch = *scheme;
+    for (int s = 0; s < 6; s++) { c->bc[s].kind = EFVM_BC_OUTFLOW; for (int q = 0; q < 5; q++) c->bc[s].par[q] = 0.0; }
+    const int n[3] = {grid->nx, grid->ny, grid->nz};
+    for (int d = 0; d < 3; d++) c->h[d] = (grid->hi[d] - grid->lo[d]) / (double)n[d];
+    c->NX = grid->nx + 2 * NG3; c->NY = grid->ny + 2 * NG3; c->NZ = grid->nz + 2 * NG3;
+    c->nraw = (long)c->NX * c->NY * c->NZ;
+    c->ncell = (long)grid->nx * grid->ny * grid->nz;
+    c->esz = scheme->precision == EFVM_FP64 ? 8 : 4;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        efvm3_destroy(c);
+        return EFVM_ERR_HIP;
+    }
+    efvm_status st = EFVM_OK;
+    for (int s = 0; s < 3 && st == EFVM_OK; s++)
+        for (int m = 0; m < 5 && st == EFVM_OK; m++) st = alloc3(c, &c->P[s][m], c->esz * c->nraw);
+    if (st == EFVM_OK) st = alloc3(c, (void**)&c->clock, sizeof(DevClock));
+    if (st == EFVM_OK) st = alloc3(c, (void**)&c->part, sizeof(double) * 5 * 1024);
+    if (st == EFVM_OK) st = alloc3(c, (void**)&c->out, sizeof(double) * 16);
+    if (st == EFVM_OK) st = alloc3(c, (void**)&c->yw, sizeof(double) * 3 * grid->ny);
+    if (st == EFVM_OK) st = alloc3(c, (void**)&c->zw, sizeof(double) * 3 * grid->nz);
+    if (st != EFVM_OK) { efvm3_destroy(c); return st; }
+    for (int s = 0; s < 3; s++)
+        for (int m = 0; m < 5; m++) (void)hipMemsetAsync(c->P[s][m], 0, c->esz * c->nraw, c->stream);
+    const std::vector<double> yw = axis_weights(c, 1, grid->ny), zw = axis_weights(c, 2, grid->nz);
+    if (hipMemcpyAsync(c->yw, yw.data(), sizeof(double) * yw.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->zw, zw.data(), sizeof(double) * zw.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        launch3_clock_set(c->clock, 0.0, 0.0, 0, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+        efvm3_destroy(c);
+        return EFVM_ERR_HIP;
+    }
+    *out = c;
+    return EFVM_OK;
+}
+
+void efvm3_destroy(efvm_ctx3* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph3(c);
+    for (int s = 0; s < 3; s++) for (int m = 0; m < 5; m++) if (c->P[s][m]) (void)hipFree(c->P[s][m]);
+    if (c->clock) (void)hipFree(c->clock);
+    if (c->part) (void)hipFree(c->part);
+    if (c->out) (void)hipFree(c->out);
+    if (c->yw) (void)hipFree(c->yw);
+    if (c->zw) (void)hipFree(c->zw);
+    free_ibm3(c);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* efvm3_last_error(const efvm_ctx3* c) { return c ? c->err.c_str() : "null context"; }
+
+efvm_status efvm3_set_boundaries(efvm_ctx3* c, const efvm_bc sides[6]) {
+    if (!c || !sides) return EFVM_ERR_ARG;
+    for (int s = 0; s < 6; s++) {
+        if (sides[s].kind < 0 || sides[s].kind > EFVM_BC_TABLE) return fail3(c, EFVM_ERR_CONFIG, "unknown boundary kind");
+        if (sides[s].kind == EFVM_BC_TABLE) return fail3(c, EFVM_ERR_UNSUPPORTED, "ghost tables are 2-D only");
+    }
+    for (int ax = 0; ax < 3; ax++)
+        if ((sides[2 * ax].kind == EFVM_BC_PERIODIC) != (sides[2 * ax + 1].kind == EFVM_BC_PERIODIC))
+            return fail3(c, EFVM_ERR_CONFIG, std::string("periodic boundary on axis ") + std::to_string(ax) +
+                                                 " requires both sides periodic (crates/cfd_boundary/src/lib.rs:437-448)");
+    for (int s = 0; s < 6; s++) c->bc[s] = sides[s];
+    drop_graph3(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm3_set_time_step(efvm_ctx3* c, double cfl, double fixed_dt) {
+    if (!c || !(cfl > 0.0)) return EFVM_ERR_ARG;
+    if (c->rk_stage != 0) return fail3(c, EFVM_ERR_ARG, "time-step control changes at a step boundary only");
+    c->sch.cfl = cfl;
+    c->sch.fixed_dt = fixed_dt;
+    drop_graph3(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm3_set_body_force(efvm_ctx3* c, double gx) {
+    if (!c) return EFVM_ERR_ARG;
+    c->body_force = 1; c->gx = gx;
+    drop_graph3(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm3_set_sponge(efvm_ctx3* c, double x0, double x1, double sigma, const double target[5]) {
+    if (!c || !target || !(x1 > x0)) return EFVM_ERR_ARG;
+    if (c->sch.integrator != EFVM_INTEGRATOR_RK3) return fail3(c, EFVM_ERR_UNSUPPORTED, "sponge needs RK3");
+    c->sponge = 1;
+    std::memset(&c->sp, 0, sizeof(c->sp));
+    c->sp.x0 = x0; c->sp.x1 = x1; c->sp.sigma = sigma; c->sp.uniform = 1;
+    for (int m = 0; m < 5; m++) c->sp.target[m] = target[m];
+    drop_graph3(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm3_upload_state(efvm_ctx3* c, const double* cons5, size_t n) {
+    if (!c || !cons5) return EFVM_ERR_ARG;
+    const bool raw = (long)n == c->nraw;
+    if (!raw && (long)n != c->ncell) return fail3(c, EFVM_ERR_ARG, "upload: n_cells must be nx*ny*nz or the raw padded count");
+    const int nx = c->grid.nx, ny = c->grid.ny, nz = c->grid.nz;
+    std::vector<char> h[5];
+    for (int m = 0; m < 5; m++) h[m].resize(c->esz * c->nraw);
+    for (int k = 0; k < c->NZ; k++)
+        for (int j = 0; j < c->NY; j++)
+            for (int i = 0; i < c->NX; i++) {
+                const long q = raw_index(c, i, j, k);
+                long src;
+                if (raw) {
+                    src = q;
+                } else {   // ghost cells take their nearest interior cell
+                    const int ii = std::min(std::max(i - NG3, 0), nx - 1), jj = std::min(std::max(j - NG3, 0), ny - 1),
+                              kk = std::min(std::max(k - NG3, 0), nz - 1);
+                    src = ii + (long)nx * (jj + (long)ny * kk);
+                }
+                for (int m = 0; m < 5; m++) {
+                    const double v = cons5[5 * src + m];
+                    if (c->esz == 8) reinterpret_cast<double*>(h[m].data())[q] = v;
+                    else reinterpret_cast<float*>(h[m].data())[q] = (float)v;
+                }
+            }
+    c->rk_stage = 0;
+    for (int m = 0; m < 5; m++)
+        HIPCHK3(c, hipMemcpyAsync(c->P[c->cur][m], h[m].data(), c->esz * c->nraw, hipMemcpyHostToDevice, c->stream));
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm3_download_state(efvm_ctx3* c, double* cons5, size_t n) {
+    if (!c || !cons5) return EFVM_ERR_ARG;
+    const bool raw = (long)n == c->nraw;
+    if (!raw && (long)n != c->ncell) return fail3(c, EFVM_ERR_ARG, "download: n_cells must be nx*ny*nz or the raw padded count");
+    const int set = current_set(c);
+    std::vector<char> h[5];
+    for (int m = 0; m < 5; m++) {
+        h[m].resize(c->esz * c->nraw);
+        HIPCHK3(c, hipMemcpyAsync(h[m].data(), c->P[set][m], c->esz * c->nraw, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    const int nx = c->grid.nx, ny = c->grid.ny, nz = c->grid.nz;
+    auto val = [&](int m, long q) {
+        return c->esz == 8 ? reinterpret_cast<double*>(h[m].data())[q] : (double)reinterpret_cast<float*>(h[m].data())[q];
+    };
+    if (raw) {
+        for (long q = 0; q < c->nraw; q++) for (int m = 0; m < 5; m++) cons5[5 * q + m] = val(m, q);
+    } else {
+        long o = 0;
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++, o++) {
+                    const long q = raw_index(c, i + NG3, j + NG3, k + NG3);
+                    for (int m = 0; m < 5; m++) cons5[5 * o + m] = val(m, q);
+                }
+    }
+    return EFVM_OK;
+}
+
+efvm_status efvm3_get_clock(efvm_ctx3* c, double* dt, double* time, long* step, int* stage) {
+    if (!c) return EFVM_ERR_ARG;
+    DevClock h;
+    HIPCHK3(c, hipMemcpyAsync(&h, c->clock, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    if (dt) *dt = h.dt;
+    if (time) *time = h.time;
+    if (step) *step = (long)h.step;
+    if (stage) *stage = c->rk_stage;
+    if (h.bad) return fail3(c, EFVM_ERR_NONPHYSICAL, "CFL scan saw " + std::to_string(h.bad) + " non-finite or non-positive-density cells");
+    return EFVM_OK;
+}
+
+efvm_status efvm3_set_clock(efvm_ctx3* c, double time, long step) {
+    if (!c) return EFVM_ERR_ARG;
+    HIPCHK3(c, launch3_clock_set(c->clock, 0.0, time, step, c->stream));
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    c->rk_stage = 0; c->parity = 0;
+    return EFVM_OK;
+}
+
+efvm_status efvm3_cfl(efvm_ctx3* c, double* dt_out) {
+    if (!c || !dt_out) return EFVM_ERR_ARG;
+    if (c->sch.fixed_dt > 0.0) { *dt_out = c->sch.fixed_dt; return EFVM_OK; }
+    Args3 a = args3(c);
+    const int set = current_set(c);
+    planes3(a, c->P[set], c->P[set], c->P[set]);
+    LAUNCH3(c, launch3_cfl(a, (int)c->esz, c->part, &c->clock->lam_tmp, c->stream), "cfl");
+    double lam = 0.0;
+    HIPCHK3(c, hipMemcpyAsync(&lam, &c->clock->lam_tmp, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    *dt_out = lam > 0.0 ? c->sch.cfl / lam : INFINITY;
+    return EFVM_OK;
+}
+
+efvm_status efvm3_run(efvm_ctx3* c) {
+    if (!c) return EFVM_ERR_ARG;
+    return run3(c);
+}
+
+efvm_status efvm3_step(efvm_ctx3* c, int n_steps) {
+    if (!c || n_steps < 0) return EFVM_ERR_ARG;
+    HIPCHK3(c, hipEventRecord(c->ev0, c->stream));
+    const int nsub = c->sch.integrator == EFVM_INTEGRATOR_RK3 ? 3 : 1;
+    for (int k = 0; k < n_steps; k++)
+        for (int q = 0; q < nsub; q++) {
+            efvm_status st = run3(c);
+            if (st != EFVM_OK) return st;
+        }
+    HIPCHK3(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = 1;
+    return EFVM_OK;
+}
+
+efvm_status efvm3_synchronize(efvm_ctx3* c) {
+    if (!c) return EFVM_ERR_ARG;
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm3_graph_capture(efvm_ctx3* c, int n_steps) {
+    if (!c || n_steps <= 0 || (n_steps & 1)) return fail3(c, EFVM_ERR_ARG, "graph capture needs an even positive step count");
+    if (c->rk_stage != 0) return fail3(c, EFVM_ERR_ARG, "graph capture must start at a step boundary");
+    // the first step after a body is added scans the CFL before the mask refresh flips the cells
+    // inside it; a replayed graph cannot repeat that order, so that step runs eagerly
+    if (c->n_body > 0 && !c->ibm_flipped)
+        return fail3(c, EFVM_ERR_ARG, "graph capture after a body is added needs one eager step first");
+    drop_graph3(c);
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    const int cur0 = c->cur, par0 = c->parity;
+    HIPCHK3(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    efvm_status st = EFVM_OK;
+    const int nsub = c->sch.integrator == EFVM_INTEGRATOR_RK3 ? 3 : 1;
+    for (int k = 0; k < n_steps && st == EFVM_OK; k++)
+        for (int q = 0; q < nsub && st == EFVM_OK; q++) st = run3(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (st != EFVM_OK) { if (g) (void)hipGraphDestroy(g); return st; }
+    if (e != hipSuccess) return fail3(c, EFVM_ERR_HIP, std::string("end capture: ") + hipGetErrorString(e));
+    c->graph = g;
+    HIPCHK3(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+    c->cur = cur0; c->parity = par0; c->rk_stage = 0;
+    return EFVM_OK;
+}
+
+efvm_status efvm3_graph_launch(efvm_ctx3* c) {
+    if (!c || !c->graph_exec) return fail3(c, EFVM_ERR_ARG, "no captured graph");
+    HIPCHK3(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK3(c, hipGraphLaunch(c->graph_exec, c->stream));
+    HIPCHK3(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = 1;
+    return EFVM_OK;
+}
+
+efvm_status efvm3_last_elapsed_ms(efvm_ctx3* c, double* ms) {
+    if (!c || !ms || !c->timed) return EFVM_ERR_ARG;
+    HIPCHK3(c, hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    HIPCHK3(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f;
+    return EFVM_OK;
+}
+
+efvm_status efvm3_ibm_add_sphere(efvm_ctx3* c, double cx, double cy, double cz, double r, const double fs[5], int n_markers) {
+    if (!c || !fs || !(r > 0.0) || n_markers < 1) return EFVM_ERR_ARG;
+    if (c->body_kind == 1) return fail3(c, EFVM_ERR_UNSUPPORTED, "a context holds spheres or cylinders, not both");
+    if (c->n_body >= 8) return fail3(c, EFVM_ERR_UNSUPPORTED, "at most 8 bodies");
+    if (c->rk_stage != 0) return fail3(c, EFVM_ERR_ARG, "bodies are added at a step boundary");
+    c->body_kind = 2;
+    c->body[c->n_body][0] = cx; c->body[c->n_body][1] = cy; c->body[c->n_body][2] = cz; c->body[c->n_body][3] = r;
+    c->n_body++;
+    c->sphere_markers = n_markers;
+    std::memcpy(c->freestream, fs, sizeof(double) * 5);
+    return build_ibm3(c);
+}
+
+efvm_status efvm3_ibm_add_cylinder(efvm_ctx3* c, double cx, double cy, double r, const double fs[5]) {
+    if (!c || !fs || !(r > 0.0)) return EFVM_ERR_ARG;
+    if (c->body_kind == 2) return fail3(c, EFVM_ERR_UNSUPPORTED, "a context holds spheres or cylinders, not both");
+    if (c->n_body >= 8) return fail3(c, EFVM_ERR_UNSUPPORTED, "at most 8 bodies");
+    if (c->rk_stage != 0) return fail3(c, EFVM_ERR_ARG, "bodies are added at a step boundary");
+    c->body_kind = 1;
+    c->body[c->n_body][0] = cx; c->body[c->n_body][1] = cy; c->body[c->n_body][2] = 0.0; c->body[c->n_body][3] = r;
+    c->n_body++;
+    std::memcpy(c->freestream, fs, sizeof(double) * 5);
+    return build_ibm3(c);
+}
+
+efvm_status efvm3_marker_force(efvm_ctx3* c, double out[6]) {
+    if (!c || !out) return EFVM_ERR_ARG;
+    if (c->n_body == 0 || c->n_mark == 0) { for (int d = 0; d < 6; d++) out[d] = 0.0; return EFVM_OK; }
+    Args3 a = args3(c);
+    const int set = current_set(c);
+    planes3(a, c->P[set], c->P[set], c->P[set]);
+    LAUNCH3(c, launch3_markers(a, (int)c->esz, c->n_mark, c->markers, c->h[0], c->out, c->stream), "marker force");
+    HIPCHK3(c, hipMemcpyAsync(out, c->out, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm3_conservation_sums(efvm_ctx3* c, double out[5]) {
+    if (!c || !out) return EFVM_ERR_ARG;
+    Args3 a = args3(c);
+    const int set = current_set(c);
+    planes3(a, c->P[set], c->P[set], c->P[set]);
+    const double vol = c->h[0] * c->h[1] * c->h[2];
+    const int nb = (int)std::min<long>(1024, (c->ncell + 255) / 256);
+    LAUNCH3(c, launch3_sums(a, (int)c->esz, c->part, nb, vol, c->out + 8, c->stream), "sums");
+    HIPCHK3(c, hipMemcpyAsync(out, c->out + 8, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
+}
+
+}  // extern "C"

@@ -588,3 +588,186 @@ class Mlp:
         self._chk(self._L.efvm_mlp_infer(self._p, _pf(X), len(X), _pf(Y)))
         return Y
 
+
+
+# ---------------------------------------------------------------------------
+# 3-D structured meshes (efvm3_*)
+# ---------------------------------------------------------------------------
+class Grid3(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("lo", C.c_double * 3), ("hi", C.c_double * 3)]
+
+
+EXPORTS += [
+    "efvm3_create", "efvm3_destroy", "efvm3_last_error", "efvm3_set_boundaries", "efvm3_set_time_step",
+    "efvm3_set_body_force", "efvm3_set_sponge", "efvm3_upload_state", "efvm3_download_state", "efvm3_get_clock",
+    "efvm3_set_clock", "efvm3_cfl", "efvm3_run", "efvm3_step", "efvm3_synchronize", "efvm3_graph_capture",
+    "efvm3_graph_launch", "efvm3_ibm_add_sphere", "efvm3_ibm_add_cylinder", "efvm3_marker_force",
+    "efvm3_conservation_sums", "efvm3_last_elapsed_ms",
+]
+
+
+def _bind3(L):
+    if getattr(L, "_bound3", False):
+        return
+    vp, d, pd = C.c_void_p, C.c_double, C.POINTER(C.c_double)
+    sig = {
+        "efvm3_create": [C.POINTER(Grid3), C.POINTER(Gas), C.POINTER(Scheme), C.POINTER(vp)],
+        "efvm3_set_boundaries": [vp, C.POINTER(Bc)],
+        "efvm3_set_time_step": [vp, d, d],
+        "efvm3_set_body_force": [vp, d],
+        "efvm3_set_sponge": [vp, d, d, d, pd],
+        "efvm3_upload_state": [vp, pd, C.c_size_t],
+        "efvm3_download_state": [vp, pd, C.c_size_t],
+        "efvm3_get_clock": [vp, pd, pd, C.POINTER(C.c_long), C.POINTER(C.c_int)],
+        "efvm3_set_clock": [vp, d, C.c_long],
+        "efvm3_cfl": [vp, pd],
+        "efvm3_run": [vp],
+        "efvm3_step": [vp, C.c_int],
+        "efvm3_synchronize": [vp],
+        "efvm3_graph_capture": [vp, C.c_int],
+        "efvm3_graph_launch": [vp],
+        "efvm3_ibm_add_sphere": [vp, d, d, d, d, pd, C.c_int],
+        "efvm3_ibm_add_cylinder": [vp, d, d, d, pd],
+        "efvm3_marker_force": [vp, pd],
+        "efvm3_conservation_sums": [vp, pd],
+        "efvm3_last_elapsed_ms": [vp, pd],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.efvm3_destroy.argtypes = [vp]
+    L.efvm3_destroy.restype = None
+    L.efvm3_last_error.argtypes = [vp]
+    L.efvm3_last_error.restype = C.c_char_p
+    L._bound3 = True
+
+
+class Context3:
+    """A 3-D UniformMesh context (nx, ny, nz cells, five stored components), 1:1 over efvm3_*."""
+
+    def __init__(self, nx, ny, nz, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), gamma=1.4, r_gas=287.058, mu=None,
+                 flux="hllc", muscl=False, limiter="minmod", integrator="rk3", precision="fp64",
+                 viscous=False, seam_mode=0, cfl=0.4, fixed_dt=None):
+        L = load_library()
+        _bind3(L)
+        self._L = L
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        g = Grid3(self.nx, self.ny, self.nz, (C.c_double * 3)(*lo), (C.c_double * 3)(*hi))
+        gs = Gas(gamma, r_gas, 0 if mu is None else 1, 0.0 if mu is None else float(mu))
+        sc = Scheme(FLUX[flux], 1 if muscl else 0, LIMITER[limiter], 1 if integrator == "rk3" else 0,
+                    0 if precision == "fp64" else 1, int(viscous), int(seam_mode), cfl,
+                    -1.0 if fixed_dt is None else float(fixed_dt))
+        self.precision = precision
+        self.grid, self.gas, self.scheme = g, gs, sc
+        self.lo, self.hi = tuple(lo), tuple(hi)
+        p = C.c_void_p()
+        st = L.efvm3_create(C.byref(g), C.byref(gs), C.byref(sc), C.byref(p))
+        if st != OK:
+            raise EfvmError(st, "efvm3_create failed")
+        self._p = p
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._L.efvm3_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st: int):
+        if st != OK:
+            raise EfvmError(st, self._L.efvm3_last_error(self._p).decode())
+
+    @property
+    def ncell(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    @property
+    def nraw(self) -> int:
+        return (self.nx + 4) * (self.ny + 4) * (self.nz + 4)
+
+    def set_boundaries(self, sides):
+        """sides: 6 x (kind, params) for XLo, XHi, YLo, YHi, ZLo, ZHi."""
+        arr = (Bc * 6)()
+        for s, spec in enumerate(sides):
+            kind, par = (spec, ()) if isinstance(spec, str) else spec
+            arr[s].kind = BC[kind]
+            for q, v in enumerate(par):
+                arr[s].par[q] = v
+        self._chk(self._L.efvm3_set_boundaries(self._p, arr))
+
+    def set_time_step(self, cfl=0.4, fixed_dt=None):
+        self._chk(self._L.efvm3_set_time_step(self._p, cfl, -1.0 if fixed_dt is None else float(fixed_dt)))
+
+    def set_body_force(self, gx):
+        self._chk(self._L.efvm3_set_body_force(self._p, gx))
+
+    def set_sponge(self, x0, x1, sigma, target):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        self._chk(self._L.efvm3_set_sponge(self._p, x0, x1, sigma, _pd(t)))
+
+    def upload(self, cons: np.ndarray):
+        """Interior (nz, ny, nx, 5) or the raw padded array (nz+4, ny+4, nx+4, 5)."""
+        a = np.ascontiguousarray(cons, dtype=np.float64).reshape(-1, 5)
+        if a.shape[0] not in (self.ncell, self.nraw):
+            raise ValueError("state has the wrong number of cells")
+        self._chk(self._L.efvm3_upload_state(self._p, _pd(a), a.shape[0]))
+
+    def download(self, raw=False) -> np.ndarray:
+        shape = (self.nz + 4, self.ny + 4, self.nx + 4, 5) if raw else (self.nz, self.ny, self.nx, 5)
+        out = np.empty(shape)
+        self._chk(self._L.efvm3_download_state(self._p, _pd(out), out.size // 5))
+        return out
+
+    def clock(self):
+        dt, t = C.c_double(), C.c_double()
+        st, sg = C.c_long(), C.c_int()
+        self._chk(self._L.efvm3_get_clock(self._p, C.byref(dt), C.byref(t), C.byref(st), C.byref(sg)))
+        return dict(dt=dt.value, time=t.value, step=st.value, stage=sg.value)
+
+    def set_clock(self, time=0.0, step=0):
+        self._chk(self._L.efvm3_set_clock(self._p, time, step))
+
+    def cfl_dt(self) -> float:
+        d = C.c_double()
+        self._chk(self._L.efvm3_cfl(self._p, C.byref(d)))
+        return d.value
+
+    def run(self):
+        self._chk(self._L.efvm3_run(self._p))
+
+    def step(self, n=1):
+        self._chk(self._L.efvm3_step(self._p, n))
+
+    def synchronize(self):
+        self._chk(self._L.efvm3_synchronize(self._p))
+
+    def graph_capture(self, n_steps):
+        self._chk(self._L.efvm3_graph_capture(self._p, n_steps))
+
+    def graph_launch(self):
+        self._chk(self._L.efvm3_graph_launch(self._p))
+
+    def last_elapsed_ms(self) -> float:
+        d = C.c_double()
+        self._chk(self._L.efvm3_last_elapsed_ms(self._p, C.byref(d)))
+        return d.value
+
+    def add_sphere(self, center, r, freestream, markers=2000):
+        fs = np.ascontiguousarray(freestream, dtype=np.float64)
+        self._chk(self._L.efvm3_ibm_add_sphere(self._p, center[0], center[1], center[2], r, _pd(fs), int(markers)))
+
+    def add_cylinder(self, cx, cy, r, freestream):
+        fs = np.ascontiguousarray(freestream, dtype=np.float64)
+        self._chk(self._L.efvm3_ibm_add_cylinder(self._p, cx, cy, r, _pd(fs)))
+
+    def marker_force(self) -> np.ndarray:
+        out = np.zeros(6)
+        self._chk(self._L.efvm3_marker_force(self._p, _pd(out)))
+        return out
+
+    def sums(self) -> np.ndarray:
+        out = np.zeros(5)
+        self._chk(self._L.efvm3_conservation_sums(self._p, _pd(out)))
+        return out

@@ -10,9 +10,10 @@
  *
  * State wire order is the reference's `Component` serialisation of `ConsVar`:
  * 5 doubles per cell (rho, rho_u, rho_v, rho_w, rho_E)
- * (crates/cfd_state/src/lib.rs:54-76), interior cells only, x fastest.  The GPU
- * path is 2-D (the reference's nz = 1 UniformMesh with z slip walls): rho_w must
- * be 0 on upload and is 0 on download.
+ * (crates/cfd_state/src/lib.rs:54-76), interior cells only, x fastest.  The
+ * efvm_* context is the 2-D path (the reference's nz = 1 UniformMesh with z slip
+ * walls): rho_w must be 0 on upload and is 0 on download.  The efvm3_* context
+ * at the end of this header takes any nz and all five components.
  *
  * Errors: every call returns an efvm_status; no C++ exception or abort crosses
  * the ABI (the reference panics, e.g. crates/cfd_solver/src/lib.rs:99,
@@ -282,6 +283,69 @@ efvm_status efvm_mlp_infer_stats(efvm_ctx* ctx, int reset, unsigned long long ou
  * (axis*2 + side)*4 + var, side 0 = left / lower, var = rho, u, v, p, for the
  * east (axis 0) and north (axis 1) face of each cell. */
 efvm_status efvm_learned_faces(efvm_ctx* ctx, float* out, size_t n);
+
+/* ---------------------------------------------------------------------------
+ * 3-D structured meshes: UniformMesh {nx, ny, nz, ng = 2, bounds_lo[3], bounds_hi[3]}
+ * with any nz (field_core UniformMeshConfig; the solver systems of
+ * crates/cfd_solver/src/lib.rs:93-122,170-339,449-677 over all three axes).  The same
+ * plugins as efvm_ctx, six boundary sides, five stored components (rho_w included),
+ * and the sphere body of crates/cfd_ibm/src/sphere.rs.  FP64 reproduces the reference's
+ * operation order; FP32 runs the same operations in single precision.  Reconstruction
+ * is first order or MUSCL; the learned reconstruction and ghost tables are 2-D only.
+ * ------------------------------------------------------------------------- */
+typedef struct { int nx, ny, nz; double lo[3], hi[3]; } efvm_grid3;
+typedef struct efvm_ctx3 efvm_ctx3;
+
+/* FieldDefaultPlugins{UniformMeshConfig} + CfdStatePlugin + IdealGasPlugin + SolverPlugin +
+ * FluxPlugin / MusclPlugin + IntegratorPlugin for a 3-D mesh; every side starts as
+ * SupersonicOutflow, the registry default (crates/cfd_boundary/src/lib.rs:366-381). */
+efvm_status efvm3_create(const efvm_grid3* grid, const efvm_gas* gas, const efvm_scheme* scheme, efvm_ctx3** out);
+void efvm3_destroy(efvm_ctx3* ctx);
+const char* efvm3_last_error(const efvm_ctx3* ctx);
+/* BoundaryRegistry (crates/cfd_boundary/src/lib.rs:369-449): sides[6] = XLo, XHi, YLo, YHi,
+ * ZLo, ZHi; a half-periodic axis is EFVM_ERR_CONFIG, EFVM_BC_TABLE is EFVM_ERR_UNSUPPORTED. */
+efvm_status efvm3_set_boundaries(efvm_ctx3* ctx, const efvm_bc sides[6]);
+/* SolverConfig {cfl, fixed_dt} (crates/cfd_solver/src/lib.rs:39-58), at a step boundary. */
+efvm_status efvm3_set_time_step(efvm_ctx3* ctx, double cfl, double fixed_dt);
+/* Uniform x body force (crates/cfd_solver/tests/poiseuille_channel_validation.rs:105-120). */
+efvm_status efvm3_set_body_force(efvm_ctx3* ctx, double gx);
+/* outlet_sponge_system of examples/sphere_drag_validation/main.rs:119-160: after each
+ * completed step, u += min(sigma s^2 dt, 1) (target - u) for cell centres x >= x_start,
+ * s = (x - x_start) / (x_end - x_start) clamped to [0, 1]; target = conserved state. */
+efvm_status efvm3_set_sponge(efvm_ctx3* ctx, double x_start, double x_end, double sigma, const double target[5]);
+/* CfdState.u in Component wire order (5 doubles per cell, x fastest).  n_cells =
+ * nx*ny*nz (interior; ghost cells are set to their nearest interior cell) or
+ * (nx+4)(ny+4)(nz+4) (the reference's raw padded array, ghosts as given).
+ * Upload resets the substage counter; both are synchronous. */
+efvm_status efvm3_upload_state(efvm_ctx3* ctx, const double* cons5, size_t n_cells);
+efvm_status efvm3_download_state(efvm_ctx3* ctx, double* cons5, size_t n_cells);
+/* SolverState {dt, time, step, rk_stage} (crates/cfd_solver/src/lib.rs:61-73). */
+efvm_status efvm3_get_clock(efvm_ctx3* ctx, double* dt, double* time, long* step, int* rk_stage);
+efvm_status efvm3_set_clock(efvm_ctx3* ctx, double time, long step);
+/* cfl_system (crates/cfd_solver/src/lib.rs:626-677) of the current state, FP64. */
+efvm_status efvm3_cfl(efvm_ctx3* ctx, double* dt_out);
+/* One app.run() (one substage through every phase) / n whole steps, enqueued only. */
+efvm_status efvm3_run(efvm_ctx3* ctx);
+efvm_status efvm3_step(efvm_ctx3* ctx, int n_steps);
+efvm_status efvm3_synchronize(efvm_ctx3* ctx);
+/* hipGraph of n_steps whole steps (even: the CFL slots alternate by step parity). */
+efvm_status efvm3_graph_capture(efvm_ctx3* ctx, int n_steps);
+efvm_status efvm3_graph_launch(efvm_ctx3* ctx);
+/* SphereIbmPlugin::new(vec![SphereBody::fixed(center, r)], freestream).with_markers(n)
+ * (crates/cfd_ibm/src/sphere.rs:82-253) with GhostCellIbmPlugin: solid mask, ghost list
+ * with image points, and n_markers equal-area spherical-Fibonacci surface markers.
+ * Static bodies, up to 8; all bodies of one context are spheres or all cylinders. */
+efvm_status efvm3_ibm_add_sphere(efvm_ctx3* ctx, double cx, double cy, double cz, double r,
+                                 const double freestream[5], int n_markers);
+/* CylinderIbmPlugin (crates/cfd_ibm/src/lib.rs:155-254,566-598): z-aligned cylinder through
+ * every layer, 64 x nz markers. */
+efvm_status efvm3_ibm_add_cylinder(efvm_ctx3* ctx, double cx, double cy, double r, const double freestream[5]);
+/* MarkerForce (pressure xyz, viscous xyz), FP64 (crates/cfd_ibm/src/lib.rs:442-480,601-615). */
+efvm_status efvm3_marker_force(efvm_ctx3* ctx, double out[6]);
+/* FP64 volume-weighted interior sums of (rho, rho_u, rho_v, rho_w, rho_E). */
+efvm_status efvm3_conservation_sums(efvm_ctx3* ctx, double out[5]);
+/* Busy time of the last efvm3_step / efvm3_graph_launch (HIP events), milliseconds. */
+efvm_status efvm3_last_elapsed_ms(efvm_ctx3* ctx, double* ms);
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,11 @@ typedef struct {
     int body_force; double body_gx;   /* crates/cfd_solver/tests/poiseuille_channel_validation.rs:105-120 */
     int n_cyl; double cyl[4][3];      /* cx, cy, r */
     double freestream[5];
+    /* SphereIbmPlugin (crates/cfd_ibm/src/sphere.rs): fixed spheres, n Fibonacci markers */
+    int n_sph; double sph[4][4];      /* cx, cy, cz, r */
+    int sph_markers;
+    /* uniform-target sponge of examples/sphere_drag_validation/main.rs:119-160 */
+    int sp_uniform; double sp_target[5];
 } orc_cfg;
 
 typedef struct {
@@ -547,6 +552,33 @@ static void cyl_normal(const double *cy, double x, double y, double *n) {
     double r = sqrt(dx * dx + dy * dy);
     if (r < 1e-12) { n[0] = 1.0; n[1] = 0.0; n[2] = 0.0; } else { n[0] = dx / r; n[1] = dy / r; n[2] = 0.0; }
 }
+static double sph_sd(const double *b, const double *p) {              /* sphere.rs:43-48 */
+    double d0 = p[0] - b[0], d1 = p[1] - b[1], d2 = p[2] - b[2];
+    return sqrt(d0 * d0 + d1 * d1 + d2 * d2) - b[3];
+}
+static void sph_normal(const double *b, const double *p, double *n) {  /* sphere.rs:50-59 */
+    double d0 = p[0] - b[0], d1 = p[1] - b[1], d2 = p[2] - b[2];
+    double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    if (r < 1e-12) { n[0] = 1.0; n[1] = 0.0; n[2] = 0.0; } else { n[0] = d0 / r; n[1] = d1 / r; n[2] = d2 / r; }
+}
+/* sphere_markers (sphere.rs:213-237): golden-angle spherical Fibonacci lattice, out[7 m..] =
+   position, radial normal, area 4 pi R^2 / n. */
+void orc_sphere_markers(const double *b, int n, double *out) {
+    const double PI = 3.14159265358979323846;
+    double golden = PI * (3.0 - sqrt(5.0));
+    double area = 4.0 * PI * b[3] * b[3] / (double)n;
+    for (int m = 0; m < n; m++) {
+        double zr = 1.0 - 2.0 * ((double)m + 0.5) / (double)n;
+        double rr = sqrt(fmax(1.0 - zr * zr, 0.0));
+        double phi = golden * (double)m;
+        double dir[3] = {rr * cos(phi), rr * sin(phi), zr};
+        double *o = out + 7 * m;
+        o[0] = b[0] + b[3] * dir[0]; o[1] = b[1] + b[3] * dir[1]; o[2] = b[2] + b[3] * dir[2];
+        o[3] = dir[0]; o[4] = dir[1]; o[5] = dir[2]; o[6] = area;
+    }
+}
+double orc_sphere_sd(const double *b, const double *p) { return sph_sd(b, p); }
+void orc_sphere_normal(const double *b, const double *p, double *n) { sph_normal(b, p, n); }
 static int locate(const orc_sim *s, const double *p, int lo[3], double t[3]) {
     int n[3] = {s->c.nx, s->c.ny, s->c.nz};
     for (int d = 0; d < 3; d++) {
@@ -585,8 +617,10 @@ static int sample_trilinear(const orc_sim *s, const double *pt, pv_t *out) {  /*
     *out = o;
     return 1;
 }
+static int n_bodies(const orc_sim *s) { return s->c.n_cyl + s->c.n_sph; }
 static void ibm_setup(orc_sim *s) {
-    if (s->c.n_cyl <= 0) return;
+    if (n_bodies(s) <= 0) return;
+    int sph = s->c.n_sph > 0;
     double g = s->c.gamma;
     cv_t fs = {s->c.freestream[0], s->c.freestream[1], s->c.freestream[2], s->c.freestream[3], s->c.freestream[4]};
     cv_t stagnant = {fs.r, 0.0, 0.0, 0.0, fs.re - kinetic(&fs)};   /* cfd_ibm:663-667 */
@@ -595,8 +629,10 @@ static void ibm_setup(orc_sim *s) {
     for (int k = 0; k < s->c.nz; k++) for (int j = 0; j < s->c.ny; j++) for (int i = 0; i < s->c.nx; i++) {
         long c = IDX(s, i + NG, j + NG, k + NG);
         double x = centroid(s, 0, i + NG), y = centroid(s, 1, j + NG);
+        double p3[3] = {x, y, centroid(s, 2, k + NG)};
         int inside = 0;
-        for (int b = 0; b < s->c.n_cyl; b++) if (cyl_sd(s->c.cyl[b], x, y) < 0.0) inside = 1;
+        if (sph) { for (int b = 0; b < s->c.n_sph; b++) if (sph_sd(s->c.sph[b], p3) < 0.0) inside = 1; }   /* sphere.rs:82-111 */
+        else for (int b = 0; b < s->c.n_cyl; b++) if (cyl_sd(s->c.cyl[b], x, y) < 0.0) inside = 1;
         if (inside && !s->solid[c]) { s->solid[c] = 1; s->u[c] = stagnant; memset(&s->rhs[c], 0, sizeof(cv_t)); }
         else if (!inside && s->solid[c]) { s->solid[c] = 0; s->u[c] = fs; memset(&s->rhs[c], 0, sizeof(cv_t)); }
     }
@@ -616,6 +652,12 @@ static void ibm_setup(orc_sim *s) {
         if (!anyfluid) continue;
         double cc[3] = {centroid(s, 0, ir), centroid(s, 1, jr), centroid(s, 2, kr)};
         double best_abs = 1.79769313486231570815e308, best_signed = 0.0, bn[3] = {1.0, 0.0, 0.0};
+        if (sph) {                                                     /* compute_sphere_ghosts, sphere.rs:116-187 */
+            for (int b = 0; b < s->c.n_sph; b++) {
+                double sd = sph_sd(s->c.sph[b], cc);
+                if (fabs(sd) < best_abs) { best_abs = fabs(sd); best_signed = sd; sph_normal(s->c.sph[b], cc, bn); }
+            }
+        } else
         for (int b = 0; b < s->c.n_cyl; b++) {
             double sd = cyl_sd(s->c.cyl[b], cc[0], cc[1]);
             if (fabs(sd) < best_abs) { best_abs = fabs(sd); best_signed = sd; cyl_normal(s->c.cyl[b], cc[0], cc[1], bn); }
@@ -626,6 +668,15 @@ static void ibm_setup(orc_sim *s) {
         s->ghost_img[s->n_ghost][1] = cc[1] + push * bn[1];
         s->ghost_img[s->n_ghost][2] = cc[2] + push * bn[2];
         s->n_ghost++;
+    }
+    if (sph) {                                                        /* compute_sphere_markers, sphere.rs:189-207 */
+        int n = s->c.sph_markers > 12 ? s->c.sph_markers : 12;
+        free(s->mark);
+        s->mark = malloc(sizeof(*s->mark) * (size_t)s->c.n_sph * n);
+        s->n_mark = 0;
+        for (int b = 0; b < s->c.n_sph; b++) { orc_sphere_markers(s->c.sph[b], n, &s->mark[s->n_mark][0]); s->n_mark += n; }
+        s->ibm_ready = 1;
+        return;
     }
     /* compute_cylinder_markers (cfd_ibm:566-598) */
     int nk = s->c.nz;
@@ -878,9 +929,15 @@ static void sponge_system(orc_sim *s) {                             /* cylinder 
         if (cx < x0) continue;
         double sc = (cx - x0) * inv_w; sc = sc < 0.0 ? 0.0 : (sc > 1.0 ? 1.0 : sc);
         double f = fmin(sig * sc * sc * dt, 1.0);
-        double uy = 4.0 * um * cy * (h - cy) / (h * h);
-        pv_t pv = {rho, uy, 0.0, 0.0, p, p / (rho * 287.058)};
-        cv_t tg = prim_to_cons(&pv, g);
+        cv_t tg;
+        if (s->c.sp_uniform) {                                       /* sphere main.rs:119-160 */
+            cv_t t = {s->c.sp_target[0], s->c.sp_target[1], s->c.sp_target[2], s->c.sp_target[3], s->c.sp_target[4]};
+            tg = t;
+        } else {
+            double uy = 4.0 * um * cy * (h - cy) / (h * h);
+            pv_t pv = {rho, uy, 0.0, 0.0, p, p / (rho * 287.058)};
+            tg = prim_to_cons(&pv, g);
+        }
         cv_t *u = &s->u[idx];
         u->r += f * (tg.r - u->r); u->ru += f * (tg.ru - u->ru); u->rv += f * (tg.rv - u->rv);
         u->rw += f * (tg.rw - u->rw); u->re += f * (tg.re - u->re);
@@ -894,15 +951,15 @@ int orc_run(orc_sim *s) {
     halo_system(s);                      /* Halo */
     boundary_system(s);                  /* Boundary (+ parabolic inflow after the stock fill) */
     zero_rhs(s);                         /* PreFlux */
-    if (s->c.n_cyl > 0) apply_ibm_ghost_bc(s);
+    if (n_bodies(s) > 0) apply_ibm_ghost_bc(s);
     if (s->c.recon == RECON_MUSCL) slopes_system(s);
     flux_sweep(s);                       /* Flux: inviscid, then viscous */
     if (s->c.viscous) viscous_system(s);
     if (s->c.body_force) body_force_system(s);   /* PostFlux */
-    if (s->c.n_cyl > 0) zero_solid_rhs(s);
+    if (n_bodies(s) > 0) zero_solid_rhs(s);
     int done = update_system(s);         /* Update */
     if (s->c.sponge) sponge_system(s);   /* PostUpdate */
-    if (s->c.n_cyl > 0) marker_force_system(s);  /* Output */
+    if (n_bodies(s) > 0) marker_force_system(s);  /* Output */
     return done;
 }
 /* One full time step (3 substages for RK3, 1 for Euler). */

@@ -45,6 +45,9 @@ class OrcCfg(C.Structure):
         ("body_force", C.c_int), ("body_gx", C.c_double),
         ("n_cyl", C.c_int), ("cyl", (C.c_double * 3) * 4),
         ("freestream", C.c_double * 5),
+        ("n_sph", C.c_int), ("sph", (C.c_double * 4) * 4),
+        ("sph_markers", C.c_int),
+        ("sp_uniform", C.c_int), ("sp_target", C.c_double * 5),
     ]
 
 
@@ -89,6 +92,10 @@ def lib():
         L.orc_vortex_l1.restype = d
         L.orc_vortex_l1.argtypes = [C.c_void_p, d, d, d, d, d, C.c_int]
         L.orc_vortex_point.argtypes = [d, d, d, d, d, d, d, d, C.c_int, pd]
+        L.orc_sphere_markers.argtypes = [pd, C.c_int, pd]
+        L.orc_sphere_sd.restype = d
+        L.orc_sphere_sd.argtypes = [pd, pd]
+        L.orc_sphere_normal.argtypes = [pd, pd, pd]
         L.orc_ic_pulse.argtypes = [C.c_void_p, d, d, d, d, d, d, d]
         L.orc_pulse_l1.restype = d
         L.orc_pulse_l1.argtypes = [C.c_void_p, d, d, d, d, d, d, d]
@@ -170,6 +177,24 @@ def limiter(name: str, a: float, b: float) -> float:
     return lib().orc_limiter(LIMITER[name], a, b)
 
 
+def sphere_markers(center, radius, n) -> np.ndarray:
+    """sphere_markers (crates/cfd_ibm/src/sphere.rs:213-237): rows of position, normal, area."""
+    b = _vec(list(center) + [radius])
+    out = np.zeros((n, 7))
+    lib().orc_sphere_markers(_p(b), n, _p(out))
+    return out
+
+
+def sphere_signed_distance(center, radius, p) -> float:
+    return lib().orc_sphere_sd(_p(_vec(list(center) + [radius])), _p(_vec(p)))
+
+
+def sphere_normal(center, radius, p) -> np.ndarray:
+    out = np.zeros(3)
+    lib().orc_sphere_normal(_p(_vec(list(center) + [radius])), _p(_vec(p)), _p(out))
+    return out
+
+
 def vortex_point(L, beta, gamma, mean_u, mean_v, t, x, y, periodic=False) -> np.ndarray:
     out = np.zeros(5)
     lib().orc_vortex_point(L, beta, gamma, mean_u, mean_v, t, x, y, int(periodic), _p(out))
@@ -182,7 +207,8 @@ class Sim:
     def __init__(self, nx, ny, nz=1, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), gamma=1.4, r_gas=R_AIR,
                  flux="hllc", muscl=False, limiter="minmod", integrator="rk3", viscous=False,
                  mu=None, cfl=0.4, fixed_dt=None, bcs=None, bc_params=None, seam_mode=0,
-                 parabolic_inflow=None, sponge=None, body_gx=None, cylinders=(), freestream=None):
+                 parabolic_inflow=None, sponge=None, body_gx=None, cylinders=(), freestream=None,
+                 spheres=(), sphere_markers=2000, sponge_uniform=None):
         c = OrcCfg()
         c.nx, c.ny, c.nz = nx, ny, nz
         for d in range(3):
@@ -221,6 +247,20 @@ class Sim:
         if freestream is not None:
             for q in range(5):
                 c.freestream[q] = freestream[q]
+        # SphereIbmPlugin bodies (cx, cy, cz, r); a simulation holds spheres or cylinders
+        if spheres and cylinders:
+            raise ValueError("spheres and cylinders in one simulation")
+        c.n_sph = len(spheres)
+        for b, sp in enumerate(spheres):
+            for q in range(4):
+                c.sph[b][q] = sp[q]
+        c.sph_markers = int(sphere_markers)
+        if sponge_uniform is not None:   # (x_start, x_end, sigma, target conserved state)
+            x0, x1, sig, tgt = sponge_uniform
+            c.sponge, c.sp_uniform = 1, 1
+            c.sp_x0, c.sp_x1, c.sp_sigma = x0, x1, sig
+            for q in range(5):
+                c.sp_target[q] = tgt[q]
         self.cfg = c
         self.nx, self.ny, self.nz = nx, ny, nz
         self.h = tuple((hi[d] - lo[d]) / (nx, ny, nz)[d] for d in range(3))

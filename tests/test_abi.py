@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_entry_points():
     text = open(os.path.join(ROOT, "include", "efvm.h")).read()
-    return sorted(set(re.findall(r"\b(efvm_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(efvm3?_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_header_declarations_match_binding_list():
