@@ -294,17 +294,17 @@ efvm_status device_alloc(efvm_ctx* c, void** p, size_t bytes) {
 }
 
 void free_ibm(efvm_ctx* c) {
-    if (c->solid) hipFree(c->solid);
-    if (c->ghost_idx) hipFree(c->ghost_idx);
-    if (c->ghost_img) hipFree(c->ghost_img);
-    if (c->markers) hipFree(c->markers);
-    if (c->stagnant) hipFree(c->stagnant);
+    if (c->solid) (void)hipFree(c->solid);
+    if (c->ghost_idx) (void)hipFree(c->ghost_idx);
+    if (c->ghost_img) (void)hipFree(c->ghost_img);
+    if (c->markers) (void)hipFree(c->markers);
+    if (c->stagnant) (void)hipFree(c->stagnant);
     c->solid = nullptr; c->ghost_idx = nullptr; c->ghost_img = nullptr; c->markers = nullptr; c->stagnant = nullptr;
 }
 
 void drop_graph(efvm_ctx* c) {
-    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) hipGraphDestroy(c->graph);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     c->graph_exec = nullptr; c->graph = nullptr; c->graph_steps = 0;
 }
 
@@ -348,7 +348,7 @@ efvm_status efvm_ctx_create(const efvm_grid* grid, const efvm_gas* gas, const ef
     if (st == EFVM_OK) st = device_alloc(c, (void**)&c->scratch, sizeof(double) * (1024 * 4 + 16));
     if (st != EFVM_OK) { efvm_ctx_destroy(c); return st; }
     for (int s = 0; s < 3; s++)
-        for (int m = 0; m < 4; m++) hipMemsetAsync(c->P[s][m], 0, c->esz * c->ncell, c->stream);
+        for (int m = 0; m < 4; m++) (void)hipMemsetAsync(c->P[s][m], 0, c->esz * c->ncell, c->stream);
     // viscous y weights from cell-centre distances (crates/cfd_solver/src/lib.rs:483-493)
     std::vector<double> yw(3 * grid->ny);
     double dy = (grid->hi_y - grid->lo_y) / (double)grid->ny;
@@ -357,8 +357,8 @@ efvm_status efvm_ctx_create(const efvm_grid* grid, const efvm_gas* gas, const ef
         double hm = yc(j) - yc(j - 1), hp = yc(j + 1) - yc(j);
         yw[j] = 1.0 / hm; yw[grid->ny + j] = 1.0 / hp; yw[2 * grid->ny + j] = 2.0 / (hm + hp);
     }
-    hipMemcpyAsync(c->yw, yw.data(), sizeof(double) * yw.size(), hipMemcpyHostToDevice, c->stream);
-    launch_clock_set(c->clock, 0.0, 0.0, 0, c->stream);
+    (void)hipMemcpyAsync(c->yw, yw.data(), sizeof(double) * yw.size(), hipMemcpyHostToDevice, c->stream);
+    (void)launch_clock_set(c->clock, 0.0, 0.0, 0, c->stream);
     if (hipStreamSynchronize(c->stream) != hipSuccess) { efvm_ctx_destroy(c); return EFVM_ERR_HIP; }
     *out = c;
     return EFVM_OK;
@@ -366,14 +366,14 @@ efvm_status efvm_ctx_create(const efvm_grid* grid, const efvm_gas* gas, const ef
 
 void efvm_ctx_destroy(efvm_ctx* c) {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
-    for (int s = 0; s < 3; s++) for (int m = 0; m < 4; m++) if (c->P[s][m]) hipFree(c->P[s][m]);
-    if (c->clock) hipFree(c->clock);
+    for (int s = 0; s < 3; s++) for (int m = 0; m < 4; m++) if (c->P[s][m]) (void)hipFree(c->P[s][m]);
+    if (c->clock) (void)hipFree(c->clock);
     if (c->lam_part) (void)hipFree(c->lam_part);
     for (int s = 0; s < 4; s++)
         if (c->bc_tab[s]) (void)hipFree(c->bc_tab[s]);
-    if (c->yw) hipFree(c->yw);
+    if (c->yw) (void)hipFree(c->yw);
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->snap) (void)hipFree(c->snap);
     if (c->faces) (void)hipFree(c->faces);
@@ -381,9 +381,9 @@ void efvm_ctx_destroy(efvm_ctx* c) {
     if (c->lstats) (void)hipFree(c->lstats);
     if (c->dlist) (void)hipFree(c->dlist);
     free_ibm(c);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -614,7 +614,7 @@ efvm_status efvm_graph_capture(efvm_ctx* c, int n_steps) {
         for (int q = 0; q < nsub && st == EFVM_OK; q++) st = run_substage(c);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (st != EFVM_OK) { if (g) hipGraphDestroy(g); return st; }
+    if (st != EFVM_OK) { if (g) (void)hipGraphDestroy(g); return st; }
     if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("end capture: ") + hipGetErrorString(e));
     c->graph = g;
     HIPCHK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));

@@ -44,6 +44,7 @@ struct efvm_ctx3 {
     double* ghost_img = nullptr;
     int n_ghost = 0;
     double* markers = nullptr;
+    double* mforce = nullptr;     // per-marker force contributions (6 per marker)
     int n_mark = 0;
     double* stagnant = nullptr;
     int ibm_flipped = 1;
@@ -138,8 +139,10 @@ void free_ibm3(efvm_ctx3* c) {
     if (c->ghost_idx) (void)hipFree(c->ghost_idx);
     if (c->ghost_img) (void)hipFree(c->ghost_img);
     if (c->markers) (void)hipFree(c->markers);
+    if (c->mforce) (void)hipFree(c->mforce);
     if (c->stagnant) (void)hipFree(c->stagnant);
     c->solid = nullptr; c->ghost_idx = nullptr; c->ghost_img = nullptr; c->markers = nullptr; c->stagnant = nullptr;
+    c->mforce = nullptr;
     c->n_ghost = 0; c->n_mark = 0;
 }
 
@@ -310,6 +313,7 @@ efvm_status build_ibm3(efvm_ctx3* c) {
     if ((st = alloc3(c, (void**)&c->ghost_idx, sizeof(long) * std::max<size_t>(1, gidx.size()))) != EFVM_OK) return st;
     if ((st = alloc3(c, (void**)&c->ghost_img, sizeof(double) * std::max<size_t>(3, gimg.size()))) != EFVM_OK) return st;
     if ((st = alloc3(c, (void**)&c->markers, sizeof(double) * std::max<size_t>(7, mk.size()))) != EFVM_OK) return st;
+    if ((st = alloc3(c, (void**)&c->mforce, sizeof(double) * 6 * std::max<size_t>(1, mk.size() / 7))) != EFVM_OK) return st;
     if ((st = alloc3(c, (void**)&c->stagnant, sizeof(double) * 5)) != EFVM_OK) return st;
     HIPCHK3(c, hipMemcpy(c->solid, solid.data(), c->nraw, hipMemcpyHostToDevice));
     if (!gidx.empty()) {
@@ -633,7 +637,7 @@ efvm_status efvm3_marker_force(efvm_ctx3* c, double out[6]) {
     Args3 a = args3(c);
     const int set = current_set(c);
     planes3(a, c->P[set], c->P[set], c->P[set]);
-    LAUNCH3(c, launch3_markers(a, (int)c->esz, c->n_mark, c->markers, c->h[0], c->out, c->stream), "marker force");
+    LAUNCH3(c, launch3_markers(a, (int)c->esz, c->n_mark, c->markers, c->h[0], c->mforce, c->out, c->stream), "marker force");
     HIPCHK3(c, hipMemcpyAsync(out, c->out, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
     HIPCHK3(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;

@@ -51,7 +51,8 @@ hipError_t launch3_stage(const Args3& a, int esz, hipStream_t st);
 int cfl3_blocks(long ncell);
 hipError_t launch3_cfl(const Args3& a, int esz, double* part, double* target, hipStream_t st);
 hipError_t launch3_sponge(const Args3& a, int esz, const Sponge3& s, hipStream_t st);
-hipError_t launch3_markers(const Args3& a, int esz, int n, const double* mk, double off, double* out, hipStream_t st);
+hipError_t launch3_markers(const Args3& a, int esz, int n, const double* mk, double off, double* contrib, double* out,
+                           hipStream_t st);
 hipError_t launch3_sums(const Args3& a, int esz, double* part, int nb, double vol, double* out, hipStream_t st);
 hipError_t launch3_clock_set(DevClock* clk, double dt, double time, long long step, hipStream_t st);
 

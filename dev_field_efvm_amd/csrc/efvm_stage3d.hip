@@ -50,7 +50,10 @@ __device__ __forceinline__ double dt_of(const Args3& a) {
 
 // ---------------------------------------------------------------------------
 // Halo: ghost slabs of periodic axis d take the interior cell n[d] away along d
-// (the other two indices as they are, ghosts of earlier axes included).
+// (the other two indices as they are, ghosts of earlier axes included).  An axis of
+// one cell has its outer ghost layer two cells away from any interior cell: it takes
+// the periodic image too (the reference copies a not yet refreshed ghost there; no
+// stencil, sample or sum reads that layer).
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void halo3_kernel(Args3 a, int d) {
@@ -64,7 +67,9 @@ __global__ __launch_bounds__(256) void halo3_kernel(Args3 a, int d) {
     const int r = id[d];
     if (r >= NGH && r < NGH + n[d]) return;
     const long st[3] = {1, a.NX, (long)a.NX * a.NY};
-    const long src = q + (long)(r < NGH ? n[d] : -n[d]) * st[d];
+    int rs = (r - NGH) % n[d];
+    if (rs < 0) rs += n[d];
+    const long src = q + (long)(rs + NGH - r) * st[d];
     T* const* U = (T* const*)a.uin;
     st5(U, q, ld5((const T* const*)a.uin, src));
 }
@@ -191,45 +196,44 @@ __global__ __launch_bounds__(256) void flip3_kernel(Args3 a, const double* __res
 }
 
 // MarkerForce (cfd_ibm:442-480, 601-615): per marker, sample at offset `off` along the
-// normal; pressure -p n A and viscous mu u_t / off A.  Lanes evaluate markers; lane 0
-// adds them in marker order, so the FP64 total does not depend on scheduling.
+// normal; pressure -p n A and viscous mu u_t / off A.  One thread per marker writes its six
+// contributions (zero where the sample fails); one block then adds them in a fixed order
+// (strided partial sums, then a fixed tree), so the FP64 total does not depend on scheduling.
 template <class T>
-__global__ void marker3_kernel(Args3 a, int n, const double* __restrict__ mk, double off, double* out) {
-    __shared__ double part[64][6];
-    __shared__ int okf[64];
-    const int l = threadIdx.x;
-    double tot[6] = {0, 0, 0, 0, 0, 0};
-    for (int base = 0; base < n; base += 64) {
-        const int q = base + l;
-        double c6[6] = {0, 0, 0, 0, 0, 0};
-        bool ok = false;
-        if (q < n) {
-            const double* m = mk + 7 * q;
-            const double sp[3] = {m[0] + off * m[3], m[1] + off * m[4], m[2] + off * m[5]};
-            P6<double> im;
-            if (sample3<T>(a, (const T* const*)a.uin, sp, im) && isfinite(im.p) && isfinite(im.t) && isfinite(im.u) &&
-                isfinite(im.v) && isfinite(im.w)) {
-                const double us[3] = {im.u - 0.0, im.v - 0.0, im.w - 0.0};
-                const double udn = us[0] * m[3] + us[1] * m[4] + us[2] * m[5];
-                const double mu = a.visc_const ? a.mu : ref::sutherland(im.t);
-                for (int d = 0; d < 3; d++) {
-                    const double ut = us[d] - udn * m[3 + d];
-                    c6[d] = -im.p * m[3 + d] * m[6];
-                    c6[3 + d] = mu * ut / off * m[6];
-                }
-                ok = true;
-            }
+__global__ __launch_bounds__(256) void marker3_eval_kernel(Args3 a, int n, const double* __restrict__ mk, double off,
+                                                           double* __restrict__ contrib) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double* m = mk + 7 * q;
+    const double sp[3] = {m[0] + off * m[3], m[1] + off * m[4], m[2] + off * m[5]};
+    double c6[6] = {0, 0, 0, 0, 0, 0};
+    P6<double> im;
+    if (sample3<T>(a, (const T* const*)a.uin, sp, im) && isfinite(im.p) && isfinite(im.t) && isfinite(im.u) &&
+        isfinite(im.v) && isfinite(im.w)) {
+        const double us[3] = {im.u - 0.0, im.v - 0.0, im.w - 0.0};
+        const double udn = us[0] * m[3] + us[1] * m[4] + us[2] * m[5];
+        const double mu = a.visc_const ? a.mu : ref::sutherland(im.t);
+        for (int d = 0; d < 3; d++) {
+            const double ut = us[d] - udn * m[3 + d];
+            c6[d] = -im.p * m[3 + d] * m[6];
+            c6[3 + d] = mu * ut / off * m[6];
         }
-        for (int d = 0; d < 6; d++) part[l][d] = c6[d];
-        okf[l] = ok;
-        __syncthreads();
-        if (l == 0) {
-            for (int k = 0; k < 64 && base + k < n; k++)
-                if (okf[k]) for (int d = 0; d < 6; d++) tot[d] += part[k][d];
-        }
+    }
+    for (int d = 0; d < 6; d++) contrib[6 * (long)q + d] = c6[d];
+}
+__global__ __launch_bounds__(256) void marker3_sum_kernel(const double* __restrict__ contrib, int n, double* out) {
+    __shared__ double sh[256][6];
+    const int t = threadIdx.x;
+    double s6[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = t; q < n; q += 256)
+        for (int d = 0; d < 6; d++) s6[d] += contrib[6 * (long)q + d];
+    for (int d = 0; d < 6; d++) sh[t][d] = s6[d];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) for (int d = 0; d < 6; d++) sh[t][d] += sh[t + w][d];
         __syncthreads();
     }
-    if (l == 0) for (int d = 0; d < 6; d++) out[d] = tot[d];
+    if (t == 0) for (int d = 0; d < 6; d++) out[d] = sh[0][d];
 }
 
 // ---------------------------------------------------------------------------
@@ -573,9 +577,13 @@ hipError_t launch3_sponge(const Args3& a, int esz, const Sponge3& s, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch3_markers(const Args3& a, int esz, int n, const double* mk, double off, double* out, hipStream_t st) {
-    if (esz == 8) marker3_kernel<double><<<1, 64, 0, st>>>(a, n, mk, off, out);
-    else marker3_kernel<float><<<1, 64, 0, st>>>(a, n, mk, off, out);
+hipError_t launch3_markers(const Args3& a, int esz, int n, const double* mk, double off, double* contrib, double* out,
+                           hipStream_t st) {
+    if (n > 0) {
+        if (esz == 8) marker3_eval_kernel<double><<<(n + 255) / 256, 256, 0, st>>>(a, n, mk, off, contrib);
+        else marker3_eval_kernel<float><<<(n + 255) / 256, 256, 0, st>>>(a, n, mk, off, contrib);
+    }
+    marker3_sum_kernel<<<1, 256, 0, st>>>(contrib, n, out);
     return hipGetLastError();
 }
 
