@@ -6,11 +6,11 @@
 //  * per face state one reciprocal (1/rho) and one reciprocal square root
 //    (1/a, with a = (gamma p/rho) * rsq), shared by the flux that uses it;
 //  * Thornber's z = min(|v|_max / a_max, 1) uses the stored 1/a;
-//  * HLLC star state without divisions except 1/(S_K - S*):
-//      rho* = rho_K (S_K - u_K) D,  (rho E)* = D [(S_K - u_K) E_K + (S* - u_K)((S_K - u_K) rho_K S* + p_K)],
-//    D = 1/(S_K - S*)  (Toro sec. 10.4, multiplied through by rho*);
-//  * upwind side chosen by selects; the star correction added by select so an
-//    out-of-fan inf/NaN never leaks into the flux.
+//  * HLLC star region in flux form, F*_K = [S* (S_K U_K - F_K) + S_K p* D*] D,
+//    D = 1/(S_K - S*), with S* and D from one reciprocal (Toro sec. 10.4: the same
+//    flux as F_K + S_K (U*_K - U_K), without forming the star state);
+//  * upwind side chosen by selects; the star flux taken by select so an out-of-fan
+//    inf/NaN never leaks into the flux.
 // Mathematically identical to the reference; not bitwise.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -115,29 +115,38 @@ __host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R)
     const bool left = N <= 0.f;
     const float rK = pick(left, L.r, R.r), uK = pick(left, ul, ur), vK = pick(left, vl, vr);
     const float pK = pick(left, L.p, R.p), eK = pick(left, el, er), unK = pick(left, unl, unr), sK = pick(left, sl, sr);
+    const float fK = pick(left, fl, fr);            // rho_K (S_K - u_K)
     const float X = sK * Dn - N;
     const float rr = rcp(Dn * X);
     const float ss = N * X * rr;
     const float D = Dn * Dn * rr;
-    const float ruK = rK * uK, rvK = rK * vK;
+    // physical flux of side K
+    const float mK = rK * unK;
     C4<float> f;
-    f.r = rK * unK;
-    f.ru = ruK * unK + (DIR == 0 ? pK : 0.f);
-    f.rv = rvK * unK + (DIR == 1 ? pK : 0.f);
+    f.r = mK;
+    f.ru = mK * uK + (DIR == 0 ? pK : 0.f);
+    f.rv = mK * vK + (DIR == 1 ? pK : 0.f);
     f.re = (eK + pK) * unK;
-    const bool fan = (left & (sl < 0.f)) | (!left & (sr >= 0.f));   // lane-mask logic, no select
-    const float sm = sK - unK;
-    const float rs = rK * sm * D;
-    const float es = D * (sm * eK + (ss - unK) * (sm * rK * ss + pK));
-    const float mun = rs * ss;                 // normal momentum of the star state
-    const float dr = sK * (rs - rK);
-    const float dru = sK * ((DIR == 0 ? mun : rs * uK) - ruK);
-    const float drv = sK * ((DIR == 1 ? mun : rs * vK) - rvK);
-    const float dre = sK * (es - eK);
-    f.r += fan ? dr : 0.f;
-    f.ru += fan ? dru : 0.f;
-    f.rv += fan ? drv : 0.f;
-    f.re += fan ? dre : 0.f;
+    // star-region flux in flux form (Toro sec. 10.4, eqs. 10.37-10.41 combined):
+    //   F*_K = [S* (S_K U_K - F_K) + S_K p* D*] / (S_K - S*),  D* = (0, n, S*),
+    //   p* = p_K + rho_K (S_K - u_K)(S* - u_K),
+    // with S_K U_K - F_K = (fK, fK u_n - p_K (normal), fK u_t, E_K (S_K - u_n) - p_K u_n).
+    const float SD = ss * D;
+    const float ps = pK + fK * (ss - unK);
+    const float sp = sK * ps;
+    const float ms = fK * SD;                                  // mass
+    const float ns = (ss * (fK * unK - pK) + sp) * D;          // normal momentum
+    const float es = SD * ((eK * (sK - unK) - pK * unK) + sp); // energy
+    const float fsr = ms, fsu = DIR == 0 ? ns : ms * uK, fsv = DIR == 1 ? ns : ms * vK;
+    // inside the fan the star flux, outside the side's own flux: lane selects, so an
+    // out-of-fan inf/NaN of the star terms never reaches the flux
+    // left: S_L < 0; right: S_R >= 0 -- one compare of S_K against the side flag, so the
+    // condition stays a lane mask (no 0/1 values in vector registers)
+    const bool fan = (sK < 0.f) == left;
+    f.r = pick(fan, fsr, f.r);
+    f.ru = pick(fan, fsu, f.ru);
+    f.rv = pick(fan, fsv, f.rv);
+    f.re = pick(fan, es, f.re);
     return f;
 }
 
