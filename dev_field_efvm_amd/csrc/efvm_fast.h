@@ -93,7 +93,8 @@ __host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R)
     float el = L.re, er = R.re;
     if (THORNBER) {
         float ql = ul * ul + vl * vl, qr = ur * ur + vr * vr;
-        float z = fminf(fsqrt(fmaxf(ql, qr)) * fminf(L.ia, R.ia), 1.f);
+        const float vm = fsqrt(fmaxf(ql, qr));
+        float z = fminf(fminf(vm * L.ia, vm * R.ia), 1.f);
         float hz = 0.5f * z;
         float au = 0.5f * (ul + ur), av = 0.5f * (vl + vr);
         float du = hz * (ul - ur), dv = hz * (vl - vr);

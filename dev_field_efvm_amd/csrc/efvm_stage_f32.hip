@@ -26,7 +26,7 @@ namespace efvm {
 using F4 = fast::C4<float>;
 using fast::Prep;
 
-enum { FEAT_VISC = 1, FEAT_IBM = 2, FEAT_CBC = 4 };
+enum { FEAT_VISC = 1, FEAT_IBM = 2, FEAT_CBC = 4, FEAT_BODY = 8 };
 
 // lane i receives the value of lane i-1 (from_left) / lane i+1 (from_right); the
 // lane with no source (0, resp. 63) receives 0 (bound_ctrl), so the move needs no
@@ -83,17 +83,17 @@ __device__ __forceinline__ F4 load_row(const StageArgs& a, const float* const* U
     return v;
 }
 
-// Half the limited slope: the face states are u -/+ half_slope.  For MC the
-// factor 1/2 is folded into the limiter's own power-of-two scalings (bitwise equal
-// to 0.5 * mc(a, b)).
+// Half the limited slope: the face states are u -/+ half_slope.  minmod(x, y) is the
+// median of (0, x, y), and the three-argument minmod is minmod applied twice, so
+//   MC / 2     = minmod(a, b, (a + b) / 4) = med3(0, med3(0, a, b), (a + b) / 4),
+//   minmod / 2 = med3(0, a, b) / 2,
+// the same values as 0.5 * mc(a, b) and 0.5 * minmod(a, b) (the factors 1/2 and 1/4
+// are exact) in two or three VALU operations.
 template <int LIM>
 __device__ __forceinline__ float half_slope(float a, float b) {
-    if (LIM == 1) {
-        const float s = a + b;
-        const float m = fminf(fminf(fabsf(a), fabsf(b)), 0.25f * fabsf(s));
-        return a * b > 0.f ? copysignf(m, s) : 0.f;
-    }
-    return 0.5f * fast::minmod(a, b);
+    const float m = __builtin_amdgcn_fmed3f(0.f, a, b);
+    if (LIM == 1) return __builtin_amdgcn_fmed3f(0.f, m, 0.25f * (a + b));
+    return 0.5f * m;
 }
 template <int LIM>
 __device__ __forceinline__ F4 half_slope4(const F4& dr, const F4& dl) {
@@ -199,8 +199,8 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
             const bool yS_rec = row_ghost ? (a.per_y && seam_rec) : !(jc == 0 && !a.per_y);
             const bool yN_rec = row_ghost ? (a.per_y && seam_rec) : !(jc == ny - 1 && !a.per_y);
             // row flags are wave-uniform: keep them (and the factors) in scalar registers
-            const float hS = __builtin_amdgcn_readfirstlane((int)yS_rec) ? -1.f : 0.f;
-            const float hN = __builtin_amdgcn_readfirstlane((int)yN_rec) ? 1.f : 0.f;
+            const float hS = yS_rec ? -1.f : 0.f;
+            const float hN = yN_rec ? 1.f : 0.f;
             const F4 sy = half_slope4<LIM>(fwd, bwd);
             lo_p = fast::prep(axpy4(uc, hS, sy), g, gm1);
             hi_p = fast::prep(axpy4(uc, hN, sy), g, gm1);
@@ -280,7 +280,7 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
             float rrv = xin.rv * dtix + (gin.rv - gout.rv) * dtiy;
             float rre = xin.re * dtix + (gin.re - gout.re) * dtiy;
             if (VISC) { rru += dt * vin[0]; rrv += dt * vin[1]; rre += dt * vin[2]; }
-            if (a.body_force) { rru += um.r * gxdt; rre += gxdt * um.ru; }
+            if ((FEAT & FEAT_BODY) && a.body_force) { rru += um.r * gxdt; rre += gxdt * um.ru; }
             if (IBM && a.solid[q]) { rr = 0.f; rru = 0.f; rrv = 0.f; rre = 0.f; }
             float o0, o1, o2, o3;
             if (ca != 0.f) {
@@ -470,6 +470,7 @@ int features_of(const StageArgs& a) {
             a.bc_kind[s] == BC_TABLE)
             f |= FEAT_CBC;
     if (a.parabolic) f |= FEAT_CBC;
+    if (a.body_force) f |= FEAT_BODY;
     return f;
 }
 }  // namespace
@@ -480,7 +481,7 @@ int features_of(const StageArgs& a) {
     do {                                                                                    \
         const int lim = a_.muscl ? a_.limiter : LIM_NONE;                                   \
         const int ft = features_of(a_);                                                     \
-        const int all = FEAT_VISC | FEAT_IBM | FEAT_CBC;                                    \
+        const int all = FEAT_VISC | FEAT_IBM | FEAT_CBC | FEAT_BODY;                        \
         switch (a_.flux) {                                                                  \
         case FLUX_RUSANOV:                                                                  \
             if (lim == LIM_NONE) { if (ft == 0) LAUNCH(FLUX_RUSANOV, LIM_NONE, 0); else LAUNCH(FLUX_RUSANOV, LIM_NONE, all); } \
