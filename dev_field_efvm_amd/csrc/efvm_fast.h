@@ -31,15 +31,25 @@ inline float rsq(float x) { return 1.0f / std::sqrt(x); }
 inline float fsqrt(float x) { return std::sqrt(x); }
 #endif
 
+// minmod(a, b) is the median of (0, a, b); MC = sign(a) min(2|a|, 2|b|, |a+b|/2) when
+// a b > 0, else 0, is the three-argument minmod(2a, 2b, (a+b)/2), i.e. minmod applied twice
+// (on the device one med3 each; the host forms give the same values)
 __host__ __device__ __forceinline__ float minmod(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fmed3f(0.f, a, b);
+#else
     float m = fabsf(a) < fabsf(b) ? a : b;
     return a * b > 0.f ? m : 0.f;
+#endif
 }
-// MC: sign(a) min(2|a|, 2|b|, |a+b|/2) when a b > 0, else 0
 __host__ __device__ __forceinline__ float mc(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fmed3f(0.f, 2.f * __builtin_amdgcn_fmed3f(0.f, a, b), 0.5f * (a + b));
+#else
     float s = a + b;
     float m = fminf(2.f * fminf(fabsf(a), fabsf(b)), 0.5f * fabsf(s));
     return a * b > 0.f ? copysignf(m, s) : 0.f;
+#endif
 }
 template <int LIM> __host__ __device__ __forceinline__ float limit(float a, float b) {
     if (LIM == 1) return mc(a, b);
@@ -94,11 +104,11 @@ __host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R)
     if (THORNBER) {
         float ql = ul * ul + vl * vl, qr = ur * ur + vr * vr;
         const float vm = fsqrt(fmaxf(ql, qr));
-        float z = fminf(fminf(vm * L.ia, vm * R.ia), 1.f);
-        float hz = 0.5f * z;
-        float au = 0.5f * (ul + ur), av = 0.5f * (vl + vr);
-        float du = hz * (ul - ur), dv = hz * (vl - vr);
-        ul = au + du; ur = au - du; vl = av + dv; vr = av - dv;
+        const float z = fminf(fminf(vm * L.ia, vm * R.ia), 1.f);
+        // avg +/- z (v_L - v_R)/2  ==  v_L - w dv,  v_R + w dv  with w = (1 - z)/2, dv = v_L - v_R
+        const float w = fmaf(-0.5f, z, 0.5f);
+        const float du = ul - ur, dv = vl - vr;
+        ul = fmaf(-w, du, ul); ur = fmaf(w, du, ur); vl = fmaf(-w, dv, vl); vr = fmaf(w, dv, vr);
         // rebuilt energies keep the internal energy (and p, a) unchanged
         el = L.re + 0.5f * L.r * ((ul * ul + vl * vl) - ql);
         er = R.re + 0.5f * R.r * ((ur * ur + vr * vr) - qr);
