@@ -104,6 +104,7 @@ struct efvm_ctx {
     int snap_parity = 0, snap_lam = 0, snap_flip = 0;
     int learned = 0;               // learned face reconstruction attached
     int learned_bypass = 0;        // gate evaluated only where the ungated face value is near the TVD bound
+    int learned_own4 = 0;          // the attached network's correction is the 4-tap own-variable form
     float* faces = nullptr;        // 16 planes of face primitives
     unsigned* dlist = nullptr;     // bypass: deferred faces (2 nx ny entries) and their count
     mlp::Packed* wpack = nullptr;  // the attached network's inference weights
@@ -253,6 +254,7 @@ efvm_status run_substage(efvm_ctx* c) {
         fa.flux = c->sch.flux;
         fa.list = c->dlist;
         fa.n_list = c->dlist ? c->dlist + 2 * c->ncell : nullptr;
+        fa.own4 = c->learned_own4;
         hipError_t e = c->learned_bypass ? launch_mlp_faces_bypass(fa, c->stream) : launch_mlp_faces(fa, c->stream);
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("face kernels: ") + hipGetErrorString(e));
         a.face_count = c->learned_bypass ? fa.n_list : nullptr;
@@ -1155,6 +1157,7 @@ struct efvm_mlp {
     long loss_cap = 0;
     mlp::Packed* packed = nullptr;
     int packed_valid = 0;
+    int own4 = 0;                  // the packed correction is the 4-tap own-variable form
     std::string err;
 };
 
@@ -1233,6 +1236,7 @@ efvm_status mlp_repack(efvm_mlp* m) {
     MHIPCHK(m, hipStreamSynchronize(m->stream));
     mlp::Packed P;
     pack_weights(h.data(), P);
+    m->own4 = P.own4;
     MHIPCHK(m, hipMemcpyAsync(m->packed, &P, sizeof(P), hipMemcpyHostToDevice, m->stream));
     MHIPCHK(m, hipStreamSynchronize(m->stream));
     m->packed_valid = 1;
@@ -1579,6 +1583,7 @@ efvm_status efvm_set_learned(efvm_ctx* c, efvm_mlp* m) {
     HIPCHK(c, hipStreamSynchronize(m->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->learned = 1;
+    c->learned_own4 = m->own4;
     return EFVM_OK;
 }
 
@@ -1628,6 +1633,7 @@ efvm_status efvm_learned_faces(efvm_ctx* c, float* out, size_t n) {
     fa.flux = c->sch.flux;
     fa.list = nullptr;
     fa.n_list = nullptr;
+    fa.own4 = c->learned_own4;
     HIPCHK(c, launch_mlp_faces(fa, c->stream));
     HIPCHK(c, hipMemcpyAsync(out, c->faces, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

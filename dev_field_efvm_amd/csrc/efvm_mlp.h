@@ -33,7 +33,7 @@ constexpr int OFF_L = OFF_B3 + NOUT;
 constexpr int OFF_BL = OFF_L + NOUT * NIN;
 constexpr int NPARAM = OFF_BL + NOUT;   // 5848
 constexpr int TB = 64;                  // samples per training block (one wave)
-constexpr int STAT_SLOTS = 4096;        // waves of the largest face-kernel launch
+constexpr int STAT_SLOTS = 8192;        // waves of the largest face-kernel launch
 
 // features: x[4k+v], k = 0..4 <-> stencil cells -2..2 (k = 3 is the neighbour across
 // the face), v = rho, u_n, u_t, p.  x = (w_k - w_0) / sigma_v for k != 2 and
@@ -100,6 +100,7 @@ struct FaceArgs {
                                  // elsewhere the gate is fully open (phi = 1)
     unsigned* list;              // deferred faces of the bypass passes: axis * nx*ny + row * nx + column
     unsigned* n_list;            // their count (device)
+    int own4;                    // host copy of w->own4 (selects the light pass's specialisation)
 };
 
 // one training minibatch (loss and gradient)

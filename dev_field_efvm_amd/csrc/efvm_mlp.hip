@@ -407,16 +407,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 
 // PASS_LIGHT on its own: one side at a time, so only the stencil and the two face
-// values stay live (no network operands, no per-side feature state).
-// PASS_LIGHT on its own: one side at a time, so only the stencil and the two face
-// values stay live (no network operands, no per-side feature state).
+// values stay live (no network operands, no per-side feature state).  OWN4 (the
+// correction is a 4-tap stencil per variable, the shipped network's form) is a template
+// parameter: the general featurisation's registers would otherwise set the occupancy.
+template <bool OWN4>
 __global__ __launch_bounds__(256) void faces_light_kernel(FaceArgs a) {
     __shared__ float skip[SK_N];
     stage_skip(a.w, skip);
     const int lane = lane_id(), wv = threadIdx.x >> 6;
     const float gm = a.gamma, gm1 = a.gamma - 1.f;
     const long np = (long)a.nx * a.ny;
-    const bool own4 = a.w->own4 != 0;
+    constexpr bool own4 = OWN4;
     unsigned long long ntot = 0;
     for (int task = blockIdx.x * 4 + wv; task < a.n_tasks; task += gridDim.x * 4) {
         int axis, row, cc;
@@ -924,10 +925,13 @@ hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st) {
 // the sequence can be captured in a graph; the stage kernel that consumes the face
 // states clears the counter for the next stage (it starts at zero when allocated).
 hipError_t launch_mlp_faces_bypass(const mlp::FaceArgs& a, hipStream_t st) {
-    // two tasks per wave (fewer, longer waves measured faster than one task each); at most
-    // STAT_SLOTS waves.  The list pass is persistent and small: deferred faces are rare.
-    const int lb = std::min((a.n_tasks + 7) / 8, mlp::STAT_SLOTS / 4);
-    mlp::faces_light_kernel<<<lb, 256, 0, st>>>(a);
+    // one task (64 faces) per wave, at most STAT_SLOTS waves: the 4-tap light pass holds 67
+    // VGPRs, so 7 of its waves fit per SIMD and a 512^2 stage's 8192 waves are nearly all in
+    // flight at once (two tasks per wave measured 2% slower).  The list pass is persistent and
+    // small: deferred faces are rare.
+    const int lb = std::min((a.n_tasks + 3) / 4, mlp::STAT_SLOTS / 4);
+    if (a.own4) mlp::faces_light_kernel<true><<<lb, 256, 0, st>>>(a);
+    else mlp::faces_light_kernel<false><<<lb, 256, 0, st>>>(a);
     mlp::faces_list_kernel<<<128, 256, 0, st>>>(a);
     return hipGetLastError();
 }
