@@ -288,7 +288,19 @@ def bench_sphere(args):
         ms.append(ctx.last_elapsed_ms())
     per_sub_s = min(ms) * 1e-3 / 60.0
     cells = case.nx * case.ny * case.nz
-    return {
+    # stage-kernel roofline: HBM bytes per raw cell-substage are U and U0 read and the new set
+    # written, 5 components each; the kernel is bound by its FP64 arithmetic, not by these bytes
+    raw = (case.nx + 4) * (case.ny + 4) * (case.nz + 4)
+    esz = 8 if prec == "fp64" else 4
+    stage_bytes = raw * 3 * 5 * esz
+    ctx.step(1)
+    ctx.synchronize()
+    t_stage = _time_stage3(ctx)
+    roof = {"bound": "hbm", "achieved": round(stage_bytes / t_stage / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(stage_bytes / t_stage / 1e9 / PEAK_HBM_GBS, 4), "traffic": None,
+            "kernel": "one 3-D substage, all its kernels (stage3_kernel is 78% of it, bound by FP64 arithmetic: six HLLC-Thornber faces per cell in the reference's form)",
+            "mean_substage_us": round(t_stage * 1e6, 2), "algorithmic_bytes_per_launch": stage_bytes}
+    out = {
         "metric": METRIC, "value": cells / per_sub_s, "unit": "cell-updates/s", "n_gpus": 1, "steps": 9000 // 3,
         "warmup": 0, "ms_per_step": 3e3 * per_sub_s, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if prec == "fp64" else "f32",
@@ -297,7 +309,40 @@ def bench_sphere(args):
                                "HLLC+viscous+RK3, 9000 substages", "cd": r["cd"], "cd_published": 2.5402,
                    "cd_ptp": r["ptp"], "cd_ptp_published": 0.027, "sim_time": r["time"],
                    "wall_s": round(r["wall_s"], 2), "us_per_substage": round(per_sub_s * 1e6, 2)},
+        "roofline": roof,
     }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_sphere()
+    return out
+
+
+def _time_stage3(ctx, substages: int = 30):
+    """Mean GPU time of one substage of the 3-D context (HIP events around whole steps, which
+    also hold the fill, mirror, CFL and sponge kernels: an upper bound for the stage kernel)."""
+    ctx.step(substages // 3)
+    return ctx.last_elapsed_ms() * 1e-3 / substages
+
+
+def cpu_baseline_sphere(n: int = 48, steps: int = 300):
+    """The oracle (single thread, FP64) on the sphere case scaled to n x n/2 x n/2 cells."""
+    from oracle import orc
+    from dev_field_efvm_amd.sphere import SphereCase
+    c = SphereCase(nx=n, ny=n // 2, nz=n // 2, markers=2000)
+    fs = c.freestream()
+    sim = orc.Sim(c.nx, c.ny, c.nz, lo=c.lo, hi=c.hi, flux="hllc", viscous=True, mu=c.mu, cfl=c.cfl,
+                  bcs=["subsonic_inflow", "pressure_outlet", "symmetry", "symmetry", "symmetry", "symmetry"],
+                  bc_params=[(c.rho, c.u_inf, 0.0, 0.0, c.t_inf), (c.p,), None, None, None, None],
+                  spheres=[(0.0, 0.0, 0.0, 0.5 * c.D)], freestream=fs, sphere_markers=c.markers,
+                  sponge_uniform=(c.sponge_x0, c.hi[0], c.sponge_sigma, fs))
+    import numpy as np
+    sim.set_interior(np.broadcast_to(fs, (c.nz, c.ny, c.nx, 5)).copy())
+    sim.step(1)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    dt = time.perf_counter() - t0
+    cells = c.nx * c.ny * c.nz
+    return {"value": cells * 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/efvm_oracle.c FP64 sphere case at {c.nx}x{c.ny}x{c.nz}, {steps} steps ({dt:.1f} s, 1 thread)"}
 
 
 def bench_copy(args):
