@@ -588,20 +588,13 @@ class App:
 
     # restart files (examples/restart_determinism/main.rs:177-226 write_checkpoint / read_checkpoint)
     def write_checkpoint(self, path: str):
-        self._need_2d("restart files")
         self.ctx.checkpoint_save(path)
 
     def read_checkpoint(self, path: str):
-        self._need_2d("restart files")
         self.ctx.checkpoint_load(path)
-
-    def _need_2d(self, what):
-        if isinstance(self.ctx, Context3):
-            raise EfvmError(ERR_UNSUPPORTED, f"{what} are built for the 2-D context")
 
     def finish(self):
         """The cleanup hooks: OutputPlugin writes its VTK file."""
         out = self._find(OutputPlugin)
         if out is not None:
-            self._need_2d("VTK output files")
             self.ctx.write_vtk(out.path)

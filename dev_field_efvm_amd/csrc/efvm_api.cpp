@@ -1688,6 +1688,58 @@ void fmt_e(FILE* f, double x) {
 
 }  // namespace
 
+namespace efvm {
+// write_vtk_to (crates/cfd_output/src/lib.rs:36-113): RECTILINEAR_GRID over nn[0] x nn[1] x nn[2]
+// cells (x fastest), face coordinates from the centroids, density, pressure, temperature,
+// velocity and Mach in %.10e with non-finite values written as 0.
+efvm_status vtk_write_cells(const int nn[3], const double lo[3], const double h[3], double gm, double R,
+                            const double* cons5, const char* path) {
+    FILE* f = std::fopen(path, "w");
+    if (!f) return EFVM_ERR_ARG;
+    std::fprintf(f, "# vtk DataFile Version 3.0\ndev_field_efvm output\nASCII\nDATASET RECTILINEAR_GRID\n");
+    std::fprintf(f, "DIMENSIONS %d %d %d\n", nn[0] + 1, nn[1] + 1, nn[2] + 1);
+    // faces = centroid - half spacing, plus the last centroid + half spacing
+    const char* label[3] = {"X", "Y", "Z"};
+    for (int ax = 0; ax < 3; ax++) {
+        std::fprintf(f, "%s_COORDINATES %d double\n", label[ax], nn[ax] + 1);
+        for (int m = 0; m < nn[ax]; m++) {
+            const double cc = lo[ax] + ((double)m + 0.5) * h[ax];
+            fmt_e(f, cc - 0.5 * h[ax]);
+            std::fputc('\n', f);
+        }
+        const double cl = lo[ax] + ((double)(nn[ax] - 1) + 0.5) * h[ax];
+        fmt_e(f, cl + 0.5 * h[ax]);
+        std::fputc('\n', f);
+    }
+    const long nc = (long)nn[0] * nn[1] * nn[2];
+    std::fprintf(f, "CELL_DATA %ld\n", nc);
+    auto cell = [&](long q) { return C5<double>{cons5[5 * q], cons5[5 * q + 1], cons5[5 * q + 2], cons5[5 * q + 3], cons5[5 * q + 4]}; };
+    auto scalar = [&](const char* name, auto fn) {
+        std::fprintf(f, "SCALARS %s double 1\nLOOKUP_TABLE default\n", name);
+        for (long q = 0; q < nc; q++) {
+            fmt_e(f, fn(cell(q)));
+            std::fputc('\n', f);
+        }
+    };
+    scalar("density", [](const C5<double>& u) { return u.r; });
+    scalar("pressure", [&](const C5<double>& u) { return ref::pressure(u, gm); });
+    scalar("temperature", [&](const C5<double>& u) { return ref::cons_to_prim(u, gm, R).t; });
+    std::fprintf(f, "VECTORS velocity double\n");
+    for (long q = 0; q < nc; q++) {
+        double v[3];
+        ref::velocity(cell(q), v);
+        fmt_e(f, v[0]); std::fputc(' ', f); fmt_e(f, v[1]); std::fputc(' ', f); fmt_e(f, v[2]); std::fputc('\n', f);
+    }
+    scalar("mach", [&](const C5<double>& u) {
+        double v[3];
+        ref::velocity(u, v);
+        return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / ref::sound(u, gm);
+    });
+    return std::fclose(f) == 0 ? EFVM_OK : EFVM_ERR_ARG;
+}
+}  // namespace efvm
+
+
 extern "C" {
 
 efvm_status efvm_checkpoint_save(efvm_ctx* c, const char* path) {
@@ -1782,54 +1834,11 @@ efvm_status efvm_checkpoint_load(efvm_ctx* c, const char* path) {
 
 efvm_status efvm_write_vtk(const efvm_grid* g, const efvm_gas* gas, const double* cons5, size_t n_cells, const char* path) {
     if (!g || !gas || !cons5 || !path || g->nx < 1 || g->ny < 1) return EFVM_ERR_ARG;
-    const int ni = g->nx, nj = g->ny, nk = 1;
-    if ((long)n_cells != (long)ni * nj) return EFVM_ERR_ARG;
-    FILE* f = std::fopen(path, "w");
-    if (!f) return EFVM_ERR_ARG;
-    std::fprintf(f, "# vtk DataFile Version 3.0\ndev_field_efvm output\nASCII\nDATASET RECTILINEAR_GRID\n");
-    std::fprintf(f, "DIMENSIONS %d %d %d\n", ni + 1, nj + 1, nk + 1);
-    // faces = centroid - half spacing, plus the last centroid + half spacing
-    const double h[3] = {(g->hi_x - g->lo_x) / (double)ni, (g->hi_y - g->lo_y) / (double)nj, g->dz};
+    if ((long)n_cells != (long)g->nx * g->ny) return EFVM_ERR_ARG;
+    const int nn[3] = {g->nx, g->ny, 1};
+    const double h[3] = {(g->hi_x - g->lo_x) / (double)g->nx, (g->hi_y - g->lo_y) / (double)g->ny, g->dz};
     const double lo[3] = {g->lo_x, g->lo_y, 0.0};
-    const int nn[3] = {ni, nj, nk};
-    const char* label[3] = {"X", "Y", "Z"};
-    for (int ax = 0; ax < 3; ax++) {
-        std::fprintf(f, "%s_COORDINATES %d double\n", label[ax], nn[ax] + 1);
-        for (int m = 0; m < nn[ax]; m++) {
-            const double cc = lo[ax] + ((double)m + 0.5) * h[ax];
-            fmt_e(f, cc - 0.5 * h[ax]);
-            std::fputc('\n', f);
-        }
-        const double cl = lo[ax] + ((double)(nn[ax] - 1) + 0.5) * h[ax];
-        fmt_e(f, cl + 0.5 * h[ax]);
-        std::fputc('\n', f);
-    }
-    const long nc = (long)ni * nj;
-    std::fprintf(f, "CELL_DATA %ld\n", nc);
-    auto cell = [&](long q) { return C5<double>{cons5[5 * q], cons5[5 * q + 1], cons5[5 * q + 2], cons5[5 * q + 3], cons5[5 * q + 4]}; };
-    auto scalar = [&](const char* name, auto fn) {
-        std::fprintf(f, "SCALARS %s double 1\nLOOKUP_TABLE default\n", name);
-        for (long q = 0; q < nc; q++) {
-            fmt_e(f, fn(cell(q)));
-            std::fputc('\n', f);
-        }
-    };
-    const double gm = gas->gamma, R = gas->r_gas;
-    scalar("density", [](const C5<double>& u) { return u.r; });
-    scalar("pressure", [&](const C5<double>& u) { return ref::pressure(u, gm); });
-    scalar("temperature", [&](const C5<double>& u) { return ref::cons_to_prim(u, gm, R).t; });
-    std::fprintf(f, "VECTORS velocity double\n");
-    for (long q = 0; q < nc; q++) {
-        double v[3];
-        ref::velocity(cell(q), v);
-        fmt_e(f, v[0]); std::fputc(' ', f); fmt_e(f, v[1]); std::fputc(' ', f); fmt_e(f, v[2]); std::fputc('\n', f);
-    }
-    scalar("mach", [&](const C5<double>& u) {
-        double v[3];
-        ref::velocity(u, v);
-        return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / ref::sound(u, gm);
-    });
-    return std::fclose(f) == 0 ? EFVM_OK : EFVM_ERR_ARG;
+    return efvm::vtk_write_cells(nn, lo, h, gas->gamma, gas->r_gas, cons5, path);
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
@@ -17,6 +18,11 @@
 
 #include "../../include/efvm.h"
 #include "efvm_stage3d.h"
+
+namespace efvm {
+efvm_status vtk_write_cells(const int nn[3], const double lo[3], const double h[3], double gm, double R,
+                            const double* cons5, const char* path);
+}
 
 using namespace efvm;
 
@@ -641,6 +647,97 @@ efvm_status efvm3_marker_force(efvm_ctx3* c, double out[6]) {
     HIPCHK3(c, hipMemcpyAsync(out, c->out, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
     HIPCHK3(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;
+}
+
+// CFDRST01 (examples/restart_determinism/main.rs:177-226): "CFDRST01", dt, time (f64), step
+// (u64), rk stage (u8), then the columns u, rhs, u0, each a u64 cell count and 5 f64 per cell
+// over the raw padded array, little-endian.  The 3-D context stores that array, so u carries
+// its ghost cells as they are; rhs is written as 0 (the PreFlux phase clears it) and u0 is the
+// RK backup U^n.
+efvm_status efvm3_checkpoint_save(efvm_ctx3* c, const char* path) {
+    if (!c || !path) return EFVM_ERR_ARG;
+    const long nraw = c->nraw;
+    std::vector<double> u(5 * nraw), u0(5 * nraw);
+    efvm_status st = efvm3_download_state(c, u.data(), nraw);    // the current substage's state
+    if (st != EFVM_OK) return st;
+    {
+        std::vector<char> h[5];
+        for (int m = 0; m < 5; m++) {
+            h[m].resize(c->esz * nraw);
+            HIPCHK3(c, hipMemcpyAsync(h[m].data(), c->P[c->cur][m], c->esz * nraw, hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK3(c, hipStreamSynchronize(c->stream));
+        for (long q = 0; q < nraw; q++)
+            for (int m = 0; m < 5; m++)
+                u0[5 * q + m] = c->esz == 8 ? reinterpret_cast<double*>(h[m].data())[q]
+                                            : (double)reinterpret_cast<float*>(h[m].data())[q];
+    }
+    double dt = 0.0, time = 0.0;
+    long step = 0;
+    int stage = 0;
+    if ((st = efvm3_get_clock(c, &dt, &time, &step, &stage)) != EFVM_OK) return st;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail3(c, EFVM_ERR_ARG, std::string("checkpoint: cannot open ") + path);
+    const unsigned long long step_u = (unsigned long long)step, len = (unsigned long long)nraw;
+    const unsigned char stage_u = (unsigned char)stage;
+    bool ok = std::fwrite("CFDRST01", 1, 8, f) == 8 && std::fwrite(&dt, 8, 1, f) == 1 && std::fwrite(&time, 8, 1, f) == 1 &&
+              std::fwrite(&step_u, 8, 1, f) == 1 && std::fwrite(&stage_u, 1, 1, f) == 1;
+    const std::vector<double> zero(5 * nraw, 0.0);
+    const double* cols[3] = {u.data(), zero.data(), u0.data()};
+    for (int k = 0; k < 3 && ok; k++)
+        ok = std::fwrite(&len, 8, 1, f) == 1 && std::fwrite(cols[k], sizeof(double), 5 * nraw, f) == (size_t)(5 * nraw);
+    const int cl = std::fclose(f);
+    if (!ok || cl != 0) return fail3(c, EFVM_ERR_ARG, std::string("checkpoint: short write to ") + path);
+    return EFVM_OK;
+}
+
+// Load a step-boundary CFDRST01 file of the same grid: the u column (ghosts included) becomes
+// the state, the clock continues from its dt, time and step, and the next step recomputes the
+// CFL, so the run continues bit for bit (crates/cfd_solver/tests/restart.rs:86-105).  Bodies
+// keep the context's mask state: on a context that has not run, the first substage refreshes
+// the mask as a new App does.
+efvm_status efvm3_checkpoint_load(efvm_ctx3* c, const char* path) {
+    if (!c || !path) return EFVM_ERR_ARG;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail3(c, EFVM_ERR_ARG, std::string("checkpoint: cannot open ") + path);
+    char magic[8];
+    double dt = 0.0, time = 0.0;
+    unsigned long long step = 0, len = 0;
+    unsigned char stage = 0;
+    const long nraw = c->nraw;
+    std::vector<double> u(5 * nraw), skip(5 * nraw);
+    efvm_status st = EFVM_OK;
+    if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "CFDRST01", 8) != 0) st = EFVM_ERR_CONFIG;
+    else if (std::fread(&dt, 8, 1, f) != 1 || std::fread(&time, 8, 1, f) != 1 || std::fread(&step, 8, 1, f) != 1 ||
+             std::fread(&stage, 1, 1, f) != 1)
+        st = EFVM_ERR_CONFIG;
+    else if (stage > 2) st = EFVM_ERR_CONFIG;
+    else if (stage != 0) st = EFVM_ERR_UNSUPPORTED;
+    for (int k = 0; k < 3 && st == EFVM_OK; k++) {
+        if (std::fread(&len, 8, 1, f) != 1 || (long)len != nraw) { st = EFVM_ERR_CONFIG; break; }
+        double* dst = k == 0 ? u.data() : skip.data();
+        if (std::fread(dst, sizeof(double), 5 * nraw, f) != (size_t)(5 * nraw)) st = EFVM_ERR_CONFIG;
+    }
+    char extra;
+    if (st == EFVM_OK && std::fread(&extra, 1, 1, f) != 0) st = EFVM_ERR_CONFIG;
+    std::fclose(f);
+    if (st == EFVM_ERR_UNSUPPORTED) return fail3(c, st, "checkpoint: restarts are taken at a step boundary (rk stage 0)");
+    if (st != EFVM_OK) return fail3(c, st, "checkpoint: not a CFDRST01 file of this grid (magic, header, cell counts or length)");
+    if ((st = efvm3_upload_state(c, u.data(), nraw)) != EFVM_OK) return st;
+    drop_graph3(c);
+    HIPCHK3(c, launch3_clock_set(c->clock, dt, time, (long long)step, c->stream));
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    c->parity = 0;
+    return EFVM_OK;
+}
+
+efvm_status efvm3_write_vtk(const efvm_grid3* g, const efvm_gas* gas, const double* cons5, size_t n_cells, const char* path) {
+    if (!g || !gas || !cons5 || !path || g->nx < 1 || g->ny < 1 || g->nz < 1) return EFVM_ERR_ARG;
+    if ((long)n_cells != (long)g->nx * g->ny * g->nz) return EFVM_ERR_ARG;
+    const int nn[3] = {g->nx, g->ny, g->nz};
+    double lo[3], h[3];
+    for (int d = 0; d < 3; d++) { lo[d] = g->lo[d]; h[d] = (g->hi[d] - g->lo[d]) / (double)nn[d]; }
+    return vtk_write_cells(nn, lo, h, gas->gamma, gas->r_gas, cons5, path);
 }
 
 efvm_status efvm3_conservation_sums(efvm_ctx3* c, double out[5]) {

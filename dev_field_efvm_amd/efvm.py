@@ -602,7 +602,8 @@ EXPORTS += [
     "efvm3_set_body_force", "efvm3_set_sponge", "efvm3_upload_state", "efvm3_download_state", "efvm3_get_clock",
     "efvm3_set_clock", "efvm3_cfl", "efvm3_run", "efvm3_step", "efvm3_synchronize", "efvm3_graph_capture",
     "efvm3_graph_launch", "efvm3_ibm_add_sphere", "efvm3_ibm_add_cylinder", "efvm3_marker_force",
-    "efvm3_conservation_sums", "efvm3_last_elapsed_ms",
+    "efvm3_conservation_sums", "efvm3_last_elapsed_ms", "efvm3_checkpoint_save", "efvm3_checkpoint_load",
+    "efvm3_write_vtk",
 ]
 
 
@@ -631,6 +632,9 @@ def _bind3(L):
         "efvm3_marker_force": [vp, pd],
         "efvm3_conservation_sums": [vp, pd],
         "efvm3_last_elapsed_ms": [vp, pd],
+        "efvm3_checkpoint_save": [vp, C.c_char_p],
+        "efvm3_checkpoint_load": [vp, C.c_char_p],
+        "efvm3_write_vtk": [C.POINTER(Grid3), C.POINTER(Gas), pd, C.c_size_t, C.c_char_p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -771,3 +775,31 @@ class Context3:
         out = np.zeros(5)
         self._chk(self._L.efvm3_conservation_sums(self._p, _pd(out)))
         return out
+
+    def checkpoint_save(self, path: str):
+        """CFDRST01 restart file of the current state (raw padded array) and clock."""
+        self._chk(self._L.efvm3_checkpoint_save(self._p, os.fsencode(path)))
+
+    def checkpoint_load(self, path: str):
+        """Resume from a step-boundary CFDRST01 file of the same grid."""
+        self._chk(self._L.efvm3_checkpoint_load(self._p, os.fsencode(path)))
+
+    def write_vtk(self, path: str):
+        """VTK legacy output of the current state."""
+        write_vtk3(path, self.download(), self.nx, self.ny, self.nz, self.lo, self.hi, self.gas.gamma, self.gas.r_gas)
+
+
+def write_vtk3(path: str, state, nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), gamma=1.4,
+               r_gas=287.058):
+    """VTK legacy RECTILINEAR_GRID file of a wire-order 3-D state (nz, ny, nx, 5) on the host."""
+    L = load_library()
+    _bind3(L)
+    g = Grid3(int(nx), int(ny), int(nz), (C.c_double * 3)(*lo), (C.c_double * 3)(*hi))
+    gs = Gas(gamma, r_gas, 0, 0.0)
+    u = np.ascontiguousarray(state, dtype=np.float64).reshape(-1)
+    if u.size != 5 * nx * ny * nz:
+        raise ValueError("state must hold 5 values per cell")
+    st = L.efvm3_write_vtk(C.byref(g), C.byref(gs), u.ctypes.data_as(C.POINTER(C.c_double)), nx * ny * nz,
+                           os.fsencode(path))
+    if st != OK:
+        raise EfvmError(st, f"efvm3_write_vtk({path}) failed")

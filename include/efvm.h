@@ -346,6 +346,16 @@ efvm_status efvm3_marker_force(efvm_ctx3* ctx, double out[6]);
 efvm_status efvm3_conservation_sums(efvm_ctx3* ctx, double out[5]);
 /* Busy time of the last efvm3_step / efvm3_graph_launch (HIP events), milliseconds. */
 efvm_status efvm3_last_elapsed_ms(efvm_ctx3* ctx, double* ms);
+/* CFDRST01 checkpoints of examples/restart_determinism/main.rs:177-226 over the 3-D raw padded
+ * array ((nx+4)(ny+4)(nz+4) cells): u with its ghost cells, rhs as 0, u0 the RK backup.  Load
+ * takes a step-boundary file of the same grid and continues bit for bit
+ * (crates/cfd_solver/tests/restart.rs:86-105). */
+efvm_status efvm3_checkpoint_save(efvm_ctx3* ctx, const char* path);
+efvm_status efvm3_checkpoint_load(efvm_ctx3* ctx, const char* path);
+/* VTK legacy RECTILINEAR_GRID of nx*ny*nz interior cells, write_vtk_to of
+ * crates/cfd_output/src/lib.rs:36-113 (the 2-D writer's fields and format).  Host only. */
+efvm_status efvm3_write_vtk(const efvm_grid3* grid, const efvm_gas* gas, const double* cons5, size_t n_cells,
+                            const char* path);
 
 #ifdef __cplusplus
 }

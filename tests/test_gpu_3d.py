@@ -309,3 +309,40 @@ def test_app_mirror_builds_the_sphere_example():
     assert np.array_equal(app.state(), ctx.download())
     assert np.array_equal(app.marker_force(), ctx.marker_force())
     assert app.solver_state().step == 10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_3d_checkpoint_restart_is_bitwise(tmp_path, precision):
+    """restored_run_continues_bit_for_bit (crates/cfd_solver/tests/restart.rs:86-105) on the 3-D
+    context: save at a step boundary, continue; a fresh context loads the file and continues;
+    states and clocks agree bit for bit.  Also the CFDRST01 layout over the raw padded array."""
+    import struct
+    n, hi = (10, 8, 6), (1.0, 0.8, 0.6)
+    st = smooth_state(*n, hi, seed=13)
+    bcs = ["periodic", "periodic", "slip", "noslip", "outflow", "symmetry"]
+
+    def make():
+        c = efvm.Context3(*n, hi=hi, precision=precision, flux="hllc", muscl=True, limiter="mc", cfl=0.4)
+        c.set_boundaries(bcs)
+        return c
+    a = make()
+    a.upload(st)
+    a.step(3)
+    path = str(tmp_path / "r3.bin")
+    a.checkpoint_save(path)
+    a.step(4)
+    b = make()
+    b.checkpoint_load(path)
+    b.step(4)
+    assert np.array_equal(a.download(raw=True), b.download(raw=True))
+    assert a.clock() == b.clock()
+    raw = (n[0] + 4) * (n[1] + 4) * (n[2] + 4)
+    data = open(path, "rb").read()
+    assert data[:8] == b"CFDRST01" and len(data) == 8 + 8 + 8 + 8 + 1 + 3 * (8 + 40 * raw)
+    assert struct.unpack_from("<Q", data, 24)[0] == 3 and data[32] == 0
+    assert struct.unpack_from("<Q", data, 33)[0] == raw
+    # a file of another grid is refused
+    c = efvm.Context3(n[0] + 1, n[1], n[2], hi=hi, precision=precision)
+    with pytest.raises(EfvmError) as e:
+        c.checkpoint_load(path)
+    assert e.value.status == ERR_CONFIG

@@ -81,3 +81,28 @@ def test_vtk_values_follow_the_eos(tmp_path):
             k += 1
     # signed, at least two-digit exponents, as the reference's fmt_e writes them
     assert re.fullmatch(r"-?\d\.\d{10}e[+-]\d{2}", text.splitlines()[6]), text.splitlines()[6]
+
+
+def test_vtk_3d_grid_and_cell_order(tmp_path):
+    """efvm3_write_vtk: the same writer over an nx x ny x nz mesh; cells x fastest, then y, then z
+    (the VTK rectilinear-grid cell order and the wire order), with rho_w in the velocity."""
+    nx, ny, nz, gamma = 3, 2, 4, 1.4
+    st = np.zeros((nz, ny, nx, 5))
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                P = np.array([1.0 + 0.1 * i + 0.01 * j + 0.001 * k, 5.0 * i, -3.0 * j, 2.0 * k, 1e5, 300.0])
+                st[k, j, i] = orc.prim_to_cons(P, gamma)
+    path = str(tmp_path / "v3.vtk")
+    efvm.write_vtk3(path, st, nx, ny, nz, lo=(0.0, -1.0, 2.0), hi=(3.0, 1.0, 6.0), gamma=gamma)
+    text, b = parse_vtk(path)
+    assert "DIMENSIONS 4 3 5" in text and "CELL_DATA 24" in text
+    zs = re.search(r"Z_COORDINATES 5 double\n((?:.*\n){5})", text).group(1).split()
+    assert np.allclose([float(z) for z in zs], [2.0, 3.0, 4.0, 5.0, 6.0], atol=1e-14)
+    q = 0
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                assert np.isclose(b["density"][q, 0], 1.0 + 0.1 * i + 0.01 * j + 0.001 * k, rtol=1e-10)
+                assert np.allclose(b["velocity"][q], [5.0 * i, -3.0 * j, 2.0 * k], rtol=1e-9, atol=1e-12)
+                q += 1
