@@ -2,7 +2,9 @@
 
 The stage kernels are wave strips (64 columns x `strip_rows` rows per wave);
 the shape trades halo re-reads (4 extra rows per strip) against parallelism
-(strips per CU) and the tail of the last wave round.  The tuner times real
+(strips per CU) and the tail of the last wave round.  The FP32 kernel runs a
+strip's strip_rows + 1 row steps in groups of four, so the candidate heights
+are one less than a multiple of four (no idle steps).  The tuner times real
 steps of the caller's own context or batch with HIP events on its stream
 (library-side `efvm_last_elapsed_ms`), restores the state from a device
 snapshot afterwards, and caches the winner as JSON keyed by
@@ -18,9 +20,9 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE_PATH = os.environ.get("EFVM_TUNE_CACHE", os.path.join(HERE, "tuned.json"))
 
-CTX_ROWS = (8, 16, 24, 28, 32, 40, 48, 64, 128, 256)
+CTX_ROWS = (7, 15, 23, 27, 31, 39, 43, 47, 55, 63, 127, 255)
 CTX_WAVES = (1, 2, 4)
-BATCH_ROWS = (8, 16, 32, 64, 128)
+BATCH_ROWS = (7, 15, 31, 63, 127)
 BATCH_BLOCKS = (0, 512, 1024, 2048, 4096)
 
 

@@ -1,8 +1,10 @@
 """Build the HIP library ``libefvm.so`` in-tree for gfx950 with hipcc.
 
 The FP64 parity translation unit is compiled with ``-ffp-contract=off`` (the
-reference never fuses multiply-adds); the FP32 production unit allows
-contraction.  All HIP code is built with packed-FP32 instructions disabled.
+reference never fuses multiply-adds); the FP32 production unit fuses a
+multiply-add only inside one source expression (``-ffp-contract=on``), which the
+front end decides once per expression: the four compiled copies of the row step
+then round alike, and results do not depend on the strip height.  All HIP code is built with packed-FP32 instructions disabled.
 """
 from __future__ import annotations
 
@@ -22,7 +24,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-x", "hip", f"--offload-arch={ARCH}",
           "-Wno-unused-result", "-Wno-unused-command-line-argument"]
 UNITS = [
     ("efvm_stage_f64.hip", ["-ffp-contract=off"]),
-    ("efvm_stage_f32.hip", ["-ffp-contract=fast", "-fgpu-flush-denormals-to-zero"]),
+    ("efvm_stage_f32.hip", ["-ffp-contract=on", "-fgpu-flush-denormals-to-zero"]),
     ("efvm_mlp.hip", ["-ffp-contract=fast"]),
     ("efvm_batch.hip", ["-ffp-contract=fast"]),
     ("efvm_stage3d.hip", ["-ffp-contract=off"]),

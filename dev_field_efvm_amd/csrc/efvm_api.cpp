@@ -83,7 +83,7 @@ struct efvm_ctx {
     int rk_stage = 0;
     int parity = 0;
     int lam_valid = 0;
-    int strip_h = 32;
+    int strip_h = 31;   // rows per strip; the FP32 kernel runs row steps in groups of four (strip_h + 1 of them)
     int waves_per_block = 4;
     int xcd_remap = 1;
     int parabolic = 0;
@@ -918,7 +918,7 @@ efvm_status efvm_batch_create(int n_cases, const efvm_case* cases, const efvm_sc
     b->sch = *scheme;
     b->cases.assign(cases, cases + n_cases);
     b->hc.resize(n_cases);
-    b->strip_rows.assign(n_cases, 32);
+    b->strip_rows.assign(n_cases, 31);
     b->offset.resize(n_cases);
     long tot = 0;
     for (int i = 0; i < n_cases; i++) {

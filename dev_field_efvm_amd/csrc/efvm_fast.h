@@ -100,18 +100,17 @@ __host__ __device__ __forceinline__ float pick(bool c, float a, float b) {
 template <int DIR, bool THORNBER>
 __host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R) {
     float ul = L.u, vl = L.v, ur = R.u, vr = R.v;
-    float el = L.re, er = R.re;
+    float ql = 0.f, qr = 0.f;
     if (THORNBER) {
-        float ql = ul * ul + vl * vl, qr = ur * ur + vr * vr;
+        ql = ul * ul + vl * vl; qr = ur * ur + vr * vr;
         const float vm = fsqrt(fmaxf(ql, qr));
-        const float z = fminf(fminf(vm * L.ia, vm * R.ia), 1.f);
+        // min(vm / a_L, vm / a_R): vm >= 0 and rounding is monotone, so scaling the
+        // smaller 1/a gives the same value as the smaller of the two products
+        const float z = fminf(vm * fminf(L.ia, R.ia), 1.f);
         // avg +/- z (v_L - v_R)/2  ==  v_L - w dv,  v_R + w dv  with w = (1 - z)/2, dv = v_L - v_R
         const float w = fmaf(-0.5f, z, 0.5f);
         const float du = ul - ur, dv = vl - vr;
         ul = fmaf(-w, du, ul); ur = fmaf(w, du, ur); vl = fmaf(-w, dv, vl); vr = fmaf(w, dv, vr);
-        // rebuilt energies keep the internal energy (and p, a) unchanged
-        el = L.re + 0.5f * L.r * ((ul * ul + vl * vl) - ql);
-        er = R.re + 0.5f * R.r * ((ur * ur + vr * vr) - qr);
     }
     const float unl = DIR == 0 ? ul : vl, unr = DIR == 0 ? ur : vr;
     const float sl = fminf(unl - L.a, unr - R.a);
@@ -125,7 +124,11 @@ __host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R)
     const float Dn = fl - fr;
     const bool left = N <= 0.f;
     const float rK = pick(left, L.r, R.r), uK = pick(left, ul, ur), vK = pick(left, vl, vr);
-    const float pK = pick(left, L.p, R.p), eK = pick(left, el, er), unK = pick(left, unl, unr), sK = pick(left, sl, sr);
+    const float pK = pick(left, L.p, R.p), unK = pick(left, unl, unr), sK = pick(left, sl, sr);
+    // energy of side K; with Thornber's velocities it is rebuilt so that the internal
+    // energy (and p, a) are unchanged -- for the upwind side only, the one the flux uses
+    float eK = pick(left, L.re, R.re);
+    if (THORNBER) eK = eK + 0.5f * rK * ((uK * uK + vK * vK) - pick(left, ql, qr));
     const float fK = pick(left, fl, fr);            // rho_K (S_K - u_K)
     const float X = sK * Dn - N;
     const float rr = rcp(Dn * X);

@@ -1,4 +1,4 @@
-// efvm_stage_f32.hip -- FP32 production stage kernel (contraction allowed).
+// efvm_stage_f32.hip -- FP32 production stage kernel (multiply-adds fused within one expression).
 //
 // Same wave-strip decomposition and boundary semantics as the FP64 parity
 // kernel (efvm_stage_f64.hip); the numerics are the 2-D schemes of efvm_fast.h.
@@ -64,8 +64,13 @@ __device__ __forceinline__ F4 reflect(int kind, int axis, F4 v) {
 // periodic image or, for a ghost lane, the owner column); `xkind` is -1 for a
 // real column, else the boundary kind of the x side the lane is a ghost of
 // (selected once per lane, so no per-lane indexing of the argument block).
+// A row that is not `live` is read through a zero-length descriptor: the loads are
+// still issued (and return 0) so that every row step has the same memory
+// operations on every path, which lets the compiler count outstanding loads
+// exactly instead of waiting for all of them at the first use.
 template <int FEAT>
-__device__ __forceinline__ F4 load_row(const StageArgs& a, const float* const* U, int c, int cq, int xkind, int r) {
+__device__ __forceinline__ F4 load_row(const StageArgs& a, const float* const* U, int c, int cq, int xkind, int r,
+                                       bool live = true) {
     int rr = r, ykind = -1;
     if (r < 0 || r >= a.ny) {
         if (a.per_y) { rr = r % a.ny; if (rr < 0) rr += a.ny; }
@@ -76,7 +81,7 @@ __device__ __forceinline__ F4 load_row(const StageArgs& a, const float* const* U
         return {cl.r, cl.ru, cl.rv, cl.re};
     }
     const long rb = (long)rr * a.nx;
-    const unsigned off = (unsigned)cq * 4u, nb = (unsigned)a.nx * 4u;
+    const unsigned off = (unsigned)cq * 4u, nb = live ? (unsigned)a.nx * 4u : 0u;
     F4 v = {row_ld(U[0] + rb, nb, off), row_ld(U[1] + rb, nb, off), row_ld(U[2] + rb, nb, off), row_ld(U[3] + rb, nb, off)};
     if (ykind >= 0) v = reflect(ykind, 1, v);
     if (xkind >= 0) v = reflect(xkind, 0, v);
@@ -218,16 +223,20 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
                     F4& gin, F4& gout, F4& xin, F4& xout, float* vin, float* vout, int& s_m, int s_c, int s_p,
                     const F4& u0in, F4& u0out) {
         const bool full = jc < j1;
-        const bool out = jc > j0;
+        const bool out = jc > j0 && jc <= j1;   // a step past j1 writes nothing
         const long rb = (long)(jc - 1) * nx;    // wave-uniform row base
         const unsigned coff = (unsigned)c * 4u;  // meaningful on output lanes only
         const unsigned rowb = (unsigned)nx * 4u;
         const F4 u0 = u0in;
-        // U0 of row jc (the next step's output row): loaded a whole row step ahead of its use
-        if (full && ca != 0.f && out_lane) {
+        // U0 of row jc (the next step's output row): loaded a whole row step ahead of its use.
+        // The loads are issued on every step and lane (a zero-length descriptor when U0 is not
+        // needed; a lane outside the mesh reads 0 past the row), so that the memory operations
+        // of a step do not depend on the path and the loads in flight are counted exactly.
+        {
             const long rn = (long)jc * nx;
-            u0out = {row_ld(U0[0] + rn, rowb, coff), row_ld(U0[1] + rn, rowb, coff), row_ld(U0[2] + rn, rowb, coff),
-                     row_ld(U0[3] + rn, rowb, coff)};
+            const unsigned u0b = (full && ca != 0.f) ? rowb : 0u;
+            u0out = {row_ld(U0[0] + rn, u0b, coff), row_ld(U0[1] + rn, u0b, coff), row_ld(U0[2] + rn, u0b, coff),
+                     row_ld(U0[3] + rn, u0b, coff)};
         }
         dout = sub4(up, uc);
         Prep lo_p;
@@ -273,7 +282,11 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
                 }
             }
         }
-        if (out && out_lane) {
+        // The new row jc-1: computed on every step and lane and stored through a descriptor
+        // that drops what is not an output (zero length on the first step, an offset past
+        // the row on halo lanes), so the stores too are the same on every path.
+        {
+            const bool live = out && out_lane;
             const long q = rb + c;
             float rr = xin.r * dtix + (gin.r - gout.r) * dtiy;
             float rru = xin.ru * dtix + (gin.ru - gout.ru) * dtiy;
@@ -281,7 +294,7 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
             float rre = xin.re * dtix + (gin.re - gout.re) * dtiy;
             if (VISC) { rru += dt * vin[0]; rrv += dt * vin[1]; rre += dt * vin[2]; }
             if ((FEAT & FEAT_BODY) && a.body_force) { rru += um.r * gxdt; rre += gxdt * um.ru; }
-            if (IBM && a.solid[q]) { rr = 0.f; rru = 0.f; rrv = 0.f; rre = 0.f; }
+            if (IBM && live && a.solid[q]) { rr = 0.f; rru = 0.f; rrv = 0.f; rre = 0.f; }
             float o0, o1, o2, o3;
             if (ca != 0.f) {
                 // ca u0 + cb (u + dt L) written as u0 + cb ((u - u0) + dt L): the state keeps the
@@ -292,17 +305,19 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
             } else {
                 o0 = um.r + rr; o1 = um.ru + rru; o2 = um.rv + rrv; o3 = um.re + rre;
             }
-            row_st(O[0] + rb, rowb, coff, o0); row_st(O[1] + rb, rowb, coff, o1); row_st(O[2] + rb, rowb, coff, o2);
-            row_st(O[3] + rb, rowb, coff, o3);
-            if (fuse) {
+            const unsigned sb = out ? rowb : 0u, so = out_lane ? coff : 0x80000000u;
+            row_st(O[0] + rb, sb, so, o0); row_st(O[1] + rb, sb, so, o1); row_st(O[2] + rb, sb, so, o2);
+            row_st(O[3] + rb, sb, so, o3);
+            if (fuse && live) {
                 float l;
                 if (lam_cell_f32<VISC>(a, o0, o1, o2, o3, l)) lam_max = fmaxf(lam_max, l);
                 else bad++;
             }
         }
-        if (full && jc + 3 <= j1 + 1) {
-            um = load_row<FEAT>(a, U, c, cq, xkind, jc + 3);
-            if (IBM) s_m = solid_at(jc + 3);
+        {
+            const bool more = full && jc + 3 <= j1 + 1;
+            um = load_row<FEAT>(a, U, c, cq, xkind, jc + 3, more);
+            if (IBM && more) s_m = solid_at(jc + 3);
         }
     };
 
@@ -324,14 +339,21 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
         Prep lo_unused;
         y_states(j0 - 1, R0, D0, sub4(R0, rm2), lo_unused, H0);
     }
+    // Drain the prologue's loads (vmcnt(0); expcnt and lgkmcnt left at their maxima).
+    // The loop head is reached from here and from its back edge; with nothing left
+    // in flight from the prologue, the waits inside the loop are the back edge's,
+    // which keep two row steps of loads in flight.
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     // ---- steady state: rows j0 .. j1, unrolled x4 with rotated roles ----
+    // The loop runs whole groups of four row steps and has no exit but its condition,
+    // so the waits for the rows loaded two steps ahead are counted from the back edge
+    // alone.  Steps past j1 (up to three, none when strip_rows + 1 is a multiple of 4)
+    // store, load and accumulate nothing.  Every row goes through the same four
+    // compiled steps, whatever the strip height, so results do not depend on it.
     for (int jc = j0; jc <= j1; jc += 4) {
         step(jc, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, V0, V1, S0, S1, S2, Q0, Q1);
-        if (jc + 1 > j1) break;
         step(jc + 1, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, V1, V0, S1, S2, S3, Q1, Q0);
-        if (jc + 2 > j1) break;
         step(jc + 2, R2, R3, R0, D0, D1, H0, H1, G0, G1, X0, X1, V0, V1, S2, S3, S0, Q0, Q1);
-        if (jc + 3 > j1) break;
         step(jc + 3, R3, R0, R1, D1, D0, H1, H0, G1, G0, X1, X0, V1, V0, S3, S0, S1, Q1, Q0);
     }
 
