@@ -249,7 +249,9 @@ class FnBoundary(BoundaryCondition):
 
 class BoundaryInputs:
     """crates/cfd_boundary/src/lib.rs:108-111: supplied ghost states, here per side as
-    (states (n, 5), given (n,)), n = ny for x sides and nx for y sides."""
+    (states (n, 5), given (n,)), one per boundary face: n = ny for x sides and nx for y
+    sides of a 2-D mesh; on a 3-D mesh ny nz (j + ny k), nx nz (i + nx k) and nx ny
+    (i + nx j) for x, y and z sides."""
 
     def __init__(self):
         self.sides = {}
@@ -501,9 +503,6 @@ class App:
     def _prepare3(self, m, gasp, flux, integ, solver, reg, gpu, visc):
         """The 3-D context: six sides, bodies, the uniform sponge, body force."""
         cfg = solver.config
-        for s in range(6):
-            if isinstance(reg.get(BoundarySide(s)), (FromOutside, FnBoundary)):
-                raise EfvmError(ERR_UNSUPPORTED, "FromOutside / FnBoundary tables are 2-D only")
         if ParabolicInflow in self.resources or OutletSponge in self.resources:
             raise EfvmError(ERR_UNSUPPORTED, "the parabolic channel inflow and sponge are 2-D cases")
         self.ctx = Context3(m.nx, m.ny, m.nz, lo=tuple(m.bounds_lo), hi=tuple(m.bounds_hi), gamma=gasp.gas.gamma,
@@ -511,6 +510,24 @@ class App:
                             limiter=flux.limiter.value, integrator=integ.kind, precision=gpu.precision,
                             viscous=cfg.viscous, seam_mode=gpu.seam_mode, cfl=cfg.cfl, fixed_dt=cfg.fixed_dt)
         self.ctx.set_boundaries([(reg.get(BoundarySide(s)).kind, reg.get(BoundarySide(s)).params) for s in range(6)])
+        # ghost tables: FnBoundary from the face centroids, FromOutside from BoundaryInputs
+        lo3, hi3, n3 = tuple(m.bounds_lo), tuple(m.bounds_hi), (m.nx, m.ny, m.nz)
+        for s in range(6):
+            bc = reg.get(BoundarySide(s))
+            if isinstance(bc, FnBoundary):
+                ax = s // 2
+                u, w = (1, 2) if ax == 0 else ((0, 2) if ax == 1 else (0, 1))   # in-plane axes, faster first
+                cu = [lo3[u] + (q + 0.5) * (hi3[u] - lo3[u]) / n3[u] for q in range(n3[u])]
+                cw = [lo3[w] + (q + 0.5) * (hi3[w] - lo3[w]) / n3[w] for q in range(n3[w])]
+                fc = []
+                for b in cw:
+                    for a_ in cu:
+                        c = [0.0, 0.0, 0.0]
+                        c[ax] = lo3[ax] if s % 2 == 0 else hi3[ax]
+                        c[u], c[w] = a_, b
+                        fc.append(tuple(c))
+                self.ctx.set_ghost_table(s, np.array([bc.fn(c) for c in fc]), np.ones(len(fc), bool))
+        self.apply_boundary_inputs()
         sp = self.resources.get(Sponge)
         if sp:
             self.ctx.set_sponge(sp.x_start, sp.x_end, sp.sigma, sp.target)

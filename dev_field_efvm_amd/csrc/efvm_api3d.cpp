@@ -53,6 +53,7 @@ struct efvm_ctx3 {
     double* mforce = nullptr;     // per-marker force contributions (6 per marker)
     int n_mark = 0;
     double* stagnant = nullptr;
+    double* bc_tab[6] = {};       // FromOutside ghost tables (6 doubles per face), allocated on first use
     int ibm_flipped = 1;
     int body_kind = 0;            // 0 none, 1 cylinders, 2 spheres
     int n_body = 0;
@@ -117,6 +118,7 @@ Args3 args3(const efvm_ctx3* c) {
     a.yw = c->yw; a.zw = c->zw;
     a.body_force = c->body_force; a.gx = c->gx;
     a.solid = c->n_body > 0 ? c->solid : nullptr;
+    for (int s = 0; s < 6; s++) a.tab[s] = c->bc[s].kind == EFVM_BC_TABLE ? c->bc_tab[s] : nullptr;
     a.euler = c->sch.integrator == EFVM_INTEGRATOR_EULER;
     a.cfl = c->sch.cfl; a.fixed_dt = c->sch.fixed_dt;
     a.clock = c->clock;
@@ -397,6 +399,7 @@ void efvm3_destroy(efvm_ctx3* c) {
     if (c->out) (void)hipFree(c->out);
     if (c->yw) (void)hipFree(c->yw);
     if (c->zw) (void)hipFree(c->zw);
+    for (int s = 0; s < 6; s++) if (c->bc_tab[s]) (void)hipFree(c->bc_tab[s]);
     free_ibm3(c);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -410,7 +413,6 @@ efvm_status efvm3_set_boundaries(efvm_ctx3* c, const efvm_bc sides[6]) {
     if (!c || !sides) return EFVM_ERR_ARG;
     for (int s = 0; s < 6; s++) {
         if (sides[s].kind < 0 || sides[s].kind > EFVM_BC_TABLE) return fail3(c, EFVM_ERR_CONFIG, "unknown boundary kind");
-        if (sides[s].kind == EFVM_BC_TABLE) return fail3(c, EFVM_ERR_UNSUPPORTED, "ghost tables are 2-D only");
     }
     for (int ax = 0; ax < 3; ax++)
         if ((sides[2 * ax].kind == EFVM_BC_PERIODIC) != (sides[2 * ax + 1].kind == EFVM_BC_PERIODIC))
@@ -418,6 +420,29 @@ efvm_status efvm3_set_boundaries(efvm_ctx3* c, const efvm_bc sides[6]) {
                                                  " requires both sides periodic (crates/cfd_boundary/src/lib.rs:437-448)");
     for (int s = 0; s < 6; s++) c->bc[s] = sides[s];
     drop_graph3(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm3_set_ghost_table(efvm_ctx3* c, int side, const double* cons5, const unsigned char* given, size_t n) {
+    if (!c || !cons5 || !given || side < 0 || side > 5) return EFVM_ERR_ARG;
+    if (c->bc[side].kind != EFVM_BC_TABLE) return fail3(c, EFVM_ERR_CONFIG, "ghost table on a side whose kind is not EFVM_BC_TABLE");
+    const int axis = side / 2;
+    const size_t nf = axis == 0 ? (size_t)c->grid.ny * c->grid.nz
+                                : (axis == 1 ? (size_t)c->grid.nx * c->grid.nz : (size_t)c->grid.nx * c->grid.ny);
+    if (n != nf)
+        return fail3(c, EFVM_ERR_ARG, "ghost table: one entry per boundary face (ny nz for x sides, nx nz for y sides, nx ny for z sides)");
+    std::vector<double> h(6 * nf);
+    for (size_t k = 0; k < nf; k++) {
+        for (int m = 0; m < 5; m++) h[6 * k + m] = cons5[5 * k + m];
+        h[6 * k + 5] = given[k] ? 1.0 : 0.0;
+    }
+    if (!c->bc_tab[side]) {
+        efvm_status st = alloc3(c, (void**)&c->bc_tab[side], sizeof(double) * 6 * nf);
+        if (st != EFVM_OK) return st;
+        drop_graph3(c);   // a captured graph holds arguments without this table
+    }
+    HIPCHK3(c, hipMemcpyAsync(c->bc_tab[side], h.data(), sizeof(double) * 6 * nf, hipMemcpyHostToDevice, c->stream));
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;
 }
 

@@ -18,6 +18,7 @@ struct Args3 {
     int bc_kind[6];        // XLo XHi YLo YHi ZLo ZHi
     double bc_par[6][5];
     double bc_cons[6][5];  // SupersonicInflow ghost state (prim_to_cons of its parameters)
+    const double* tab[6];  // FromOutside ghost tables: per face (rho, rho_u, rho_v, rho_w, rho_E, given), or null
     double gamma, r_gas;
     int flux, muscl, limiter;
     int viscous, visc_const;

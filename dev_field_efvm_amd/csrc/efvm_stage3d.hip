@@ -128,6 +128,17 @@ __global__ __launch_bounds__(256) void bc3_kernel(Args3 a) {
     const long qo = own[0] + (long)a.NX * (own[1] + (long)a.NY * own[2]);
     const long qg = gh[0] + (long)a.NX * (gh[1] + (long)a.NY * gh[2]);
     const V5<double> in = to_d(ld5((const T* const*)a.uin, qo));
+    if (a.bc_kind[side] == BC_TABLE) {
+        // FromOutside (crates/cfd_boundary/src/lib.rs:135-144): the supplied state where one
+        // is given, else the interior copy.  Table index over the side, the lower-numbered
+        // in-plane axis fastest: x side j + ny k, y side i + nx k, z side i + nx j.
+        const int i = own[0] - NGH, j = own[1] - NGH, k = own[2] - NGH;
+        const long t = axis == 0 ? j + (long)a.ny * k : (axis == 1 ? i + (long)a.nx * k : i + (long)a.nx * j);
+        const double* e = a.tab[side] ? a.tab[side] + 6 * t : nullptr;
+        const V5<double> o = (e && e[5] != 0.0) ? V5<double>{e[0], e[1], e[2], e[3], e[4]} : in;
+        st5((T* const*)a.uin, qg, from_d<T>(o));
+        return;
+    }
     st5((T* const*)a.uin, qg, from_d<T>(bc_ghost5(a, side, in)));
 }
 

@@ -603,7 +603,7 @@ EXPORTS += [
     "efvm3_set_clock", "efvm3_cfl", "efvm3_run", "efvm3_step", "efvm3_synchronize", "efvm3_graph_capture",
     "efvm3_graph_launch", "efvm3_ibm_add_sphere", "efvm3_ibm_add_cylinder", "efvm3_marker_force",
     "efvm3_conservation_sums", "efvm3_last_elapsed_ms", "efvm3_checkpoint_save", "efvm3_checkpoint_load",
-    "efvm3_write_vtk",
+    "efvm3_write_vtk", "efvm3_set_ghost_table",
 ]
 
 
@@ -614,6 +614,7 @@ def _bind3(L):
     sig = {
         "efvm3_create": [C.POINTER(Grid3), C.POINTER(Gas), C.POINTER(Scheme), C.POINTER(vp)],
         "efvm3_set_boundaries": [vp, C.POINTER(Bc)],
+        "efvm3_set_ghost_table": [vp, C.c_int, pd, C.POINTER(C.c_ubyte), C.c_size_t],
         "efvm3_set_time_step": [vp, d, d],
         "efvm3_set_body_force": [vp, d],
         "efvm3_set_sponge": [vp, d, d, d, pd],
@@ -700,6 +701,18 @@ class Context3:
             for q, v in enumerate(par):
                 arr[s].par[q] = v
         self._chk(self._L.efvm3_set_boundaries(self._p, arr))
+
+    def set_ghost_table(self, side: int, states, given):
+        """BoundaryInputs of a FromOutside side (kind "table"), side 0..5 = XLo .. ZHi: states
+        (n, 5) in the wire order and given (n,) bools, one per boundary face with the
+        lower-numbered in-plane axis fastest (x side j + ny k, y side i + nx k, z side
+        i + nx j); faces not given copy the interior cell."""
+        st = np.ascontiguousarray(np.asarray(states, dtype=np.float64).reshape(-1, 5))
+        g = np.ascontiguousarray(np.asarray(given, dtype=np.uint8).reshape(-1))
+        if g.shape[0] != st.shape[0]:
+            raise ValueError("states and given differ in length")
+        self._chk(self._L.efvm3_set_ghost_table(self._p, int(side), _pd(st), g.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                                st.shape[0]))
 
     def set_time_step(self, cfl=0.4, fixed_dt=None):
         self._chk(self._L.efvm3_set_time_step(self._p, cfl, -1.0 if fixed_dt is None else float(fixed_dt)))

@@ -291,7 +291,7 @@ efvm_status efvm_learned_faces(efvm_ctx* ctx, float* out, size_t n);
  * plugins as efvm_ctx, six boundary sides, five stored components (rho_w included),
  * and the sphere body of crates/cfd_ibm/src/sphere.rs.  FP64 reproduces the reference's
  * operation order; FP32 runs the same operations in single precision.  Reconstruction
- * is first order or MUSCL; the learned reconstruction and ghost tables are 2-D only.
+ * is first order or MUSCL; the learned reconstruction is 2-D only.
  * ------------------------------------------------------------------------- */
 typedef struct { int nx, ny, nz; double lo[3], hi[3]; } efvm_grid3;
 typedef struct efvm_ctx3 efvm_ctx3;
@@ -303,8 +303,15 @@ efvm_status efvm3_create(const efvm_grid3* grid, const efvm_gas* gas, const efvm
 void efvm3_destroy(efvm_ctx3* ctx);
 const char* efvm3_last_error(const efvm_ctx3* ctx);
 /* BoundaryRegistry (crates/cfd_boundary/src/lib.rs:369-449): sides[6] = XLo, XHi, YLo, YHi,
- * ZLo, ZHi; a half-periodic axis is EFVM_ERR_CONFIG, EFVM_BC_TABLE is EFVM_ERR_UNSUPPORTED. */
+ * ZLo, ZHi; a half-periodic axis is EFVM_ERR_CONFIG. */
 efvm_status efvm3_set_boundaries(efvm_ctx3* ctx, const efvm_bc sides[6]);
+/* BoundaryInputs (crates/cfd_boundary/src/lib.rs:106-111) of a FromOutside side (kind
+ * EFVM_BC_TABLE, lib.rs:115-147): cons5[5 n] ghost states in Component wire order and
+ * given[n]; a face with given = 0 copies its interior cell (SupersonicOutflow).  One entry
+ * per boundary face, the lower-numbered in-plane axis fastest: n = ny nz for an x side
+ * (j + ny k), nx nz for a y side (i + nx k), nx ny for a z side (i + nx j).  May be called
+ * again between steps; a captured graph reads the table in place once it exists. */
+efvm_status efvm3_set_ghost_table(efvm_ctx3* ctx, int side, const double* cons5, const unsigned char* given, size_t n);
 /* SolverConfig {cfl, fixed_dt} (crates/cfd_solver/src/lib.rs:39-58), at a step boundary. */
 efvm_status efvm3_set_time_step(efvm_ctx3* ctx, double cfl, double fixed_dt);
 /* Uniform x body force (crates/cfd_solver/tests/poiseuille_channel_validation.rs:105-120). */

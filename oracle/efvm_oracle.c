@@ -520,8 +520,10 @@ static void boundary_system(orc_sim *s) {                           /* cfd_bound
             gh[axis] = hi ? NG + n[axis] : NG - 1;
             cv_t in = s->u[IDX(s, own[0], own[1], own[2])];
             if (s->c.bc[side] == BC_TABLE) {
-                /* the face index along the side: the row of an x side, the column of a y side */
-                long f = axis == 0 ? q1 : (axis == 1 ? q2 : 0);
+                /* the face index over the side, the lower-numbered in-plane axis fastest:
+                 * x side j + ny k, y side i + nx k, z side i + nx j (nz = 1: the row of an
+                 * x side, the column of a y side) */
+                long f = axis == 0 ? q1 + (long)ny * q2 : (axis == 1 ? q2 + (long)nx * q1 : q1 + (long)nx * q2);
                 const double *t = (s->tab[side] && f < s->tab_n[side]) ? s->tab[side] + 6 * f : NULL;
                 if (t && t[5] != 0.0) { cv_t g = {t[0], t[1], t[2], t[3], t[4]}; s->u[IDX(s, gh[0], gh[1], gh[2])] = g; }
                 else s->u[IDX(s, gh[0], gh[1], gh[2])] = in;

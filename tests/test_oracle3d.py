@@ -162,3 +162,66 @@ def test_sphere_flow_keeps_the_mirror_symmetries():
     f = s.marker_force()
     fx, fy, fz = f[0] + f[3], f[1] + f[4], f[2] + f[5]
     assert fx > 0 and abs(fy) < 0.05 * fx and abs(fz) < 0.05 * fx
+
+
+# ------------------------------------------------ FromOutside ghost tables in 3-D
+def _table_plane(n, seed):
+    """Per-face outside states of a 2-D side (n faces) and their given flags."""
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 0.2 * rng.random(n)
+    u, v, p = 0.3 * rng.standard_normal(n), 0.3 * rng.standard_normal(n), 1.0 + 0.2 * rng.random(n)
+    st = np.stack([rho, rho * u, rho * v, np.zeros(n), p / 0.4 + 0.5 * rho * (u * u + v * v)], axis=-1)
+    return st, rng.random(n) < 0.6
+
+
+def test_ghost_tables_index_the_faces_of_a_3d_side():
+    """FromOutside tables on an x and a y side of a 2-D run (the nz = 1 layout, pinned by the
+    reference's boundary-face tests in test_oracle.py) equal, layer by layer, the run
+    extruded along z with each table repeated per layer (x side j + ny k, y side i + nx k),
+    and the run laid in the x-z plane with the y-side table moved to the z side (i + nx j)."""
+    n, h, steps, layers = 12, 1.0 / 12, 4, 3
+    V = vortex_plane(n)
+    tx, gx = _table_plane(n, 1)          # x-lo, one per row
+    ty, gy = _table_plane(n, 2)          # y-hi, one per column
+    kw = dict(flux="hllc", muscl=True, limiter="minmod", cfl=0.3)
+    base_sim = orc.Sim(n, n, 1, hi=(L2, L2, h), bcs=["table", "outflow", "slip", "table", "slip", "slip"], **kw)
+    base_sim.set_ghost_table(0, tx, gx)
+    base_sim.set_ghost_table(3, ty, gy)
+    base_sim.set_interior(V[None])
+    base_sim.step(steps)
+    base = base_sim.interior()[0]
+    scale = np.abs(base).max()
+    # extruded along z (periodic z): tables repeated per layer
+    ext_sim = orc.Sim(n, n, layers, hi=(L2, L2, layers * h),
+                      bcs=["table", "outflow", "slip", "table", "periodic", "periodic"], **kw)
+    ext_sim.set_ghost_table(0, np.tile(tx, (layers, 1)), np.tile(gx, layers))
+    ext_sim.set_ghost_table(3, np.tile(ty, (layers, 1)), np.tile(gy, layers))
+    ext_sim.set_interior(np.repeat(V[None], layers, axis=0))
+    ext_sim.step(steps)
+    ext = ext_sim.interior()
+    for k in range(layers):
+        assert np.abs(ext[k] - base).max() < 1e-12 * scale
+    # x-z plane: 2-D (x, y, u, v) -> 3-D (x, z, u, w), uniform (periodic) along y
+    def perm(st):
+        return st[:, [0, 1, 3, 2, 4]]
+    S = np.zeros((n, layers, n, 5))
+    for j in range(layers):
+        S[:, j, :, :] = V[:, :, [0, 1, 3, 2, 4]]
+    xz_sim = orc.Sim(n, layers, n, hi=(L2, layers * h, L2),
+                     bcs=["table", "outflow", "periodic", "periodic", "slip", "table"], **kw)
+    # x side: index j + ny k with k = the 2-D row; z side: index i + nx j
+    xz_sim.set_ghost_table(0, np.repeat(perm(tx), layers, axis=0), np.repeat(gx, layers))
+    xz_sim.set_ghost_table(5, np.tile(perm(ty), (layers, 1)), np.tile(gy, layers))
+    xz_sim.set_interior(S)
+    xz_sim.step(steps)
+    xz = xz_sim.interior()
+    for j in range(layers):
+        back = xz[:, j, :, :][..., [0, 1, 3, 2, 4]]
+        assert np.abs(back - base).max() < 1e-12 * scale
+    # a transposed table (rows and layers swapped) is caught by the same comparison
+    bad = orc.Sim(n, layers, n, hi=(L2, layers * h, L2), bcs=["table", "outflow", "periodic", "periodic", "slip", "table"], **kw)
+    bad.set_ghost_table(0, np.tile(perm(tx), (layers, 1)), np.tile(gx, layers))
+    bad.set_ghost_table(5, np.tile(perm(ty), (layers, 1)), np.tile(gy, layers))
+    bad.set_interior(S)
+    bad.step(steps)
+    assert np.abs(bad.interior()[:, 0, :, :][..., [0, 1, 3, 2, 4]] - base).max() > 1e-6 * scale
