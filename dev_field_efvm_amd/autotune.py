@@ -106,7 +106,7 @@ def tune_batch(batch, sides, steps: int = 2, rows=BATCH_ROWS, blocks=BATCH_BLOCK
         batch.set_grid(db[key]["blocks"], 4)
         return choice, db[key]["blocks"], db[key]["ms"]
     batch.snapshot()
-    choice = {c: 32 for c in classes}
+    choice = {c: 31 for c in classes}
     _apply_batch(batch, sides, choice)
     nb = 0
     batch.set_grid(nb, 4)
