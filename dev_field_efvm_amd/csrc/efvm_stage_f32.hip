@@ -189,6 +189,9 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
     // where the reference keeps that side first order; fma(0, s, u) == u for finite s
     const float hW = xW_rec ? -1.f : 0.f, hE = xE_rec ? 1.f : 0.f;
     const bool out_lane = lane >= 2 && lane < WAVE - 2 && c < nx;
+    // U0 byte offset of the lane's column: a lane left of the mesh (c < 0) gets an offset
+    // far past any row (as the halo stores do), one right of it is past the row already
+    const unsigned u0off = c >= 0 ? (unsigned)c * 4u : 0x80000000u;
     auto solid_at = [&](int rr) -> int {
         if (!IBM || col_ghost || rr < 0 || rr >= ny) return 0;
         return a.solid[(long)rr * nx + c];
@@ -230,13 +233,13 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
         const F4 u0 = u0in;
         // U0 of row jc (the next step's output row): loaded a whole row step ahead of its use.
         // The loads are issued on every step and lane (a zero-length descriptor when U0 is not
-        // needed; a lane outside the mesh reads 0 past the row), so that the memory operations
+        // needed; a lane outside the mesh reads 0 from past the row), so that the memory operations
         // of a step do not depend on the path and the loads in flight are counted exactly.
         {
             const long rn = (long)jc * nx;
             const unsigned u0b = (full && ca != 0.f) ? rowb : 0u;
-            u0out = {row_ld(U0[0] + rn, u0b, coff), row_ld(U0[1] + rn, u0b, coff), row_ld(U0[2] + rn, u0b, coff),
-                     row_ld(U0[3] + rn, u0b, coff)};
+            u0out = {row_ld(U0[0] + rn, u0b, u0off), row_ld(U0[1] + rn, u0b, u0off), row_ld(U0[2] + rn, u0b, u0off),
+                     row_ld(U0[3] + rn, u0b, u0off)};
         }
         dout = sub4(up, uc);
         Prep lo_p;
