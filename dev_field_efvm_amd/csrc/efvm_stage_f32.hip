@@ -351,8 +351,10 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
     // The loop runs whole groups of four row steps and has no exit but its condition,
     // so the waits for the rows loaded two steps ahead are counted from the back edge
     // alone.  Steps past j1 (up to three, none when strip_rows + 1 is a multiple of 4)
-    // store, load and accumulate nothing.  Every row goes through the same four
-    // compiled steps, whatever the strip height, so results do not depend on it.
+    // store, load and accumulate nothing.  Which of the four compiled steps takes a row
+    // depends on the strip height; the unit fuses multiply-adds only within one source
+    // expression (-ffp-contract=on), so the four copies round alike and results do not
+    // depend on the height.
     for (int jc = j0; jc <= j1; jc += 4) {
         step(jc, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, V0, V1, S0, S1, S2, Q0, Q1);
         step(jc + 1, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, V1, V0, S1, S2, S3, Q1, Q0);
