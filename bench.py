@@ -70,9 +70,9 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> float:
     return round(2 * nbytes / (best * 1e-3) / 1e9, 1)
 
 
-def load_traffic(n: int, prec: str):
+def load_traffic(n: int, prec: str, name: str = None):
     """HBM bytes per stage launch from the committed rocprofv3 PMC summary, if present."""
-    path = os.path.join(ROOT, "profiles", f"stage_{prec}_{n}_pmc.json")
+    path = os.path.join(ROOT, "profiles", name or f"stage_{prec}_{n}_pmc.json")
     if not os.path.exists(path):
         return None
     try:
@@ -126,7 +126,9 @@ def bench_batch(args):
                    "sides": [min(sides), max(sides)], "tuned_rows": {str(k): v for k, v in rows.items()},
                    "tuned_blocks": blocks, "tune_s": round(t_tune, 2), "sim_time_case0_1": times},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": "stage_f32_batch_kernel",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": load_traffic(len(sides), "fp32", f"batch_fp32_{len(sides)}_pmc.json"),
+                     "kernel": "stage_f32_batch_kernel",
                      "mean_launch_us": round(mean_launch_s * 1e6, 2), "algorithmic_bytes_per_launch": bytes_per_launch},
     }
 
