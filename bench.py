@@ -84,8 +84,8 @@ def load_traffic(n: int, prec: str, name: str = None):
 def bench_batch(args):
     """Batched ensemble: 512 independent 2-D Euler cases (64^2..1024^2 log-uniform, gamma in
     [1.3, 1.67], periodic vortices and four-quadrant Riemann problems), MUSCL-HLLC-RK3 FP32,
-    per-case dt, one persistent launch per stage, 100 steps replayed from a hipGraph, launch
-    shapes from the autotuner."""
+    per-case dt, one launch per stage with a wave per (case, strip), 100 steps replayed from a
+    hipGraph, launch shapes from the autotuner."""
     import torch
     import dev_field_efvm_amd as efvm
     from dev_field_efvm_amd import cases
@@ -122,7 +122,7 @@ def bench_batch(args):
         "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded vortices and four-quadrant Riemann problems, host FP64)",
         "config": {"workload": f"batched ensemble of {args.cases} 2-D Euler cases, MUSCL(MC)+HLLC+SSP-RK3, per-case dt, "
-                               "persistent launch per stage, hipGraph", "total_cells": total,
+                               "one launch per stage (a wave per case strip), hipGraph", "total_cells": total,
                    "sides": [min(sides), max(sides)], "tuned_rows": {str(k): v for k, v in rows.items()},
                    "tuned_blocks": blocks, "tune_s": round(t_tune, 2), "sim_time_case0_1": times},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -23,7 +23,7 @@ CACHE_PATH = os.environ.get("EFVM_TUNE_CACHE", os.path.join(HERE, "tuned.json"))
 CTX_ROWS = (7, 15, 23, 27, 31, 39, 43, 47, 55, 63, 127, 255)
 CTX_WAVES = (1, 2, 4)
 BATCH_ROWS = (7, 15, 31, 63, 127)
-BATCH_BLOCKS = (0, 512, 1024, 2048, 4096)
+BATCH_BLOCKS = (0, 2048, 4096, 8192)
 
 
 def arch_name() -> str:

@@ -140,7 +140,7 @@ __device__ __forceinline__ Prim3 prim3(const F4& q, float gm1, float ir_gas) {
 }
 
 // One wave processes strip `s` of the case `a`; planes and stage/parity are passed
-// separately so the persistent batched kernel can reuse it.  The wave owning
+// separately so the batched kernel can reuse it.  The wave owning
 // strip 0 does the per-case clock bookkeeping.
 //
 // Row loop: a prologue builds the y "north" state of row j0-1, then one step per
@@ -462,9 +462,10 @@ __global__ __launch_bounds__(256) void stage_f32_faces_kernel(StageArgs a, const
 }
 
 // ---------------------------------------------------------------------------
-// Persistent batched stage: hundreds of independent cases of uneven size in one
-// launch.  A host-built work table lists (case, strip) pairs, largest strips
-// first; a fixed grid of waves strides through it.  Every case keeps its own
+// Batched stage: hundreds of independent cases of uneven size in one launch.  A
+// host-built work table lists (case, strip) pairs, largest strips first.  By default
+// the grid has one wave per pair and the dispatcher balances them; a smaller grid
+// strides through the table (measured up to 13% slower on 512 cases).  Every case keeps its own
 // DevClock (per-case dt from its own CFL maximum) and its own plane sets
 // A, B, C; all cases advance in lock-step stage by stage.
 // ---------------------------------------------------------------------------

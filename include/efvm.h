@@ -171,7 +171,8 @@ const char* efvm_batch_last_error(const efvm_batch* b);
 efvm_status efvm_batch_set_launch(efvm_batch* b, int case_id, int strip_rows);
 /* rows per wave strip for every case at once (n_cases ints) */
 efvm_status efvm_batch_set_rows(efvm_batch* b, const int* rows);
-/* persistent grid: workgroups (0 = one per 4 strips, capped) and waves per workgroup */
+/* launch grid: workgroups (0 = one wave per work item, the default; fewer make the
+ * waves stride through the work table) and waves per workgroup */
 efvm_status efvm_batch_set_grid(efvm_batch* b, int blocks, int waves_per_block);
 efvm_status efvm_batch_upload(efvm_batch* b, int case_id, const double* cons5, size_t n_cells);
 efvm_status efvm_batch_download(efvm_batch* b, int case_id, double* cons5, size_t n_cells);
