@@ -152,7 +152,7 @@ efvm_status efvm_last_elapsed_ms(efvm_ctx* ctx, double* ms);
 
 /* ---------------------------------------------------------------------------
  * Batched ensembles: many independent 2-D cases of uneven size advanced by one
- * persistent stage launch per RK stage, each with its own gamma, boundaries and
+ * stage launch per RK stage (a wave per case strip), each with its own gamma, boundaries and
  * CFL-limited dt.  The reference runs one App per case; this has no single
  * counterpart and replaces N copies of the context path above.  FP32, RK3.
  * ------------------------------------------------------------------------- */
