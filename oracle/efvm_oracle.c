@@ -40,8 +40,18 @@
  *   - locate(): s = (p - lo)/h - 0.5, i0 = floor(s), t = s - i0.
  * These are parity-unpinned by any reference test except where the tests
  * under tests/ say otherwise.
+ *
+ * orc_set_threads(s, n) runs the loops whose result does not depend on their
+ * order (slopes, halo and boundary fills, viscous terms, RK update, CFL maximum)
+ * on n OpenMP threads, and the face sweep in two phases: every face flux first,
+ * then each cell's right-hand side summed in exactly the order the serial sweep
+ * scatters into it.  The result is bitwise the serial one (tests/test_oracle.py
+ * checks it); the serial form stays the definition.
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -105,6 +115,11 @@ typedef struct {
     /* FromOutside + BoundaryInputs (cfd_boundary:108-147): per boundary face of a side,
        5 conserved values and a given flag; not given -> SupersonicOutflow */
     double *tab[6]; long tab_n[6];
+    /* threads > 1: the order-independent loops run in parallel and the face sweep in two
+       phases (fluxes, then each cell's sum in the sweep's order): bitwise the serial result */
+    int threads;
+    cv_t *fplus[3];                  /* + face flux of every interior cell, per axis */
+    cv_t *fminus[3];                 /* - face flux of the low-edge cells, per axis */
 } orc_sim;
 
 static inline long IDX(const orc_sim *s, int i, int j, int k) { return i + (long)s->NX * (j + (long)s->NY * k); }
@@ -313,9 +328,21 @@ void orc_free(orc_sim *s) {
     free(s->u); free(s->rhs); free(s->u0); free(s->slope); free(s->prim); free(s->solid);
     for (int q = 0; q < 6; q++) free(s->tab[q]);
     free(s->ghost_idx); free(s->ghost_img); free(s->mark);
+    for (int d = 0; d < 3; d++) { free(s->fplus[d]); free(s->fminus[d]); }
     free(s);
 }
 long orc_ntot(const orc_sim *s) { return s->ntot; }
+/* Number of OpenMP threads for the step (1 = the serial sweep); returns the count in use. */
+int orc_set_threads(orc_sim *s, int n) {
+#ifdef _OPENMP
+    s->threads = n > 1 ? n : 1;
+#else
+    (void)n;
+    s->threads = 1;
+#endif
+    return s->threads;
+}
+#define OMP_THREADS(s) num_threads((s)->threads > 1 ? (s)->threads : 1) if((s)->threads > 1)
 
 /* Raw (padded) state, Component wire order (rho, rho_u, rho_v, rho_w, rho_E) per cell. */
 void orc_set_raw(orc_sim *s, const double *src) { memcpy(s->u, src, sizeof(cv_t) * s->ntot); }
@@ -450,11 +477,14 @@ void orc_sums(const orc_sim *s, double *out) {
 }
 
 /* ---------------- systems ---------------- */
+#pragma omp declare reduction(max_fmax : double : omp_out = fmax(omp_out, omp_in)) initializer(omp_priv = 0.0)
 static void cfl_system(orc_sim *s) {                                /* cfd_solver:626-677 */
     if (s->stage != STAGE_FIRST) return;
     if (s->c.fixed_dt > 0.0) { s->dt = s->c.fixed_dt; return; }
     double g = s->c.gamma, lmax = 0.0;
     double dx = s->h[0], dy = s->h[1], dz = s->h[2];
+    /* a maximum does not depend on the visiting order (fmax ignores a NaN lambda either way) */
+#pragma omp parallel for OMP_THREADS(s) reduction(max_fmax : lmax)
     for (int i = 0; i < s->c.nx; i++) for (int j = 0; j < s->c.ny; j++) for (int k = 0; k < s->c.nz; k++) {
         const cv_t *u = &s->u[IDX(s, i + NG, j + NG, k + NG)];
         if (u->r <= 0.0) continue;
@@ -476,6 +506,8 @@ static void halo_system(orc_sim *s) {                               /* periodic 
     int n[3] = {s->c.nx, s->c.ny, s->c.nz};
     for (int d = 0; d < 3; d++) {
         if (!axis_periodic(s, d)) continue;
+        /* within one axis every write is a ghost cell and every read an interior cell along d */
+#pragma omp parallel for collapse(2) OMP_THREADS(s)
         for (int k = 0; k < s->NZ; k++) for (int j = 0; j < s->NY; j++) for (int i = 0; i < s->NX; i++) {
             int id[3] = {i, j, k};
             int r = id[d];
@@ -488,12 +520,12 @@ static void halo_system(orc_sim *s) {                               /* periodic 
     }
 }
 
-static cv_t bc_ghost(const orc_sim *s, int side, const cv_t *in) {  /* cfd_boundary:173-362 */
-    double g = s->c.gamma;
-    const double *par = s->c.bc_par[side];
+static cv_t bc_ghost_of(const orc_cfg *c, int side, const cv_t *in) {  /* cfd_boundary:173-362 */
+    double g = c->gamma;
+    const double *par = c->bc_par[side];
     cv_t o = *in;
     int axis = side / 2;
-    switch (s->c.bc[side]) {
+    switch (c->bc[side]) {
     case BC_OUTFLOW: return o;
     case BC_SLIP: case BC_SYMMETRY:
         if (axis == 0) o.ru = -o.ru; else if (axis == 1) o.rv = -o.rv; else o.rw = -o.rw;
@@ -501,9 +533,22 @@ static cv_t bc_ghost(const orc_sim *s, int side, const cv_t *in) {  /* cfd_bound
     case BC_NOSLIP: o.ru = -o.ru; o.rv = -o.rv; o.rw = -o.rw; return o;
     case BC_SUP_INFLOW: { pv_t p = {par[0], par[1], par[2], par[3], par[4], par[4] / (par[0] * 287.058)}; return prim_to_cons(&p, g); }
     case BC_SUB_INFLOW: { double pin = pressure(in, g); pv_t p = {par[0], par[1], par[2], par[3], pin, par[4]}; return prim_to_cons(&p, g); }
-    case BC_PRESSURE_OUTLET: { pv_t p = cons_to_prim(in, g, s->c.r_gas); p.p = par[0]; p.t = par[0] / (p.r * 287.058); return prim_to_cons(&p, g); }
+    case BC_PRESSURE_OUTLET: { pv_t p = cons_to_prim(in, g, c->r_gas); p.p = par[0]; p.t = par[0] / (p.r * 287.058); return prim_to_cons(&p, g); }
     default: return o;
     }
+}
+static cv_t bc_ghost(const orc_sim *s, int side, const cv_t *in) { return bc_ghost_of(&s->c, side, in); }
+/* BoundaryCondition::ghost of one stock condition for a box side (side 0..5 = XLo..ZHi),
+ * par = its parameters (inflow rho, u, v, w, p | T; outlet p): the unit-test entry point. */
+void orc_bc_ghost(int kind, int side, const double *par, const double *interior, double gamma, double r_gas, double *out) {
+    orc_cfg c;
+    memset(&c, 0, sizeof(c));
+    c.gamma = gamma; c.r_gas = r_gas;
+    c.bc[side] = kind;
+    for (int q = 0; q < 5; q++) c.bc_par[side][q] = par ? par[q] : 0.0;
+    cv_t in = {interior[0], interior[1], interior[2], interior[3], interior[4]};
+    cv_t o = bc_ghost_of(&c, side, &in);
+    out[0] = o.r; out[1] = o.ru; out[2] = o.rv; out[3] = o.rw; out[4] = o.re;
 }
 
 static void boundary_system(orc_sim *s) {                           /* cfd_boundary:574-601 */
@@ -513,6 +558,7 @@ static void boundary_system(orc_sim *s) {                           /* cfd_bound
         int axis = side / 2, hi = side & 1;
         int n[3] = {nx, ny, nz};
         int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+#pragma omp parallel for collapse(2) OMP_THREADS(s)
         for (int q2 = 0; q2 < n[a2]; q2++) for (int q1 = 0; q1 < n[a1]; q1++) {
             int own[3], gh[3];
             own[a1] = q1 + NG; own[a2] = q2 + NG; gh[a1] = own[a1]; gh[a2] = own[a2];
@@ -525,7 +571,7 @@ static void boundary_system(orc_sim *s) {                           /* cfd_bound
                  * x side, the column of a y side) */
                 long f = axis == 0 ? q1 + (long)ny * q2 : (axis == 1 ? q2 + (long)nx * q1 : q1 + (long)nx * q2);
                 const double *t = (s->tab[side] && f < s->tab_n[side]) ? s->tab[side] + 6 * f : NULL;
-                if (t && t[5] != 0.0) { cv_t g = {t[0], t[1], t[2], t[3], t[4]}; s->u[IDX(s, gh[0], gh[1], gh[2])] = g; }
+                if (t && t[5] != 0.0) { cv_t gv = {t[0], t[1], t[2], t[3], t[4]}; s->u[IDX(s, gh[0], gh[1], gh[2])] = gv; }
                 else s->u[IDX(s, gh[0], gh[1], gh[2])] = in;
                 continue;
             }
@@ -771,6 +817,7 @@ static void zero_rhs(orc_sim *s) { memset(s->rhs, 0, sizeof(cv_t) * s->ntot); }
 static void slopes_system(orc_sim *s) {                             /* cfd_solver:233-268 */
     int lim = s->c.limiter;
     memset(s->slope, 0, sizeof(*s->slope) * s->ntot);
+#pragma omp parallel for collapse(2) OMP_THREADS(s)
     for (int k = NG; k < NG + s->c.nz; k++) for (int j = NG; j < NG + s->c.ny; j++) for (int i = NG; i < NG + s->c.nx; i++) {
         long c = IDX(s, i, j, k);
         cv_t us = s->u[c];
@@ -798,7 +845,80 @@ static inline cv_t recon(cv_t u, const cv_t sl[3], double frac_axis, int axis) {
     return u;
 }
 
+/* The flux through one face of the sweep: cell (i, j, k), axis ax, side 0 (+ face) or 1 (- face
+ * at the low edge).  Sets *scatter when the serial sweep also adds it to the + neighbour. */
+static cv_t sweep_face(const orc_sim *s, int i, int j, int k, int ax, int side, int *scatter) {
+    int muscl = s->c.recon == RECON_MUSCL;
+    long st[3] = {1, s->NX, (long)s->NX * s->NY};
+    int n[3] = {s->c.nx, s->c.ny, s->c.nz};
+    int id[3] = {i, j, k};
+    long own = IDX(s, i + NG, j + NG, k + NG);
+    int sign = side == 0 ? 1 : -1;
+    long oth = own + sign * st[ax];
+    int other_local = side == 0 ? (id[ax] + 1 < n[ax]) : 0;
+    int edge = side == 0 ? (id[ax] + 1 == n[ax]) : 1;
+    int patch = edge && !axis_periodic(s, ax);
+    cv_t ul = s->u[own], ur = s->u[oth];
+    if (muscl && !patch) {
+        ul = recon(s->u[own], s->slope[own], 0.5 * sign, ax);
+        if (other_local) ur = recon(s->u[oth], s->slope[oth], -0.5 * sign, ax);
+        else if (s->c.seam_mode == 1) {
+            int o3[3] = {i + NG, j + NG, k + NG}; o3[ax] += sign;
+            long w = wrap_raw(s, o3[0], o3[1], o3[2]);
+            ur = recon(s->u[w], s->slope[w], -0.5 * sign, ax);
+        }
+    }
+    *scatter = !patch && other_local;
+    return flux_normal_axis(s->c.flux, &ul, &ur, s->c.gamma, ax, sign);
+}
+
+/* Threaded sweep, bitwise the serial one.  Phase 1 evaluates every face flux; phase 2 forms each
+ * cell's right-hand side in the serial sweep's order: the scatters from the - neighbours along z,
+ * y, x (visited earlier, in that order), then the cell's own faces x+, x-(low edge), y+, y-, z+, z-. */
+static void flux_sweep_threaded(orc_sim *s) {
+    int n[3] = {s->c.nx, s->c.ny, s->c.nz};
+    long nin = (long)n[0] * n[1] * n[2];
+    double vol = cell_volume(s);
+    double area[3] = {s->h[1] * s->h[2], s->h[0] * s->h[2], s->h[0] * s->h[1]};
+    long edge_n[3] = {(long)n[1] * n[2], (long)n[0] * n[2], (long)n[0] * n[1]};
+    for (int d = 0; d < 3; d++) {
+        if (!s->fplus[d]) s->fplus[d] = (cv_t *)malloc(sizeof(cv_t) * nin);
+        if (!s->fminus[d]) s->fminus[d] = (cv_t *)malloc(sizeof(cv_t) * edge_n[d]);
+    }
+#pragma omp parallel for collapse(2) OMP_THREADS(s)
+    for (int k = 0; k < n[2]; k++) for (int j = 0; j < n[1]; j++) for (int i = 0; i < n[0]; i++) {
+        long q = i + (long)n[0] * (j + (long)n[1] * k);
+        int id[3] = {i, j, k}, sc;
+        for (int ax = 0; ax < 3; ax++) {
+            s->fplus[ax][q] = sweep_face(s, i, j, k, ax, 0, &sc);
+            if (id[ax] == 0) {
+                long e = ax == 0 ? j + (long)n[1] * k : (ax == 1 ? i + (long)n[0] * k : i + (long)n[0] * j);
+                s->fminus[ax][e] = sweep_face(s, i, j, k, ax, 1, &sc);
+            }
+        }
+    }
+#pragma omp parallel for collapse(2) OMP_THREADS(s)
+    for (int k = 0; k < n[2]; k++) for (int j = 0; j < n[1]; j++) for (int i = 0; i < n[0]; i++) {
+        long q = i + (long)n[0] * (j + (long)n[1] * k);
+        long sq[3] = {1, n[0], (long)n[0] * n[1]};
+        int id[3] = {i, j, k};
+        cv_t acc = s->rhs[IDX(s, i + NG, j + NG, k + NG)];
+        for (int ax = 2; ax >= 0; ax--)               /* + faces of the z, y, x neighbours */
+            if (id[ax] >= 1) acc = cv_add(acc, cv_mul(s->fplus[ax][q - sq[ax]], area[ax] / vol));
+        for (int ax = 0; ax < 3; ax++) {
+            double inv = area[ax] / vol;
+            acc = cv_sub(acc, cv_mul(s->fplus[ax][q], inv));
+            if (id[ax] == 0) {
+                long e = ax == 0 ? j + (long)n[1] * k : (ax == 1 ? i + (long)n[0] * k : i + (long)n[0] * j);
+                acc = cv_sub(acc, cv_mul(s->fminus[ax][e], inv));
+            }
+        }
+        s->rhs[IDX(s, i + NG, j + NG, k + NG)] = acc;
+    }
+}
+
 static void flux_sweep(orc_sim *s) {                                /* cfd_solver:93-122 / 293-339 */
+    if (s->threads > 1) { flux_sweep_threaded(s); return; }
     int muscl = s->c.recon == RECON_MUSCL;
     int scheme = s->c.flux;
     double g = s->c.gamma;
@@ -841,6 +961,7 @@ static inline double d2_pre(double um, double u0, double up, double ihm, double 
 }
 static void viscous_system(orc_sim *s) {                            /* cfd_solver:463-560 */
     double g = s->c.gamma, R = s->c.r_gas;
+#pragma omp parallel for OMP_THREADS(s)
     for (long c = 0; c < s->ntot; c++) s->prim[c] = cons_to_prim(&s->u[c], g, R);
     int nx = s->c.nx, ny = s->c.ny, nz = s->c.nz;
     double dx = s->h[0];
@@ -857,6 +978,7 @@ static void viscous_system(orc_sim *s) {                            /* cfd_solve
         double hm = centroid(s, 2, kr) - centroid(s, 2, kr - 1), hp = centroid(s, 2, kr + 1) - centroid(s, 2, kr);
         zim[k] = 1.0 / hm; zip[k] = 1.0 / hp; zis[k] = 2.0 / (hm + hp);
     }
+#pragma omp parallel for collapse(2) OMP_THREADS(s)
     for (int k = 0; k < nz; k++) for (int j = 0; j < ny; j++) for (int i = 0; i < nx; i++) {
         int ir = i + NG, jr = j + NG, kr = k + NG;
         long c = IDX(s, ir, jr, kr);
@@ -901,17 +1023,21 @@ static int update_system(orc_sim *s) {                              /* integrate
     double dt = s->dt;
     int completed = 0;
     if (s->c.integrator == INTEG_EULER) {
+#pragma omp parallel for OMP_THREADS(s)
         for (long c = 0; c < s->ntot; c++) s->u[c] = cv_add(s->u[c], cv_mul(s->rhs[c], dt));
         s->stage = STAGE_FIRST; completed = 1;
     } else if (s->stage == STAGE_FIRST) {
         memcpy(s->u0, s->u, sizeof(cv_t) * s->ntot);
+#pragma omp parallel for OMP_THREADS(s)
         for (long c = 0; c < s->ntot; c++) s->u[c] = cv_add(s->u[c], cv_mul(s->rhs[c], dt));
         s->stage = STAGE_SECOND;
     } else if (s->stage == STAGE_SECOND) {
+#pragma omp parallel for OMP_THREADS(s)
         for (long c = 0; c < s->ntot; c++)
             s->u[c] = cv_add(cv_mul(s->u0[c], 0.75), cv_mul(cv_add(s->u[c], cv_mul(s->rhs[c], dt)), 0.25));
         s->stage = STAGE_THIRD;
     } else {
+#pragma omp parallel for OMP_THREADS(s)
         for (long c = 0; c < s->ntot; c++)
             s->u[c] = cv_add(cv_mul(s->u0[c], 1.0 / 3.0), cv_mul(cv_add(s->u[c], cv_mul(s->rhs[c], dt)), 2.0 / 3.0));
         s->stage = STAGE_FIRST; completed = 1;

@@ -73,6 +73,8 @@ def lib():
         L.orc_free.argtypes = [C.c_void_p]
         L.orc_ntot.restype = C.c_long
         L.orc_ntot.argtypes = [C.c_void_p]
+        L.orc_set_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
         for n in ("orc_set_raw", "orc_get_raw", "orc_set_interior", "orc_get_interior", "orc_get_rhs_interior",
                   "orc_marker_force", "orc_wall_force", "orc_sums"):
             getattr(L, n).argtypes = [C.c_void_p, pd]
@@ -111,6 +113,7 @@ def lib():
         L.orc_sound_speed.argtypes = [pd, d]
         L.orc_sutherland.restype = d
         L.orc_sutherland.argtypes = [d]
+        L.orc_bc_ghost.argtypes = [C.c_int, C.c_int, pd, pd, d, d, pd]
         L.orc_limiter.restype = d
         L.orc_limiter.argtypes = [C.c_int, d, d]
         L.orc_ibm_ghosts.restype = C.c_long
@@ -171,6 +174,15 @@ def sound_speed(U, gamma: float) -> float:
 
 def sutherland(t: float) -> float:
     return lib().orc_sutherland(t)
+
+
+def bc_ghost(kind: str, side: int, interior, params=None, gamma: float = 1.4, r_gas: float = R_AIR) -> np.ndarray:
+    """BoundaryCondition::ghost of a stock condition on box side `side` (0..5 = XLo..ZHi)."""
+    out = np.zeros(5)
+    par = _vec(list(params or []) + [0.0] * (5 - len(params or [])))
+    u = _vec(interior)
+    lib().orc_bc_ghost(BC[kind], side, _p(par), _p(u), gamma, r_gas, _p(out))
+    return out
 
 
 def limiter(name: str, a: float, b: float) -> float:
@@ -274,6 +286,10 @@ class Sim:
         if getattr(self, "ptr", None):
             lib().orc_free(self.ptr)
             self.ptr = None
+
+    def set_threads(self, n: int) -> int:
+        """Run steps on `n` OpenMP threads (bitwise the serial result); returns the count in use."""
+        return lib().orc_set_threads(self.ptr, int(n))
 
     @property
     def ntot(self) -> int:

@@ -353,3 +353,229 @@ def test_position_dependent_condition_as_a_table():
     speeds = raw[2, 2:6, 1, 1]                    # rho_u of the first ghost layer beyond x-lo
     assert np.all(np.abs(speeds - 400.0 * y * (1.0 - y)) < 1e-12)
     assert speeds[1] > speeds[0]
+
+
+# ---------------------------------------------------------------- stock boundary conditions
+# crates/cfd_boundary/src/lib.rs:686-718 (the reference's unit tests) and closed forms for the
+# conditions those tests do not cover (SubsonicInflow, PressureOutlet, NoSlipWall, Symmetry).
+XLO, XHI, YLO, YHI, ZLO, ZHI = range(6)
+
+
+def _prim(u, gamma=AIR, r_gas=R):
+    return orc.cons_to_prim(u, gamma, r_gas)
+
+
+def test_outflow_copies_interior():
+    """cfd_boundary/src/lib.rs:688-693."""
+    interior = orc.prim_to_cons([1.2, 100.0, 0.0, 0.0, 101325.0, 288.0], AIR)
+    g = orc.bc_ghost("outflow", XLO, interior)
+    assert np.array_equal(g, interior)
+
+
+def test_slip_wall_reflects_only_normal_momentum():
+    """cfd_boundary/src/lib.rs:695-706, extended to every side and the Symmetry plane."""
+    interior = orc.prim_to_cons([1.2, 100.0, 50.0, 25.0, 101325.0, 288.0], AIR)
+    for kind in ("slip", "symmetry"):
+        for side in range(6):
+            g = orc.bc_ghost(kind, side, interior)
+            axis = side // 2
+            for m in range(5):
+                if m == 1 + axis:
+                    assert g[m] == -interior[m], (kind, side)
+                else:
+                    assert g[m] == interior[m], (kind, side, m)
+
+
+def test_no_slip_wall_reflects_all_momentum():
+    """NoSlipWall (cfd_boundary/src/lib.rs:268-290): every momentum component flips, rho and
+    rho_E are kept, so the wall-interpolated velocity is zero and the wall adiabatic."""
+    interior = orc.prim_to_cons([1.2, 100.0, 50.0, 25.0, 101325.0, 288.0], AIR)
+    for side in range(6):
+        g = orc.bc_ghost("noslip", side, interior)
+        assert np.array_equal(g[1:4], -interior[1:4]) and g[0] == interior[0] and g[4] == interior[4]
+        assert np.all(0.5 * (g[1:4] + interior[1:4]) == 0.0)
+
+
+def test_supersonic_inflow_is_prescribed():
+    """cfd_boundary/src/lib.rs:708-718, plus the closed form of the prescribed state:
+    (rho, rho u, rho v, rho w, p / (gamma - 1) + rho |u|^2 / 2), independent of the interior."""
+    interior = orc.prim_to_cons([0.1, 0.0, 0.0, 0.0, 1000.0, 200.0], AIR)
+    par = (1.0, 500.0, 0.0, 0.0, 2e5)
+    g = orc.bc_ghost("supersonic_inflow", XLO, interior, par)
+    prim = _prim(g)
+    assert abs(prim[0] - 1.0) < 1e-9 and abs(prim[1] - 500.0) < 1e-6 and abs(prim[4] - 2e5) < 1.0
+    assert np.array_equal(g[:4], [1.0, 500.0, 0.0, 0.0])
+    assert g[4] == pytest.approx(2e5 / (AIR - 1.0) + 0.5 * 500.0 ** 2, rel=1e-15)
+    other = orc.prim_to_cons([2.0, -30.0, 10.0, 0.0, 5e4, 90.0], AIR)
+    assert np.array_equal(orc.bc_ghost("supersonic_inflow", YHI, other, par), g)
+    # a planted slip (the prescribed temperature used as the pressure) is caught
+    assert abs(_prim(orc.prim_to_cons([1.0, 500.0, 0, 0, 2e5 / (1.0 * R), 0.0], AIR))[4] - 2e5) > 1.0
+
+
+def test_subsonic_inflow_prescribes_velocity_and_keeps_interior_pressure():
+    """SubsonicInflow (cfd_boundary/src/lib.rs:312-334): rho, u, v, w (and T) prescribed, the
+    pressure taken from the interior cell."""
+    interior = orc.prim_to_cons([0.9, 20.0, -5.0, 3.0, 8.5e4, 330.0], AIR)
+    p_int = _prim(interior)[4]
+    par = (1.1, 40.0, 2.0, -1.0, 300.0)
+    g = orc.bc_ghost("subsonic_inflow", XLO, interior, par)
+    rho, u, v, w = par[:4]
+    assert g[0] == rho and g[1] == rho * u and g[2] == rho * v and g[3] == rho * w
+    assert g[4] == pytest.approx(p_int / (AIR - 1.0) + 0.5 * rho * (u * u + v * v + w * w), rel=1e-15)
+    assert _prim(g)[4] == pytest.approx(p_int, rel=1e-13)
+
+
+def test_pressure_outlet_sets_pressure_and_keeps_density_and_velocity():
+    """PressureOutlet (cfd_boundary/src/lib.rs:336-362): back pressure set, rho and velocity
+    extrapolated, total energy rebuilt from them."""
+    interior = orc.prim_to_cons([0.8, 120.0, -15.0, 4.0, 9.0e4, 390.0], AIR)
+    g = orc.bc_ghost("pressure_outlet", XHI, interior, (7.5e4,))
+    pi = _prim(interior)
+    assert g[0] == interior[0]
+    assert np.allclose(g[1:4], interior[1:4], rtol=1e-15, atol=0)
+    ke = 0.5 * pi[0] * (pi[1] ** 2 + pi[2] ** 2 + pi[3] ** 2)
+    assert g[4] == pytest.approx(7.5e4 / (AIR - 1.0) + ke, rel=1e-15)
+    assert _prim(g)[4] == pytest.approx(7.5e4, rel=1e-12)
+
+
+def test_boundary_phase_writes_the_ghost_of_each_condition():
+    """The Boundary phase puts bc.ghost(owner) into the first ghost layer of every face of a side
+    (cfd_boundary/src/lib.rs:574-601): a 5 x 3 box with a different condition on each x/y side."""
+    par = [(1.0, 300.0, 0.0, 0.0, 1.5e5), (7e4,), None, None, None, None]
+    s = orc.Sim(5, 3, 1, hi=(1.0, 0.6, 0.2), flux="rusanov", integrator="euler", fixed_dt=1e-12,
+                bcs=["supersonic_inflow", "pressure_outlet", "noslip", "symmetry", "slip", "slip"], bc_params=par)
+    rng = np.random.default_rng(5)
+    st = np.array([[orc.prim_to_cons([1.0 + 0.1 * rng.random(), 50 * rng.random(), 20 * rng.random(), 0.0,
+                                       1e5 * (1 + 0.1 * rng.random()), 300.0], AIR) for i in range(5)] for j in range(3)])
+    s.set_interior(st.reshape(1, 3, 5, 5))
+    s.run()
+    raw = s.raw()[2]
+    for j in range(3):
+        assert np.array_equal(raw[j + 2, 1], orc.bc_ghost("supersonic_inflow", XLO, st[j, 0], par[0]))
+        assert np.array_equal(raw[j + 2, 7], orc.bc_ghost("pressure_outlet", XHI, st[j, 4], par[1]))
+    for i in range(5):
+        assert np.array_equal(raw[1, i + 2], orc.bc_ghost("noslip", YLO, st[0, i]))
+        assert np.array_equal(raw[5, i + 2], orc.bc_ghost("symmetry", YHI, st[2, i]))
+
+
+# ---------------------------------------------------------------- staircase wall forces
+# compute_wall_forces (crates/cfd_ibm/src/lib.rs:309-362): pressure -p n A and shear
+# mu u_t / (dn / 2) A summed over the solid-fluid faces of the staircase body.
+def _cylinder_box(nx=40, ny=32, r=0.23):
+    L = 1.0
+    fs = orc.prim_to_cons([1.0, 0.0, 0.0, 0.0, 1e5, 1e5 / R], AIR)
+    s = orc.Sim(nx, ny, 1, hi=(L, L * ny / nx, 0.05), flux="hllc", integrator="euler", fixed_dt=1e-12,
+                viscous=True, mu=2e-3, bcs=["slip"] * 6, cylinders=[(0.5 * L + 0.013, 0.5 * L * ny / nx - 0.007, r)],
+                freestream=fs)
+    s.set_interior(np.tile(fs, (1, ny, nx, 1)))
+    s.run()                                    # Setup phase builds the solid mask
+    return s
+
+
+def _fluid_state(s, p, u, v):
+    """Raw state with primitive fields p(x, y), u, v (rho = 1 everywhere)."""
+    nx, ny = s.nx, s.ny
+    raw = np.zeros((5, ny + 4, nx + 4, 5))
+    xs = (np.arange(nx + 4) - 2 + 0.5) * s.h[0]
+    ys = (np.arange(ny + 4) - 2 + 0.5) * s.h[1]
+    X, Y = np.meshgrid(xs, ys)
+    P, U, V = p(X, Y), u(X, Y), v(X, Y)
+    raw[:, :, :, 0] = 1.0
+    raw[:, :, :, 1] = U
+    raw[:, :, :, 2] = V
+    raw[:, :, :, 4] = P / 0.4 + 0.5 * (U * U + V * V)
+    return raw
+
+
+def _runs(mask2):
+    """(number of solid runs along x, along y, number of solid-fluid faces normal to x, to y)."""
+    fx = np.sum(mask2[:, 1:] != mask2[:, :-1])
+    fy = np.sum(mask2[1:, :] != mask2[:-1, :])
+    return fx, fy
+
+
+def test_wall_force_of_uniform_pressure_on_a_closed_body_is_zero():
+    """A closed staircase surface has zero net area vector: uniform p at rest gives F = 0."""
+    s = _cylinder_box()
+    mask = s.solid()[2].astype(bool)
+    assert 20 < mask.sum() < 400 and not mask[:2].any() and not mask[:, :2].any()
+    s.set_raw(_fluid_state(s, lambda x, y: 1e5 + 0 * x, lambda x, y: 0 * x, lambda x, y: 0 * x))
+    f = s.wall_force()
+    fx, fy = _runs(mask)
+    scale = 1e5 * s.h[1] * s.h[2] * fx
+    assert np.all(np.abs(f) <= 1e-14 * scale), f
+
+
+def test_wall_force_of_a_linear_pressure_field_telescopes():
+    """p = p0 + G x: along each row the faces of a solid run see p at the fluid cells just outside
+    it, so F_x = -G * A_x * sum over runs of (x_right_fluid - x_left_fluid) = -G A_x dx sum(n_run + 1),
+    and F_y = 0 (p does not vary along a column).  Closed form from the mask's run lengths."""
+    s = _cylinder_box()
+    mask = s.solid()[2].astype(bool)
+    G = 3.0e3
+    s.set_raw(_fluid_state(s, lambda x, y: 1e5 + G * x, lambda x, y: 0 * x, lambda x, y: 0 * x))
+    f = s.wall_force()
+    ax = s.h[1] * s.h[2]
+    total = 0
+    for row in mask:
+        idx = np.flatnonzero(row)
+        if idx.size:
+            assert idx.size == idx[-1] - idx[0] + 1          # one run per row (convex body)
+            total += idx.size + 1
+    assert f[0] == pytest.approx(-G * ax * s.h[0] * total, rel=1e-11)
+    assert abs(f[1]) <= 1e-11 * abs(f[0])
+    assert np.all(f[3:] == 0.0)
+    # the force approaches -G times the body's area times its depth (the pressure-gradient
+    # analogue of Archimedes' law): the staircase adds half a cell at each end of every run
+    assert f[0] == pytest.approx(-G * math.pi * 0.23 ** 2 * 0.05, rel=0.25)
+
+
+def test_wall_shear_of_a_uniform_stream():
+    """u = U, v = 0 everywhere (constant mu): the faces normal to y and to z carry x shear.  On a
+    one-cell-thick mesh the z neighbours of a solid cell are ghost cells, never solid, so both z
+    faces of every solid cell count, as in the reference:
+    F_vx = mu U [2/dy (dx dz) N_y + 2/dz (dx dy) 2 N_solid], N_y = solid-fluid faces normal to y."""
+    s = _cylinder_box()
+    mask = s.solid()[2].astype(bool)
+    U = 7.0
+    s.set_raw(_fluid_state(s, lambda x, y: 1e5 + 0 * x, lambda x, y: U + 0 * x, lambda x, y: 0 * x))
+    f = s.wall_force()
+    fx, fy = _runs(mask)
+    dx, dy, dz = s.h
+    expect = 2e-3 * U * (2.0 / dy * dx * dz * fy + 2.0 / dz * dx * dy * 2 * mask.sum())
+    assert f[3] == pytest.approx(expect, rel=1e-12)
+    assert f[4] == 0.0 and f[5] == 0.0
+
+
+# ---------------------------------------------------------------- threaded oracle
+def test_threaded_oracle_is_bitwise_the_serial_one():
+    """orc_set_threads runs the order-independent loops in parallel and the face sweep in two
+    phases; every configuration below must give bitwise the serial state and clock."""
+    n = 40
+    from dev_field_efvm_amd import cases
+    st = cases.vortex_modes_state(n, n, seed=3).reshape(1, n, n, 5)
+    z, y, x = np.meshgrid(*[(np.arange(m) + 0.5) / m for m in (5, 6, 7)], indexing="ij")
+    rho = 1 + 0.2 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
+    st3 = np.stack([rho, rho * 0.3, rho * -0.2, rho * 0.25, 1 / 0.4 + 0.5 * rho * 0.1925], axis=-1)
+    fs = orc.prim_to_cons([1.0, 50.0, 0, 0, 1e5, 1e5 / R], AIR)
+    cfgs = [
+        (dict(nx=n, ny=n, nz=1, hi=(10, 10, 10 / n), muscl=True, limiter="mc", bcs=["periodic"] * 4 + ["slip"] * 2), st),
+        (dict(nx=n, ny=n, nz=1, hi=(10, 10, 10 / n), flux="rusanov", muscl=True, seam_mode=1,
+              bcs=["periodic"] * 4 + ["slip"] * 2), st),
+        (dict(nx=n, ny=n, nz=1, hi=(1, 1, 0.025), muscl=True, limiter="mc", viscous=True,
+              bcs=["noslip"] * 4 + ["slip"] * 2), st),
+        (dict(nx=7, ny=6, nz=5, muscl=True, limiter="mc", viscous=True,
+              bcs=["outflow", "slip", "noslip", "symmetry", "periodic", "periodic"]), st3),
+        (dict(nx=44, ny=16, nz=1, hi=(0.44, 0.16, 0.01), viscous=True, mu=1e-3, cfl=0.3,
+              bcs=["outflow", "pressure_outlet", "noslip", "noslip", "slip", "slip"], bc_params=[None, (1e5,), None, None, None, None],
+              cylinders=[(0.1, 0.08, 0.03)], freestream=fs), np.tile(fs, (1, 16, 44, 1))),
+    ]
+    for kw, ic in cfgs:
+        a, b = orc.Sim(**kw), orc.Sim(**kw)
+        assert b.set_threads(4) == 4
+        a.set_interior(ic)
+        b.set_interior(ic)
+        a.step(4)
+        b.step(4)
+        assert a.info() == b.info()
+        assert np.array_equal(a.raw().view(np.int64), b.raw().view(np.int64)), kw
