@@ -12,6 +12,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -26,7 +27,6 @@ UNITS = [
     ("efvm_stage_f64.hip", ["-ffp-contract=off"]),
     ("efvm_stage_f32.hip", ["-ffp-contract=on", "-fgpu-flush-denormals-to-zero"]),
     ("efvm_mlp.hip", ["-ffp-contract=fast"]),
-    ("efvm_batch.hip", ["-ffp-contract=fast"]),
     ("efvm_stage3d.hip", ["-ffp-contract=off"]),
     ("efvm_api.cpp", ["-ffp-contract=off"]),
     ("efvm_api3d.cpp", ["-ffp-contract=off"]),
@@ -40,20 +40,29 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build libefvm")
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    """Compile every translation unit (in parallel, `jobs` at a time; default one per unit up to
+    the host's cores) and link libefvm.so.  A missing source is an error."""
     os.makedirs(OBJ, exist_ok=True)
     cc = hipcc()
-    objs = []
+    cmds, objs = [], []
     for src, extra in UNITS:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
-            continue
+            raise FileNotFoundError(path)
         obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-        cmd = [cc, *COMMON, *extra, "-c", path, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([cc, *COMMON, *extra, "-c", path, "-o", obj])
         objs.append(obj)
+    jobs = jobs or min(len(cmds), os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for cmd, r in zip(cmds, pool.map(lambda c: subprocess.run(c, capture_output=True, text=True), cmds)):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            errs = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
+            if r.returncode != 0:
+                raise RuntimeError(f"{' '.join(cmd)} failed:\n{r.stderr}")
+            if verbose and errs:
+                print(errs, flush=True)
     tmp = LIB + ".tmp"
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:

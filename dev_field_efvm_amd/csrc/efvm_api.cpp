@@ -1,10 +1,11 @@
 // efvm_api.cpp -- implementation of the C ABI declared in include/efvm.h.
 //
-// A context owns three plane sets A, B, C (4 SoA planes each) in HBM:
-//   RK3 step:  stage 1  A -> B ;  stage 2  (B, U0=A) -> C ;  stage 3  (C, U0=A) -> A
-// Stage 3 writes U^{n+1} over U^n in place (pointwise read of U0, no stencil on
-// it), so U0 never needs a copy and the whole step is 128 B/cell of HBM traffic
-// in FP32 (256 B in FP64).  Euler ping-pongs A <-> B.  dt lives in device
+// A context owns three plane sets (4 SoA planes each) in HBM, in three roles: `cur`
+// holds U^n, `prev` U^{n-1}, the third is free.  An RK3 step is
+//   stage 1  cur -> prev ;  stage 2  (prev, U0=cur) -> free ;  stage 3  (free, U0=cur) -> prev
+// and then cur and prev swap: U^n stays intact through the step (the reference's u0 after
+// the step, crates/cfd_solver/src/integrate.rs:76-78), U0 never needs a copy, and the whole
+// step is 128 B/cell of HBM traffic in FP32 (256 B in FP64).  Euler ping-pongs cur <-> prev.  dt lives in device
 // memory (DevClock): the stage kernels read lambda_max and form dt = cfl/lambda
 // themselves, so a step needs no host round trip and can be graph-captured.
 #include <hip/hip_runtime.h>
@@ -66,7 +67,9 @@ struct efvm_ctx {
     size_t esz = 8;
     long ncell = 0;
     void* P[3][4] = {};            // plane sets A, B, C
-    int cur = 0;                   // index of the set holding U^n (A)
+    int cur = 0;                   // index of the set holding U^n
+    int prev = 1;                  // index of the set holding U^{n-1} (U1 / U2's input during a step)
+    int have_prev = 0;             // prev holds the state before the last completed step (or a loaded u0)
     DevClock* clock = nullptr;
     double* lam_part = nullptr;    // FP32: per-wave maxima of the fused CFL
     double* bc_tab[4] = {nullptr, nullptr, nullptr, nullptr};   // EFVM_BC_TABLE ghost tables
@@ -98,10 +101,11 @@ struct efvm_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_steps = 0;
+    int graph_cur = 0, graph_parity = 0;   // plane set and CFL slot the captured steps start from
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int timed = 0;
     void* snap = nullptr;          // autotuner snapshot (state + clock)
-    int snap_parity = 0, snap_lam = 0, snap_flip = 0;
+    int snap_parity = 0, snap_lam = 0, snap_flip = 0, snap_cur = 0, snap_prev = 1, snap_have_prev = 0;
     int learned = 0;               // learned face reconstruction attached
     int learned_bypass = 0;        // gate evaluated only where the ungated face value is near the TVD bound
     int learned_own4 = 0;          // the attached network's correction is the 4-tap own-variable form
@@ -180,6 +184,14 @@ IbmArgs ibm_args(const efvm_ctx* c, const StageArgs& a) {
     return b;
 }
 
+// The plane set holding the state of the current substage: U^n at a step boundary, U1 / U2
+// after the first / second RK3 substage.
+int stage_set(const efvm_ctx* c) {
+    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 1) return c->prev;
+    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 2) return 3 - c->cur - c->prev;
+    return c->cur;
+}
+
 void set_planes(StageArgs& a, void* const* in, void* const* u0, void* const* out) {
     for (int m = 0; m < 4; m++) { a.uin[m] = in[m]; a.u0[m] = u0[m]; a.uout[m] = out[m]; }
 }
@@ -205,8 +217,8 @@ efvm_status run_substage(efvm_ctx* c) {
     // RK3: A = U^n (set cur, never rotates), B = U1, C = U2.  Euler ping-pongs sets 0 <-> 1,
     // so an even number of steps returns the state to its starting set (graph replay).
     void* const* A = c->P[c->cur];
-    void* const* B = rk3 ? c->P[(c->cur + 1) % 3] : c->P[c->cur ^ 1];
-    void* const* Cc = c->P[(c->cur + 2) % 3];
+    void* const* B = c->P[c->prev];
+    void* const* Cc = c->P[3 - c->cur - c->prev];
     efvm_status st;
     // Setup: CFL of U^n (first substage only) ...
     if (c->rk_stage == 0 && !c->lam_valid && c->sch.fixed_dt <= 0.0) {
@@ -225,7 +237,7 @@ efvm_status run_substage(efvm_ctx* c) {
     void* const* in = A;
     void* const* out = B;
     if (rk3 && c->rk_stage == 1) { in = B; out = Cc; }
-    if (rk3 && c->rk_stage == 2) { in = Cc; out = A; }
+    if (rk3 && c->rk_stage == 2) { in = Cc; out = B; }
     set_planes(a, in, A, out);
     const bool last = !rk3 || c->rk_stage == 2;
     a.fuse_cfl = last && rk3 && !c->sponge && c->sch.fixed_dt <= 0.0;
@@ -270,7 +282,8 @@ efvm_status run_substage(efvm_ctx* c) {
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("cfl reduction: ") + hipGetErrorString(e));
     }
     if (last) {
-        if (!rk3) c->cur ^= 1;   // Euler: U^{n+1} is in B
+        std::swap(c->cur, c->prev);   // U^{n+1} is in the old prev set; U^n becomes prev
+        c->have_prev = 1;
         StageArgs b = base_args(c);
         set_planes(b, c->P[c->cur], c->P[c->cur], c->P[c->cur]);
         if (c->sponge) {
@@ -468,9 +481,21 @@ efvm_status efvm_set_body_force(efvm_ctx* c, double gx) {
     return EFVM_OK;
 }
 
+static efvm_status upload_set(efvm_ctx* c, int set, const double* cons5, size_t n);
+
 efvm_status efvm_upload_state(efvm_ctx* c, const double* cons5, size_t n) {
     if (!c || !cons5) return EFVM_ERR_ARG;
     if ((long)n != c->ncell) return fail(c, EFVM_ERR_ARG, "upload: n_cells != nx*ny");
+    efvm_status st = upload_set(c, c->cur, cons5, n);
+    if (st != EFVM_OK) return st;
+    c->rk_stage = 0;
+    c->lam_valid = 0;
+    c->have_prev = 0;
+    return EFVM_OK;
+}
+
+// Wire-order host state -> the planes of set `set` (synchronous).
+static efvm_status upload_set(efvm_ctx* c, int set, const double* cons5, size_t n) {
     std::vector<char> h[4];
     for (int m = 0; m < 4; m++) h[m].resize(c->esz * n);
     const int comp[4] = {0, 1, 2, 4};
@@ -482,11 +507,9 @@ efvm_status efvm_upload_state(efvm_ctx* c, const double* cons5, size_t n) {
             else reinterpret_cast<float*>(h[m].data())[q] = (float)v;
         }
     }
-    c->rk_stage = 0;
     for (int m = 0; m < 4; m++)
-        HIPCHK(c, hipMemcpyAsync(c->P[c->cur][m], h[m].data(), c->esz * n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->P[set][m], h[m].data(), c->esz * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->lam_valid = 0;
     return EFVM_OK;
 }
 
@@ -494,9 +517,7 @@ efvm_status efvm_download_state(efvm_ctx* c, double* cons5, size_t n) {
     if (!c || !cons5) return EFVM_ERR_ARG;
     if ((long)n != c->ncell) return fail(c, EFVM_ERR_ARG, "download: n_cells != nx*ny");
     // the state of the current substage: A at a step boundary, else B / C
-    int set = c->cur;
-    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 1) set = (c->cur + 1) % 3;
-    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 2) set = (c->cur + 2) % 3;
+    int set = stage_set(c);
     std::vector<char> h[4];
     for (int m = 0; m < 4; m++) {
         h[m].resize(c->esz * n);
@@ -608,7 +629,7 @@ efvm_status efvm_graph_capture(efvm_ctx* c, int n_steps) {
         c->ibm_flipped = 1;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const int cur0 = c->cur, par0 = c->parity;
+    const int cur0 = c->cur, prev0 = c->prev, par0 = c->parity, have0 = c->have_prev;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     efvm_status st = EFVM_OK;
     const int nsub = c->sch.integrator == EFVM_INTEGRATOR_RK3 ? 3 : 1;
@@ -623,16 +644,35 @@ efvm_status efvm_graph_capture(efvm_ctx* c, int n_steps) {
     c->graph_steps = n_steps;
     // capture only recorded work: restore the host-side step machinery (an even step count
     // returns both the CFL slot parity and the Euler ping-pong to their starting values)
-    c->cur = cur0; c->parity = par0; c->rk_stage = 0;
+    c->cur = cur0; c->prev = prev0; c->parity = par0; c->rk_stage = 0; c->have_prev = have0;
+    c->graph_cur = cur0; c->graph_parity = par0;
     return EFVM_OK;
 }
 
 efvm_status efvm_graph_launch(efvm_ctx* c) {
     if (!c || !c->graph_exec) return fail(c, EFVM_ERR_ARG, "no captured graph");
+    // The captured steps read U^n from the plane set and lambda_max from the CFL slot they were
+    // captured at.  Eager work since then may have left the context mid-step (a lone run()
+    // substage), on the other Euler plane set or CFL slot (an odd number of eager steps), or
+    // with a new state whose CFL maximum is not computed yet (an upload or a checkpoint load).
+    if (c->rk_stage != 0) return fail(c, EFVM_ERR_ARG, "graph replay must start at a step boundary");
+    if (c->cur != c->graph_cur || c->parity != c->graph_parity)
+        return fail(c, EFVM_ERR_ARG, "the state is on another plane set or CFL slot than at capture (an odd number of "
+                                     "eager steps since): capture the graph again");
+    if (!c->lam_valid && c->sch.fixed_dt <= 0.0) {
+        efvm_status st = launch_cfl_into(c, c->P[c->cur], &c->clock->lam[c->parity]);
+        if (st != EFVM_OK) return st;
+        c->lam_valid = 1;
+    }
+    if (c->n_cyl > 0 && !c->ibm_flipped) {
+        HIPCHK(c, launch_ibm_flip(c->P[c->cur], (int)c->esz, c->solid, c->ncell, c->stagnant, c->stream));
+        c->ibm_flipped = 1;
+    }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timed = 1;
+    c->have_prev = 1;             // the replayed steps end with the roles they started with
     return EFVM_OK;
 }
 
@@ -730,9 +770,7 @@ efvm_status efvm_marker_force(efvm_ctx* c, double out[6]) {
     if (!c || !out) return EFVM_ERR_ARG;
     if (c->n_cyl == 0) { for (int d = 0; d < 6; d++) out[d] = 0.0; return EFVM_OK; }
     StageArgs a = base_args(c);
-    int set = c->cur;
-    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 1) set = (c->cur + 1) % 3;
-    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 2) set = (c->cur + 2) % 3;
+    int set = stage_set(c);
     set_planes(a, c->P[set], c->P[set], c->P[set]);
     IbmArgs ib = ibm_args(c, a);
     double* dout = c->scratch + 4096;
@@ -800,6 +838,7 @@ struct efvm_batch {
     int blocks = 0, waves = 4;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    int graph_parity = 0;          // CFL slot the captured steps start from
     void* snap = nullptr;
     int snap_parity = 0, snap_lam = 0;
     std::string err;
@@ -1050,11 +1089,22 @@ efvm_status efvm_batch_graph_capture(efvm_batch* b, int n_steps) {
     if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("end capture: ") + hipGetErrorString(e));
     b->graph = g;
     BHIPCHK(b, hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+    b->graph_parity = par0;
     return EFVM_OK;
 }
 
 efvm_status efvm_batch_graph_launch(efvm_batch* b) {
     if (!b || !b->graph_exec) return bfail(b, EFVM_ERR_ARG, "no captured graph");
+    // the captured steps read each case's lambda_max from the CFL slot of the capture; an odd
+    // number of eager steps since moved it, and an upload needs a fresh CFL scan first
+    if (b->parity != b->graph_parity)
+        return bfail(b, EFVM_ERR_ARG, "the cases are on another CFL slot than at capture (an odd number of eager "
+                                      "steps since): capture the graph again");
+    if (!b->lam_valid && b->sch.fixed_dt <= 0.0) {
+        efvm_status st = batch_cfl_all(b, b->parity);
+        if (st != EFVM_OK) return st;
+        b->lam_valid = 1;
+    }
     BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
     BHIPCHK(b, hipGraphLaunch(b->graph_exec, b->stream));
     BHIPCHK(b, hipEventRecord(b->ev1, b->stream));
@@ -1119,19 +1169,28 @@ efvm_status efvm_snapshot(efvm_ctx* c, int op) {
     const size_t pbytes = c->esz * c->ncell;
     if (!c->snap) {
         if (op == 1) return fail(c, EFVM_ERR_ARG, "no snapshot saved");
-        HIPCHK(c, hipMalloc(&c->snap, 4 * pbytes + sizeof(DevClock)));
+        HIPCHK(c, hipMalloc(&c->snap, 8 * pbytes + sizeof(DevClock)));
     }
     char* s = (char*)c->snap;
+    if (op == 1) { c->cur = c->snap_cur; c->prev = c->snap_prev; }
+    // U^n and U^{n-1} (the latter is the checkpoint's u0 column)
     for (int m = 0; m < 4; m++) {
-        if (op == 0) HIPCHK(c, hipMemcpyAsync(s + m * pbytes, c->P[c->cur][m], pbytes, hipMemcpyDeviceToDevice, c->stream));
-        else HIPCHK(c, hipMemcpyAsync(c->P[c->cur][m], s + m * pbytes, pbytes, hipMemcpyDeviceToDevice, c->stream));
+        if (op == 0) {
+            HIPCHK(c, hipMemcpyAsync(s + m * pbytes, c->P[c->cur][m], pbytes, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(s + (4 + m) * pbytes, c->P[c->prev][m], pbytes, hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            HIPCHK(c, hipMemcpyAsync(c->P[c->cur][m], s + m * pbytes, pbytes, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->P[c->prev][m], s + (4 + m) * pbytes, pbytes, hipMemcpyDeviceToDevice, c->stream));
+        }
     }
     if (op == 0) {
-        HIPCHK(c, hipMemcpyAsync(s + 4 * pbytes, c->clock, sizeof(DevClock), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(s + 8 * pbytes, c->clock, sizeof(DevClock), hipMemcpyDeviceToDevice, c->stream));
         c->snap_parity = c->parity; c->snap_lam = c->lam_valid; c->snap_flip = c->ibm_flipped;
+        c->snap_cur = c->cur; c->snap_prev = c->prev; c->snap_have_prev = c->have_prev;
     } else {
-        HIPCHK(c, hipMemcpyAsync(c->clock, s + 4 * pbytes, sizeof(DevClock), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->clock, s + 8 * pbytes, sizeof(DevClock), hipMemcpyDeviceToDevice, c->stream));
         c->parity = c->snap_parity; c->lam_valid = c->snap_lam; c->ibm_flipped = c->snap_flip;
+        c->have_prev = c->snap_have_prev;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;
@@ -1748,10 +1807,15 @@ efvm_status efvm_checkpoint_save(efvm_ctx* c, const char* path) {
     if (!c || !path) return EFVM_ERR_ARG;
     const int nx = c->grid.nx, ny = c->grid.ny;
     const long n = c->ncell, nraw = (long)(nx + 4) * (ny + 4) * 5;
-    std::vector<double> u(5 * n), u0(5 * n);
+    std::vector<double> u(5 * n), u0(5 * n, 0.0);
     efvm_status st = efvm_download_state(c, u.data(), n);          // the current substage's state
     if (st != EFVM_OK) return st;
-    if ((st = download_set(c, c->cur, u0.data())) != EFVM_OK) return st;   // the RK backup U^n
+    // the reference's u0 (save_state at the first substage, crates/cfd_solver/src/integrate.rs:76-78):
+    // mid-step U^n; at a step boundary the state before the completed step, U^{n-1}, which the
+    // plane-set rotation keeps (zeros before the first step, as a new CfdState holds)
+    const bool rk3 = c->sch.integrator == EFVM_INTEGRATOR_RK3;   // Euler never saves: u0 stays zero
+    const int u0_set = !rk3 ? -1 : (c->rk_stage != 0 ? c->cur : (c->have_prev ? c->prev : -1));
+    if (u0_set >= 0 && (st = download_set(c, u0_set, u0.data())) != EFVM_OK) return st;
     double dt = 0.0, time = 0.0;
     long step = 0;
     int stage = 0;
@@ -1764,7 +1828,9 @@ efvm_status efvm_checkpoint_save(efvm_ctx* c, const char* path) {
     put(b, time);
     put(b, (unsigned long long)step);
     put(b, (unsigned char)stage);
-    // u, rhs (zero between substages: the device path keeps no rhs column), u0; ghosts zero
+    // u, rhs, u0 over the raw padded array.  The device keeps neither ghost cells (the loaders
+    // form them from the interior) nor an rhs column (the divergence lives in registers): ghost
+    // cells and the rhs column are written as zeros.  Interior u and u0 are the reference's.
     const double* cols[3] = {u.data(), nullptr, u0.data()};
     for (int k = 0; k < 3; k++) {
         put(b, (unsigned long long)nraw);
@@ -1813,21 +1879,28 @@ efvm_status efvm_checkpoint_load(efvm_ctx* c, const char* path) {
         return fail(c, EFVM_ERR_UNSUPPORTED, "checkpoint: restarts are taken at a step boundary (rk stage 0)");
     const int nx = c->grid.nx, ny = c->grid.ny;
     const long n = c->ncell, nraw = (long)(nx + 4) * (ny + 4) * 5;
-    std::vector<double> u(5 * n);
+    std::vector<double> u(5 * n), u0(5 * n);
     for (int k = 0; k < 3; k++) {
         unsigned long long len;
         if (!take(&len, 8)) return fail(c, EFVM_ERR_CONFIG, "checkpoint: truncated column");
         if ((long)len != nraw) return fail(c, EFVM_ERR_CONFIG, "checkpoint: cell count does not match this grid");
         if (pos + 40 * (size_t)nraw > b.size()) return fail(c, EFVM_ERR_CONFIG, "checkpoint: truncated column");
-        if (k == 0)
+        double* dst = k == 0 ? u.data() : (k == 2 ? u0.data() : nullptr);
+        if (dst)
             for (int j = 0; j < ny; j++)
                 for (int i = 0; i < nx; i++)
-                    std::memcpy(u.data() + 5 * ((long)j * nx + i), b.data() + pos + 40 * (size_t)raw_index(nx, ny, i, j), 40);
+                    std::memcpy(dst + 5 * ((long)j * nx + i), b.data() + pos + 40 * (size_t)raw_index(nx, ny, i, j), 40);
         pos += 40 * (size_t)nraw;
     }
     if (pos != b.size()) return fail(c, EFVM_ERR_CONFIG, "checkpoint: trailing bytes");
+    for (long q = 0; q < n; q++)
+        if (u0[5 * q + 3] != 0.0) return fail(c, EFVM_ERR_UNSUPPORTED, "checkpoint: rho_w of u0 must be 0 on the 2-D GPU path");
     efvm_status st = efvm_upload_state(c, u.data(), n);      // rk stage 0; the next step recomputes the CFL
     if (st != EFVM_OK) return st;
+    // the file's u0 (the state before its last step) goes to the prev set: a checkpoint written
+    // next carries it unchanged; the first step overwrites it
+    if ((st = upload_set(c, c->prev, u0.data(), n)) != EFVM_OK) return st;
+    c->have_prev = 1;
     drop_graph(c);
     HIPCHK(c, launch_clock_set(c->clock, dt, time, (long long)step, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

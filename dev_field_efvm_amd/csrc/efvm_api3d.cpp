@@ -40,6 +40,11 @@ struct efvm_ctx3 {
     size_t esz = 8;
     void* P[3][5] = {};      // raw padded state sets A, B, C
     int cur = 0;             // set holding U^n
+    int prev = 1;            // set holding U^{n-1} (U1 during a step); the third set holds U2
+    int have_prev = 0;       // prev holds the state before the last completed step (or a loaded u0)
+    int rhs_state = 0;       // the checkpoint's rhs column: 0 zeros (no substage since an upload),
+                             // 1 recomputed from the last substage's input, 2 the loaded column
+    std::vector<double> loaded_rhs;
     DevClock* clock = nullptr;
     double* part = nullptr;  // CFL per-block maxima, then sums partials (5 per block)
     double* out = nullptr;   // marker force (6) and sums (5)
@@ -68,6 +73,7 @@ struct efvm_ctx3 {
     double gx = 0.0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    int graph_cur = 0, graph_parity = 0;   // plane set and CFL slot the captured steps start from
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int timed = 0;
     std::string err;
@@ -131,8 +137,8 @@ void planes3(Args3& a, void* const* in, void* const* u0, void* const* out) {
 
 // the set holding the state of the current substage: A at a step boundary, else B / C
 int current_set(const efvm_ctx3* c) {
-    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 1) return (c->cur + 1) % 3;
-    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 2) return (c->cur + 2) % 3;
+    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 1) return c->prev;
+    if (c->sch.integrator == EFVM_INTEGRATOR_RK3 && c->rk_stage == 2) return 3 - c->cur - c->prev;
     return c->cur;
 }
 
@@ -164,9 +170,11 @@ void free_ibm3(efvm_ctx3* c) {
 // PreFlux (mirror) -> Flux .. Update (fused) -> PostUpdate (sponge).  Enqueue only.
 efvm_status run3(efvm_ctx3* c) {
     const bool rk3 = c->sch.integrator == EFVM_INTEGRATOR_RK3;
+    // roles: cur = U^n, prev = U^{n-1} (receives U1, then U^{n+1}), the third set U2; after the
+    // step cur and prev swap, so U^n survives the step (the reference's u0 afterwards)
     void* const* A = c->P[c->cur];
-    void* const* B = rk3 ? c->P[(c->cur + 1) % 3] : c->P[c->cur ^ 1];
-    void* const* Cc = c->P[(c->cur + 2) % 3];
+    void* const* B = c->P[c->prev];
+    void* const* Cc = c->P[3 - c->cur - c->prev];
     const int esz = (int)c->esz;
     Args3 a = args3(c);
     a.parity = c->parity;
@@ -174,7 +182,7 @@ efvm_status run3(efvm_ctx3* c) {
     void* const* in = A;
     void* const* out = B;
     if (rk3 && c->rk_stage == 1) { in = B; out = Cc; }
-    if (rk3 && c->rk_stage == 2) { in = Cc; out = A; }
+    if (rk3 && c->rk_stage == 2) { in = Cc; out = B; }
     planes3(a, in, A, out);
     if (c->rk_stage == 0 && c->sch.fixed_dt <= 0.0)
         LAUNCH3(c, launch3_cfl(a, esz, c->part, &c->clock->lam[c->parity], c->stream), "cfl");
@@ -186,9 +194,11 @@ efvm_status run3(efvm_ctx3* c) {
     if (c->n_body > 0 && c->n_ghost > 0)
         LAUNCH3(c, launch3_mirror(a, esz, c->n_ghost, c->ghost_idx, c->ghost_img, c->stream), "ibm mirror");
     LAUNCH3(c, launch3_stage(a, esz, c->stream), "stage");
+    c->rhs_state = 1;
     const bool last = !rk3 || c->rk_stage == 2;
     if (last) {
-        if (!rk3) c->cur ^= 1;
+        std::swap(c->cur, c->prev);
+        c->have_prev = 1;
         if (c->sponge) {
             Args3 b = args3(c);
             planes3(b, c->P[c->cur], c->P[c->cur], c->P[c->cur]);
@@ -502,6 +512,8 @@ efvm_status efvm3_upload_state(efvm_ctx3* c, const double* cons5, size_t n) {
     for (int m = 0; m < 5; m++)
         HIPCHK3(c, hipMemcpyAsync(c->P[c->cur][m], h[m].data(), c->esz * c->nraw, hipMemcpyHostToDevice, c->stream));
     HIPCHK3(c, hipStreamSynchronize(c->stream));
+    c->have_prev = 0;
+    c->rhs_state = 0;
     return EFVM_OK;
 }
 
@@ -603,7 +615,7 @@ efvm_status efvm3_graph_capture(efvm_ctx3* c, int n_steps) {
         return fail3(c, EFVM_ERR_ARG, "graph capture after a body is added needs one eager step first");
     drop_graph3(c);
     HIPCHK3(c, hipStreamSynchronize(c->stream));
-    const int cur0 = c->cur, par0 = c->parity;
+    const int cur0 = c->cur, prev0 = c->prev, par0 = c->parity, have0 = c->have_prev, rhs0 = c->rhs_state;
     HIPCHK3(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     efvm_status st = EFVM_OK;
     const int nsub = c->sch.integrator == EFVM_INTEGRATOR_RK3 ? 3 : 1;
@@ -615,12 +627,21 @@ efvm_status efvm3_graph_capture(efvm_ctx3* c, int n_steps) {
     if (e != hipSuccess) return fail3(c, EFVM_ERR_HIP, std::string("end capture: ") + hipGetErrorString(e));
     c->graph = g;
     HIPCHK3(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
-    c->cur = cur0; c->parity = par0; c->rk_stage = 0;
+    c->cur = cur0; c->prev = prev0; c->parity = par0; c->rk_stage = 0; c->have_prev = have0; c->rhs_state = rhs0;
+    c->graph_cur = cur0; c->graph_parity = par0;
     return EFVM_OK;
 }
 
 efvm_status efvm3_graph_launch(efvm_ctx3* c) {
     if (!c || !c->graph_exec) return fail3(c, EFVM_ERR_ARG, "no captured graph");
+    // the captured steps start from the plane set and CFL slot of the capture (each step scans
+    // its own CFL, so an upload needs nothing more): refuse to start mid-step or elsewhere
+    if (c->rk_stage != 0) return fail3(c, EFVM_ERR_ARG, "graph replay must start at a step boundary");
+    if (c->cur != c->graph_cur || c->parity != c->graph_parity)
+        return fail3(c, EFVM_ERR_ARG, "the state is on another plane set or CFL slot than at capture (an odd number "
+                                      "of eager steps since): capture the graph again");
+    c->have_prev = 1;
+    c->rhs_state = 1;
     HIPCHK3(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK3(c, hipGraphLaunch(c->graph_exec, c->stream));
     HIPCHK3(c, hipEventRecord(c->ev1, c->stream));
@@ -676,27 +697,80 @@ efvm_status efvm3_marker_force(efvm_ctx3* c, double out[6]) {
 
 // CFDRST01 (examples/restart_determinism/main.rs:177-226): "CFDRST01", dt, time (f64), step
 // (u64), rk stage (u8), then the columns u, rhs, u0, each a u64 cell count and 5 f64 per cell
-// over the raw padded array, little-endian.  The 3-D context stores that array, so u carries
-// its ghost cells as they are; rhs is written as 0 (the PreFlux phase clears it) and u0 is the
-// RK backup U^n.
+// over the raw padded array, little-endian.  The 3-D context stores that array, ghost cells
+// included, so every column is the reference's:
+//   u    the current substage's state;
+//   rhs  the last substage's right-hand side (crates/cfd_solver/src/lib.rs:79-88 clears it only
+//        at the next PreFlux), recomputed from that substage's input set, whose ghost and IBM
+//        mirror cells the substage filled in place; zero before any substage;
+//   u0   save_state's copy taken at the first substage (crates/cfd_solver/src/integrate.rs:76-78):
+//        U^n mid-step, the state before the completed step at a step boundary (kept by the
+//        plane-set rotation); zero before the first step and for Euler, which never saves.
+static efvm_status read_set3(efvm_ctx3* c, int set, double* dst) {
+    const long nraw = c->nraw;
+    std::vector<char> h[5];
+    for (int m = 0; m < 5; m++) {
+        h[m].resize(c->esz * nraw);
+        HIPCHK3(c, hipMemcpyAsync(h[m].data(), c->P[set][m], c->esz * nraw, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    for (long q = 0; q < nraw; q++)
+        for (int m = 0; m < 5; m++)
+            dst[5 * q + m] = c->esz == 8 ? reinterpret_cast<double*>(h[m].data())[q]
+                                         : (double)reinterpret_cast<float*>(h[m].data())[q];
+    return EFVM_OK;
+}
+
+static efvm_status rhs_column3(efvm_ctx3* c, double* dst) {
+    const long nraw = c->nraw;
+    if (c->rhs_state == 0) { std::fill(dst, dst + 5 * nraw, 0.0); return EFVM_OK; }
+    if (c->rhs_state == 2) { std::copy(c->loaded_rhs.begin(), c->loaded_rhs.end(), dst); return EFVM_OK; }
+    const bool rk3 = c->sch.integrator == EFVM_INTEGRATOR_RK3;
+    // input of the last substage: U2 (the third set) after stage 3, U^{n} (now prev) after an
+    // Euler step, U^n (cur) after stage 1, U1 (prev) after stage 2
+    int src = c->prev;
+    if (c->rk_stage == 0 && rk3) src = 3 - c->cur - c->prev;
+    if (c->rk_stage == 1) src = c->cur;
+    void* tmp[5] = {};
+    efvm_status st = EFVM_OK;
+    for (int m = 0; m < 5 && st == EFVM_OK; m++) st = alloc3(c, &tmp[m], c->esz * nraw);
+    if (st == EFVM_OK) {
+        Args3 a = args3(c);
+        a.parity = c->parity;
+        a.stage = 0;
+        a.rhs_only = 1;
+        planes3(a, c->P[src], c->P[src], tmp);
+        hipError_t e = launch3_stage(a, (int)c->esz, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) st = fail3(c, EFVM_ERR_HIP, std::string("rhs column: ") + hipGetErrorString(e));
+    }
+    if (st == EFVM_OK) {
+        std::vector<char> h[5];
+        for (int m = 0; m < 5 && st == EFVM_OK; m++) {
+            h[m].resize(c->esz * nraw);
+            if (hipMemcpy(h[m].data(), tmp[m], c->esz * nraw, hipMemcpyDeviceToHost) != hipSuccess)
+                st = fail3(c, EFVM_ERR_HIP, "rhs column copy");
+        }
+        if (st == EFVM_OK)
+            for (long q = 0; q < nraw; q++)
+                for (int m = 0; m < 5; m++)
+                    dst[5 * q + m] = c->esz == 8 ? reinterpret_cast<double*>(h[m].data())[q]
+                                                 : (double)reinterpret_cast<float*>(h[m].data())[q];
+    }
+    for (int m = 0; m < 5; m++) if (tmp[m]) (void)hipFree(tmp[m]);
+    return st;
+}
+
 efvm_status efvm3_checkpoint_save(efvm_ctx3* c, const char* path) {
     if (!c || !path) return EFVM_ERR_ARG;
     const long nraw = c->nraw;
-    std::vector<double> u(5 * nraw), u0(5 * nraw);
+    std::vector<double> u(5 * nraw), rhs(5 * nraw), u0(5 * nraw, 0.0);
     efvm_status st = efvm3_download_state(c, u.data(), nraw);    // the current substage's state
     if (st != EFVM_OK) return st;
-    {
-        std::vector<char> h[5];
-        for (int m = 0; m < 5; m++) {
-            h[m].resize(c->esz * nraw);
-            HIPCHK3(c, hipMemcpyAsync(h[m].data(), c->P[c->cur][m], c->esz * nraw, hipMemcpyDeviceToHost, c->stream));
-        }
-        HIPCHK3(c, hipStreamSynchronize(c->stream));
-        for (long q = 0; q < nraw; q++)
-            for (int m = 0; m < 5; m++)
-                u0[5 * q + m] = c->esz == 8 ? reinterpret_cast<double*>(h[m].data())[q]
-                                            : (double)reinterpret_cast<float*>(h[m].data())[q];
-    }
+    if ((st = rhs_column3(c, rhs.data())) != EFVM_OK) return st;
+    const bool rk3 = c->sch.integrator == EFVM_INTEGRATOR_RK3;
+    const int u0_set = !rk3 ? -1 : (c->rk_stage != 0 ? c->cur : (c->have_prev ? c->prev : -1));
+    if (u0_set >= 0 && (st = read_set3(c, u0_set, u0.data())) != EFVM_OK) return st;
     double dt = 0.0, time = 0.0;
     long step = 0;
     int stage = 0;
@@ -707,8 +781,7 @@ efvm_status efvm3_checkpoint_save(efvm_ctx3* c, const char* path) {
     const unsigned char stage_u = (unsigned char)stage;
     bool ok = std::fwrite("CFDRST01", 1, 8, f) == 8 && std::fwrite(&dt, 8, 1, f) == 1 && std::fwrite(&time, 8, 1, f) == 1 &&
               std::fwrite(&step_u, 8, 1, f) == 1 && std::fwrite(&stage_u, 1, 1, f) == 1;
-    const std::vector<double> zero(5 * nraw, 0.0);
-    const double* cols[3] = {u.data(), zero.data(), u0.data()};
+    const double* cols[3] = {u.data(), rhs.data(), u0.data()};
     for (int k = 0; k < 3 && ok; k++)
         ok = std::fwrite(&len, 8, 1, f) == 1 && std::fwrite(cols[k], sizeof(double), 5 * nraw, f) == (size_t)(5 * nraw);
     const int cl = std::fclose(f);
@@ -730,7 +803,7 @@ efvm_status efvm3_checkpoint_load(efvm_ctx3* c, const char* path) {
     unsigned long long step = 0, len = 0;
     unsigned char stage = 0;
     const long nraw = c->nraw;
-    std::vector<double> u(5 * nraw), skip(5 * nraw);
+    std::vector<double> u(5 * nraw), rhs(5 * nraw), u0(5 * nraw);
     efvm_status st = EFVM_OK;
     if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "CFDRST01", 8) != 0) st = EFVM_ERR_CONFIG;
     else if (std::fread(&dt, 8, 1, f) != 1 || std::fread(&time, 8, 1, f) != 1 || std::fread(&step, 8, 1, f) != 1 ||
@@ -740,7 +813,7 @@ efvm_status efvm3_checkpoint_load(efvm_ctx3* c, const char* path) {
     else if (stage != 0) st = EFVM_ERR_UNSUPPORTED;
     for (int k = 0; k < 3 && st == EFVM_OK; k++) {
         if (std::fread(&len, 8, 1, f) != 1 || (long)len != nraw) { st = EFVM_ERR_CONFIG; break; }
-        double* dst = k == 0 ? u.data() : skip.data();
+        double* dst = k == 0 ? u.data() : (k == 1 ? rhs.data() : u0.data());
         if (std::fread(dst, sizeof(double), 5 * nraw, f) != (size_t)(5 * nraw)) st = EFVM_ERR_CONFIG;
     }
     char extra;
@@ -749,6 +822,24 @@ efvm_status efvm3_checkpoint_load(efvm_ctx3* c, const char* path) {
     if (st == EFVM_ERR_UNSUPPORTED) return fail3(c, st, "checkpoint: restarts are taken at a step boundary (rk stage 0)");
     if (st != EFVM_OK) return fail3(c, st, "checkpoint: not a CFDRST01 file of this grid (magic, header, cell counts or length)");
     if ((st = efvm3_upload_state(c, u.data(), nraw)) != EFVM_OK) return st;
+    // u0 (the state before the file's last step) goes to the prev set and rhs is kept on the host:
+    // a checkpoint written before the next substage carries both unchanged
+    {
+        std::vector<char> h[5];
+        for (int m = 0; m < 5; m++) {
+            h[m].resize(c->esz * nraw);
+            for (long q = 0; q < nraw; q++) {
+                const double v = u0[5 * q + m];
+                if (c->esz == 8) reinterpret_cast<double*>(h[m].data())[q] = v;
+                else reinterpret_cast<float*>(h[m].data())[q] = (float)v;
+            }
+            HIPCHK3(c, hipMemcpyAsync(c->P[c->prev][m], h[m].data(), c->esz * nraw, hipMemcpyHostToDevice, c->stream));
+        }
+        HIPCHK3(c, hipStreamSynchronize(c->stream));
+    }
+    c->have_prev = 1;
+    c->loaded_rhs.swap(rhs);
+    c->rhs_state = 2;
     drop_graph3(c);
     HIPCHK3(c, launch3_clock_set(c->clock, dt, time, (long long)step, c->stream));
     HIPCHK3(c, hipStreamSynchronize(c->stream));

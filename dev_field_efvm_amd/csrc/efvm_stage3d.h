@@ -30,6 +30,7 @@ struct Args3 {
     double gx;
     const uint8_t* solid;  // raw-indexed mask, or null
     int euler, stage, parity;
+    int rhs_only;          // 1: write the right-hand side to uout instead of the RK update (no clock change)
     double cfl, fixed_dt;
     DevClock* clock;
     void* uin[5];

@@ -351,6 +351,10 @@ __device__ __forceinline__ void finish_cell(const Args3& a, const T* const* U, c
         }
         if (solid_c) rhs = V5<T>{T(0), T(0), T(0), T(0), T(0)};
     }
+    if (a.rhs_only) {   // the substage's rhs column (zero on ghost and solid cells), for checkpoints
+        st5(O, q, rhs);
+        return;
+    }
     const V5<T> inc = ref::add(uc, ref::mul(rhs, dt));
     V5<T> o;
     if (a.euler || a.stage == 0) o = inc;
@@ -368,7 +372,7 @@ template <class T, int FLUX, bool MUSCL>
 __global__ __launch_bounds__(256) void stage3_kernel(Args3 a) {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const double dtd = dt_of(a);
-    if (q == 0 && a.stage == 0) a.clock->dt = dtd;
+    if (q == 0 && a.stage == 0 && !a.rhs_only) a.clock->dt = dtd;
     if (q >= a.nraw) return;
     const T* const* U = (const T* const*)a.uin;
     const int i = (int)(q % a.NX);
@@ -395,7 +399,7 @@ __global__ __launch_bounds__(256) void stage3_kernel(Args3 a) {
         }
     }
     finish_cell<T>(a, U, (const T* const*)a.u0, (T* const*)a.uout, q, id, interior, rhs, T(dtd));
-    if (q == 0 && (a.euler || a.stage == 2)) {
+    if (q == 0 && (a.euler || a.stage == 2) && !a.rhs_only) {
         a.clock->time += dtd;
         a.clock->step += 1;
     }

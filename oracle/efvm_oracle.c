@@ -347,6 +347,9 @@ int orc_set_threads(orc_sim *s, int n) {
 /* Raw (padded) state, Component wire order (rho, rho_u, rho_v, rho_w, rho_E) per cell. */
 void orc_set_raw(orc_sim *s, const double *src) { memcpy(s->u, src, sizeof(cv_t) * s->ntot); }
 void orc_get_raw(const orc_sim *s, double *dst) { memcpy(dst, s->u, sizeof(cv_t) * s->ntot); }
+/* The rhs and u0 columns over the raw array, as a CFDRST01 file holds them. */
+void orc_get_rhs_raw(const orc_sim *s, double *dst) { memcpy(dst, s->rhs, sizeof(cv_t) * s->ntot); }
+void orc_get_u0_raw(const orc_sim *s, double *dst) { memcpy(dst, s->u0, sizeof(cv_t) * s->ntot); }
 /* Interior cells only, i fastest, then j, then k; 5 values per cell. */
 void orc_set_interior(orc_sim *s, const double *src) {
     long q = 0;

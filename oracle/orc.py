@@ -75,7 +75,7 @@ def lib():
         L.orc_ntot.argtypes = [C.c_void_p]
         L.orc_set_threads.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
-        for n in ("orc_set_raw", "orc_get_raw", "orc_set_interior", "orc_get_interior", "orc_get_rhs_interior",
+        for n in ("orc_set_raw", "orc_get_raw", "orc_get_rhs_raw", "orc_get_u0_raw", "orc_set_interior", "orc_get_interior", "orc_get_rhs_interior",
                   "orc_marker_force", "orc_wall_force", "orc_sums"):
             getattr(L, n).argtypes = [C.c_void_p, pd]
         L.orc_info.argtypes = [C.c_void_p, pd, pd, C.POINTER(C.c_long), C.POINTER(C.c_int)]
@@ -307,6 +307,12 @@ class Sim:
     def raw(self) -> np.ndarray:
         out = np.zeros((self.nz + 4, self.ny + 4, self.nx + 4, 5))
         lib().orc_get_raw(self.ptr, _p(out))
+        return out
+
+    def raw_column(self, name: str) -> np.ndarray:
+        """The rhs or u0 column over the raw padded array (shape (nz+4, ny+4, nx+4, 5))."""
+        out = np.zeros((self.nz + 4, self.ny + 4, self.nx + 4, 5))
+        getattr(lib(), f"orc_get_{name}_raw")(self.ptr, _p(out))
         return out
 
     def set_raw(self, a: np.ndarray):

@@ -98,3 +98,28 @@ def test_batch_rejects_bad_configs():
     with pytest.raises(efvm.EfvmError) as e:
         efvm.Batch([spec])
     assert e.value.status == 2
+
+
+def test_batch_graph_replay_after_upload_and_odd_steps():
+    """The batch graph replays from the current state: an upload is followed by a fresh CFL scan
+    of every case before the replay; an odd number of eager steps (another CFL slot) is refused."""
+    specs, states = small_ensemble(n_cases=6, seed=13)
+    states2 = [1.1 * st for st in states]          # denser, same velocity, 10% higher pressure
+    eager, graph = efvm.Batch(specs), efvm.Batch(specs)
+    for i, st in enumerate(states):
+        eager.upload(i, st)
+        graph.upload(i, st)
+    graph.graph_capture(2)
+    graph.graph_launch()
+    eager.step(2)
+    for i, st in enumerate(states2):
+        eager.upload(i, st)
+        graph.upload(i, st)
+    graph.graph_launch()
+    eager.step(2)
+    for i in range(len(specs)):
+        assert np.array_equal(eager.download(i), graph.download(i)), i
+        assert eager.clock(i)["dt"] == graph.clock(i)["dt"]
+    graph.step(1)
+    with pytest.raises(efvm.EfvmError):
+        graph.graph_launch()

@@ -7,6 +7,7 @@ import pytest
 
 import dev_field_efvm_amd as efvm
 from dev_field_efvm_amd import cases
+from oracle import orc
 
 pytestmark = pytest.mark.gpu
 
@@ -40,17 +41,10 @@ def test_restored_run_continues_bit_for_bit(tmp_path, precision):
     assert not np.array_equal(ref.download(), cases.vortex_modes_state(n, n, seed=3))
 
 
-def test_checkpoint_layout(tmp_path):
-    n = 24
-    c = ctx(n, "fp64")
-    c.step(3)
-    path = str(tmp_path / "ck.bin")
-    c.checkpoint_save(path)
+def read_columns(path, n):
     data = open(path, "rb").read()
     assert data[:8] == b"CFDRST01"
-    dt, t, step, stage = struct.unpack_from("<ddQB", data, 8)
-    clk = c.clock()
-    assert (dt, t, step, stage) == (clk["dt"], clk["time"], 3, 0)
+    head = struct.unpack_from("<ddQB", data, 8)
     pos = 8 + 25
     nraw = (n + 4) * (n + 4) * 5
     cols = []
@@ -60,14 +54,58 @@ def test_checkpoint_layout(tmp_path):
         cols.append(np.frombuffer(data, dtype="<f8", count=5 * nraw, offset=pos + 8).reshape(5, n + 4, n + 4, 5))
         pos += 8 + 40 * nraw
     assert pos == len(data)
-    u, rhs, u0 = cols
+    return head, cols
+
+
+def test_checkpoint_layout(tmp_path):
+    """u over the raw array with the device's interior; u0 is the reference's save_state copy
+    (crates/cfd_solver/src/integrate.rs:76-78): the state before the completed step at a step
+    boundary, U^n mid-step.  The 2-D device keeps no ghost cells and no rhs column, so those are
+    written as zeros (the reference refills both before their next use)."""
+    n = 24
+    c = ctx(n, "fp64")
+    c.step(2)
+    before = c.download()
+    c.step(1)
+    path = str(tmp_path / "ck.bin")
+    c.checkpoint_save(path)
+    (dt, t, step, stage), (u, rhs, u0) = read_columns(path, n)
+    clk = c.clock()
+    assert (dt, t, step, stage) == (clk["dt"], clk["time"], 3, 0)
     interior = u[2, 2:n + 2, 2:n + 2]
     assert np.array_equal(interior, c.download())
     ghost = u.copy()
     ghost[2, 2:n + 2, 2:n + 2] = 0.0
     assert not ghost.any()                          # ghosts are written as zero
     assert not rhs.any()
-    assert np.array_equal(u0[2, 2:n + 2, 2:n + 2], interior)   # at a step boundary the backup is the state
+    assert np.array_equal(u0[2, 2:n + 2, 2:n + 2], before)
+    # against the oracle: its u0 column after the same three steps
+    sim = orc.Sim(n, n, 1, hi=(10.0, 10.0, 10.0 / n), flux="hllc", muscl=True, limiter="mc", cfl=0.4, seam_mode=1,
+                  bcs=["periodic"] * 4 + ["slip", "slip"])
+    sim.set_interior(cases.vortex_modes_state(n, n, seed=3).reshape(1, n, n, 5))
+    sim.step(3)
+    ref_u0 = sim.raw_column("u0")[2, 2:n + 2, 2:n + 2]
+    err = max(float(np.max(np.abs(u0[2, 2:n + 2, 2:n + 2, m] - ref_u0[..., m])) / np.max(np.abs(ref_u0[..., m])))
+              for m in (0, 1, 2, 4))
+    print(f"u0 column vs oracle rel L-inf {err:.2e}")
+    assert err <= 1e-12
+    # mid-step: after one substage the backup is U^n
+    c.run()
+    mid = str(tmp_path / "mid.bin")
+    c.checkpoint_save(mid)
+    (_, _, _, stage), (u_m, _, u0_m) = read_columns(mid, n)
+    assert stage == 1 and np.array_equal(u0_m[2, 2:n + 2, 2:n + 2], interior)
+    # load then save: the u0 column survives the round trip; the first step replaces it
+    d = ctx(n, "fp64")
+    d.checkpoint_load(path)
+    again = str(tmp_path / "again.bin")
+    d.checkpoint_save(again)
+    assert open(again, "rb").read() == open(path, "rb").read()
+    # a fresh state (no step yet) carries a zero backup, as a new CfdState does
+    e = ctx(n, "fp64")
+    fresh = str(tmp_path / "fresh.bin")
+    e.checkpoint_save(fresh)
+    assert not read_columns(fresh, n)[1][2].any()
 
 
 def test_checkpoint_load_rejects_other_grids_and_mid_step_files(tmp_path):

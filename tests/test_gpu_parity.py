@@ -264,8 +264,9 @@ def test_poiseuille_viscous_noslip_body_force_fp64():
 
 
 def test_viscous_sutherland_muscl_fp64():
-    """Sutherland mu(T) uses pow(): device and host pow may differ in the last bit, so this
-    one case is held to 1e-10 rather than 1e-12 (documented tolerance)."""
+    """Sutherland mu(T) through pow(): the device's pow rounds like the host's on these
+    temperatures (measured: the states agree bit for bit after 20 steps), so the case is held
+    to the common 1e-12 bound."""
     n = 40
     sim, ctx = make_pair(n, n, (1.0, 1.0, 0.025), ["noslip", "noslip", "noslip", "noslip"], viscous=True,
                          muscl=True, limiter="mc", cfl=0.3)
@@ -274,7 +275,9 @@ def test_viscous_sutherland_muscl_fp64():
     load_both(sim, ctx, st)
     sim.step(20)
     ctx.step(20)
-    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= 1e-10
+    err = rel_linf(gpu_state(ctx), ora_state(sim))
+    print(f"sutherland viscous MUSCL rel L-inf {err:.3e}")
+    assert err <= TOL64
 
 
 # ------------------------------------------------------------------ cylinder (ghost-cell IBM)
@@ -394,6 +397,55 @@ def test_graph_replay_equals_eager():
             a, b = eager.download(), graph.download()
             assert np.array_equal(a, b), (prec, rep)
             assert eager.clock()["time"] == graph.clock()["time"]
+
+
+def test_graph_replay_after_upload_and_eager_work():
+    """A captured graph replays from the state the context holds now: after an upload (the CFL
+    maximum of the new state is computed first), after eager whole steps, and it refuses to start
+    mid-step or from the other Euler plane set, where it would read the wrong planes or CFL slot."""
+    n = 96
+    st1, st2 = cases.vortex_modes_state(n, n, seed=11), cases.vortex_modes_state(n, n, seed=12)
+    for prec in ("fp64", "fp32"):
+        graph = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc", precision=prec)
+        eager = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc", precision=prec)
+        for c in (graph, eager):
+            c.set_boundaries(["periodic"] * 4)
+            c.upload(st1)
+        graph.graph_capture(2)
+        graph.graph_launch()
+        eager.step(2)
+        # a new state: the replayed first step must take its dt from this state
+        graph.upload(st2)
+        eager.upload(st2)
+        graph.graph_launch()
+        eager.step(2)
+        assert np.array_equal(graph.download(), eager.download()), prec
+        assert graph.clock()["dt"] == eager.clock()["dt"]
+        # eager whole steps in between (an odd count moves RK3's CFL slot: refused, then an even count)
+        graph.step(1)
+        eager.step(1)
+        with pytest.raises(efvm.EfvmError):
+            graph.graph_launch()
+        graph.step(1)
+        eager.step(1)
+        graph.graph_launch()
+        eager.step(2)
+        assert np.array_equal(graph.download(), eager.download()), prec
+        # a lone substage leaves the context mid-step: refused
+        graph.run()
+        with pytest.raises(efvm.EfvmError):
+            graph.graph_launch()
+        graph.close()
+        eager.close()
+    # Euler: an odd number of eager steps leaves U^n on the other plane set
+    e = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, integrator="euler", precision="fp64")
+    e.set_boundaries(["periodic"] * 4)
+    e.upload(st1)
+    e.graph_capture(2)
+    e.step(1)
+    with pytest.raises(efvm.EfvmError):
+        e.graph_launch()
+    e.close()
 
 
 def test_run_substages_equal_step():
