@@ -61,7 +61,8 @@ def _time_ctx(ctx, steps):
 
 def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=True, verbose=False, store=True):
     """Pick (strip_rows, waves_per_block) for a context; returns (rows, waves, ms_per_step)."""
-    key = f"{arch_name()}:{ctx.precision}:ctx:{size_class(ctx.nx)}x{size_class(ctx.ny)}"
+    kernel = ctx.launch()[2].replace(" ", "-")
+    key = f"{arch_name()}:{ctx.precision}:ctx:{size_class(ctx.nx)}x{size_class(ctx.ny)}:{kernel}"
     db = _load()
     if use_cache and key in db:
         r, w = db[key]["rows"], db[key]["waves"]

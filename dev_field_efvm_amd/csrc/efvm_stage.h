@@ -29,6 +29,7 @@ namespace efvm {
 
 constexpr int WAVE = 64;
 constexpr int STRIP_OUT = WAVE - 4;   // output columns per strip
+constexpr int STRIP_OUT2 = 2 * WAVE - 4;   // output columns per strip of the two-column periodic path
 
 struct DevClock {
     double lam[2];        // CFL spectral-radius maxima (ping-pong by step parity), as doubles
@@ -72,7 +73,19 @@ struct StageArgs {
     int xcd_remap;
     double* lam_part;            // FP32 fused CFL: per-wave maxima (null: atomics on the clock)
     unsigned* face_count;        // learned bypass: deferred-face count, cleared by the stage kernel
+    unsigned plane_stride;       // bytes from one plane of a set to the next when the four planes of
+                                 // every set lie at a common stride (0: they do not)
+    int pair_path;               // FP32: the two-column periodic strip kernel runs this stage
+                                 // (n_strips_x then counts strips of STRIP_OUT2 columns)
 };
+
+// The two-column periodic strip kernel applies: both axes periodic, no boundary features, an
+// even nx (column pairs never straddle the seam), at least four cells per axis (one wrap per
+// row or column index), and each plane set addressable by one buffer descriptor of under 2 GiB.
+inline bool pair_path_fits(int nx, int ny, bool per_x, bool per_y, bool no_features, unsigned plane_stride) {
+    return per_x && per_y && no_features && nx % 2 == 0 && nx >= 4 && ny >= 4 && plane_stride != 0 &&
+           (unsigned long long)plane_stride * 4ull < 0x80000000ull;
+}
 
 // One case of a batched ensemble: its stage arguments and plane sets A, B, C.
 struct BatchCase {

@@ -388,6 +388,218 @@ __global__ __launch_bounds__(256) void stage_f32_kernel(StageArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Two-column periodic strips: both axes periodic, no boundary features, nx even, and the
+// four planes of every set at one stride, so that each set is one buffer descriptor and a
+// plane is selected by the scalar offset.  A wave owns 128 consecutive columns, two per
+// lane (c0 = x0 + 2 lane - 2 and c0 + 1), and writes the 124 columns of lanes 1..62.  Per
+// row step a lane evaluates the x face between its two columns without any lane exchange,
+// the face west of c0 from the east state of the left lane's second column (one DPP move
+// per value), and the y faces of both columns: the lane exchanges of a cell halve, and so
+// do the halo lanes.  Every cell's arithmetic is stage_f32_strip's, expression for
+// expression (the same slopes, face states, fluxes, divergence, RK combine and CFL), so
+// the results are bitwise those of the one-column kernel.
+// ---------------------------------------------------------------------------
+struct Pair { F4 a, b; };   // the lane's two columns
+struct PrepPair { Prep a, b; };
+
+__device__ __forceinline__ Pair sub_pair(const Pair& x, const Pair& y) { return {sub4(x.a, y.a), sub4(x.b, y.b)}; }
+
+template <int FLUX, int LIM>
+__device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const int s, const float* U, const float* U0,
+                                                     float* O, const int stage, const int parity) {
+    constexpr bool MUSCL = LIM != LIM_NONE;
+    const int lane = lane_id();
+    const int nx = a.nx, ny = a.ny;
+    const int sxi = s % a.n_strips_x, syi = s / a.n_strips_x;
+    const int c0 = sxi * STRIP_OUT2 - 2 + 2 * lane;     // even: the pair never straddles the seam (nx even)
+    const int j0 = syi * a.strip_h;
+    const int j1 = min(j0 + a.strip_h, ny);
+    const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0);
+    const double dtd = a.fixed_dt > 0.0 ? a.fixed_dt : a.cfl / a.clock->lam[parity];
+    const float dt = (float)dtd;
+    const bool fuse = a.fuse_cfl && (a.euler || stage == 2);
+    if (s == 0 && lane == 0 && stage == 0) { a.clock->dt = dtd; a.clock->lam[parity ^ 1] = 0.0; }
+    float ca, cb;
+    if (a.euler || stage == 0) { ca = 0.f; cb = 1.f; }
+    else if (stage == 1) { ca = 0.75f; cb = 0.25f; }
+    else { ca = 1.f / 3.f; cb = 2.f / 3.f; }
+    const float dtix = dt * (float)a.inv_x, dtiy = dt * (float)a.inv_y;
+
+    // columns: the periodic image for loads; columns outside the mesh are seam ghosts, which
+    // seam mode 0 leaves unreconstructed (as stage_f32_strip does)
+    int cq0 = c0 % nx;
+    if (cq0 < 0) cq0 += nx;
+    const unsigned ld_off = (unsigned)cq0 * 4u;
+    const bool seam_rec = a.seam_mode == 1;
+    const bool ga = c0 < 0 || c0 >= nx, gb = c0 + 1 < 0 || c0 + 1 >= nx;
+    const float hWa = MUSCL && (!ga || seam_rec) ? -1.f : 0.f, hEa = MUSCL && (!ga || seam_rec) ? 1.f : 0.f;
+    const float hWb = MUSCL && (!gb || seam_rec) ? -1.f : 0.f, hEb = MUSCL && (!gb || seam_rec) ? 1.f : 0.f;
+    const bool out_lane = lane >= 1 && lane <= WAVE - 2 && c0 < nx;
+    // stores and U0 loads of lanes that write nothing take an offset of 2^31, past the set (under
+    // 2^31 bytes) with or without the scalar offset added, so the descriptor drops them
+    const unsigned st_off = out_lane ? (unsigned)c0 * 4u : 0x80000000u;
+    const unsigned ps = a.plane_stride, set_bytes = 4u * a.plane_stride, rowb = (unsigned)nx * 4u;
+
+    // rows: one wrap suffices (ny >= 4 and rows span j0 - 2 .. j1 + 3)
+    auto wrap_row = [&](int r) -> unsigned {
+        if (r < 0) r += ny;
+        else if (r >= ny) r -= ny;
+        return (unsigned)r * rowb;
+    };
+    // a row of the two columns from a set; `live` false issues the loads through a zero-length
+    // descriptor (they return 0), so every row step has the same memory operations
+    auto ld_pair = [&](const float* set, unsigned roff, unsigned voff, bool live) -> Pair {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)set, 0, live ? (int)set_bytes : 0, BUF_DWORD3);
+        Pair p;
+        float v[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const float2 x = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, roff + (unsigned)m * ps, 0));
+            v[m][0] = x.x;
+            v[m][1] = x.y;
+        }
+        p.a = {v[0][0], v[1][0], v[2][0], v[3][0]};
+        p.b = {v[0][1], v[1][1], v[2][1], v[3][1]};
+        return p;
+    };
+    auto load_row = [&](int r, bool live) -> Pair { return ld_pair(U, wrap_row(r), ld_off, live); };
+
+    float lam_max = 0.f;
+    int bad = 0;
+
+    // y reconstruction of row jc for both columns
+    auto y_states = [&](int jc, const Pair& uc, const Pair& fwd, const Pair& bwd, PrepPair& lo_p, PrepPair& hi_p) {
+        if (MUSCL) {
+            const bool row_ghost = (jc < 0 || jc >= ny);
+            const bool rec = !row_ghost || seam_rec;
+            const float hS = rec ? -1.f : 0.f;
+            const float hN = rec ? 1.f : 0.f;
+            const F4 sa = half_slope4<LIM>(fwd.a, bwd.a), sb = half_slope4<LIM>(fwd.b, bwd.b);
+            lo_p.a = fast::prep(axpy4(uc.a, hS, sa), g, gm1);
+            hi_p.a = fast::prep(axpy4(uc.a, hN, sa), g, gm1);
+            lo_p.b = fast::prep(axpy4(uc.b, hS, sb), g, gm1);
+            hi_p.b = fast::prep(axpy4(uc.b, hN, sb), g, gm1);
+        } else {
+            lo_p.a = fast::prep(uc.a, g, gm1);
+            lo_p.b = fast::prep(uc.b, g, gm1);
+            hi_p = lo_p;
+        }
+    };
+
+    // one row step (the roles of stage_f32_strip's step, for the pair)
+    auto step = [&](int jc, Pair& um, const Pair& uc, const Pair& up, Pair& din, Pair& dout, PrepPair& hin,
+                    PrepPair& hout, Pair& gin, Pair& gout, Pair& xin, Pair& xout, const Pair& u0in, Pair& u0out) {
+        const bool full = jc < j1;
+        const bool out = jc > j0 && jc <= j1;
+        u0out = ld_pair(U0, (unsigned)jc * rowb, st_off, full && ca != 0.f);
+        dout = sub_pair(up, uc);
+        PrepPair lo_p;
+        y_states(jc, uc, dout, din, lo_p, hout);
+        gout.a = fast::flux<FLUX, 1>(hin.a, lo_p.a);
+        gout.b = fast::flux<FLUX, 1>(hin.b, lo_p.b);
+        if (full) {
+            Prep Wa, Ea, Wb, Eb;
+            if (MUSCL) {
+                const F4 uL = from_left4(uc.b), uR = from_right4(uc.a);   // columns c0 - 1 and c0 + 2
+                const F4 d01 = sub4(uc.b, uc.a);
+                const F4 sa = half_slope4<LIM>(d01, sub4(uc.a, uL));
+                const F4 sb = half_slope4<LIM>(sub4(uR, uc.b), d01);
+                Wa = fast::prep(axpy4(uc.a, hWa, sa), g, gm1);
+                Ea = fast::prep(axpy4(uc.a, hEa, sa), g, gm1);
+                Wb = fast::prep(axpy4(uc.b, hWb, sb), g, gm1);
+                Eb = fast::prep(axpy4(uc.b, hEb, sb), g, gm1);
+            } else {
+                Wa = lo_p.a; Ea = lo_p.a; Wb = lo_p.b; Eb = lo_p.b;
+            }
+            const F4 Fmid = fast::flux<FLUX, 0>(Ea, Wb);             // the face between the pair
+            const F4 Fwa = fast::flux<FLUX, 0>(from_leftP(Eb), Wa);  // the face west of c0
+            const F4 Fe = from_right4(Fwa);                          // the face east of c0 + 1
+            xout.a = sub4(Fwa, Fmid);
+            xout.b = sub4(Fmid, Fe);
+        }
+        {
+            const bool live = out && out_lane;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)O, 0, out ? (int)set_bytes : 0, BUF_DWORD3);
+            const unsigned roff = (unsigned)(jc - 1) * rowb;
+            float o[4][2];
+            const F4* X[2] = {&xin.a, &xin.b};
+            const F4* G0[2] = {&gin.a, &gin.b};
+            const F4* G1[2] = {&gout.a, &gout.b};
+            const F4* UM[2] = {&um.a, &um.b};
+            const F4* Q[2] = {&u0in.a, &u0in.b};
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const F4 &xi = *X[k], &gi = *G0[k], &go = *G1[k], &uu = *UM[k], &u0 = *Q[k];
+                const float rr = xi.r * dtix + (gi.r - go.r) * dtiy;
+                const float rru = xi.ru * dtix + (gi.ru - go.ru) * dtiy;
+                const float rrv = xi.rv * dtix + (gi.rv - go.rv) * dtiy;
+                const float rre = xi.re * dtix + (gi.re - go.re) * dtiy;
+                if (ca != 0.f) {
+                    o[0][k] = u0.r + cb * ((uu.r - u0.r) + rr); o[1][k] = u0.ru + cb * ((uu.ru - u0.ru) + rru);
+                    o[2][k] = u0.rv + cb * ((uu.rv - u0.rv) + rrv); o[3][k] = u0.re + cb * ((uu.re - u0.re) + rre);
+                } else {
+                    o[0][k] = uu.r + rr; o[1][k] = uu.ru + rru; o[2][k] = uu.rv + rrv; o[3][k] = uu.re + rre;
+                }
+                if (fuse && live) {
+                    float l;
+                    if (lam_cell_f32<false>(a, o[0][k], o[1][k], o[2][k], o[3][k], l)) lam_max = fmaxf(lam_max, l);
+                    else bad++;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                typedef int v2i __attribute__((ext_vector_type(2)));
+                const v2i w = {__builtin_bit_cast(int, o[m][0]), __builtin_bit_cast(int, o[m][1])};
+                __builtin_amdgcn_raw_buffer_store_b64(w, rs, st_off, roff + (unsigned)m * ps, 0);
+            }
+        }
+        um = load_row(jc + 3, full && jc + 3 <= j1 + 1);
+    };
+
+    // prologue: rows j0-2 .. j0+2, north states of row j0-1
+    Pair R0 = load_row(j0 - 1, true), R1 = load_row(j0, true), R2 = load_row(j0 + 1, true), R3 = load_row(j0 + 2, true);
+    Pair D0, D1;
+    PrepPair H0, H1;
+    const F4 z4{0.f, 0.f, 0.f, 0.f};
+    Pair G0{z4, z4}, G1{z4, z4}, X0{z4, z4}, X1{z4, z4}, Q0{z4, z4}, Q1{z4, z4};
+    {
+        const Pair rm2 = load_row(j0 - 2, true);
+        D0 = sub_pair(R1, R0);
+        PrepPair lo_unused;
+        y_states(j0 - 1, R0, D0, sub_pair(R0, rm2), lo_unused, H0);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    for (int jc = j0; jc <= j1; jc += 4) {
+        step(jc, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, Q0, Q1);
+        step(jc + 1, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, Q1, Q0);
+        step(jc + 2, R2, R3, R0, D0, D1, H0, H1, G0, G1, X0, X1, Q0, Q1);
+        step(jc + 3, R3, R0, R1, D1, D0, H1, H0, G1, G0, X1, X0, Q1, Q0);
+    }
+
+    if (fuse) {
+        const double w = wave_max((double)lam_max);
+        const long long b = wave_sum_ll((long long)bad);
+        if (lane == 0) {
+            if (a.lam_part) a.lam_part[s] = w;
+            else if (w > 0.0) atomic_max_pos_double(&a.clock->lam[parity ^ 1], w);
+            if (b) atomicAdd((unsigned long long*)&a.clock->bad, (unsigned long long)b);
+        }
+    }
+    if (s == 0 && lane == 0 && (a.euler || stage == 2)) {
+        a.clock->time += dtd;
+        a.clock->step += 1;
+    }
+}
+
+template <int FLUX, int LIM>
+__global__ __launch_bounds__(256) void stage_f32_pair_kernel(StageArgs a) {
+    const int s = strip_index(a, blockDim.x / WAVE);
+    if (s >= a.n_strips_x * a.n_strips_y) return;
+    stage_f32_pair_strip<FLUX, LIM>(a, s, (const float*)a.uin[0], (const float*)a.u0[0], (float*)a.uout[0], a.stage,
+                                    a.parity);
+}
+
+// ---------------------------------------------------------------------------
 // Stage from precomputed face states (learned reconstruction, periodic grids).
 // F holds 16 planes of primitives, plane (axis * 2 + side) * 4 + var: the left /
 // right states of the east (axis 0) and north (axis 1) face of every cell, as the
@@ -533,6 +745,20 @@ hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st) {
     const int waves = block / WAVE;
     const int nstrips = a.n_strips_x * a.n_strips_y;
     const dim3 grid((nstrips + waves - 1) / waves);
+    if (a.pair_path) {
+        const int lim = a.muscl ? a.limiter : LIM_NONE;
+#define EFVM_PAIR(F)                                                                                     \
+        do {                                                                                             \
+            if (lim == LIM_MC) stage_f32_pair_kernel<F, LIM_MC><<<grid, block, 0, st>>>(a);              \
+            else if (lim == LIM_MINMOD) stage_f32_pair_kernel<F, LIM_MINMOD><<<grid, block, 0, st>>>(a); \
+            else stage_f32_pair_kernel<F, LIM_NONE><<<grid, block, 0, st>>>(a);                          \
+        } while (0)
+        if (a.flux == FLUX_RUSANOV) EFVM_PAIR(FLUX_RUSANOV);
+        else if (a.flux == FLUX_HLLC) EFVM_PAIR(FLUX_HLLC);
+        else EFVM_PAIR(FLUX_HLLC_TEXTBOOK);
+#undef EFVM_PAIR
+        return hipGetLastError();
+    }
     const StageArgs& a_ = a;
 #define EFVM_LAUNCH_ONE(F, L, X) stage_f32_kernel<F, L, X><<<grid, block, 0, st>>>(a)
     EFVM_F32_DISPATCH(EFVM_LAUNCH_ONE);

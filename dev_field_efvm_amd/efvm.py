@@ -24,7 +24,7 @@ BC = {"outflow": 0, "periodic": 1, "slip": 2, "noslip": 3, "symmetry": 4,
 
 # Every entry point include/efvm.h declares (checked by the CPU test suite).
 EXPORTS = [
-    "efvm_ctx_create", "efvm_ctx_destroy", "efvm_last_error", "efvm_set_launch", "efvm_set_boundaries",
+    "efvm_ctx_create", "efvm_ctx_destroy", "efvm_last_error", "efvm_set_launch", "efvm_get_launch", "efvm_set_boundaries",
     "efvm_set_parabolic_inflow", "efvm_set_sponge", "efvm_set_body_force", "efvm_set_time_step", "efvm_upload_state",
     "efvm_download_state", "efvm_get_clock", "efvm_set_clock", "efvm_cfl", "efvm_run", "efvm_step",
     "efvm_advance_to", "efvm_synchronize", "efvm_graph_capture", "efvm_graph_launch",
@@ -91,6 +91,7 @@ def load_library(path: str = LIB_PATH):
     sig = {
         "efvm_ctx_create": [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Scheme), C.POINTER(vp)],
         "efvm_set_launch": [vp, C.c_int, C.c_int],
+        "efvm_get_launch": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "efvm_set_boundaries": [vp, C.POINTER(Bc)],
         "efvm_set_parabolic_inflow": [vp, d, d, d],
         "efvm_set_sponge": [vp, d, d, d, d, d, d, d],
@@ -175,6 +176,14 @@ class Context:
 
     def set_launch(self, strip_rows=0, waves_per_block=0):
         self._chk(self._L.efvm_set_launch(self._p, strip_rows, waves_per_block))
+
+    KERNELS = ("one-column strips", "two-column periodic strips", "fp64 strips", "learned face kernels")
+
+    def launch(self):
+        """(strip_rows, waves_per_block, kernel name) of the stage kernel this context runs."""
+        r, w, k = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self._L.efvm_get_launch(self._p, C.byref(r), C.byref(w), C.byref(k)))
+        return r.value, w.value, self.KERNELS[k.value]
 
     def set_boundaries(self, sides):
         """sides: 4 x (kind, params) for XLo, XHi, YLo, YHi."""

@@ -89,6 +89,11 @@ const char* efvm_last_error(const efvm_ctx* ctx);
 /* Tile shape of the stage kernel (rows per wave strip, waves per workgroup);
  * 0 keeps the current value.  Used by the autotuner. */
 efvm_status efvm_set_launch(efvm_ctx* ctx, int strip_rows, int waves_per_block);
+/* The current launch shape and the stage kernel the context's configuration selects:
+ * *kernel = 0 one-column strips (64 columns per wave), 1 two-column periodic strips (128 per
+ * wave; FP32, both axes periodic, no boundary features, nx even), 2 FP64 strips, 3 learned
+ * face kernels.  Any pointer may be null.  No reference counterpart (a tuning query). */
+efvm_status efvm_get_launch(const efvm_ctx* ctx, int* strip_rows, int* waves_per_block, int* kernel);
 
 /* BoundaryRegistry + StructuredBoundaryPlugin (crates/cfd_boundary/src/lib.rs:369-533):
  * sides[4] = XLo, XHi, YLo, YHi.  A half-periodic axis is EFVM_ERR_CONFIG. */
