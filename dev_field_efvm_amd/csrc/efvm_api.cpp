@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -39,6 +40,7 @@ template <class T> hipError_t launch_marker_force(const StageArgs& sa, const Ibm
 template <class T> hipError_t launch_sponge(const StageArgs& a, double x0, double x1, double sig, double um, double h,
                                             double rho, double p, hipStream_t st);
 template <class T> hipError_t launch_sums(const StageArgs& a, double* part, int nb, double vol, double* out, hipStream_t st);
+template <class T> hipError_t launch_sums_partial(const StageArgs& a, double* part, int nb, hipStream_t st);
 hipError_t launch_ibm_flip(void* const* planes, int esz, const uint8_t* solid, long n, const double* stagnant, hipStream_t st);
 hipError_t launch_stage_f32_faces(const StageArgs& a, const float* F, hipStream_t st);
 hipError_t launch_stage_f32_flux(const StageArgs& a, const float* F, hipStream_t st);
@@ -819,6 +821,67 @@ efvm_status efvm_conservation_sums(efvm_ctx* c, double out[5]) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     out[0] = h[0]; out[1] = h[1]; out[2] = h[2]; out[3] = 0.0; out[4] = h[3];
     return EFVM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free conservation sums.  The per-block partials live in a fixed device array
+// (each GPU holds its own copy for the life of the process); its address is taken once per
+// device with that device current.  Host code uses only that address, never the array's name.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+namespace {
+constexpr int NOALLOC_BLOCKS = 1024;                     // partial-sum blocks (4 doubles each)
+static __device__ __align__(256) double scratch[NOALLOC_BLOCKS * 4];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+__global__ void sums5_final_kernel(const double* __restrict__ part, int nb, double vol, double* out5) {
+    if (threadIdx.x != 0) return;
+    double s[4] = {0, 0, 0, 0};
+    for (int b = 0; b < nb; b++) for (int m = 0; m < 4; m++) s[m] += part[4 * b + m];
+    out5[0] = s[0] * vol; out5[1] = s[1] * vol; out5[2] = s[2] * vol; out5[3] = 0.0; out5[4] = s[3] * vol;
+}
+
+// The device address of `scratch` on the current device (taken once per device).
+hipError_t scratch_ptr(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+extern "C" {
+
+efvm_status efvm_state_sums_noalloc(const void* const planes[4], int precision, long long n_cells, double cell_volume,
+                                    double* out5, void* stream) {
+    if (!planes || !out5 || !planes[0] || !planes[1] || !planes[2] || !planes[3]) return EFVM_ERR_ARG;
+    if (precision != EFVM_FP32 && precision != EFVM_FP64) return EFVM_ERR_ARG;
+    if (n_cells < 1 || n_cells > (long long)EFVM_NOALLOC_MAX_CELLS) return EFVM_ERR_ARG;
+    double* part = nullptr;
+    if (scratch_ptr(&part) != hipSuccess) return EFVM_ERR_HIP;
+    StageArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nx = (int)n_cells;
+    a.ny = 1;
+    for (int m = 0; m < 4; m++) a.uin[m] = planes[m];
+    const hipStream_t st = (hipStream_t)stream;
+    const int nb = (int)std::min<long long>(NOALLOC_BLOCKS, (n_cells + 255) / 256);
+    // every partial the final pass reads is written by this call: nothing to clear
+    hipError_t e = precision == EFVM_FP64 ? launch_sums_partial<double>(a, part, nb, st)
+                                          : launch_sums_partial<float>(a, part, nb, st);
+    if (e != hipSuccess) return EFVM_ERR_HIP;
+    sums5_final_kernel<<<1, 64, 0, st>>>(part, nb, cell_volume, out5);
+    return hipGetLastError() == hipSuccess ? EFVM_OK : EFVM_ERR_HIP;
 }
 
 efvm_status efvm_device_state(efvm_ctx* c, void* planes[4]) {

@@ -473,6 +473,15 @@ hipError_t launch_sums(const StageArgs& a, double* part, int nb, double vol, dou
 template hipError_t launch_sums<double>(const StageArgs&, double*, int, double, double*, hipStream_t);
 template hipError_t launch_sums<float>(const StageArgs&, double*, int, double, double*, hipStream_t);
 
+// The per-block partials alone (4 doubles per block into `part`), for callers that finish the sum themselves.
+template <class T>
+hipError_t launch_sums_partial(const StageArgs& a, double* part, int nb, hipStream_t st) {
+    sums_partial_kernel<T><<<nb, 256, 0, st>>>(a, part);
+    return hipGetLastError();
+}
+template hipError_t launch_sums_partial<double>(const StageArgs&, double*, int, hipStream_t);
+template hipError_t launch_sums_partial<float>(const StageArgs&, double*, int, hipStream_t);
+
 }  // namespace efvm
 
 // ---- cylinder mask refresh: newly solid cells take the stagnant state (cfd_ibm:155-183) ----

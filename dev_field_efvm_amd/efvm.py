@@ -30,8 +30,24 @@ EXPORTS = [
     "efvm_advance_to", "efvm_synchronize", "efvm_graph_capture", "efvm_graph_launch",
     "efvm_ibm_add_cylinder", "efvm_marker_force", "efvm_conservation_sums", "efvm_device_state",
     "efvm_last_elapsed_ms", "efvm_snapshot", "efvm_checkpoint_save", "efvm_checkpoint_load", "efvm_write_vtk",
-    "efvm_set_ghost_table",
+    "efvm_set_ghost_table", "efvm_state_sums_noalloc",
 ]
+
+
+def state_sums_noalloc(planes, precision: str, n_cells: int, cell_volume: float, out5_ptr: int, stream: int = 0):
+    """efvm_state_sums_noalloc: FP64 volume-weighted sums of four device planes (pointers, as
+    efvm_device_state gives them) into 5 doubles at device address `out5_ptr`, enqueued on
+    `stream` (a hipStream_t as an integer; 0 = the default stream).  No allocation, no
+    synchronisation: the caller synchronises before reading the output."""
+    L = load_library()
+    fn = L.efvm_state_sums_noalloc
+    fn.restype = C.c_int
+    fn.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_longlong, C.c_double, C.c_void_p, C.c_void_p]
+    arr = (C.c_void_p * 4)(*[int(p) for p in planes])
+    st = fn(arr, {"fp64": 0, "fp32": 1}[precision], int(n_cells), float(cell_volume), C.c_void_p(int(out5_ptr)),
+            C.c_void_p(int(stream)) if stream else None)
+    if st != 0:
+        raise EfvmError(st, "efvm_state_sums_noalloc failed")
 
 
 def write_vtk(path: str, state, nx: int, ny: int, lo=(0.0, 0.0), hi=(1.0, 1.0), dz=None, gamma=1.4,

@@ -147,6 +147,17 @@ efvm_status efvm_ibm_add_cylinder(efvm_ctx* ctx, double cx, double cy, double r,
 efvm_status efvm_marker_force(efvm_ctx* ctx, double out[6]);
 /* FP64 volume-weighted interior sums of (rho, rho_u, rho_v, rho_w, rho_E). */
 efvm_status efvm_conservation_sums(efvm_ctx* ctx, double out[5]);
+/* The same sums with no allocation and no host synchronisation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture, serving loops).
+ * planes: the four stored components (rho, rho_u, rho_v, rho_E) of n_cells cells in caller
+ * device memory, FP32 or FP64 (precision = EFVM_FP32 / EFVM_FP64), e.g. efvm_device_state's;
+ * out5: 5 doubles of caller device memory, written (rho_w = 0) on `stream` (a hipStream_t,
+ * null for the default stream).  The per-block partials live in a fixed 32 KiB device array
+ * per GPU: calls that overlap on one GPU race on it.  1 <= n_cells <= EFVM_NOALLOC_MAX_CELLS,
+ * else EFVM_ERR_ARG. */
+#define EFVM_NOALLOC_MAX_CELLS 2147483647LL
+efvm_status efvm_state_sums_noalloc(const void* const planes[4], int precision, long long n_cells, double cell_volume,
+                                    double* out5, void* stream);
 
 /* Raw device planes (rho, rho_u, rho_v, rho_E), nx*ny elements each of the
  * context precision (double or float), for zero-copy consumers. */
