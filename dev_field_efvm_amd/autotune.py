@@ -27,11 +27,10 @@ BATCH_BLOCKS = (0, 2048, 4096, 8192)
 
 
 def arch_name() -> str:
-    try:
-        import torch
-        return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
-    except Exception:
-        return "unknown"
+    """The device architecture from the library's own HIP runtime (torch's may not be the one
+    the library links, and need not be initialised)."""
+    from .efvm import device_arch
+    return device_arch(0).split(":")[0]
 
 
 def _load():
@@ -60,30 +59,38 @@ def _time_ctx(ctx, steps):
 
 
 def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=True, verbose=False, store=True):
-    """Pick (strip_rows, waves_per_block) for a context; returns (rows, waves, ms_per_step)."""
-    kernel = ctx.launch()[2].replace(" ", "-")
-    key = f"{arch_name()}:{ctx.precision}:ctx:{size_class(ctx.nx)}x{size_class(ctx.ny)}:{kernel}"
+    """Pick (strip_rows, waves_per_block) and, where both apply, the one- or two-column strip
+    kernel for a context; returns (rows, waves, ms_per_step).  The two-column kernel halves the
+    lane exchanges per cell but covers 124 columns per wave: at sizes just above a multiple of
+    124 (512, 2048) its last strip is nearly empty, and the one-column kernel can win."""
+    fits = ctx.launch()[3]
+    key = f"{arch_name()}:{ctx.precision}:ctx:{size_class(ctx.nx)}x{size_class(ctx.ny)}:" + \
+          ("periodic-plain" if fits else "general")
     db = _load()
     if use_cache and key in db:
-        r, w = db[key]["rows"], db[key]["waves"]
-        ctx.set_launch(r, w)
-        return r, w, db[key]["ms"]
+        e = db[key]
+        ctx.set_pair_kernel(e.get("pair", 1))
+        ctx.set_launch(e["rows"], e["waves"])
+        return e["rows"], e["waves"], e["ms"]
     ctx.snapshot()
     best = None
-    for r in rows:
-        for w in waves:
-            ctx.set_launch(r, w)
-            ms = _time_ctx(ctx, steps)
-            ctx.snapshot(restore=True)
-            if verbose:
-                print(f"  rows={r:4d} waves={w}  {ms:.4f} ms/step", flush=True)
-            if best is None or ms < best[2]:
-                best = (r, w, ms)
+    for pair in ((1, 0) if fits else (1,)):
+        ctx.set_pair_kernel(pair)
+        for r in rows:
+            for w in waves:
+                ctx.set_launch(r, w)
+                ms = _time_ctx(ctx, steps)
+                ctx.snapshot(restore=True)
+                if verbose:
+                    print(f"  pair={pair} rows={r:4d} waves={w}  {ms:.4f} ms/step", flush=True)
+                if best is None or ms < best[2]:
+                    best = (r, w, ms, pair)
+    ctx.set_pair_kernel(best[3])
     ctx.set_launch(best[0], best[1])
     if store:
-        db[key] = {"rows": best[0], "waves": best[1], "ms": best[2], "when": time.time()}
+        db[key] = {"rows": best[0], "waves": best[1], "ms": best[2], "pair": best[3], "when": time.time()}
         _store(db)
-    return best
+    return best[:3]
 
 
 def _time_batch(b, steps):

@@ -92,6 +92,7 @@ struct efvm_ctx {
     int lam_valid = 0;
     int strip_h = 31;   // rows per strip; the FP32 kernel runs row steps in groups of four (strip_h + 1 of them)
     int waves_per_block = 4;
+    int allow_pair = 1;            // FP32 periodic grids may use the two-column strip kernel
     int xcd_remap = 1;
     int parabolic = 0;
     double inflow[3] = {0, 0, 0};
@@ -133,6 +134,12 @@ efvm_status fail(efvm_ctx* c, efvm_status s, const std::string& msg) {
     } while (0)
 
 bool periodic_axis(const efvm_ctx* c, int axis) { return c->bc[2 * axis].kind == EFVM_BC_PERIODIC; }
+
+// FP32 periodic grids without boundary features can run the two-column strip kernel.
+bool pair_fits(const efvm_ctx* c, const StageArgs& a) {
+    const bool plain = !a.viscous && !a.solid && !a.parabolic && !a.body_force;
+    return c->esz == 4 && !c->learned && pair_path_fits(a.nx, a.ny, a.per_x, a.per_y, plain, a.plane_stride);
+}
 
 StageArgs base_args(const efvm_ctx* c) {
     StageArgs a;
@@ -179,9 +186,7 @@ StageArgs base_args(const efvm_ctx* c) {
     a.lam_part = c->lam_part;
     a.face_count = nullptr;
     a.plane_stride = c->plane_stride <= 0xFFFFFFFFu ? (unsigned)c->plane_stride : 0u;
-    // FP32 periodic grids without boundary features: the two-column strip kernel
-    const bool plain = !a.viscous && !a.solid && !a.parabolic && !a.body_force;
-    a.pair_path = c->esz == 4 && !c->learned && pair_path_fits(g.nx, g.ny, a.per_x, a.per_y, plain, a.plane_stride);
+    a.pair_path = c->allow_pair && pair_fits(c, a);
     if (a.pair_path) a.n_strips_x = (g.nx + STRIP_OUT2 - 1) / STRIP_OUT2;
     return a;
 }
@@ -429,11 +434,27 @@ efvm_status efvm_set_launch(efvm_ctx* c, int strip_rows, int waves_per_block) {
     return EFVM_OK;
 }
 
-efvm_status efvm_get_launch(const efvm_ctx* c, int* strip_rows, int* waves_per_block, int* kernel) {
+efvm_status efvm_get_launch(const efvm_ctx* c, int* strip_rows, int* waves_per_block, int* kernel, int* pair_fits_out) {
     if (!c) return EFVM_ERR_ARG;
     if (strip_rows) *strip_rows = c->strip_h;
     if (waves_per_block) *waves_per_block = c->waves_per_block;
     if (kernel) *kernel = c->learned ? 3 : (c->esz == 8 ? 2 : (base_args(c).pair_path ? 1 : 0));
+    if (pair_fits_out) *pair_fits_out = pair_fits(c, base_args(c)) ? 1 : 0;
+    return EFVM_OK;
+}
+
+efvm_status efvm_device_arch(int device, char* arch, size_t n) {
+    if (!arch || n == 0) return EFVM_ERR_ARG;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return EFVM_ERR_NO_DEVICE;
+    std::snprintf(arch, n, "%s", p.gcnArchName);
+    return EFVM_OK;
+}
+
+efvm_status efvm_set_pair_kernel(efvm_ctx* c, int allow) {
+    if (!c || (allow != 0 && allow != 1)) return EFVM_ERR_ARG;
+    c->allow_pair = allow;
+    drop_graph(c);
     return EFVM_OK;
 }
 

@@ -24,7 +24,7 @@ BC = {"outflow": 0, "periodic": 1, "slip": 2, "noslip": 3, "symmetry": 4,
 
 # Every entry point include/efvm.h declares (checked by the CPU test suite).
 EXPORTS = [
-    "efvm_ctx_create", "efvm_ctx_destroy", "efvm_last_error", "efvm_set_launch", "efvm_get_launch", "efvm_set_boundaries",
+    "efvm_ctx_create", "efvm_ctx_destroy", "efvm_last_error", "efvm_set_launch", "efvm_get_launch", "efvm_set_pair_kernel", "efvm_device_arch", "efvm_set_boundaries",
     "efvm_set_parabolic_inflow", "efvm_set_sponge", "efvm_set_body_force", "efvm_set_time_step", "efvm_upload_state",
     "efvm_download_state", "efvm_get_clock", "efvm_set_clock", "efvm_cfl", "efvm_run", "efvm_step",
     "efvm_advance_to", "efvm_synchronize", "efvm_graph_capture", "efvm_graph_launch",
@@ -32,6 +32,19 @@ EXPORTS = [
     "efvm_last_elapsed_ms", "efvm_snapshot", "efvm_checkpoint_save", "efvm_checkpoint_load", "efvm_write_vtk",
     "efvm_set_ghost_table", "efvm_state_sums_noalloc",
 ]
+
+
+def device_arch(device: int = 0) -> str:
+    """gcnArchName of a HIP device as the library's runtime reports it ("gfx950:..."), or
+    "unknown" without a device."""
+    L = load_library()
+    fn = L.efvm_device_arch
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int, C.c_char_p, C.c_size_t]
+    buf = C.create_string_buffer(256)
+    if fn(int(device), buf, 256) != 0:
+        return "unknown"
+    return buf.value.decode()
 
 
 def state_sums_noalloc(planes, precision: str, n_cells: int, cell_volume: float, out5_ptr: int, stream: int = 0):
@@ -107,7 +120,8 @@ def load_library(path: str = LIB_PATH):
     sig = {
         "efvm_ctx_create": [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Scheme), C.POINTER(vp)],
         "efvm_set_launch": [vp, C.c_int, C.c_int],
-        "efvm_get_launch": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "efvm_get_launch": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "efvm_set_pair_kernel": [vp, C.c_int],
         "efvm_set_boundaries": [vp, C.POINTER(Bc)],
         "efvm_set_parabolic_inflow": [vp, d, d, d],
         "efvm_set_sponge": [vp, d, d, d, d, d, d, d],
@@ -196,10 +210,14 @@ class Context:
     KERNELS = ("one-column strips", "two-column periodic strips", "fp64 strips", "learned face kernels")
 
     def launch(self):
-        """(strip_rows, waves_per_block, kernel name) of the stage kernel this context runs."""
-        r, w, k = C.c_int(), C.c_int(), C.c_int()
-        self._chk(self._L.efvm_get_launch(self._p, C.byref(r), C.byref(w), C.byref(k)))
-        return r.value, w.value, self.KERNELS[k.value]
+        """(strip_rows, waves_per_block, kernel name, whether the two-column kernel applies)."""
+        r, w, k, f = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._chk(self._L.efvm_get_launch(self._p, C.byref(r), C.byref(w), C.byref(k), C.byref(f)))
+        return r.value, w.value, self.KERNELS[k.value], bool(f.value)
+
+    def set_pair_kernel(self, allow: bool):
+        """Allow (default) or forbid the two-column periodic strip kernel."""
+        self._chk(self._L.efvm_set_pair_kernel(self._p, int(bool(allow))))
 
     def set_boundaries(self, sides):
         """sides: 4 x (kind, params) for XLo, XHi, YLo, YHi."""

@@ -89,11 +89,18 @@ const char* efvm_last_error(const efvm_ctx* ctx);
 /* Tile shape of the stage kernel (rows per wave strip, waves per workgroup);
  * 0 keeps the current value.  Used by the autotuner. */
 efvm_status efvm_set_launch(efvm_ctx* ctx, int strip_rows, int waves_per_block);
-/* The current launch shape and the stage kernel the context's configuration selects:
- * *kernel = 0 one-column strips (64 columns per wave), 1 two-column periodic strips (128 per
- * wave; FP32, both axes periodic, no boundary features, nx even), 2 FP64 strips, 3 learned
- * face kernels.  Any pointer may be null.  No reference counterpart (a tuning query). */
-efvm_status efvm_get_launch(const efvm_ctx* ctx, int* strip_rows, int* waves_per_block, int* kernel);
+/* The current launch shape and the stage kernel the context runs: *kernel = 0 one-column
+ * strips (64 columns per wave), 1 two-column periodic strips (128 per wave), 2 FP64 strips,
+ * 3 learned face kernels; *pair_fits = 1 when the two-column kernel applies (FP32, both axes
+ * periodic, no boundary features, nx even).  Any pointer may be null.  No reference
+ * counterpart (a tuning query). */
+efvm_status efvm_get_launch(const efvm_ctx* ctx, int* strip_rows, int* waves_per_block, int* kernel, int* pair_fits);
+/* The gcnArchName of HIP device `device` (e.g. "gfx950:sramecc+:xnack-") into arch[n]: the
+ * autotuner's cache key.  No reference counterpart. */
+efvm_status efvm_device_arch(int device, char* arch, size_t n);
+/* Use the two-column periodic kernel where it applies (1, the default) or always the
+ * one-column kernel (0).  Used by the autotuner; no reference counterpart. */
+efvm_status efvm_set_pair_kernel(efvm_ctx* ctx, int allow);
 
 /* BoundaryRegistry + StructuredBoundaryPlugin (crates/cfd_boundary/src/lib.rs:369-533):
  * sides[4] = XLo, XHi, YLo, YHi.  A half-periodic axis is EFVM_ERR_CONFIG. */
