@@ -32,21 +32,33 @@ METRIC = "cell-updates/s per RK stage, % HBM peak; time-to-L1-error learned vs M
 BYTES_PER_CELL_STEP = {"fp32": 4 * 4 * (2 + 3 + 3), "fp64": 8 * 4 * (2 + 3 + 3)}
 
 
-def cpu_baseline(n_cpu: int = 1024, steps: int = 14):
-    """Time the oracle (single thread, -O3, FP64) on an n_cpu^2 sample of the same workload."""
+def cpu_baseline(n_cpu: int = 1024, steps: int = 6, steps_threaded: int = 30):
+    """The oracle (-O3 FP64, the reference path restated) on an n_cpu^2 sample of the same
+    workload, on this host: on all the cores it may use (its threaded step, bitwise the serial
+    one; OMP_NUM_THREADS or the visible count) and on one thread."""
     from oracle import orc
     from dev_field_efvm_amd import cases
     st = cases.vortex_modes_state(n_cpu, n_cpu, seed=1)
-    sim = orc.Sim(n_cpu, n_cpu, 1, hi=(10.0, 10.0, 10.0 / n_cpu), flux="hllc", muscl=True, limiter="mc",
-                  cfl=0.4, bcs=["periodic"] * 4 + ["slip", "slip"])
-    sim.set_interior(st.reshape(1, n_cpu, n_cpu, 5))
-    sim.step(1)                      # warm-up (page-in)
-    t0 = time.perf_counter()
-    sim.step(steps)
-    dt = time.perf_counter() - t0
-    return {"value": n_cpu * n_cpu * 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle/efvm_oracle.c FP64 MUSCL-MC+HLLC+RK3, periodic {n_cpu}^2 vortex+modes, {steps} steps "
-                      f"({dt:.1f} s, 1 thread, {os.cpu_count()} host cores visible)"}
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0) or os.cpu_count() or 1)
+    out = {}
+    for label, th, k in (("all", threads, steps_threaded), ("one", 1, steps)):
+        sim = orc.Sim(n_cpu, n_cpu, 1, hi=(10.0, 10.0, 10.0 / n_cpu), flux="hllc", muscl=True, limiter="mc",
+                      cfl=0.4, bcs=["periodic"] * 4 + ["slip", "slip"])
+        sim.set_interior(st.reshape(1, n_cpu, n_cpu, 5))
+        used = sim.set_threads(th)
+        sim.step(1)                      # warm-up (page-in)
+        t0 = time.perf_counter()
+        sim.step(k)
+        dt = time.perf_counter() - t0
+        out[label] = (n_cpu * n_cpu * 3 * k / dt, used, k, dt)
+        del sim
+    va, ca, ka, da = out["all"]
+    v1, _, k1, d1 = out["one"]
+    return {"value": va, "unit": "cell-updates/s", "cores": ca, "kind": "port",
+            "single_thread_value": v1,
+            "sample": f"oracle/efvm_oracle.c FP64 MUSCL-MC+HLLC+RK3, periodic {n_cpu}^2 vortex+modes: {ka} steps on "
+                      f"{ca} threads ({da:.1f} s) and {k1} steps on 1 thread ({d1:.1f} s); {os.cpu_count()} host "
+                      f"cores visible"}
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> float:
@@ -157,6 +169,46 @@ def bench_cylinder(args):
     }
 
 
+TRUTH_PATH = os.path.join(ROOT, "tests", "golden", "learned_truth_seed101_512.npz")
+
+
+def learned_comparison(seed: int = 101, n: int = 512, T: float = 1.0, truth_rho=None):
+    """The second half of the metric: time to the plain-MUSCL 2048^2 density L1 error.
+
+    Learned n^2 (network on every face side, and the bypass mode that evaluates the gate only
+    where the open-gate face value is near the TVD bound) against plain MUSCL at n^2 and 4n^2,
+    on held-out seed `seed` to exactly t = T; density L1 against an FP64 MUSCL run 16x finer
+    box-filtered to n^2 (the committed tests/golden/learned_truth_seed101_512.npz unless
+    `truth_rho` is given).  GPU time: HIP events around hipGraph replays of each whole run."""
+    import numpy as np
+    from dev_field_efvm_amd import learned
+    from dev_field_efvm_amd.efvm import Mlp
+    if truth_rho is None:
+        z = np.load(TRUTH_PATH)
+        if int(z["seed"]) != seed or z["rho"].shape != (n, n) or float(z["T"]) != T:
+            raise ValueError("the committed truth is for seed 101, 512^2, t = 1")
+        truth_rho, truth_src = z["rho"].astype(np.float64), f"committed FP64 MUSCL {int(z['n_truth'])}^2"
+    else:
+        truth_src = f"FP64 MUSCL {16 * n}^2 run here"
+    net = Mlp(learned.load_weights())
+    runs = {}
+    for name, nn, m, byp in (("learned", n, net, False), ("learned_bypass", n, net, True), ("muscl", n, None, False),
+                             ("muscl_fine", 4 * n, None, False)):
+        r = learned.run_case(seed, nn, T, m, bypass=byp)
+        runs[name] = dict(l1=float(np.mean(np.abs(learned.box(r["rho"], nn // n) - truth_rho))), steps=r["steps"],
+                          gpu_ms=round(r["gpu_ms"], 3), faces=r["faces"], fallback_faces=r["fallback"],
+                          network_face_sides=r["gated_sides"])
+    fine = runs["muscl_fine"]
+    out = {"seed": seed, "n": n, "T": T, "truth": truth_src}
+    for name in ("learned", "learned_bypass"):
+        out[name] = {"time_ratio_vs_muscl_fine": round(runs[name]["gpu_ms"] / fine["gpu_ms"], 4),
+                     "reaches_muscl_fine_l1": runs[name]["l1"] <= fine["l1"], "target_met": (
+                         runs[name]["gpu_ms"] / fine["gpu_ms"] <= 0.1 and runs[name]["l1"] <= fine["l1"])}
+    out["l1_gain_vs_muscl_same_grid"] = round(runs["muscl"]["l1"] / runs["learned"]["l1"], 2)
+    out["runs"] = runs
+    return out
+
+
 def bench_learned(args):
     """Learned reconstruction on the periodic [0,10]^2 box (8 Fourier modes of amplitude 0.05
     plus 1-4 isentropic vortices, held-out seed): learned 512^2 against plain MUSCL at 512^2
@@ -170,9 +222,12 @@ def bench_learned(args):
     torch.cuda.synchronize()
     net = Mlp(learned.load_weights())
     n, fine, truth_n, T, seed = args.learned_n, 4 * args.learned_n, 16 * args.learned_n, 1.0, args.learned_seed
-    truth = learned.run_case(seed, truth_n, T, precision="fp64", timed=False,
-                             progress=lambda d, s: print(f"truth {truth_n}^2: {d}/{s} steps", file=sys.stderr, flush=True))
-    ref = learned.box(truth["rho"], truth_n // n)
+    if seed == 101 and n == 512 and not args.learned_truth and os.path.exists(TRUTH_PATH):
+        ref = np.load(TRUTH_PATH)["rho"].astype(np.float64)
+    else:
+        truth = learned.run_case(seed, truth_n, T, precision="fp64", timed=False,
+                                 progress=lambda d, s: print(f"truth {truth_n}^2: {d}/{s} steps", file=sys.stderr, flush=True))
+        ref = learned.box(truth["rho"], truth_n // n)
     runs = {}
     for name, nn, m, byp in (("learned", n, net, False), ("learned_bypass", n, net, True), ("muscl", n, None, False),
                              ("muscl_fine", fine, None, False)):
@@ -384,6 +439,8 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-MUSCL time-to-error comparison")
+    ap.add_argument("--learned-truth", action="store_true", help="recompute the learned workload's FP64 truth")
     ap.add_argument("--cpu-n", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -442,15 +499,17 @@ def main():
     # warm-up, then the timed region: K whole steps, bracketed by device synchronisation
     ctx.step(args.warmup)
     ctx.synchronize()
-    use_graph = args.graph and args.steps % 2 == 0 and args.steps > 0
+    # the timed steps replay one hipGraph of g steps (the largest of 20, 10, 4, 2 dividing K)
+    g = next((k for k in (20, 10, 4, 2) if args.steps % k == 0), 0)
+    use_graph = args.graph and g > 0
     if use_graph:
-        ctx.graph_capture(2)
+        ctx.graph_capture(g)
         ctx.graph_launch()
         ctx.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     if use_graph:
-        for _ in range(args.steps // 2):
+        for _ in range(args.steps // g):
             ctx.graph_launch()
         ctx.synchronize()
     else:
@@ -491,7 +550,7 @@ def main():
         "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic (isentropic vortex + 8 seeded Fourier modes, host FP64)",
         "config": {"workload": f"2-D Euler MUSCL(MC)+HLLC+SSP-RK3, periodic {n}x{n}, CFL 0.4, fused FP64 CFL",
-                   "grid": [n, n], "graph": bool(use_graph), "sim_time": clk["time"], "launch": tuned,
+                   "grid": [n, n], "graph_steps": g if use_graph else 0, "sim_time": clk["time"], "launch": tuned,
                    "parallelism": "single GPU",
                    # the same K steps with one upload of the host state before and one download
                    # after (the boundary's host buffers, over PCIe); not the headline value
@@ -499,6 +558,9 @@ def main():
                                       "upload_s": round(t_up, 4), "download_s": round(t_down, 4)}},
         "roofline": roof,
     }
+    ctx.close()
+    if not args.no_learned:
+        out["learned"] = learned_comparison()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
     print(json.dumps(out), flush=True)

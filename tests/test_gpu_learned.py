@@ -273,10 +273,37 @@ def test_learned_run_conserves_and_replays_from_graph():
 
 
 def test_trained_network_beats_muscl():
-    """What the network gains: learned 128^2 vs plain MUSCL 128^2, both against MUSCL 1024^2
-    box-filtered to 128^2, on a seed outside the training set."""
+    """What the learned reconstruction as a whole (the cubic-exact linear correction and the
+    network's gate) gains over plain MUSCL: learned 128^2 vs MUSCL 128^2, both against MUSCL
+    1024^2 box-filtered to 128^2, on a seed outside the training set.  The gate's own share is
+    test_trained_gate_against_the_gate_fixed_open."""
     net = trained_net()
     out = learned.evaluate(net, seed=201, n=128, fine=512, truth_n=1024, T=0.5)
     print({k: out[k] for k in ("learned", "muscl", "muscl_fine", "l1_gain_vs_muscl")})
     assert out["learned"]["fallback_frac"] < 0.05
     assert out["l1_gain_vs_muscl"] > 2.0
+
+
+def test_trained_gate_against_the_gate_fixed_open():
+    """What the network itself gains over the method without it: the same learned correction with
+    the network's gate fixed at 1 (output weights zero, output bias 40: sigmoid = 1 in FP32),
+    both in network mode (no bypass), on a held-out seed at 256^2, density L1 at t = 1 against an
+    FP64 MUSCL run 8x finer box-filtered to 256^2."""
+    p = learned.load_weights()
+    nin, nh, nout = learned.NIN, learned.NH, learned.NOUT
+    o3 = nh * nin + nh + nh * nh + nh                  # W3 (nout x nh), then b3 (nout)
+    open_gate = p.copy()
+    open_gate[o3:o3 + nout * nh] = 0.0
+    open_gate[o3 + nout * nh:o3 + nout * nh + nout] = 40.0
+    seed, n, T = 103, 256, 1.0
+    truth = learned.run_case(seed, 8 * n, T, precision="fp64", timed=False)
+    ref = learned.box(truth["rho"], 8)
+    errs = {}
+    for name, params in (("trained gate", p), ("gate fixed at 1", open_gate)):
+        r = learned.run_case(seed, n, T, efvm.Mlp(params), timed=False)
+        errs[name] = float(np.mean(np.abs(r["rho"] - ref)))
+        if name == "gate fixed at 1":
+            assert r["gated_sides"] == 2 * r["faces"]     # the network still ran on every face side
+    gain = errs["gate fixed at 1"] / errs["trained gate"]
+    print(f"held-out seed {seed}, {n}^2: L1 {errs}, the network's gain x{gain:.3f}")
+    assert errs["trained gate"] < errs["gate fixed at 1"]
