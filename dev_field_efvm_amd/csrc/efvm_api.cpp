@@ -34,7 +34,11 @@ struct IbmArgs {
     int per_x, per_y;
     const uint8_t* solid;
 };
-template <class T> hipError_t launch_ibm_mirror(const StageArgs& sa, const IbmArgs& a, int n, const int* idx, const double* img, hipStream_t st);
+template <class T> hipError_t launch_ibm_mirror(const StageArgs& sa, const IbmArgs& a, int n, const int* idx, const double* img,
+                                                void* pre, hipStream_t st);
+template <class T> hipError_t launch_wall_force(const StageArgs& sa, const IbmArgs& a, int n, const int* idx, int n_src,
+                                                void* const* set0, void* const* set1, const void* pre0, const void* pre1,
+                                                double w0, double w1, int visc_const, double mu, double* out, hipStream_t st);
 template <class T> hipError_t launch_marker_force(const StageArgs& sa, const IbmArgs& a, int n, const double* mk, double off,
                                                   int visc_const, double mu, double* out, hipStream_t st);
 template <class T> hipError_t launch_sponge(const StageArgs& a, double x0, double x1, double sig, double um, double h,
@@ -74,6 +78,7 @@ struct efvm_ctx {
     int cur = 0;                   // index of the set holding U^n
     int prev = 1;                  // index of the set holding U^{n-1} (U1 / U2's input during a step)
     int have_prev = 0;             // prev holds the state before the last completed step (or a loaded u0)
+    int ghost_hist = 0;            // the last completed step's substage inputs and pre-mirror ghosts are resident
     DevClock* clock = nullptr;
     double* lam_part = nullptr;    // FP32: per-wave maxima of the fused CFL
     double* bc_tab[4] = {nullptr, nullptr, nullptr, nullptr};   // EFVM_BC_TABLE ghost tables
@@ -81,6 +86,7 @@ struct efvm_ctx {
     uint8_t* solid = nullptr;
     int* ghost_idx = nullptr;
     double* ghost_img = nullptr;
+    void* ghost_pre[2] = {};       // each ghost before the IBM mirror: of the first / the latest later substage
     int n_ghost = 0;
     double* markers = nullptr;
     int n_mark = 0;
@@ -258,8 +264,8 @@ efvm_status run_substage(efvm_ctx* c) {
     // PreFlux: IBM mirror into the stage input
     if (c->n_cyl > 0 && c->n_ghost > 0) {
         IbmArgs ib = ibm_args(c, a);
-        hipError_t e = c->esz == 8 ? launch_ibm_mirror<double>(a, ib, c->n_ghost, c->ghost_idx, c->ghost_img, c->stream)
-                                   : launch_ibm_mirror<float>(a, ib, c->n_ghost, c->ghost_idx, c->ghost_img, c->stream);
+        hipError_t e = c->esz == 8 ? launch_ibm_mirror<double>(a, ib, c->n_ghost, c->ghost_idx, c->ghost_img, c->ghost_pre[c->rk_stage ? 1 : 0], c->stream)
+                                   : launch_ibm_mirror<float>(a, ib, c->n_ghost, c->ghost_idx, c->ghost_img, c->ghost_pre[c->rk_stage ? 1 : 0], c->stream);
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "ibm mirror");
     }
     if (c->learned) {
@@ -298,6 +304,7 @@ efvm_status run_substage(efvm_ctx* c) {
     if (last) {
         std::swap(c->cur, c->prev);   // U^{n+1} is in the old prev set; U^n becomes prev
         c->have_prev = 1;
+        c->ghost_hist = 1;            // U^n (prev) and U2 (the third set) stay resident
         StageArgs b = base_args(c);
         set_planes(b, c->P[c->cur], c->P[c->cur], c->P[c->cur]);
         if (c->sponge) {
@@ -326,9 +333,11 @@ void free_ibm(efvm_ctx* c) {
     if (c->solid) (void)hipFree(c->solid);
     if (c->ghost_idx) (void)hipFree(c->ghost_idx);
     if (c->ghost_img) (void)hipFree(c->ghost_img);
+    for (int k = 0; k < 2; k++) if (c->ghost_pre[k]) (void)hipFree(c->ghost_pre[k]);
     if (c->markers) (void)hipFree(c->markers);
     if (c->stagnant) (void)hipFree(c->stagnant);
-    c->solid = nullptr; c->ghost_idx = nullptr; c->ghost_img = nullptr; c->markers = nullptr; c->stagnant = nullptr;
+    c->solid = nullptr; c->ghost_idx = nullptr; c->ghost_img = nullptr; c->ghost_pre[0] = c->ghost_pre[1] = nullptr; c->markers = nullptr;
+    c->stagnant = nullptr;
 }
 
 void drop_graph(efvm_ctx* c) {
@@ -535,6 +544,7 @@ efvm_status efvm_upload_state(efvm_ctx* c, const double* cons5, size_t n) {
     c->rk_stage = 0;
     c->lam_valid = 0;
     c->have_prev = 0;
+    c->ghost_hist = 0;
     return EFVM_OK;
 }
 
@@ -717,6 +727,7 @@ efvm_status efvm_graph_launch(efvm_ctx* c) {
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timed = 1;
     c->have_prev = 1;             // the replayed steps end with the roles they started with
+    c->ghost_hist = 1;
     return EFVM_OK;
 }
 
@@ -795,6 +806,8 @@ efvm_status efvm_ibm_add_cylinder(efvm_ctx* c, double cx, double cy, double r, c
     if ((st = device_alloc(c, (void**)&c->solid, n)) != EFVM_OK) return st;
     if ((st = device_alloc(c, (void**)&c->ghost_idx, sizeof(int) * std::max<size_t>(1, gidx.size()))) != EFVM_OK) return st;
     if ((st = device_alloc(c, (void**)&c->ghost_img, sizeof(double) * std::max<size_t>(3, gimg.size()))) != EFVM_OK) return st;
+    for (int k = 0; k < 2; k++)
+        if ((st = device_alloc(c, &c->ghost_pre[k], 4 * c->esz * std::max<size_t>(1, gidx.size()))) != EFVM_OK) return st;
     if ((st = device_alloc(c, (void**)&c->markers, sizeof(double) * mk.size())) != EFVM_OK) return st;
     if ((st = device_alloc(c, (void**)&c->stagnant, sizeof(double) * 5)) != EFVM_OK) return st;
     HIPCHK(c, hipMemcpy(c->solid, solid.data(), n, hipMemcpyHostToDevice));
@@ -807,6 +820,7 @@ efvm_status efvm_ibm_add_cylinder(efvm_ctx* c, double cx, double cy, double r, c
     c->n_ghost = (int)gidx.size();
     c->n_mark = (int)(mk.size() / 7);
     c->ibm_flipped = 0;
+    c->ghost_hist = 0;
     return EFVM_OK;
 }
 
@@ -822,6 +836,38 @@ efvm_status efvm_marker_force(efvm_ctx* c, double out[6]) {
     hipError_t e = c->esz == 8 ? launch_marker_force<double>(a, ib, c->n_mark, c->markers, dx, a.visc_const, a.mu, dout, c->stream)
                                : launch_marker_force<float>(a, ib, c->n_mark, c->markers, dx, a.visc_const, a.mu, dout, c->stream);
     if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "marker force");
+    HIPCHK(c, hipMemcpyAsync(out, dout, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
+}
+
+efvm_status efvm_wall_force(efvm_ctx* c, double out[6]) {
+    if (!c || !out) return EFVM_ERR_ARG;
+    // the mask takes the body on the first substage (the reference's Setup phase): no solid cell before
+    if (c->n_cyl == 0 || !c->ibm_flipped) { for (int d = 0; d < 6; d++) out[d] = 0.0; return EFVM_OK; }
+    // Ghost cells: fills of the substage inputs since the step boundary, combined by the updates.
+    // Mid-step U^n is cur and U1 prev; after a step U^n is prev and U2 the third set.
+    const bool rk3 = c->sch.integrator == EFVM_INTEGRATOR_RK3;
+    const int free_set = 3 - c->cur - c->prev;
+    int n_src = 0, s0 = 0, s1 = 0;
+    double w0 = 1.0, w1 = 0.0;
+    if (c->rk_stage == 1) { n_src = 1; s0 = c->cur; }
+    else if (c->rk_stage == 2) { n_src = 2; s0 = c->cur; s1 = c->prev; w0 = 0.75; w1 = 0.25; }
+    else if (c->ghost_hist && !rk3) { n_src = 1; s0 = c->prev; }
+    else if (c->ghost_hist) { n_src = 2; s0 = c->prev; s1 = free_set; w0 = 1.0 / 3.0; w1 = 2.0 / 3.0; }
+    const int set = stage_set(c);
+    StageArgs a = base_args(c);
+    set_planes(a, c->P[set], c->P[set], c->P[set]);
+    IbmArgs ib = ibm_args(c, a);
+    double* dout = c->scratch + 4096;
+    void* const* p0 = c->P[s0];
+    void* const* p1 = c->P[s1];
+    hipError_t e = c->esz == 8
+        ? launch_wall_force<double>(a, ib, c->n_ghost, c->ghost_idx, n_src, p0, p1, c->ghost_pre[0], c->ghost_pre[1], w0, w1,
+                                    a.visc_const, a.mu, dout, c->stream)
+        : launch_wall_force<float>(a, ib, c->n_ghost, c->ghost_idx, n_src, p0, p1, c->ghost_pre[0], c->ghost_pre[1], w0, w1,
+                                   a.visc_const, a.mu, dout, c->stream);
+    if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "wall force");
     HIPCHK(c, hipMemcpyAsync(out, dout, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;
@@ -1296,6 +1342,7 @@ efvm_status efvm_snapshot(efvm_ctx* c, int op) {
         HIPCHK(c, hipMemcpyAsync(c->clock, s + 8 * pbytes, sizeof(DevClock), hipMemcpyDeviceToDevice, c->stream));
         c->parity = c->snap_parity; c->lam_valid = c->snap_lam; c->ibm_flipped = c->snap_flip;
         c->have_prev = c->snap_have_prev;
+        c->ghost_hist = 0;            // the third set and the pre-mirror ghosts are not in the snapshot
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return EFVM_OK;

@@ -695,6 +695,19 @@ efvm_status efvm3_marker_force(efvm_ctx3* c, double out[6]) {
     return EFVM_OK;
 }
 
+efvm_status efvm3_wall_force(efvm_ctx3* c, double out[6]) {
+    if (!c || !out) return EFVM_ERR_ARG;
+    // the mask takes the bodies on the first substage (the reference's Setup phase): no solid cell before
+    if (c->n_body == 0 || !c->ibm_flipped) { for (int d = 0; d < 6; d++) out[d] = 0.0; return EFVM_OK; }
+    Args3 a = args3(c);
+    const int set = current_set(c);
+    planes3(a, c->P[set], c->P[set], c->P[set]);
+    LAUNCH3(c, launch3_wall(a, (int)c->esz, c->n_ghost, c->ghost_idx, c->out, c->stream), "wall force");
+    HIPCHK3(c, hipMemcpyAsync(out, c->out, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
+}
+
 // CFDRST01 (examples/restart_determinism/main.rs:177-226): "CFDRST01", dt, time (f64), step
 // (u64), rk stage (u8), then the columns u, rhs, u0, each a u64 cell count and 5 f64 per cell
 // over the raw padded array, little-endian.  The 3-D context stores that array, ghost cells

@@ -196,35 +196,42 @@ __device__ __forceinline__ Cell<T> bc_apply(const StageArgs& a, int side, Cell<T
     }
 }
 
+// Ghost state across boundary side `side` from its owner cell: the cylinder case's parabolic
+// inflow on x-lo, else the side's stock condition.
 template <class T>
-__device__ __forceinline__ Cell<T> load_cell(const StageArgs& a, const T* const* U, int c, int r) {
+__device__ __forceinline__ Cell<T> ghost_of_owner(const StageArgs& a, int side, Cell<T> own, int orow, int oc) {
+    if (side == 0 && a.parabolic) {
+        // examples/cylinder_drag_validation/main.rs:151-177 (overrides the x-lo fill)
+        double y = a.lo_y + ((double)orow + 0.5) * a.dy;
+        double um = a.inflow_umax, h = a.inflow_h, rho = a.inflow_rho;
+        double uy = 4.0 * um * y * (h - y) / (h * h);
+        C5<double> u = {double(own.r), double(own.ru), double(own.rv), 0.0, double(own.re)};
+        double pin = ref::pressure(u, a.gamma);
+        P6<double> p = {rho, uy, 0.0, 0.0, pin, pin / (rho * 287.058)};
+        C5<double> o = ref::prim_to_cons(p, a.gamma);
+        return {T(o.r), T(o.ru), T(o.rv), T(o.re)};
+    }
+    return bc_apply(a, side, own, orow, oc);
+}
+
+// Where a cell index outside the grid takes its value from: the periodic image (side -1), or
+// the owner cell of the boundary ghost on side 0..3 (x-lo, x-hi, y-lo, y-hi).
+struct GhostOwner { int side, col, row; };
+__device__ __forceinline__ GhostOwner ghost_owner(const StageArgs& a, int c, int r) {
     const int nx = a.nx, ny = a.ny;
     if (a.per_x) { c %= nx; if (c < 0) c += nx; }
     if (a.per_y) { r %= ny; if (r < 0) r += ny; }
     bool cin = (c >= 0 && c < nx), rin = (r >= 0 && r < ny);
-    if (cin && rin) return load_plain(a, U, c, r);
-    if (!cin) {
-        int side = c < 0 ? 0 : 1;
-        int oc = c < 0 ? 0 : nx - 1;
-        int orow = r < 0 ? 0 : (r >= ny ? ny - 1 : r);
-        Cell<T> own = load_plain(a, U, oc, orow);
-        if (side == 0 && a.parabolic) {
-            // examples/cylinder_drag_validation/main.rs:151-177 (overrides the x-lo fill)
-            double y = a.lo_y + ((double)orow + 0.5) * a.dy;
-            double um = a.inflow_umax, h = a.inflow_h, rho = a.inflow_rho;
-            double uy = 4.0 * um * y * (h - y) / (h * h);
-            C5<double> u = {double(own.r), double(own.ru), double(own.rv), 0.0, double(own.re)};
-            double pin = ref::pressure(u, a.gamma);
-            P6<double> p = {rho, uy, 0.0, 0.0, pin, pin / (rho * 287.058)};
-            C5<double> o = ref::prim_to_cons(p, a.gamma);
-            return {T(o.r), T(o.ru), T(o.rv), T(o.re)};
-        }
-        return bc_apply(a, side, own, orow, oc);
-    }
-    int side = r < 0 ? 2 : 3;
-    int orow = r < 0 ? 0 : ny - 1;
-    Cell<T> own = load_plain(a, U, c, orow);
-    return bc_apply(a, side, own, orow, c);
+    if (cin && rin) return {-1, c, r};
+    if (!cin) return {c < 0 ? 0 : 1, c < 0 ? 0 : nx - 1, r < 0 ? 0 : (r >= ny ? ny - 1 : r)};
+    return {r < 0 ? 2 : 3, c, r < 0 ? 0 : ny - 1};
+}
+
+template <class T>
+__device__ __forceinline__ Cell<T> load_cell(const StageArgs& a, const T* const* U, int c, int r) {
+    GhostOwner g = ghost_owner(a, c, r);
+    Cell<T> own = load_plain(a, U, g.col, g.row);
+    return g.side < 0 ? own : ghost_of_owner(a, g.side, own, g.row, g.col);
 }
 
 __device__ __forceinline__ void atomic_max_pos_double(double* addr, double v) {

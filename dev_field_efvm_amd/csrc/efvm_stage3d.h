@@ -12,6 +12,7 @@ struct Args3 {
     int NX, NY, NZ;        // raw (padded) extents, n + 2 NGH
     long nraw, ncell;
     double lo[3], h[3];
+    const double* spacing[3];   // per-axis cell widths (interior cells) of a stretched axis, or null: h
     double aov[3];         // face area / cell volume per axis
     int per[3];
     int seam_mode;
@@ -53,6 +54,7 @@ hipError_t launch3_stage(const Args3& a, int esz, hipStream_t st);
 int cfl3_blocks(long ncell);
 hipError_t launch3_cfl(const Args3& a, int esz, double* part, double* target, hipStream_t st);
 hipError_t launch3_sponge(const Args3& a, int esz, const Sponge3& s, hipStream_t st);
+hipError_t launch3_wall(const Args3& a, int esz, int n, const long* idx, double* out, hipStream_t st);
 hipError_t launch3_markers(const Args3& a, int esz, int n, const double* mk, double off, double* contrib, double* out,
                            hipStream_t st);
 hipError_t launch3_sums(const Args3& a, int esz, double* part, int nb, double vol, double* out, hipStream_t st);

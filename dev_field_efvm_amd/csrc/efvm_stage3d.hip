@@ -247,6 +247,62 @@ __global__ __launch_bounds__(256) void marker3_sum_kernel(const double* __restri
     if (t == 0) for (int d = 0; d < 6; d++) out[d] = sh[0][d];
 }
 
+// Staircase wall force (cfd_ibm:309-362): pressure -p n A and wall shear mu u_t / (dn/2) A over
+// every solid-fluid face of the raw state.  Its ghost cells hold the reference's values: each
+// substage fills its input's ghosts and its update carries them into the output.  The ghost list
+// is every solid cell with a fluid neighbour in raw order (buried cells have no fluid face), so
+// lane l evaluates the six faces of ghost base + l and lane 0 adds the terms in cell order and
+// face order x+, x-, y+, y-, z+, z-: the FP64 sum is the serial one.
+template <class T>
+__global__ void wall3_kernel(Args3 a, int n, const long* __restrict__ idx, double* out) {
+    __shared__ double term[64][6][3];   // per face: pressure term, then the two tangential shear terms
+    __shared__ unsigned char fluid[64][6];
+    const int l = threadIdx.x;
+    const T* const* U = (const T* const*)a.uin;
+    const long st[3] = {1, a.NX, (long)a.NX * a.NY};
+    double fp[3] = {0.0, 0.0, 0.0}, fv[3] = {0.0, 0.0, 0.0};
+    for (int base = 0; base < n; base += 64) {
+        const int q = base + l;
+        if (q < n) {
+            const long c = idx[q];
+            const int jr = (int)((c / a.NX) % a.NY), kr = (int)(c / ((long)a.NX * a.NY)), ir = (int)(c % a.NX);
+            const int rr[3] = {ir, jr, kr};
+            double h[3];
+            for (int d = 0; d < 3; d++) h[d] = a.spacing[d] ? a.spacing[d][rr[d] - NGH] : a.h[d];
+            const double fa[3] = {h[1] * h[2], h[0] * h[2], h[0] * h[1]};
+            for (int f = 0; f < 6; f++) {
+                const int dim = f >> 1;
+                const double sg = (f & 1) ? -1.0 : 1.0;
+                const long nb = (f & 1) ? c - st[dim] : c + st[dim];
+                const bool fl = !(a.solid && a.solid[nb]);
+                fluid[l][f] = fl;
+                if (!fl) continue;
+                const V5<double> u = to_d(ld5(U, nb));
+                term[l][f][0] = ref::pressure(u, a.gamma) * sg * fa[dim];
+                const P6<double> pr = ref::cons_to_prim(u, a.gamma, a.r_gas);
+                const double mu = a.visc_const ? a.mu : ref::sutherland(pr.t);
+                const double vel[3] = {pr.u, pr.v, pr.w};
+                for (int td = 0, t = 1; td < 3; td++)
+                    if (td != dim) term[l][f][t++] = mu * vel[td] / (0.5 * h[dim]) * fa[dim];
+            }
+        }
+        __syncthreads();
+        if (l == 0) {
+            for (int g = 0; g < 64 && base + g < n; g++)
+                for (int f = 0; f < 6; f++) {
+                    if (!fluid[g][f]) continue;
+                    const int dim = f >> 1;
+                    fp[dim] -= term[g][f][0];
+                    for (int td = 0, t = 1; td < 3; td++)
+                        if (td != dim) fv[td] += term[g][f][t++];
+                }
+        }
+        __syncthreads();
+    }
+    if (l == 0)
+        for (int d = 0; d < 3; d++) { out[d] = fp[d]; out[3 + d] = fv[d]; }
+}
+
 // ---------------------------------------------------------------------------
 // Fused flux / viscous / RK kernel
 // ---------------------------------------------------------------------------
@@ -611,6 +667,12 @@ hipError_t launch3_markers(const Args3& a, int esz, int n, const double* mk, dou
         else marker3_eval_kernel<float><<<(n + 255) / 256, 256, 0, st>>>(a, n, mk, off, contrib);
     }
     marker3_sum_kernel<<<1, 256, 0, st>>>(contrib, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch3_wall(const Args3& a, int esz, int n, const long* idx, double* out, hipStream_t st) {
+    if (esz == 8) wall3_kernel<double><<<1, 64, 0, st>>>(a, n, idx, out);
+    else wall3_kernel<float><<<1, 64, 0, st>>>(a, n, idx, out);
     return hipGetLastError();
 }
 

@@ -286,12 +286,19 @@ __device__ bool sample_trilinear(const StageArgs& sa, const IbmArgs& a, const T*
 }
 
 // Mirror: u_ghost = prim_to_cons(rho_img, 2*0 - u_img, p_img).  Sampling skips solid
-// cells and writes only solid cells, so ghosts are independent of one another.
+// cells and writes only solid cells, so ghosts are independent of one another.  `pre`
+// (4 planes of n) keeps each ghost's value from before the mirror: the boundary fill runs
+// before the mirror, so ghost cells filled from a solid cell hold that value.
 template <class T>
-__global__ void ibm_mirror_kernel(StageArgs sa, IbmArgs a, int n, const int* __restrict__ idx, const double* __restrict__ img) {
+__global__ void ibm_mirror_kernel(StageArgs sa, IbmArgs a, int n, const int* __restrict__ idx, const double* __restrict__ img,
+                                  T* __restrict__ pre) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const T* U[4] = {(const T*)sa.uin[0], (const T*)sa.uin[1], (const T*)sa.uin[2], (const T*)sa.uin[3]};
+    if (pre) {
+        const long k0 = idx[q];
+        for (int m = 0; m < 4; m++) pre[(long)m * n + q] = U[m][k0];
+    }
     P6<double> im;
     if (!sample_trilinear<T>(sa, a, U, img + 3 * q, im)) return;
     P6<double> m = {im.r, 2.0 * 0.0 - im.u, 2.0 * 0.0 - im.v, 2.0 * 0.0 - im.w, im.p, im.t};
@@ -343,6 +350,123 @@ __global__ void marker_force_kernel(StageArgs sa, IbmArgs a, int n, const double
     }
 }
 __global__ void zero6(double* out) { if (threadIdx.x < 6) out[threadIdx.x] = 0.0; }
+
+// Staircase wall force (cfd_ibm:309-362): pressure -p n A and wall shear mu u_t / (dn/2) A
+// summed over every solid-fluid face.  The ghost list holds every solid cell in row-major
+// order (on the one-cell-thick mesh each has fluid z neighbours), so lane l evaluates the six
+// faces of ghost base + l and lane 0 adds the terms in cell order and face order x+, x-, y+,
+// y-, z+, z-: the FP64 sum is the serial one.
+//
+// Neighbours inside the grid come from the state U.  A ghost cell holds what the substages
+// since the last step boundary made of it: each substage fills it from its input (a z ghost is
+// the slip image of the cell, an x/y ghost the periodic image or the boundary condition of its
+// owner cell, taken before the IBM mirror overwrites solid cells), and each update then combines
+// it like an interior cell with a zero right-hand side.  So the ghost is the fill of src[0]
+// (one substage since the step boundary), or w0 fill(src[0]) + w1 fill(src[1]) (RK3: 3/4, 1/4
+// after two substages, 1/3, 2/3 after the step); with no substage run it is taken from U.
+struct WallSources {
+    int n_src;                 // 0, 1 or 2
+    const void* set[2][4];     // the substage inputs whose fills the ghosts combine
+    const void* pre[2];        // their solid cells before the mirror (4 planes of the ghost count)
+    double w[2];
+};
+
+template <class T>
+__device__ Cell<T> pre_mirror(const StageArgs& sa, const T* const* X, const T* pre, const int* idx, int n, int col, int row) {
+    const long k = (long)row * sa.nx + col;
+    if (sa.solid && sa.solid[k]) {
+        int lo = 0, hi = n - 1;   // idx ascends: binary search for the cell's ghost slot
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (idx[mid] < k) lo = mid + 1; else hi = mid;
+        }
+        return {pre[lo], pre[(long)n + lo], pre[2L * n + lo], pre[3L * n + lo]};
+    }
+    return load_plain(sa, X, col, row);
+}
+
+// The fill from one source of the ghost across face f of cell (i, j) (ghost slot q).
+template <class T>
+__device__ C5<double> ghost_fill(const StageArgs& sa, const T* const* X, const T* pre, const int* idx, int n, int q, int i,
+                                 int j, int f) {
+    Cell<T> cl;
+    if ((f >> 1) == 2) {
+        // slip image of the cell itself: only rho_w changes sign, and it is zero on this mesh
+        cl = {pre[q], pre[(long)n + q], pre[2L * n + q], pre[3L * n + q]};
+    } else {
+        const int ci = i + (f == 0 ? 1 : f == 1 ? -1 : 0), cj = j + (f == 2 ? 1 : f == 3 ? -1 : 0);
+        const GhostOwner g = ghost_owner(sa, ci, cj);
+        const Cell<T> own = pre_mirror<T>(sa, X, pre, idx, n, g.col, g.row);
+        cl = g.side < 0 ? own : ghost_of_owner(sa, g.side, own, g.row, g.col);
+    }
+    return {double(cl.r), double(cl.ru), double(cl.rv), 0.0, double(cl.re)};
+}
+
+template <class T>
+__global__ void wall_force_kernel(StageArgs sa, IbmArgs a, int n, const int* __restrict__ idx, WallSources ws,
+                                  int visc_const, double mu_c, double* out) {
+    __shared__ double term[64][6][3];   // per face: pressure term, then the two tangential shear terms
+    __shared__ unsigned char fluid[64][6];
+    const int l = threadIdx.x;
+    const T* U[4] = {(const T*)sa.uin[0], (const T*)sa.uin[1], (const T*)sa.uin[2], (const T*)sa.uin[3]};
+    const T* X0[4] = {(const T*)ws.set[0][0], (const T*)ws.set[0][1], (const T*)ws.set[0][2], (const T*)ws.set[0][3]};
+    const T* X1[4] = {(const T*)ws.set[1][0], (const T*)ws.set[1][1], (const T*)ws.set[1][2], (const T*)ws.set[1][3]};
+    const double fa[3] = {a.dy * a.dz, a.dx * a.dz, a.dx * a.dy}, dn[3] = {a.dx, a.dy, a.dz};
+    double fp[3] = {0.0, 0.0, 0.0}, fv[3] = {0.0, 0.0, 0.0};
+    for (int base = 0; base < n; base += 64) {
+        const int q = base + l;
+        if (q < n) {
+            const long k = idx[q];
+            const int i = (int)(k % a.nx), j = (int)(k / a.nx);
+            for (int f = 0; f < 6; f++) {
+                const int dim = f >> 1;
+                const double sg = (f & 1) ? -1.0 : 1.0;
+                const int ci = i + (f == 0 ? 1 : f == 1 ? -1 : 0), cj = j + (f == 2 ? 1 : f == 3 ? -1 : 0);
+                const bool inside = dim < 2 && ci >= 0 && ci < a.nx && cj >= 0 && cj < a.ny;
+                C5<double> u;
+                bool fl = true;
+                if (inside) {
+                    fl = !a.solid[(long)cj * a.nx + ci];
+                    const Cell<T> cl = load_plain(sa, U, ci, cj);
+                    u = {double(cl.r), double(cl.ru), double(cl.rv), 0.0, double(cl.re)};
+                } else if (ws.n_src == 0) {
+                    const Cell<T> cl = dim == 2 ? load_plain(sa, U, i, j) : load_cell(sa, U, ci, cj);
+                    u = {double(cl.r), double(cl.ru), double(cl.rv), 0.0, double(cl.re)};
+                } else {
+                    u = ghost_fill<T>(sa, X0, (const T*)ws.pre[0], idx, n, q, i, j, f);
+                    if (ws.n_src == 2) {
+                        // the update of a ghost: w0 u0 + w1 (u + 0 dt), component by component
+                        const C5<double> v = ghost_fill<T>(sa, X1, (const T*)ws.pre[1], idx, n, q, i, j, f);
+                        u = {u.r * ws.w[0] + v.r * ws.w[1], u.ru * ws.w[0] + v.ru * ws.w[1], u.rv * ws.w[0] + v.rv * ws.w[1],
+                             0.0, u.re * ws.w[0] + v.re * ws.w[1]};
+                    }
+                }
+                fluid[l][f] = fl;
+                if (!fl) continue;
+                term[l][f][0] = ref::pressure(u, a.gamma) * sg * fa[dim];
+                const P6<double> pr = ref::cons_to_prim(u, a.gamma, a.r_gas);
+                const double mu = visc_const ? mu_c : ref::sutherland(pr.t);
+                const double vel[3] = {pr.u, pr.v, pr.w};
+                for (int td = 0, t = 1; td < 3; td++)
+                    if (td != dim) term[l][f][t++] = mu * vel[td] / (0.5 * dn[dim]) * fa[dim];
+            }
+        }
+        __syncthreads();
+        if (l == 0) {
+            for (int c = 0; c < 64 && base + c < n; c++)
+                for (int f = 0; f < 6; f++) {
+                    if (!fluid[c][f]) continue;
+                    const int dim = f >> 1;
+                    fp[dim] -= term[c][f][0];
+                    for (int td = 0, t = 1; td < 3; td++)
+                        if (td != dim) fv[td] += term[c][f][t++];
+                }
+        }
+        __syncthreads();
+    }
+    if (l == 0)
+        for (int d = 0; d < 3; d++) { out[d] = fp[d]; out[3 + d] = fv[d]; }
+}
 
 // Outlet sponge (examples/cylinder_drag_validation/main.rs:186-229), after a completed step.
 template <class T>
@@ -437,13 +561,30 @@ hipError_t launch_clock_set(DevClock* clk, double dt, double time, long long ste
 }
 
 template <class T>
-hipError_t launch_ibm_mirror(const StageArgs& sa, const IbmArgs& a, int n, const int* idx, const double* img, hipStream_t st) {
+hipError_t launch_ibm_mirror(const StageArgs& sa, const IbmArgs& a, int n, const int* idx, const double* img, void* pre, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    ibm_mirror_kernel<T><<<(n + 127) / 128, 128, 0, st>>>(sa, a, n, idx, img);
+    ibm_mirror_kernel<T><<<(n + 127) / 128, 128, 0, st>>>(sa, a, n, idx, img, (T*)pre);
     return hipGetLastError();
 }
-template hipError_t launch_ibm_mirror<double>(const StageArgs&, const IbmArgs&, int, const int*, const double*, hipStream_t);
-template hipError_t launch_ibm_mirror<float>(const StageArgs&, const IbmArgs&, int, const int*, const double*, hipStream_t);
+template hipError_t launch_ibm_mirror<double>(const StageArgs&, const IbmArgs&, int, const int*, const double*, void*, hipStream_t);
+template hipError_t launch_ibm_mirror<float>(const StageArgs&, const IbmArgs&, int, const int*, const double*, void*, hipStream_t);
+
+template <class T>
+hipError_t launch_wall_force(const StageArgs& sa, const IbmArgs& a, int n, const int* idx, int n_src, void* const* set0,
+                             void* const* set1, const void* pre0, const void* pre1, double w0, double w1, int visc_const,
+                             double mu, double* out, hipStream_t st) {
+    WallSources ws;
+    ws.n_src = n_src;
+    for (int m = 0; m < 4; m++) { ws.set[0][m] = set0 ? set0[m] : nullptr; ws.set[1][m] = set1 ? set1[m] : nullptr; }
+    ws.pre[0] = pre0; ws.pre[1] = pre1;
+    ws.w[0] = w0; ws.w[1] = w1;
+    wall_force_kernel<T><<<1, 64, 0, st>>>(sa, a, n, idx, ws, visc_const, mu, out);
+    return hipGetLastError();
+}
+template hipError_t launch_wall_force<double>(const StageArgs&, const IbmArgs&, int, const int*, int, void* const*, void* const*,
+                                              const void*, const void*, double, double, int, double, double*, hipStream_t);
+template hipError_t launch_wall_force<float>(const StageArgs&, const IbmArgs&, int, const int*, int, void* const*, void* const*,
+                                             const void*, const void*, double, double, int, double, double*, hipStream_t);
 
 template <class T>
 hipError_t launch_marker_force(const StageArgs& sa, const IbmArgs& a, int n, const double* mk, double off, int visc_const,

@@ -28,7 +28,7 @@ EXPORTS = [
     "efvm_set_parabolic_inflow", "efvm_set_sponge", "efvm_set_body_force", "efvm_set_time_step", "efvm_upload_state",
     "efvm_download_state", "efvm_get_clock", "efvm_set_clock", "efvm_cfl", "efvm_run", "efvm_step",
     "efvm_advance_to", "efvm_synchronize", "efvm_graph_capture", "efvm_graph_launch",
-    "efvm_ibm_add_cylinder", "efvm_marker_force", "efvm_conservation_sums", "efvm_device_state",
+    "efvm_ibm_add_cylinder", "efvm_marker_force", "efvm_wall_force", "efvm_conservation_sums", "efvm_device_state",
     "efvm_last_elapsed_ms", "efvm_snapshot", "efvm_checkpoint_save", "efvm_checkpoint_load", "efvm_write_vtk",
     "efvm_set_ghost_table", "efvm_state_sums_noalloc",
 ]
@@ -140,6 +140,7 @@ def load_library(path: str = LIB_PATH):
         "efvm_graph_launch": [vp],
         "efvm_ibm_add_cylinder": [vp, d, d, d, pd],
         "efvm_marker_force": [vp, pd],
+        "efvm_wall_force": [vp, pd],
         "efvm_conservation_sums": [vp, pd],
         "efvm_device_state": [vp, C.POINTER(vp)],
         "efvm_last_elapsed_ms": [vp, pd],
@@ -298,6 +299,12 @@ class Context:
     def marker_force(self) -> np.ndarray:
         out = np.zeros(6)
         self._chk(self._L.efvm_marker_force(self._p, _pd(out)))
+        return out
+
+    def wall_force(self) -> np.ndarray:
+        """Staircase wall force (pressure xyz, viscous xyz) on the cylinders, FP64."""
+        out = np.zeros(6)
+        self._chk(self._L.efvm_wall_force(self._p, _pd(out)))
         return out
 
     def sums(self) -> np.ndarray:
@@ -644,7 +651,7 @@ EXPORTS += [
     "efvm3_create", "efvm3_destroy", "efvm3_last_error", "efvm3_set_boundaries", "efvm3_set_time_step",
     "efvm3_set_body_force", "efvm3_set_sponge", "efvm3_upload_state", "efvm3_download_state", "efvm3_get_clock",
     "efvm3_set_clock", "efvm3_cfl", "efvm3_run", "efvm3_step", "efvm3_synchronize", "efvm3_graph_capture",
-    "efvm3_graph_launch", "efvm3_ibm_add_sphere", "efvm3_ibm_add_cylinder", "efvm3_marker_force",
+    "efvm3_graph_launch", "efvm3_ibm_add_sphere", "efvm3_ibm_add_cylinder", "efvm3_marker_force", "efvm3_wall_force",
     "efvm3_conservation_sums", "efvm3_last_elapsed_ms", "efvm3_checkpoint_save", "efvm3_checkpoint_load",
     "efvm3_write_vtk", "efvm3_set_ghost_table",
 ]
@@ -674,6 +681,7 @@ def _bind3(L):
         "efvm3_ibm_add_sphere": [vp, d, d, d, d, pd, C.c_int],
         "efvm3_ibm_add_cylinder": [vp, d, d, d, pd],
         "efvm3_marker_force": [vp, pd],
+        "efvm3_wall_force": [vp, pd],
         "efvm3_conservation_sums": [vp, pd],
         "efvm3_last_elapsed_ms": [vp, pd],
         "efvm3_checkpoint_save": [vp, C.c_char_p],
@@ -825,6 +833,12 @@ class Context3:
     def marker_force(self) -> np.ndarray:
         out = np.zeros(6)
         self._chk(self._L.efvm3_marker_force(self._p, _pd(out)))
+        return out
+
+    def wall_force(self) -> np.ndarray:
+        """Staircase wall force (pressure xyz, viscous xyz) on the bodies, FP64."""
+        out = np.zeros(6)
+        self._chk(self._L.efvm3_wall_force(self._p, _pd(out)))
         return out
 
     def sums(self) -> np.ndarray:

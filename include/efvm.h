@@ -152,6 +152,11 @@ efvm_status efvm_graph_launch(efvm_ctx* ctx);
 efvm_status efvm_ibm_add_cylinder(efvm_ctx* ctx, double cx, double cy, double r, const double freestream[5]);
 /* MarkerForce (pressure xyz, viscous xyz), FP64 (crates/cfd_ibm/src/lib.rs:442-480,601-615). */
 efvm_status efvm_marker_force(efvm_ctx* ctx, double out[6]);
+/* WallForce (pressure xyz, viscous xyz), FP64: the staircase estimate over solid-fluid faces
+ * (crates/cfd_ibm/src/lib.rs:309-362, refreshed by wall_force_system, lib.rs:383-395), summed
+ * in the reference's cell and face order.  Ghost-cell neighbours take the values the last
+ * substage's boundary fill gave them; zero before the first substage creates the mask. */
+efvm_status efvm_wall_force(efvm_ctx* ctx, double out[6]);
 /* FP64 volume-weighted interior sums of (rho, rho_u, rho_v, rho_w, rho_E). */
 efvm_status efvm_conservation_sums(efvm_ctx* ctx, double out[5]);
 /* The same sums with no allocation and no host synchronisation.  No counterpart in the
@@ -373,6 +378,10 @@ efvm_status efvm3_ibm_add_sphere(efvm_ctx3* ctx, double cx, double cy, double cz
 efvm_status efvm3_ibm_add_cylinder(efvm_ctx3* ctx, double cx, double cy, double r, const double freestream[5]);
 /* MarkerForce (pressure xyz, viscous xyz), FP64 (crates/cfd_ibm/src/lib.rs:442-480,601-615). */
 efvm_status efvm3_marker_force(efvm_ctx3* ctx, double out[6]);
+/* WallForce (pressure xyz, viscous xyz), FP64 staircase estimate over solid-fluid faces of the
+ * raw state, ghost cells included (crates/cfd_ibm/src/lib.rs:309-362), in the reference's
+ * summation order; zero before the first substage creates the mask. */
+efvm_status efvm3_wall_force(efvm_ctx3* ctx, double out[6]);
 /* FP64 volume-weighted interior sums of (rho, rho_u, rho_v, rho_w, rho_E). */
 efvm_status efvm3_conservation_sums(efvm_ctx3* ctx, double out[5]);
 /* Busy time of the last efvm3_step / efvm3_graph_launch (HIP events), milliseconds. */

@@ -324,6 +324,117 @@ def test_cylinder_ibm_muscl_fp64_matches_oracle():
     assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
 
 
+def wall_force_scales(sim, gamma=1.4):
+    """Bounds on the summed magnitudes of the staircase wall force's pressure and shear terms:
+    the largest pressure (resp. mu |u| / (dn/2)) of the state times the solid-fluid face area."""
+    st = sim.interior()[0]
+    rho, mom, en = st[..., 0], st[..., 1:4], st[..., 4]
+    p = (gamma - 1.0) * (en - 0.5 * (mom ** 2).sum(-1) / rho)
+    speed = np.sqrt((mom ** 2).sum(-1)) / rho
+    solid = sim.solid()[2, 2:-2, 2:-2].astype(bool)     # the one z layer of the raw mask, interior
+    nx, ny = solid.shape[1], solid.shape[0]
+    lx, ly, lz = sim.h
+    faces_x = (solid[:, :-1] != solid[:, 1:]).sum()
+    faces_y = (solid[:-1, :] != solid[1:, :]).sum()
+    area = faces_x * ly * lz + faces_y * lx * lz + 2 * solid.sum() * lx * ly
+    mu = sim.cfg.mu_const if sim.cfg.visc_model == 1 else 1.0e-4
+    return np.abs(p).max() * area, mu * speed.max() / (0.5 * min(lx, ly, lz)) * area
+
+
+def check_wall_force(sim, ctx, what, rtol=1e-11):
+    fo, fg = sim.wall_force(), ctx.wall_force()
+    sp, sv = wall_force_scales(sim)
+    dp, dv = np.abs(fg[:3] - fo[:3]).max() / sp, np.abs(fg[3:] - fo[3:]).max() / sv
+    print(f"{what}: wall force oracle {fo} gpu {fg} rel. to term scale: pressure {dp:.2e} shear {dv:.2e}")
+    assert dp <= rtol and dv <= rtol, what
+    return fo, fg
+
+
+def test_cylinder_wall_force_fp64_matches_oracle():
+    """Staircase wall force (crates/cfd_ibm/src/lib.rs:309-362) on the cylinder case, after whole
+    steps and after each RK3 substage: the device sums the faces in the oracle's order, and its
+    ghost cells (z slip images of the solid cells, the walls' boundary ghosts) carry the fills and
+    updates of the substages since the step boundary, as the reference's raw array does.
+    (Measured: bitwise equal to the oracle at every check.)"""
+    sim, ctx, q = cylinder_setup(220, 41, muscl=True)
+    assert np.array_equal(ctx.wall_force(), np.zeros(6)) and np.array_equal(sim.wall_force(), np.zeros(6))
+    for k in range(6):
+        sim.step(1)
+        ctx.step(1)
+        fo, _ = check_wall_force(sim, ctx, f"step {k + 1}")
+    assert fo[0] > 0.0 and abs(fo[2]) == 0.0        # drag along the flow; the two z faces cancel exactly
+    for s in range(3):
+        sim.run()
+        ctx.run()
+        check_wall_force(sim, ctx, f"substage {s + 1}")
+
+
+def test_cylinder_wall_force_euler_and_first_order_fp64():
+    """Forward Euler keeps one substage per step: the ghosts are the fill of U^n."""
+    sim, ctx, q = cylinder_setup(220, 41)
+    for k in range(4):
+        sim.step(1)
+        ctx.step(1)
+    check_wall_force(sim, ctx, "rk3 first order")
+    D, H, L, lz = 0.1, 0.41, 2.2, 0.041
+    rho0, p0, u_mean = 1.0, 1.0e5, 50.0
+    fs = orc.prim_to_cons([rho0, u_mean, 0, 0, p0, p0 / (rho0 * R)], 1.4)
+    common = dict(flux="rusanov", cfl=0.3, viscous=True, mu=0.1, integrator="euler")
+    sim = orc.Sim(110, 21, 1, hi=(L, H, lz), bcs=["supersonic_inflow", "outflow", "slip", "slip", "slip", "slip"],
+                  bc_params=[(rho0, u_mean, 0.0, 0.0, p0), None, None, None, None, None], cylinders=[(0.4, 0.2, 0.5 * D)],
+                  freestream=fs, **common)
+    sim.set_interior(np.broadcast_to(fs, (1, 21, 110, 5)).copy())
+    ctx = efvm.Context(110, 21, hi=(L, H), dz=lz, **common)
+    ctx.set_boundaries([("supersonic_inflow", (rho0, u_mean, 0.0, 0.0, p0)), "outflow", "slip", "slip"])
+    ctx.add_cylinder(0.4, 0.2, 0.5 * D, fs)
+    ctx.upload(sim.interior()[0])
+    for k in range(5):
+        sim.step(1)
+        ctx.step(1)
+        check_wall_force(sim, ctx, f"euler step {k + 1}")
+
+
+def test_wall_force_of_bodies_cut_by_the_boundaries_fp64():
+    """Two cylinders cut by the periodic x seam and the no-slip y-lo wall: solid cells whose
+    neighbours are periodic images of solid cells and wall ghosts of themselves, both taken as
+    the substages' fills saw them (before the IBM mirror overwrote the solid cells)."""
+    rho0, p0, u0 = 1.0, 1.0e5, 40.0
+    fs = orc.prim_to_cons([rho0, u0, 0, 0, p0, p0 / (rho0 * R)], 1.4)
+    nx, ny, L, H, lz = 100, 50, 1.0, 0.5, 0.01
+    cyl = [(0.0, 0.0, 0.08), (L, 0.0, 0.08), (0.5, 0.25, 0.06)]
+    common = dict(flux="hllc", muscl=True, limiter="minmod", cfl=0.3, viscous=True, mu=0.02)
+    sim = orc.Sim(nx, ny, 1, hi=(L, H, lz), bcs=["periodic", "periodic", "noslip", "slip", "slip", "slip"],
+                  cylinders=cyl, freestream=fs, **common)
+    sim.set_interior(np.broadcast_to(fs, (1, ny, nx, 5)).copy())
+    ctx = efvm.Context(nx, ny, hi=(L, H), dz=lz, **common)
+    ctx.set_boundaries(["periodic", "periodic", "noslip", "slip"])
+    for c in cyl:
+        ctx.add_cylinder(*c, fs)
+    ctx.upload(sim.interior()[0])
+    for k in range(4):
+        sim.step(1)
+        ctx.step(1)
+        check_wall_force(sim, ctx, f"step {k + 1}")
+    assert rel_linf(gpu_state(ctx), ora_state(sim)) <= TOL64
+    for s in range(2):
+        sim.run()
+        ctx.run()
+        check_wall_force(sim, ctx, f"substage {s + 1}")
+
+
+def test_fp32_cylinder_wall_force_tracks_fp64():
+    out = {}
+    for prec in ("fp64", "fp32"):
+        sim, ctx, q = cylinder_setup(220, 41, precision=prec)
+        ctx.step(20)
+        out[prec] = ctx.wall_force()
+    sim.step(20)
+    sp, sv = wall_force_scales(sim)
+    d = np.abs(out["fp32"] - out["fp64"])
+    print("fp32", out["fp32"], "fp64", out["fp64"], "rel. to term scale", d[:3].max() / sp, d[3:].max() / sv)
+    assert d[:3].max() <= 1e-5 * sp and d[3:].max() <= 1e-5 * sv
+
+
 # ------------------------------------------------------------------ FP32 production path
 def test_fp32_error_against_fp64_config2_physics():
     """FP32 (flux/update arithmetic) vs the FP64 path on the config-2 physics, 256^2, 50 steps.
