@@ -45,6 +45,19 @@ class UniformMeshConfig:
     ng: int = 2
     bounds_lo: Sequence[float] = (0.0, 0.0, 0.0)
     bounds_hi: Sequence[float] = (1.0, 1.0, 1.0)
+    # stretched y / z: ny+1 (nz+1) increasing cell edges from bounds_lo to bounds_hi, or None
+    y_edges: Optional[Sequence[float]] = None
+    z_edges: Optional[Sequence[float]] = None
+
+    def centroids(self, axis: int) -> np.ndarray:
+        """Interior cell centroids along an axis: edge midpoints on a stretched axis."""
+        e = (None, self.y_edges, self.z_edges)[axis]
+        if e is not None:
+            e = np.asarray(e, dtype=np.float64)
+            return 0.5 * (e[:-1] + e[1:])
+        n = (self.nx, self.ny, self.nz)[axis]
+        lo, hi = self.bounds_lo[axis], self.bounds_hi[axis]
+        return lo + (np.arange(n) + 0.5) * ((hi - lo) / n)
 
 
 @dataclass
@@ -450,7 +463,8 @@ class App:
         reg.validate_periodic_pairs()
         self.mesh = m
         thin = m.nz == 1 and all(reg.get(s).kind in ("slip", "symmetry") for s in (BoundarySide.ZLo, BoundarySide.ZHi))
-        if not thin or self._find(SphereIbmPlugin) is not None or Sponge in self.resources:
+        stretched = m.y_edges is not None or m.z_edges is not None
+        if not thin or stretched or self._find(SphereIbmPlugin) is not None or Sponge in self.resources:
             return self._prepare3(m, gasp, flux, integ, solver, reg, gpu, visc)
         lo, hi = m.bounds_lo, m.bounds_hi
         cfg = solver.config
@@ -510,15 +524,17 @@ class App:
                             limiter=flux.limiter.value, integrator=integ.kind, precision=gpu.precision,
                             viscous=cfg.viscous, seam_mode=gpu.seam_mode, cfl=cfg.cfl, fixed_dt=cfg.fixed_dt)
         self.ctx.set_boundaries([(reg.get(BoundarySide(s)).kind, reg.get(BoundarySide(s)).params) for s in range(6)])
+        for axis, e in ((1, m.y_edges), (2, m.z_edges)):
+            if e is not None:
+                self.ctx.set_edges(axis, e)
         # ghost tables: FnBoundary from the face centroids, FromOutside from BoundaryInputs
-        lo3, hi3, n3 = tuple(m.bounds_lo), tuple(m.bounds_hi), (m.nx, m.ny, m.nz)
+        lo3, hi3 = tuple(m.bounds_lo), tuple(m.bounds_hi)
         for s in range(6):
             bc = reg.get(BoundarySide(s))
             if isinstance(bc, FnBoundary):
                 ax = s // 2
                 u, w = (1, 2) if ax == 0 else ((0, 2) if ax == 1 else (0, 1))   # in-plane axes, faster first
-                cu = [lo3[u] + (q + 0.5) * (hi3[u] - lo3[u]) / n3[u] for q in range(n3[u])]
-                cw = [lo3[w] + (q + 0.5) * (hi3[w] - lo3[w]) / n3[w] for q in range(n3[w])]
+                cu, cw = m.centroids(u), m.centroids(w)
                 fc = []
                 for b in cw:
                     for a_ in cu:
@@ -566,8 +582,7 @@ class App:
     def cell_centroids(self):
         """x, y (2-D context) or x, y, z (3-D context) centroid arrays, x fastest."""
         m = self._find(FieldDefaultPlugins).mesh
-        c = [m.bounds_lo[d] + (np.arange(n) + 0.5) * ((m.bounds_hi[d] - m.bounds_lo[d]) / n)
-             for d, n in enumerate((m.nx, m.ny, m.nz))]
+        c = [m.centroids(d) for d in range(3)]
         if isinstance(self.ctx, Context3):
             z, y, x = np.meshgrid(c[2], c[1], c[0], indexing="ij")
             return x, y, z
@@ -599,6 +614,10 @@ class App:
 
     def marker_force(self) -> np.ndarray:
         return self.ctx.marker_force()
+
+    def wall_force(self) -> np.ndarray:
+        """WallForce (pressure xyz, viscous xyz), the staircase estimate (cfd_ibm:309-362, 383-395)."""
+        return self.ctx.wall_force()
 
     def conservation_sums(self) -> np.ndarray:
         return self.ctx.sums()

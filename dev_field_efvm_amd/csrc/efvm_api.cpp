@@ -1906,7 +1906,7 @@ namespace efvm {
 // cells (x fastest), face coordinates from the centroids, density, pressure, temperature,
 // velocity and Mach in %.10e with non-finite values written as 0.
 efvm_status vtk_write_cells(const int nn[3], const double lo[3], const double h[3], double gm, double R,
-                            const double* cons5, const char* path) {
+                            const double* cons5, const char* path, const double* const* edges) {
     FILE* f = std::fopen(path, "w");
     if (!f) return EFVM_ERR_ARG;
     std::fprintf(f, "# vtk DataFile Version 3.0\ndev_field_efvm output\nASCII\nDATASET RECTILINEAR_GRID\n");
@@ -1914,14 +1914,16 @@ efvm_status vtk_write_cells(const int nn[3], const double lo[3], const double h[
     // faces = centroid - half spacing, plus the last centroid + half spacing
     const char* label[3] = {"X", "Y", "Z"};
     for (int ax = 0; ax < 3; ax++) {
+        // a stretched axis: cell m on [e_m, e_{m+1}], centroid their midpoint, spacing their difference
+        const double* e = edges ? edges[ax] : nullptr;
+        auto cen = [&](int m) { return e ? 0.5 * (e[m] + e[m + 1]) : lo[ax] + ((double)m + 0.5) * h[ax]; };
+        auto wid = [&](int m) { return e ? e[m + 1] - e[m] : h[ax]; };
         std::fprintf(f, "%s_COORDINATES %d double\n", label[ax], nn[ax] + 1);
         for (int m = 0; m < nn[ax]; m++) {
-            const double cc = lo[ax] + ((double)m + 0.5) * h[ax];
-            fmt_e(f, cc - 0.5 * h[ax]);
+            fmt_e(f, cen(m) - 0.5 * wid(m));
             std::fputc('\n', f);
         }
-        const double cl = lo[ax] + ((double)(nn[ax] - 1) + 0.5) * h[ax];
-        fmt_e(f, cl + 0.5 * h[ax]);
+        fmt_e(f, cen(nn[ax] - 1) + 0.5 * wid(nn[ax] - 1));
         std::fputc('\n', f);
     }
     const long nc = (long)nn[0] * nn[1] * nn[2];
@@ -2065,7 +2067,7 @@ efvm_status efvm_write_vtk(const efvm_grid* g, const efvm_gas* gas, const double
     const int nn[3] = {g->nx, g->ny, 1};
     const double h[3] = {(g->hi_x - g->lo_x) / (double)g->nx, (g->hi_y - g->lo_y) / (double)g->ny, g->dz};
     const double lo[3] = {g->lo_x, g->lo_y, 0.0};
-    return efvm::vtk_write_cells(nn, lo, h, gas->gamma, gas->r_gas, cons5, path);
+    return efvm::vtk_write_cells(nn, lo, h, gas->gamma, gas->r_gas, cons5, path, nullptr);
 }
 
 }  // extern "C"

@@ -21,7 +21,7 @@
 
 namespace efvm {
 efvm_status vtk_write_cells(const int nn[3], const double lo[3], const double h[3], double gm, double R,
-                            const double* cons5, const char* path);
+                            const double* cons5, const char* path, const double* const* edges);
 }
 
 using namespace efvm;
@@ -50,6 +50,9 @@ struct efvm_ctx3 {
     double* out = nullptr;   // marker force (6) and sums (5)
     double* yw = nullptr;
     double* zw = nullptr;
+    std::vector<double> edges[3];  // stretched y / z: the n+1 cell edges (empty: uniform spacing h)
+    double* width_d[3] = {};      // device: interior cell widths of a stretched axis
+    double* cen_d[3] = {};        // device: interior cell centroids of a stretched axis
     uint8_t* solid = nullptr;     // raw-indexed
     long* ghost_idx = nullptr;
     double* ghost_img = nullptr;
@@ -92,7 +95,21 @@ efvm_status fail3(efvm_ctx3* c, efvm_status s, const std::string& msg) {
     } while (0)
 
 bool periodic3(const efvm_ctx3* c, int axis) { return c->bc[2 * axis].kind == EFVM_BC_PERIODIC; }
-double centroid3(const efvm_ctx3* c, int d, int raw) { return c->grid.lo[d] + ((double)(raw - NG3) + 0.5) * c->h[d]; }
+// Cell centroid along axis d at raw position `raw`.  A stretched axis (UniformMeshConfig y_edges /
+// z_edges) has cell i on [e_i, e_{i+1}] with centroid (e_i + e_{i+1}) / 2; its ghost centroids
+// are the periodic images of the opposite cells on a periodic axis, else the mirror images of the
+// interior cells across the boundary edge.
+double centroid3(const efvm_ctx3* c, int d, int raw) {
+    const std::vector<double>& e = c->edges[d];
+    if (e.empty()) return c->grid.lo[d] + ((double)(raw - NG3) + 0.5) * c->h[d];
+    const int n = (int)e.size() - 1, i = raw - NG3;
+    auto cc = [&](int q) { return 0.5 * (e[q] + e[q + 1]); };
+    if (i >= 0 && i < n) return cc(i);
+    const int g = i < 0 ? -i : i - n + 1;   // ghost layer 1 or 2
+    const double len = e[n] - e[0];
+    if (periodic3(c, d)) return i < 0 ? cc(n - g) - len : cc(g - 1) + len;
+    return i < 0 ? 2.0 * e[0] - cc(g - 1) : 2.0 * e[n] - cc(n - g);
+}
 long raw_index(const efvm_ctx3* c, int i, int j, int k) { return i + (long)c->NX * (j + (long)c->NY * k); }
 
 Args3 args3(const efvm_ctx3* c) {
@@ -122,6 +139,7 @@ Args3 args3(const efvm_ctx3* c) {
     a.viscous = c->sch.viscous; a.visc_const = c->gas.viscosity == EFVM_VISC_CONSTANT; a.mu = c->gas.mu;
     a.dxi2 = 1.0 / (c->h[0] * c->h[0]);
     a.yw = c->yw; a.zw = c->zw;
+    for (int d = 0; d < 3; d++) { a.spacing[d] = c->width_d[d]; a.cen[d] = c->cen_d[d]; }
     a.body_force = c->body_force; a.gx = c->gx;
     a.solid = c->n_body > 0 ? c->solid : nullptr;
     for (int s = 0; s < 6; s++) a.tab[s] = c->bc[s].kind == EFVM_BC_TABLE ? c->bc_tab[s] : nullptr;
@@ -227,6 +245,15 @@ std::vector<double> axis_weights(const efvm_ctx3* c, int d, int n) {
         w[q] = 1.0 / hm; w[n + q] = 1.0 / hp; w[2 * (size_t)n + q] = 2.0 / (hm + hp);
     }
     return w;
+}
+
+// The y and z viscous weight tables of the current geometry and boundaries.
+efvm_status upload_weights3(efvm_ctx3* c) {
+    const std::vector<double> yw = axis_weights(c, 1, c->grid.ny), zw = axis_weights(c, 2, c->grid.nz);
+    HIPCHK3(c, hipMemcpyAsync(c->yw, yw.data(), sizeof(double) * yw.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK3(c, hipMemcpyAsync(c->zw, zw.data(), sizeof(double) * zw.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK3(c, hipStreamSynchronize(c->stream));
+    return EFVM_OK;
 }
 
 // Signed distance and outward normal of body b at point p (sphere.rs:43-59; lib.rs cylinder).
@@ -388,10 +415,8 @@ efvm_status efvm3_create(const efvm_grid3* grid, const efvm_gas* gas, const efvm
     if (st != EFVM_OK) { efvm3_destroy(c); return st; }
     for (int s = 0; s < 3; s++)
         for (int m = 0; m < 5; m++) (void)hipMemsetAsync(c->P[s][m], 0, c->esz * c->nraw, c->stream);
-    const std::vector<double> yw = axis_weights(c, 1, grid->ny), zw = axis_weights(c, 2, grid->nz);
-    if (hipMemcpyAsync(c->yw, yw.data(), sizeof(double) * yw.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipMemcpyAsync(c->zw, zw.data(), sizeof(double) * zw.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        launch3_clock_set(c->clock, 0.0, 0.0, 0, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+    if (upload_weights3(c) != EFVM_OK || launch3_clock_set(c->clock, 0.0, 0.0, 0, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
         efvm3_destroy(c);
         return EFVM_ERR_HIP;
     }
@@ -409,6 +434,10 @@ void efvm3_destroy(efvm_ctx3* c) {
     if (c->out) (void)hipFree(c->out);
     if (c->yw) (void)hipFree(c->yw);
     if (c->zw) (void)hipFree(c->zw);
+    for (int d = 0; d < 3; d++) {
+        if (c->width_d[d]) (void)hipFree(c->width_d[d]);
+        if (c->cen_d[d]) (void)hipFree(c->cen_d[d]);
+    }
     for (int s = 0; s < 6; s++) if (c->bc_tab[s]) (void)hipFree(c->bc_tab[s]);
     free_ibm3(c);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -430,7 +459,35 @@ efvm_status efvm3_set_boundaries(efvm_ctx3* c, const efvm_bc sides[6]) {
                                                  " requires both sides periodic (crates/cfd_boundary/src/lib.rs:437-448)");
     for (int s = 0; s < 6; s++) c->bc[s] = sides[s];
     drop_graph3(c);
-    return EFVM_OK;
+    // a stretched axis's ghost centroids, and so its viscous weights, follow its periodicity
+    return upload_weights3(c);
+}
+
+efvm_status efvm3_set_edges(efvm_ctx3* c, int axis, const double* edges, size_t n_edges) {
+    if (!c || !edges) return EFVM_ERR_ARG;
+    if (axis != 1 && axis != 2)
+        return fail3(c, EFVM_ERR_UNSUPPORTED, "only y and z may be stretched: the viscous x term takes one spacing "
+                                              "for every cell (crates/cfd_solver/src/lib.rs:479-480)");
+    const int n = axis == 1 ? c->grid.ny : c->grid.nz;
+    if (n_edges != (size_t)n + 1) return fail3(c, EFVM_ERR_ARG, "edges: n + 1 values along the axis");
+    if (edges[0] != c->grid.lo[axis] || edges[n] != c->grid.hi[axis])
+        return fail3(c, EFVM_ERR_ARG, "edges: the first and last edge are the grid's lo and hi");
+    for (int i = 0; i < n; i++)
+        if (!(edges[i + 1] > edges[i])) return fail3(c, EFVM_ERR_ARG, "edges: strictly increasing");
+    if (c->n_body > 0) return fail3(c, EFVM_ERR_UNSUPPORTED, "immersed bodies need uniform spacing");
+    if (c->rk_stage != 0) return fail3(c, EFVM_ERR_ARG, "the mesh changes at a step boundary only");
+    std::vector<double> w(n), cen(n);
+    for (int i = 0; i < n; i++) { w[i] = edges[i + 1] - edges[i]; cen[i] = 0.5 * (edges[i] + edges[i + 1]); }
+    for (double** p : {&c->width_d[axis], &c->cen_d[axis]})
+        if (!*p) {
+            efvm_status st = alloc3(c, (void**)p, sizeof(double) * n);
+            if (st != EFVM_OK) return st;
+        }
+    c->edges[axis].assign(edges, edges + n + 1);
+    drop_graph3(c);
+    HIPCHK3(c, hipMemcpyAsync(c->width_d[axis], w.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK3(c, hipMemcpyAsync(c->cen_d[axis], cen.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    return upload_weights3(c);
 }
 
 efvm_status efvm3_set_ghost_table(efvm_ctx3* c, int side, const double* cons5, const unsigned char* given, size_t n) {
@@ -661,6 +718,7 @@ efvm_status efvm3_last_elapsed_ms(efvm_ctx3* c, double* ms) {
 efvm_status efvm3_ibm_add_sphere(efvm_ctx3* c, double cx, double cy, double cz, double r, const double fs[5], int n_markers) {
     if (!c || !fs || !(r > 0.0) || n_markers < 1) return EFVM_ERR_ARG;
     if (c->body_kind == 1) return fail3(c, EFVM_ERR_UNSUPPORTED, "a context holds spheres or cylinders, not both");
+    if (!c->edges[1].empty() || !c->edges[2].empty()) return fail3(c, EFVM_ERR_UNSUPPORTED, "immersed bodies need uniform spacing");
     if (c->n_body >= 8) return fail3(c, EFVM_ERR_UNSUPPORTED, "at most 8 bodies");
     if (c->rk_stage != 0) return fail3(c, EFVM_ERR_ARG, "bodies are added at a step boundary");
     c->body_kind = 2;
@@ -674,6 +732,7 @@ efvm_status efvm3_ibm_add_sphere(efvm_ctx3* c, double cx, double cy, double cz, 
 efvm_status efvm3_ibm_add_cylinder(efvm_ctx3* c, double cx, double cy, double r, const double fs[5]) {
     if (!c || !fs || !(r > 0.0)) return EFVM_ERR_ARG;
     if (c->body_kind == 2) return fail3(c, EFVM_ERR_UNSUPPORTED, "a context holds spheres or cylinders, not both");
+    if (!c->edges[1].empty() || !c->edges[2].empty()) return fail3(c, EFVM_ERR_UNSUPPORTED, "immersed bodies need uniform spacing");
     if (c->n_body >= 8) return fail3(c, EFVM_ERR_UNSUPPORTED, "at most 8 bodies");
     if (c->rk_stage != 0) return fail3(c, EFVM_ERR_ARG, "bodies are added at a step boundary");
     c->body_kind = 1;
@@ -866,7 +925,23 @@ efvm_status efvm3_write_vtk(const efvm_grid3* g, const efvm_gas* gas, const doub
     const int nn[3] = {g->nx, g->ny, g->nz};
     double lo[3], h[3];
     for (int d = 0; d < 3; d++) { lo[d] = g->lo[d]; h[d] = (g->hi[d] - g->lo[d]) / (double)nn[d]; }
-    return vtk_write_cells(nn, lo, h, gas->gamma, gas->r_gas, cons5, path);
+    return vtk_write_cells(nn, lo, h, gas->gamma, gas->r_gas, cons5, path, nullptr);
+}
+
+efvm_status efvm3_write_vtk_edges(const efvm_grid3* g, const double* y_edges, const double* z_edges, const efvm_gas* gas,
+                                  const double* cons5, size_t n_cells, const char* path) {
+    if (!g || !gas || !cons5 || !path || g->nx < 1 || g->ny < 1 || g->nz < 1) return EFVM_ERR_ARG;
+    if ((long)n_cells != (long)g->nx * g->ny * g->nz) return EFVM_ERR_ARG;
+    const int nn[3] = {g->nx, g->ny, g->nz};
+    const double* edges[3] = {nullptr, y_edges, z_edges};
+    for (int d = 1; d < 3; d++)
+        if (edges[d]) {
+            if (edges[d][0] != g->lo[d] || edges[d][nn[d]] != g->hi[d]) return EFVM_ERR_ARG;
+            for (int i = 0; i < nn[d]; i++) if (!(edges[d][i + 1] > edges[d][i])) return EFVM_ERR_ARG;
+        }
+    double lo[3], h[3];
+    for (int d = 0; d < 3; d++) { lo[d] = g->lo[d]; h[d] = (g->hi[d] - g->lo[d]) / (double)nn[d]; }
+    return vtk_write_cells(nn, lo, h, gas->gamma, gas->r_gas, cons5, path, edges);
 }
 
 efvm_status efvm3_conservation_sums(efvm_ctx3* c, double out[5]) {
@@ -874,7 +949,9 @@ efvm_status efvm3_conservation_sums(efvm_ctx3* c, double out[5]) {
     Args3 a = args3(c);
     const int set = current_set(c);
     planes3(a, c->P[set], c->P[set], c->P[set]);
-    const double vol = c->h[0] * c->h[1] * c->h[2];
+    // a stretched mesh weighs each cell by its own volume inside the kernel
+    const bool stretched = c->width_d[1] || c->width_d[2];
+    const double vol = stretched ? 1.0 : c->h[0] * c->h[1] * c->h[2];
     const int nb = (int)std::min<long>(1024, (c->ncell + 255) / 256);
     LAUNCH3(c, launch3_sums(a, (int)c->esz, c->part, nb, vol, c->out + 8, c->stream), "sums");
     HIPCHK3(c, hipMemcpyAsync(out, c->out + 8, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));

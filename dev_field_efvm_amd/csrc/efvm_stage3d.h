@@ -13,6 +13,7 @@ struct Args3 {
     long nraw, ncell;
     double lo[3], h[3];
     const double* spacing[3];   // per-axis cell widths (interior cells) of a stretched axis, or null: h
+    const double* cen[3];       // per-axis cell centroids (interior cells) of a stretched axis, or null
     double aov[3];         // face area / cell volume per axis
     int per[3];
     int seam_mode;

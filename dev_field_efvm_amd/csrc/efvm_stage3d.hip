@@ -419,6 +419,20 @@ __device__ __forceinline__ void finish_cell(const Args3& a, const T* const* U, c
     st5(O, q, o);
 }
 
+// Face area over cell volume of interior cell (., j, k) per axis (crates/cfd_solver/src/lib.rs:
+// 418-424: area dy dz, dx dz, dx dy over the volume dx dy dz of the cell's own spacings; a face
+// shared along an axis has the same area on both sides, so each cell scales what it receives by
+// its own ratios).  Uniform spacing: the precomputed ratios.
+__device__ __forceinline__ void cell_aov(const Args3& a, int j, int k, double o[3]) {
+    if (!a.spacing[1] && !a.spacing[2]) {
+        o[0] = a.aov[0]; o[1] = a.aov[1]; o[2] = a.aov[2];
+        return;
+    }
+    const double dx = a.h[0], dy = a.spacing[1] ? a.spacing[1][j] : a.h[1], dz = a.spacing[2] ? a.spacing[2][k] : a.h[2];
+    const double vol = dx * dy * dz;
+    o[0] = (dy * dz) / vol; o[1] = (dx * dz) / vol; o[2] = (dx * dy) / vol;
+}
+
 // One thread per raw cell: each interior cell evaluates its own faces and those of its
 // lower neighbours, so every interior face is evaluated twice, by the two cells it
 // separates, with the same operands.  (An x-y tiled variant that evaluates each face once
@@ -440,16 +454,18 @@ __global__ __launch_bounds__(256) void stage3_kernel(Args3 a) {
     V5<T> rhs = {T(0), T(0), T(0), T(0), T(0)};
     const bool interior = id[0] >= 0 && id[0] < n[0] && id[1] >= 0 && id[1] < n[1] && id[2] >= 0 && id[2] < n[2];
     if (interior) {
+        double aov[3];
+        cell_aov(a, id[1], id[2], aov);
         // + faces of the lower neighbours, in the sweep's visiting order (z, y, x)
         for (int ax = 2; ax >= 0; --ax) {
             if (id[ax] > 0) {
                 const V5<T> f = plus_face<T, FLUX, MUSCL>(a, U, q - st[ax], ax, id[ax] - 1, st[ax], n[ax]);
-                rhs = ref::add(rhs, ref::mul(f, T(a.aov[ax])));
+                rhs = ref::add(rhs, ref::mul(f, T(aov[ax])));
             }
         }
         // own faces: + face, then the - face on a low edge, per axis
         for (int ax = 0; ax < 3; ++ax) {
-            const T inv = T(a.aov[ax]);
+            const T inv = T(aov[ax]);
             rhs = ref::sub(rhs, ref::mul(plus_face<T, FLUX, MUSCL>(a, U, q, ax, id[ax], st[ax], n[ax]), inv));
             if (id[ax] == 0) rhs = ref::sub(rhs, ref::mul(minus_face<T, FLUX, MUSCL>(a, U, q, ax, st[ax], n[ax]), inv));
         }
@@ -482,11 +498,12 @@ __global__ __launch_bounds__(256) void cfl3_kernel(Args3 a, double* part) {
         double v[3];
         ref::velocity(u, v);
         const double s = ref::sound(u, a.gamma);
-        double lam = (fabs(v[0]) + s) / a.h[0] + (fabs(v[1]) + s) / a.h[1] + (fabs(v[2]) + s) / a.h[2];
+        const double dx = a.h[0], dy = a.spacing[1] ? a.spacing[1][j] : a.h[1], dz = a.spacing[2] ? a.spacing[2][k] : a.h[2];
+        double lam = (fabs(v[0]) + s) / dx + (fabs(v[1]) + s) / dy + (fabs(v[2]) + s) / dz;
         if (a.viscous) {
             const P6<double> p = ref::cons_to_prim(u, a.gamma, a.r_gas);
             const double nu = (a.visc_const ? a.mu : ref::sutherland(p.t)) / u.r;
-            lam += 2.0 * nu * (1.0 / (a.h[0] * a.h[0]) + 1.0 / (a.h[1] * a.h[1]) + 1.0 / (a.h[2] * a.h[2]));
+            lam += 2.0 * nu * (1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz));
         }
         if (lam == lam) m = fmax(m, lam);
         else bad++;
@@ -534,7 +551,7 @@ __global__ __launch_bounds__(256) void sponge3_kernel(Args3 a, Sponge3 s) {
     if (s.uniform) {
         tg = {s.target[0], s.target[1], s.target[2], s.target[3], s.target[4]};
     } else {
-        const double cy = a.lo[1] + ((double)j + 0.5) * a.h[1];
+        const double cy = a.cen[1] ? a.cen[1][j] : a.lo[1] + ((double)j + 0.5) * a.h[1];
         const double uy = 4.0 * s.umax * cy * (s.height - cy) / (s.height * s.height);
         P6<double> pv = {s.rho, uy, 0.0, 0.0, s.p, s.p / (s.rho * 287.058)};
         tg = ref::prim_to_cons(pv, a.gamma);
@@ -562,7 +579,12 @@ __global__ __launch_bounds__(256) void sums3_kernel(Args3 a, double* part) {
         const long jk = c / a.nx;
         const int j = (int)(jk % a.ny), k = (int)(jk / a.ny);
         const long q = (i + NGH) + (long)a.NX * ((j + NGH) + (long)a.NY * (k + NGH));
-        for (int m = 0; m < 5; m++) s5[m] += (double)U[m][q];
+        if (a.spacing[1] || a.spacing[2]) {   // stretched: each cell weighed by its volume
+            const double vol = a.h[0] * (a.spacing[1] ? a.spacing[1][j] : a.h[1]) * (a.spacing[2] ? a.spacing[2][k] : a.h[2]);
+            for (int m = 0; m < 5; m++) s5[m] += (double)U[m][q] * vol;
+        } else {
+            for (int m = 0; m < 5; m++) s5[m] += (double)U[m][q];
+        }
     }
     for (int m = 0; m < 5; m++) sh[threadIdx.x][m] = s5[m];
     __syncthreads();

@@ -653,7 +653,7 @@ EXPORTS += [
     "efvm3_set_clock", "efvm3_cfl", "efvm3_run", "efvm3_step", "efvm3_synchronize", "efvm3_graph_capture",
     "efvm3_graph_launch", "efvm3_ibm_add_sphere", "efvm3_ibm_add_cylinder", "efvm3_marker_force", "efvm3_wall_force",
     "efvm3_conservation_sums", "efvm3_last_elapsed_ms", "efvm3_checkpoint_save", "efvm3_checkpoint_load",
-    "efvm3_write_vtk", "efvm3_set_ghost_table",
+    "efvm3_write_vtk", "efvm3_set_ghost_table", "efvm3_set_edges", "efvm3_write_vtk_edges",
 ]
 
 
@@ -687,6 +687,8 @@ def _bind3(L):
         "efvm3_checkpoint_save": [vp, C.c_char_p],
         "efvm3_checkpoint_load": [vp, C.c_char_p],
         "efvm3_write_vtk": [C.POINTER(Grid3), C.POINTER(Gas), pd, C.c_size_t, C.c_char_p],
+        "efvm3_set_edges": [vp, C.c_int, pd, C.c_size_t],
+        "efvm3_write_vtk_edges": [C.POINTER(Grid3), pd, pd, C.POINTER(Gas), pd, C.c_size_t, C.c_char_p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -717,6 +719,7 @@ class Context3:
         self.precision = precision
         self.grid, self.gas, self.scheme = g, gs, sc
         self.lo, self.hi = tuple(lo), tuple(hi)
+        self.edges = {}
         p = C.c_void_p()
         st = L.efvm3_create(C.byref(g), C.byref(gs), C.byref(sc), C.byref(p))
         if st != OK:
@@ -752,6 +755,13 @@ class Context3:
             for q, v in enumerate(par):
                 arr[s].par[q] = v
         self._chk(self._L.efvm3_set_boundaries(self._p, arr))
+
+    def set_edges(self, axis: int, edges):
+        """Stretch axis 1 (y) or 2 (z): its n+1 strictly increasing cell edges from lo to hi
+        (the reference's UniformMeshConfig y_edges / z_edges)."""
+        e = np.ascontiguousarray(edges, dtype=np.float64)
+        self._chk(self._L.efvm3_set_edges(self._p, int(axis), _pd(e), e.size))
+        self.edges[int(axis)] = e.copy()
 
     def set_ghost_table(self, side: int, states, given):
         """BoundaryInputs of a FromOutside side (kind "table"), side 0..5 = XLo .. ZHi: states
@@ -856,12 +866,14 @@ class Context3:
 
     def write_vtk(self, path: str):
         """VTK legacy output of the current state."""
-        write_vtk3(path, self.download(), self.nx, self.ny, self.nz, self.lo, self.hi, self.gas.gamma, self.gas.r_gas)
+        write_vtk3(path, self.download(), self.nx, self.ny, self.nz, self.lo, self.hi, self.gas.gamma, self.gas.r_gas,
+                   y_edges=self.edges.get(1), z_edges=self.edges.get(2))
 
 
 def write_vtk3(path: str, state, nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), gamma=1.4,
-               r_gas=287.058):
-    """VTK legacy RECTILINEAR_GRID file of a wire-order 3-D state (nz, ny, nx, 5) on the host."""
+               r_gas=287.058, y_edges=None, z_edges=None):
+    """VTK legacy RECTILINEAR_GRID file of a wire-order 3-D state (nz, ny, nx, 5) on the host;
+    y_edges / z_edges: the cell edges of a stretched axis."""
     L = load_library()
     _bind3(L)
     g = Grid3(int(nx), int(ny), int(nz), (C.c_double * 3)(*lo), (C.c_double * 3)(*hi))
@@ -869,7 +881,16 @@ def write_vtk3(path: str, state, nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), 
     u = np.ascontiguousarray(state, dtype=np.float64).reshape(-1)
     if u.size != 5 * nx * ny * nz:
         raise ValueError("state must hold 5 values per cell")
-    st = L.efvm3_write_vtk(C.byref(g), C.byref(gs), u.ctypes.data_as(C.POINTER(C.c_double)), nx * ny * nz,
-                           os.fsencode(path))
+    pu = u.ctypes.data_as(C.POINTER(C.c_double))
+    if y_edges is None and z_edges is None:
+        st = L.efvm3_write_vtk(C.byref(g), C.byref(gs), pu, nx * ny * nz, os.fsencode(path))
+    else:
+        ey = None if y_edges is None else np.ascontiguousarray(y_edges, dtype=np.float64)
+        ez = None if z_edges is None else np.ascontiguousarray(z_edges, dtype=np.float64)
+        for e, n in ((ey, ny), (ez, nz)):
+            if e is not None and e.size != n + 1:
+                raise ValueError("edges: n + 1 values along the axis")
+        st = L.efvm3_write_vtk_edges(C.byref(g), None if ey is None else _pd(ey), None if ez is None else _pd(ez),
+                                     C.byref(gs), pu, nx * ny * nz, os.fsencode(path))
     if st != OK:
         raise EfvmError(st, f"efvm3_write_vtk({path}) failed")

@@ -334,6 +334,16 @@ const char* efvm3_last_error(const efvm_ctx3* ctx);
 /* BoundaryRegistry (crates/cfd_boundary/src/lib.rs:369-449): sides[6] = XLo, XHi, YLo, YHi,
  * ZLo, ZHi; a half-periodic axis is EFVM_ERR_CONFIG. */
 efvm_status efvm3_set_boundaries(efvm_ctx3* ctx, const efvm_bc sides[6]);
+/* Stretched y (axis 1) or z (axis 2) spacing, UniformMeshConfig { y_edges, z_edges } of the
+ * reference's mesh: the axis's n+1 strictly increasing cell edges from lo to hi.  Cell i spans
+ * [e_i, e_{i+1}]: spacing e_{i+1} - e_i, centroid (e_i + e_{i+1}) / 2.  The face area over cell
+ * volume of the flux divergence (crates/cfd_solver/src/lib.rs:418-434), the CFL spacing
+ * (lib.rs:644-660), the viscous y / z weights from centroid distances (lib.rs:483-501) and the
+ * conservation sums follow the cell.  Ghost centroids: periodic images on a periodic axis, else
+ * mirror images across the boundary edge.  x stays uniform (the viscous x term takes
+ * spacing(0, 0) for every cell, lib.rs:479-480); immersed bodies need uniform spacing.  At a
+ * step boundary. */
+efvm_status efvm3_set_edges(efvm_ctx3* ctx, int axis, const double* edges, size_t n_edges);
 /* BoundaryInputs (crates/cfd_boundary/src/lib.rs:106-111) of a FromOutside side (kind
  * EFVM_BC_TABLE, lib.rs:115-147): cons5[5 n] ghost states in Component wire order and
  * given[n]; a face with given = 0 copies its interior cell (SupersonicOutflow).  One entry
@@ -396,6 +406,11 @@ efvm_status efvm3_checkpoint_load(efvm_ctx3* ctx, const char* path);
  * crates/cfd_output/src/lib.rs:36-113 (the 2-D writer's fields and format).  Host only. */
 efvm_status efvm3_write_vtk(const efvm_grid3* grid, const efvm_gas* gas, const double* cons5, size_t n_cells,
                             const char* path);
+/* The same with stretched y / z axes (edges as efvm3_set_edges takes them, null = uniform):
+ * each axis's coordinates are centroid - spacing/2 per cell and centroid + spacing/2 of the
+ * last (crates/cfd_output/src/lib.rs:50-72). */
+efvm_status efvm3_write_vtk_edges(const efvm_grid3* grid, const double* y_edges, const double* z_edges,
+                                  const efvm_gas* gas, const double* cons5, size_t n_cells, const char* path);
 
 #ifdef __cplusplus
 }

@@ -100,6 +100,9 @@ typedef struct {
     int NX, NY, NZ;
     long ntot;
     double h[3];
+    /* stretched y / z axes (UniformMeshConfig y_edges / z_edges): widths of the interior cells
+       and centroids of every raw position along the axis; null: uniform spacing h */
+    double *width[3], *cen[3];
     cv_t *u, *rhs, *u0;
     cv_t (*slope)[3];
     pv_t *prim;
@@ -297,11 +300,17 @@ static inline cv_t limited(cv_t f, cv_t b, int lim) {
 }
 
 /* ---------------- mesh helpers (field_core semantics, see header) ---------------- */
-static inline double centroid(const orc_sim *s, int d, int raw) { return s->c.lo[d] + ((double)(raw - NG) + 0.5) * s->h[d]; }
+static inline double centroid(const orc_sim *s, int d, int raw) {
+    return s->cen[d] ? s->cen[d][raw] : s->c.lo[d] + ((double)(raw - NG) + 0.5) * s->h[d];
+}
+/* StructuredMesh::spacing(axis, i) of interior cell i */
+static inline double spacing(const orc_sim *s, int d, int i) { return s->width[d] ? s->width[d][i] : s->h[d]; }
 static inline int is_local(const orc_sim *s, int i, int j, int k) {
     return i >= NG && i < NG + s->c.nx && j >= NG && j < NG + s->c.ny && k >= NG && k < NG + s->c.nz;
 }
-static inline double cell_volume(const orc_sim *s) { return s->h[0] * s->h[1] * s->h[2]; }
+/* cell volume of interior cell (., j, k): dx dy_j dz_k (x is uniform) */
+static inline double cell_volume_at(const orc_sim *s, int j, int k) { return s->h[0] * spacing(s, 1, j) * spacing(s, 2, k); }
+static inline double cell_volume(const orc_sim *s) { return cell_volume_at(s, 0, 0); }
 static inline int axis_periodic(const orc_sim *s, int d) { return s->c.bc[2 * d] == BC_PERIODIC && s->c.bc[2 * d + 1] == BC_PERIODIC; }
 
 /* ---------------- lifecycle ---------------- */
@@ -328,7 +337,7 @@ void orc_free(orc_sim *s) {
     free(s->u); free(s->rhs); free(s->u0); free(s->slope); free(s->prim); free(s->solid);
     for (int q = 0; q < 6; q++) free(s->tab[q]);
     free(s->ghost_idx); free(s->ghost_img); free(s->mark);
-    for (int d = 0; d < 3; d++) { free(s->fplus[d]); free(s->fminus[d]); }
+    for (int d = 0; d < 3; d++) { free(s->fplus[d]); free(s->fminus[d]); free(s->width[d]); free(s->cen[d]); }
     free(s);
 }
 long orc_ntot(const orc_sim *s) { return s->ntot; }
@@ -343,6 +352,32 @@ int orc_set_threads(orc_sim *s, int n) {
     return s->threads;
 }
 #define OMP_THREADS(s) num_threads((s)->threads > 1 ? (s)->threads : 1) if((s)->threads > 1)
+
+/* Stretched axis (UniformMeshConfig y_edges / z_edges): n+1 increasing cell-edge coordinates
+ * of axis d (1 = y, 2 = z) from lo to hi.  Interior cell i spans [e_i, e_{i+1}]: spacing
+ * e_{i+1} - e_i, centroid (e_i + e_{i+1}) / 2.  Ghost centroids: on a periodic axis the
+ * periodic image of the opposite cell, else the mirror image of the interior cell across the
+ * boundary edge.  x stays uniform (the viscous x term uses spacing(0, 0) for every cell,
+ * crates/cfd_solver/src/lib.rs:479-480), and immersed bodies need a uniform mesh.
+ * Returns 0, or -1 for a bad axis, count, order, end points, or a body. */
+static int n_bodies(const orc_sim *s);
+int orc_set_edges(orc_sim *s, int d, const double *e, int n_edges) {
+    int n = d == 1 ? s->c.ny : (d == 2 ? s->c.nz : -1);
+    if (n < 0 || n_edges != n + 1 || n_bodies(s) > 0) return -1;
+    if (e[0] != s->c.lo[d] || e[n] != s->c.hi[d]) return -1;
+    for (int i = 0; i < n; i++) if (!(e[i + 1] > e[i])) return -1;
+    free(s->width[d]); free(s->cen[d]);
+    s->width[d] = (double *)malloc(sizeof(double) * n);
+    s->cen[d] = (double *)malloc(sizeof(double) * (n + 2 * NG));
+    for (int i = 0; i < n; i++) { s->width[d][i] = e[i + 1] - e[i]; s->cen[d][NG + i] = 0.5 * (e[i] + e[i + 1]); }
+    double len = e[n] - e[0];
+    for (int g = 1; g <= NG; g++) {
+        double *lo = &s->cen[d][NG - g], *hi = &s->cen[d][NG + n - 1 + g];
+        if (axis_periodic(s, d)) { *lo = s->cen[d][NG + n - g] - len; *hi = s->cen[d][NG + g - 1] + len; }
+        else { *lo = 2.0 * e[0] - s->cen[d][NG + g - 1]; *hi = 2.0 * e[n] - s->cen[d][NG + n - g]; }
+    }
+    return 0;
+}
 
 /* Raw (padded) state, Component wire order (rho, rho_u, rho_v, rho_w, rho_E) per cell. */
 void orc_set_raw(orc_sim *s, const double *src) { memcpy(s->u, src, sizeof(cv_t) * s->ntot); }
@@ -406,8 +441,9 @@ void orc_ic_vortex(orc_sim *s, double L, double beta, double gam, double mu_, do
 }
 /* Volume-weighted L1 density error against the translated vortex (main.rs:189-211). */
 double orc_vortex_l1(const orc_sim *s, double L, double beta, double gam, double mu_, double mv, int periodic) {
-    double num = 0.0, vol = 0.0, cvv = cell_volume(s);
+    double num = 0.0, vol = 0.0;
     for (int j = 0; j < s->c.ny; j++) for (int i = 0; i < s->c.nx; i++) {
+        const double cvv = cell_volume_at(s, j, 0);
         long c = IDX(s, i + NG, j + NG, NG);
         double x = centroid(s, 0, i + NG), y = centroid(s, 1, j + NG);
         double ex;
@@ -471,9 +507,9 @@ void orc_ic_channel(orc_sim *s, double um, double h, double rho0, double p0, dou
 }
 /* Cell-volume-weighted interior sums of the 5 conserved components. */
 void orc_sums(const orc_sim *s, double *out) {
-    double cvv = cell_volume(s);
     for (int q = 0; q < 5; q++) out[q] = 0.0;
     for (int k = 0; k < s->c.nz; k++) for (int j = 0; j < s->c.ny; j++) for (int i = 0; i < s->c.nx; i++) {
+        const double cvv = cell_volume_at(s, j, k);
         const cv_t *u = &s->u[IDX(s, i + NG, j + NG, k + NG)];
         out[0] += u->r * cvv; out[1] += u->ru * cvv; out[2] += u->rv * cvv; out[3] += u->rw * cvv; out[4] += u->re * cvv;
     }
@@ -485,10 +521,10 @@ static void cfl_system(orc_sim *s) {                                /* cfd_solve
     if (s->stage != STAGE_FIRST) return;
     if (s->c.fixed_dt > 0.0) { s->dt = s->c.fixed_dt; return; }
     double g = s->c.gamma, lmax = 0.0;
-    double dx = s->h[0], dy = s->h[1], dz = s->h[2];
     /* a maximum does not depend on the visiting order (fmax ignores a NaN lambda either way) */
 #pragma omp parallel for OMP_THREADS(s) reduction(max_fmax : lmax)
     for (int i = 0; i < s->c.nx; i++) for (int j = 0; j < s->c.ny; j++) for (int k = 0; k < s->c.nz; k++) {
+        const double dx = spacing(s, 0, i), dy = spacing(s, 1, j), dz = spacing(s, 2, k);
         const cv_t *u = &s->u[IDX(s, i + NG, j + NG, k + NG)];
         if (u->r <= 0.0) continue;
         double v[3]; vel_of(u, v);
@@ -794,7 +830,7 @@ static void wall_force_system(orc_sim *s) {                         /* cfd_ibm:3
         int ir = i + NG, jr = j + NG, kr = k + NG;
         long c = IDX(s, ir, jr, kr);
         if (!s->solid[c]) continue;
-        double dx = s->h[0], dy = s->h[1], dz = s->h[2];
+        double dx = spacing(s, 0, i), dy = spacing(s, 1, j), dz = spacing(s, 2, k);
         double fa[3] = {dy * dz, dx * dz, dx * dy}, dn[3] = {dx, dy, dz};
         long nb[6] = {IDX(s, ir + 1, jr, kr), IDX(s, ir - 1, jr, kr), IDX(s, ir, jr + 1, kr), IDX(s, ir, jr - 1, kr), IDX(s, ir, jr, kr + 1), IDX(s, ir, jr, kr - 1)};
         int dims[6] = {0, 0, 1, 1, 2, 2};
@@ -881,8 +917,6 @@ static cv_t sweep_face(const orc_sim *s, int i, int j, int k, int ax, int side, 
 static void flux_sweep_threaded(orc_sim *s) {
     int n[3] = {s->c.nx, s->c.ny, s->c.nz};
     long nin = (long)n[0] * n[1] * n[2];
-    double vol = cell_volume(s);
-    double area[3] = {s->h[1] * s->h[2], s->h[0] * s->h[2], s->h[0] * s->h[1]};
     long edge_n[3] = {(long)n[1] * n[2], (long)n[0] * n[2], (long)n[0] * n[1]};
     for (int d = 0; d < 3; d++) {
         if (!s->fplus[d]) s->fplus[d] = (cv_t *)malloc(sizeof(cv_t) * nin);
@@ -905,6 +939,9 @@ static void flux_sweep_threaded(orc_sim *s) {
         long q = i + (long)n[0] * (j + (long)n[1] * k);
         long sq[3] = {1, n[0], (long)n[0] * n[1]};
         int id[3] = {i, j, k};
+        /* area / volume of this cell: faces shared along an axis have the same area on both sides */
+        const double dx = spacing(s, 0, i), dy = spacing(s, 1, j), dz = spacing(s, 2, k), vol = dx * dy * dz;
+        const double area[3] = {dy * dz, dx * dz, dx * dy};
         cv_t acc = s->rhs[IDX(s, i + NG, j + NG, k + NG)];
         for (int ax = 2; ax >= 0; ax--)               /* + faces of the z, y, x neighbours */
             if (id[ax] >= 1) acc = cv_add(acc, cv_mul(s->fplus[ax][q - sq[ax]], area[ax] / vol));
@@ -925,13 +962,14 @@ static void flux_sweep(orc_sim *s) {                                /* cfd_solve
     int muscl = s->c.recon == RECON_MUSCL;
     int scheme = s->c.flux;
     double g = s->c.gamma;
-    double vol = cell_volume(s);
-    double area[3] = {s->h[1] * s->h[2], s->h[0] * s->h[2], s->h[0] * s->h[1]};
     long st[3] = {1, s->NX, (long)s->NX * s->NY};
     int n[3] = {s->c.nx, s->c.ny, s->c.nz};
     for (int k = 0; k < n[2]; k++) for (int j = 0; j < n[1]; j++) for (int i = 0; i < n[0]; i++) {
         int id[3] = {i, j, k};
         long own = IDX(s, i + NG, j + NG, k + NG);
+        /* cfd_solver:418-424: the owner's spacings give the face area and its volume */
+        const double dx = spacing(s, 0, i), dy = spacing(s, 1, j), dz = spacing(s, 2, k), vol = dx * dy * dz;
+        const double area[3] = {dy * dz, dx * dz, dx * dy};
         for (int ax = 0; ax < 3; ax++) {
             double inv = area[ax] / vol;
             for (int side = 0; side < 2; side++) {        /* 0: + face; 1: - face at the low edge */
@@ -953,7 +991,12 @@ static void flux_sweep(orc_sim *s) {                                /* cfd_solve
                 }
                 cv_t fn = flux_normal_axis(scheme, &ul, &ur, g, ax, sign);
                 s->rhs[own] = cv_sub(s->rhs[own], cv_mul(fn, inv));
-                if (!patch && other_local) s->rhs[oth] = cv_add(s->rhs[oth], cv_mul(fn, area[ax] / vol));
+                if (!patch && other_local) {
+                    /* the face's area over the neighbour's volume (cfd_solver:432-434) */
+                    int jo = j + (ax == 1), ko = k + (ax == 2);
+                    double vol_o = spacing(s, 0, i + (ax == 0)) * spacing(s, 1, jo) * spacing(s, 2, ko);
+                    s->rhs[oth] = cv_add(s->rhs[oth], cv_mul(fn, area[ax] / vol_o));
+                }
             }
         }
     }

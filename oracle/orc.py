@@ -75,6 +75,8 @@ def lib():
         L.orc_ntot.argtypes = [C.c_void_p]
         L.orc_set_threads.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_edges.restype = C.c_int
+        L.orc_set_edges.argtypes = [C.c_void_p, C.c_int, pd, C.c_int]
         for n in ("orc_set_raw", "orc_get_raw", "orc_get_rhs_raw", "orc_get_u0_raw", "orc_set_interior", "orc_get_interior", "orc_get_rhs_interior",
                   "orc_marker_force", "orc_wall_force", "orc_sums"):
             getattr(L, n).argtypes = [C.c_void_p, pd]
@@ -286,6 +288,22 @@ class Sim:
         if getattr(self, "ptr", None):
             lib().orc_free(self.ptr)
             self.ptr = None
+
+    def set_edges(self, axis: int, edges):
+        """Stretch axis 1 (y) or 2 (z): its n+1 increasing cell-edge coordinates, lo to hi."""
+        e = np.ascontiguousarray(edges, dtype=np.float64)
+        if lib().orc_set_edges(self.ptr, int(axis), _p(e), int(e.size)) != 0:
+            raise ValueError("edges: axis 1 or 2, n+1 increasing values from lo to hi, no immersed body")
+        self.edges = getattr(self, "edges", {})
+        self.edges[int(axis)] = e.copy()
+
+    def centroids(self, axis: int) -> np.ndarray:
+        """Interior cell centroids along an axis."""
+        e = getattr(self, "edges", {}).get(axis)
+        if e is not None:
+            return 0.5 * (e[:-1] + e[1:])
+        n = (self.nx, self.ny, self.nz)[axis]
+        return self.lo[axis] + (np.arange(n) + 0.5) * self.h[axis]
 
     def set_threads(self, n: int) -> int:
         """Run steps on `n` OpenMP threads (bitwise the serial result); returns the count in use."""

@@ -132,6 +132,77 @@ def test_euler_outflow_body_force_fixed_dt_fp64():
     check_clock(sim, ctx)
 
 
+# ------------------------------------------------------------------ stretched y / z
+def tanh_edges(n, lo, hi, beta):
+    """n cells on [lo, hi], clustered towards both ends."""
+    x = np.tanh(beta * (2.0 * np.arange(n + 1) / n - 1.0)) / np.tanh(beta)
+    e = lo + 0.5 * (hi - lo) * (1.0 + x)
+    e[0], e[-1] = lo, hi
+    return e
+
+
+@pytest.mark.parametrize("ybc,zbc", [("walls", "periodic"), ("periodic", "noslip")])
+def test_stretched_y_z_viscous_muscl_fp64_matches_oracle(ybc, zbc):
+    """UniformMeshConfig y_edges / z_edges: face area over cell volume per cell, CFL spacing per
+    cell, viscous y / z weights from centroid distances (periodic images or mirror images across
+    the wall for the ghosts), volume-weighted sums."""
+    n, hi = (10, 14, 8), (1.0e-3, 1.4e-3, 0.8e-3)
+    ysides = ["noslip", "slip"] if ybc == "walls" else ["periodic", "periodic"]
+    bcs = ["periodic", "periodic"] + ysides + [zbc, zbc]
+    sim, ctx = make_pair(n, hi, bcs, flux="hllc", muscl=True, limiter="mc", viscous=True, cfl=0.3)
+    ey, ez = tanh_edges(n[1], 0.0, hi[1], 1.4), tanh_edges(n[2], 0.0, hi[2], 1.1)
+    for obj in (sim, ctx):
+        obj.set_edges(1, ey)
+        obj.set_edges(2, ez)
+    load_both(sim, ctx, smooth_state(*n, hi, seed=9, vel=100.0, pres=1.0e5))
+    for _ in range(4):
+        sim.step(1)
+        ctx.step(1)
+        assert rel_linf(ctx.download(), sim.interior()) < TOL64
+    check_clock(sim, ctx)
+    so, sg = sim.sums(), ctx.sums()
+    assert np.all(np.abs(sg - so) <= 1e-12 * np.abs(so).max()), (so, sg)
+
+
+def test_stretched_poiseuille_on_the_gpu():
+    """The oracle's stretched-channel pin on the device: 32 tanh-clustered cells in y (beta 1.2),
+    run to 8 diffusion times as crates/cfd_solver/tests/poiseuille_channel_validation.rs does.
+    The device profile equals the oracle's and meets the parabola below the 0.5 % gate."""
+    p0 = 1.0 * R * 300.0
+    e = tanh_edges(32, 0.0, 1.0, 1.2)
+    bcs = ["outflow", "outflow", "noslip", "noslip", "slip", "slip"]
+    sim, ctx = make_pair((3, 32, 1), (1.0, 1.0, 1.0), bcs, flux="hllc", cfl=0.3, viscous=True, mu=10.0, body_gx=160.0)
+    ctx.set_body_force(160.0)
+    for obj in (sim, ctx):
+        obj.set_edges(1, e)
+    fs = orc.prim_to_cons([1.0, 0, 0, 0, p0, 300.0], 1.4)
+    load_both(sim, ctx, np.broadcast_to(fs, (1, 32, 3, 5)).copy())
+    steps = sim.advance_to(0.081)
+    ctx.step(steps)
+    check_clock(sim, ctx)
+    ug, uo = ctx.download()[0, :, 1, :], sim.interior()[0, :, 1, :]
+    assert np.max(np.abs(ug - uo)) <= 1e-9 * np.abs(uo).max()
+    ys, w = sim.centroids(1), np.diff(e)
+    pu = 160.0 / 20.0 * ys * (1 - ys)
+    err = math.sqrt(np.sum(w * (ug[:, 1] / ug[:, 0] - pu) ** 2) / np.sum(w * pu ** 2))
+    print("stretched Poiseuille on the device:", steps, "steps, relative L2", err)
+    assert err < 5e-3
+
+
+def test_stretched_edges_errors_3d():
+    c = efvm.Context3(4, 6, 5, hi=(1.0, 1.0, 1.0))
+    for axis, e, status in ((0, np.linspace(0, 1, 5), ERR_UNSUPPORTED), (1, np.linspace(0, 1, 6), ERR_ARG),
+                            (1, [0, 0.3, 0.2, 0.5, 0.7, 0.9, 1.0], ERR_ARG), (2, [0, 0.1, 0.2, 0.3, 0.4, 0.9], ERR_ARG)):
+        with pytest.raises(EfvmError) as err:
+            c.set_edges(axis, e)
+        assert err.value.status == status, (axis, e)
+    c.set_edges(2, [0.0, 0.1, 0.3, 0.6, 0.8, 1.0])
+    fs = orc.prim_to_cons([1.0, 10.0, 0, 0, 1e5, 1e5 / R], 1.4)
+    with pytest.raises(EfvmError) as err:
+        c.add_sphere((0.5, 0.5, 0.5), 0.2, fs)
+    assert err.value.status == ERR_UNSUPPORTED
+
+
 # ------------------------------------------------------------------ immersed bodies
 def sphere_pair(n=16, precision="fp64", sponge=True, markers=500):
     rho, p, u = 1.0, 1.0e5, 56.1
@@ -401,6 +472,47 @@ def test_app_from_outside_and_fn_boundary_3d():
     assert rel_linf(got, sim.interior()) <= TOL64
     assert np.abs(got[-1, :, :, 3]).max() > 0.0       # the ZHi inlet drove rho_w in the top layer
     assert not np.array_equal(got[:, 0, 0], got[:, 1, 0])   # the XLo table differs by row
+
+
+def test_app_stretched_mesh_and_fn_boundary():
+    """UniformMeshConfig { y_edges, z_edges } through the plugin mirror: the App builds the
+    stretched 3-D context, evaluates an FnBoundary on ZHi at the stretched face centroids and
+    the initial condition at the stretched cell centroids, and matches the oracle."""
+    from dev_field_efvm_amd import app as A
+    nx, ny, nz = 5, 7, 6
+    p0 = 1e5
+    air = lambda rho, u, w=0.0: orc.prim_to_cons([rho, u, 0.0, w, p0, p0 / (rho * R)], 1.4)
+    inlet = lambda c: air(1.0, 30.0 * c[0], -200.0 * c[1] * (1.0 - c[1]))
+    ey, ez = tanh_edges(ny, 0.0, 1.0, 1.3), tanh_edges(nz, 0.0, 0.5, 0.9)
+    init = lambda x, y, z: np.stack([1.0 + 0.1 * y + 0.05 * z, 10.0 * y * (1 - y), 0 * x, 0 * x,
+                                     p0 / 0.4 + 0.5 * (10.0 * y * (1 - y)) ** 2 / (1.0 + 0.1 * y + 0.05 * z)], axis=-1)
+    app = A.App()
+    app.add_plugins(A.FieldDefaultPlugins(A.UniformMeshConfig(nx=nx, ny=ny, nz=nz, ng=2, bounds_lo=(0, 0, 0),
+                                                              bounds_hi=(1.0, 1.0, 0.5), y_edges=ey, z_edges=ez)))
+    app.add_plugins(A.CfdStatePlugin(init))
+    reg = (A.BoundaryRegistry().with_axis(0, A.Periodic()).with_axis(1, A.NoSlipWall())
+           .with_(A.BoundarySide.ZLo, A.SlipWall()).with_(A.BoundarySide.ZHi, A.FnBoundary(inlet)))
+    app.add_plugins(A.StructuredBoundaryPlugin(reg))
+    app.add_plugins(A.FluxPlugin.hllc())
+    app.add_plugins(A.IntegratorPlugin.rk3())
+    app.add_resource(A.Viscosity.constant(0.05))
+    app.add_plugins(A.SolverPlugin(A.SolverConfig(cfl=0.3, viscous=True)))
+    app.prepare()
+    for _ in range(6):
+        app.run()
+    sim = orc.Sim(nx, ny, nz, hi=(1.0, 1.0, 0.5), flux="hllc", viscous=True, mu=0.05, cfl=0.3,
+                  bcs=["periodic", "periodic", "noslip", "noslip", "slip", "table"])
+    sim.set_edges(1, ey)
+    sim.set_edges(2, ez)
+    yc, zc = sim.centroids(1), sim.centroids(2)
+    xc = (np.arange(nx) + 0.5) / nx
+    z3, y3, x3 = np.meshgrid(zc, yc, xc, indexing="ij")
+    sim.set_interior(init(x3, y3, z3))
+    fc = [(xc[i], yc[j], 0.5) for j in range(ny) for i in range(nx)]             # i + nx j
+    sim.set_ghost_table(5, [inlet(c) for c in fc], [True] * len(fc))
+    for _ in range(6):
+        sim.run()
+    assert rel_linf(app.state(), sim.interior()) <= TOL64
 
 
 # ------------------------------------------------------------------ published figure

@@ -4,6 +4,7 @@ writes_well_formed_vtk does (:168-200), and its values against the oracle's EOS.
 import re
 
 import numpy as np
+import pytest
 
 import dev_field_efvm_amd as efvm
 from oracle import orc
@@ -106,3 +107,24 @@ def test_vtk_3d_grid_and_cell_order(tmp_path):
                 assert np.isclose(b["density"][q, 0], 1.0 + 0.1 * i + 0.01 * j + 0.001 * k, rtol=1e-10)
                 assert np.allclose(b["velocity"][q], [5.0 * i, -3.0 * j, 2.0 * k], rtol=1e-9, atol=1e-12)
                 q += 1
+
+
+def test_vtk_3d_stretched_axes(tmp_path):
+    """efvm3_write_vtk_edges: a stretched axis's coordinates are each cell's centroid minus half
+    its spacing, then the last centroid plus half its spacing (crates/cfd_output/src/lib.rs:50-72),
+    which is the edge list up to rounding; bad edges are refused."""
+    nx, ny, nz = 2, 3, 4
+    st = np.broadcast_to(orc.prim_to_cons([1.0, 1.0, 0, 0, 1e5, 300.0], 1.4), (nz, ny, nx, 5)).copy()
+    ey, ez = [-1.0, -0.7, 0.2, 1.0], [2.0, 2.1, 2.5, 4.0, 6.0]
+    path = str(tmp_path / "s3.vtk")
+    efvm.write_vtk3(path, st, nx, ny, nz, lo=(0.0, -1.0, 2.0), hi=(3.0, 1.0, 6.0), y_edges=ey, z_edges=ez)
+    text, _ = parse_vtk(path)
+    ys = [float(v) for v in re.search(r"Y_COORDINATES 4 double\n((?:.*\n){4})", text).group(1).split()]
+    zs = [float(v) for v in re.search(r"Z_COORDINATES 5 double\n((?:.*\n){5})", text).group(1).split()]
+    xs = [float(v) for v in re.search(r"X_COORDINATES 3 double\n((?:.*\n){3})", text).group(1).split()]
+    assert np.allclose(ys, ey, rtol=0, atol=1e-15) and np.allclose(zs, ez, rtol=0, atol=1e-15)
+    assert np.allclose(xs, [0.0, 1.5, 3.0], rtol=0, atol=1e-15)
+    with pytest.raises(efvm.EfvmError):
+        efvm.write_vtk3(path, st, nx, ny, nz, lo=(0.0, -1.0, 2.0), hi=(3.0, 1.0, 6.0), y_edges=[-1.0, 0.5, 0.2, 1.0])
+    with pytest.raises(efvm.EfvmError):
+        efvm.write_vtk3(path, st, nx, ny, nz, lo=(0.0, -1.0, 2.0), hi=(3.0, 1.0, 6.0), z_edges=[2.0, 2.1, 2.5, 4.0, 5.0])

@@ -143,6 +143,88 @@ def test_poiseuille_profile_matches_parabola():
     assert rel(15.0) > 10 * 5e-3
 
 
+AIR_G = 1.4
+
+
+def stretched_edges(n, height, beta):
+    """tanh clustering of n cells towards both ends of [0, height]."""
+    x = np.tanh(beta * (2.0 * np.arange(n + 1) / n - 1.0)) / np.tanh(beta)
+    e = 0.5 * height * (1.0 + x)
+    e[0], e[-1] = 0.0, height
+    return e
+
+
+def test_stretched_y_second_difference_is_exact_on_a_quadratic():
+    """crates/cfd_solver/src/lib.rs:483-501 builds the y weights from cell-centroid distances, so
+    the three-point second difference of a quadratic is exact on any spacing: with u = a + b y +
+    c y^2 (rho = 1, p uniform, v = w = 0) the rho_u right-hand side away from the walls is mu 2c
+    (HLLC carries no x momentum across a pure shear).  Weights from the mean spacing miss it by
+    O(1) on this grid."""
+    n, height, mu, c2 = 20, 1.0, 0.5, -7.0
+    e = stretched_edges(n, height, 1.6)
+    s = orc.Sim(3, n, 1, hi=(0.3, height, 0.1), flux="hllc", viscous=True, mu=mu, integrator="euler", fixed_dt=1e-9,
+                bcs=["periodic", "periodic", "noslip", "noslip", "slip", "slip"])
+    s.set_edges(1, e)
+    yc = s.centroids(1)
+    assert np.allclose(yc, 0.5 * (e[:-1] + e[1:]), rtol=0, atol=0)
+    u = 3.0 + 2.0 * yc + c2 * yc ** 2
+    p0 = 1.0e5
+    st = np.zeros((1, n, 3, 5))
+    for j in range(n):
+        st[0, j, :, :] = orc.prim_to_cons([1.0, u[j], 0.0, 0.0, p0, p0 / R], AIR_G)
+    s.set_interior(st)
+    s.run()
+    got = s.rhs()[0, 1:-1, 1, 1]
+    assert np.max(np.abs(got / (2.0 * mu * c2) - 1.0)) < 1e-9
+    h = height / n
+    uniform = (u[2:] - 2.0 * u[1:-1] + u[:-2]) / h ** 2
+    assert np.max(np.abs(uniform / (2.0 * c2) - 1.0)) > 0.5
+
+
+def test_stretched_poiseuille_matches_the_parabola():
+    """crates/cfd_solver/tests/poiseuille_channel_validation.rs on a y grid clustered towards both
+    walls (tanh, beta 1.2): the steady profile matches the parabola in the width-weighted relative
+    L2 norm below the reference's 0.5 % gate, at second order from 16 to 32 cells, and a
+    parabola with 1.5 mu stays far outside it."""
+    p0 = 1.0 * R * 300.0
+    errs = []
+    for ny in (16, 32):
+        s = orc.Sim(3, ny, 1, hi=(1.0, 1.0, 1.0), flux="hllc", cfl=0.3, viscous=True, mu=10.0, body_gx=160.0,
+                    bcs=["outflow", "outflow", "noslip", "noslip", "slip", "slip"])
+        e = stretched_edges(ny, 1.0, 1.2)
+        s.set_edges(1, e)
+        s.ic_uniform(orc.prim_to_cons([1.0, 0, 0, 0, p0, 300.0], 1.4))
+        s.advance_to(0.081)
+        u = s.interior()[0, :, 1, :]
+        ys, w = s.centroids(1), np.diff(e)
+
+        def rel(mu):
+            pu = 160.0 / (2 * mu) * ys * (1 - ys)
+            return math.sqrt(np.sum(w * (u[:, 1] / u[:, 0] - pu) ** 2) / np.sum(w * pu ** 2))
+
+        errs.append(rel(10.0))
+        assert rel(15.0) > 10 * 5e-3
+    print("stretched Poiseuille errors (16, 32 cells):", errs)
+    assert errs[1] < 5e-3 and errs[0] / errs[1] > 3.5
+
+
+def test_stretched_edges_are_validated():
+    s = orc.Sim(4, 5, 2, hi=(1.0, 1.0, 1.0), bcs=["periodic"] * 2 + ["slip"] * 4)
+    with pytest.raises(ValueError):
+        s.set_edges(0, np.linspace(0, 1, 5))                     # x stays uniform
+    with pytest.raises(ValueError):
+        s.set_edges(1, np.linspace(0, 1, 5))                     # 5 cells need 6 edges
+    with pytest.raises(ValueError):
+        s.set_edges(1, [0.0, 0.3, 0.2, 0.5, 0.8, 1.0])           # not increasing
+    with pytest.raises(ValueError):
+        s.set_edges(2, [0.0, 0.5, 0.9])                          # does not end at hi
+    s.set_edges(2, [0.0, 0.2, 1.0])
+    assert np.allclose(s.centroids(2), [0.1, 0.6])
+    b = orc.Sim(6, 6, 1, hi=(1.0, 1.0, 0.1), cylinders=[(0.5, 0.5, 0.2)], freestream=[1, 0, 0, 0, 2.5])
+    with pytest.raises(ValueError):
+        b.set_edges(1, np.linspace(0, 1, 7))                     # bodies need a uniform mesh
+
+
 # ---------------------------------------------------------------- unit known-answer tests
 AIR = 1.4
 
