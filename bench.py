@@ -11,8 +11,9 @@ Prints ONE JSON line (rank 0):
   value     = N_cells * 3 * steps / wall time of the timed steps (state resident in HBM)
   roofline  = algorithmic HBM bytes per stage launch / mean stage-kernel time (HIP events
               on the solver's stream) vs the MI355X HBM peak (8 TB/s)
-  cpu_baseline = the CPU FP64 oracle (a single-threaded port of the reference path) on a
-              bounded sample of the same workload, run on this host.
+  cpu_baseline = the CPU FP64 oracle (the reference path restated in C) on a bounded sample
+              of the same workload on this host's cores (its threaded step, bitwise the serial
+              one), with the single-thread figure beside it.
 """
 from __future__ import annotations
 
@@ -534,7 +535,7 @@ def main():
     achieved = bytes_per_launch / mean_launch_s / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": load_traffic(n, args.precision),
-            "kernel": f"stage_{args.precision}_kernel", "mean_launch_us": round(mean_launch_s * 1e6, 2),
+            "kernel": ctx.launch()[2], "mean_launch_us": round(mean_launch_s * 1e6, 2),
             "algorithmic_bytes_per_launch": bytes_per_launch, "measured_copy_GBps": copy_gbs}
     out = {
         "metric": METRIC,

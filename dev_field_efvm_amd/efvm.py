@@ -208,7 +208,8 @@ class Context:
     def set_launch(self, strip_rows=0, waves_per_block=0):
         self._chk(self._L.efvm_set_launch(self._p, strip_rows, waves_per_block))
 
-    KERNELS = ("one-column strips", "two-column periodic strips", "fp64 strips", "learned face kernels")
+    KERNELS = ("stage_f32_kernel (one-column strips)", "stage_f32_pair_kernel (two-column periodic strips)",
+               "stage_f64_kernel", "learned face kernels")
 
     def launch(self):
         """(strip_rows, waves_per_block, kernel name, whether the two-column kernel applies)."""
