@@ -1028,23 +1028,69 @@ StageArgs batch_case_args(const efvm_batch* b, int i) {
     return a;
 }
 
+// The work table, XCD-aware.  Workgroup k of a launch runs on XCD k % 8 and each XCD has its own
+// L2, so a strip shares the rows it reads beyond its own (two above, two below) with its
+// vertical neighbours only when they run on the same XCD.  Each case's strips, in row-major
+// order (a band of whole strip rows at a time), are dealt to per-XCD queues: a case of fewer
+// strips than one queue's share goes whole to the least-loaded queue, a larger one is cut into
+// contiguous bands, one per queue.  Cases go largest first, so each queue ends on small work.
+// Position p of the table belongs to workgroup p / waves, hence to XCD (p / waves) % 8, and is
+// filled from that XCD's queue (or, once it is empty, from the longest remaining one).
+std::vector<int2> batch_work_table(const efvm_batch* b) {
+    constexpr int NXCD = 8;
+    std::vector<int> order(b->n);
+    for (int i = 0; i < b->n; i++) order[i] = i;
+    auto cells = [&](int i) { return (long)b->cases[i].nx * b->cases[i].ny; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cells(x) > cells(y); });
+    std::vector<std::vector<int2>> q(NXCD);
+    std::vector<long> load(NXCD, 0);
+    long total = 0;
+    for (int i = 0; i < b->n; i++) total += cells(i);
+    const long share = std::max<long>(1, total / NXCD);
+    for (int i : order) {
+        const StageArgs& a = b->hc[i].a;
+        const int ns = a.n_strips_x * a.n_strips_y;
+        const long per = std::max<long>(1, cells(i) / ns);
+        if (cells(i) <= share / 4 || ns < NXCD) {
+            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            for (int s = 0; s < ns; s++) q[k].push_back(make_int2(i, s));
+            load[k] += cells(i);
+        } else {
+            for (int k = 0; k < NXCD; k++) {
+                const int s0 = (int)((long)ns * k / NXCD), s1 = (int)((long)ns * (k + 1) / NXCD);
+                for (int s = s0; s < s1; s++) q[k].push_back(make_int2(i, s));
+                load[k] += per * (s1 - s0);
+            }
+        }
+    }
+    size_t n_items = 0;
+    for (auto& v : q) n_items += v.size();
+    std::vector<size_t> head(NXCD, 0);
+    std::vector<int2> work;
+    work.reserve(n_items);
+    const int W = std::max(1, b->waves);
+    while (work.size() < n_items) {
+        const int k = (int)((work.size() / W) % NXCD);
+        int src = k;
+        if (head[src] == q[src].size()) {
+            size_t best = 0;
+            for (int t = 0; t < NXCD; t++)
+                if (q[t].size() - head[t] > best) { best = q[t].size() - head[t]; src = t; }
+        }
+        work.push_back(q[src][head[src]++]);
+    }
+    return work;
+}
+
 efvm_status batch_rebuild(efvm_batch* b) {
     batch_drop_graph(b);
-    std::vector<std::pair<long, int2>> items;
     for (int i = 0; i < b->n; i++) {
         BatchCase& bc = b->hc[i];
         bc.a = batch_case_args(b, i);
-        long nc = (long)b->cases[i].nx * b->cases[i].ny;
         for (int s = 0; s < 3; s++)
             for (int m = 0; m < 4; m++) bc.P[s][m] = b->pool + (long)(4 * s + m) * b->total_cells + b->offset[i];
-        int ns = bc.a.n_strips_x * bc.a.n_strips_y;
-        long strip_cells = (long)std::min(b->cases[i].ny, b->strip_rows[i]) * std::min(b->cases[i].nx, (int)STRIP_OUT);
-        for (int s = 0; s < ns; s++) items.push_back({strip_cells * 1000000L + nc, make_int2(i, s)});
     }
-    // largest strips of the largest cases first (LPT order over the persistent grid)
-    std::stable_sort(items.begin(), items.end(), [](const std::pair<long, int2>& x, const std::pair<long, int2>& y) { return x.first > y.first; });
-    std::vector<int2> work(items.size());
-    for (size_t q = 0; q < items.size(); q++) work[q] = items[q].second;
+    const std::vector<int2> work = batch_work_table(b);
     if (b->dwork) (void)hipFree(b->dwork);
     b->dwork = nullptr;
     BHIPCHK(b, hipMalloc((void**)&b->dwork, sizeof(int2) * std::max<size_t>(1, work.size())));
@@ -1169,8 +1215,7 @@ efvm_status efvm_batch_set_rows(efvm_batch* b, const int* rows) {
 efvm_status efvm_batch_set_grid(efvm_batch* b, int blocks, int waves_per_block) {
     if (!b || blocks < 0 || waves_per_block < 1 || waves_per_block > 4) return bfail(b, EFVM_ERR_ARG, "bad grid");
     b->blocks = blocks; b->waves = waves_per_block;
-    batch_drop_graph(b);
-    return EFVM_OK;
+    return batch_rebuild(b);   // the work table's XCD order depends on the waves per workgroup
 }
 
 efvm_status efvm_batch_upload(efvm_batch* b, int i, const double* cons5, size_t n) {
