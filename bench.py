@@ -550,7 +550,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic (isentropic vortex + 8 seeded Fourier modes, host FP64)",
-        "config": {"workload": f"2-D Euler MUSCL(MC)+HLLC+SSP-RK3, periodic {n}x{n}, CFL 0.4, fused FP64 CFL",
+        "config": {"workload": f"2-D Euler MUSCL(MC)+HLLC+SSP-RK3, periodic {n}x{n}, CFL 0.4, fused CFL "
+                               f"(per-cell lambda in {'FP32' if args.precision == 'fp32' else 'FP64'}, maximum and dt in FP64)",
                    "grid": [n, n], "graph_steps": g if use_graph else 0, "sim_time": clk["time"], "launch": tuned,
                    "parallelism": "single GPU",
                    # the same K steps with one upload of the host state before and one download

@@ -9,6 +9,9 @@ Sources (read as text):
   crates/cfd_solver/tests/isentropic_vortex_muscl_convergence.rs:176-191  MUSCL order gate
   crates/cfd_solver/tests/poiseuille_channel_validation.rs:88-91          Poiseuille L2
   examples/cylinder_drag_validation/config.toml:24-26                    Cd grid series
+  examples/make_validation_plots.py:109-117, examples/sphere_drag_validation/README.md:5,
+  examples/sphere_drag_validation/config.toml:21-27 (D/dx = 12 row), the example's own
+  setup (main.rs, config.toml)                                             sphere drag
 """
 import json
 import os
@@ -26,6 +29,15 @@ DATA = {
     "muscl_mc_gate": {"N": [40, 80, 160], "min_fitted_order": 1.8, "max_err_finest": 3.0e-4},
     "poiseuille": {"rel_l2_measured": 7.6e-4, "gate": 5.0e-3, "wrong_mu_control": 0.37},
     "cylinder_cd": {"D_over_dx": [10, 15, 20], "cd": [5.126, 5.466, 5.6312], "cd_ref": 5.5795},
+    "sphere_drag": {"cd": 2.5402, "cd_schiller_naumann": 2.6095,
+                    "source": "examples/make_validation_plots.py:109-117; examples/sphere_drag_validation/README.md:5",
+                    "ptp_percent": 2.7,
+                    "ptp_source": "examples/sphere_drag_validation/config.toml:21-27 (D/dx = 12 row)",
+                    "setup": {"grid": [144, 72, 72], "lo": [-0.3, -0.3, -0.3], "hi": [0.9, 0.3, 0.3], "D": 0.1,
+                              "rho": 1.0, "p": 100000.0, "u_inf": 56.1, "reynolds": 20.0, "markers": 2000, "cfl": 0.3,
+                              "muscl": False, "sponge": {"x_start": 0.5, "sigma": 20000.0}, "runs": 9000,
+                              "average_frac": 0.2, "bcs": "subsonic inflow x-lo, pressure outlet x-hi, symmetry y/z",
+                              "scheme": "hllc + viscous, rk3"}},
 }
 
 if __name__ == "__main__":

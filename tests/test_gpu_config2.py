@@ -85,7 +85,8 @@ def test_config2_fp64_200_steps_matches_the_oracle_fixture(fixture):
 
 
 def test_config2_fp32_error_against_the_fp64_fixture(fixture):
-    """FP32 flux and update arithmetic (FP64 CFL reduction): its error after the same 200 steps."""
+    """FP32 flux and update arithmetic (per-cell CFL lambda in FP32, its maximum in FP64): its error after
+    the same 200 steps."""
     st, _, _, clk = run("fp32", 200)
     e_sub, e_row, e_col, e_ext = errors(st, fixture)
     rel_t = abs(clk["time"] - fixture["time"][-1]) / fixture["time"][-1]

@@ -17,6 +17,17 @@ import exact_riemann as er
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PUB = json.load(open(os.path.join(HERE, "golden", "published.json")))
+
+
+def test_published_figures_are_the_transcription_script_output():
+    """tests/golden/make_published.py regenerates published.json exactly (every entry, the
+    sphere drag included), so the committed golden figures have one written source."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_published", os.path.join(HERE, "golden", "make_published.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    assert json.loads(json.dumps(mod.DATA, indent=1)) == PUB
+    assert json.dumps(mod.DATA, indent=1) == open(os.path.join(HERE, "golden", "published.json")).read()
 R = orc.R_AIR
 
 
