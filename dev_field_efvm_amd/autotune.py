@@ -63,9 +63,9 @@ def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=
     kernel for a context; returns (rows, waves, ms_per_step).  The two-column kernel halves the
     lane exchanges per cell but covers 124 columns per wave: at sizes just above a multiple of
     124 (512, 2048) its last strip is nearly empty, and the one-column kernel can win."""
-    fits = ctx.launch()[3]
+    fits = ctx.launch()[3]          # 0: one-column only; 1: the periodic two-column form; 2: the general one
     key = f"{arch_name()}:{ctx.precision}:ctx:{size_class(ctx.nx)}x{size_class(ctx.ny)}:" + \
-          ("periodic-plain" if fits else "general")
+          ("general", "periodic-plain", "walls-plain")[fits]
     db = _load()
     if use_cache and key in db:
         e = db[key]
@@ -110,16 +110,27 @@ def tune_batch(batch, sides, steps: int = 2, rows=BATCH_ROWS, blocks=BATCH_BLOCK
     db = _load()
     if use_cache and key in db:
         choice = {int(k): v for k, v in db[key]["rows"].items()}
+        batch.set_pair_kernel(db[key].get("pair", 0))
         _apply_batch(batch, sides, choice)
         batch.set_grid(db[key]["blocks"], 4)
         return choice, db[key]["blocks"], db[key]["ms"]
     batch.snapshot()
+    # the two-column kernel for the cases it takes, or the one-column kernel for all: both at
+    # the default heights, the faster kept (then heights and grid are tuned for it)
     choice = {c: 31 for c in classes}
     _apply_batch(batch, sides, choice)
     nb = 0
     batch.set_grid(nb, 4)
-    best_ms = _time_batch(batch, steps)
-    batch.snapshot(restore=True)
+    best_ms, pair = None, 0
+    for p in (1, 0):
+        batch.set_pair_kernel(p)
+        ms = _time_batch(batch, steps)
+        batch.snapshot(restore=True)
+        if verbose:
+            print(f"  pair={p}  {ms:.3f} ms/step", flush=True)
+        if best_ms is None or ms < best_ms:
+            best_ms, pair = ms, p
+    batch.set_pair_kernel(pair)
     for c in sorted(classes, reverse=True):          # largest classes dominate the time
         for r in rows:
             if r == choice[c]:
@@ -146,7 +157,8 @@ def tune_batch(batch, sides, steps: int = 2, rows=BATCH_ROWS, blocks=BATCH_BLOCK
             best_ms, nb = ms, g
     batch.set_grid(nb, 4)
     if store:
-        db[key] = {"rows": {str(k): v for k, v in choice.items()}, "blocks": nb, "ms": best_ms, "when": time.time()}
+        db[key] = {"rows": {str(k): v for k, v in choice.items()}, "blocks": nb, "ms": best_ms, "pair": pair,
+                   "when": time.time()}
         _store(db)
     return choice, nb, best_ms
 

@@ -142,9 +142,10 @@ efvm_status fail(efvm_ctx* c, efvm_status s, const std::string& msg) {
 bool periodic_axis(const efvm_ctx* c, int axis) { return c->bc[2 * axis].kind == EFVM_BC_PERIODIC; }
 
 // FP32 periodic grids without boundary features can run the two-column strip kernel.
-bool pair_fits(const efvm_ctx* c, const StageArgs& a) {
+int pair_fits(const efvm_ctx* c, const StageArgs& a) {
     const bool plain = !a.viscous && !a.solid && !a.parabolic && !a.body_force;
-    return c->esz == 4 && !c->learned && pair_path_fits(a.nx, a.ny, a.per_x, a.per_y, plain, a.plane_stride);
+    if (c->esz != 4 || c->learned) return 0;
+    return pair_path_kind(a.nx, a.ny, a.per_x, a.per_y, plain, a.bc_kind, a.plane_stride);
 }
 
 StageArgs base_args(const efvm_ctx* c) {
@@ -192,7 +193,7 @@ StageArgs base_args(const efvm_ctx* c) {
     a.lam_part = c->lam_part;
     a.face_count = nullptr;
     a.plane_stride = c->plane_stride <= 0xFFFFFFFFu ? (unsigned)c->plane_stride : 0u;
-    a.pair_path = c->allow_pair && pair_fits(c, a);
+    a.pair_path = c->allow_pair ? pair_fits(c, a) : 0;
     if (a.pair_path) a.n_strips_x = (g.nx + STRIP_OUT2 - 1) / STRIP_OUT2;
     return a;
 }
@@ -448,7 +449,7 @@ efvm_status efvm_get_launch(const efvm_ctx* c, int* strip_rows, int* waves_per_b
     if (strip_rows) *strip_rows = c->strip_h;
     if (waves_per_block) *waves_per_block = c->waves_per_block;
     if (kernel) *kernel = c->learned ? 3 : (c->esz == 8 ? 2 : (base_args(c).pair_path ? 1 : 0));
-    if (pair_fits_out) *pair_fits_out = pair_fits(c, base_args(c)) ? 1 : 0;
+    if (pair_fits_out) *pair_fits_out = pair_fits(c, base_args(c));
     return EFVM_OK;
 }
 
@@ -966,6 +967,8 @@ efvm_status efvm_device_state(efvm_ctx* c, void* planes[4]) {
 namespace efvm {
 hipError_t launch_stage_f32_batch(const BatchCase* cases, const StageArgs& rep, const int2* work, int n_work, int stage,
                                   int parity, int blocks, int waves, hipStream_t st);
+hipError_t launch_stage_f32_batch_pair(const BatchCase* cases, const StageArgs& rep, const int2* work, int n_work, int stage,
+                                       int parity, int blocks, int waves, hipStream_t st);
 }
 
 struct efvm_batch {
@@ -975,8 +978,11 @@ struct efvm_batch {
     std::vector<BatchCase> hc;
     std::vector<int> strip_rows;
     BatchCase* dc = nullptr;
-    int2* dwork = nullptr;
+    int2* dwork = nullptr;         // (case, strip) work items of the one-column kernel
     int n_work = 0;
+    int2* dwork_pair = nullptr;    // and of the two-column kernel
+    int n_work_pair = 0;
+    int allow_pair = 1;            // cases the two-column kernel takes run on it
     DevClock* clocks = nullptr;
     float* pool = nullptr;
     std::vector<long> offset;      // per-case start (cells) in the pool
@@ -1023,6 +1029,10 @@ StageArgs batch_case_args(const efvm_batch* b, int i) {
     tmp.clock = b->clocks + i;
     tmp.strip_h = b->strip_rows[i];
     tmp.xcd_remap = 0;
+    tmp.esz = 4;
+    // the pool holds plane m of set s of every case at (4 s + m) * total_cells: one plane stride
+    tmp.plane_stride = (size_t)b->total_cells * 4 <= 0xFFFFFFFFu ? (size_t)b->total_cells * 4 : 0;
+    tmp.allow_pair = b->allow_pair;
     StageArgs a = base_args(&tmp);
     a.fuse_cfl = b->sch.fixed_dt <= 0.0 ? 1 : 0;
     return a;
@@ -1036,16 +1046,17 @@ StageArgs batch_case_args(const efvm_batch* b, int i) {
 // contiguous bands, one per queue.  Cases go largest first, so each queue ends on small work.
 // Position p of the table belongs to workgroup p / waves, hence to XCD (p / waves) % 8, and is
 // filled from that XCD's queue (or, once it is empty, from the longest remaining one).
-std::vector<int2> batch_work_table(const efvm_batch* b) {
+std::vector<int2> batch_work_table(const efvm_batch* b, bool pair) {
     constexpr int NXCD = 8;
-    std::vector<int> order(b->n);
-    for (int i = 0; i < b->n; i++) order[i] = i;
+    std::vector<int> order;
+    for (int i = 0; i < b->n; i++)
+        if ((b->hc[i].a.pair_path != 0) == pair) order.push_back(i);
     auto cells = [&](int i) { return (long)b->cases[i].nx * b->cases[i].ny; };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cells(x) > cells(y); });
     std::vector<std::vector<int2>> q(NXCD);
     std::vector<long> load(NXCD, 0);
     long total = 0;
-    for (int i = 0; i < b->n; i++) total += cells(i);
+    for (int i : order) total += cells(i);
     const long share = std::max<long>(1, total / NXCD);
     for (int i : order) {
         const StageArgs& a = b->hc[i].a;
@@ -1090,14 +1101,22 @@ efvm_status batch_rebuild(efvm_batch* b) {
         for (int s = 0; s < 3; s++)
             for (int m = 0; m < 4; m++) bc.P[s][m] = b->pool + (long)(4 * s + m) * b->total_cells + b->offset[i];
     }
-    const std::vector<int2> work = batch_work_table(b);
-    if (b->dwork) (void)hipFree(b->dwork);
-    b->dwork = nullptr;
+    const std::vector<int2> work = batch_work_table(b, false), work_pair = batch_work_table(b, true);
+    for (int2** d : {&b->dwork, &b->dwork_pair}) {
+        if (*d) (void)hipFree(*d);
+        *d = nullptr;
+    }
     BHIPCHK(b, hipMalloc((void**)&b->dwork, sizeof(int2) * std::max<size_t>(1, work.size())));
-    BHIPCHK(b, hipMemcpyAsync(b->dwork, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, hipMalloc((void**)&b->dwork_pair, sizeof(int2) * std::max<size_t>(1, work_pair.size())));
+    if (!work.empty())
+        BHIPCHK(b, hipMemcpyAsync(b->dwork, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, b->stream));
+    if (!work_pair.empty())
+        BHIPCHK(b, hipMemcpyAsync(b->dwork_pair, work_pair.data(), sizeof(int2) * work_pair.size(), hipMemcpyHostToDevice,
+                                  b->stream));
     BHIPCHK(b, hipMemcpyAsync(b->dc, b->hc.data(), sizeof(BatchCase) * b->n, hipMemcpyHostToDevice, b->stream));
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     b->n_work = (int)work.size();
+    b->n_work_pair = (int)work_pair.size();
     return EFVM_OK;
 }
 
@@ -1120,12 +1139,21 @@ efvm_status batch_enqueue_steps(efvm_batch* b, int n_steps) {
         }
     // 0 = one wave per (case, strip) work item: the hardware dispatcher then balances the
     // uneven strips; a persistent grid striding through the table measured up to 13% slower
-    int blocks = b->blocks > 0 ? b->blocks : std::max(1, (b->n_work + b->waves - 1) / b->waves);
+    const int blocks = b->blocks > 0 ? b->blocks : std::max(1, (b->n_work + b->waves - 1) / b->waves);
+    const int blocks_pair = b->blocks > 0 ? b->blocks : std::max(1, (b->n_work_pair + b->waves - 1) / b->waves);
     for (int k = 0; k < n_steps; k++) {
         for (int st = 0; st < 3; st++) {
-            hipError_t e = launch_stage_f32_batch(b->dc, rep, b->dwork, b->n_work, st, b->parity, blocks,
-                                                  b->waves, b->stream);
-            if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("batch stage: ") + hipGetErrorString(e));
+            // the two kernels take disjoint cases, so their order within a stage does not matter
+            if (b->n_work_pair > 0) {
+                hipError_t e = launch_stage_f32_batch_pair(b->dc, rep, b->dwork_pair, b->n_work_pair, st, b->parity,
+                                                           blocks_pair, b->waves, b->stream);
+                if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("batch stage: ") + hipGetErrorString(e));
+            }
+            if (b->n_work > 0) {
+                hipError_t e = launch_stage_f32_batch(b->dc, rep, b->dwork, b->n_work, st, b->parity, blocks,
+                                                      b->waves, b->stream);
+                if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("batch stage: ") + hipGetErrorString(e));
+            }
         }
         b->parity ^= 1;
     }
@@ -1188,6 +1216,7 @@ void efvm_batch_destroy(efvm_batch* b) {
     if (b->clocks) (void)hipFree(b->clocks);
     if (b->dc) (void)hipFree(b->dc);
     if (b->dwork) (void)hipFree(b->dwork);
+    if (b->dwork_pair) (void)hipFree(b->dwork_pair);
     if (b->snap) (void)hipFree(b->snap);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1209,6 +1238,12 @@ efvm_status efvm_batch_set_rows(efvm_batch* b, const int* rows) {
     for (int i = 0; i < b->n; i++)
         if (rows[i] < 1 || rows[i] > 4096) return bfail(b, EFVM_ERR_ARG, "bad launch shape");
     b->strip_rows.assign(rows, rows + b->n);
+    return batch_rebuild(b);
+}
+
+efvm_status efvm_batch_set_pair_kernel(efvm_batch* b, int allow) {
+    if (!b || allow < 0 || allow > 1) return EFVM_ERR_ARG;
+    b->allow_pair = allow;
     return batch_rebuild(b);
 }
 

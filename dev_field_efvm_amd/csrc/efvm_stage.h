@@ -75,16 +75,30 @@ struct StageArgs {
     unsigned* face_count;        // learned bypass: deferred-face count, cleared by the stage kernel
     unsigned plane_stride;       // bytes from one plane of a set to the next when the four planes of
                                  // every set lie at a common stride (0: they do not)
-    int pair_path;               // FP32: the two-column periodic strip kernel runs this stage
-                                 // (n_strips_x then counts strips of STRIP_OUT2 columns)
+    int pair_path;               // FP32: a two-column strip kernel runs this stage (1 the periodic
+                                 // form, 2 the general one; n_strips_x then counts strips of
+                                 // STRIP_OUT2 columns)
 };
 
-// The two-column periodic strip kernel applies: both axes periodic, no boundary features, an
-// even nx (column pairs never straddle the seam), at least four cells per axis (one wrap per
-// row or column index), and each plane set addressable by one buffer descriptor of under 2 GiB.
-inline bool pair_path_fits(int nx, int ny, bool per_x, bool per_y, bool no_features, unsigned plane_stride) {
-    return per_x && per_y && no_features && nx % 2 == 0 && nx >= 4 && ny >= 4 && plane_stride != 0 &&
-           (unsigned long long)plane_stride * 4ull < 0x80000000ull;
+// Which two-column strip kernel applies (0: none, 1: the periodic form, 2: the general form).
+// Both need no boundary features (viscous terms, immersed body, parabolic inflow, body force),
+// only boundaries whose ghost is a copy, a reflection or a periodic image (outflow, slip, no-slip,
+// symmetry, periodic: the EOS-based conditions and ghost tables stay with the one-column
+// kernel), and every plane set addressable by one buffer descriptor of under 2 GiB.  The
+// periodic form needs both axes periodic, nx even (column pairs never straddle the seam) and
+// at least four cells per axis (one wrap per row or column index).  The general form takes any
+// mix of those boundaries, with nx even on a periodic x axis, nx >= 2, and ny >= 4 on a
+// periodic y axis.
+inline int pair_path_kind(int nx, int ny, bool per_x, bool per_y, bool no_features, const int* bc_kind,
+                          unsigned plane_stride) {
+    if (!no_features || plane_stride == 0 || (unsigned long long)plane_stride * 4ull >= 0x80000000ull) return 0;
+    for (int s = 0; s < 4; s++)
+        if (bc_kind[s] != BC_OUTFLOW && bc_kind[s] != BC_PERIODIC && bc_kind[s] != BC_SLIP && bc_kind[s] != BC_NOSLIP &&
+            bc_kind[s] != BC_SYMMETRY)
+            return 0;
+    if (per_x && per_y && nx % 2 == 0 && nx >= 4 && ny >= 4) return 1;
+    if ((per_x && nx % 2 != 0) || nx < 2 || (per_y && ny < 4)) return 0;
+    return 2;
 }
 
 // One case of a batched ensemble: its stage arguments and plane sets A, B, C.
@@ -227,11 +241,27 @@ __device__ __forceinline__ GhostOwner ghost_owner(const StageArgs& a, int c, int
     return {r < 0 ? 2 : 3, c, r < 0 ? 0 : ny - 1};
 }
 
+// The value at (column c, row r), in the grid or not: the periodic image, or the ghost of the
+// owner cell across the boundary (the stage kernels' loaders; kept apart from ghost_owner so
+// that the hot paths keep their registers, the combined form spilled in the FP32 kernel).
 template <class T>
 __device__ __forceinline__ Cell<T> load_cell(const StageArgs& a, const T* const* U, int c, int r) {
-    GhostOwner g = ghost_owner(a, c, r);
-    Cell<T> own = load_plain(a, U, g.col, g.row);
-    return g.side < 0 ? own : ghost_of_owner(a, g.side, own, g.row, g.col);
+    const int nx = a.nx, ny = a.ny;
+    if (a.per_x) { c %= nx; if (c < 0) c += nx; }
+    if (a.per_y) { r %= ny; if (r < 0) r += ny; }
+    bool cin = (c >= 0 && c < nx), rin = (r >= 0 && r < ny);
+    if (cin && rin) return load_plain(a, U, c, r);
+    if (!cin) {
+        int side = c < 0 ? 0 : 1;
+        int oc = c < 0 ? 0 : nx - 1;
+        int orow = r < 0 ? 0 : (r >= ny ? ny - 1 : r);
+        Cell<T> own = load_plain(a, U, oc, orow);
+        return ghost_of_owner(a, side, own, orow, oc);
+    }
+    int side = r < 0 ? 2 : 3;
+    int orow = r < 0 ? 0 : ny - 1;
+    Cell<T> own = load_plain(a, U, c, orow);
+    return bc_apply(a, side, own, orow, c);
 }
 
 __device__ __forceinline__ void atomic_max_pos_double(double* addr, double v) {

@@ -388,30 +388,44 @@ __global__ __launch_bounds__(256) void stage_f32_kernel(StageArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Two-column periodic strips: both axes periodic, no boundary features, nx even, and the
-// four planes of every set at one stride, so that each set is one buffer descriptor and a
-// plane is selected by the scalar offset.  A wave owns 128 consecutive columns, two per
-// lane (c0 = x0 + 2 lane - 2 and c0 + 1), and writes the 124 columns of lanes 1..62.  Per
-// row step a lane evaluates the x face between its two columns without any lane exchange,
-// the face west of c0 from the east state of the left lane's second column (one DPP move
-// per value), and the y faces of both columns: the lane exchanges of a cell halve, and so
-// do the halo lanes.  Every cell's arithmetic is stage_f32_strip's, expression for
-// expression (the same slopes, face states, fluxes, divergence, RK combine and CFL), so
-// the results are bitwise those of the one-column kernel.
+// Two-column strips, no boundary features, and the four planes of every set at one stride,
+// so that each set is one buffer descriptor and a plane is selected by the scalar offset.
+// A wave owns 128 consecutive columns, two per lane (c0 = x0 + 2 lane - 2 and c0 + 1), and
+// writes the 124 columns of lanes 1..62.  Per row step a lane evaluates the x face between
+// its two columns without any lane exchange, the face west of c0 from the east state of the
+// left lane's second column (one DPP move per value), and the y faces of both columns: the
+// lane exchanges of a cell halve, and so do the halo lanes.  Every cell's arithmetic is
+// stage_f32_strip's, expression for expression (the same slopes, face states, fluxes,
+// divergence, RK combine and CFL), so the results are bitwise those of the one-column kernel.
+//
+// GEN = false: both axes periodic, nx even; a lane loads its two columns with one 8-byte load
+// per plane and stores them with one.  GEN = true: any mix of periodic, outflow, slip, no-slip
+// and symmetry sides, and odd nx on a non-periodic x axis.  A ghost column takes its owner's
+// value (the periodic image, or the edge column reflected as the side's condition says), a
+// ghost row likewise; the lane loads the two columns at the clamped owner column of its first
+// column and picks each column's value from the pair, and stores each column on its own, so
+// the last pair of an odd row writes one column.
 // ---------------------------------------------------------------------------
 struct Pair { F4 a, b; };   // the lane's two columns
 struct PrepPair { Prep a, b; };
 
 __device__ __forceinline__ Pair sub_pair(const Pair& x, const Pair& y) { return {sub4(x.a, y.a), sub4(x.b, y.b)}; }
 
-template <int FLUX, int LIM>
+// ghost reflection factors (for rho_u, rho_v) of a side's condition on an axis: slip and
+// symmetry flip the normal momentum, no-slip both, outflow and periodic neither
+__device__ __forceinline__ float2 reflect_uv(int kind, int axis) {
+    const bool normal = kind == BC_SLIP || kind == BC_SYMMETRY, both = kind == BC_NOSLIP;
+    return {both || (normal && axis == 0) ? -1.f : 1.f, both || (normal && axis == 1) ? -1.f : 1.f};
+}
+
+template <int FLUX, int LIM, bool GEN>
 __device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const int s, const float* U, const float* U0,
                                                      float* O, const int stage, const int parity) {
     constexpr bool MUSCL = LIM != LIM_NONE;
     const int lane = lane_id();
     const int nx = a.nx, ny = a.ny;
     const int sxi = s % a.n_strips_x, syi = s / a.n_strips_x;
-    const int c0 = sxi * STRIP_OUT2 - 2 + 2 * lane;     // even: the pair never straddles the seam (nx even)
+    const int c0 = sxi * STRIP_OUT2 - 2 + 2 * lane;     // even: the pair never straddles a periodic seam
     const int j0 = syi * a.strip_h;
     const int j1 = min(j0 + a.strip_h, ny);
     const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0);
@@ -425,26 +439,64 @@ __device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const i
     else { ca = 1.f / 3.f; cb = 2.f / 3.f; }
     const float dtix = dt * (float)a.inv_x, dtiy = dt * (float)a.inv_y;
 
-    // columns: the periodic image for loads; columns outside the mesh are seam ghosts, which
-    // seam mode 0 leaves unreconstructed (as stage_f32_strip does)
-    int cq0 = c0 % nx;
-    if (cq0 < 0) cq0 += nx;
-    const unsigned ld_off = (unsigned)cq0 * 4u;
     const bool seam_rec = a.seam_mode == 1;
     const bool ga = c0 < 0 || c0 >= nx, gb = c0 + 1 < 0 || c0 + 1 >= nx;
-    const float hWa = MUSCL && (!ga || seam_rec) ? -1.f : 0.f, hEa = MUSCL && (!ga || seam_rec) ? 1.f : 0.f;
-    const float hWb = MUSCL && (!gb || seam_rec) ? -1.f : 0.f, hEb = MUSCL && (!gb || seam_rec) ? 1.f : 0.f;
     const bool out_lane = lane >= 1 && lane <= WAVE - 2 && c0 < nx;
+    // columns: the load offset (two columns from there) and, for GEN, which of the two each of
+    // the lane's columns takes and the reflection of a ghost column
+    unsigned ld_off;
+    bool sel_a = false, sel_b = true;
+    float2 xa = {1.f, 1.f}, xb = {1.f, 1.f};
+    float hWa, hEa, hWb, hEb;
+    if (!GEN) {
+        // the periodic image for loads; columns outside the mesh are seam ghosts, which seam
+        // mode 0 leaves unreconstructed (as stage_f32_strip does)
+        int cq0 = c0 % nx;
+        if (cq0 < 0) cq0 += nx;
+        ld_off = (unsigned)cq0 * 4u;
+        hWa = MUSCL && (!ga || seam_rec) ? -1.f : 0.f; hEa = MUSCL && (!ga || seam_rec) ? 1.f : 0.f;
+        hWb = MUSCL && (!gb || seam_rec) ? -1.f : 0.f; hEb = MUSCL && (!gb || seam_rec) ? 1.f : 0.f;
+    } else {
+        auto owner = [&](int c) {
+            if (a.per_x) { int q = c % nx; return q < 0 ? q + nx : q; }
+            return c < 0 ? 0 : (c >= nx ? nx - 1 : c);
+        };
+        const int oa = owner(c0), ob = owner(c0 + 1);
+        const int lc = a.per_x ? oa : min(oa, nx - 2);
+        sel_a = oa != lc;
+        sel_b = ob != lc;
+        ld_off = (unsigned)lc * 4u;
+        if (!a.per_x) {
+            const float2 lo = reflect_uv(a.bc_kind[0], 0), hi = reflect_uv(a.bc_kind[1], 0), one = {1.f, 1.f};
+            xa = ga ? (c0 < 0 ? lo : hi) : one;
+            xb = gb ? (c0 + 1 < 0 ? lo : hi) : one;
+        }
+        // face state = u + h * half_slope: a ghost column is reconstructed only across a
+        // periodic seam in seam mode 1, and a boundary side of an edge column not at all
+        auto hw = [&](int c, bool gh) { return MUSCL && (gh ? (a.per_x && seam_rec) : !(c == 0 && !a.per_x)) ? -1.f : 0.f; };
+        auto he = [&](int c, bool gh) { return MUSCL && (gh ? (a.per_x && seam_rec) : !(c == nx - 1 && !a.per_x)) ? 1.f : 0.f; };
+        hWa = hw(c0, ga); hEa = he(c0, ga); hWb = hw(c0 + 1, gb); hEb = he(c0 + 1, gb);
+    }
     // stores and U0 loads of lanes that write nothing take an offset of 2^31, past the set (under
     // 2^31 bytes) with or without the scalar offset added, so the descriptor drops them
     const unsigned st_off = out_lane ? (unsigned)c0 * 4u : 0x80000000u;
+    const unsigned st_off_b = out_lane && !gb ? (unsigned)(c0 + 1) * 4u : 0x80000000u;
+    const unsigned u0_off = GEN ? (out_lane ? ld_off : 0x80000000u) : st_off;
     const unsigned ps = a.plane_stride, set_bytes = 4u * a.plane_stride, rowb = (unsigned)nx * 4u;
 
-    // rows: one wrap suffices (ny >= 4 and rows span j0 - 2 .. j1 + 3)
-    auto wrap_row = [&](int r) -> unsigned {
-        if (r < 0) r += ny;
-        else if (r >= ny) r -= ny;
-        return (unsigned)r * rowb;
+    // rows: one wrap suffices (ny >= 4 and rows span j0 - 2 .. j1 + 3); GEN: a non-periodic
+    // ghost row is its edge row reflected as the side's condition says (wave-uniform factors)
+    struct RowRef { unsigned off; float2 f; };
+    auto row_of = [&](int r) -> RowRef {
+        float2 f = {1.f, 1.f};
+        if (r < 0 || r >= ny) {
+            if (!GEN || a.per_y) r = r < 0 ? r + ny : r - ny;
+            else {
+                f = reflect_uv(r < 0 ? a.bc_kind[2] : a.bc_kind[3], 1);
+                r = r < 0 ? 0 : ny - 1;
+            }
+        }
+        return {(unsigned)r * rowb, f};
     };
     // a row of the two columns from a set; `live` false issues the loads through a zero-length
     // descriptor (they return 0), so every row step has the same memory operations
@@ -458,11 +510,25 @@ __device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const i
             v[m][0] = x.x;
             v[m][1] = x.y;
         }
-        p.a = {v[0][0], v[1][0], v[2][0], v[3][0]};
-        p.b = {v[0][1], v[1][1], v[2][1], v[3][1]};
+        if (GEN) {
+            p.a = {sel_a ? v[0][1] : v[0][0], sel_a ? v[1][1] : v[1][0], sel_a ? v[2][1] : v[2][0], sel_a ? v[3][1] : v[3][0]};
+            p.b = {sel_b ? v[0][1] : v[0][0], sel_b ? v[1][1] : v[1][0], sel_b ? v[2][1] : v[2][0], sel_b ? v[3][1] : v[3][0]};
+        } else {
+            p.a = {v[0][0], v[1][0], v[2][0], v[3][0]};
+            p.b = {v[0][1], v[1][1], v[2][1], v[3][1]};
+        }
         return p;
     };
-    auto load_row = [&](int r, bool live) -> Pair { return ld_pair(U, wrap_row(r), ld_off, live); };
+    auto load_row = [&](int r, bool live) -> Pair {
+        const RowRef rw = row_of(r);
+        Pair p = ld_pair(U, rw.off, ld_off, live);
+        if (GEN) {
+            // the y and x reflections of stage_f32_strip's loader (each a sign flip)
+            p.a.ru *= rw.f.x * xa.x; p.a.rv *= rw.f.y * xa.y;
+            p.b.ru *= rw.f.x * xb.x; p.b.rv *= rw.f.y * xb.y;
+        }
+        return p;
+    };
 
     float lam_max = 0.f;
     int bad = 0;
@@ -471,9 +537,17 @@ __device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const i
     auto y_states = [&](int jc, const Pair& uc, const Pair& fwd, const Pair& bwd, PrepPair& lo_p, PrepPair& hi_p) {
         if (MUSCL) {
             const bool row_ghost = (jc < 0 || jc >= ny);
-            const bool rec = !row_ghost || seam_rec;
-            const float hS = rec ? -1.f : 0.f;
-            const float hN = rec ? 1.f : 0.f;
+            float hS, hN;
+            if (!GEN) {
+                const bool rec = !row_ghost || seam_rec;
+                hS = rec ? -1.f : 0.f;
+                hN = rec ? 1.f : 0.f;
+            } else {
+                const bool yS = row_ghost ? (a.per_y && seam_rec) : !(jc == 0 && !a.per_y);
+                const bool yN = row_ghost ? (a.per_y && seam_rec) : !(jc == ny - 1 && !a.per_y);
+                hS = yS ? -1.f : 0.f;
+                hN = yN ? 1.f : 0.f;
+            }
             const F4 sa = half_slope4<LIM>(fwd.a, bwd.a), sb = half_slope4<LIM>(fwd.b, bwd.b);
             lo_p.a = fast::prep(axpy4(uc.a, hS, sa), g, gm1);
             hi_p.a = fast::prep(axpy4(uc.a, hN, sa), g, gm1);
@@ -491,7 +565,7 @@ __device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const i
                     PrepPair& hout, Pair& gin, Pair& gout, Pair& xin, Pair& xout, const Pair& u0in, Pair& u0out) {
         const bool full = jc < j1;
         const bool out = jc > j0 && jc <= j1;
-        u0out = ld_pair(U0, (unsigned)jc * rowb, st_off, full && ca != 0.f);
+        u0out = ld_pair(U0, (unsigned)jc * rowb, u0_off, full && ca != 0.f);
         dout = sub_pair(up, uc);
         PrepPair lo_p;
         y_states(jc, uc, dout, din, lo_p, hout);
@@ -540,17 +614,26 @@ __device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const i
                 } else {
                     o[0][k] = uu.r + rr; o[1][k] = uu.ru + rru; o[2][k] = uu.rv + rrv; o[3][k] = uu.re + rre;
                 }
-                if (fuse && live) {
+                // the second column of an odd row's last pair is a ghost: no CFL contribution
+                if (fuse && live && (k == 0 || !GEN || !gb)) {
                     float l;
                     if (lam_cell_f32<false>(a, o[0][k], o[1][k], o[2][k], o[3][k], l)) lam_max = fmaxf(lam_max, l);
                     else bad++;
                 }
             }
+            if (!GEN) {
 #pragma unroll
-            for (int m = 0; m < 4; m++) {
-                typedef int v2i __attribute__((ext_vector_type(2)));
-                const v2i w = {__builtin_bit_cast(int, o[m][0]), __builtin_bit_cast(int, o[m][1])};
-                __builtin_amdgcn_raw_buffer_store_b64(w, rs, st_off, roff + (unsigned)m * ps, 0);
+                for (int m = 0; m < 4; m++) {
+                    typedef int v2i __attribute__((ext_vector_type(2)));
+                    const v2i w = {__builtin_bit_cast(int, o[m][0]), __builtin_bit_cast(int, o[m][1])};
+                    __builtin_amdgcn_raw_buffer_store_b64(w, rs, st_off, roff + (unsigned)m * ps, 0);
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, o[m][0]), rs, st_off, roff + (unsigned)m * ps, 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, o[m][1]), rs, st_off_b, roff + (unsigned)m * ps, 0);
+                }
             }
         }
         um = load_row(jc + 3, full && jc + 3 <= j1 + 1);
@@ -591,12 +674,12 @@ __device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const i
     }
 }
 
-template <int FLUX, int LIM>
+template <int FLUX, int LIM, bool GEN>
 __global__ __launch_bounds__(256) void stage_f32_pair_kernel(StageArgs a) {
     const int s = strip_index(a, blockDim.x / WAVE);
     if (s >= a.n_strips_x * a.n_strips_y) return;
-    stage_f32_pair_strip<FLUX, LIM>(a, s, (const float*)a.uin[0], (const float*)a.u0[0], (float*)a.uout[0], a.stage,
-                                    a.parity);
+    stage_f32_pair_strip<FLUX, LIM, GEN>(a, s, (const float*)a.uin[0], (const float*)a.u0[0], (float*)a.uout[0],
+                                         a.stage, a.parity);
 }
 
 // ---------------------------------------------------------------------------
@@ -700,6 +783,25 @@ __global__ __launch_bounds__(256) void stage_f32_batch_kernel(const BatchCase* _
     }
 }
 
+// The batch's two-column work items: cases the general two-column kernel takes (stock
+// copy / reflection / periodic sides, no features), through the same strip code.
+template <int FLUX, int LIM>
+__global__ __launch_bounds__(256) void stage_f32_batch_pair_kernel(const BatchCase* __restrict__ cases,
+                                                                   const int2* __restrict__ work, int n_work,
+                                                                   int stage, int parity) {
+    const int wpb = blockDim.x / WAVE;
+    const int nw = gridDim.x * wpb;
+    for (int w = blockIdx.x * wpb + (int)(threadIdx.x / WAVE); w < n_work; w += nw) {
+        const int wu = __builtin_amdgcn_readfirstlane(w);
+        const int2 wk = work[wu];
+        const BatchCase& bc = cases[wk.x];
+        const int in = stage == 0 ? 0 : (stage == 1 ? 1 : 2);
+        const int out = stage == 0 ? 1 : (stage == 1 ? 2 : 0);
+        stage_f32_pair_strip<FLUX, LIM, true>(bc.a, wk.y, (const float*)bc.P[in][0], (const float*)bc.P[0][0],
+                                              (float*)bc.P[out][0], stage, parity);
+    }
+}
+
 namespace {
 int features_of(const StageArgs& a) {
     int f = 0;
@@ -747,16 +849,18 @@ hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st) {
     const dim3 grid((nstrips + waves - 1) / waves);
     if (a.pair_path) {
         const int lim = a.muscl ? a.limiter : LIM_NONE;
-#define EFVM_PAIR(F)                                                                                     \
-        do {                                                                                             \
-            if (lim == LIM_MC) stage_f32_pair_kernel<F, LIM_MC><<<grid, block, 0, st>>>(a);              \
-            else if (lim == LIM_MINMOD) stage_f32_pair_kernel<F, LIM_MINMOD><<<grid, block, 0, st>>>(a); \
-            else stage_f32_pair_kernel<F, LIM_NONE><<<grid, block, 0, st>>>(a);                          \
+#define EFVM_PAIR_G(F, G)                                                                                   \
+        do {                                                                                                \
+            if (lim == LIM_MC) stage_f32_pair_kernel<F, LIM_MC, G><<<grid, block, 0, st>>>(a);              \
+            else if (lim == LIM_MINMOD) stage_f32_pair_kernel<F, LIM_MINMOD, G><<<grid, block, 0, st>>>(a); \
+            else stage_f32_pair_kernel<F, LIM_NONE, G><<<grid, block, 0, st>>>(a);                          \
         } while (0)
+#define EFVM_PAIR(F) do { if (a.pair_path == 1) EFVM_PAIR_G(F, false); else EFVM_PAIR_G(F, true); } while (0)
         if (a.flux == FLUX_RUSANOV) EFVM_PAIR(FLUX_RUSANOV);
         else if (a.flux == FLUX_HLLC) EFVM_PAIR(FLUX_HLLC);
         else EFVM_PAIR(FLUX_HLLC_TEXTBOOK);
 #undef EFVM_PAIR
+#undef EFVM_PAIR_G
         return hipGetLastError();
     }
     const StageArgs& a_ = a;
@@ -898,6 +1002,23 @@ hipError_t launch_stage_f32_faces(const StageArgs& a, const float* F, hipStream_
     case FLUX_HLLC: stage_f32_faces_kernel<FLUX_HLLC><<<grid, 256, 0, st>>>(a, F); break;
     default: stage_f32_faces_kernel<FLUX_HLLC_TEXTBOOK><<<grid, 256, 0, st>>>(a, F); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_f32_batch_pair(const BatchCase* cases, const StageArgs& a_, const int2* work, int n_work, int stage,
+                                       int parity, int blocks, int waves, hipStream_t st) {
+    const dim3 grid(blocks), block(WAVE * waves);
+    const int lim = a_.muscl ? a_.limiter : LIM_NONE;
+#define EFVM_BPAIR(F)                                                                                                  \
+    do {                                                                                                               \
+        if (lim == LIM_MC) stage_f32_batch_pair_kernel<F, LIM_MC><<<grid, block, 0, st>>>(cases, work, n_work, stage, parity); \
+        else if (lim == LIM_MINMOD) stage_f32_batch_pair_kernel<F, LIM_MINMOD><<<grid, block, 0, st>>>(cases, work, n_work, stage, parity); \
+        else stage_f32_batch_pair_kernel<F, LIM_NONE><<<grid, block, 0, st>>>(cases, work, n_work, stage, parity);      \
+    } while (0)
+    if (a_.flux == FLUX_RUSANOV) EFVM_BPAIR(FLUX_RUSANOV);
+    else if (a_.flux == FLUX_HLLC) EFVM_BPAIR(FLUX_HLLC);
+    else EFVM_BPAIR(FLUX_HLLC_TEXTBOOK);
+#undef EFVM_BPAIR
     return hipGetLastError();
 }
 

@@ -208,14 +208,15 @@ class Context:
     def set_launch(self, strip_rows=0, waves_per_block=0):
         self._chk(self._L.efvm_set_launch(self._p, strip_rows, waves_per_block))
 
-    KERNELS = ("stage_f32_kernel (one-column strips)", "stage_f32_pair_kernel (two-column periodic strips)",
+    KERNELS = ("stage_f32_kernel (one-column strips)", "stage_f32_pair_kernel (two-column strips)",
                "stage_f64_kernel", "learned face kernels")
 
     def launch(self):
-        """(strip_rows, waves_per_block, kernel name, whether the two-column kernel applies)."""
+        """(strip_rows, waves_per_block, kernel name, which two-column kernel applies: 0 none, 1 the
+        periodic form, 2 the general form)."""
         r, w, k, f = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         self._chk(self._L.efvm_get_launch(self._p, C.byref(r), C.byref(w), C.byref(k), C.byref(f)))
-        return r.value, w.value, self.KERNELS[k.value], bool(f.value)
+        return r.value, w.value, self.KERNELS[k.value], f.value
 
     def set_pair_kernel(self, allow: bool):
         """Allow (default) or forbid the two-column periodic strip kernel."""
@@ -389,6 +390,7 @@ def _bind_batch(L):
         "efvm_batch_last_elapsed_ms": [vp, pd],
         "efvm_batch_snapshot": [vp, C.c_int],
         "efvm_batch_set_rows": [vp, C.POINTER(C.c_int)],
+        "efvm_batch_set_pair_kernel": [vp, C.c_int],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -405,7 +407,7 @@ EXPORTS += [
     "efvm_batch_create", "efvm_batch_destroy", "efvm_batch_last_error", "efvm_batch_set_launch",
     "efvm_batch_set_grid", "efvm_batch_upload", "efvm_batch_download", "efvm_batch_step",
     "efvm_batch_graph_capture", "efvm_batch_graph_launch", "efvm_batch_clock", "efvm_batch_synchronize",
-    "efvm_batch_last_elapsed_ms", "efvm_batch_snapshot", "efvm_batch_set_rows",
+    "efvm_batch_last_elapsed_ms", "efvm_batch_snapshot", "efvm_batch_set_rows", "efvm_batch_set_pair_kernel",
 ]
 
 
@@ -463,6 +465,10 @@ class Batch:
 
     def set_grid(self, blocks, waves_per_block=4):
         self._chk(self._L.efvm_batch_set_grid(self._p, blocks, waves_per_block))
+
+    def set_pair_kernel(self, allow: bool):
+        """Allow (default) or forbid the two-column strip kernel for the cases it takes."""
+        self._chk(self._L.efvm_batch_set_pair_kernel(self._p, int(bool(allow))))
 
     def upload(self, i, cons):
         a = np.ascontiguousarray(cons, dtype=np.float64).reshape(-1, 5)

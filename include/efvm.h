@@ -90,16 +90,20 @@ const char* efvm_last_error(const efvm_ctx* ctx);
  * 0 keeps the current value.  Used by the autotuner. */
 efvm_status efvm_set_launch(efvm_ctx* ctx, int strip_rows, int waves_per_block);
 /* The current launch shape and the stage kernel the context runs: *kernel = 0 one-column
- * strips (64 columns per wave), 1 two-column periodic strips (128 per wave), 2 FP64 strips,
- * 3 learned face kernels; *pair_fits = 1 when the two-column kernel applies (FP32, both axes
- * periodic, no boundary features, nx even).  Any pointer may be null.  No reference
- * counterpart (a tuning query). */
+ * strips (64 columns per wave), 1 two-column strips (128 per wave), 2 FP64 strips, 3 learned
+ * face kernels; *pair_fits = which two-column form applies (FP32, no boundary features):
+ * 0 none, 1 the periodic form (both axes periodic, nx even), 2 the general form (periodic,
+ * outflow, slip, no-slip and symmetry sides; nx even on a periodic x axis).  Any pointer may
+ * be null.  No reference counterpart (a tuning query). */
 efvm_status efvm_get_launch(const efvm_ctx* ctx, int* strip_rows, int* waves_per_block, int* kernel, int* pair_fits);
 /* The gcnArchName of HIP device `device` (e.g. "gfx950:sramecc+:xnack-") into arch[n]: the
  * autotuner's cache key.  No reference counterpart. */
 efvm_status efvm_device_arch(int device, char* arch, size_t n);
-/* Use the two-column periodic kernel where it applies (1, the default) or always the
- * one-column kernel (0).  Used by the autotuner; no reference counterpart. */
+/* Use a two-column strip kernel where one applies (1, the default) or always the one-column
+ * kernel (0).  The periodic form takes grids periodic on both axes with nx even; the general
+ * form takes any mix of periodic, outflow, slip, no-slip and symmetry sides (nx even on a
+ * periodic x axis) with no boundary features.  Results are bitwise the same.  Used by the
+ * autotuner; no reference counterpart. */
 efvm_status efvm_set_pair_kernel(efvm_ctx* ctx, int allow);
 
 /* BoundaryRegistry + StructuredBoundaryPlugin (crates/cfd_boundary/src/lib.rs:369-533):
@@ -201,6 +205,10 @@ efvm_status efvm_batch_set_launch(efvm_batch* b, int case_id, int strip_rows);
 efvm_status efvm_batch_set_rows(efvm_batch* b, const int* rows);
 /* launch grid: workgroups (0 = one wave per work item, the default; fewer make the
  * waves stride through the work table) and waves per workgroup */
+/* Allow (1, the default) or forbid (0) the two-column strip kernel for the cases it takes
+ * (periodic, outflow, slip, no-slip and symmetry sides, no features; nx even on a periodic x
+ * axis).  No counterpart in the reference: a launch choice, results are bitwise the same. */
+efvm_status efvm_batch_set_pair_kernel(efvm_batch* batch, int allow);
 efvm_status efvm_batch_set_grid(efvm_batch* b, int blocks, int waves_per_block);
 efvm_status efvm_batch_upload(efvm_batch* b, int case_id, const double* cons5, size_t n_cells);
 efvm_status efvm_batch_download(efvm_batch* b, int case_id, double* cons5, size_t n_cells);

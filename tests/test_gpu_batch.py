@@ -47,6 +47,25 @@ def test_batch_equals_single_contexts_bitwise():
         assert bc["time"] == clk["time"] and bc["step"] == 6
 
 
+def test_batch_two_column_kernel_is_bitwise_the_one_column_kernel():
+    """The batch runs the cases the general two-column kernel takes (here every case: periodic
+    vortices with even and odd sides, outflow Riemann problems) on it; forbidding it sends every
+    case to the one-column kernel, with bitwise the same states and clocks."""
+    specs, states = small_ensemble(n_cases=12, seed=5)
+    assert any(sp["nx"] % 2 for sp in specs) and any(sp["nx"] % 2 == 0 for sp in specs)
+    out = []
+    for allow in (True, False):
+        b = efvm.Batch(specs)
+        b.set_pair_kernel(allow)
+        for i, st in enumerate(states):
+            b.upload(i, st)
+        b.step(7)
+        out.append(([b.download(i) for i in range(len(specs))], [b.clock(i) for i in range(len(specs))]))
+    for i in range(len(specs)):
+        assert np.array_equal(out[0][0][i], out[1][0][i]), (i, specs[i]["kind"], specs[i]["nx"])
+        assert out[0][1][i] == out[1][1][i]
+
+
 def test_batch_per_case_dt_differs():
     specs, states = small_ensemble()
     b = efvm.Batch(specs)

@@ -435,6 +435,60 @@ def test_fp32_cylinder_wall_force_tracks_fp64():
     assert d[:3].max() <= 1e-5 * sp and d[3:].max() <= 1e-5 * sv
 
 
+# ------------------------------------------------------------------ the general two-column kernel
+def rough_state(nx, ny, seed, hi=(1.0, 1.0)):
+    """Positive density and pressure with seeded modes in all four components and a jump."""
+    rng = np.random.default_rng(seed)
+    y, x = np.meshgrid((np.arange(ny) + 0.5) / ny * hi[1], (np.arange(nx) + 0.5) / nx * hi[0], indexing="ij")
+    def field(base, amp):
+        f = np.full_like(x, base)
+        for _ in range(3):
+            k = rng.integers(1, 4, size=2)
+            f += amp * np.sin(2 * np.pi * k[0] * x / hi[0] + rng.uniform(0, 6.3)) * np.cos(2 * np.pi * k[1] * y / hi[1])
+        return f
+    rho = field(1.0, 0.1) + 0.3 * (x > 0.6 * hi[0])
+    u, v, p = field(0.3, 0.2), field(-0.2, 0.2), field(1.0, 0.1)
+    return np.stack([rho, rho * u, rho * v, 0 * x, p / 0.4 + 0.5 * rho * (u * u + v * v)], axis=-1)
+
+
+GENERAL_PAIR_CASES = [
+    dict(n=(130, 37), bcs=["outflow"] * 4, flux="hllc", muscl=True, limiter="mc", integrator="rk3"),
+    dict(n=(131, 40), bcs=["slip", "noslip", "symmetry", "outflow"], flux="rusanov", muscl=True, limiter="minmod",
+         integrator="rk3"),
+    dict(n=(128, 33), bcs=["periodic", "periodic", "slip", "noslip"], flux="hllc", muscl=True, limiter="mc",
+         integrator="rk3", seam_mode=1),
+    dict(n=(125, 36), bcs=["outflow", "noslip", "periodic", "periodic"], flux="hllc_textbook", muscl=False,
+         integrator="rk3"),
+    dict(n=(253, 21), bcs=["noslip", "outflow", "outflow", "slip"], flux="hllc", muscl=True, limiter="mc",
+         integrator="euler"),
+    dict(n=(5, 4), bcs=["outflow", "slip", "noslip", "symmetry"], flux="hllc", muscl=True, limiter="minmod",
+         integrator="rk3"),
+]
+
+
+@pytest.mark.parametrize("cfg", GENERAL_PAIR_CASES, ids=lambda c: f"{c['n'][0]}x{c['n'][1]}-{'-'.join(c['bcs'])}")
+def test_general_two_column_kernel_is_bitwise_the_one_column_kernel(cfg):
+    """Outflow, slip, no-slip and symmetry sides, odd widths, a periodic axis beside walls, first
+    order, Euler: the two-column kernel's ghost columns and rows (owner values reflected as each
+    side's condition says), its edge columns' first-order boundary faces and its one-column
+    stores of an odd row's last pair give bitwise the one-column kernel's states and clocks."""
+    nx, ny = cfg["n"]
+    kw = {k: v for k, v in cfg.items() if k not in ("n", "bcs")}
+    out = []
+    for allow in (True, False):
+        c = efvm.Context(nx, ny, hi=(1.0, 1.0), dz=1.0 / nx, precision="fp32", cfl=0.4, **kw)
+        c.set_boundaries(cfg["bcs"])
+        c.set_pair_kernel(allow)
+        kernel = c.launch()[2]
+        assert ("pair" in kernel) == allow, kernel
+        c.upload(rough_state(nx, ny, seed=nx + ny))
+        c.step(9)
+        c.run()                     # and one lone substage
+        out.append((c.download(), c.clock()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
 # ------------------------------------------------------------------ FP32 production path
 def test_fp32_error_against_fp64_config2_physics():
     """FP32 (flux/update arithmetic) vs the FP64 path on the config-2 physics, 256^2, 50 steps.
