@@ -55,11 +55,17 @@ def cpu_baseline(n_cpu: int = 1024, steps: int = 6, steps_threaded: int = 30):
         del sim
     va, ca, ka, da = out["all"]
     v1, _, k1, d1 = out["one"]
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), model)
+    except OSError:
+        pass
     return {"value": va, "unit": "cell-updates/s", "cores": ca, "kind": "port",
             "single_thread_value": v1,
             "sample": f"oracle/efvm_oracle.c FP64 MUSCL-MC+HLLC+RK3, periodic {n_cpu}^2 vortex+modes: {ka} steps on "
                       f"{ca} threads ({da:.1f} s) and {k1} steps on 1 thread ({d1:.1f} s); {os.cpu_count()} host "
-                      f"cores visible"}
+                      f"cores visible, {model}"}
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> float:
@@ -110,7 +116,7 @@ def bench_batch(args):
     sides = [s["nx"] for s in specs]
     total = sum(s["nx"] * s["ny"] for s in specs)
     t0 = time.perf_counter()
-    rows, blocks, tuned_ms = tune_batch(b, sides, steps=2, use_cache=not args.retune)
+    rows, blocks, tuned_ms = tune_batch(b, sides, steps=4, use_cache=not args.retune)
     t_tune = time.perf_counter() - t0
     b.step(args.warmup)
     b.synchronize()

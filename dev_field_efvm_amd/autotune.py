@@ -93,13 +93,19 @@ def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=
     return best[:3]
 
 
-def _time_batch(b, steps):
+def _time_batch(b, steps, repeats=2):
+    """Best of `repeats` timings of `steps` steps (HIP events), after one warm-up step: a
+    single short timing of a 95 M-cell ensemble varies by a few percent from run to run."""
     b.step(1)
-    b.step(steps)
-    return b.last_elapsed_ms() / steps
+    best = None
+    for _ in range(repeats):
+        b.step(steps)
+        ms = b.last_elapsed_ms() / steps
+        best = ms if best is None else min(best, ms)
+    return best
 
 
-def tune_batch(batch, sides, steps: int = 2, rows=BATCH_ROWS, blocks=BATCH_BLOCKS, use_cache=True, verbose=False, store=True,
+def tune_batch(batch, sides, steps: int = 4, rows=BATCH_ROWS, blocks=BATCH_BLOCKS, use_cache=True, verbose=False, store=True,
                key_extra=""):
     """Per size class strip rows, then the persistent grid size (coordinate descent).
 
