@@ -25,6 +25,7 @@
 namespace efvm {
 hipError_t launch_stage_f64(const StageArgs& a, int block, hipStream_t st);
 hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st);
+hipError_t launch_stage_f32_pair23(const StageArgs& a, int block, hipStream_t st);
 template <class T> hipError_t launch_cfl(const StageArgs& a, const void* const* planes, double* target, hipStream_t st);
 hipError_t launch_clock_set(DevClock* clk, double dt, double time, long long step, hipStream_t st);
 struct IbmArgs {
@@ -99,7 +100,13 @@ struct efvm_ctx {
     int strip_h = 31;   // rows per strip; the FP32 kernel runs row steps in groups of four (strip_h + 1 of them)
     int waves_per_block = 4;
     int allow_pair = 1;            // FP32 periodic grids may use the two-column strip kernel
+    int allow_fuse23 = 1;          // ... and run RK3 stages 2 and 3 as one fused launch (seam mode 1)
+    int strip_h23 = 73;            // rows per strip of the fused stage-2+3 kernel
+    int waves23 = 4;               // its waves per block
     int xcd_remap = 1;
+    // consecutive stage launches sweep the strips in opposite directions, so each launch starts
+    // where the previous one ended, on rows whose lines the device's last-level cache still holds
+    unsigned dir_count = 0;
     int parabolic = 0;
     double inflow[3] = {0, 0, 0};
     int sponge = 0;
@@ -232,6 +239,29 @@ efvm_status launch_cfl_into(efvm_ctx* c, void* const* planes, double* target) {
     return EFVM_OK;
 }
 
+// After the last substage of a step: U^{n+1} is in the old prev set and U^n becomes prev; the
+// sponge, the CFL of the new state (unless the stage fused it), the CFL slot parity.  u2_kept:
+// U2 is in the third set (not after a fused step), which the wall force's ghost cells read.
+efvm_status end_of_step(efvm_ctx* c, bool cfl_fused, int u2_kept) {
+    std::swap(c->cur, c->prev);
+    c->have_prev = 1;
+    c->ghost_hist = u2_kept;
+    StageArgs b = base_args(c);
+    set_planes(b, c->P[c->cur], c->P[c->cur], c->P[c->cur]);
+    if (c->sponge) {
+        hipError_t e = c->esz == 8 ? launch_sponge<double>(b, c->sp[0], c->sp[1], c->sp[2], c->sp[3], c->sp[4], c->sp[5], c->sp[6], c->stream)
+                                   : launch_sponge<float>(b, c->sp[0], c->sp[1], c->sp[2], c->sp[3], c->sp[4], c->sp[5], c->sp[6], c->stream);
+        if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "sponge");
+    }
+    if (c->sch.fixed_dt <= 0.0 && !cfl_fused) {
+        efvm_status st = launch_cfl_into(c, c->P[c->cur], &c->clock->lam[c->parity ^ 1]);
+        if (st != EFVM_OK) return st;
+    }
+    c->parity ^= 1;
+    c->rk_stage = 0;
+    return EFVM_OK;
+}
+
 // One substage (one app.run()).  Enqueue only; no host synchronisation.
 efvm_status run_substage(efvm_ctx* c) {
     const bool rk3 = c->sch.integrator == EFVM_INTEGRATOR_RK3;
@@ -255,6 +285,7 @@ efvm_status run_substage(efvm_ctx* c) {
     StageArgs a = base_args(c);
     a.parity = c->parity;
     a.stage = rk3 ? c->rk_stage : 0;
+    a.reverse = (int)(c->dir_count++ & 1u);
     void* const* in = A;
     void* const* out = B;
     if (rk3 && c->rk_stage == 1) { in = B; out = Cc; }
@@ -302,24 +333,64 @@ efvm_status run_substage(efvm_ctx* c) {
         hipError_t e = launch_lam_reduce(a.lam_part, nw, &c->clock->lam[a.parity ^ 1], c->stream);
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("cfl reduction: ") + hipGetErrorString(e));
     }
-    if (last) {
-        std::swap(c->cur, c->prev);   // U^{n+1} is in the old prev set; U^n becomes prev
-        c->have_prev = 1;
-        c->ghost_hist = 1;            // U^n (prev) and U2 (the third set) stay resident
-        StageArgs b = base_args(c);
-        set_planes(b, c->P[c->cur], c->P[c->cur], c->P[c->cur]);
-        if (c->sponge) {
-            hipError_t e = c->esz == 8 ? launch_sponge<double>(b, c->sp[0], c->sp[1], c->sp[2], c->sp[3], c->sp[4], c->sp[5], c->sp[6], c->stream)
-                                       : launch_sponge<float>(b, c->sp[0], c->sp[1], c->sp[2], c->sp[3], c->sp[4], c->sp[5], c->sp[6], c->stream);
-            if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "sponge");
-        }
-        if (c->sch.fixed_dt <= 0.0 && !a.fuse_cfl) {
-            if ((st = launch_cfl_into(c, c->P[c->cur], &c->clock->lam[c->parity ^ 1])) != EFVM_OK) return st;
-        }
-        c->parity ^= 1;
-        c->rk_stage = 0;
-    } else {
-        c->rk_stage++;
+    if (last) return end_of_step(c, a.fuse_cfl != 0, 1);
+    c->rk_stage++;
+    return EFVM_OK;
+}
+
+// Whole steps.  FP32 RK3 on the periodic two-column path runs stage 1, then
+// stages 2 and 3 fused (stage_f32_pair23_kernel): U1 goes to the third set, U^{n+1} to prev,
+// and the roles swap as after three substages; results are bitwise the same.
+bool fused_path_fits(const efvm_ctx* c, const StageArgs& a) {
+    return c->esz == 4 && !c->learned && c->sch.integrator == EFVM_INTEGRATOR_RK3 && a.pair_path == 1 &&
+           a.ny >= 8 && c->n_cyl == 0;
+}
+bool fused_step_fits(const efvm_ctx* c, const StageArgs& a) {
+    return c->allow_fuse23 && c->rk_stage == 0 && fused_path_fits(c, a);
+}
+
+StageArgs fused_args(const efvm_ctx* c, StageArgs a) {
+    a.strip_h = c->strip_h23;
+    a.n_strips_x = (a.nx + STRIP_OUT23 - 1) / STRIP_OUT23;
+    a.n_strips_y = (a.ny + c->strip_h23 - 1) / c->strip_h23;
+    return a;
+}
+
+efvm_status run_step_fused(efvm_ctx* c) {
+    void* const* A = c->P[c->cur];
+    void* const* B = c->P[c->prev];
+    void* const* Cc = c->P[3 - c->cur - c->prev];
+    efvm_status st;
+    if (!c->lam_valid && c->sch.fixed_dt <= 0.0) {
+        if ((st = launch_cfl_into(c, A, &c->clock->lam[c->parity])) != EFVM_OK) return st;
+        c->lam_valid = 1;
+    }
+    StageArgs a = base_args(c);
+    a.parity = c->parity;
+    a.stage = 0;
+    a.reverse = (int)(c->dir_count++ & 1u);
+    set_planes(a, A, A, Cc);             // stage 1: U^n -> U1 in the third set
+    if ((st = launch_stage(c, a)) != EFVM_OK) return st;
+    StageArgs b = fused_args(c, a);
+    b.stage = 2;
+    b.reverse = (int)(c->dir_count++ & 1u);
+    set_planes(b, Cc, A, B);             // stages 2 and 3: U1, U^n -> U^{n+1} in prev
+    b.fuse_cfl = !c->sponge && c->sch.fixed_dt <= 0.0;
+    hipError_t e = launch_stage_f32_pair23(b, WAVE * c->waves23, c->stream);
+    if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("fused stage kernel launch: ") + hipGetErrorString(e));
+    if (b.fuse_cfl && b.lam_part) {
+        e = launch_lam_reduce(b.lam_part, b.n_strips_x * b.n_strips_y, &c->clock->lam[b.parity ^ 1], c->stream);
+        if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("cfl reduction: ") + hipGetErrorString(e));
+    }
+    return end_of_step(c, b.fuse_cfl != 0, 0);
+}
+
+efvm_status run_step(efvm_ctx* c) {
+    if (fused_step_fits(c, base_args(c))) return run_step_fused(c);
+    const int nsub = c->sch.integrator == EFVM_INTEGRATOR_RK3 ? 3 : 1;
+    for (int q = 0; q < nsub; q++) {
+        efvm_status st = run_substage(c);
+        if (st != EFVM_OK) return st;
     }
     return EFVM_OK;
 }
@@ -450,6 +521,26 @@ efvm_status efvm_get_launch(const efvm_ctx* c, int* strip_rows, int* waves_per_b
     if (waves_per_block) *waves_per_block = c->waves_per_block;
     if (kernel) *kernel = c->learned ? 3 : (c->esz == 8 ? 2 : (base_args(c).pair_path ? 1 : 0));
     if (pair_fits_out) *pair_fits_out = pair_fits(c, base_args(c));
+    return EFVM_OK;
+}
+
+efvm_status efvm_set_fused_step(efvm_ctx* c, int allow, int strip_rows, int waves_per_block) {
+    if (!c) return EFVM_ERR_ARG;
+    if (strip_rows < 0 || strip_rows > 4096 || waves_per_block < 0 || waves_per_block > 4)
+        return fail(c, EFVM_ERR_ARG, "fused launch shape out of range");
+    c->allow_fuse23 = allow ? 1 : 0;
+    if (strip_rows) c->strip_h23 = strip_rows;
+    if (waves_per_block) c->waves23 = waves_per_block;
+    drop_graph(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm_get_fused_step(const efvm_ctx* c, int* allow, int* strip_rows, int* waves_per_block, int* fits) {
+    if (!c) return EFVM_ERR_ARG;
+    if (allow) *allow = c->allow_fuse23;
+    if (strip_rows) *strip_rows = c->strip_h23;
+    if (waves_per_block) *waves_per_block = c->waves23;
+    if (fits) *fits = fused_path_fits(c, base_args(c)) ? 1 : 0;
     return EFVM_OK;
 }
 
@@ -631,11 +722,18 @@ efvm_status efvm_step(efvm_ctx* c, int n_steps) {
     if (!c || n_steps < 0) return EFVM_ERR_ARG;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     const int nsub = c->sch.integrator == EFVM_INTEGRATOR_RK3 ? 3 : 1;
-    for (int k = 0; k < n_steps; k++)
-        for (int q = 0; q < nsub; q++) {
-            efvm_status st = run_substage(c);
+    for (int k = 0; k < n_steps; k++) {
+        // from a step boundary a whole step; from mid-step (after run() substages) three substages
+        if (c->rk_stage == 0) {
+            efvm_status st = run_step(c);
             if (st != EFVM_OK) return st;
+        } else {
+            for (int q = 0; q < nsub; q++) {
+                efvm_status st = run_substage(c);
+                if (st != EFVM_OK) return st;
+            }
         }
+    }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timed = 1;
     return EFVM_OK;
@@ -645,13 +743,12 @@ efvm_status efvm_advance_to(efvm_ctx* c, double t_final, long max_steps, long* d
     if (!c) return EFVM_ERR_ARG;
     if (c->rk_stage != 0) return fail(c, EFVM_ERR_ARG, "advance_to must start at a step boundary");
     long k = 0;
-    const int nsub = c->sch.integrator == EFVM_INTEGRATOR_RK3 ? 3 : 1;
     double t = 0.0;
     efvm_status st = efvm_get_clock(c, nullptr, &t, nullptr, nullptr);
     if (st != EFVM_OK && st != EFVM_ERR_NONPHYSICAL) return st;
     // the loop condition needs the clock every step (t >= t_final before each whole step)
     while (t < t_final && k < max_steps) {
-        for (int q = 0; q < nsub; q++) if ((st = run_substage(c)) != EFVM_OK) return st;
+        if ((st = run_step(c)) != EFVM_OK) return st;
         k++;
         DevClock h;
         HIPCHK(c, hipMemcpyAsync(&h, c->clock, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -687,9 +784,7 @@ efvm_status efvm_graph_capture(efvm_ctx* c, int n_steps) {
     const int cur0 = c->cur, prev0 = c->prev, par0 = c->parity, have0 = c->have_prev;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     efvm_status st = EFVM_OK;
-    const int nsub = c->sch.integrator == EFVM_INTEGRATOR_RK3 ? 3 : 1;
-    for (int k = 0; k < n_steps && st == EFVM_OK; k++)
-        for (int q = 0; q < nsub && st == EFVM_OK; q++) st = run_substage(c);
+    for (int k = 0; k < n_steps && st == EFVM_OK; k++) st = run_step(c);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (st != EFVM_OK) { if (g) (void)hipGraphDestroy(g); return st; }

@@ -6,9 +6,9 @@
 //  * per face state one reciprocal (1/rho) and one reciprocal square root
 //    (1/a, with a = (gamma p/rho) * rsq), shared by the flux that uses it;
 //  * Thornber's z = min(|v|_max / a_max, 1) uses the stored 1/a;
-//  * HLLC star region in flux form, F*_K = [S* (S_K U_K - F_K) + S_K p* D*] D,
-//    D = 1/(S_K - S*), with S* and D from one reciprocal (Toro sec. 10.4: the same
-//    flux as F_K + S_K (U*_K - U_K), without forming the star state);
+//  * HLLC as F_K + S_K (U*_K - U_K) with U*_K - U_K written out as one factor
+//    Phi/S_K = (S* - u_K)/(S_K - S*) times [rho, rho S_K, rho v_t, h + f_K S*]
+//    (Toro sec. 10.4), S* and 1/(S_K - S*) from one reciprocal;
 //  * upwind side chosen by selects; the star flux taken by select so an out-of-fan
 //    inf/NaN never leaks into the flux.
 // Mathematically identical to the reference; not bitwise.
@@ -60,6 +60,7 @@ template <int LIM> __host__ __device__ __forceinline__ float limit(float a, floa
 struct Prep {
     float r, ru, rv, re;   // conserved
     float u, v, p;         // velocity, pressure
+    float q;               // |v|^2
     float a, ia, ir;       // sound speed, 1/a, 1/rho
 };
 __host__ __device__ __forceinline__ Prep prep(const C4<float>& q, float g, float gm1) {
@@ -68,7 +69,9 @@ __host__ __device__ __forceinline__ Prep prep(const C4<float>& q, float g, float
     o.ir = rcp(q.r);
     o.u = q.ru * o.ir;
     o.v = q.rv * o.ir;
-    o.p = gm1 * (q.re - 0.5f * (q.ru * o.u + q.rv * o.v));
+    const float k = q.ru * o.u + q.rv * o.v;   // rho |v|^2
+    o.q = k * o.ir;
+    o.p = gm1 * (q.re - 0.5f * k);
     float x = g * o.p * o.ir;          // a^2
     o.ia = rsq(x);
     o.a = x * o.ia;
@@ -81,7 +84,8 @@ __host__ __device__ __forceinline__ Prep prep_prim(float r, float u, float v, fl
     o.r = r; o.u = u; o.v = v; o.p = p;
     o.ru = r * u;
     o.rv = r * v;
-    o.re = p * igm1 + 0.5f * r * (u * u + v * v);
+    o.q = u * u + v * v;
+    o.re = p * igm1 + 0.5f * r * o.q;
     o.ir = rcp(r);
     float x = g * p * o.ir;
     o.ia = rsq(x);
@@ -97,13 +101,18 @@ __host__ __device__ __forceinline__ float pick(bool c, float a, float b) {
     return __builtin_bit_cast(float, (__builtin_bit_cast(unsigned, a) & m) | (__builtin_bit_cast(unsigned, b) & ~m));
 }
 
+// HLLC (with Thornber's low-Mach velocity blend when THORNBER) through a face normal to axis
+// DIR, from the upwind side K of the contact: F = F_K + Phi [rho_K, rho_K S_K, rho_K v_t,
+// h_K + f_K S*], Phi = S_K (S* - u_K) / (S_K - S*) inside the wave fan (S_L < 0 <= S* or
+// S* < 0 <= S_R), Phi = 0 outside it.  This is F*_K = F_K + S_K (U*_K - U_K) (Toro sec. 10.4)
+// with U*_K - U_K = (S* - u_K)/(S_K - S*) [rho_K, rho_K S_K, rho_K v_t, h_K + f_K S*] written
+// out, f_K = rho_K (S_K - u_K), h_K = E_K + p_K.  gg = gamma/(gamma-1): with Thornber's
+// velocities the internal energy is kept, so h_K = gg p_K + rho_K |v'_K|^2 / 2.
 template <int DIR, bool THORNBER>
-__host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R) {
+__host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R, float gg) {
     float ul = L.u, vl = L.v, ur = R.u, vr = R.v;
-    float ql = 0.f, qr = 0.f;
     if (THORNBER) {
-        ql = ul * ul + vl * vl; qr = ur * ur + vr * vr;
-        const float vm = fsqrt(fmaxf(ql, qr));
+        const float vm = fsqrt(fmaxf(L.q, R.q));
         // min(vm / a_L, vm / a_R): vm >= 0 and rounding is monotone, so scaling the
         // smaller 1/a gives the same value as the smaller of the two products
         const float z = fminf(vm * fminf(L.ia, R.ia), 1.f);
@@ -124,43 +133,26 @@ __host__ __device__ __forceinline__ C4<float> hllc(const Prep& L, const Prep& R)
     const float Dn = fl - fr;
     const bool left = N <= 0.f;
     const float rK = pick(left, L.r, R.r), uK = pick(left, ul, ur), vK = pick(left, vl, vr);
-    const float pK = pick(left, L.p, R.p), unK = pick(left, unl, unr), sK = pick(left, sl, sr);
-    // energy of side K; with Thornber's velocities it is rebuilt so that the internal
-    // energy (and p, a) are unchanged -- for the upwind side only, the one the flux uses
-    float eK = pick(left, L.re, R.re);
-    if (THORNBER) eK = eK + 0.5f * rK * ((uK * uK + vK * vK) - pick(left, ql, qr));
-    const float fK = pick(left, fl, fr);            // rho_K (S_K - u_K)
+    const float pK = pick(left, L.p, R.p), sK = pick(left, sl, sr), fK = pick(left, fl, fr);
+    const float unK = DIR == 0 ? uK : vK, utK = DIR == 0 ? vK : uK;
+    const float hK = THORNBER ? fmaf(0.5f * rK, uK * uK + vK * vK, gg * pK) : pick(left, L.re, R.re) + pK;
     const float X = sK * Dn - N;
     const float rr = rcp(Dn * X);
     const float ss = N * X * rr;
     const float D = Dn * Dn * rr;
-    // physical flux of side K
-    const float mK = rK * unK;
-    C4<float> f;
-    f.r = mK;
-    f.ru = mK * uK + (DIR == 0 ? pK : 0.f);
-    f.rv = mK * vK + (DIR == 1 ? pK : 0.f);
-    f.re = (eK + pK) * unK;
-    // star-region flux in flux form (Toro sec. 10.4, eqs. 10.37-10.41 combined):
-    //   F*_K = [S* (S_K U_K - F_K) + S_K p* D*] / (S_K - S*),  D* = (0, n, S*),
-    //   p* = p_K + rho_K (S_K - u_K)(S* - u_K),
-    // with S_K U_K - F_K = (fK, fK u_n - p_K (normal), fK u_t, E_K (S_K - u_n) - p_K u_n).
-    const float SD = ss * D;
-    const float ps = pK + fK * (ss - unK);
-    const float sp = sK * ps;
-    const float ms = fK * SD;                                  // mass
-    const float ns = (ss * (fK * unK - pK) + sp) * D;          // normal momentum
-    const float es = SD * ((eK * (sK - unK) - pK * unK) + sp); // energy
-    const float fsr = ms, fsu = DIR == 0 ? ns : ms * uK, fsv = DIR == 1 ? ns : ms * vK;
-    // inside the fan the star flux, outside the side's own flux: lane selects, so an
-    // out-of-fan inf/NaN of the star terms never reaches the flux
-    // left: S_L < 0; right: S_R >= 0 -- one compare of S_K against the side flag, so the
-    // condition stays a lane mask (no 0/1 values in vector registers)
+    // left: in the fan when S_L < 0; right: when S_R >= 0 -- one compare of S_K against the
+    // side flag.  Outside the fan Phi and S* are taken as 0 by lane selects, so an out-of-fan
+    // inf/NaN of the star terms never reaches the flux.
     const bool fan = (sK < 0.f) == left;
-    f.r = pick(fan, fsr, f.r);
-    f.ru = pick(fan, fsu, f.ru);
-    f.rv = pick(fan, fsv, f.rv);
-    f.re = pick(fan, es, f.re);
+    const float phi = pick(fan, sK * (ss - unK) * D, 0.f);
+    const float sf = pick(fan, ss, 0.f);
+    C4<float> f;
+    const float m = rK * (unK + phi);                  // rho_K u_K + Phi rho_K
+    const float fn = m * unK + (pK + phi * fK);        // rho_K u_K^2 + p_K + Phi rho_K S_K
+    f.r = m;
+    f.ru = DIR == 0 ? fn : m * utK;
+    f.rv = DIR == 1 ? fn : m * utK;
+    f.re = hK * unK + phi * (hK + fK * sf);
     return f;
 }
 
@@ -177,9 +169,9 @@ __host__ __device__ __forceinline__ C4<float> rusanov(const Prep& L, const Prep&
 }
 
 template <int FLUX, int DIR>
-__host__ __device__ __forceinline__ C4<float> flux(const Prep& L, const Prep& R) {
+__host__ __device__ __forceinline__ C4<float> flux(const Prep& L, const Prep& R, float gg) {
     if (FLUX == 0) return rusanov<DIR>(L, R);
-    return hllc<DIR, FLUX == 1>(L, R);
+    return hllc<DIR, FLUX == 1>(L, R, gg);
 }
 
 }  // namespace fast

@@ -173,12 +173,12 @@ __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >=
 // the lab-frame primitives (rho, u, v, p) of the left / lower and right / upper sides.
 __device__ __forceinline__ void put_face(const FaceArgs& a, int axis, long cell, long np, const float L[4], const float R[4]) {
     if (a.out_mode == 1) {
-        const float gm = a.gamma, igm1 = 1.f / (a.gamma - 1.f);
+        const float gm = a.gamma, igm1 = 1.f / (a.gamma - 1.f), gg = gm * igm1;
         const fast::Prep pl = fast::prep_prim(L[0], L[1], L[2], L[3], gm, igm1);
         const fast::Prep pr = fast::prep_prim(R[0], R[1], R[2], R[3], gm, igm1);
         fast::C4<float> f;
-        if (axis == 0) f = a.flux == 0 ? fast::flux<0, 0>(pl, pr) : (a.flux == 1 ? fast::flux<1, 0>(pl, pr) : fast::flux<2, 0>(pl, pr));
-        else f = a.flux == 0 ? fast::flux<0, 1>(pl, pr) : (a.flux == 1 ? fast::flux<1, 1>(pl, pr) : fast::flux<2, 1>(pl, pr));
+        if (axis == 0) f = a.flux == 0 ? fast::flux<0, 0>(pl, pr, gg) : (a.flux == 1 ? fast::flux<1, 0>(pl, pr, gg) : fast::flux<2, 0>(pl, pr, gg));
+        else f = a.flux == 0 ? fast::flux<0, 1>(pl, pr, gg) : (a.flux == 1 ? fast::flux<1, 1>(pl, pr, gg) : fast::flux<2, 1>(pl, pr, gg));
         float* Fo = a.F + (long)(axis * 4) * np + cell;
         Fo[0] = f.r; Fo[np] = f.ru; Fo[2 * np] = f.rv; Fo[3 * np] = f.re;
     } else {

@@ -30,6 +30,7 @@ namespace efvm {
 constexpr int WAVE = 64;
 constexpr int STRIP_OUT = WAVE - 4;   // output columns per strip
 constexpr int STRIP_OUT2 = 2 * WAVE - 4;   // output columns per strip of the two-column periodic path
+constexpr int STRIP_OUT23 = 2 * WAVE - 8;  // output columns per strip of the fused stage-2+3 kernel
 
 struct DevClock {
     double lam[2];        // CFL spectral-radius maxima (ping-pong by step parity), as doubles
@@ -75,6 +76,8 @@ struct StageArgs {
     unsigned* face_count;        // learned bypass: deferred-face count, cleared by the stage kernel
     unsigned plane_stride;       // bytes from one plane of a set to the next when the four planes of
                                  // every set lie at a common stride (0: they do not)
+    int reverse;                 // deal the strips in reverse order (alternate launches sweep the
+                                 // grid in opposite directions; results do not depend on the order)
     int pair_path;               // FP32: a two-column strip kernel runs this stage (1 the periodic
                                  // form, 2 the general one; n_strips_x then counts strips of
                                  // STRIP_OUT2 columns)
@@ -124,7 +127,10 @@ __device__ __forceinline__ int strip_index(const StageArgs& a, int waves_per_blo
     if (a.xcd_remap && nb >= 8) {
         int per = nb / 8, rem = nb % 8;
         int x = b % 8, k = b / 8;                 // k-th workgroup dealt to XCD x
+        if (a.reverse) k = per + (x < rem ? 1 : 0) - 1 - k;
         b = x * per + (x < rem ? x : rem) + k;    // contiguous range per XCD (bijective)
+    } else if (a.reverse) {
+        b = nb - 1 - b;
     }
     // wave-uniform: keeps the strip's rows, and every per-row decision, in scalar registers
     return __builtin_amdgcn_readfirstlane(b * waves_per_block + (int)(threadIdx.x / WAVE));

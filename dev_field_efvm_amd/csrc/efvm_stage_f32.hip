@@ -44,7 +44,7 @@ __device__ __forceinline__ F4 from_right4(const F4& v) { return {from_right(v.r)
 __device__ __forceinline__ Prep from_leftP(const Prep& v) {
     Prep o;
     o.r = from_left(v.r); o.ru = from_left(v.ru); o.rv = from_left(v.rv); o.re = from_left(v.re);
-    o.u = from_left(v.u); o.v = from_left(v.v); o.p = from_left(v.p);
+    o.u = from_left(v.u); o.v = from_left(v.v); o.p = from_left(v.p); o.q = from_left(v.q);
     o.a = from_left(v.a); o.ia = from_left(v.ia); o.ir = from_left(v.ir);
     return o;
 }
@@ -162,6 +162,7 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
     const int j0 = syi * a.strip_h;
     const int j1 = min(j0 + a.strip_h, ny);
     const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0);
+    const float gg = (float)(a.gamma / (a.gamma - 1.0));   // gamma / (gamma - 1): enthalpy per pressure
     const double dtd = a.fixed_dt > 0.0 ? a.fixed_dt : a.cfl / a.clock->lam[parity];
     const float dt = (float)dtd;
     const bool fuse = a.fuse_cfl && (a.euler || stage == 2);
@@ -244,7 +245,7 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
         dout = sub4(up, uc);
         Prep lo_p;
         y_states(jc, uc, dout, din, lo_p, hout);
-        gout = fast::flux<FLUX, 1>(hin, lo_p);
+        gout = fast::flux<FLUX, 1>(hin, lo_p, gg);
         if (full) {
             Prep W_p, E_p;
             if (MUSCL) {
@@ -257,7 +258,7 @@ __device__ __forceinline__ void stage_f32_strip(const StageArgs& a, const int s,
                 E_p = lo_p;
             }
             const Prep El = from_leftP(E_p);
-            const F4 Fw = fast::flux<FLUX, 0>(El, W_p);
+            const F4 Fw = fast::flux<FLUX, 0>(El, W_p, gg);
             const F4 Fe = from_right4(Fw);
             xout = sub4(Fw, Fe);
             if (VISC) {
@@ -429,6 +430,7 @@ __device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const i
     const int j0 = syi * a.strip_h;
     const int j1 = min(j0 + a.strip_h, ny);
     const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0);
+    const float gg = (float)(a.gamma / (a.gamma - 1.0));   // gamma / (gamma - 1): enthalpy per pressure
     const double dtd = a.fixed_dt > 0.0 ? a.fixed_dt : a.cfl / a.clock->lam[parity];
     const float dt = (float)dtd;
     const bool fuse = a.fuse_cfl && (a.euler || stage == 2);
@@ -569,8 +571,8 @@ __device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const i
         dout = sub_pair(up, uc);
         PrepPair lo_p;
         y_states(jc, uc, dout, din, lo_p, hout);
-        gout.a = fast::flux<FLUX, 1>(hin.a, lo_p.a);
-        gout.b = fast::flux<FLUX, 1>(hin.b, lo_p.b);
+        gout.a = fast::flux<FLUX, 1>(hin.a, lo_p.a, gg);
+        gout.b = fast::flux<FLUX, 1>(hin.b, lo_p.b, gg);
         if (full) {
             Prep Wa, Ea, Wb, Eb;
             if (MUSCL) {
@@ -585,8 +587,8 @@ __device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const i
             } else {
                 Wa = lo_p.a; Ea = lo_p.a; Wb = lo_p.b; Eb = lo_p.b;
             }
-            const F4 Fmid = fast::flux<FLUX, 0>(Ea, Wb);             // the face between the pair
-            const F4 Fwa = fast::flux<FLUX, 0>(from_leftP(Eb), Wa);  // the face west of c0
+            const F4 Fmid = fast::flux<FLUX, 0>(Ea, Wb, gg);             // the face between the pair
+            const F4 Fwa = fast::flux<FLUX, 0>(from_leftP(Eb), Wa, gg);  // the face west of c0
             const F4 Fe = from_right4(Fwa);                          // the face east of c0 + 1
             xout.a = sub4(Fwa, Fmid);
             xout.b = sub4(Fmid, Fe);
@@ -683,6 +685,263 @@ __global__ __launch_bounds__(256) void stage_f32_pair_kernel(StageArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Stages 2 and 3 of SSP-RK3 in one pass (the periodic two-column path).
+//
+// One wave runs two row pipelines over its strip.  Pipeline A is stage 2: it reads U1 and U^n
+// and finishes one row of U2 per row step, rows j0-2 .. j1+1 (the strip's rows and the two halo
+// rows on each side that stage 3 needs).  Pipeline B is stage 3: it takes each U2 row from A in
+// registers, two row steps behind A, reads U^n again (the row A read a few steps earlier, so a
+// cache hit) and stores rows j0 .. j1-1 of U^{n+1}, with the fused CFL.  U2 never goes to
+// memory: per cell and step the two stages read 32 B and write 16 B instead of 64 and 32.
+//
+// A's results are valid on lanes 1..62 (its x stencil reaches one lane each side), so B stores
+// lanes 2..61: STRIP_OUT23 = 120 columns per wave.  A computes the halo cells of U2 (ghost rows
+// and columns across a periodic seam among them) as the waves that own their periodic images
+// do: every cell reconstructed, and, in seam mode 0, a seam face evaluated twice, once as each
+// side's owner sees it (the cell across the seam unreconstructed).  B, which only writes real
+// cells, applies stage_f32_pair_strip's ghost rules.  The results are bitwise those of
+// stage_f32_pair_kernel run as stages 2 and 3.
+// ---------------------------------------------------------------------------
+template <int FLUX, int LIM>
+__device__ __forceinline__ void stage_f32_pair23_strip(const StageArgs& a, const int s, const float* U1,
+                                                       const float* U0, float* O, const int parity) {
+    constexpr bool MUSCL = LIM != LIM_NONE;
+    const int lane = lane_id();
+    const int nx = a.nx, ny = a.ny;
+    const int sxi = s % a.n_strips_x, syi = s / a.n_strips_x;
+    const int c0 = sxi * STRIP_OUT23 - 4 + 2 * lane;    // even: the pair never straddles a periodic seam
+    const int j0 = syi * a.strip_h;
+    const int j1 = min(j0 + a.strip_h, ny);
+    const int a0 = j0 - 2, a1 = j1 + 2;                 // A finishes U2 rows a0 .. a1-1
+    const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0);
+    const float gg = (float)(a.gamma / (a.gamma - 1.0));   // gamma / (gamma - 1): enthalpy per pressure
+    const double dtd = a.fixed_dt > 0.0 ? a.fixed_dt : a.cfl / a.clock->lam[parity];
+    const float dt = (float)dtd;
+    const bool fuse = a.fuse_cfl != 0;
+    const float dtix = dt * (float)a.inv_x, dtiy = dt * (float)a.inv_y;
+    const float cbA = 0.25f, cbB = 2.f / 3.f;          // the stage-2 and stage-3 increments
+    const bool seam_rec = a.seam_mode == 1;
+    int cq0 = c0 % nx;
+    if (cq0 < 0) cq0 += nx;
+    const unsigned ld_off = (unsigned)cq0 * 4u;
+    // A: every column reconstructed; B: a ghost column only across the seam in seam mode 1
+    const float hW = MUSCL ? -1.f : 0.f, hE = MUSCL ? 1.f : 0.f;
+    const bool ga = c0 < 0 || c0 >= nx, gb = c0 + 1 < 0 || c0 + 1 >= nx;
+    const float hWa = MUSCL && (!ga || seam_rec) ? -1.f : 0.f, hEa = MUSCL && (!ga || seam_rec) ? 1.f : 0.f;
+    const float hWb = MUSCL && (!gb || seam_rec) ? -1.f : 0.f, hEb = MUSCL && (!gb || seam_rec) ? 1.f : 0.f;
+    // seam mode 0: lanes whose first column lies on the x seam (its west face is the seam face),
+    // and whether the wave has any (wave-uniform)
+    const bool xseam_lane = !seam_rec && MUSCL && cq0 == 0;
+    const bool xseam_wave = __builtin_amdgcn_readfirstlane((int)(__ballot(xseam_lane) != 0)) != 0;
+    const bool out_lane = lane >= 2 && lane <= WAVE - 3 && c0 < nx;
+    const unsigned st_off = out_lane ? (unsigned)c0 * 4u : 0x80000000u;
+    const unsigned ps = a.plane_stride, set_bytes = 4u * a.plane_stride, rowb = (unsigned)nx * 4u;
+
+    // row byte offset with one periodic wrap (ny >= 8; rows span j0 - 4 .. j1 + 5)
+    auto row_off = [&](int r) -> unsigned { return (unsigned)(r < 0 ? r + ny : (r >= ny ? r - ny : r)) * rowb; };
+    auto ld_pair = [&](const float* set, unsigned roff, unsigned voff, bool live) -> Pair {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)set, 0, live ? (int)set_bytes : 0, BUF_DWORD3);
+        Pair p;
+        float v[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const float2 x = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, roff + (unsigned)m * ps, 0));
+            v[m][0] = x.x;
+            v[m][1] = x.y;
+        }
+        p.a = {v[0][0], v[1][0], v[2][0], v[3][0]};
+        p.b = {v[0][1], v[1][1], v[2][1], v[3][1]};
+        return p;
+    };
+    auto load_row = [&](int r, bool live) -> Pair { return ld_pair(U1, row_off(r), ld_off, live); };
+
+    float lam_max = 0.f;
+    int bad = 0;
+
+    // y reconstruction of one row for both columns (rec: reconstructed, else the cell value)
+    auto y_states = [&](bool rec, const Pair& uc, const Pair& fwd, const Pair& bwd, PrepPair& lo_p, PrepPair& hi_p) {
+        if (MUSCL) {
+            const float hS = rec ? -1.f : 0.f, hN = rec ? 1.f : 0.f;
+            const F4 sa = half_slope4<LIM>(fwd.a, bwd.a), sb = half_slope4<LIM>(fwd.b, bwd.b);
+            lo_p.a = fast::prep(axpy4(uc.a, hS, sa), g, gm1);
+            hi_p.a = fast::prep(axpy4(uc.a, hN, sa), g, gm1);
+            lo_p.b = fast::prep(axpy4(uc.b, hS, sb), g, gm1);
+            hi_p.b = fast::prep(axpy4(uc.b, hN, sb), g, gm1);
+        } else {
+            lo_p.a = fast::prep(uc.a, g, gm1);
+            lo_p.b = fast::prep(uc.b, g, gm1);
+            hi_p = lo_p;
+        }
+    };
+    // the x faces of a row (stage_f32_pair_strip's step) with the reconstruction flags given;
+    // `dual`: A's seam lanes take the seam face west of c0 as their own column's owner sees it
+    // and hand the left lane the face as that lane's owner sees it
+    auto x_faces = [&](bool dual, float hwa, float hea, float hwb, float heb, const Pair& uc,
+                       const PrepPair& lo_p, Pair& xout) {
+        Prep Wa, Ea, Wb, Eb;
+        if (MUSCL) {
+            const F4 uL = from_left4(uc.b), uR = from_right4(uc.a);   // columns c0 - 1 and c0 + 2
+            const F4 d01 = sub4(uc.b, uc.a);
+            const F4 sa = half_slope4<LIM>(d01, sub4(uc.a, uL));
+            const F4 sb = half_slope4<LIM>(sub4(uR, uc.b), d01);
+            Wa = fast::prep(axpy4(uc.a, hwa, sa), g, gm1);
+            Ea = fast::prep(axpy4(uc.a, hea, sa), g, gm1);
+            Wb = fast::prep(axpy4(uc.b, hwb, sb), g, gm1);
+            Eb = fast::prep(axpy4(uc.b, heb, sb), g, gm1);
+        } else {
+            Wa = lo_p.a; Ea = lo_p.a; Wb = lo_p.b; Eb = lo_p.b;
+        }
+        const F4 Fmid = fast::flux<FLUX, 0>(Ea, Wb, gg);
+        const Prep EbL = from_leftP(Eb);
+        F4 Fwa = fast::flux<FLUX, 0>(EbL, Wa, gg);
+        F4 Fgive = Fwa;
+        if (dual) {
+            const F4 Fown = fast::flux<FLUX, 0>(from_leftP(fast::prep(uc.b, g, gm1)), Wa, gg);
+            const F4 Fleft = fast::flux<FLUX, 0>(EbL, fast::prep(uc.a, g, gm1), gg);
+            Fwa = {fast::pick(xseam_lane, Fown.r, Fwa.r), fast::pick(xseam_lane, Fown.ru, Fwa.ru),
+                   fast::pick(xseam_lane, Fown.rv, Fwa.rv), fast::pick(xseam_lane, Fown.re, Fwa.re)};
+            Fgive = {fast::pick(xseam_lane, Fleft.r, Fgive.r), fast::pick(xseam_lane, Fleft.ru, Fgive.ru),
+                     fast::pick(xseam_lane, Fleft.rv, Fgive.rv), fast::pick(xseam_lane, Fleft.re, Fgive.re)};
+        }
+        const F4 Fe = from_right4(Fgive);
+        xout.a = sub4(Fwa, Fmid);
+        xout.b = sub4(Fmid, Fe);
+    };
+    // increment-form RK combine of one column (stage_f32_pair_strip's expressions)
+    auto combine = [&](float cb, const F4& xi, const F4& gi, const F4& go, const F4& uu, const F4& u0) -> F4 {
+        const float rr = xi.r * dtix + (gi.r - go.r) * dtiy;
+        const float rru = xi.ru * dtix + (gi.ru - go.ru) * dtiy;
+        const float rrv = xi.rv * dtix + (gi.rv - go.rv) * dtiy;
+        const float rre = xi.re * dtix + (gi.re - go.re) * dtiy;
+        F4 o;
+        o.r = u0.r + cb * ((uu.r - u0.r) + rr); o.ru = u0.ru + cb * ((uu.ru - u0.ru) + rru);
+        o.rv = u0.rv + cb * ((uu.rv - u0.rv) + rrv); o.re = u0.re + cb * ((uu.re - u0.re) + rre);
+        return o;
+    };
+
+    // A: row step jc (stage 2): the y face jc-1/2 and the x faces of row jc, U2 of row jc-1 into
+    // `res`; U^n of row jc for the next step; U1 of row jc+3 into the slot of row jc-1.  At a y
+    // seam in seam mode 0 the face below row jc is evaluated twice: row jc-1 is finished with the
+    // face its owner sees (row jc unreconstructed), and row jc carries the other one.
+    auto stepA = [&](int jc, Pair& um, const Pair& uc, const Pair& up, Pair& din, Pair& dout, PrepPair& hin,
+                     PrepPair& hout, Pair& gin, Pair& gout, Pair& xin, Pair& xout, const Pair& u0in, Pair& u0out,
+                     Pair& res) {
+        const bool full = jc < a1;
+        u0out = ld_pair(U0, row_off(jc), ld_off, full);
+        dout = sub_pair(up, uc);
+        PrepPair lo_p;
+        y_states(true, uc, dout, din, lo_p, hout);
+        Pair gnow;
+        if (!seam_rec && MUSCL && jc % ny == 0) {
+            const Prep ua = fast::prep(uc.a, g, gm1), ub = fast::prep(uc.b, g, gm1);
+            gnow.a = fast::flux<FLUX, 1>(hin.a, ua, gg);
+            gnow.b = fast::flux<FLUX, 1>(hin.b, ub, gg);
+            gout.a = fast::flux<FLUX, 1>(fast::prep(um.a, g, gm1), lo_p.a, gg);
+            gout.b = fast::flux<FLUX, 1>(fast::prep(um.b, g, gm1), lo_p.b, gg);
+        } else {
+            gout.a = fast::flux<FLUX, 1>(hin.a, lo_p.a, gg);
+            gout.b = fast::flux<FLUX, 1>(hin.b, lo_p.b, gg);
+            gnow = gout;
+        }
+        if (full) x_faces(xseam_wave, hW, hE, hW, hE, uc, lo_p, xout);
+        res.a = combine(cbA, xin.a, gin.a, gnow.a, um.a, u0in.a);
+        res.b = combine(cbA, xin.b, gin.b, gnow.b, um.b, u0in.b);
+        um = load_row(jc + 3, full && jc + 3 <= a1 + 1);
+    };
+    // B: row step jb (stage 3) on U2 rows jb-1 (um), jb (uc), jb+1 (up, A's result of this step),
+    // with stage_f32_pair_strip's ghost rules; steps j0-2 and j0-1 only build the carried
+    // difference and north state
+    auto stepB = [&](int jb, const Pair& um, const Pair& uc, const Pair& up, Pair& din, Pair& dout, PrepPair& hin,
+                     PrepPair& hout, Pair& gin, Pair& gout, Pair& xin, Pair& xout, const Pair& u0in, Pair& u0out) {
+        const bool full = jb >= j0 && jb < j1;
+        const bool out = jb > j0 && jb <= j1;
+        u0out = ld_pair(U0, (unsigned)max(jb, 0) * rowb, st_off, full);
+        dout = sub_pair(up, uc);
+        PrepPair lo_p;
+        y_states(!(jb < 0 || jb >= ny) || seam_rec, uc, dout, din, lo_p, hout);
+        if (jb >= j0) {
+            gout.a = fast::flux<FLUX, 1>(hin.a, lo_p.a, gg);
+            gout.b = fast::flux<FLUX, 1>(hin.b, lo_p.b, gg);
+        }
+        if (full) x_faces(false, hWa, hEa, hWb, hEb, uc, lo_p, xout);
+        const bool live = out && out_lane;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)O, 0, out ? (int)set_bytes : 0, BUF_DWORD3);
+        const unsigned roff = (unsigned)max(jb - 1, 0) * rowb;
+        const F4 oa = combine(cbB, xin.a, gin.a, gout.a, um.a, u0in.a);
+        const F4 ob = combine(cbB, xin.b, gin.b, gout.b, um.b, u0in.b);
+        if (fuse && live) {
+            float l;
+            if (lam_cell_f32<false>(a, oa.r, oa.ru, oa.rv, oa.re, l)) lam_max = fmaxf(lam_max, l);
+            else bad++;
+            if (lam_cell_f32<false>(a, ob.r, ob.ru, ob.rv, ob.re, l)) lam_max = fmaxf(lam_max, l);
+            else bad++;
+        }
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        const float o[4][2] = {{oa.r, ob.r}, {oa.ru, ob.ru}, {oa.rv, ob.rv}, {oa.re, ob.re}};
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const v2i w = {__builtin_bit_cast(int, o[m][0]), __builtin_bit_cast(int, o[m][1])};
+            __builtin_amdgcn_raw_buffer_store_b64(w, rs, st_off, roff + (unsigned)m * ps, 0);
+        }
+    };
+
+    // A's prologue: U1 rows a0-2 .. a0+2, the north state of row a0-1 (reconstructed: A's rows all are)
+    Pair R0 = load_row(a0 - 1, true), R1 = load_row(a0, true), R2 = load_row(a0 + 1, true), R3 = load_row(a0 + 2, true);
+    Pair D0, D1;
+    PrepPair H0, H1;
+    const F4 z4{0.f, 0.f, 0.f, 0.f};
+    Pair G0{z4, z4}, G1{z4, z4}, X0{z4, z4}, X1{z4, z4}, Q0{z4, z4}, Q1{z4, z4};
+    {
+        const Pair rm2 = load_row(a0 - 2, true);
+        D0 = sub_pair(R1, R0);
+        PrepPair lo_unused;
+        y_states(true, R0, D0, sub_pair(R0, rm2), lo_unused, H0);
+    }
+    // B's registers: U2 rows in a window of four, filled by A
+    Pair S0{z4, z4}, S1{z4, z4}, S2, S3;
+    Pair E0{z4, z4}, E1;
+    PrepPair K0, K1;
+    Pair GB0{z4, z4}, GB1{z4, z4}, XB0{z4, z4}, XB1{z4, z4}, QB0{z4, z4}, QB1{z4, z4};
+    K0 = H0;   // defined values for the first warm-up step (whose y flux is skipped)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    // A alone: rows a0 (faces only) and a0+1 (finishes U2 row a0 = j0-2)
+    stepA(a0, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, Q0, Q1, S0);
+    stepA(a0 + 1, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, Q1, Q0, S1);
+    for (int t = j0; t <= a1; t += 4) {
+        stepA(t, R2, R3, R0, D0, D1, H0, H1, G0, G1, X0, X1, Q0, Q1, S2);
+        stepB(t - 2, S0, S1, S2, E0, E1, K0, K1, GB0, GB1, XB0, XB1, QB0, QB1);
+        stepA(t + 1, R3, R0, R1, D1, D0, H1, H0, G1, G0, X1, X0, Q1, Q0, S3);
+        stepB(t - 1, S1, S2, S3, E1, E0, K1, K0, GB1, GB0, XB1, XB0, QB1, QB0);
+        stepA(t + 2, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, Q0, Q1, S0);
+        stepB(t, S2, S3, S0, E0, E1, K0, K1, GB0, GB1, XB0, XB1, QB0, QB1);
+        stepA(t + 3, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, Q1, Q0, S1);
+        stepB(t + 1, S3, S0, S1, E1, E0, K1, K0, GB1, GB0, XB1, XB0, QB1, QB0);
+    }
+
+    if (fuse) {
+        const double w = wave_max((double)lam_max);
+        const long long b = wave_sum_ll((long long)bad);
+        if (lane == 0) {
+            if (a.lam_part) a.lam_part[s] = w;
+            else if (w > 0.0) atomic_max_pos_double(&a.clock->lam[parity ^ 1], w);
+            if (b) atomicAdd((unsigned long long*)&a.clock->bad, (unsigned long long)b);
+        }
+    }
+    if (s == 0 && lane == 0) {
+        a.clock->time += dtd;
+        a.clock->step += 1;
+    }
+}
+
+template <int FLUX, int LIM>
+__global__ __launch_bounds__(256) void stage_f32_pair23_kernel(StageArgs a) {
+    const int s = strip_index(a, blockDim.x / WAVE);
+    if (s >= a.n_strips_x * a.n_strips_y) return;
+    stage_f32_pair23_strip<FLUX, LIM>(a, s, (const float*)a.uin[0], (const float*)a.u0[0], (float*)a.uout[0],
+                                      a.parity);
+}
+
+// ---------------------------------------------------------------------------
 // Stage from precomputed face states (learned reconstruction, periodic grids).
 // F holds 16 planes of primitives, plane (axis * 2 + side) * 4 + var: the left /
 // right states of the east (axis 0) and north (axis 1) face of every cell, as the
@@ -697,6 +956,7 @@ __global__ __launch_bounds__(256) void stage_f32_faces_kernel(StageArgs a, const
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int stage = a.stage, parity = a.parity;
     const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0), igm1 = (float)(1.0 / (a.gamma - 1.0));
+    const float gg = (float)(a.gamma / (a.gamma - 1.0));
     const double dtd = a.fixed_dt > 0.0 ? a.fixed_dt : a.cfl / a.clock->lam[parity];
     const float dt = (float)dtd;
     const bool fuse = a.fuse_cfl && (a.euler || stage == 2);
@@ -716,10 +976,10 @@ __global__ __launch_bounds__(256) void stage_f32_faces_kernel(StageArgs a, const
             const float* f = F + plane * np + c;
             return fast::prep_prim(f[0], f[np], f[2 * np], f[3 * np], g, igm1);
         };
-        const F4 Fe = fast::flux<FLUX, 0>(state(0, q), state(4, q));
-        const F4 Fw = fast::flux<FLUX, 0>(state(0, qw), state(4, qw));
-        const F4 Gn = fast::flux<FLUX, 1>(state(8, q), state(12, q));
-        const F4 Gs = fast::flux<FLUX, 1>(state(8, qs), state(12, qs));
+        const F4 Fe = fast::flux<FLUX, 0>(state(0, q), state(4, q), gg);
+        const F4 Fw = fast::flux<FLUX, 0>(state(0, qw), state(4, qw), gg);
+        const F4 Gn = fast::flux<FLUX, 1>(state(8, q), state(12, q), gg);
+        const F4 Gs = fast::flux<FLUX, 1>(state(8, qs), state(12, qs), gg);
         const float dtix = dt * (float)a.inv_x, dtiy = dt * (float)a.inv_y;
         const float d[4] = {(Fw.r - Fe.r) * dtix + (Gs.r - Gn.r) * dtiy, (Fw.ru - Fe.ru) * dtix + (Gs.ru - Gn.ru) * dtiy,
                             (Fw.rv - Fe.rv) * dtix + (Gs.rv - Gn.rv) * dtiy, (Fw.re - Fe.re) * dtix + (Gs.re - Gn.re) * dtiy};
@@ -867,6 +1127,26 @@ hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st) {
 #define EFVM_LAUNCH_ONE(F, L, X) stage_f32_kernel<F, L, X><<<grid, block, 0, st>>>(a)
     EFVM_F32_DISPATCH(EFVM_LAUNCH_ONE);
 #undef EFVM_LAUNCH_ONE
+    return hipGetLastError();
+}
+
+// Fused stages 2 and 3 (stage_f32_pair23_kernel): a.uin = U1, a.u0 = U^n, a.uout = U^{n+1};
+// a.n_strips_x counts strips of STRIP_OUT23 columns.
+hipError_t launch_stage_f32_pair23(const StageArgs& a, int block, hipStream_t st) {
+    const int waves = block / WAVE;
+    const int nstrips = a.n_strips_x * a.n_strips_y;
+    const dim3 grid((nstrips + waves - 1) / waves);
+    const int lim = a.muscl ? a.limiter : LIM_NONE;
+#define EFVM_P23(F)                                                                                   \
+    do {                                                                                              \
+        if (lim == LIM_MC) stage_f32_pair23_kernel<F, LIM_MC><<<grid, block, 0, st>>>(a);             \
+        else if (lim == LIM_MINMOD) stage_f32_pair23_kernel<F, LIM_MINMOD><<<grid, block, 0, st>>>(a); \
+        else stage_f32_pair23_kernel<F, LIM_NONE><<<grid, block, 0, st>>>(a);                         \
+    } while (0)
+    if (a.flux == FLUX_RUSANOV) EFVM_P23(FLUX_RUSANOV);
+    else if (a.flux == FLUX_HLLC) EFVM_P23(FLUX_HLLC);
+    else EFVM_P23(FLUX_HLLC_TEXTBOOK);
+#undef EFVM_P23
     return hipGetLastError();
 }
 

@@ -24,7 +24,7 @@ BC = {"outflow": 0, "periodic": 1, "slip": 2, "noslip": 3, "symmetry": 4,
 
 # Every entry point include/efvm.h declares (checked by the CPU test suite).
 EXPORTS = [
-    "efvm_ctx_create", "efvm_ctx_destroy", "efvm_last_error", "efvm_set_launch", "efvm_get_launch", "efvm_set_pair_kernel", "efvm_device_arch", "efvm_set_boundaries",
+    "efvm_ctx_create", "efvm_ctx_destroy", "efvm_last_error", "efvm_set_launch", "efvm_get_launch", "efvm_set_pair_kernel", "efvm_set_fused_step", "efvm_get_fused_step", "efvm_device_arch", "efvm_set_boundaries",
     "efvm_set_parabolic_inflow", "efvm_set_sponge", "efvm_set_body_force", "efvm_set_time_step", "efvm_upload_state",
     "efvm_download_state", "efvm_get_clock", "efvm_set_clock", "efvm_cfl", "efvm_run", "efvm_step",
     "efvm_advance_to", "efvm_synchronize", "efvm_graph_capture", "efvm_graph_launch",
@@ -122,6 +122,8 @@ def load_library(path: str = LIB_PATH):
         "efvm_set_launch": [vp, C.c_int, C.c_int],
         "efvm_get_launch": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "efvm_set_pair_kernel": [vp, C.c_int],
+        "efvm_set_fused_step": [vp, C.c_int, C.c_int, C.c_int],
+        "efvm_get_fused_step": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "efvm_set_boundaries": [vp, C.POINTER(Bc)],
         "efvm_set_parabolic_inflow": [vp, d, d, d],
         "efvm_set_sponge": [vp, d, d, d, d, d, d, d],
@@ -221,6 +223,18 @@ class Context:
     def set_pair_kernel(self, allow: bool):
         """Allow (default) or forbid the two-column periodic strip kernel."""
         self._chk(self._L.efvm_set_pair_kernel(self._p, int(bool(allow))))
+
+    def set_fused_step(self, allow: bool = True, strip_rows: int = 0, waves_per_block: int = 0):
+        """Allow (default) or forbid running RK3 stages 2 and 3 of whole steps as one fused launch,
+        and set its launch shape (0 keeps the current value)."""
+        self._chk(self._L.efvm_set_fused_step(self._p, int(bool(allow)), strip_rows, waves_per_block))
+
+    def fused_step(self):
+        """(allowed, strip_rows, waves_per_block, fits): fits = the configuration admits the fused
+        stage-2+3 launch (FP32 RK3, periodic two-column path, ny >= 8)."""
+        a, r, w, f = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._chk(self._L.efvm_get_fused_step(self._p, C.byref(a), C.byref(r), C.byref(w), C.byref(f)))
+        return bool(a.value), r.value, w.value, bool(f.value)
 
     def set_boundaries(self, sides):
         """sides: 4 x (kind, params) for XLo, XHi, YLo, YHi."""

@@ -644,6 +644,71 @@ def test_launch_shapes_give_identical_results():
         assert np.array_equal(s, ref), (rows, waves)
 
 
+# grids for the fused stage-2+3 launch: strips holding the x and y seams, a strip holding several
+# periodic images of each column (nx < 120), ragged last strips, ny at the minimum of 8
+FUSED_GRIDS = [(130, 64, "hllc", "mc"), (40, 12, "hllc", "mc"), (250, 138, "hllc", "minmod"),
+               (64, 64, "rusanov", "mc"), (96, 40, "hllc", None), (8, 8, "hllc", "mc"), (246, 9, "hllc_textbook", "mc")]
+
+
+@pytest.mark.parametrize("seam_mode", [0, 1])
+def test_fused_step_is_bitwise_the_substage_launches(seam_mode):
+    """Whole FP32 RK3 steps on the periodic two-column path run stage 1 and one fused stage-2+3
+    launch (U2 kept in registers).  State and clock must be bitwise those of the three substage
+    launches (fusion off), at several fused strip heights, in both seam modes (in seam mode 0 the
+    fused launch evaluates each seam face as each side's owner sees it)."""
+    for nx, ny, flux, lim in FUSED_GRIDS:
+        st = cases.vortex_modes_state(nx, ny, seed=3, L=10.0)
+        ref = None
+        for fused, rows in [(False, 0), (True, 73), (True, 5), (True, 9), (True, 1)]:
+            c = efvm.Context(nx, ny, hi=(10.0, 10.0), dz=10.0 / nx, flux=flux, muscl=lim is not None,
+                             limiter=lim or "minmod", precision="fp32", seam_mode=seam_mode)
+            c.set_boundaries(["periodic"] * 4)
+            c.set_fused_step(fused, rows, 0)
+            assert c.fused_step()[3], (nx, ny)
+            c.upload(st)
+            c.step(7)
+            out = (c.download(), c.clock())
+            if ref is None:
+                ref = out
+                continue
+            assert np.array_equal(out[0], ref[0]), (nx, ny, flux, lim, rows)
+            assert out[1] == ref[1], (nx, ny, flux, lim, rows)
+
+
+def test_fused_step_graph_replay_and_mid_step_entry():
+    """Graph replay of fused steps equals eager fused steps (several replays with eager work
+    between); a step entered mid-step (after one run() substage) runs the separate launches and
+    lands on the same state as whole steps."""
+    n = 136
+    st = cases.vortex_modes_state(n, n, seed=5)
+
+    def ctx():
+        c = efvm.Context(n, n, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc", precision="fp32")
+        c.set_boundaries(["periodic"] * 4)
+        c.upload(st)
+        return c
+
+    a = ctx()
+    a.step(12)
+    b = ctx()
+    b.graph_capture(4)
+    b.graph_launch()
+    b.step(2)
+    b.graph_launch()
+    b.step(2)
+    assert np.array_equal(a.download(), b.download())
+    assert a.clock() == b.clock()
+    c = ctx()
+    c.step(3)
+    c.run(); c.run(); c.run()     # step 4 as three run() substages
+    c.run()                       # stage 1 of step 5
+    c.step(1)                     # from mid-step: stages 2, 3 of step 5 and stage 1 of step 6
+    c.run(); c.run()              # the rest of step 6
+    c.step(6)                     # whole (fused) steps 7 .. 12
+    assert np.array_equal(a.download(), c.download())
+    assert a.clock() == c.clock()
+
+
 def test_error_paths():
     c = efvm.Context(16, 16)
     with pytest.raises(efvm.EfvmError) as e:
