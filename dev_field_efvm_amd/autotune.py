@@ -1,6 +1,6 @@
 """Launch-shape autotuner for the stage kernels.
 
-The stage kernels are wave strips (64 columns x `strip_rows` rows per wave);
+The stage kernels are wave strips (64 or 128 columns x `strip_rows` rows per wave);
 the shape trades halo re-reads (4 extra rows per strip) against parallelism
 (strips per CU) and the tail of the last wave round.  The FP32 kernel runs a
 strip's strip_rows + 1 row steps in groups of four, so the candidate heights
@@ -22,6 +22,9 @@ CACHE_PATH = os.environ.get("EFVM_TUNE_CACHE", os.path.join(HERE, "tuned.json"))
 
 CTX_ROWS = (7, 15, 23, 27, 31, 39, 43, 47, 55, 63, 127, 255)
 CTX_WAVES = (1, 2, 4)
+# strip heights of the fused stage-2+3 launch (a strip of h rows runs h + 5 row steps of stage 2
+# and h + 3 of stage 3, the loop in groups of four: h = 4k + 1 leaves no idle step)
+FUSED_ROWS = (13, 21, 25, 29, 33, 37, 41, 45, 49, 57, 65, 73, 97)
 BATCH_ROWS = (7, 15, 31, 63, 127)
 BATCH_BLOCKS = (0, 2048, 4096, 8192)
 
@@ -58,12 +61,15 @@ def _time_ctx(ctx, steps):
     return ctx.last_elapsed_ms() / steps
 
 
-def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=True, verbose=False, store=True):
+def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=True, verbose=False, store=True,
+                 fused_rows=FUSED_ROWS):
     """Pick (strip_rows, waves_per_block) and, where both apply, the one- or two-column strip
-    kernel for a context; returns (rows, waves, ms_per_step).  The two-column kernel halves the
-    lane exchanges per cell but covers 124 columns per wave: at sizes just above a multiple of
-    124 (512, 2048) its last strip is nearly empty, and the one-column kernel can win."""
+    kernel for a context, then, where it applies, the fused stage-2+3 launch's strip height (or
+    no fusion); returns (rows, waves, ms_per_step).  The two-column kernel halves the lane
+    exchanges per cell but covers 124 columns per wave: at sizes just above a multiple of 124
+    (512, 2048) its last strip is nearly empty, and the one-column kernel can win."""
     fits = ctx.launch()[3]          # 0: one-column only; 1: the periodic two-column form; 2: the general one
+    fused_fits = ctx.fused_step()[2]
     key = f"{arch_name()}:{ctx.precision}:ctx:{size_class(ctx.nx)}x{size_class(ctx.ny)}:" + \
           ("general", "periodic-plain", "walls-plain")[fits]
     db = _load()
@@ -71,8 +77,12 @@ def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=
         e = db[key]
         ctx.set_pair_kernel(e.get("pair", 1))
         ctx.set_launch(e["rows"], e["waves"])
+        if fused_fits:
+            ctx.set_fused_step(e.get("fused", 0) > 0, e.get("fused", 0))
         return e["rows"], e["waves"], e["ms"]
     ctx.snapshot()
+    if fused_fits:
+        ctx.set_fused_step(False)
     best = None
     for pair in ((1, 0) if fits else (1,)):
         ctx.set_pair_kernel(pair)
@@ -87,8 +97,21 @@ def tune_context(ctx, steps: int = 3, rows=CTX_ROWS, waves=CTX_WAVES, use_cache=
                     best = (r, w, ms, pair)
     ctx.set_pair_kernel(best[3])
     ctx.set_launch(best[0], best[1])
+    fused = 0
+    if fused_fits:
+        # stage 1 keeps the shape just chosen; the fused launch's strip height, or none
+        for r in fused_rows:
+            ctx.set_fused_step(True, r)
+            ms = _time_ctx(ctx, steps)
+            ctx.snapshot(restore=True)
+            if verbose:
+                print(f"  fused rows={r:4d}  {ms:.4f} ms/step", flush=True)
+            if ms < best[2]:
+                best, fused = (best[0], best[1], ms, best[3]), r
+        ctx.set_fused_step(fused > 0, fused)
     if store:
-        db[key] = {"rows": best[0], "waves": best[1], "ms": best[2], "pair": best[3], "when": time.time()}
+        db[key] = {"rows": best[0], "waves": best[1], "ms": best[2], "pair": best[3], "fused": fused,
+                   "when": time.time()}
         _store(db)
     return best[:3]
 

@@ -4,7 +4,11 @@ The FP64 parity translation unit is compiled with ``-ffp-contract=off`` (the
 reference never fuses multiply-adds); the FP32 production unit fuses a
 multiply-add only inside one source expression (``-ffp-contract=on``), which the
 front end decides once per expression: the four compiled copies of the row step
-then round alike, and results do not depend on the strip height.  All HIP code is built with packed-FP32 instructions disabled.
+then round alike, and results do not depend on the strip height.  The FP32 unit also runs
+with the IEEE mode bit clear and no NaN operands assumed (``-fno-honor-nans
+-mno-amdgpu-ieee``): min/max then need no operand canonicalisation, and its one NaN test
+(the CFL's non-physical-cell check) is written on bit patterns.  All HIP code is built with
+packed-FP32 instructions disabled.
 """
 from __future__ import annotations
 
@@ -25,7 +29,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-x", "hip", f"--offload-arch={ARCH}",
           "-Wno-unused-result", "-Wno-unused-command-line-argument"]
 UNITS = [
     ("efvm_stage_f64.hip", ["-ffp-contract=off"]),
-    ("efvm_stage_f32.hip", ["-ffp-contract=on", "-fgpu-flush-denormals-to-zero"]),
+    ("efvm_stage_f32.hip", ["-ffp-contract=on", "-fgpu-flush-denormals-to-zero", "-fno-honor-nans", "-mno-amdgpu-ieee"]),
     ("efvm_mlp.hip", ["-ffp-contract=fast"]),
     ("efvm_stage3d.hip", ["-ffp-contract=off"]),
     ("efvm_api.cpp", ["-ffp-contract=off"]),

@@ -25,7 +25,7 @@
 namespace efvm {
 hipError_t launch_stage_f64(const StageArgs& a, int block, hipStream_t st);
 hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st);
-hipError_t launch_stage_f32_pair23(const StageArgs& a, int block, hipStream_t st);
+hipError_t launch_stage_f32_pair23(const StageArgs& a, hipStream_t st);
 template <class T> hipError_t launch_cfl(const StageArgs& a, const void* const* planes, double* target, hipStream_t st);
 hipError_t launch_clock_set(DevClock* clk, double dt, double time, long long step, hipStream_t st);
 struct IbmArgs {
@@ -102,7 +102,6 @@ struct efvm_ctx {
     int allow_pair = 1;            // FP32 periodic grids may use the two-column strip kernel
     int allow_fuse23 = 1;          // ... and run RK3 stages 2 and 3 as one fused launch (seam mode 1)
     int strip_h23 = 73;            // rows per strip of the fused stage-2+3 kernel
-    int waves23 = 4;               // its waves per block
     int xcd_remap = 1;
     // consecutive stage launches sweep the strips in opposite directions, so each launch starts
     // where the previous one ended, on rows whose lines the device's last-level cache still holds
@@ -376,7 +375,7 @@ efvm_status run_step_fused(efvm_ctx* c) {
     b.reverse = (int)(c->dir_count++ & 1u);
     set_planes(b, Cc, A, B);             // stages 2 and 3: U1, U^n -> U^{n+1} in prev
     b.fuse_cfl = !c->sponge && c->sch.fixed_dt <= 0.0;
-    hipError_t e = launch_stage_f32_pair23(b, WAVE * c->waves23, c->stream);
+    hipError_t e = launch_stage_f32_pair23(b, c->stream);
     if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("fused stage kernel launch: ") + hipGetErrorString(e));
     if (b.fuse_cfl && b.lam_part) {
         e = launch_lam_reduce(b.lam_part, b.n_strips_x * b.n_strips_y, &c->clock->lam[b.parity ^ 1], c->stream);
@@ -524,22 +523,19 @@ efvm_status efvm_get_launch(const efvm_ctx* c, int* strip_rows, int* waves_per_b
     return EFVM_OK;
 }
 
-efvm_status efvm_set_fused_step(efvm_ctx* c, int allow, int strip_rows, int waves_per_block) {
+efvm_status efvm_set_fused_step(efvm_ctx* c, int allow, int strip_rows) {
     if (!c) return EFVM_ERR_ARG;
-    if (strip_rows < 0 || strip_rows > 4096 || waves_per_block < 0 || waves_per_block > 4)
-        return fail(c, EFVM_ERR_ARG, "fused launch shape out of range");
+    if (strip_rows < 0 || strip_rows > 4096) return fail(c, EFVM_ERR_ARG, "fused strip height out of range");
     c->allow_fuse23 = allow ? 1 : 0;
     if (strip_rows) c->strip_h23 = strip_rows;
-    if (waves_per_block) c->waves23 = waves_per_block;
     drop_graph(c);
     return EFVM_OK;
 }
 
-efvm_status efvm_get_fused_step(const efvm_ctx* c, int* allow, int* strip_rows, int* waves_per_block, int* fits) {
+efvm_status efvm_get_fused_step(const efvm_ctx* c, int* allow, int* strip_rows, int* fits) {
     if (!c) return EFVM_ERR_ARG;
     if (allow) *allow = c->allow_fuse23;
     if (strip_rows) *strip_rows = c->strip_h23;
-    if (waves_per_block) *waves_per_block = c->waves23;
     if (fits) *fits = fused_path_fits(c, base_args(c)) ? 1 : 0;
     return EFVM_OK;
 }

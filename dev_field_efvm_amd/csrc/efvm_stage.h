@@ -121,7 +121,7 @@ template <class T> __device__ __forceinline__ T shfl_dn1(T v) { return __shfl_do
 // Wave index with an XCD-aware remap: the hardware deals consecutive workgroups
 // round-robin over the 8 XCDs; remapping gives each XCD a contiguous range of
 // strips, so strips sharing halo rows/columns share one L2.
-__device__ __forceinline__ int strip_index(const StageArgs& a, int waves_per_block) {
+__device__ __forceinline__ int block_index(const StageArgs& a) {
     int nb = gridDim.x;
     int b = blockIdx.x;
     if (a.xcd_remap && nb >= 8) {
@@ -132,8 +132,11 @@ __device__ __forceinline__ int strip_index(const StageArgs& a, int waves_per_blo
     } else if (a.reverse) {
         b = nb - 1 - b;
     }
+    return b;
+}
+__device__ __forceinline__ int strip_index(const StageArgs& a, int waves_per_block) {
     // wave-uniform: keeps the strip's rows, and every per-row decision, in scalar registers
-    return __builtin_amdgcn_readfirstlane(b * waves_per_block + (int)(threadIdx.x / WAVE));
+    return __builtin_amdgcn_readfirstlane(block_index(a) * waves_per_block + (int)(threadIdx.x / WAVE));
 }
 
 // Plane element at a wave-uniform base plus a per-lane byte offset (32-bit): the

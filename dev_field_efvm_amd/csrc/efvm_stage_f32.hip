@@ -129,7 +129,12 @@ __device__ __forceinline__ bool lam_cell_f32(const StageArgs& a, float o0, float
         const float mu = a.visc_const ? (float)a.mu : (float)ref::sutherland((double)t);
         l = fmaf(2.f * mu * ir, vsum, l);
     }
-    return o0 > 0.f && l == l && p > 0.f;
+    // positive density and pressure and a finite lambda, tested on the bit patterns (the unit is
+    // compiled assuming no NaNs, so a float compare could be folded): a positive float is a
+    // positive integer, and a finite non-negative lambda lies below the pattern of +inf (NaNs
+    // and negative values compare above it as unsigned integers)
+    return (int)__builtin_bit_cast(unsigned, o0) > 0 && (int)__builtin_bit_cast(unsigned, p) > 0 &&
+           __builtin_bit_cast(unsigned, l) < 0x7f800000u;
 }
 
 struct Prim3 { float u, v, t; };
@@ -687,12 +692,14 @@ __global__ __launch_bounds__(256) void stage_f32_pair_kernel(StageArgs a) {
 // ---------------------------------------------------------------------------
 // Stages 2 and 3 of SSP-RK3 in one pass (the periodic two-column path).
 //
-// One wave runs two row pipelines over its strip.  Pipeline A is stage 2: it reads U1 and U^n
-// and finishes one row of U2 per row step, rows j0-2 .. j1+1 (the strip's rows and the two halo
-// rows on each side that stage 3 needs).  Pipeline B is stage 3: it takes each U2 row from A in
-// registers, two row steps behind A, reads U^n again (the row A read a few steps earlier, so a
-// cache hit) and stores rows j0 .. j1-1 of U^{n+1}, with the fused CFL.  U2 never goes to
-// memory: per cell and step the two stages read 32 B and write 16 B instead of 64 and 32.
+// A workgroup of two waves owns a strip.  Wave A (role 0) is stage 2: it reads U1 and U^n and
+// finishes one row of U2 per row step, rows j0-2 .. j1+1 (the strip's rows and the two halo rows
+// on each side that stage 3 needs), and puts each row in a two-slot LDS ring.  Wave B (role 1)
+// is stage 3: after each row step's barrier it takes the row A just finished, two row steps
+// behind A, reads U^n again (the row A read a few steps earlier: a cache hit) and stores rows
+// j0 .. j1-1 of U^{n+1}, with the fused CFL.  U2 never goes to memory: per cell and step the two
+// stages read 32 B and write 16 B instead of 64 and 32, and each wave carries one pipeline's
+// registers, so three waves fit on a SIMD.
 //
 // A's results are valid on lanes 1..62 (its x stencil reaches one lane each side), so B stores
 // lanes 2..61: STRIP_OUT23 = 120 columns per wave.  A computes the halo cells of U2 (ghost rows
@@ -702,9 +709,12 @@ __global__ __launch_bounds__(256) void stage_f32_pair_kernel(StageArgs a) {
 // cells, applies stage_f32_pair_strip's ghost rules.  The results are bitwise those of
 // stage_f32_pair_kernel run as stages 2 and 3.
 // ---------------------------------------------------------------------------
-template <int FLUX, int LIM>
-__device__ __forceinline__ void stage_f32_pair23_strip(const StageArgs& a, const int s, const float* U1,
-                                                       const float* U0, float* O, const int parity) {
+// SEAMS: seam mode 0 and the strip holds a periodic seam (an x seam lane, or the first or last
+// row band): the seam faces are evaluated twice and B applies the ghost rules.  Elsewhere every
+// cell the wave touches is reconstructed.  FUSE: the CFL epilogue.
+template <int FLUX, int LIM, bool SEAMS, bool FUSE>
+__device__ __forceinline__ void stage_f32_pair23_strip(const StageArgs& a, const int s, const int role, float2* ring,
+                                                       const float* U1, const float* U0, float* O, const int parity) {
     constexpr bool MUSCL = LIM != LIM_NONE;
     const int lane = lane_id();
     const int nx = a.nx, ny = a.ny;
@@ -717,7 +727,6 @@ __device__ __forceinline__ void stage_f32_pair23_strip(const StageArgs& a, const
     const float gg = (float)(a.gamma / (a.gamma - 1.0));   // gamma / (gamma - 1): enthalpy per pressure
     const double dtd = a.fixed_dt > 0.0 ? a.fixed_dt : a.cfl / a.clock->lam[parity];
     const float dt = (float)dtd;
-    const bool fuse = a.fuse_cfl != 0;
     const float dtix = dt * (float)a.inv_x, dtiy = dt * (float)a.inv_y;
     const float cbA = 0.25f, cbB = 2.f / 3.f;          // the stage-2 and stage-3 increments
     const bool seam_rec = a.seam_mode == 1;
@@ -731,8 +740,8 @@ __device__ __forceinline__ void stage_f32_pair23_strip(const StageArgs& a, const
     const float hWb = MUSCL && (!gb || seam_rec) ? -1.f : 0.f, hEb = MUSCL && (!gb || seam_rec) ? 1.f : 0.f;
     // seam mode 0: lanes whose first column lies on the x seam (its west face is the seam face),
     // and whether the wave has any (wave-uniform)
-    const bool xseam_lane = !seam_rec && MUSCL && cq0 == 0;
-    const bool xseam_wave = __builtin_amdgcn_readfirstlane((int)(__ballot(xseam_lane) != 0)) != 0;
+    const bool xseam_lane = SEAMS && !seam_rec && MUSCL && cq0 == 0;
+    const bool xseam_wave = SEAMS && __builtin_amdgcn_readfirstlane((int)(__ballot(xseam_lane) != 0)) != 0;
     const bool out_lane = lane >= 2 && lane <= WAVE - 3 && c0 < nx;
     const unsigned st_off = out_lane ? (unsigned)c0 * 4u : 0x80000000u;
     const unsigned ps = a.plane_stride, set_bytes = 4u * a.plane_stride, rowb = (unsigned)nx * 4u;
@@ -832,7 +841,7 @@ __device__ __forceinline__ void stage_f32_pair23_strip(const StageArgs& a, const
         PrepPair lo_p;
         y_states(true, uc, dout, din, lo_p, hout);
         Pair gnow;
-        if (!seam_rec && MUSCL && jc % ny == 0) {
+        if (SEAMS && !seam_rec && MUSCL && jc % ny == 0) {
             const Prep ua = fast::prep(uc.a, g, gm1), ub = fast::prep(uc.b, g, gm1);
             gnow.a = fast::flux<FLUX, 1>(hin.a, ua, gg);
             gnow.b = fast::flux<FLUX, 1>(hin.b, ub, gg);
@@ -858,23 +867,27 @@ __device__ __forceinline__ void stage_f32_pair23_strip(const StageArgs& a, const
         u0out = ld_pair(U0, (unsigned)max(jb, 0) * rowb, st_off, full);
         dout = sub_pair(up, uc);
         PrepPair lo_p;
-        y_states(!(jb < 0 || jb >= ny) || seam_rec, uc, dout, din, lo_p, hout);
+        y_states(!SEAMS || !(jb < 0 || jb >= ny) || seam_rec, uc, dout, din, lo_p, hout);
         if (jb >= j0) {
             gout.a = fast::flux<FLUX, 1>(hin.a, lo_p.a, gg);
             gout.b = fast::flux<FLUX, 1>(hin.b, lo_p.b, gg);
         }
-        if (full) x_faces(false, hWa, hEa, hWb, hEb, uc, lo_p, xout);
+        if (full) {
+            if (SEAMS) x_faces(false, hWa, hEa, hWb, hEb, uc, lo_p, xout);
+            else x_faces(false, hW, hE, hW, hE, uc, lo_p, xout);
+        }
         const bool live = out && out_lane;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)O, 0, out ? (int)set_bytes : 0, BUF_DWORD3);
         const unsigned roff = (unsigned)max(jb - 1, 0) * rowb;
         const F4 oa = combine(cbB, xin.a, gin.a, gout.a, um.a, u0in.a);
         const F4 ob = combine(cbB, xin.b, gin.b, gout.b, um.b, u0in.b);
-        if (fuse && live) {
-            float l;
-            if (lam_cell_f32<false>(a, oa.r, oa.ru, oa.rv, oa.re, l)) lam_max = fmaxf(lam_max, l);
-            else bad++;
-            if (lam_cell_f32<false>(a, ob.r, ob.ru, ob.rv, ob.re, l)) lam_max = fmaxf(lam_max, l);
-            else bad++;
+        if (FUSE) {
+            // every lane evaluates; selects keep the lanes and rows that store nothing out
+            float la, lb;
+            const bool oka = lam_cell_f32<false>(a, oa.r, oa.ru, oa.rv, oa.re, la);
+            const bool okb = lam_cell_f32<false>(a, ob.r, ob.ru, ob.rv, ob.re, lb);
+            lam_max = fmaxf(lam_max, fmaxf(fast::pick(live && oka, la, 0.f), fast::pick(live && okb, lb, 0.f)));
+            bad += (int)(live && !oka) + (int)(live && !okb);
         }
         typedef int v2i __attribute__((ext_vector_type(2)));
         const float o[4][2] = {{oa.r, ob.r}, {oa.ru, ob.ru}, {oa.rv, ob.rv}, {oa.re, ob.re}};
@@ -885,40 +898,66 @@ __device__ __forceinline__ void stage_f32_pair23_strip(const StageArgs& a, const
         }
     };
 
-    // A's prologue: U1 rows a0-2 .. a0+2, the north state of row a0-1 (reconstructed: A's rows all are)
-    Pair R0 = load_row(a0 - 1, true), R1 = load_row(a0, true), R2 = load_row(a0 + 1, true), R3 = load_row(a0 + 2, true);
-    Pair D0, D1;
-    PrepPair H0, H1;
+    // the ring slot of the U2 row A finishes at its row step t (row t-1), plane m
+    auto slot = [&](int t, int m) -> float2* { return ring + (((t - 1) & 1) * 4 + m) * WAVE + lane; };
     const F4 z4{0.f, 0.f, 0.f, 0.f};
-    Pair G0{z4, z4}, G1{z4, z4}, X0{z4, z4}, X1{z4, z4}, Q0{z4, z4}, Q1{z4, z4};
-    {
-        const Pair rm2 = load_row(a0 - 2, true);
-        D0 = sub_pair(R1, R0);
-        PrepPair lo_unused;
-        y_states(true, R0, D0, sub_pair(R0, rm2), lo_unused, H0);
+    if (role == 0) {
+        auto put = [&](int t, const Pair& r) {
+            *slot(t, 0) = {r.a.r, r.b.r}; *slot(t, 1) = {r.a.ru, r.b.ru};
+            *slot(t, 2) = {r.a.rv, r.b.rv}; *slot(t, 3) = {r.a.re, r.b.re};
+        };
+        // A's prologue: U1 rows a0-2 .. a0+2, the north state of row a0-1 (reconstructed: A's rows all are)
+        Pair R0 = load_row(a0 - 1, true), R1 = load_row(a0, true), R2 = load_row(a0 + 1, true), R3 = load_row(a0 + 2, true);
+        Pair D0, D1, T;
+        PrepPair H0, H1;
+        Pair G0{z4, z4}, G1{z4, z4}, X0{z4, z4}, X1{z4, z4}, Q0{z4, z4}, Q1{z4, z4};
+        {
+            const Pair rm2 = load_row(a0 - 2, true);
+            D0 = sub_pair(R1, R0);
+            PrepPair lo_unused;
+            y_states(true, R0, D0, sub_pair(R0, rm2), lo_unused, H0);
+        }
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        // rows a0 (faces only) and a0+1 (finishes U2 row a0 = j0-2), then in step with B
+        stepA(a0, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, Q0, Q1, T); put(a0, T); __syncthreads();
+        stepA(a0 + 1, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, Q1, Q0, T); put(a0 + 1, T); __syncthreads();
+        for (int t = j0; t <= a1; t += 4) {
+            stepA(t, R2, R3, R0, D0, D1, H0, H1, G0, G1, X0, X1, Q0, Q1, T); put(t, T); __syncthreads();
+            stepA(t + 1, R3, R0, R1, D1, D0, H1, H0, G1, G0, X1, X0, Q1, Q0, T); put(t + 1, T); __syncthreads();
+            stepA(t + 2, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, Q0, Q1, T); put(t + 2, T); __syncthreads();
+            stepA(t + 3, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, Q1, Q0, T); put(t + 3, T); __syncthreads();
+        }
+        return;
     }
-    // B's registers: U2 rows in a window of four, filled by A
-    Pair S0{z4, z4}, S1{z4, z4}, S2, S3;
+    auto get = [&](int t) -> Pair {
+        const float2 r = *slot(t, 0), u = *slot(t, 1), v = *slot(t, 2), e = *slot(t, 3);
+        return {{r.x, u.x, v.x, e.x}, {r.y, u.y, v.y, e.y}};
+    };
+    // B: U2 rows in a window of four, each taken from the ring after A's barrier
+    Pair S0{z4, z4}, S1, S2, S3;
     Pair E0{z4, z4}, E1;
     PrepPair K0, K1;
     Pair GB0{z4, z4}, GB1{z4, z4}, XB0{z4, z4}, XB1{z4, z4}, QB0{z4, z4}, QB1{z4, z4};
-    K0 = H0;   // defined values for the first warm-up step (whose y flux is skipped)
-    __builtin_amdgcn_s_waitcnt(0x0F70);
-    // A alone: rows a0 (faces only) and a0+1 (finishes U2 row a0 = j0-2)
-    stepA(a0, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, Q0, Q1, S0);
-    stepA(a0 + 1, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, Q1, Q0, S1);
+    {
+        // defined values for the first warm-up step (whose y flux is not used)
+        const Prep p0 = fast::prep(z4, g, gm1);
+        K0 = {p0, p0};
+    }
+    __syncthreads();
+    __syncthreads();
+    S1 = get(a0 + 1);                                   // U2 row j0-2
     for (int t = j0; t <= a1; t += 4) {
-        stepA(t, R2, R3, R0, D0, D1, H0, H1, G0, G1, X0, X1, Q0, Q1, S2);
+        __syncthreads(); S2 = get(t);
         stepB(t - 2, S0, S1, S2, E0, E1, K0, K1, GB0, GB1, XB0, XB1, QB0, QB1);
-        stepA(t + 1, R3, R0, R1, D1, D0, H1, H0, G1, G0, X1, X0, Q1, Q0, S3);
+        __syncthreads(); S3 = get(t + 1);
         stepB(t - 1, S1, S2, S3, E1, E0, K1, K0, GB1, GB0, XB1, XB0, QB1, QB0);
-        stepA(t + 2, R0, R1, R2, D0, D1, H0, H1, G0, G1, X0, X1, Q0, Q1, S0);
+        __syncthreads(); S0 = get(t + 2);
         stepB(t, S2, S3, S0, E0, E1, K0, K1, GB0, GB1, XB0, XB1, QB0, QB1);
-        stepA(t + 3, R1, R2, R3, D1, D0, H1, H0, G1, G0, X1, X0, Q1, Q0, S1);
+        __syncthreads(); S1 = get(t + 3);
         stepB(t + 1, S3, S0, S1, E1, E0, K1, K0, GB1, GB0, XB1, XB0, QB1, QB0);
     }
 
-    if (fuse) {
+    if (FUSE) {
         const double w = wave_max((double)lam_max);
         const long long b = wave_sum_ll((long long)bad);
         if (lane == 0) {
@@ -933,12 +972,25 @@ __device__ __forceinline__ void stage_f32_pair23_strip(const StageArgs& a, const
     }
 }
 
-template <int FLUX, int LIM>
-__global__ __launch_bounds__(256) void stage_f32_pair23_kernel(StageArgs a) {
-    const int s = strip_index(a, blockDim.x / WAVE);
-    if (s >= a.n_strips_x * a.n_strips_y) return;
-    stage_f32_pair23_strip<FLUX, LIM>(a, s, (const float*)a.uin[0], (const float*)a.u0[0], (float*)a.uout[0],
-                                      a.parity);
+template <int FLUX, int LIM, bool FUSE>
+__global__ __launch_bounds__(2 * WAVE) void stage_f32_pair23_kernel(StageArgs a) {
+    __shared__ float2 ring[2 * 4 * WAVE];
+    const int s = __builtin_amdgcn_readfirstlane(block_index(a));   // one strip per workgroup
+    const int role = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const float* U1 = (const float*)a.uin[0];
+    const float* U0 = (const float*)a.u0[0];
+    float* O = (float*)a.uout[0];
+    // seam mode 0 (MUSCL): strips holding a periodic seam take the seam-aware instance
+    bool seams = false;
+    if (a.seam_mode == 0 && LIM != LIM_NONE) {
+        const int sxi = s % a.n_strips_x, j0 = (s / a.n_strips_x) * a.strip_h;
+        const int cfirst = sxi * STRIP_OUT23 - 4, clast = cfirst + 2 * WAVE - 1;
+        const bool xs = cfirst <= 0 || clast >= a.nx;                 // holds column 0 or column nx
+        const bool ys = j0 - 2 <= 0 || min(j0 + a.strip_h, a.ny) + 2 >= a.ny;
+        seams = xs || ys;
+    }
+    if (seams) stage_f32_pair23_strip<FLUX, LIM, true, FUSE>(a, s, role, ring, U1, U0, O, a.parity);
+    else stage_f32_pair23_strip<FLUX, LIM, false, FUSE>(a, s, role, ring, U1, U0, O, a.parity);
 }
 
 // ---------------------------------------------------------------------------
@@ -1131,22 +1183,22 @@ hipError_t launch_stage_f32(const StageArgs& a, int block, hipStream_t st) {
 }
 
 // Fused stages 2 and 3 (stage_f32_pair23_kernel): a.uin = U1, a.u0 = U^n, a.uout = U^{n+1};
-// a.n_strips_x counts strips of STRIP_OUT23 columns.
-hipError_t launch_stage_f32_pair23(const StageArgs& a, int block, hipStream_t st) {
-    const int waves = block / WAVE;
-    const int nstrips = a.n_strips_x * a.n_strips_y;
-    const dim3 grid((nstrips + waves - 1) / waves);
+// a.n_strips_x counts strips of STRIP_OUT23 columns; one workgroup of two waves per strip.
+hipError_t launch_stage_f32_pair23(const StageArgs& a, hipStream_t st) {
+    const dim3 grid(a.n_strips_x * a.n_strips_y), block(2 * WAVE);
     const int lim = a.muscl ? a.limiter : LIM_NONE;
-#define EFVM_P23(F)                                                                                   \
-    do {                                                                                              \
-        if (lim == LIM_MC) stage_f32_pair23_kernel<F, LIM_MC><<<grid, block, 0, st>>>(a);             \
-        else if (lim == LIM_MINMOD) stage_f32_pair23_kernel<F, LIM_MINMOD><<<grid, block, 0, st>>>(a); \
-        else stage_f32_pair23_kernel<F, LIM_NONE><<<grid, block, 0, st>>>(a);                         \
+#define EFVM_P23_C(F, C)                                                                                 \
+    do {                                                                                                 \
+        if (lim == LIM_MC) stage_f32_pair23_kernel<F, LIM_MC, C><<<grid, block, 0, st>>>(a);             \
+        else if (lim == LIM_MINMOD) stage_f32_pair23_kernel<F, LIM_MINMOD, C><<<grid, block, 0, st>>>(a); \
+        else stage_f32_pair23_kernel<F, LIM_NONE, C><<<grid, block, 0, st>>>(a);                         \
     } while (0)
+#define EFVM_P23(F) do { if (a.fuse_cfl) EFVM_P23_C(F, true); else EFVM_P23_C(F, false); } while (0)
     if (a.flux == FLUX_RUSANOV) EFVM_P23(FLUX_RUSANOV);
     else if (a.flux == FLUX_HLLC) EFVM_P23(FLUX_HLLC);
     else EFVM_P23(FLUX_HLLC_TEXTBOOK);
 #undef EFVM_P23
+#undef EFVM_P23_C
     return hipGetLastError();
 }
 

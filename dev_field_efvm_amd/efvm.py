@@ -122,8 +122,8 @@ def load_library(path: str = LIB_PATH):
         "efvm_set_launch": [vp, C.c_int, C.c_int],
         "efvm_get_launch": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "efvm_set_pair_kernel": [vp, C.c_int],
-        "efvm_set_fused_step": [vp, C.c_int, C.c_int, C.c_int],
-        "efvm_get_fused_step": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "efvm_set_fused_step": [vp, C.c_int, C.c_int],
+        "efvm_get_fused_step": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "efvm_set_boundaries": [vp, C.POINTER(Bc)],
         "efvm_set_parabolic_inflow": [vp, d, d, d],
         "efvm_set_sponge": [vp, d, d, d, d, d, d, d],
@@ -224,17 +224,17 @@ class Context:
         """Allow (default) or forbid the two-column periodic strip kernel."""
         self._chk(self._L.efvm_set_pair_kernel(self._p, int(bool(allow))))
 
-    def set_fused_step(self, allow: bool = True, strip_rows: int = 0, waves_per_block: int = 0):
+    def set_fused_step(self, allow: bool = True, strip_rows: int = 0):
         """Allow (default) or forbid running RK3 stages 2 and 3 of whole steps as one fused launch,
-        and set its launch shape (0 keeps the current value)."""
-        self._chk(self._L.efvm_set_fused_step(self._p, int(bool(allow)), strip_rows, waves_per_block))
+        and set its strip height (0 keeps the current value)."""
+        self._chk(self._L.efvm_set_fused_step(self._p, int(bool(allow)), strip_rows))
 
     def fused_step(self):
-        """(allowed, strip_rows, waves_per_block, fits): fits = the configuration admits the fused
-        stage-2+3 launch (FP32 RK3, periodic two-column path, ny >= 8)."""
-        a, r, w, f = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        self._chk(self._L.efvm_get_fused_step(self._p, C.byref(a), C.byref(r), C.byref(w), C.byref(f)))
-        return bool(a.value), r.value, w.value, bool(f.value)
+        """(allowed, strip_rows, fits): fits = the configuration admits the fused stage-2+3 launch
+        (FP32 RK3, periodic two-column path, ny >= 8)."""
+        a, r, f = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self._L.efvm_get_fused_step(self._p, C.byref(a), C.byref(r), C.byref(f)))
+        return bool(a.value), r.value, bool(f.value)
 
     def set_boundaries(self, sides):
         """sides: 4 x (kind, params) for XLo, XHi, YLo, YHi."""

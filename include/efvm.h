@@ -107,16 +107,16 @@ efvm_status efvm_device_arch(int device, char* arch, size_t n);
 efvm_status efvm_set_pair_kernel(efvm_ctx* ctx, int allow);
 /* Run RK3 stages 2 and 3 of each whole step as one fused launch where it applies (allow = 1, the
  * default): FP32, the periodic two-column path (both axes periodic, nx even, no boundary
- * features), ny >= 8.  U2 then stays in registers and each step moves 80 B per cell
- * instead of 128; results are bitwise those of three substage launches.  strip_rows and
- * waves_per_block set the fused kernel's launch shape (0 keeps the current value).  Steps started
- * mid-step (after run() substages) and run() itself use the separate stage launches.  Used by
- * the autotuner; no reference counterpart (crates/cfd_solver/src/integrate.rs:69-101 runs the
- * three stages as three passes). */
-efvm_status efvm_set_fused_step(efvm_ctx* ctx, int allow, int strip_rows, int waves_per_block);
-/* The fused step's setting and launch shape, and whether this context's configuration admits it
+ * features), ny >= 8.  A workgroup of two waves per strip runs stage 2 and stage 3, U2 passing
+ * between them through LDS, so each step moves 80 B per cell instead of 128; results are
+ * bitwise those of three substage launches.  strip_rows sets the fused strips' height (0 keeps
+ * the current value).  Steps started mid-step (after run() substages) and run() itself use the
+ * separate stage launches.  Used by the autotuner; no reference counterpart
+ * (crates/cfd_solver/src/integrate.rs:69-101 runs the three stages as three passes). */
+efvm_status efvm_set_fused_step(efvm_ctx* ctx, int allow, int strip_rows);
+/* The fused step's setting and strip height, and whether this context's configuration admits it
  * (*fits).  Any pointer may be null.  No reference counterpart (a tuning query). */
-efvm_status efvm_get_fused_step(const efvm_ctx* ctx, int* allow, int* strip_rows, int* waves_per_block, int* fits);
+efvm_status efvm_get_fused_step(const efvm_ctx* ctx, int* allow, int* strip_rows, int* fits);
 
 /* BoundaryRegistry + StructuredBoundaryPlugin (crates/cfd_boundary/src/lib.rs:369-533):
  * sides[4] = XLo, XHi, YLo, YHi.  A half-periodic axis is EFVM_ERR_CONFIG. */

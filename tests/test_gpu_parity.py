@@ -663,8 +663,8 @@ def test_fused_step_is_bitwise_the_substage_launches(seam_mode):
             c = efvm.Context(nx, ny, hi=(10.0, 10.0), dz=10.0 / nx, flux=flux, muscl=lim is not None,
                              limiter=lim or "minmod", precision="fp32", seam_mode=seam_mode)
             c.set_boundaries(["periodic"] * 4)
-            c.set_fused_step(fused, rows, 0)
-            assert c.fused_step()[3], (nx, ny)
+            c.set_fused_step(fused, rows)
+            assert c.fused_step()[2], (nx, ny)
             c.upload(st)
             c.step(7)
             out = (c.download(), c.clock())
