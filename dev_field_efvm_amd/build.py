@@ -5,8 +5,9 @@ reference never fuses multiply-adds); the FP32 production unit fuses a
 multiply-add only inside one source expression (``-ffp-contract=on``), which the
 front end decides once per expression: the four compiled copies of the row step
 then round alike, and results do not depend on the strip height.  The FP32 unit also runs
-with the IEEE mode bit clear and no NaN operands assumed (``-fno-honor-nans
--mno-amdgpu-ieee``): min/max then need no operand canonicalisation, and its one NaN test
+with the IEEE mode bit clear, no NaN operands and no signed zeros assumed (``-fno-honor-nans
+-mno-amdgpu-ieee -fno-signed-zeros``): min/max then need no operand canonicalisation, factors
+2 and 1/2 ride on the output modifiers of the instructions before them, and its one NaN test
 (the CFL's non-physical-cell check) is written on bit patterns.  All HIP code is built with
 packed-FP32 instructions disabled.
 """
@@ -29,7 +30,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-x", "hip", f"--offload-arch={ARCH}",
           "-Wno-unused-result", "-Wno-unused-command-line-argument"]
 UNITS = [
     ("efvm_stage_f64.hip", ["-ffp-contract=off"]),
-    ("efvm_stage_f32.hip", ["-ffp-contract=on", "-fgpu-flush-denormals-to-zero", "-fno-honor-nans", "-mno-amdgpu-ieee"]),
+    ("efvm_stage_f32.hip", ["-ffp-contract=on", "-fgpu-flush-denormals-to-zero", "-fno-honor-nans", "-mno-amdgpu-ieee",
+                            "-fno-signed-zeros"]),
     ("efvm_mlp.hip", ["-ffp-contract=fast"]),
     ("efvm_stage3d.hip", ["-ffp-contract=off"]),
     ("efvm_api.cpp", ["-ffp-contract=off"]),

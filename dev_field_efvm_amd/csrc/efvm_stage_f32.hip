@@ -97,7 +97,9 @@ __device__ __forceinline__ F4 load_row(const StageArgs& a, const float* const* U
 template <int LIM>
 __device__ __forceinline__ float half_slope(float a, float b) {
     const float m = __builtin_amdgcn_fmed3f(0.f, a, b);
-    if (LIM == 1) return __builtin_amdgcn_fmed3f(0.f, m, 0.25f * (a + b));
+    // med3(0, m, (a+b)/4) written with the factors 2 and 1/2 (exact, so the same value), which
+    // the unit's IEEE-off build folds into the output modifiers of the med3 and add
+    if (LIM == 1) return 0.5f * __builtin_amdgcn_fmed3f(0.f, 2.f * m, 0.5f * (a + b));
     return 0.5f * m;
 }
 template <int LIM>
