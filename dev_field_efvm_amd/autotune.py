@@ -26,6 +26,7 @@ CTX_WAVES = (1, 2, 4)
 # and h + 3 of stage 3, the loop in groups of four: h = 4k + 1 leaves no idle step)
 FUSED_ROWS = (13, 21, 25, 29, 33, 37, 41, 45, 49, 57, 65, 73, 97)
 BATCH_ROWS = (7, 15, 31, 63, 127)
+FUSED_BATCH_ROWS = (13, 21, 29, 37, 45, 57)
 BATCH_BLOCKS = (0, 2048, 4096, 8192)
 
 
@@ -140,6 +141,7 @@ def tune_batch(batch, sides, steps: int = 4, rows=BATCH_ROWS, blocks=BATCH_BLOCK
     if use_cache and key in db:
         choice = {int(k): v for k, v in db[key]["rows"].items()}
         batch.set_pair_kernel(db[key].get("pair", 0))
+        batch.set_fused(db[key].get("fused", 0) > 0, db[key].get("fused", 0))
         _apply_batch(batch, sides, choice)
         batch.set_grid(db[key]["blocks"], 4)
         return choice, db[key]["blocks"], db[key]["ms"]
@@ -160,6 +162,19 @@ def tune_batch(batch, sides, steps: int = 4, rows=BATCH_ROWS, blocks=BATCH_BLOCK
         if best_ms is None or ms < best_ms:
             best_ms, pair = ms, p
     batch.set_pair_kernel(pair)
+    # one height for every class first, then class by class from the best uniform height
+    for r in rows:
+        if r == choice[classes[0]]:
+            continue
+        trial = {c: r for c in classes}
+        _apply_batch(batch, sides, trial)
+        ms = _time_batch(batch, steps)
+        batch.snapshot(restore=True)
+        if verbose:
+            print(f"  all classes rows={r:4d}  {ms:.3f} ms/step", flush=True)
+        if ms < best_ms:
+            best_ms, choice = ms, trial
+    _apply_batch(batch, sides, choice)
     for c in sorted(classes, reverse=True):          # largest classes dominate the time
         for r in rows:
             if r == choice[c]:
@@ -185,9 +200,23 @@ def tune_batch(batch, sides, steps: int = 4, rows=BATCH_ROWS, blocks=BATCH_BLOCK
         if ms < best_ms:
             best_ms, nb = ms, g
     batch.set_grid(nb, 4)
+    # the fused stage-2+3 launch of the periodic two-column cases: its strip height, or none
+    fused = batch.fused()[1] if batch.fused()[2] > 0 else 0
+    if pair and batch.fused()[2] > 0:
+        for r in (0,) + FUSED_BATCH_ROWS:
+            if r == fused:
+                continue
+            batch.set_fused(r > 0, r)
+            ms = _time_batch(batch, steps)
+            batch.snapshot(restore=True)
+            if verbose:
+                print(f"  fused rows={r:4d}  {ms:.3f} ms/step", flush=True)
+            if ms < best_ms:
+                best_ms, fused = ms, r
+    batch.set_fused(fused > 0, fused)
     if store:
         db[key] = {"rows": {str(k): v for k, v in choice.items()}, "blocks": nb, "ms": best_ms, "pair": pair,
-                   "when": time.time()}
+                   "fused": fused, "when": time.time()}
         _store(db)
     return choice, nb, best_ms
 

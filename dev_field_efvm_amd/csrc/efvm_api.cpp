@@ -1060,6 +1060,8 @@ hipError_t launch_stage_f32_batch(const BatchCase* cases, const StageArgs& rep, 
                                   int parity, int blocks, int waves, hipStream_t st);
 hipError_t launch_stage_f32_batch_pair(const BatchCase* cases, const StageArgs& rep, const int2* work, int n_work, int stage,
                                        int parity, int blocks, int waves, hipStream_t st);
+hipError_t launch_stage_f32_batch_pair23(const BatchCase* cases, const StageArgs& rep, const int2* work, int n_work,
+                                         int parity, hipStream_t st);
 }
 
 struct efvm_batch {
@@ -1071,9 +1073,15 @@ struct efvm_batch {
     BatchCase* dc = nullptr;
     int2* dwork = nullptr;         // (case, strip) work items of the one-column kernel
     int n_work = 0;
-    int2* dwork_pair = nullptr;    // and of the two-column kernel
+    int2* dwork_pair = nullptr;    // and of the two-column kernel (stage 1 of every case it takes)
     int n_work_pair = 0;
+    int2* dwork_pair_nf = nullptr; // the two-column cases that are not fused (their stages 2 and 3)
+    int n_work_pair_nf = 0;
+    int2* dwork_fused = nullptr;   // (case, strip) items of the fused stage-2+3 launch
+    int n_work_fused = 0;
     int allow_pair = 1;            // cases the two-column kernel takes run on it
+    int allow_fuse = 1;            // ... and the periodic ones run stages 2 and 3 fused
+    int strip_h23 = 37;            // rows per strip of the fused launch
     DevClock* clocks = nullptr;
     float* pool = nullptr;
     std::vector<long> offset;      // per-case start (cells) in the pool
@@ -1137,11 +1145,16 @@ StageArgs batch_case_args(const efvm_batch* b, int i) {
 // contiguous bands, one per queue.  Cases go largest first, so each queue ends on small work.
 // Position p of the table belongs to workgroup p / waves, hence to XCD (p / waves) % 8, and is
 // filled from that XCD's queue (or, once it is empty, from the longest remaining one).
-std::vector<int2> batch_work_table(const efvm_batch* b, bool pair) {
+// kind: 0 the one-column cases, 1 every two-column case (stage 1), 2 the two-column cases that are
+// not fused (stages 2 and 3), 3 the fused cases in the fused launch's strips (one workgroup per item).
+std::vector<int2> batch_work_table(const efvm_batch* b, int kind) {
     constexpr int NXCD = 8;
     std::vector<int> order;
-    for (int i = 0; i < b->n; i++)
-        if ((b->hc[i].a.pair_path != 0) == pair) order.push_back(i);
+    for (int i = 0; i < b->n; i++) {
+        const bool pair = b->hc[i].a.pair_path != 0, fused = b->hc[i].fused != 0;
+        if ((kind == 0 && !pair) || (kind == 1 && pair) || (kind == 2 && pair && !fused) || (kind == 3 && fused))
+            order.push_back(i);
+    }
     auto cells = [&](int i) { return (long)b->cases[i].nx * b->cases[i].ny; };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cells(x) > cells(y); });
     std::vector<std::vector<int2>> q(NXCD);
@@ -1150,7 +1163,7 @@ std::vector<int2> batch_work_table(const efvm_batch* b, bool pair) {
     for (int i : order) total += cells(i);
     const long share = std::max<long>(1, total / NXCD);
     for (int i : order) {
-        const StageArgs& a = b->hc[i].a;
+        const StageArgs& a = kind == 3 ? b->hc[i].a23 : b->hc[i].a;
         const int ns = a.n_strips_x * a.n_strips_y;
         const long per = std::max<long>(1, cells(i) / ns);
         if (cells(i) <= share / 4 || ns < NXCD) {
@@ -1170,7 +1183,7 @@ std::vector<int2> batch_work_table(const efvm_batch* b, bool pair) {
     std::vector<size_t> head(NXCD, 0);
     std::vector<int2> work;
     work.reserve(n_items);
-    const int W = std::max(1, b->waves);
+    const int W = kind == 3 ? 1 : std::max(1, b->waves);
     while (work.size() < n_items) {
         const int k = (int)((work.size() / W) % NXCD);
         int src = k;
@@ -1184,37 +1197,56 @@ std::vector<int2> batch_work_table(const efvm_batch* b, bool pair) {
     return work;
 }
 
+// The set holding case i's U^n: A, or for a fused case at odd step parity C.
+int batch_cur_set(const efvm_batch* b, int i) { return b->hc[i].fused && (b->parity & 1) ? 2 : 0; }
+
 efvm_status batch_rebuild(efvm_batch* b) {
     batch_drop_graph(b);
+    // a case's U^n moves between sets A and C when it becomes fused or stops being fused at odd
+    // parity: bring it back to A first (stream order keeps it behind the steps enqueued before)
+    std::vector<int> was_cur(b->n);
+    for (int i = 0; i < b->n; i++) was_cur[i] = batch_cur_set(b, i);
     for (int i = 0; i < b->n; i++) {
         BatchCase& bc = b->hc[i];
         bc.a = batch_case_args(b, i);
         for (int s = 0; s < 3; s++)
             for (int m = 0; m < 4; m++) bc.P[s][m] = b->pool + (long)(4 * s + m) * b->total_cells + b->offset[i];
+        bc.fused = b->allow_fuse && bc.a.pair_path == 1 && bc.a.ny >= 8 ? 1 : 0;
+        bc.a23 = bc.a;
+        bc.a23.strip_h = b->strip_h23;
+        bc.a23.n_strips_x = (bc.a.nx + STRIP_OUT23 - 1) / STRIP_OUT23;
+        bc.a23.n_strips_y = (bc.a.ny + b->strip_h23 - 1) / b->strip_h23;
     }
-    const std::vector<int2> work = batch_work_table(b, false), work_pair = batch_work_table(b, true);
-    for (int2** d : {&b->dwork, &b->dwork_pair}) {
-        if (*d) (void)hipFree(*d);
-        *d = nullptr;
+    for (int i = 0; i < b->n; i++) {
+        const int now = batch_cur_set(b, i);
+        if (now != was_cur[i]) {
+            const size_t bytes = sizeof(float) * b->cases[i].nx * b->cases[i].ny;
+            for (int m = 0; m < 4; m++)
+                BHIPCHK(b, hipMemcpyAsync(b->hc[i].P[now][m], b->hc[i].P[was_cur[i]][m], bytes, hipMemcpyDeviceToDevice, b->stream));
+        }
     }
-    BHIPCHK(b, hipMalloc((void**)&b->dwork, sizeof(int2) * std::max<size_t>(1, work.size())));
-    BHIPCHK(b, hipMalloc((void**)&b->dwork_pair, sizeof(int2) * std::max<size_t>(1, work_pair.size())));
-    if (!work.empty())
-        BHIPCHK(b, hipMemcpyAsync(b->dwork, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, b->stream));
-    if (!work_pair.empty())
-        BHIPCHK(b, hipMemcpyAsync(b->dwork_pair, work_pair.data(), sizeof(int2) * work_pair.size(), hipMemcpyHostToDevice,
-                                  b->stream));
+    const std::vector<int2> work = batch_work_table(b, 0), work_pair = batch_work_table(b, 1),
+                            work_pair_nf = batch_work_table(b, 2), work_fused = batch_work_table(b, 3);
+    struct T { int2** d; int* n; const std::vector<int2>* v; };
+    for (const T& t : {T{&b->dwork, &b->n_work, &work}, T{&b->dwork_pair, &b->n_work_pair, &work_pair},
+                       T{&b->dwork_pair_nf, &b->n_work_pair_nf, &work_pair_nf},
+                       T{&b->dwork_fused, &b->n_work_fused, &work_fused}}) {
+        if (*t.d) (void)hipFree(*t.d);
+        *t.d = nullptr;
+        BHIPCHK(b, hipMalloc((void**)t.d, sizeof(int2) * std::max<size_t>(1, t.v->size())));
+        if (!t.v->empty())
+            BHIPCHK(b, hipMemcpyAsync(*t.d, t.v->data(), sizeof(int2) * t.v->size(), hipMemcpyHostToDevice, b->stream));
+        *t.n = (int)t.v->size();
+    }
     BHIPCHK(b, hipMemcpyAsync(b->dc, b->hc.data(), sizeof(BatchCase) * b->n, hipMemcpyHostToDevice, b->stream));
     BHIPCHK(b, hipStreamSynchronize(b->stream));
-    b->n_work = (int)work.size();
-    b->n_work_pair = (int)work_pair.size();
     return EFVM_OK;
 }
 
 efvm_status batch_cfl_all(efvm_batch* b, int slot) {
     for (int i = 0; i < b->n; i++) {
         StageArgs a = b->hc[i].a;
-        hipError_t e = launch_cfl_f32(a, b->hc[i].P[0], &b->clocks[i].lam[slot], b->stream);
+        hipError_t e = launch_cfl_f32(a, b->hc[i].P[batch_cur_set(b, i)], &b->clocks[i].lam[slot], b->stream);
         if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, "batch cfl");
     }
     return EFVM_OK;
@@ -1230,21 +1262,26 @@ efvm_status batch_enqueue_steps(efvm_batch* b, int n_steps) {
         }
     // 0 = one wave per (case, strip) work item: the hardware dispatcher then balances the
     // uneven strips; a persistent grid striding through the table measured up to 13% slower
-    const int blocks = b->blocks > 0 ? b->blocks : std::max(1, (b->n_work + b->waves - 1) / b->waves);
-    const int blocks_pair = b->blocks > 0 ? b->blocks : std::max(1, (b->n_work_pair + b->waves - 1) / b->waves);
+    auto nblocks = [&](int n_items) { return b->blocks > 0 ? b->blocks : std::max(1, (n_items + b->waves - 1) / b->waves); };
     for (int k = 0; k < n_steps; k++) {
         for (int st = 0; st < 3; st++) {
-            // the two kernels take disjoint cases, so their order within a stage does not matter
-            if (b->n_work_pair > 0) {
-                hipError_t e = launch_stage_f32_batch_pair(b->dc, rep, b->dwork_pair, b->n_work_pair, st, b->parity,
-                                                           blocks_pair, b->waves, b->stream);
+            // the kernels of a stage take disjoint cases, so their order does not matter; the fused
+            // cases take part in stage 1 only, then run stages 2 and 3 in their own launch
+            const int2* wp = st == 0 ? b->dwork_pair : b->dwork_pair_nf;
+            const int np = st == 0 ? b->n_work_pair : b->n_work_pair_nf;
+            if (np > 0) {
+                hipError_t e = launch_stage_f32_batch_pair(b->dc, rep, wp, np, st, b->parity, nblocks(np), b->waves, b->stream);
                 if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("batch stage: ") + hipGetErrorString(e));
             }
             if (b->n_work > 0) {
-                hipError_t e = launch_stage_f32_batch(b->dc, rep, b->dwork, b->n_work, st, b->parity, blocks,
+                hipError_t e = launch_stage_f32_batch(b->dc, rep, b->dwork, b->n_work, st, b->parity, nblocks(b->n_work),
                                                       b->waves, b->stream);
                 if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("batch stage: ") + hipGetErrorString(e));
             }
+        }
+        if (b->n_work_fused > 0) {
+            hipError_t e = launch_stage_f32_batch_pair23(b->dc, rep, b->dwork_fused, b->n_work_fused, b->parity, b->stream);
+            if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("batch fused stages: ") + hipGetErrorString(e));
         }
         b->parity ^= 1;
     }
@@ -1308,6 +1345,8 @@ void efvm_batch_destroy(efvm_batch* b) {
     if (b->dc) (void)hipFree(b->dc);
     if (b->dwork) (void)hipFree(b->dwork);
     if (b->dwork_pair) (void)hipFree(b->dwork_pair);
+    if (b->dwork_pair_nf) (void)hipFree(b->dwork_pair_nf);
+    if (b->dwork_fused) (void)hipFree(b->dwork_fused);
     if (b->snap) (void)hipFree(b->snap);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1338,6 +1377,25 @@ efvm_status efvm_batch_set_pair_kernel(efvm_batch* b, int allow) {
     return batch_rebuild(b);
 }
 
+efvm_status efvm_batch_set_fused(efvm_batch* b, int allow, int strip_rows) {
+    if (!b || allow < 0 || allow > 1 || strip_rows < 0 || strip_rows > 4096) return bfail(b, EFVM_ERR_ARG, "bad fused setting");
+    b->allow_fuse = allow;
+    if (strip_rows) b->strip_h23 = strip_rows;
+    return batch_rebuild(b);
+}
+
+efvm_status efvm_batch_get_fused(const efvm_batch* b, int* allow, int* strip_rows, int* n_fused) {
+    if (!b) return EFVM_ERR_ARG;
+    if (allow) *allow = b->allow_fuse;
+    if (strip_rows) *strip_rows = b->strip_h23;
+    if (n_fused) {
+        int k = 0;
+        for (int i = 0; i < b->n; i++) k += b->hc[i].fused;
+        *n_fused = k;
+    }
+    return EFVM_OK;
+}
+
 efvm_status efvm_batch_set_grid(efvm_batch* b, int blocks, int waves_per_block) {
     if (!b || blocks < 0 || waves_per_block < 1 || waves_per_block > 4) return bfail(b, EFVM_ERR_ARG, "bad grid");
     b->blocks = blocks; b->waves = waves_per_block;
@@ -1354,8 +1412,9 @@ efvm_status efvm_batch_upload(efvm_batch* b, int i, const double* cons5, size_t 
         if (cons5[5 * q + 3] != 0.0) return bfail(b, EFVM_ERR_UNSUPPORTED, "rho_w must be 0");
         for (int m = 0; m < 4; m++) h[m * nc + q] = (float)cons5[5 * q + comp[m]];
     }
+    const int cur = batch_cur_set(b, i);
     for (int m = 0; m < 4; m++)
-        BHIPCHK(b, hipMemcpyAsync(b->hc[i].P[0][m], h.data() + m * nc, sizeof(float) * nc, hipMemcpyHostToDevice, b->stream));
+        BHIPCHK(b, hipMemcpyAsync(b->hc[i].P[cur][m], h.data() + m * nc, sizeof(float) * nc, hipMemcpyHostToDevice, b->stream));
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     b->lam_valid = 0;
     return EFVM_OK;
@@ -1366,8 +1425,9 @@ efvm_status efvm_batch_download(efvm_batch* b, int i, double* cons5, size_t n) {
     long nc = (long)b->cases[i].nx * b->cases[i].ny;
     if ((long)n != nc) return bfail(b, EFVM_ERR_ARG, "batch download: n_cells mismatch");
     std::vector<float> h(4 * nc);
+    const int cur = batch_cur_set(b, i);
     for (int m = 0; m < 4; m++)
-        BHIPCHK(b, hipMemcpyAsync(h.data() + m * nc, b->hc[i].P[0][m], sizeof(float) * nc, hipMemcpyDeviceToHost, b->stream));
+        BHIPCHK(b, hipMemcpyAsync(h.data() + m * nc, b->hc[i].P[cur][m], sizeof(float) * nc, hipMemcpyDeviceToHost, b->stream));
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     const int comp[4] = {0, 1, 2, 4};
     for (long q = 0; q < nc; q++) {
@@ -1466,20 +1526,29 @@ efvm_status efvm_batch_last_elapsed_ms(efvm_batch* b, double* ms) {
 efvm_status efvm_batch_snapshot(efvm_batch* b, int op) {
     if (!b || op < 0 || op > 1) return EFVM_ERR_ARG;
     static_assert(sizeof(DevClock) % 8 == 0, "clock layout");
+    // sets A and C: U^n of every case is in one of them (C for fused cases at odd parity)
     const size_t sbytes = sizeof(float) * 4 * b->total_cells, cbytes = sizeof(DevClock) * b->n;
+    float* setA = b->pool;
+    float* setC = b->pool + 8 * b->total_cells;
     if (!b->snap) {
         if (op == 1) return bfail(b, EFVM_ERR_ARG, "no snapshot saved");
-        BHIPCHK(b, hipMalloc(&b->snap, sbytes + cbytes));
+        BHIPCHK(b, hipMalloc(&b->snap, 2 * sbytes + cbytes));
     }
     char* s = (char*)b->snap;
     if (op == 0) {
-        BHIPCHK(b, hipMemcpyAsync(s, b->pool, sbytes, hipMemcpyDeviceToDevice, b->stream));
-        BHIPCHK(b, hipMemcpyAsync(s + sbytes, b->clocks, cbytes, hipMemcpyDeviceToDevice, b->stream));
+        BHIPCHK(b, hipMemcpyAsync(s, setA, sbytes, hipMemcpyDeviceToDevice, b->stream));
+        BHIPCHK(b, hipMemcpyAsync(s + sbytes, setC, sbytes, hipMemcpyDeviceToDevice, b->stream));
+        BHIPCHK(b, hipMemcpyAsync(s + 2 * sbytes, b->clocks, cbytes, hipMemcpyDeviceToDevice, b->stream));
         b->snap_parity = b->parity; b->snap_lam = b->lam_valid;
     } else {
-        BHIPCHK(b, hipMemcpyAsync(b->pool, s, sbytes, hipMemcpyDeviceToDevice, b->stream));
-        BHIPCHK(b, hipMemcpyAsync(b->clocks, s + sbytes, cbytes, hipMemcpyDeviceToDevice, b->stream));
-        b->parity = b->snap_parity; b->lam_valid = b->snap_lam;
+        if (b->snap_parity != b->parity) {
+            // fused cases move between A and C with the parity: restore under the saved one
+            b->parity = b->snap_parity;
+        }
+        BHIPCHK(b, hipMemcpyAsync(setA, s, sbytes, hipMemcpyDeviceToDevice, b->stream));
+        BHIPCHK(b, hipMemcpyAsync(setC, s + sbytes, sbytes, hipMemcpyDeviceToDevice, b->stream));
+        BHIPCHK(b, hipMemcpyAsync(b->clocks, s + 2 * sbytes, cbytes, hipMemcpyDeviceToDevice, b->stream));
+        b->lam_valid = b->snap_lam;
     }
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     return EFVM_OK;

@@ -108,6 +108,9 @@ inline int pair_path_kind(int nx, int ny, bool per_x, bool per_y, bool no_featur
 struct BatchCase {
     StageArgs a;
     void* P[3][4];
+    StageArgs a23;     // the case's arguments in the fused stage-2+3 strip geometry (when fused)
+    int fused;         // 1: whole steps run stage 1, then stages 2 and 3 in one fused launch; U^n
+                       // then lives in set A at even step parity and in set C at odd (set B holds U1)
 };
 
 // ---------------------------------------------------------------------------

@@ -974,15 +974,11 @@ __device__ __forceinline__ void stage_f32_pair23_strip(const StageArgs& a, const
     }
 }
 
+// one strip of the fused launch, in the seam-aware instance where the strip holds a periodic seam
+// (seam mode 0, MUSCL)
 template <int FLUX, int LIM, bool FUSE>
-__global__ __launch_bounds__(2 * WAVE) void stage_f32_pair23_kernel(StageArgs a) {
-    __shared__ float2 ring[2 * 4 * WAVE];
-    const int s = __builtin_amdgcn_readfirstlane(block_index(a));   // one strip per workgroup
-    const int role = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
-    const float* U1 = (const float*)a.uin[0];
-    const float* U0 = (const float*)a.u0[0];
-    float* O = (float*)a.uout[0];
-    // seam mode 0 (MUSCL): strips holding a periodic seam take the seam-aware instance
+__device__ __forceinline__ void pair23_run(const StageArgs& a, int s, int role, float2* ring, const float* U1,
+                                           const float* U0, float* O, int parity) {
     bool seams = false;
     if (a.seam_mode == 0 && LIM != LIM_NONE) {
         const int sxi = s % a.n_strips_x, j0 = (s / a.n_strips_x) * a.strip_h;
@@ -991,8 +987,33 @@ __global__ __launch_bounds__(2 * WAVE) void stage_f32_pair23_kernel(StageArgs a)
         const bool ys = j0 - 2 <= 0 || min(j0 + a.strip_h, a.ny) + 2 >= a.ny;
         seams = xs || ys;
     }
-    if (seams) stage_f32_pair23_strip<FLUX, LIM, true, FUSE>(a, s, role, ring, U1, U0, O, a.parity);
-    else stage_f32_pair23_strip<FLUX, LIM, false, FUSE>(a, s, role, ring, U1, U0, O, a.parity);
+    if (seams) stage_f32_pair23_strip<FLUX, LIM, true, FUSE>(a, s, role, ring, U1, U0, O, parity);
+    else stage_f32_pair23_strip<FLUX, LIM, false, FUSE>(a, s, role, ring, U1, U0, O, parity);
+}
+
+template <int FLUX, int LIM, bool FUSE>
+__global__ __launch_bounds__(2 * WAVE) void stage_f32_pair23_kernel(StageArgs a) {
+    __shared__ float2 ring[2 * 4 * WAVE];
+    const int s = __builtin_amdgcn_readfirstlane(block_index(a));   // one strip per workgroup
+    const int role = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    pair23_run<FLUX, LIM, FUSE>(a, s, role, ring, (const float*)a.uin[0], (const float*)a.u0[0], (float*)a.uout[0],
+                                a.parity);
+}
+
+// The batch's fused work items: one (case, strip) of a fused case per workgroup, its arguments
+// in the fused geometry; U1 in set B, U^n in set A (even step parity) or C (odd), U^{n+1} into
+// the other of the two.
+template <int FLUX, int LIM, bool FUSE>
+__global__ __launch_bounds__(2 * WAVE) void stage_f32_batch_pair23_kernel(const BatchCase* __restrict__ cases,
+                                                                          const int2* __restrict__ work, int parity) {
+    __shared__ float2 ring[2 * 4 * WAVE];
+    const int2 wk = work[blockIdx.x];
+    const int ci = __builtin_amdgcn_readfirstlane(wk.x), s = __builtin_amdgcn_readfirstlane(wk.y);
+    const int role = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const BatchCase& bc = cases[ci];
+    const int cur = (parity & 1) ? 2 : 0;
+    pair23_run<FLUX, LIM, FUSE>(bc.a23, s, role, ring, (const float*)bc.P[1][0], (const float*)bc.P[cur][0],
+                                (float*)bc.P[cur ^ 2][0], parity);
 }
 
 // ---------------------------------------------------------------------------
@@ -1109,9 +1130,11 @@ __global__ __launch_bounds__(256) void stage_f32_batch_pair_kernel(const BatchCa
         const int wu = __builtin_amdgcn_readfirstlane(w);
         const int2 wk = work[wu];
         const BatchCase& bc = cases[wk.x];
-        const int in = stage == 0 ? 0 : (stage == 1 ? 1 : 2);
+        // a fused case runs only stage 1 here, from the set holding U^n at this parity
+        const int cur = bc.fused && (parity & 1) ? 2 : 0;
+        const int in = stage == 0 ? cur : (stage == 1 ? 1 : 2);
         const int out = stage == 0 ? 1 : (stage == 1 ? 2 : 0);
-        stage_f32_pair_strip<FLUX, LIM, true>(bc.a, wk.y, (const float*)bc.P[in][0], (const float*)bc.P[0][0],
+        stage_f32_pair_strip<FLUX, LIM, true>(bc.a, wk.y, (const float*)bc.P[in][0], (const float*)bc.P[cur][0],
                                               (float*)bc.P[out][0], stage, parity);
     }
 }
@@ -1353,6 +1376,25 @@ hipError_t launch_stage_f32_batch_pair(const BatchCase* cases, const StageArgs& 
     else if (a_.flux == FLUX_HLLC) EFVM_BPAIR(FLUX_HLLC);
     else EFVM_BPAIR(FLUX_HLLC_TEXTBOOK);
 #undef EFVM_BPAIR
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_f32_batch_pair23(const BatchCase* cases, const StageArgs& a_, const int2* work, int n_work,
+                                         int parity, hipStream_t st) {
+    const dim3 grid(n_work), block(2 * WAVE);
+    const int lim = a_.muscl ? a_.limiter : LIM_NONE;
+#define EFVM_B23_C(F, C)                                                                                               \
+    do {                                                                                                               \
+        if (lim == LIM_MC) stage_f32_batch_pair23_kernel<F, LIM_MC, C><<<grid, block, 0, st>>>(cases, work, parity);   \
+        else if (lim == LIM_MINMOD) stage_f32_batch_pair23_kernel<F, LIM_MINMOD, C><<<grid, block, 0, st>>>(cases, work, parity); \
+        else stage_f32_batch_pair23_kernel<F, LIM_NONE, C><<<grid, block, 0, st>>>(cases, work, parity);               \
+    } while (0)
+#define EFVM_B23(F) do { if (a_.fuse_cfl) EFVM_B23_C(F, true); else EFVM_B23_C(F, false); } while (0)
+    if (a_.flux == FLUX_RUSANOV) EFVM_B23(FLUX_RUSANOV);
+    else if (a_.flux == FLUX_HLLC) EFVM_B23(FLUX_HLLC);
+    else EFVM_B23(FLUX_HLLC_TEXTBOOK);
+#undef EFVM_B23
+#undef EFVM_B23_C
     return hipGetLastError();
 }
 

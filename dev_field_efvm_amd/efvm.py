@@ -405,6 +405,8 @@ def _bind_batch(L):
         "efvm_batch_snapshot": [vp, C.c_int],
         "efvm_batch_set_rows": [vp, C.POINTER(C.c_int)],
         "efvm_batch_set_pair_kernel": [vp, C.c_int],
+        "efvm_batch_set_fused": [vp, C.c_int, C.c_int],
+        "efvm_batch_get_fused": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -421,7 +423,7 @@ EXPORTS += [
     "efvm_batch_create", "efvm_batch_destroy", "efvm_batch_last_error", "efvm_batch_set_launch",
     "efvm_batch_set_grid", "efvm_batch_upload", "efvm_batch_download", "efvm_batch_step",
     "efvm_batch_graph_capture", "efvm_batch_graph_launch", "efvm_batch_clock", "efvm_batch_synchronize",
-    "efvm_batch_last_elapsed_ms", "efvm_batch_snapshot", "efvm_batch_set_rows", "efvm_batch_set_pair_kernel",
+    "efvm_batch_last_elapsed_ms", "efvm_batch_snapshot", "efvm_batch_set_rows", "efvm_batch_set_pair_kernel", "efvm_batch_set_fused", "efvm_batch_get_fused",
 ]
 
 
@@ -483,6 +485,17 @@ class Batch:
     def set_pair_kernel(self, allow: bool):
         """Allow (default) or forbid the two-column strip kernel for the cases it takes."""
         self._chk(self._L.efvm_batch_set_pair_kernel(self._p, int(bool(allow))))
+
+    def set_fused(self, allow: bool = True, strip_rows: int = 0):
+        """Allow (default) or forbid the fused stage-2+3 launch for the periodic two-column cases,
+        and set its strip height (0 keeps the current one)."""
+        self._chk(self._L.efvm_batch_set_fused(self._p, int(bool(allow)), strip_rows))
+
+    def fused(self):
+        """(allowed, strip_rows, number of cases running fused)."""
+        a, r, n = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self._L.efvm_batch_get_fused(self._p, C.byref(a), C.byref(r), C.byref(n)))
+        return bool(a.value), r.value, n.value
 
     def upload(self, i, cons):
         a = np.ascontiguousarray(cons, dtype=np.float64).reshape(-1, 5)

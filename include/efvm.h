@@ -221,6 +221,14 @@ efvm_status efvm_batch_set_rows(efvm_batch* b, const int* rows);
  * (periodic, outflow, slip, no-slip and symmetry sides, no features; nx even on a periodic x
  * axis).  No counterpart in the reference: a launch choice, results are bitwise the same. */
 efvm_status efvm_batch_set_pair_kernel(efvm_batch* batch, int allow);
+/* Allow (1, the default) or forbid (0) running stages 2 and 3 of the periodic two-column cases
+ * (both axes periodic, nx even, ny >= 8) as one fused launch, a (stage 2, stage 3) wave pair per
+ * strip of strip_rows rows (0 keeps the current height).  A fused case's U^n alternates between
+ * two plane sets by step parity; upload, download, the CFL scan and snapshots follow it.  No
+ * counterpart in the reference: a launch choice, results are bitwise the same. */
+efvm_status efvm_batch_set_fused(efvm_batch* batch, int allow, int strip_rows);
+/* The fused setting, its strip height and how many cases run fused (any pointer may be null). */
+efvm_status efvm_batch_get_fused(const efvm_batch* batch, int* allow, int* strip_rows, int* n_fused);
 efvm_status efvm_batch_set_grid(efvm_batch* b, int blocks, int waves_per_block);
 efvm_status efvm_batch_upload(efvm_batch* b, int case_id, const double* cons5, size_t n_cells);
 efvm_status efvm_batch_download(efvm_batch* b, int case_id, double* cons5, size_t n_cells);

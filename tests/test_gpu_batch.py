@@ -66,6 +66,62 @@ def test_batch_two_column_kernel_is_bitwise_the_one_column_kernel():
         assert out[0][1][i] == out[1][1][i]
 
 
+def test_batch_fused_stages_are_bitwise_the_separate_launches():
+    """The periodic two-column cases (nx even, ny >= 8) run stages 2 and 3 as one fused launch,
+    their U^n alternating between two plane sets by step parity.  Against fusion off: bitwise the
+    same states and clocks after odd and even step counts, at several fused strip heights, with
+    the setting switched mid-run at odd parity (the state moves to the set the new mode reads),
+    through a snapshot save and restore, and from a graph replay."""
+    specs, states = small_ensemble(n_cases=12, seed=5)
+
+    def run(allow, rows, steps, switch_at=None):
+        b = efvm.Batch(specs)
+        b.set_fused(allow, rows)
+        for i, st in enumerate(states):
+            b.upload(i, st)
+        if switch_at is None:
+            b.step(steps)
+        else:
+            b.step(switch_at)
+            b.set_fused(not allow, rows)
+            b.step(steps - switch_at)
+        return b, [b.download(i) for i in range(len(specs))], [b.clock(i) for i in range(len(specs))]
+
+    for steps in (7, 8):
+        b0, ref, ck = run(False, 0, steps)
+        assert b0.fused()[2] == 0
+        for rows, switch in ((37, None), (5, None), (13, 3), (57, None)):
+            b, got, gk = run(True, rows, steps, switch)
+            if switch is None:
+                assert b.fused()[2] > 0
+            for i in range(len(specs)):
+                assert np.array_equal(got[i], ref[i]), (steps, rows, i, specs[i]["kind"], specs[i]["nx"], specs[i]["ny"])
+                assert gk[i] == ck[i]
+    # snapshot at odd parity, more steps, restore, the same steps again
+    b = efvm.Batch(specs)
+    for i, st in enumerate(states):
+        b.upload(i, st)
+    b.step(3)
+    b.snapshot()
+    b.step(4)
+    first = [b.download(i) for i in range(len(specs))]
+    b.snapshot(restore=True)
+    b.step(4)
+    for i in range(len(specs)):
+        assert np.array_equal(b.download(i), first[i]), i
+    # graph replay from odd parity
+    g = efvm.Batch(specs)
+    for i, st in enumerate(states):
+        g.upload(i, st)
+    g.step(3)
+    g.graph_capture(2)
+    g.graph_launch()
+    g.graph_launch()
+    g.synchronize()
+    for i in range(len(specs)):
+        assert np.array_equal(g.download(i), first[i]), i
+
+
 def test_batch_per_case_dt_differs():
     specs, states = small_ensemble()
     b = efvm.Batch(specs)
