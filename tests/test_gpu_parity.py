@@ -675,6 +675,29 @@ def test_fused_step_is_bitwise_the_substage_launches(seam_mode):
             assert out[1] == ref[1], (nx, ny, flux, lim, rows)
 
 
+def test_fused_step_with_fixed_dt_and_euler_fallback():
+    """A fixed time step compiles the fused launch without its CFL epilogue: bitwise the
+    substage launches, clock included.  Euler steps and FP64 contexts never take the fused
+    launch (fits is false) and run as before."""
+    n = 130
+    st = cases.vortex_modes_state(n, 64, seed=9)
+    out = []
+    for fused in (False, True):
+        c = efvm.Context(n, 64, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc", precision="fp32",
+                         fixed_dt=2.0e-3)
+        c.set_boundaries(["periodic"] * 4)
+        c.set_fused_step(fused, 21)
+        c.upload(st)
+        c.step(5)
+        out.append((c.download(), c.clock()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and abs(out[1][1]["time"] - 5 * 2.0e-3) < 1e-15
+    for kw in (dict(integrator="euler", precision="fp32"), dict(precision="fp64")):
+        c = efvm.Context(n, 64, hi=(10.0, 10.0), dz=10.0 / n, muscl=True, limiter="mc", **kw)
+        c.set_boundaries(["periodic"] * 4)
+        assert not c.fused_step()[2], kw
+
+
 def test_fused_step_graph_replay_and_mid_step_entry():
     """Graph replay of fused steps equals eager fused steps (several replays with eager work
     between); a step entered mid-step (after one run() substage) runs the separate launches and
