@@ -8,8 +8,10 @@ then round alike, and results do not depend on the strip height.  The FP32 unit 
 with the IEEE mode bit clear, no NaN operands and no signed zeros assumed (``-fno-honor-nans
 -mno-amdgpu-ieee -fno-signed-zeros``): min/max then need no operand canonicalisation, factors
 2 and 1/2 ride on the output modifiers of the instructions before them, and its one NaN test
-(the CFL's non-physical-cell check) is written on bit patterns.  All HIP code is built with
-packed-FP32 instructions disabled.
+(the CFL's non-physical-cell check) is written on bit patterns.  The learned-reconstruction
+unit also fuses multiply-adds per source expression only: its fused bypass stage and the
+separate passes it replaces then round alike wherever they evaluate the same expression.  All
+HIP code is built with packed-FP32 instructions disabled.
 """
 from __future__ import annotations
 
@@ -32,7 +34,7 @@ UNITS = [
     ("efvm_stage_f64.hip", ["-ffp-contract=off"]),
     ("efvm_stage_f32.hip", ["-ffp-contract=on", "-fgpu-flush-denormals-to-zero", "-fno-honor-nans", "-mno-amdgpu-ieee",
                             "-fno-signed-zeros"]),
-    ("efvm_mlp.hip", ["-ffp-contract=fast"]),
+    ("efvm_mlp.hip", ["-ffp-contract=on"]),
     ("efvm_stage3d.hip", ["-ffp-contract=off"]),
     ("efvm_api.cpp", ["-ffp-contract=off"]),
     ("efvm_api3d.cpp", ["-ffp-contract=off"]),

@@ -53,6 +53,7 @@ hipError_t launch_lam_reduce(const double* part, int n, double* target, hipStrea
 hipError_t launch_cfl_f32(const StageArgs& a, const void* const* planes, double* target, hipStream_t st);
 hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st);
 hipError_t launch_mlp_faces_bypass(const mlp::FaceArgs& a, hipStream_t st);
+hipError_t launch_mlp_light_stage(const StageArgs& a, const mlp::FaceArgs& fa, int blocks, hipStream_t st);
 hipError_t launch_mlp_infer(const mlp::Packed* P, const float* X, long n, float* Y, hipStream_t st);
 namespace mlp { struct TrainArgs; struct SampleArgs; }
 hipError_t launch_mlp_train(const mlp::TrainArgs& a, int nblk, float* grad, double* loss_out, hipStream_t st);
@@ -82,6 +83,7 @@ struct efvm_ctx {
     int ghost_hist = 0;            // the last completed step's substage inputs and pre-mirror ghosts are resident
     DevClock* clock = nullptr;
     double* lam_part = nullptr;    // FP32: per-wave maxima of the fused CFL
+    long lam_slots = 0;            // entries of lam_part
     double* bc_tab[4] = {nullptr, nullptr, nullptr, nullptr};   // EFVM_BC_TABLE ghost tables
     double* yw = nullptr;
     uint8_t* solid = nullptr;
@@ -126,8 +128,11 @@ struct efvm_ctx {
     int learned = 0;               // learned face reconstruction attached
     int learned_bypass = 0;        // gate evaluated only where the ungated face value is near the TVD bound
     int learned_own4 = 0;          // the attached network's correction is the 4-tap own-variable form
+    float learned_taps[4][4] = {}; // its open-gate taps
     float* faces = nullptr;        // 16 planes of face primitives
-    unsigned* dlist = nullptr;     // bypass: deferred faces (2 nx ny entries) and their count
+    unsigned* dlist = nullptr;     // bypass: deferred faces (2 nx ny entries), their count, then the fused
+                                   // stage's pending-cell count and finished-workgroup counter
+    int learned_fused = 1;         // bypass stages on the 4-tap correction run as one fused stage kernel
     mlp::Packed* wpack = nullptr;  // the attached network's inference weights
     unsigned long long* lstats = nullptr;   // [fallback faces, faces evaluated, sides the network gated]
     std::string err;
@@ -262,6 +267,13 @@ efvm_status end_of_step(efvm_ctx* c, bool cfl_fused, int u2_kept) {
 }
 
 // One substage (one app.run()).  Enqueue only; no host synchronisation.
+// Bypass stages whose correction is the 4-tap form run as one fused stage kernel and the
+// pending-cell pass (cells are listed as index << 4 | face mask: at most 2^28 cells).
+bool learned_fused_fits(const efvm_ctx* c) {
+    return c->learned && c->learned_bypass && c->learned_fused && c->learned_own4 && c->dlist && c->lam_part &&
+           c->ncell < (1L << 28);
+}
+
 efvm_status run_substage(efvm_ctx* c) {
     const bool rk3 = c->sch.integrator == EFVM_INTEGRATOR_RK3;
     // RK3: A = U^n (set cur, never rotates), B = U1, C = U2.  Euler ping-pongs sets 0 <-> 1,
@@ -299,6 +311,7 @@ efvm_status run_substage(efvm_ctx* c) {
                                    : launch_ibm_mirror<float>(a, ib, c->n_ghost, c->ghost_idx, c->ghost_img, c->ghost_pre[c->rk_stage ? 1 : 0], c->stream);
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "ibm mirror");
     }
+    int nw_learned = 0;   // waves of the fused learned stage (its CFL slots)
     if (c->learned) {
         // face states of the stage input from the network (with the MUSCL fallback), then the stage
         mlp::FaceArgs fa;
@@ -318,17 +331,33 @@ efvm_status run_substage(efvm_ctx* c) {
         fa.list = c->dlist;
         fa.n_list = c->dlist ? c->dlist + 2 * c->ncell : nullptr;
         fa.own4 = c->learned_own4;
-        hipError_t e = c->learned_bypass ? launch_mlp_faces_bypass(fa, c->stream) : launch_mlp_faces(fa, c->stream);
-        if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("face kernels: ") + hipGetErrorString(e));
-        a.face_count = c->learned_bypass ? fa.n_list : nullptr;
-        e = fa.out_mode == 1 ? launch_stage_f32_flux(a, c->faces, c->stream) : launch_stage_f32_faces(a, c->faces, c->stream);
-        if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("learned stage kernel: ") + hipGetErrorString(e));
+        std::memcpy(fa.taps, c->learned_taps, sizeof(fa.taps));
+        if (learned_fused_fits(c)) {
+            // one fused stage kernel (faces, fluxes, update) and the pending-cell pass
+            StageArgs b = a;
+            b.strip_h = mlp::LIGHT_ROWS;
+            b.n_strips_x = (c->grid.nx + mlp::LIGHT_OUT - 1) / mlp::LIGHT_OUT;
+            b.n_strips_y = (c->grid.ny + mlp::LIGHT_ROWS - 1) / mlp::LIGHT_ROWS;
+            fa.n_list = c->dlist + 2 * c->ncell + 1;
+            const long tiles = (long)b.n_strips_x * b.n_strips_y;
+            const int blocks = (int)std::min<long>(tiles, std::min<long>(mlp::STAT_SLOTS, c->lam_slots) / (mlp::LIGHT_ROWS + 1));
+            nw_learned = (mlp::LIGHT_ROWS + 1) * blocks;
+            hipError_t e = launch_mlp_light_stage(b, fa, blocks, c->stream);
+            if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("fused learned stage: ") + hipGetErrorString(e));
+        } else {
+            hipError_t e = c->learned_bypass ? launch_mlp_faces_bypass(fa, c->stream) : launch_mlp_faces(fa, c->stream);
+            if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("face kernels: ") + hipGetErrorString(e));
+            a.face_count = c->learned_bypass ? fa.n_list : nullptr;
+            e = fa.out_mode == 1 ? launch_stage_f32_flux(a, c->faces, c->stream) : launch_stage_f32_faces(a, c->faces, c->stream);
+            if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("learned stage kernel: ") + hipGetErrorString(e));
+        }
     } else if ((st = launch_stage(c, a)) != EFVM_OK) {
         return st;
     }
     if (a.fuse_cfl && a.lam_part) {
-        // the stage left one CFL maximum per wave: strips, or 64 cells of the learned stage
-        const int nw = c->learned ? (int)((c->ncell + 63) / 64) : a.n_strips_x * a.n_strips_y;
+        // the stage left one CFL maximum per wave: strips, 64 cells of the learned stage, or
+        // the waves of the fused learned stage
+        const int nw = c->learned ? (nw_learned ? nw_learned : (int)((c->ncell + 63) / 64)) : a.n_strips_x * a.n_strips_y;
         hipError_t e = launch_lam_reduce(a.lam_part, nw, &c->clock->lam[a.parity ^ 1], c->stream);
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("cfl reduction: ") + hipGetErrorString(e));
     }
@@ -454,10 +483,15 @@ efvm_status efvm_ctx_create(const efvm_grid* grid, const efvm_gas* gas, const ef
     }
     if (st == EFVM_OK) st = device_alloc(c, (void**)&c->clock, sizeof(DevClock));
     if (st == EFVM_OK && c->esz == 4) {
-        // slots for the most waves a fused-CFL stage can launch: strips of one row, or 64 cells
+        // slots for the most waves a fused-CFL stage can launch: strips of one row, 64 cells, or
+        // the fused learned stage's workgroups of LIGHT_ROWS + 1 waves (at most STAT_SLOTS waves)
         const long strips = (long)((grid->nx + STRIP_OUT - 1) / STRIP_OUT) * grid->ny;
-        const long n = std::max(strips, (c->ncell + 63) / 64) + 64;
+        const long light = std::min<long>((long)((grid->nx + mlp::LIGHT_OUT - 1) / mlp::LIGHT_OUT) *
+                                              ((grid->ny + mlp::LIGHT_ROWS - 1) / mlp::LIGHT_ROWS) * (mlp::LIGHT_ROWS + 1),
+                                          mlp::STAT_SLOTS);
+        const long n = std::max(std::max(strips, light), (c->ncell + 63) / 64) + 64;
         st = device_alloc(c, (void**)&c->lam_part, sizeof(double) * n);
+        c->lam_slots = n;
     }
     if (st == EFVM_OK) st = device_alloc(c, (void**)&c->yw, sizeof(double) * 3 * grid->ny);
     if (st == EFVM_OK) st = device_alloc(c, (void**)&c->scratch, sizeof(double) * (1024 * 4 + 16));
@@ -1611,6 +1645,7 @@ struct efvm_mlp {
     mlp::Packed* packed = nullptr;
     int packed_valid = 0;
     int own4 = 0;                  // the packed correction is the 4-tap own-variable form
+    float taps[4][4] = {};         // its open-gate taps (mlp::light_taps)
     std::string err;
 };
 
@@ -1690,6 +1725,7 @@ efvm_status mlp_repack(efvm_mlp* m) {
     mlp::Packed P;
     pack_weights(h.data(), P);
     m->own4 = P.own4;
+    mlp::light_taps(P.L, m->taps);
     MHIPCHK(m, hipMemcpyAsync(m->packed, &P, sizeof(P), hipMemcpyHostToDevice, m->stream));
     MHIPCHK(m, hipStreamSynchronize(m->stream));
     m->packed_valid = 1;
@@ -2037,6 +2073,7 @@ efvm_status efvm_set_learned(efvm_ctx* c, efvm_mlp* m) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->learned = 1;
     c->learned_own4 = m->own4;
+    std::memcpy(c->learned_taps, m->taps, sizeof(c->learned_taps));
     return EFVM_OK;
 }
 
@@ -2046,11 +2083,25 @@ efvm_status efvm_set_learned_bypass(efvm_ctx* c, int bypass) {
         return fail(c, EFVM_ERR_UNSUPPORTED, "bypass face list indexes faces in 32 bits: grid too large");
     drop_graph(c);
     if (bypass && !c->dlist) {
-        HIPCHK(c, hipMalloc((void**)&c->dlist, sizeof(unsigned) * (2 * c->ncell + 1)));
-        HIPCHK(c, hipMemsetAsync(c->dlist + 2 * c->ncell, 0, sizeof(unsigned), c->stream));
+        HIPCHK(c, hipMalloc((void**)&c->dlist, sizeof(unsigned) * (2 * c->ncell + 4)));
+        HIPCHK(c, hipMemsetAsync(c->dlist + 2 * c->ncell, 0, 4 * sizeof(unsigned), c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     c->learned_bypass = bypass;
+    return EFVM_OK;
+}
+
+efvm_status efvm_set_learned_fused(efvm_ctx* c, int allow) {
+    if (!c || (allow != 0 && allow != 1)) return EFVM_ERR_ARG;
+    c->learned_fused = allow;
+    drop_graph(c);
+    return EFVM_OK;
+}
+
+efvm_status efvm_get_learned_fused(const efvm_ctx* c, int* allow, int* fits) {
+    if (!c) return EFVM_ERR_ARG;
+    if (allow) *allow = c->learned_fused;
+    if (fits) *fits = learned_fused_fits(c) ? 1 : 0;
     return EFVM_OK;
 }
 
@@ -2087,6 +2138,7 @@ efvm_status efvm_learned_faces(efvm_ctx* c, float* out, size_t n) {
     fa.list = nullptr;
     fa.n_list = nullptr;
     fa.own4 = c->learned_own4;
+    std::memcpy(fa.taps, c->learned_taps, sizeof(fa.taps));
     HIPCHK(c, launch_mlp_faces(fa, c->stream));
     HIPCHK(c, hipMemcpyAsync(out, c->faces, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

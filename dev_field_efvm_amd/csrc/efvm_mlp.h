@@ -34,6 +34,9 @@ constexpr int OFF_BL = OFF_L + NOUT * NIN;
 constexpr int NPARAM = OFF_BL + NOUT;   // 5848
 constexpr int TB = 64;                  // samples per training block (one wave)
 constexpr int STAT_SLOTS = 8192;        // waves of the largest face-kernel launch
+constexpr int LIGHT_OUT = 58;            // output columns per tile of the fused bypass stage (a
+                                        // wave of 64 columns less the +-3 of the x-face stencil)
+constexpr int LIGHT_ROWS = 3;            // rows per tile (a workgroup of LIGHT_ROWS + 1 waves)
 
 // features: x[4k+v], k = 0..4 <-> stencil cells -2..2 (k = 3 is the neighbour across
 // the face), v = rho, u_n, u_t, p.  x = (w_k - w_0) / sigma_v for k != 2 and
@@ -101,7 +104,21 @@ struct FaceArgs {
     unsigned* list;              // deferred faces of the bypass passes: axis * nx*ny + row * nx + column
     unsigned* n_list;            // their count (device)
     int own4;                    // host copy of w->own4 (selects the light pass's specialisation)
+    float taps[4][4];            // own4: the open-gate value's taps on the stencil cells -2, -1, +1, +2
+                                 // per variable, L's own-variable weights with the central slope's
+                                 // -1/4 and +1/4 folded into the inner two (uniform, so they sit in
+                                 // scalar registers)
 };
+
+// the taps of FaceArgs::taps from the correction's weights L[v][f] (feature f = 4 k + v)
+inline void light_taps(const float L[NOUT][NIN], float taps[4][4]) {
+    for (int v = 0; v < 4; v++) {
+        taps[0][v] = L[v][v];
+        taps[1][v] = L[v][4 + v] - 0.25f;
+        taps[2][v] = L[v][12 + v] + 0.25f;
+        taps[3][v] = L[v][16 + v];
+    }
+}
 
 // one training minibatch (loss and gradient)
 struct TrainArgs {

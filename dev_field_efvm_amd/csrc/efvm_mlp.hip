@@ -16,6 +16,7 @@
 #include "efvm_stage.h"
 #include "efvm_fast.h"
 #include "efvm_mlp.h"
+#include "efvm_cfl_f32.h"
 
 namespace efvm {
 namespace mlp {
@@ -167,18 +168,78 @@ __device__ __forceinline__ bool side_ok(const float ws[5][4], const float f[4], 
     return ok;
 }
 
+// The open-gate face value of one oriented side with the 4-tap correction: fo = w_c +
+// sum_k t_k (w_k - w_c) over the stencil cells k = 0, 1, 3, 4 (taps tp[0..3][v]: L's own-variable
+// weights, the central slope's -1/4 and +1/4 folded into the inner two).  With the gate open and
+// no bias the feature normalisation cancels, so this is the face value the network path
+// computes for phi = 1, up to rounding.
+__device__ __forceinline__ void light_value(const float ws[5][4], const float (&tp)[4][4], float fo[4]) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const float c = ws[2][v];
+        float d = tp[0][v] * (ws[0][v] - c);
+        d = fmaf(tp[1][v], ws[1][v] - c, d);
+        d = fmaf(tp[2][v], ws[3][v] - c, d);
+        d = fmaf(tp[3][v], ws[4][v] - c, d);
+        fo[v] = c + d;
+    }
+}
+
+// second difference of a variable at a cell, one symmetric form for every face it bounds
+__device__ __forceinline__ float curvature(float wm, float w0, float wp) { return (wm + wp) - 2.f * w0; }
+
+// The strict bound (half the check's relaxation) for both sides of one face, in the frame whose
+// normal points from the left cell a to the right cell b (variables rho, u_n, u_t, p): ca, cb =
+// the cells' curvatures along the normal, fl / fr = the two face values in that frame, tl / tr =
+// the tolerances of the sides' own cells (FLOOR x (rho, a0, a0, p)).  A side passes when its
+// density and pressure are positive and every variable lies within [min(a, b) - d, max(a, b) + d],
+// d = relax |minmod(ca, cb)| + tol: side_ok's bound, written once per face (the right side's
+// oriented test negates its normal velocity and the bounds alike, so the two are the same test).
+__device__ __forceinline__ void face_test(const float a[4], const float b[4], const float ca[4], const float cb[4],
+                                          const float fl[4], const float fr[4], const float tl[4], const float tr[4],
+                                          bool& okl, bool& okr) {
+    okl = fl[0] > 0.f && fl[3] > 0.f;
+    okr = fr[0] > 0.f && fr[3] > 0.f;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const float lo = fminf(a[v], b[v]), hi = fmaxf(a[v], b[v]);
+        const float m = ca[v] * cb[v] > 0.f ? 0.25f * fminf(fabsf(ca[v]), fabsf(cb[v])) : 0.f;
+        const float dl = m + tl[v], dr = m + tr[v];
+        okl = okl && fl[v] >= lo - dl && fl[v] <= hi + dl;
+        okr = okr && fr[v] >= lo - dr && fr[v] <= hi + dr;
+    }
+}
+__device__ __forceinline__ void tol_of(float rho, float a0, float p, float t[4]) {
+    t[0] = FLOOR * rho;
+    t[1] = FLOOR * a0;
+    t[2] = FLOOR * a0;
+    t[3] = FLOOR * p;
+}
+
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// The Riemann flux through one face from the lab-frame primitives (rho, u, v, p) of its
+// left / lower (L) and right / upper (R) sides.
+template <int FLUX, int DIR>
+__device__ __forceinline__ fast::C4<float> face_flux_t(float gm, float igm1, float gg, const float L[4], const float R[4]) {
+    const fast::Prep pl = fast::prep_prim(L[0], L[1], L[2], L[3], gm, igm1);
+    const fast::Prep pr = fast::prep_prim(R[0], R[1], R[2], R[3], gm, igm1);
+    return fast::flux<FLUX, DIR>(pl, pr, gg);
+}
+__device__ __forceinline__ fast::C4<float> face_flux(const FaceArgs& a, int axis, const float L[4], const float R[4]) {
+    const float gm = a.gamma, igm1 = 1.f / (a.gamma - 1.f), gg = gm * igm1;
+    if (axis == 0)
+        return a.flux == 0 ? face_flux_t<0, 0>(gm, igm1, gg, L, R)
+                           : (a.flux == 1 ? face_flux_t<1, 0>(gm, igm1, gg, L, R) : face_flux_t<2, 0>(gm, igm1, gg, L, R));
+    return a.flux == 0 ? face_flux_t<0, 1>(gm, igm1, gg, L, R)
+                       : (a.flux == 1 ? face_flux_t<1, 1>(gm, igm1, gg, L, R) : face_flux_t<2, 1>(gm, igm1, gg, L, R));
+}
 
 // Write one face: its primitives (out_mode 0) or its Riemann flux (out_mode 1).  L, R are
 // the lab-frame primitives (rho, u, v, p) of the left / lower and right / upper sides.
 __device__ __forceinline__ void put_face(const FaceArgs& a, int axis, long cell, long np, const float L[4], const float R[4]) {
     if (a.out_mode == 1) {
-        const float gm = a.gamma, igm1 = 1.f / (a.gamma - 1.f), gg = gm * igm1;
-        const fast::Prep pl = fast::prep_prim(L[0], L[1], L[2], L[3], gm, igm1);
-        const fast::Prep pr = fast::prep_prim(R[0], R[1], R[2], R[3], gm, igm1);
-        fast::C4<float> f;
-        if (axis == 0) f = a.flux == 0 ? fast::flux<0, 0>(pl, pr, gg) : (a.flux == 1 ? fast::flux<1, 0>(pl, pr, gg) : fast::flux<2, 0>(pl, pr, gg));
-        else f = a.flux == 0 ? fast::flux<0, 1>(pl, pr, gg) : (a.flux == 1 ? fast::flux<1, 1>(pl, pr, gg) : fast::flux<2, 1>(pl, pr, gg));
+        const fast::C4<float> f = face_flux(a, axis, L, R);
         float* Fo = a.F + (long)(axis * 4) * np + cell;
         Fo[0] = f.r; Fo[np] = f.ru; Fo[2 * np] = f.rv; Fo[3 * np] = f.re;
     } else {
@@ -209,14 +270,19 @@ __device__ __forceinline__ void load_raw(const FaceArgs& a, int axis, int row, i
         for (int v = 0; v < 4; v++) q[m][v] = a.U[v][idx];
     }
 }
+// primitives (rho, u, v, p) of one cell from its conserved variables; ir = 1 / rho
+__device__ __forceinline__ void prim1(const float q[4], float gm1, float w[4], float& ir) {
+    ir = fast::rcp(q[0]);
+    w[0] = q[0];
+    w[1] = q[1] * ir;
+    w[2] = q[2] * ir;
+    w[3] = gm1 * (q[3] - 0.5f * (q[1] * w[1] + q[2] * w[2]));
+}
 __device__ __forceinline__ void to_prim(const float q[6][4], float gm1, float w[6][4]) {
 #pragma unroll
     for (int m = 0; m < 6; m++) {
-        const float ir = fast::rcp(q[m][0]);
-        w[m][0] = q[m][0];
-        w[m][1] = q[m][1] * ir;
-        w[m][2] = q[m][2] * ir;
-        w[m][3] = gm1 * (q[m][3] - 0.5f * (q[m][1] * w[m][1] + q[m][2] * w[m][2]));
+        float ir;
+        prim1(q[m], gm1, w[m], ir);
     }
 }
 __device__ __forceinline__ void load_cells(const FaceArgs& a, int axis, int row, int cc, float q[6][4], float w[6][4]) {
@@ -267,11 +333,103 @@ __device__ __forceinline__ void orient(const float w[6][4], int axis, int sd, fl
 // All passes write the face primitives (16 planes) the stage kernel reads.
 enum { PASS_ALL = 0, PASS_LIST = 2 };
 
+// One face per lane (axis, row, column of its west / south cell; `valid` lanes only count)
+// through the network: both sides' features, the gate (with `bypass`, only for the sides
+// whose ungated value fails the TVD bound with half its relaxation), the checks and the
+// MUSCL fallback.  Returns the lab-frame primitives of the left / lower (L) and right /
+// upper (Rr) face states, whether the face fell back, and which sides the network gated.
+// Wave-uniform control: every lane of the wave calls it together.
+__device__ __forceinline__ void net_face(const FaceArgs& a, const float* skip, uint32_t (*feat)[SLOT], const Regs& R,
+                                         bool bypass, bool own4, int axis, int row, int cc, bool valid, float L[4],
+                                         float Rr[4], bool& fell, bool& need0, bool& need1) {
+    const int lane = lane_id();
+    const int g = lane >> 4, jl = lane & 15;
+    const float gm = a.gamma, gm1 = a.gamma - 1.f;
+    float sig[2][4], cen[2][4], y[2][4], sk[2][4], tol[2][4];
+    need0 = true;
+    need1 = true;   // the side's gate is evaluated by the network
+    {
+        float q[6][4], w[6][4], ws[5][4], f1[4];
+        load_cells(a, axis, row, cc, q, w);
+        orient(w, axis, 0, ws);
+        featurize(ws, gm, skip, own4, feat[lane], sig[0], cen[0], y[0], sk[0], tol[0]);
+        if (bypass) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) f1[v] = fmaf(sig[0][v], y[0][v] + sk[0][v], cen[0][v]);
+            need0 = !side_ok(ws, f1, tol[0], 0.25f);
+        }
+        orient(w, axis, 1, ws);
+        featurize(ws, gm, skip, own4, feat[64 + lane], sig[1], cen[1], y[1], sk[1], tol[1]);
+        if (bypass) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) f1[v] = fmaf(sig[1][v], y[1][v] + sk[1][v], cen[1][v]);
+            need1 = !side_ok(ws, f1, tol[1], 0.25f);
+        }
+    }
+    float fo[2][4];
+    bool ok = true;
+    float q[6][4], w[6][4];
+    {
+        wave_sync();
+        const unsigned long long nm0 = __ballot(valid && need0), nm1 = __ballot(valid && need1);
+        f32x4 o0 = {0.f, 0.f, 0.f, 0.f}, o1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int b = 0; b < 8; b++) {
+            const int sd = b >> 2, qd = b & 3;
+            if ((((sd == 0 ? nm0 : nm1) >> (16 * qd)) & 0xffffull) == 0ull) continue;   // no side needs the gate
+            const bf16x8 bx = __builtin_bit_cast(bf16x8, *(const u32x4*)&feat[sd * 64 + qd * 16 + jl][4 * g]);
+            const f32x4 d3 = mlp16(R, bx);
+            if (g == qd) {
+                if (sd == 0) o0 = d3; else o1 = d3;
+            }
+        }
+        wave_sync();
+        // face values (oriented), checks
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const float b3 = skip[SK_B3 + v];
+            const float p0 = need0 ? gate(o0[v] + b3) : 1.f, p1 = need1 ? gate(o1[v] + b3) : 1.f;
+            fo[0][v] = fmaf(sig[0][v], fmaf(p0, sk[0][v], y[0][v]), cen[0][v]);
+            fo[1][v] = fmaf(sig[1][v], fmaf(p1, sk[1][v], y[1][v]), cen[1][v]);
+        }
+        load_cells(a, axis, row, cc, q, w);
+        float ws[5][4];
+        orient(w, axis, 0, ws);
+        ok = side_ok(ws, fo[0], tol[0]);
+        orient(w, axis, 1, ws);
+        ok = side_ok(ws, fo[1], tol[1]) && ok;
+    }
+    {
+        const float unl = fo[0][1], unr = -fo[1][1];
+        L[0] = fo[0][0]; L[3] = fo[0][3];
+        Rr[0] = fo[1][0]; Rr[3] = fo[1][3];
+        L[1] = axis == 0 ? unl : fo[0][2];
+        L[2] = axis == 0 ? fo[0][2] : unl;
+        Rr[1] = axis == 0 ? unr : fo[1][2];
+        Rr[2] = axis == 0 ? fo[1][2] : unr;
+    }
+    if (!ok) {
+        // the solver's MUSCL: MC-limited conserved slopes, cells m = -1..2 (indices 1..4)
+        float ql[4], qr[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const float sl = fast::mc(q[3][v] - q[2][v], q[2][v] - q[1][v]);
+            const float sr = fast::mc(q[4][v] - q[3][v], q[3][v] - q[2][v]);
+            ql[v] = fmaf(0.5f, sl, q[2][v]);
+            qr[v] = fmaf(-0.5f, sr, q[3][v]);
+        }
+        const float il = fast::rcp(ql[0]), irr = fast::rcp(qr[0]);
+        L[0] = ql[0]; L[1] = ql[1] * il; L[2] = ql[2] * il;
+        L[3] = gm1 * (ql[3] - 0.5f * (ql[1] * L[1] + ql[2] * L[2]));
+        Rr[0] = qr[0]; Rr[1] = qr[1] * irr; Rr[2] = qr[2] * irr;
+        Rr[3] = gm1 * (qr[3] - 0.5f * (qr[1] * Rr[1] + qr[2] * Rr[2]));
+    }
+    fell = !ok;
+}
+
 template <int PASS>   // PASS_ALL or PASS_LIST (PASS_LIGHT is faces_light_kernel)
 __device__ __forceinline__ void faces_body(const FaceArgs& a, const float* skip, uint32_t (*feat)[SLOT], const Regs& R) {
     const int lane = lane_id(), wv = threadIdx.x >> 6;
-    const int g = lane >> 4, jl = lane & 15;
-    const float gm = a.gamma, gm1 = a.gamma - 1.f;
     const long np = (long)a.nx * a.ny;
     const bool bypass = PASS != PASS_ALL || a.bypass;
     const bool own4 = a.w->own4 != 0;
@@ -297,90 +455,13 @@ __device__ __forceinline__ void faces_body(const FaceArgs& a, const float* skip,
             valid = ci < a.nx;
             cc = valid ? ci : a.nx - 1;
         }
-        float sig[2][4], cen[2][4], y[2][4], sk[2][4], tol[2][4];
-        bool need0 = true, need1 = true;   // the side's gate is evaluated by the network
-        {
-            float q[6][4], w[6][4], ws[5][4], f1[4];
-            load_cells(a, axis, row, cc, q, w);
-            orient(w, axis, 0, ws);
-            featurize(ws, gm, skip, own4, feat[lane], sig[0], cen[0], y[0], sk[0], tol[0]);
-            if (bypass) {
-#pragma unroll
-                for (int v = 0; v < 4; v++) f1[v] = fmaf(sig[0][v], y[0][v] + sk[0][v], cen[0][v]);
-                need0 = !side_ok(ws, f1, tol[0], 0.25f);
-            }
-            orient(w, axis, 1, ws);
-            featurize(ws, gm, skip, own4, feat[64 + lane], sig[1], cen[1], y[1], sk[1], tol[1]);
-            if (bypass) {
-#pragma unroll
-                for (int v = 0; v < 4; v++) f1[v] = fmaf(sig[1][v], y[1][v] + sk[1][v], cen[1][v]);
-                need1 = !side_ok(ws, f1, tol[1], 0.25f);
-            }
-        }
-        const long cell = (long)row * a.nx + cc;
-        float fo[2][4];
-        bool ok = true;
-        float q[6][4], w[6][4];
-        {
-            wave_sync();
-            const unsigned long long nm0 = __ballot(valid && need0), nm1 = __ballot(valid && need1);
-            nnet += __popcll(nm0) + __popcll(nm1);
-            f32x4 o0 = {0.f, 0.f, 0.f, 0.f}, o1 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
-            for (int b = 0; b < 8; b++) {
-                const int sd = b >> 2, qd = b & 3;
-                if ((((sd == 0 ? nm0 : nm1) >> (16 * qd)) & 0xffffull) == 0ull) continue;   // no side needs the gate
-                const bf16x8 bx = __builtin_bit_cast(bf16x8, *(const u32x4*)&feat[sd * 64 + qd * 16 + jl][4 * g]);
-                const f32x4 d3 = mlp16(R, bx);
-                if (g == qd) {
-                    if (sd == 0) o0 = d3; else o1 = d3;
-                }
-            }
-            wave_sync();
-            // face values (oriented), checks
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const float b3 = skip[SK_B3 + v];
-                const float p0 = need0 ? gate(o0[v] + b3) : 1.f, p1 = need1 ? gate(o1[v] + b3) : 1.f;
-                fo[0][v] = fmaf(sig[0][v], fmaf(p0, sk[0][v], y[0][v]), cen[0][v]);
-                fo[1][v] = fmaf(sig[1][v], fmaf(p1, sk[1][v], y[1][v]), cen[1][v]);
-            }
-            load_cells(a, axis, row, cc, q, w);
-            float ws[5][4];
-            orient(w, axis, 0, ws);
-            ok = side_ok(ws, fo[0], tol[0]);
-            orient(w, axis, 1, ws);
-            ok = side_ok(ws, fo[1], tol[1]) && ok;
-        }
         float L[4], Rr[4];   // lab-frame primitives of the left / right face states
-        {
-            const float unl = fo[0][1], unr = -fo[1][1];
-            L[0] = fo[0][0]; L[3] = fo[0][3];
-            Rr[0] = fo[1][0]; Rr[3] = fo[1][3];
-            L[1] = axis == 0 ? unl : fo[0][2];
-            L[2] = axis == 0 ? fo[0][2] : unl;
-            Rr[1] = axis == 0 ? unr : fo[1][2];
-            Rr[2] = axis == 0 ? fo[1][2] : unr;
-        }
-        if (!ok) {
-            // the solver's MUSCL: MC-limited conserved slopes, cells m = -1..2 (indices 1..4)
-            float ql[4], qr[4];
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const float sl = fast::mc(q[3][v] - q[2][v], q[2][v] - q[1][v]);
-                const float sr = fast::mc(q[4][v] - q[3][v], q[3][v] - q[2][v]);
-                ql[v] = fmaf(0.5f, sl, q[2][v]);
-                qr[v] = fmaf(-0.5f, sr, q[3][v]);
-            }
-            const float il = fast::rcp(ql[0]), irr = fast::rcp(qr[0]);
-            L[0] = ql[0]; L[1] = ql[1] * il; L[2] = ql[2] * il;
-            L[3] = gm1 * (ql[3] - 0.5f * (ql[1] * L[1] + ql[2] * L[2]));
-            Rr[0] = qr[0]; Rr[1] = qr[1] * irr; Rr[2] = qr[2] * irr;
-            Rr[3] = gm1 * (qr[3] - 0.5f * (qr[1] * Rr[1] + qr[2] * Rr[2]));
-        }
-        nfb += __popcll(__ballot(valid && !ok));
+        bool fell, need0, need1;
+        net_face(a, skip, feat, R, bypass, own4, axis, row, cc, valid, L, Rr, fell, need0, need1);
+        nnet += __popcll(__ballot(valid && need0)) + __popcll(__ballot(valid && need1));
+        nfb += __popcll(__ballot(valid && fell));
         ntot += __popcll(__ballot(valid));
-        if (valid) put_face(a, axis, cell, np, L, Rr);
+        if (valid) put_face(a, axis, (long)row * a.nx + cc, np, L, Rr);
     }
     if (a.stats && lane == 0 && (ntot || nnet)) {
         // the wave's own slot (kernels on one stream run one after another)
@@ -418,6 +499,7 @@ __global__ __launch_bounds__(256) void faces_light_kernel(FaceArgs a) {
     const float gm = a.gamma, gm1 = a.gamma - 1.f;
     const long np = (long)a.nx * a.ny;
     constexpr bool own4 = OWN4;
+    const float (&tp)[4][4] = a.taps;
     unsigned long long ntot = 0;
     for (int task = blockIdx.x * 4 + wv; task < a.n_tasks; task += gridDim.x * 4) {
         int axis, row, cc;
@@ -426,33 +508,38 @@ __global__ __launch_bounds__(256) void faces_light_kernel(FaceArgs a) {
         float q[6][4], w[6][4], fo[2][4];
         load_cells(a, axis, row, cc, q, w);
         bool defer = false;
+        if (own4) {
+            // gate open, 4-tap correction, no bias: the normalisation cancels and the face value
+            // is the linear stencil w0 + sum_k t_k (w_k - w0) with t = L + central slope
+            float ws[5][4];
+            orient(w, axis, 0, ws);
+            light_value(ws, tp, fo[0]);
+            float a[4], b[4], ca[4], cb[4], tl[4], tr[4], frp[4];
 #pragma unroll
-        for (int sd = 0; sd < 2; sd++) {
-            float ws[5][4], tol[4];
-            orient(w, axis, sd, ws);
-            if (own4) {
-                // gate open, 4-tap correction, no bias: the normalisation cancels and the face
-                // value is the linear stencil w0 + sum_k t_k (w_k - w0) with t = L + central slope
-                const float a0 = fast::fsqrt(gm * ws[2][3] * fast::rcp(ws[2][0]));
-                const float scale[4] = {ws[2][0], a0, a0, ws[2][3]};
+            for (int v = 0; v < 4; v++) {
+                a[v] = ws[2][v];
+                b[v] = ws[3][v];
+                ca[v] = curvature(ws[1][v], ws[2][v], ws[3][v]);
+                cb[v] = curvature(ws[2][v], ws[3][v], ws[4][v]);
+            }
+            tol_of(ws[2][0], fast::fsqrt(gm * ws[2][3] * fast::rcp(ws[2][0])), ws[2][3], tl);
+            tol_of(ws[3][0], fast::fsqrt(gm * ws[3][3] * fast::rcp(ws[3][0])), ws[3][3], tr);
+            orient(w, axis, 1, ws);
+            light_value(ws, tp, fo[1]);
+            frp[0] = fo[1][0]; frp[1] = -fo[1][1]; frp[2] = fo[1][2]; frp[3] = fo[1][3];
+            bool okl, okr;
+            face_test(a, b, ca, cb, fo[0], frp, tl, tr, okl, okr);
+            defer = !(okl && okr);
+        } else {
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const float c = ws[2][v];
-                    const float* Lv = skip + SK_L + v * NIN;
-                    float d = Lv[v] * (ws[0][v] - c);
-                    d = fmaf(Lv[4 + v] - 0.25f, ws[1][v] - c, d);
-                    d = fmaf(Lv[12 + v] + 0.25f, ws[3][v] - c, d);
-                    d = fmaf(Lv[16 + v], ws[4][v] - c, d);
-                    fo[sd][v] = c + d;
-                    tol[v] = FLOOR * scale[v];
-                }
-            } else {
-                float sig[4], cen[4], y[4], sk[4];
+            for (int sd = 0; sd < 2; sd++) {
+                float ws[5][4], sig[4], cen[4], y[4], sk[4], tol[4];
+                orient(w, axis, sd, ws);
                 featurize<false>(ws, gm, skip, own4, nullptr, sig, cen, y, sk, tol);
 #pragma unroll
                 for (int v = 0; v < 4; v++) fo[sd][v] = fmaf(sig[v], y[v] + sk[v], cen[v]);   // gate fully open
+                defer = defer || !side_ok(ws, fo[sd], tol, 0.25f);
             }
-            defer = defer || !side_ok(ws, fo[sd], tol, 0.25f);
         }
         // defer the faces with a side near the bound (wave-aggregated append)
         defer = defer && valid;
@@ -486,6 +573,380 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     Regs R;
     load_regs(a.w, lane_id(), R);
     faces_body<PASS_LIST>(a, skip, lds[threadIdx.x >> 6], R);
+}
+
+// ---------------------------------------------------------------------------
+// Fused bypass stage (periodic grids, 4-tap correction).  One wave per strip of LIGHT_OUT
+// columns and strip_h rows computes every face of its cells with the open gate, tests both
+// sides against the strict bound, takes the Riemann flux and updates the cell in place of
+// the light pass, the deferred-face pass and the flux-plane stage: the fluxes never leave
+// the registers.  A face with a side that fails the strict bound is deferred as in the
+// light pass; the cells on both sides of it are then left to pending_kernel, which gets
+// the cell and a mask of its deferred faces through the list, and the fluxes of the
+// cell's other faces through the flux planes.  Every value that reaches the state is
+// computed by the expressions of the separate passes (light_value, face_test, face_flux_t, the update
+// with its multiply-adds written out), so the two paths agree bit for bit.
+// ---------------------------------------------------------------------------
+static_assert(LIGHT_OUT == WAVE - 6, "x faces need the cells -2..+3 of the wave");
+
+__device__ __forceinline__ float lane_from_left(float x) {    // lane i gets lane i-1's x (lane 0: 0)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_from_right(float x) {   // lane i gets lane i+1's x (lane 63: 0)
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ int lane_from_left_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int lane_from_right_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, true); }
+
+
+// RK coefficients and dt of a stage (the flux-plane stage's)
+struct StageCoef { double dtd; float dtix, dtiy, ca, cb; bool fuse; };
+__device__ __forceinline__ StageCoef stage_coef(const StageArgs& a) {
+    StageCoef k;
+    k.dtd = a.fixed_dt > 0.0 ? a.fixed_dt : a.cfl / a.clock->lam[a.parity];
+    const float dt = (float)k.dtd;
+    k.fuse = a.fuse_cfl && (a.euler || a.stage == 2);
+    if (a.euler || a.stage == 0) { k.ca = 0.f; k.cb = 1.f; }
+    else if (a.stage == 1) { k.ca = 0.75f; k.cb = 0.25f; }
+    else { k.ca = 1.f / 3.f; k.cb = 2.f / 3.f; }
+    k.dtix = dt * (float)a.inv_x;
+    k.dtiy = dt * (float)a.inv_y;
+    return k;
+}
+// new state of one variable from its four face fluxes (west, east, south, north), the
+// stage input u and U^n u0; the flux-plane stage's expression
+__device__ __forceinline__ float cell_update(const StageCoef& k, float fw, float fe, float gs, float gn, float u, float u0) {
+    const float d = fmaf(fw - fe, k.dtix, (gs - gn) * k.dtiy);
+    return k.ca != 0.f ? fmaf(k.cb, (u - u0) + d, u0) : u + d;
+}
+
+// One workgroup of LIGHT_ROWS + 1 waves per tile of LIGHT_OUT columns by LIGHT_ROWS rows j0..
+// (grid-stride over tiles).  Phase A: wave w builds the face (j0-1+w, j0+w) whole (the north side
+// of its lower row, the south side of its upper row, the Riemann flux) and puts the flux in LDS;
+// waves w < LIGHT_ROWS also build the x faces of row j0+w.  Phase B: wave w updates row j0+w from
+// the LDS fluxes of faces w and w+1.  No state runs from row to row and no side is built twice,
+// so a 512^2 stage is 1152 workgroups of short waves.  (Folding the CFL maxima in the last
+// workgroup to finish, through a counter every workgroup increments, measured 30 us slower per
+// stage than the separate reduction launch: same-address atomics from workgroups that finish
+// together serialise.)
+template <int FLUX>
+__global__ __launch_bounds__(64 * (LIGHT_ROWS + 1)) void stage_light_kernel(StageArgs a, FaceArgs fa) {
+    constexpr int NW = LIGHT_ROWS + 1;   // waves per workgroup
+    __shared__ float4 sG[NW][WAVE];      // flux through face (j0-1+w, j0+w)
+    __shared__ int sFin[NW][WAVE];       // ... and whether it is final
+    const int lane = lane_id();
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int wave = blockIdx.x * NW + wv;
+    const int nx = a.nx, ny = a.ny;
+    const long np = (long)nx * ny;
+    const int n_tiles = a.n_strips_x * a.n_strips_y;
+    const StageCoef k = stage_coef(a);
+    if (wave == 0 && lane == 0 && a.stage == 0) { a.clock->dt = k.dtd; a.clock->lam[a.parity ^ 1] = 0.0; }
+    const float gm = fa.gamma, gm1 = fa.gamma - 1.f, igm1 = 1.f / (fa.gamma - 1.f), gg = gm * igm1;
+    const float (&tp)[4][4] = fa.taps;
+    const float* U[4] = {(const float*)a.uin[0], (const float*)a.uin[1], (const float*)a.uin[2], (const float*)a.uin[3]};
+    const float* U0[4] = {(const float*)a.u0[0], (const float*)a.u0[1], (const float*)a.u0[2], (const float*)a.u0[3]};
+    const bool use0 = k.ca != 0.f;   // the combine reads U^n
+    float* O[4] = {(float*)a.uout[0], (float*)a.uout[1], (float*)a.uout[2], (float*)a.uout[3]};
+    float lam_max = 0.f;
+    int bad = 0;
+    unsigned long long ntot = 0;
+
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int sxi = t % a.n_strips_x, syi = t / a.n_strips_x;
+        const int c = sxi * LIGHT_OUT - 3 + lane;   // lanes 3..60 are the tile's columns
+        int cq = c % nx;
+        if (cq < 0) cq += nx;
+        const bool out_lane = lane >= 3 && lane < WAVE - 3 && c < nx;
+        const int cw = c > 0 ? c - 1 : nx - 1;      // west neighbour (output lanes)
+        const int j0 = syi * LIGHT_ROWS, nrows = min(LIGHT_ROWS, ny - j0);
+        const int r = j0 + wv;                      // the wave's row: face (r-1, r), row r if wv < nrows
+        auto row_index = [&](int rr) -> long {
+            rr %= ny;
+            if (rr < 0) rr += ny;
+            return (long)rr * nx + cq;
+        };
+        float uq[4], u0[4], Fe[4], Fw[4];
+        bool finE = false, finW = false;
+        // ---- phase A ----
+        if (wv <= nrows) {
+            // rows r-3..r+2: primitives and sound speed
+            float wr[6][4], a0w[6];
+            {
+                float q[6][4];
+#pragma unroll
+                for (int m = 0; m < 6; m++) {
+                    const long im = row_index(r - 3 + m);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) q[m][v] = U[v][im];
+                }
+#pragma unroll
+                for (int m = 0; m < 6; m++) {
+                    float ir;
+                    prim1(q[m], gm1, wr[m], ir);
+                    a0w[m] = fast::fsqrt(gm * wr[m][3] * ir);
+                }
+            }
+            if (wv < nrows) {
+                const long iu = row_index(r);
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    uq[v] = U[v][iu];
+                    u0[v] = use0 ? U0[v][iu] : uq[v];
+                }
+            }
+            {
+                // face (r-1, r): the north side of row r-1 (rows r-3..r+1 in the frame (rho, v, u, p))
+                // and the south side of row r (rows r+2..r-2, oriented -y: (rho, -v, u, p))
+                float ws[5][4], fN[4], fS[4], fSp[4], a[4], b[4], ca[4], cb[4], tl[4], tr[4];
+#pragma unroll
+                for (int m = 0; m < 5; m++) {
+                    ws[m][0] = wr[m][0]; ws[m][1] = wr[m][2]; ws[m][2] = wr[m][1]; ws[m][3] = wr[m][3];
+                }
+                light_value(ws, tp, fN);
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    a[v] = ws[2][v];
+                    b[v] = ws[3][v];
+                    ca[v] = curvature(ws[1][v], ws[2][v], ws[3][v]);
+                    cb[v] = curvature(ws[2][v], ws[3][v], ws[4][v]);
+                }
+                tol_of(wr[2][0], a0w[2], wr[2][3], tl);
+                tol_of(wr[3][0], a0w[3], wr[3][3], tr);
+#pragma unroll
+                for (int m = 0; m < 5; m++) {
+                    ws[m][0] = wr[5 - m][0]; ws[m][1] = -wr[5 - m][2]; ws[m][2] = wr[5 - m][1]; ws[m][3] = wr[5 - m][3];
+                }
+                light_value(ws, tp, fS);
+                fSp[0] = fS[0]; fSp[1] = -fS[1]; fSp[2] = fS[2]; fSp[3] = fS[3];
+                bool okl, okr;
+                face_test(a, b, ca, cb, fN, fSp, tl, tr, okl, okr);
+                const float L[4] = {fN[0], fN[2], fN[1], fN[3]};
+                const float R[4] = {fS[0], fS[2], -fS[1], fS[3]};
+                const fast::C4<float> g = face_flux_t<FLUX, 1>(gm, igm1, gg, L, R);
+                sG[wv][lane] = make_float4(g.r, g.ru, g.rv, g.re);
+                sFin[wv][lane] = okl && okr ? 1 : 0;
+            }
+            if (wv < nrows) {
+                // x sides of row r from the lane's neighbours -2..+2, and the face east of the lane
+                // (its right side and test from lane + 1's values)
+                float l1[4], l2[4], r1[4], r2[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    l1[v] = lane_from_left(wr[3][v]);
+                    l2[v] = lane_from_left(l1[v]);
+                    r1[v] = lane_from_right(wr[3][v]);
+                    r2[v] = lane_from_right(r1[v]);
+                }
+                const float a0r = lane_from_right(a0w[3]);
+                float fE[4], fW[4];
+                {
+                    float ws[5][4];
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {   // frame (rho, u, v, p), columns -2..+2
+                        ws[0][v] = l2[v]; ws[1][v] = l1[v]; ws[2][v] = wr[3][v]; ws[3][v] = r1[v]; ws[4][v] = r2[v];
+                    }
+                    light_value(ws, tp, fE);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {   // oriented -x: (rho, -u, v, p), columns +2..-2
+                        const float s0 = v == 1 ? -r2[v] : r2[v], s1 = v == 1 ? -r1[v] : r1[v], s2 = v == 1 ? -wr[3][v] : wr[3][v];
+                        const float s3 = v == 1 ? -l1[v] : l1[v], s4 = v == 1 ? -l2[v] : l2[v];
+                        ws[0][v] = s0; ws[1][v] = s1; ws[2][v] = s2; ws[3][v] = s3; ws[4][v] = s4;
+                    }
+                    light_value(ws, tp, fW);
+                }
+                const float fWr[4] = {lane_from_right(fW[0]), lane_from_right(fW[1]), lane_from_right(fW[2]),
+                                      lane_from_right(fW[3])};
+                const float R[4] = {fWr[0], -fWr[1], fWr[2], fWr[3]};
+                bool okl, okr;
+                {
+                    float ca[4], cb[4], tl[4], tr[4];
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        ca[v] = curvature(l1[v], wr[3][v], r1[v]);
+                        cb[v] = curvature(wr[3][v], r1[v], r2[v]);
+                    }
+                    tol_of(wr[3][0], a0w[3], wr[3][3], tl);
+                    tol_of(r1[0], a0r, r1[3], tr);
+                    face_test(wr[3], r1, ca, cb, fE, R, tl, tr, okl, okr);
+                }
+                const fast::C4<float> f = face_flux_t<FLUX, 0>(gm, igm1, gg, fE, R);
+                Fe[0] = f.r; Fe[1] = f.ru; Fe[2] = f.rv; Fe[3] = f.re;
+                finE = okl && okr;
+#pragma unroll
+                for (int v = 0; v < 4; v++) Fw[v] = lane_from_left(Fe[v]);
+                finW = lane_from_left_i(finE ? 1 : 0) != 0;
+            }
+        }
+        __syncthreads();
+        // ---- phase B: the update of row r from the faces (r-1, r) and (r, r+1) ----
+        if (wv < nrows) {
+            float Gs[4], Gn[4];
+            const float4 gs = sG[wv][lane], gn = sG[wv + 1][lane];
+            Gs[0] = gs.x; Gs[1] = gs.y; Gs[2] = gs.z; Gs[3] = gs.w;
+            Gn[0] = gn.x; Gn[1] = gn.y; Gn[2] = gn.z; Gn[3] = gn.w;
+            const bool finS = sFin[wv][lane] != 0, finN = sFin[wv + 1][lane] != 0;
+            const long qc = (long)r * nx + c;
+            const unsigned mask = (finW ? 0u : 1u) | (finE ? 0u : 2u) | (finS ? 0u : 4u) | (finN ? 0u : 8u);
+            const bool pend = out_lane && mask != 0u;
+            ntot += __popcll(__ballot(out_lane && finE)) + __popcll(__ballot(out_lane && finN));
+            if (out_lane && !pend) {
+                float o[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    o[v] = cell_update(k, Fw[v], Fe[v], Gs[v], Gn[v], uq[v], u0[v]);
+                    O[v][qc] = o[v];
+                }
+                if (k.fuse) {
+                    float l;
+                    if (lam_cell_f32<false>(a, o[0], o[1], o[2], o[3], l)) lam_max = fmaxf(lam_max, l);
+                    else bad++;
+                }
+            }
+            const unsigned long long pm = __ballot(pend);
+            if (pm) {
+                // the cell waits for its deferred faces: list it with their mask and leave the
+                // fluxes of its final faces in the planes
+                unsigned base = 0u;
+                if (lane == 0) base = atomicAdd(fa.n_list, (unsigned)__popcll(pm));
+                base = __shfl(base, 0, WAVE);
+                if (pend) {
+                    fa.list[base + (unsigned)__popcll(pm & ((1ull << lane) - 1ull))] = ((unsigned)qc << 4) | mask;
+                    const long qw = (long)r * nx + cw, qs = (long)(r > 0 ? r - 1 : ny - 1) * nx + c;
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        if (finW) fa.F[v * np + qw] = Fw[v];
+                        if (finE) fa.F[v * np + qc] = Fe[v];
+                        if (finS) fa.F[(4 + v) * np + qs] = Gs[v];
+                        if (finN) fa.F[(4 + v) * np + qc] = Gn[v];
+                    }
+                }
+            }
+        }
+        __syncthreads();   // the next tile rewrites the flux slots
+    }
+    if (fa.stats && lane == 0 && ntot) fa.stats[3 * wave + 1] += ntot;   // the wave's own slot
+    if (wave == 0 && lane == 0 && (a.euler || a.stage == 2)) {
+        a.clock->time += k.dtd;
+        a.clock->step += 1;
+    }
+    if (k.fuse) {
+        const double wm = wave_max((double)lam_max);
+        const long long b = wave_sum_ll((long long)bad);
+        if (lane == 0) {
+            a.lam_part[wave] = wm;   // one slot per wave, reduced by lam_reduce_kernel
+            if (b) atomicAdd((unsigned long long*)&a.clock->bad, (unsigned long long)b);
+        }
+    }
+}
+
+// The cells stage_light_kernel left: per list entry (cell << 4 | mask of deferred faces,
+// bits west, east, south, north) the deferred faces through the network as the deferred-face
+// pass evaluates them (net_face, face_flux), the others from the flux planes, then the update
+// and the CFL.  A face deferred between two listed cells is evaluated for each, with the same
+// operands and instructions, so both see the same flux.  Persistent grid; the last workgroup
+// to finish clears the list count (n_list[1] counts finished workgroups), so the next stage
+// starts from an empty list and the sequence can be captured in a graph.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void pending_kernel(FaceArgs fa, StageArgs a) {
+    const unsigned n = *fa.n_list;
+    if (n == 0u) return;    // no face deferred (the usual case on smooth flow)
+    __shared__ __attribute__((aligned(16))) uint32_t lds[4][128][SLOT];
+    __shared__ float skip[SK_N];
+    stage_skip(fa.w, skip);
+    init_feat_slots(lds[threadIdx.x >> 6]);
+    Regs R;
+    load_regs(fa.w, lane_id(), R);
+    uint32_t (*feat)[SLOT] = lds[threadIdx.x >> 6];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int nx = fa.nx, ny = fa.ny;
+    const long np = (long)nx * ny;
+    const bool own4 = fa.w->own4 != 0;
+    const StageCoef k = stage_coef(a);
+    const bool use0 = k.ca != 0.f;
+    unsigned long long nfb = 0, ntot = 0, nnet = 0;
+    float lam_max = 0.f;
+    int bad = 0;
+    const int n_tasks = (int)((n + 63u) / 64u);
+    for (int task = blockIdx.x * 4 + wv; task < n_tasks; task += gridDim.x * 4) {
+        const unsigned e = (unsigned)task * 64u + (unsigned)lane;
+        const bool valid = e < n;
+        const unsigned ent = valid ? fa.list[e] : 0u;
+        const long qc = (long)(ent >> 4);
+        const unsigned mask = ent & 15u;
+        const int j = (int)(qc / nx), i = (int)(qc - (long)j * nx);
+        const int iw = i > 0 ? i - 1 : nx - 1, js = j > 0 ? j - 1 : ny - 1;
+        float Fw[4] = {0.f, 0.f, 0.f, 0.f}, Fe[4] = {0.f, 0.f, 0.f, 0.f}, Gs[4] = {0.f, 0.f, 0.f, 0.f},
+              Gn[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int d = 0; d < 4; d++) {   // west, east, south, north
+            const int axis = d >> 1;
+            const int row = d == 2 ? js : j, cc = d == 0 ? iw : i;
+            const bool need = valid && ((mask >> d) & 1u) != 0u;
+            float f[4] = {0.f, 0.f, 0.f, 0.f};
+            if (__ballot(need)) {
+                float L[4], Rr[4];
+                bool fell, need0, need1;
+                net_face(fa, skip, feat, R, true, own4, axis, row, cc, need, L, Rr, fell, need0, need1);
+                const bool own = need && (d & 1) != 0;   // the east and north faces are the cell's own
+                nnet += __popcll(__ballot(own && need0)) + __popcll(__ballot(own && need1));
+                nfb += __popcll(__ballot(own && fell));
+                ntot += __popcll(__ballot(own));
+                const fast::C4<float> ff = face_flux(fa, axis, L, Rr);
+                f[0] = ff.r; f[1] = ff.ru; f[2] = ff.rv; f[3] = ff.re;
+            }
+            if (valid && !need) {
+                const long fi = (long)row * nx + cc;
+#pragma unroll
+                for (int v = 0; v < 4; v++) f[v] = fa.F[(axis * 4 + v) * np + fi];
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                if (d == 0) Fw[v] = f[v];
+                else if (d == 1) Fe[v] = f[v];
+                else if (d == 2) Gs[v] = f[v];
+                else Gn[v] = f[v];
+            }
+        }
+        if (valid) {
+            float o[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const float u = ((const float*)a.uin[v])[qc];
+                const float u0 = use0 ? ((const float*)a.u0[v])[qc] : u;
+                o[v] = cell_update(k, Fw[v], Fe[v], Gs[v], Gn[v], u, u0);
+                ((float*)a.uout[v])[qc] = o[v];
+            }
+            if (k.fuse) {
+                float l;
+                if (lam_cell_f32<false>(a, o[0], o[1], o[2], o[3], l)) lam_max = fmaxf(lam_max, l);
+                else bad++;
+            }
+        }
+    }
+    if (fa.stats && lane == 0 && (ntot || nnet)) {
+        unsigned long long* st = fa.stats + 3 * (blockIdx.x * 4 + wv);
+        st[0] += nfb;
+        st[1] += ntot;
+        st[2] += nnet;
+    }
+    if (k.fuse) {
+        const double w = wave_max((double)lam_max);
+        const long long b = wave_sum_ll((long long)bad);
+        if (lane == 0) {
+            if (w > 0.0) atomic_max_pos_double(&a.clock->lam[a.parity ^ 1], w);
+            if (b) atomicAdd((unsigned long long*)&a.clock->bad, (unsigned long long)b);
+        }
+    }
+    // every thread of the block has read the count; the last block clears it
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned t = atomicAdd(fa.n_list + 1, 1u);
+        if (t == gridDim.x - 1u) {
+            fa.n_list[0] = 0u;
+            fa.n_list[1] = 0u;
+        }
+    }
 }
 
 // The matrix-core MLP alone on n given feature rows X (n x 20): Y = base + skip + MLP.
@@ -933,6 +1394,21 @@ hipError_t launch_mlp_faces_bypass(const mlp::FaceArgs& a, hipStream_t st) {
     if (a.own4) mlp::faces_light_kernel<true><<<lb, 256, 0, st>>>(a);
     else mlp::faces_light_kernel<false><<<lb, 256, 0, st>>>(a);
     mlp::faces_list_kernel<<<128, 256, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+// Fused bypass stage: stage_light_kernel over a.n_strips_x x a.n_strips_y tiles of LIGHT_OUT
+// columns by LIGHT_ROWS rows on `blocks` workgroups of LIGHT_ROWS + 1 waves (at most STAT_SLOTS
+// waves: one statistics slot and one CFL slot each), then pending_kernel for the cells it
+// listed.  fa.list / fa.n_list: the pending cells and their count, then pending_kernel's
+// finished-workgroup counter (both zero between stages).
+hipError_t launch_mlp_light_stage(const StageArgs& a, const mlp::FaceArgs& fa, int blocks, hipStream_t st) {
+    switch (fa.flux) {
+    case 0: mlp::stage_light_kernel<0><<<blocks, 64 * (mlp::LIGHT_ROWS + 1), 0, st>>>(a, fa); break;
+    case 1: mlp::stage_light_kernel<1><<<blocks, 64 * (mlp::LIGHT_ROWS + 1), 0, st>>>(a, fa); break;
+    default: mlp::stage_light_kernel<2><<<blocks, 64 * (mlp::LIGHT_ROWS + 1), 0, st>>>(a, fa); break;
+    }
+    mlp::pending_kernel<<<128, 256, 0, st>>>(fa, a);
     return hipGetLastError();
 }
 

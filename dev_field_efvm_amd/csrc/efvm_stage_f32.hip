@@ -20,6 +20,7 @@
 //    periodic/reflecting case carries none of their registers.
 #include "efvm_stage.h"
 #include "efvm_fast.h"
+#include "efvm_cfl_f32.h"
 
 namespace efvm {
 
@@ -109,34 +110,6 @@ __device__ __forceinline__ F4 half_slope4(const F4& dr, const F4& dl) {
 }
 __device__ __forceinline__ F4 axpy4(const F4& u, float h, const F4& s) {
     return {fmaf(h, s.r, u.r), fmaf(h, s.ru, u.ru), fmaf(h, s.rv, u.rv), fmaf(h, s.re, u.re)};
-}
-
-// CFL spectral radius of one FP32 cell (crates/cfd_solver/src/lib.rs:653-666, nz = 1), with
-// the operation order fixed by explicit fmaf so that the fused epilogues and the standalone
-// FP32 scan produce the same value: a run restarted from a checkpoint takes the same dt.
-// Returns false for a non-physical cell (rho <= 0, p <= 0 or a non-finite lambda).
-template <bool VISC>
-__device__ __forceinline__ bool lam_cell_f32(const StageArgs& a, float o0, float o1, float o2, float o3, float& l) {
-    const float g = (float)a.gamma, gm1 = (float)(a.gamma - 1.0);
-    const float idx = (float)(1.0 / a.dx), idy = (float)(1.0 / a.dy), idz = (float)(1.0 / a.dz);
-    const float ir = fast::rcp(o0);
-    const float vu = fabsf(o1 * ir), vv = fabsf(o2 * ir);
-    const float q2 = fmaf(o1, o1, o2 * o2);
-    const float p = gm1 * fmaf(-0.5f * q2, ir, o3);
-    const float snd = fast::fsqrt(g * p * ir);
-    l = fmaf(vu + snd, idx, fmaf(vv + snd, idy, snd * idz));
-    if (VISC) {
-        const float vsum = (float)(1.0 / (a.dx * a.dx) + 1.0 / (a.dy * a.dy) + 1.0 / (a.dz * a.dz));
-        const float t = p * ir * (float)(1.0 / a.r_gas);
-        const float mu = a.visc_const ? (float)a.mu : (float)ref::sutherland((double)t);
-        l = fmaf(2.f * mu * ir, vsum, l);
-    }
-    // positive density and pressure and a finite lambda, tested on the bit patterns (the unit is
-    // compiled assuming no NaNs, so a float compare could be folded): a positive float is a
-    // positive integer, and a finite non-negative lambda lies below the pattern of +inf (NaNs
-    // and negative values compare above it as unsigned integers)
-    return (int)__builtin_bit_cast(unsigned, o0) > 0 && (int)__builtin_bit_cast(unsigned, p) > 0 &&
-           __builtin_bit_cast(unsigned, l) < 0x7f800000u;
 }
 
 struct Prim3 { float u, v, t; };
@@ -1259,11 +1232,13 @@ __global__ __launch_bounds__(256) void stage_f32_flux_kernel(StageArgs a, const 
         for (int v = 0; v < 4; v++) {
             const float* Fx = F + v * np;
             const float* Fy = F + (4 + v) * np;
-            const float d = (Fx[qw] - Fx[q]) * dtix + (Fy[qs] - Fy[q]) * dtiy;
+            // every multiply-add written out, so that the learned stage kernels of the other
+            // unit (bitwise this kernel on the flux planes) round each cell alike
+            const float d = fmaf(Fx[qw] - Fx[q], dtix, (Fy[qs] - Fy[q]) * dtiy);
             const float u = ((const float*)a.uin[v])[q];
             if (ca != 0.f) {   // increment form of the RK combine, as in stage_f32_strip
                 const float a0 = ((const float*)a.u0[v])[q];
-                o[v] = a0 + cb * ((u - a0) + d);
+                o[v] = fmaf(cb, (u - a0) + d, a0);
             } else {
                 o[v] = u + d;
             }

@@ -365,6 +365,20 @@ class Context:
         self._chk(self._L.efvm_set_learned(self._p, None if mlp is None else mlp._p))
         self._chk(self._L.efvm_set_learned_bypass(self._p, 1 if bypass else 0))
 
+    def set_learned_fused(self, allow=True):
+        """Bypass stages with the 4-tap correction as one fused stage kernel (faces, fluxes and
+        update in one pass; cells next to a deferred face finished by a second pass), bitwise
+        the separate passes."""
+        _bind_mlp(self._L)
+        self._chk(self._L.efvm_set_learned_fused(self._p, 1 if allow else 0))
+
+    def learned_fused(self):
+        """(allowed, whether the next stage takes the fused path)."""
+        _bind_mlp(self._L)
+        a, f = C.c_int(), C.c_int()
+        self._chk(self._L.efvm_get_learned_fused(self._p, C.byref(a), C.byref(f)))
+        return bool(a.value), bool(f.value)
+
     def learned_stats(self, reset=False):
         """(faces that fell back to MUSCL, faces evaluated, face sides the network gated)."""
         _bind_mlp(self._L)
@@ -560,6 +574,8 @@ def _bind_mlp(L):
         "efvm_set_learned": [vp, vp],
         "efvm_mlp_infer_stats": [vp, C.c_int, C.POINTER(C.c_ulonglong)],
         "efvm_set_learned_bypass": [vp, C.c_int],
+        "efvm_set_learned_fused": [vp, C.c_int],
+        "efvm_get_learned_fused": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "efvm_learned_faces": [vp, pf, sz],
     }
     for name, args in sig.items():
@@ -577,7 +593,8 @@ EXPORTS += [
     "efvm_mlp_create", "efvm_mlp_destroy", "efvm_mlp_last_error", "efvm_mlp_load", "efvm_mlp_save",
     "efvm_mlp_add_snapshot", "efvm_mlp_add_snapshot_pair", "efvm_mlp_add_samples", "efvm_mlp_get_samples", "efvm_mlp_num_samples",
     "efvm_mlp_clear_samples", "efvm_mlp_train_step", "efvm_mlp_gradient", "efvm_mlp_fit_linear", "efvm_mlp_infer", "efvm_set_learned",
-    "efvm_mlp_infer_stats", "efvm_learned_faces", "efvm_set_learned_bypass",
+    "efvm_mlp_infer_stats", "efvm_learned_faces", "efvm_set_learned_bypass", "efvm_set_learned_fused",
+    "efvm_get_learned_fused",
 ]
 
 

@@ -333,6 +333,16 @@ efvm_status efvm_set_learned(efvm_ctx* ctx, efvm_mlp* m);
  * limiter can act; every other side keeps the correction fully applied.  0 (the
  * default): the network runs on every face side. */
 efvm_status efvm_set_learned_bypass(efvm_ctx* ctx, int bypass);
+/* Bypass stages with the 4-tap correction (the fitted network's form) run as one fused
+ * stage kernel: every face's open-gate value, the strict-bound test, the Riemann flux and
+ * the cell update in one pass, the fluxes kept on chip; cells next to a face that fails the
+ * strict bound are finished by a second pass through the network.  The results are bitwise
+ * those of the separate passes (allow = 0: light pass, deferred-face pass, flux-plane
+ * stage).  Default: allowed.  No counterpart in the reference (which has no learned
+ * reconstruction). */
+efvm_status efvm_set_learned_fused(efvm_ctx* ctx, int allow);
+/* allow, and whether the next stage takes the fused path */
+efvm_status efvm_get_learned_fused(const efvm_ctx* ctx, int* allow, int* fits);
 /* out[0] = faces that fell back to MUSCL, out[1] = faces evaluated, out[2] = face
  * sides whose gate the network evaluated (since reset) */
 efvm_status efvm_mlp_infer_stats(efvm_ctx* ctx, int reset, unsigned long long out[3]);

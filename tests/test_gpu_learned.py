@@ -272,6 +272,87 @@ def test_learned_run_conserves_and_replays_from_graph():
     assert fb < 0.05 * tot
 
 
+def rough_own4(seed=6, scale=0.1):
+    """A rough 4-tap network (no correction bias): many faces deferred to the network."""
+    p = own4(random_params(seed, scale=scale))
+    p[M.OFF_BL:M.OFF_BL + 4] = 0.0
+    return p
+
+
+def _bypass_pair(nx, ny, p, seed=7, fixed_dt=None):
+    """Two contexts on one state and one 4-tap network in bypass mode: the fused stage
+    kernel and the separate passes."""
+    n = max(nx, ny)
+    st = np.ascontiguousarray(cases.learned_state(seed, n, sub=1)[:ny, :nx])
+    out = []
+    for fused in (True, False):
+        c = efvm.Context(nx, ny, hi=(10.0 * nx / n, 10.0 * ny / n), dz=10.0 / n, gamma=1.4, flux="hllc", muscl=True,
+                         limiter="mc", precision="fp32", cfl=0.4, seam_mode=1, fixed_dt=fixed_dt)
+        c.set_boundaries(["periodic"] * 4)
+        c.upload(st)
+        c.set_learned(efvm.Mlp(p), bypass=True)
+        c.set_learned_fused(fused)
+        assert c.learned_fused() == (fused, fused)
+        out.append(c)
+    return out
+
+
+def _same(a, b, what):
+    ua, ub = a.download(), b.download()
+    assert np.array_equal(ua, ub), f"{what}: {np.count_nonzero(ua != ub)} values differ, max {np.abs(ua - ub).max():.3e}"
+    ca, cb = a.clock(), b.clock()
+    assert ca == cb, (what, ca, cb)
+
+
+# (nx, ny): a tile narrower than the wave (columns wrap inside it), three tiles across with a
+# ragged last one, a grid smaller than the stencil window in both axes (several periodic images
+# per wave and per window); last tiles of one, two and three rows
+@pytest.mark.parametrize("nx,ny", [(48, 48), (130, 61), (7, 9), (64, 41), (61, 130)])
+@pytest.mark.parametrize("net", ["trained", "rough"])
+def test_fused_bypass_stage_is_bitwise_the_separate_passes(nx, ny, net):
+    """The fused bypass stage (faces, fluxes and update in one kernel, the cells next to a
+    deferred face finished by the pending-cell pass) against the separate passes (light pass,
+    deferred-face pass, flux-plane stage): states, clocks (the fused CFL) and face statistics
+    bitwise equal after every substage of two RK3 steps and after two more whole steps.  The
+    rough network defers many faces, so cells with one to four deferred faces, and deferred
+    faces between two pending cells, all occur."""
+    if net == "trained":
+        if not os.path.exists(learned.WEIGHTS):
+            pytest.skip("no trained weights")
+        p = learned.load_weights()
+    else:
+        p = rough_own4()
+    fz, sep = _bypass_pair(nx, ny, p)
+    for k in range(6):
+        fz.run()
+        sep.run()
+        _same(fz, sep, f"substage {k}")
+    fz.step(2)
+    sep.step(2)
+    _same(fz, sep, "two steps")
+    sf, ss = fz.learned_stats(), sep.learned_stats()
+    print(f"{nx}x{ny} {net}: stats fused {sf} separate {ss}")
+    assert sf == ss
+    assert sf[1] == 4 * 3 * 2 * nx * ny            # every face once per substage
+    if net == "rough":
+        assert sf[2] > 0                           # faces were deferred to the network
+
+
+def test_fused_bypass_stage_graph_replay_and_fixed_dt():
+    """A graph of fused bypass steps replays bitwise the separate passes run eagerly, with the
+    CFL dt and with a fixed dt (the fused CFL compiled out of the stage)."""
+    p = rough_own4(seed=9, scale=0.08)
+    for fixed in (None, 0.004):
+        fz, sep = _bypass_pair(96, 80, p, seed=11, fixed_dt=fixed)
+        fz.graph_capture(2)
+        for rep in range(3):
+            fz.graph_launch()
+            fz.synchronize()
+            sep.step(2)
+            _same(fz, sep, f"replay {rep}, fixed dt {fixed}")
+        assert fz.learned_stats() == sep.learned_stats()
+
+
 def test_trained_network_beats_muscl():
     """What the learned reconstruction as a whole (the cubic-exact linear correction and the
     network's gate) gains over plain MUSCL: learned 128^2 vs MUSCL 128^2, both against MUSCL
