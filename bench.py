@@ -274,9 +274,10 @@ def bench_learned(args):
 
 
 def bench_learned_stage(args):
-    """Per-stage GPU time of the learned reconstruction at n^2 (face-state MLP kernel + face
-    stage kernel) next to plain MUSCL at n^2 and 4n^2, hipGraph replays of 10 steps (the
-    short run to put under a kernel-trace profiler)."""
+    """Per-stage GPU time of the learned reconstruction at n^2 (network on every face side; the
+    bypass mode as the fused stage kernel and as its separate passes) next to plain MUSCL at
+    n^2 and 4n^2, hipGraph replays of 10 steps (the short run to put under a kernel-trace
+    profiler)."""
     import torch
     from dev_field_efvm_amd import learned
     from dev_field_efvm_amd.autotune import tune_context
@@ -285,11 +286,13 @@ def bench_learned_stage(args):
     net = Mlp(learned.load_weights())
     out = {}
     for name, n, use in (("learned", args.learned_n, 1), ("learned_bypass", args.learned_n, 2),
-                         ("muscl", args.learned_n, 0), ("muscl_fine", 4 * args.learned_n, 0)):
+                         ("learned_bypass_separate", args.learned_n, 3), ("muscl", args.learned_n, 0),
+                         ("muscl_fine", 4 * args.learned_n, 0)):
         c = learned.periodic_context(n)
         c.upload(learned.initial_state(args.learned_seed, n))
         if use:
-            c.set_learned(net, bypass=use == 2)
+            c.set_learned(net, bypass=use >= 2)
+            c.set_learned_fused(use == 2)
         else:
             tune_context(c, steps=2, use_cache=False, store=False)
         c.step(2)

@@ -36,7 +36,8 @@ constexpr int TB = 64;                  // samples per training block (one wave)
 constexpr int STAT_SLOTS = 8192;        // waves of the largest face-kernel launch
 constexpr int LIGHT_OUT = 58;            // output columns per tile of the fused bypass stage (a
                                         // wave of 64 columns less the +-3 of the x-face stencil)
-constexpr int LIGHT_ROWS = 3;            // rows per tile (a workgroup of LIGHT_ROWS + 1 waves)
+constexpr int LIGHT_ROWS = 3;            // rows per tile (a workgroup of LIGHT_ROWS + 1 waves; 3 keeps a
+                                        // 512^2 stage's workgroups resident in one round)
 
 // features: x[4k+v], k = 0..4 <-> stencil cells -2..2 (k = 3 is the neighbour across
 // the face), v = rho, u_n, u_t, p.  x = (w_k - w_0) / sigma_v for k != 2 and

@@ -624,11 +624,14 @@ __device__ __forceinline__ float cell_update(const StageCoef& k, float fw, float
 // (grid-stride over tiles).  Phase A: wave w builds the face (j0-1+w, j0+w) whole (the north side
 // of its lower row, the south side of its upper row, the Riemann flux) and puts the flux in LDS;
 // waves w < LIGHT_ROWS also build the x faces of row j0+w.  Phase B: wave w updates row j0+w from
-// the LDS fluxes of faces w and w+1.  No state runs from row to row and no side is built twice,
-// so a 512^2 stage is 1152 workgroups of short waves.  (Folding the CFL maxima in the last
-// workgroup to finish, through a counter every workgroup increments, measured 30 us slower per
-// stage than the separate reduction launch: same-address atomics from workgroups that finish
-// together serialise.)
+// the LDS fluxes of faces w and w+1.  No state runs from row to row and no side is built twice.
+// A stage is latency- and issue-bound at the grid sizes the learned runs use: a 512^2 stage is
+// 1539 workgroups of 4 short waves (79 VGPRs, 6 waves per SIMD), all resident at once; strips
+// marching rows per wave (2 waves per SIMD, one long dependent chain each) measured slower than
+// the separate passes, tiles of 4 or 5 rows took a second round of workgroups.  (Folding the CFL
+// maxima in the last workgroup to finish, through a counter every workgroup increments, measured
+// 30 us slower per stage than the separate reduction launch: same-address atomics from
+// workgroups that finish together serialise.)
 template <int FLUX>
 __global__ __launch_bounds__(64 * (LIGHT_ROWS + 1)) void stage_light_kernel(StageArgs a, FaceArgs fa) {
     constexpr int NW = LIGHT_ROWS + 1;   // waves per workgroup
