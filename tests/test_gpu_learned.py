@@ -338,6 +338,20 @@ def test_fused_bypass_stage_is_bitwise_the_separate_passes(nx, ny, net):
         assert sf[2] > 0                           # faces were deferred to the network
 
 
+def test_fused_bypass_stage_grid_stride_and_long_lists():
+    """A grid whose tiles outnumber the fused kernel's workgroups (1024 x 1000: 6138 tiles on at
+    most 2048 workgroups, each striding over several), with the rough network so that the
+    pending-cell list holds thousands of cells: bitwise the separate passes."""
+    fz, sep = _bypass_pair(1024, 1000, rough_own4(seed=4, scale=0.12), seed=13)
+    for k in range(3):
+        fz.run()
+        sep.run()
+        _same(fz, sep, f"substage {k}")
+    sf, ss = fz.learned_stats(), sep.learned_stats()
+    print("1024x1000 rough: stats", sf)
+    assert sf == ss and sf[2] > 1000
+
+
 def test_fused_bypass_stage_graph_replay_and_fixed_dt():
     """A graph of fused bypass steps replays bitwise the separate passes run eagerly, with the
     CFL dt and with a fixed dt (the fused CFL compiled out of the stage)."""
