@@ -311,7 +311,7 @@ efvm_status run_substage(efvm_ctx* c) {
                                    : launch_ibm_mirror<float>(a, ib, c->n_ghost, c->ghost_idx, c->ghost_img, c->ghost_pre[c->rk_stage ? 1 : 0], c->stream);
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, "ibm mirror");
     }
-    int nw_learned = 0;   // waves of the fused learned stage (its CFL slots)
+    bool fused_light = false;   // the fused learned stage folds its CFL slots itself
     if (c->learned) {
         // face states of the stage input from the network (with the MUSCL fallback), then the stage
         mlp::FaceArgs fa;
@@ -331,6 +331,7 @@ efvm_status run_substage(efvm_ctx* c) {
         fa.list = c->dlist;
         fa.n_list = c->dlist ? c->dlist + 2 * c->ncell : nullptr;
         fa.own4 = c->learned_own4;
+        fa.n_lam_slots = 0;
         std::memcpy(fa.taps, c->learned_taps, sizeof(fa.taps));
         if (learned_fused_fits(c)) {
             // one fused stage kernel (faces, fluxes, update) and the pending-cell pass
@@ -341,7 +342,9 @@ efvm_status run_substage(efvm_ctx* c) {
             fa.n_list = c->dlist + 2 * c->ncell + 1;
             const long tiles = (long)b.n_strips_x * b.n_strips_y;
             const int blocks = (int)std::min<long>(tiles, std::min<long>(mlp::STAT_SLOTS, c->lam_slots) / (mlp::LIGHT_ROWS + 1));
-            nw_learned = (mlp::LIGHT_ROWS + 1) * blocks;
+            // the pending-cell pass folds the stage's per-wave CFL maxima (no reduction launch)
+            fa.n_lam_slots = (mlp::LIGHT_ROWS + 1) * blocks;
+            fused_light = true;
             hipError_t e = launch_mlp_light_stage(b, fa, blocks, c->stream);
             if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("fused learned stage: ") + hipGetErrorString(e));
         } else {
@@ -354,10 +357,9 @@ efvm_status run_substage(efvm_ctx* c) {
     } else if ((st = launch_stage(c, a)) != EFVM_OK) {
         return st;
     }
-    if (a.fuse_cfl && a.lam_part) {
-        // the stage left one CFL maximum per wave: strips, 64 cells of the learned stage, or
-        // the waves of the fused learned stage
-        const int nw = c->learned ? (nw_learned ? nw_learned : (int)((c->ncell + 63) / 64)) : a.n_strips_x * a.n_strips_y;
+    if (a.fuse_cfl && a.lam_part && !fused_light) {
+        // the stage left one CFL maximum per wave: strips, or 64 cells of the learned stage
+        const int nw = c->learned ? (int)((c->ncell + 63) / 64) : a.n_strips_x * a.n_strips_y;
         hipError_t e = launch_lam_reduce(a.lam_part, nw, &c->clock->lam[a.parity ^ 1], c->stream);
         if (e != hipSuccess) return fail(c, EFVM_ERR_HIP, std::string("cfl reduction: ") + hipGetErrorString(e));
     }
@@ -2138,6 +2140,7 @@ efvm_status efvm_learned_faces(efvm_ctx* c, float* out, size_t n) {
     fa.list = nullptr;
     fa.n_list = nullptr;
     fa.own4 = c->learned_own4;
+    fa.n_lam_slots = 0;
     std::memcpy(fa.taps, c->learned_taps, sizeof(fa.taps));
     HIPCHK(c, launch_mlp_faces(fa, c->stream));
     HIPCHK(c, hipMemcpyAsync(out, c->faces, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));

@@ -105,6 +105,8 @@ struct FaceArgs {
     unsigned* list;              // deferred faces of the bypass passes: axis * nx*ny + row * nx + column
     unsigned* n_list;            // their count (device)
     int own4;                    // host copy of w->own4 (selects the light pass's specialisation)
+    int n_lam_slots;             // fused bypass stage: the per-wave CFL maxima pending_kernel folds
+                                 // into the clock on the last stage (0: none)
     float taps[4][4];            // own4: the open-gate value's taps on the stencil cells -2, -1, +1, +2
                                  // per variable, L's own-variable weights with the central slope's
                                  // -1/4 and +1/4 folded into the inner two (uniform, so they sit in

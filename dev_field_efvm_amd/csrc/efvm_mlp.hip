@@ -843,7 +843,8 @@ __global__ __launch_bounds__(64 * (LIGHT_ROWS + 1)) void stage_light_kernel(Stag
     }
 }
 
-// The cells stage_light_kernel left: per list entry (cell << 4 | mask of deferred faces,
+// The cells stage_light_kernel left (and, on the last stage, the fold of its CFL slots): per
+// list entry (cell << 4 | mask of deferred faces,
 // bits west, east, south, north) the deferred faces through the network as the deferred-face
 // pass evaluates them (net_face, face_flux), the others from the flux planes, then the update
 // and the CFL.  A face deferred between two listed cells is evaluated for each, with the same
@@ -852,6 +853,24 @@ __global__ __launch_bounds__(64 * (LIGHT_ROWS + 1)) void stage_light_kernel(Stag
 // starts from an empty list and the sequence can be captured in a graph.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void pending_kernel(FaceArgs fa, StageArgs a) {
     const unsigned n = *fa.n_list;
+    const bool fuse = a.fuse_cfl && (a.euler || a.stage == 2);
+    if (fuse && fa.n_lam_slots > 0) {
+        // the stage kernel's per-wave CFL maxima -> lam[parity ^ 1], in place of a reduction
+        // launch: each workgroup folds its share of the slots (a few per thread) and takes an
+        // atomic maximum, as this kernel's own cells' maxima do below
+        __shared__ double red[4];
+        const int per = (fa.n_lam_slots + (int)gridDim.x - 1) / (int)gridDim.x;
+        const int i0 = (int)blockIdx.x * per, i1 = min(i0 + per, fa.n_lam_slots);
+        double m = 0.0;
+        for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) m = fmax(m, a.lam_part[i]);
+        m = wave_max(m);
+        if (lane_id() == 0) red[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
+            if (m > 0.0) atomic_max_pos_double(&a.clock->lam[a.parity ^ 1], m);
+        }
+    }
     if (n == 0u) return;    // no face deferred (the usual case on smooth flow)
     __shared__ __attribute__((aligned(16))) uint32_t lds[4][128][SLOT];
     __shared__ float skip[SK_N];
