@@ -554,9 +554,9 @@ def main():
     cells = n * n
     value = cells * 3 * args.steps / wall
     if fused:
-        # one step = stage 1 (stage_f32_pair_kernel) + stages 2 and 3 (stage_f32_pair23_kernel)
-        # + the CFL slot fold: the roofline is taken over the step, the unit the RK-stage metric
-        # counts three cell-stages in (128 B per cell algorithmic, 80 B moved)
+        # one step = stage 1 (stage_f32_pair_kernel) + stages 2 and 3 (stage_f32_pair23_kernel,
+        # which also takes the CFL maximum): the roofline is taken over the step, the unit the
+        # RK-stage metric counts three cell-stages in (128 B per cell algorithmic, 80 B moved)
         step_s = ev_ms * 1e-3 / args.steps
         bytes_per_step = BYTES_PER_CELL_STEP[args.precision] * cells
         achieved = bytes_per_step / step_s / 1e9
