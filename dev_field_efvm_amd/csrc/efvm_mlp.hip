@@ -5,6 +5,10 @@
 //    FP32 skip and offsets, the per-face positivity / TVD check with fallback to
 //    the solver's MUSCL (conserved variables, MC limiter), and the final face
 //    primitives for the learned stage kernel.
+//  * faces_light_kernel / faces_list_kernel: the bypass mode as separate passes (open-gate
+//    faces and their fluxes, then the deferred faces through the network).
+//  * stage_light_kernel / pending_kernel: the bypass mode as one fused stage (faces, fluxes,
+//    update and CFL in LDS-linked row tiles), then the cells next to a deferred face.
 //  * train_kernel / reduce_kernel / adam_kernel: FP32 minibatch loss, gradient
 //    (per-block partials reduced in a fixed order, so gradients are
 //    deterministic) and the Adam update.
