@@ -49,3 +49,17 @@ def test_half_periodic_registry_rejected_on_host():
     reg = efvm.BoundaryRegistry().with_(efvm.BoundarySide.XLo, efvm.Periodic())
     with pytest.raises(ValueError):
         efvm.StructuredBoundaryPlugin(reg)
+
+
+def test_learned_mode_setters_reject_bad_arguments_without_a_device():
+    """The learned-mode switches check their arguments before touching a context: a null
+    context is EFVM_ERR_ARG (1) for the setters and the getter, with no device needed."""
+    import ctypes as C
+    from dev_field_efvm_amd.efvm import _bind_mlp
+    lib = efvm.load_library()
+    _bind_mlp(lib)
+    assert lib.efvm_set_learned_bypass(None, 1) == 1
+    assert lib.efvm_set_learned_fused(None, 1) == 1
+    a, f = C.c_int(-1), C.c_int(-1)
+    assert lib.efvm_get_learned_fused(None, C.byref(a), C.byref(f)) == 1
+    assert (a.value, f.value) == (-1, -1)
