@@ -656,12 +656,28 @@ __device__ __forceinline__ void stage_f32_pair_strip(const StageArgs& a, const i
     }
 }
 
+// A strip whose 128 columns and rows j0-2 .. j1+3 all lie inside the mesh meets no boundary:
+// no ghost column or row, no wrap, every face reconstructed.  The periodic form's code computes
+// it exactly as the general form does, without the general form's per-row column selects and
+// reflections on freshly loaded values (which cost a 2048^2 periodic case 42% in the general
+// form).  The choice is wave-uniform.
+__device__ __forceinline__ bool pair_strip_interior(const StageArgs& a, int s) {
+    const int sxi = s % a.n_strips_x, syi = s / a.n_strips_x;
+    const int c_lo = sxi * STRIP_OUT2 - 2, c_hi = c_lo + 2 * WAVE - 1;
+    const int j0 = syi * a.strip_h, j1 = min(j0 + a.strip_h, a.ny);
+    return c_lo >= 0 && c_hi < a.nx && j0 >= 2 && j1 + 3 < a.ny;
+}
+
 template <int FLUX, int LIM, bool GEN>
 __global__ __launch_bounds__(256) void stage_f32_pair_kernel(StageArgs a) {
     const int s = strip_index(a, blockDim.x / WAVE);
     if (s >= a.n_strips_x * a.n_strips_y) return;
-    stage_f32_pair_strip<FLUX, LIM, GEN>(a, s, (const float*)a.uin[0], (const float*)a.u0[0], (float*)a.uout[0],
-                                         a.stage, a.parity);
+    if (GEN && pair_strip_interior(a, s))
+        stage_f32_pair_strip<FLUX, LIM, false>(a, s, (const float*)a.uin[0], (const float*)a.u0[0], (float*)a.uout[0],
+                                               a.stage, a.parity);
+    else
+        stage_f32_pair_strip<FLUX, LIM, GEN>(a, s, (const float*)a.uin[0], (const float*)a.u0[0], (float*)a.uout[0],
+                                             a.stage, a.parity);
 }
 
 // ---------------------------------------------------------------------------
@@ -1107,8 +1123,14 @@ __global__ __launch_bounds__(256) void stage_f32_batch_pair_kernel(const BatchCa
         const int cur = bc.fused && (parity & 1) ? 2 : 0;
         const int in = stage == 0 ? cur : (stage == 1 ? 1 : 2);
         const int out = stage == 0 ? 1 : (stage == 1 ? 2 : 0);
-        stage_f32_pair_strip<FLUX, LIM, true>(bc.a, wk.y, (const float*)bc.P[in][0], (const float*)bc.P[cur][0],
-                                              (float*)bc.P[out][0], stage, parity);
+        // the periodic form for periodic cases (as their single contexts run) and for strips
+        // that meet no boundary; the general form for the others
+        if (bc.a.pair_path == 1 || pair_strip_interior(bc.a, wk.y))
+            stage_f32_pair_strip<FLUX, LIM, false>(bc.a, wk.y, (const float*)bc.P[in][0], (const float*)bc.P[cur][0],
+                                                   (float*)bc.P[out][0], stage, parity);
+        else
+            stage_f32_pair_strip<FLUX, LIM, true>(bc.a, wk.y, (const float*)bc.P[in][0], (const float*)bc.P[cur][0],
+                                                  (float*)bc.P[out][0], stage, parity);
     }
 }
 
