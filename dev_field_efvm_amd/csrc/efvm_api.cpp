@@ -50,6 +50,7 @@ hipError_t launch_ibm_flip(void* const* planes, int esz, const uint8_t* solid, l
 hipError_t launch_stage_f32_faces(const StageArgs& a, const float* F, hipStream_t st);
 hipError_t launch_stage_f32_flux(const StageArgs& a, const float* F, hipStream_t st);
 hipError_t launch_lam_reduce(const double* part, int n, double* target, hipStream_t st);
+hipError_t launch_batch_lam_fold(const BatchCase* cases, int n_cases, int parity, hipStream_t st);
 hipError_t launch_cfl_f32(const StageArgs& a, const void* const* planes, double* target, hipStream_t st);
 hipError_t launch_mlp_faces(const mlp::FaceArgs& a, hipStream_t st);
 hipError_t launch_mlp_faces_bypass(const mlp::FaceArgs& a, hipStream_t st);
@@ -1121,6 +1122,8 @@ struct efvm_batch {
     int allow_fuse = 1;            // ... and the periodic ones run stages 2 and 3 fused
     int strip_h23 = 37;            // rows per strip of the fused launch
     DevClock* clocks = nullptr;
+    double* lam_slots = nullptr;   // per-case regions of per-strip CFL maxima (the last stage's
+    long n_lam_slots = 0;          // strips write them, batch_lam_fold_kernel folds them)
     float* pool = nullptr;
     std::vector<long> offset;      // per-case start (cells) in the pool
     long total_cells = 0;
@@ -1255,6 +1258,28 @@ efvm_status batch_rebuild(efvm_batch* b) {
         bc.a23.n_strips_x = (bc.a.nx + STRIP_OUT23 - 1) / STRIP_OUT23;
         bc.a23.n_strips_y = (bc.a.ny + b->strip_h23 - 1) / b->strip_h23;
     }
+    // CFL slots: one per strip of the launch that runs the case's last stage (same-address
+    // atomics from thousands of strips of one large case slowed that stage by a quarter)
+    {
+        long need = 0;
+        std::vector<long> off(b->n);
+        for (int i = 0; i < b->n; i++) {
+            const BatchCase& bc = b->hc[i];
+            off[i] = need;
+            need += std::max<long>((long)bc.a.n_strips_x * bc.a.n_strips_y, (long)bc.a23.n_strips_x * bc.a23.n_strips_y);
+        }
+        if (need > b->n_lam_slots) {
+            if (b->lam_slots) (void)hipFree(b->lam_slots);
+            b->lam_slots = nullptr;
+            b->n_lam_slots = 0;
+            BHIPCHK(b, hipMalloc((void**)&b->lam_slots, sizeof(double) * need));
+            b->n_lam_slots = need;
+        }
+        for (int i = 0; i < b->n; i++) {
+            b->hc[i].a.lam_part = b->lam_slots + off[i];
+            b->hc[i].a23.lam_part = b->lam_slots + off[i];
+        }
+    }
     for (int i = 0; i < b->n; i++) {
         const int now = batch_cur_set(b, i);
         if (now != was_cur[i]) {
@@ -1321,6 +1346,10 @@ efvm_status batch_enqueue_steps(efvm_batch* b, int n_steps) {
             hipError_t e = launch_stage_f32_batch_pair23(b->dc, rep, b->dwork_fused, b->n_work_fused, b->parity, b->stream);
             if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("batch fused stages: ") + hipGetErrorString(e));
         }
+        if (rep.fuse_cfl) {
+            hipError_t e = launch_batch_lam_fold(b->dc, b->n, b->parity, b->stream);
+            if (e != hipSuccess) return bfail(b, EFVM_ERR_HIP, std::string("batch cfl fold: ") + hipGetErrorString(e));
+        }
         b->parity ^= 1;
     }
     return EFVM_OK;
@@ -1380,6 +1409,7 @@ void efvm_batch_destroy(efvm_batch* b) {
     batch_drop_graph(b);
     if (b->pool) (void)hipFree(b->pool);
     if (b->clocks) (void)hipFree(b->clocks);
+    if (b->lam_slots) (void)hipFree(b->lam_slots);
     if (b->dc) (void)hipFree(b->dc);
     if (b->dwork) (void)hipFree(b->dwork);
     if (b->dwork_pair) (void)hipFree(b->dwork_pair);

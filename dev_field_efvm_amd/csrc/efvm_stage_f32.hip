@@ -1337,6 +1337,30 @@ __global__ __launch_bounds__(1024) void lam_reduce_kernel(const double* __restri
     }
 }
 
+// The batch's per-case CFL slots -> each case's lam[parity ^ 1] (one block per case): the strips
+// of the launch that ran the case's last stage (the fused launch's for a fused case).
+__global__ __launch_bounds__(256) void batch_lam_fold_kernel(const BatchCase* __restrict__ cases, int parity) {
+    const BatchCase& bc = cases[blockIdx.x];
+    const StageArgs& a = bc.fused ? bc.a23 : bc.a;
+    const int n = a.n_strips_x * a.n_strips_y;
+    __shared__ double red[4];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, a.lam_part[i]);
+    m = wave_max(m);
+    if (lane_id() == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) m = fmax(m, red[w]);
+        double* t = &a.clock->lam[parity ^ 1];
+        *t = fmax(*t, m);
+    }
+}
+
+hipError_t launch_batch_lam_fold(const BatchCase* cases, int n_cases, int parity, hipStream_t st) {
+    batch_lam_fold_kernel<<<n_cases, 256, 0, st>>>(cases, parity);
+    return hipGetLastError();
+}
+
 hipError_t launch_lam_reduce(const double* part, int n, double* target, hipStream_t st) {
     lam_reduce_kernel<<<1, 1024, 0, st>>>(part, n, target);
     return hipGetLastError();
