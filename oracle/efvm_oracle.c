@@ -39,7 +39,10 @@
  *     reconstructs it from the wrapped cell, which makes the seam conservative;
  *   - locate(): s = (p - lo)/h - 0.5, i0 = floor(s), t = s - i0.
  * These are parity-unpinned by any reference test except where the tests
- * under tests/ say otherwise.
+ * under tests/ say otherwise.  The fixed-problem digest of
+ * examples/restart_determinism/main.rs:79 (tests/test_oracle3d.py) fixes the
+ * sweep's axis order: the oracle matches it bit for bit, and with the axes
+ * swept z, y, x it does not.
  *
  * orc_set_threads(s, n) runs the loops whose result does not depend on their
  * order (slopes, halo and boundary fills, viscous terms, RK update, CFL maximum)

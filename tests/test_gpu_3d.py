@@ -648,3 +648,37 @@ def test_3d_checkpoint_restart_is_bitwise(tmp_path, precision):
     with pytest.raises(EfvmError) as e:
         c.checkpoint_load(path)
     assert e.value.status == ERR_CONFIG
+
+
+def test_restart_determinism_digest_on_the_gpu(tmp_path):
+    """examples/restart_determinism/main.rs on the 3-D FP64 context: 10 forward-Euler
+    Rusanov steps of the 16 x 4 x 2 problem give the reference's fixed digest (main.rs:79),
+    which fixes every conserved value bit for bit; and a CFDRST01 checkpoint written after
+    step 5, loaded into a fresh context and advanced 5 more, equals the uninterrupted run
+    bit for bit (main.rs:207-224)."""
+    import restart_digest as rd
+    sim = orc.Sim(rd.NX, rd.NY, rd.NZ, lo=rd.LO, hi=rd.HI, bcs=rd.BCS, **rd.SCHEME)
+    cen = (sim.centroids(0), sim.centroids(1), sim.centroids(2))
+    start = np.apply_along_axis(lambda q: orc.prim_to_cons(q, rd.GAMMA), -1, rd.initial_prims(cen[0], cen[1], rd.NZ))
+
+    def make():
+        c = efvm.Context3(rd.NX, rd.NY, rd.NZ, lo=rd.LO, hi=rd.HI, gamma=rd.GAMMA, r_gas=rd.R_GAS,
+                          precision="fp64", **rd.SCHEME)
+        c.set_boundaries(rd.BCS)
+        return c
+    a = make()
+    a.upload(start)
+    a.step(rd.STEPS_TOTAL)
+    assert rd.state_digest(a.download(), *cen) == rd.DIGEST_REF
+    assert a.clock()["step"] == rd.STEPS_TOTAL and a.clock()["dt"] == 2.0e-5
+    b = make()
+    b.upload(start)
+    b.step(rd.STEPS_CHECKPOINT)
+    path = str(tmp_path / "rd.bin")
+    b.checkpoint_save(path)
+    c = make()
+    c.checkpoint_load(path)
+    c.step(rd.STEPS_TOTAL - rd.STEPS_CHECKPOINT)
+    assert np.array_equal(c.download(raw=True), a.download(raw=True))
+    assert c.clock() == a.clock()
+    assert rd.state_digest(c.download(), *cen) == rd.DIGEST_REF

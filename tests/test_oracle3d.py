@@ -225,3 +225,60 @@ def test_ghost_tables_index_the_faces_of_a_3d_side():
     bad.set_interior(S)
     bad.step(steps)
     assert np.abs(bad.interior()[:, 0, :, :][..., [0, 1, 3, 2, 4]] - base).max() > 1e-6 * scale
+
+
+# ------------------------------------------------ examples/restart_determinism/main.rs
+def _restart_problem_sim():
+    import restart_digest as rd
+    s = orc.Sim(rd.NX, rd.NY, rd.NZ, lo=rd.LO, hi=rd.HI, gamma=rd.GAMMA, r_gas=rd.R_GAS, bcs=rd.BCS, **rd.SCHEME)
+    cen = (s.centroids(0), s.centroids(1), s.centroids(2))
+    prims = rd.initial_prims(cen[0], cen[1], rd.NZ)
+    s.set_interior(np.apply_along_axis(lambda q: orc.prim_to_cons(q, rd.GAMMA), -1, prims))
+    return s, cen
+
+
+def test_fixed_problem_digest_is_the_references():
+    """fixed_problem_digest_matches_reference (examples/restart_determinism/main.rs:235-240):
+    10 forward-Euler Rusanov steps of the 16 x 4 x 2 problem give the digest the reference
+    holds (main.rs:79).  The digest hashes the bits of every conserved value of every cell,
+    so this pins the oracle's whole 3-D first-order step bit for bit: the face sweep order,
+    the face areas over the cell volume, the slip-wall and outflow ghosts on all three
+    axes, the fixed dt and the update."""
+    import restart_digest as rd
+    s, cen = _restart_problem_sim()
+    start = s.interior()
+    assert rd.state_digest(start, *cen) != rd.DIGEST_REF
+    s.step(rd.STEPS_TOTAL)
+    info = s.info()
+    assert info["step"] == rd.STEPS_TOTAL and info["dt"] == 2.0e-5
+    final = s.interior()
+    assert rd.state_digest(final, *cen) == rd.DIGEST_REF
+    # the digest moves with one unit in the last place of one value of one cell
+    moved = final.copy()
+    moved[1, 2, 7, 4] = np.nextafter(moved[1, 2, 7, 4], np.inf)
+    assert rd.state_digest(moved, *cen) != rd.DIGEST_REF
+    # and with the flux plugin swapped (HLLC for Rusanov)
+    kw = dict(rd.SCHEME, flux="hllc")
+    h = orc.Sim(rd.NX, rd.NY, rd.NZ, lo=rd.LO, hi=rd.HI, gamma=rd.GAMMA, r_gas=rd.R_GAS, bcs=rd.BCS, **kw)
+    h.set_interior(start)
+    h.step(rd.STEPS_TOTAL)
+    assert rd.state_digest(h.interior(), *cen) != rd.DIGEST_REF
+
+
+def test_restart_at_step_five_continues_to_the_references_digest():
+    """restart_checkpoint_is_bitwise (main.rs:207-224) on the oracle: the raw state and the
+    clock after 5 steps, loaded into a fresh simulation and advanced 5 more, equal the
+    uninterrupted 10 steps bit for bit."""
+    import restart_digest as rd
+    a, cen = _restart_problem_sim()
+    a.step(rd.STEPS_TOTAL)
+    b, _ = _restart_problem_sim()
+    b.step(rd.STEPS_CHECKPOINT)
+    raw, clock = b.raw(), b.info()
+    c, _ = _restart_problem_sim()
+    c.set_raw(raw)
+    c.set_clock(**clock)
+    c.step(rd.STEPS_TOTAL - rd.STEPS_CHECKPOINT)
+    assert np.array_equal(c.raw(), a.raw())
+    assert c.info() == a.info()
+    assert rd.state_digest(c.interior(), *cen) == rd.DIGEST_REF
