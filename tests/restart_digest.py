@@ -32,23 +32,27 @@ _M64 = (1 << 64) - 1
 _LOW26 = (1 << 26) - 1
 
 
+def initial_prim(x0: float, x1: float):
+    """Primitive state (rho, u, v, w, p, T) of main.rs:106-115 at one centroid.  Python's
+    math module calls the C library's exp, sin and cos, as the reference's f64 methods
+    do; powi(2) is a product."""
+    d0, d1 = x0 - 0.5, x1 - 0.125
+    r2 = d0 * d0 + 16.0 * (d1 * d1)
+    bump = math.exp(-80.0 * r2)
+    rho = 1.0 + 0.04 * bump
+    u = 9.0 + 0.7 * math.sin(2.0 * math.pi * x0)
+    v = 0.15 * math.cos(4.0 * math.pi * x1)
+    p = 100_000.0 * (1.0 + 0.03 * bump)
+    return rho, u, v, 0.0, p, p / (rho * 287.058)
+
+
 def initial_prims(cx, cy, nz):
-    """Primitive state (rho, u, v, w, p, T) of main.rs:106-115 at the cell centroids, as
-    an (nz, ny, nx, 6) array.  Python's math module calls the C library's exp, sin and
-    cos, as the reference's f64 methods do; powi(2) is a product."""
+    """initial_prim at every cell centroid, as an (nz, ny, nx, 6) array."""
     ny, nx = len(cy), len(cx)
     out = np.zeros((nz, ny, nx, 6))
     for j in range(ny):
         for i in range(nx):
-            x0, x1 = float(cx[i]), float(cy[j])
-            d0, d1 = x0 - 0.5, x1 - 0.125
-            r2 = d0 * d0 + 16.0 * (d1 * d1)
-            bump = math.exp(-80.0 * r2)
-            rho = 1.0 + 0.04 * bump
-            u = 9.0 + 0.7 * math.sin(2.0 * math.pi * x0)
-            v = 0.15 * math.cos(4.0 * math.pi * x1)
-            p = 100_000.0 * (1.0 + 0.03 * bump)
-            out[:, j, i] = (rho, u, v, 0.0, p, p / (rho * 287.058))
+            out[:, j, i] = initial_prim(float(cx[i]), float(cy[j]))
     return out
 
 

@@ -682,3 +682,49 @@ def test_restart_determinism_digest_on_the_gpu(tmp_path):
     assert np.array_equal(c.download(raw=True), a.download(raw=True))
     assert c.clock() == a.clock()
     assert rd.state_digest(c.download(), *cen) == rd.DIGEST_REF
+
+
+def test_app_mirror_runs_the_restart_example(tmp_path):
+    """build_app() of examples/restart_determinism/main.rs:120-141, plugin by plugin through the
+    App mirror: 10 app.run() calls give the reference's fixed digest (main.rs:79), and a
+    checkpoint written after 5 and read into a freshly built app continues to the same state,
+    clock and digest (main.rs:143-224)."""
+    import restart_digest as rd
+    from dev_field_efvm_amd import app as A
+    mesh = A.UniformMeshConfig(nx=rd.NX, ny=rd.NY, nz=rd.NZ, ng=2, bounds_lo=rd.LO, bounds_hi=rd.HI)
+    eos = A.IdealGas.air()
+    prim = np.vectorize(rd.initial_prim, otypes=[np.float64] * 6)
+
+    def initial_state(x, y, z):
+        rho, u, v, w, p, _t = prim(x, y)
+        return eos.prim_to_cons(rho, u, v, w, p)
+
+    def build_app():
+        bcs = A.BoundaryRegistry()
+        for side in (A.BoundarySide.YLo, A.BoundarySide.YHi, A.BoundarySide.ZLo, A.BoundarySide.ZHi):
+            bcs.set(side, A.SlipWall())
+        app = A.App()
+        app.add_plugins(A.FieldDefaultPlugins(mesh)).add_plugins(A.CfdStatePlugin(initial_state))
+        app.add_plugins(A.IdealGasPlugin()).add_plugins(A.BoundaryPlugin(bcs))
+        app.add_plugins(A.FluxPlugin.rusanov()).add_plugins(A.IntegratorPlugin.euler())
+        app.add_plugins(A.SolverPlugin(A.SolverConfig(cfl=0.2, fixed_dt=2.0e-5, viscous=False, muscl=False)))
+        app.prepare()
+        return app
+    cen = (mesh.centroids(0), mesh.centroids(1), mesh.centroids(2))
+    continuous = build_app()
+    for _ in range(rd.STEPS_TOTAL):
+        continuous.run()
+    assert rd.state_digest(continuous.state(), *cen) == rd.DIGEST_REF
+    restarted = build_app()
+    for _ in range(rd.STEPS_CHECKPOINT):
+        restarted.run()
+    path = str(tmp_path / "restart.bin")
+    restarted.write_checkpoint(path)
+    resumed = build_app()
+    resumed.read_checkpoint(path)
+    for _ in range(rd.STEPS_TOTAL - rd.STEPS_CHECKPOINT):
+        resumed.run()
+    assert np.array_equal(resumed.state(), continuous.state())
+    a, b = resumed.solver_state(), continuous.solver_state()
+    assert (a.step, a.time, a.dt, a.rk_stage) == (b.step, b.time, b.dt, b.rk_stage)
+    assert rd.state_digest(resumed.state(), *cen) == rd.DIGEST_REF
